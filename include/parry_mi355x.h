@@ -1,0 +1,141 @@
+/*
+ * parry_mi355x.h -- C ABI of the MI355X (gfx950) batched collision-query path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, no HIP or torch types.
+ * Every entry point returns an int status (PQ_OK on success) and, on failure,
+ * records a message retrievable with pq_last_error() (thread-local).
+ * Per-query "no result" is a sentinel, mirroring parry's Option:
+ *   ray hit:  feature == PQ_NO_HIT  <=> None
+ * Device-pointer entry points take buffers in GPU memory (allocated with
+ * pq_device_malloc or by the caller's own runtime) and an optional HIP stream
+ * passed as `void*` (NULL = the default stream); they are asynchronous with
+ * respect to the host.  `*_host` variants copy in and out and synchronise.
+ *
+ * Citations are to dimforge/parry 0.25.3 (path:line).
+ */
+#ifndef PARRY_MI355X_H
+#define PARRY_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PQ_OK 0
+#define PQ_ERR_INVALID 1     /* bad argument (null pointer, size, index) */
+#define PQ_ERR_UNSUPPORTED 2 /* parry's Unsupported, src/query/error.rs:146-147 */
+#define PQ_ERR_OOM 3         /* device allocation failed */
+#define PQ_ERR_HIP 4         /* HIP runtime error (message has the detail) */
+#define PQ_ERR_CAPACITY 5    /* an output buffer was too small; required size reported */
+
+#define PQ_NO_HIT 0xFFFFFFFFu
+
+/* Copies the calling thread's last error message (NUL-terminated, truncated
+ * to `cap`).  Returns the full message length. */
+size_t pq_last_error(char *buf, size_t cap);
+
+/* ---- device memory helpers (for callers without HIP bindings) ---------- */
+int pq_device_malloc(void **ptr, size_t bytes);
+int pq_device_free(void *ptr);
+int pq_memcpy_htod(void *dst_device, const void *src_host, size_t bytes, void *stream);
+int pq_memcpy_dtoh(void *dst_host, const void *src_device, size_t bytes, void *stream);
+int pq_stream_synchronize(void *stream);
+
+/* ---- geometry records --------------------------------------------------- */
+
+/* parry3d::query::Ray (src/query/ray/ray.rs:75-88, #[repr(C)], f32): origin then
+ * direction; the direction need not be unit length.  A Rust `&[Ray]` can be
+ * passed as `const pq_ray*` without conversion. */
+typedef struct pq_ray {
+    float origin[3];
+    float dir[3];
+} pq_ray;
+
+/* parry3d::query::RayIntersection (src/query/ray/ray.rs:286-315) for a TriMesh:
+ * `feature` is the index k of FeatureId::Face(k) as remapped by
+ * src/query/ray/ray_trimesh.rs:25-33 (k < num_triangles: front face of
+ * triangle k; k >= num_triangles: back face of triangle k - num_triangles),
+ * or PQ_NO_HIT for None. */
+typedef struct pq_ray_hit {
+    float time_of_impact;
+    float normal[3];
+    uint32_t feature;
+} pq_ray_hit;
+
+/* parry3d BvhNodeWide (src/partitioning/bvh/bvh_tree.rs:263-266) in f32: two
+ * BvhNode records (bvh_tree.rs:456-467) of {mins, children, maxs, data}.  For
+ * a leaf, `children` is the leaf (triangle) index; otherwise the index of the
+ * child wide node.  `data` holds leaf_count in its low 30 bits (a node is a
+ * leaf iff leaf_count == 1; a right child with leaf_count == 0 marks a root
+ * with a single leaf) and the change flags in the top 2 bits. */
+typedef struct pq_bvh_node {
+    float mins[3];
+    uint32_t children;
+    float maxs[3];
+    uint32_t data;
+} pq_bvh_node;
+typedef struct pq_bvh_node_wide {
+    pq_bvh_node left;
+    pq_bvh_node right;
+} pq_bvh_node_wide;
+
+/* ---- Bvh construction (host only, no GPU needed) ---------------------------- */
+
+/* Bvh::from_leaves(BvhBuildStrategy::Binned, aabbs) (src/partitioning/bvh/bvh_tree.rs:1835-1837,
+ * 1891-1955): parry's binned top-down build (bvh_binned_build.rs:39-176) with the
+ * depth-first layout produced by its refit (bvh_refit.rs:170-320).
+ *   aabbs: n records of {mins[3], maxs[3]} (f32); leaf i gets `children` = i.
+ * Writes *num_nodes wide records (max(1, n-1) for n >= 1; 0 for n == 0) and the
+ * depth in wide-node levels.  PQ_ERR_CAPACITY (with *num_nodes set) if
+ * `capacity` is too small. */
+int pq_bvh_from_leaves(const float *aabbs, uint32_t n, pq_bvh_node_wide *out, uint32_t capacity,
+                       uint32_t *num_nodes, uint32_t *depth);
+
+/* ---- TriMesh ray casts --------------------------------------------------- */
+
+typedef struct pq_trimesh pq_trimesh;
+
+/* Creates a device-resident TriMesh (parry: TriMesh::new, src/shape/trimesh.rs:607-612,
+ * which builds its Bvh with the binned strategy, trimesh.rs:1159-1171).
+ *   vertices: 3*num_vertices floats; indices: 3*num_triangles u32.
+ *   nodes:    NULL to build the Bvh here with parry's binned builder
+ *             (bvh_binned_build.rs:39-176 + depth-first refit, bvh_refit.rs:170-320);
+ *             or the caller's Bvh exported as depth-first wide nodes
+ *             (e.g. from a Rust Bvh), num_nodes records.
+ * Host pointers; the data is copied.  Fails with PQ_ERR_INVALID for an empty
+ * index buffer (TriMeshBuilderError::EmptyIndices, trimesh.rs:723-725) or an
+ * out-of-range vertex index. */
+int pq_trimesh_create(const float *vertices, uint32_t num_vertices, const uint32_t *indices,
+                      uint32_t num_triangles, const pq_bvh_node_wide *nodes, uint32_t num_nodes,
+                      pq_trimesh **out);
+int pq_trimesh_destroy(pq_trimesh *mesh);
+/* Number of Bvh wide nodes, and the tree depth in wide-node levels. */
+int pq_trimesh_info(const pq_trimesh *mesh, uint32_t *num_nodes, uint32_t *depth, uint32_t *num_triangles);
+/* Copies the mesh's Bvh (depth-first wide nodes, leaf `children` = triangle index) to host memory. */
+int pq_trimesh_export_bvh(const pq_trimesh *mesh, pq_bvh_node_wide *out_host, uint32_t capacity);
+
+/* Batched RayCast::cast_local_ray_and_get_normal on a TriMesh
+ * (src/query/ray/ray.rs:393-403 -> ray_trimesh.rs:16-35 -> Bvh::find_best,
+ * bvh_traverse.rs:335-417).  One result per ray; bit-identical triangle/feature
+ * choice to parry3d (f32).  Device pointers; asynchronous on `stream`. */
+int pq_trimesh_cast_local_rays(const pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays, float max_toi,
+                               int solid, pq_ray_hit *hits, void *stream);
+/* Instrumented variant (no counterpart in parry): also writes, per ray, two
+ * u32 counts to `counts` (device, 2*num_rays): the number of Bvh wide-node
+ * records fetched (iterations of find_best's loop, bvh_traverse.rs:361-416)
+ * and of triangles tested.  Used for traffic accounting and to prove the
+ * traversal identical to parry's, not for throughput. */
+int pq_trimesh_cast_local_rays_counted(const pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays, float max_toi,
+                                       int solid, pq_ray_hit *hits, uint32_t *counts, void *stream);
+/* Same with host buffers (copies in/out, synchronous). */
+int pq_trimesh_cast_local_rays_host(const pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays, float max_toi,
+                                    int solid, pq_ray_hit *hits);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARRY_MI355X_H */

@@ -1,0 +1,150 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+never by the product package.  ``ensure_built()`` runs the oracle's Makefile
+when the shared object is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_f32 = ctypes.c_float
+_f64 = ctypes.c_double
+_int = ctypes.c_int
+
+_SIG = {
+    "orc_trimesh_create_f32": (_vp, [_vp, _u32, _vp, _u32]),
+    "orc_trimesh_create_f64": (_vp, [_vp, _u32, _vp, _u32]),
+    "orc_trimesh_free_f32": (None, [_vp]),
+    "orc_trimesh_free_f64": (None, [_vp]),
+    "orc_trimesh_num_nodes_f32": (_u32, [_vp]),
+    "orc_trimesh_depth_f32": (_u32, [_vp]),
+    "orc_trimesh_nodes_f32": (None, [_vp, _vp]),
+    "orc_cast_rays_f32": (None, [_vp, _vp, _u64, _f32, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int]),
+    "orc_cast_rays_f64": (None, [_vp, _vp, _u64, _f64, _int, _vp, _vp, _vp, _vp, _int]),
+    "orc_cast_rays_brute_f32": (None, [_vp, _vp, _u64, _f32, _vp, _vp, _vp, _int]),
+    "orc_aabb_cast_ray_f32": (_int, [_vp, _vp, _vp, _f32, _int, _vp]),
+    "orc_triangle_cast_ray_f32": (_int, [_vp, _vp, _f32, _vp, _vp]),
+}
+
+_lib = None
+
+
+def ensure_built() -> str:
+    if not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        ensure_built()
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIG.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def default_threads() -> int:
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
+class OracleTriMesh:
+    """parry3d ``TriMesh`` restated on the CPU, in f32 (parry3d) or f64 (parry3d-f64)."""
+
+    def __init__(self, vertices: np.ndarray, indices: np.ndarray, f64: bool = False):
+        self.f64 = f64
+        dt = np.float64 if f64 else np.float32
+        self.v = np.ascontiguousarray(vertices, dtype=dt).reshape(-1, 3)
+        self.i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+        l = lib()
+        create = l.orc_trimesh_create_f64 if f64 else l.orc_trimesh_create_f32
+        self.h = create(self.v.ctypes.data, self.v.shape[0], self.i.ctypes.data, self.i.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            (lib().orc_trimesh_free_f64 if self.f64 else lib().orc_trimesh_free_f32)(self.h)
+            self.h = None
+
+    def num_triangles(self) -> int:
+        return self.i.shape[0]
+
+    def bvh_nodes(self) -> np.ndarray:
+        """Depth-first BvhNodeWide records (64-byte layout), f32 only."""
+        from parry_amd.shape import BVH_NODE_WIDE_DTYPE
+
+        assert not self.f64
+        n = lib().orc_trimesh_num_nodes_f32(self.h)
+        out = np.zeros(n, dtype=BVH_NODE_WIDE_DTYPE)
+        lib().orc_trimesh_nodes_f32(self.h, out.ctypes.data)
+        return out
+
+    def depth(self) -> int:
+        assert not self.f64
+        return lib().orc_trimesh_depth_f32(self.h)
+
+    def cast_rays(self, rays: np.ndarray, max_toi: float, solid: bool = True, threads: int | None = None,
+                  per_ray_counts: bool = False, brute: bool = False):
+        """Returns dict(feature u32 [n], toi [n], normal [n,3], W, K, max_stack[, w, k])."""
+        dt = np.float64 if self.f64 else np.float32
+        r = np.ascontiguousarray(rays, dtype=dt).reshape(-1, 6)
+        n = r.shape[0]
+        fid = np.zeros(n, np.uint32)
+        toi = np.zeros(n, dt)
+        nrm = np.zeros((n, 3), dt)
+        cnt = np.zeros(3, np.uint64)
+        th = default_threads() if threads is None else threads
+        l = lib()
+        out = {"feature": fid, "toi": toi, "normal": nrm}
+        if brute:
+            assert not self.f64
+            l.orc_cast_rays_brute_f32(self.h, r.ctypes.data, n, max_toi, fid.ctypes.data, toi.ctypes.data, nrm.ctypes.data, th)
+            return out
+        if self.f64:
+            l.orc_cast_rays_f64(self.h, r.ctypes.data, n, max_toi, int(solid), fid.ctypes.data, toi.ctypes.data,
+                                nrm.ctypes.data, cnt.ctypes.data, th)
+        else:
+            w = np.zeros(n, np.uint32) if per_ray_counts else None
+            k = np.zeros(n, np.uint32) if per_ray_counts else None
+            l.orc_cast_rays_f32(self.h, r.ctypes.data, n, max_toi, int(solid), fid.ctypes.data, toi.ctypes.data,
+                                nrm.ctypes.data, cnt.ctypes.data, None if w is None else w.ctypes.data,
+                                None if k is None else k.ctypes.data, th)
+            if per_ray_counts:
+                out["w"], out["k"] = w, k
+        out["W"], out["K"], out["max_stack"] = int(cnt[0]), int(cnt[1]), int(cnt[2])
+        return out
+
+
+def aabb_cast_ray(mins, maxs, ray, max_toi: float, solid: bool):
+    a = np.asarray(mins, np.float32)
+    b = np.asarray(maxs, np.float32)
+    r = np.asarray(ray, np.float32)
+    out = np.zeros(1, np.float32)
+    ok = lib().orc_aabb_cast_ray_f32(a.ctypes.data, b.ctypes.data, r.ctypes.data, max_toi, int(solid), out.ctypes.data)
+    return float(out[0]) if ok else None
+
+
+def triangle_cast_ray(abc, ray, max_toi: float):
+    t = np.asarray(abc, np.float32).reshape(9)
+    r = np.asarray(ray, np.float32)
+    out = np.zeros(4, np.float32)
+    fid = np.zeros(1, np.uint32)
+    ok = lib().orc_triangle_cast_ray_f32(t.ctypes.data, r.ctypes.data, max_toi, out.ctypes.data, fid.ctypes.data)
+    return (out, int(fid[0])) if ok else None

@@ -1,0 +1,180 @@
+// oracle_capi.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// extern "C" surface of the CPU restatement in parry_oracle.hpp, loaded with
+// ctypes by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+// Independent queries are split over a std::thread pool in dynamic chunks:
+// the analogue of a rayon par_iter over rays (parry's own `parallel` feature is
+// unused in src/, so the pool is this file's choice, not a reference pattern).
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#include "parry_oracle.hpp"
+
+using namespace oracle;
+
+namespace {
+
+void parallel_for(uint64_t n, int nthreads, uint64_t chunk, const std::function<void(uint64_t, uint64_t)> &fn) {
+    if (nthreads <= 1 || n <= chunk) {
+        fn(0, n);
+        return;
+    }
+    std::atomic<uint64_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t) {
+        pool.emplace_back([&]() {
+            for (;;) {
+                uint64_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                fn(b, std::min(n, b + chunk));
+            }
+        });
+    }
+    for (auto &th : pool) th.join();
+}
+
+template <class T> TriMesh<T> *make_mesh(const T *verts, uint32_t nv, const uint32_t *idx, uint32_t nt) {
+    auto *m = new TriMesh<T>();
+    m->vertices.resize(nv);
+    for (uint32_t i = 0; i < nv; ++i) m->vertices[i] = v3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    m->indices.assign(idx, idx + 3 * (size_t)nt);
+    trimesh_build(*m);
+    return m;
+}
+
+template <class T>
+void cast_rays(const TriMesh<T> *m, const T *rays, uint64_t n, T max_toi, int solid, uint32_t *fid, T *toi,
+               T *normal, uint64_t *counters, uint32_t *per_ray_w, uint32_t *per_ray_k, int nthreads,
+               int brute) {
+    std::atomic<uint64_t> w_sum{0}, k_sum{0};
+    std::atomic<uint32_t> max_stack{0};
+    parallel_for(n, nthreads, 1024, [&](uint64_t b, uint64_t e) {
+        RayCounters c;
+        for (uint64_t i = b; i < e; ++i) {
+            Ray<T> r{v3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), v3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5])};
+            RayHit<T> h{};
+            uint64_t w0 = c.wide_loads, k0 = c.leaf_tests;
+            bool hit = brute ? trimesh_cast_ray_brute(*m, r, max_toi, &h)
+                             : trimesh_cast_ray(*m, r, max_toi, solid != 0, &h, &c);
+            if (per_ray_w) per_ray_w[i] = (uint32_t)(c.wide_loads - w0);
+            if (per_ray_k) per_ray_k[i] = (uint32_t)(c.leaf_tests - k0);
+            if (hit) {
+                fid[i] = h.fid;
+                toi[i] = h.toi;
+                normal[3 * i] = h.normal.x;
+                normal[3 * i + 1] = h.normal.y;
+                normal[3 * i + 2] = h.normal.z;
+            } else {
+                fid[i] = UINT32_MAX;
+                toi[i] = T(0);
+                normal[3 * i] = normal[3 * i + 1] = normal[3 * i + 2] = T(0);
+            }
+        }
+        w_sum += c.wide_loads;
+        k_sum += c.leaf_tests;
+        uint32_t ms = max_stack.load();
+        while (c.max_stack > ms && !max_stack.compare_exchange_weak(ms, c.max_stack)) {
+        }
+    });
+    if (counters) {
+        counters[0] = w_sum.load();
+        counters[1] = k_sum.load();
+        counters[2] = max_stack.load();
+    }
+}
+
+// 64-byte record, same field order as BvhNodeWide in f32 (bvh_tree.rs:263-266, 456-467).
+struct WideF32 {
+    float lmins[3];
+    uint32_t lchildren;
+    float lmaxs[3];
+    uint32_t ldata;
+    float rmins[3];
+    uint32_t rchildren;
+    float rmaxs[3];
+    uint32_t rdata;
+};
+static_assert(sizeof(WideF32) == 64, "wide record is 64 bytes");
+
+}  // namespace
+
+extern "C" {
+
+void *orc_trimesh_create_f32(const float *verts, uint32_t nv, const uint32_t *idx, uint32_t nt) {
+    return make_mesh<float>(verts, nv, idx, nt);
+}
+void *orc_trimesh_create_f64(const double *verts, uint32_t nv, const uint32_t *idx, uint32_t nt) {
+    return make_mesh<double>(verts, nv, idx, nt);
+}
+void orc_trimesh_free_f32(void *m) { delete (TriMesh<float> *)m; }
+void orc_trimesh_free_f64(void *m) { delete (TriMesh<double> *)m; }
+
+uint32_t orc_trimesh_num_nodes_f32(void *m) { return (uint32_t)((TriMesh<float> *)m)->bvh.nodes.size(); }
+uint32_t orc_trimesh_depth_f32(void *m) { return bvh_depth(((TriMesh<float> *)m)->bvh); }
+
+// Copies the Bvh wide nodes as 64-byte BvhNodeWide-layout records.
+void orc_trimesh_nodes_f32(void *mp, void *out) {
+    auto *m = (TriMesh<float> *)mp;
+    auto *o = (WideF32 *)out;
+    for (size_t i = 0; i < m->bvh.nodes.size(); ++i) {
+        const auto &w = m->bvh.nodes[i];
+        WideF32 r;
+        for (int d = 0; d < 3; ++d) {
+            r.lmins[d] = w.left.mins[d];
+            r.lmaxs[d] = w.left.maxs[d];
+            r.rmins[d] = w.right.mins[d];
+            r.rmaxs[d] = w.right.maxs[d];
+        }
+        r.lchildren = w.left.children;
+        r.ldata = w.left.data;
+        r.rchildren = w.right.children;
+        r.rdata = w.right.data;
+        o[i] = r;
+    }
+}
+
+// counters (nullable): [sum W, sum K, max stack depth].
+void orc_cast_rays_f32(void *m, const float *rays, uint64_t n, float max_toi, int solid, uint32_t *fid, float *toi,
+                       float *normal, uint64_t *counters, uint32_t *per_ray_w, uint32_t *per_ray_k, int nthreads) {
+    cast_rays<float>((TriMesh<float> *)m, rays, n, max_toi, solid, fid, toi, normal, counters, per_ray_w, per_ray_k,
+                     nthreads, 0);
+}
+void orc_cast_rays_f64(void *m, const double *rays, uint64_t n, double max_toi, int solid, uint32_t *fid,
+                       double *toi, double *normal, uint64_t *counters, int nthreads) {
+    cast_rays<double>((TriMesh<double> *)m, rays, n, max_toi, solid, fid, toi, normal, counters, nullptr, nullptr,
+                      nthreads, 0);
+}
+void orc_cast_rays_brute_f32(void *m, const float *rays, uint64_t n, float max_toi, uint32_t *fid, float *toi,
+                             float *normal, int nthreads) {
+    cast_rays<float>((TriMesh<float> *)m, rays, n, max_toi, 1, fid, toi, normal, nullptr, nullptr, nullptr, nthreads,
+                     1);
+}
+
+// Single-AABB slab test (Aabb::cast_local_ray, ray_aabb.rs:12-49). Returns 1 on hit.
+int orc_aabb_cast_ray_f32(const float *mins, const float *maxs, const float *ray, float max_toi, int solid,
+                          float *out) {
+    Aabb<float> a{v3(mins[0], mins[1], mins[2]), v3(maxs[0], maxs[1], maxs[2])};
+    Ray<float> r{v3(ray[0], ray[1], ray[2]), v3(ray[3], ray[4], ray[5])};
+    return aabb_cast_local_ray(a, r, max_toi, solid != 0, out) ? 1 : 0;
+}
+
+// Single-triangle ray cast (ray_triangle.rs:49-62). Returns 1 on hit; out = [toi, nx, ny, nz], *fid = 0 front / 1 back.
+int orc_triangle_cast_ray_f32(const float *abc, const float *ray, float max_toi, float *out, uint32_t *fid) {
+    Ray<float> r{v3(ray[0], ray[1], ray[2]), v3(ray[3], ray[4], ray[5])};
+    RayHit<float> h;
+    if (!triangle_cast(v3(abc[0], abc[1], abc[2]), v3(abc[3], abc[4], abc[5]), v3(abc[6], abc[7], abc[8]), r, max_toi,
+                       &h))
+        return 0;
+    out[0] = h.toi;
+    out[1] = h.normal.x;
+    out[2] = h.normal.y;
+    out[3] = h.normal.z;
+    *fid = h.fid;
+    return 1;
+}
+
+}  // extern "C"

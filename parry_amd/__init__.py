@@ -1,0 +1,21 @@
+"""parry_amd -- MI355X (gfx950) batched collision queries behind parry3d's API.
+
+The compute path is the HIP library ``libparry_mi355x.so`` (C ABI in
+include/parry_mi355x.h); this package is its Python host mirror.
+"""
+from .query import Ray, RayIntersection, FeatureId, RAY_DTYPE, RAY_HIT_DTYPE
+from .shape import TriMesh, TriMeshBuilderError, BVH_NODE_WIDE_DTYPE, split_hits
+from . import scenes
+
+__all__ = [
+    "Ray",
+    "RayIntersection",
+    "FeatureId",
+    "RAY_DTYPE",
+    "RAY_HIT_DTYPE",
+    "TriMesh",
+    "TriMeshBuilderError",
+    "BVH_NODE_WIDE_DTYPE",
+    "split_hits",
+    "scenes",
+]

@@ -1,0 +1,104 @@
+"""ctypes binding of ``libparry_mi355x.so`` (the C ABI in include/parry_mi355x.h).
+
+The library is required: importing a function from here when the shared
+object is missing raises immediately; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libparry_mi355x.so")
+
+PQ_OK = 0
+PQ_ERR_INVALID = 1
+PQ_ERR_UNSUPPORTED = 2
+PQ_ERR_OOM = 3
+PQ_ERR_HIP = 4
+PQ_ERR_CAPACITY = 5
+PQ_NO_HIT = 0xFFFFFFFF
+
+_c = ctypes
+_vp = _c.c_void_p
+_u32 = _c.c_uint32
+_u64 = _c.c_uint64
+_f32 = _c.c_float
+_int = _c.c_int
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "pq_last_error": (_c.c_size_t, [_c.c_char_p, _c.c_size_t]),
+    "pq_device_malloc": (_int, [_c.POINTER(_vp), _c.c_size_t]),
+    "pq_device_free": (_int, [_vp]),
+    "pq_memcpy_htod": (_int, [_vp, _vp, _c.c_size_t, _vp]),
+    "pq_memcpy_dtoh": (_int, [_vp, _vp, _c.c_size_t, _vp]),
+    "pq_stream_synchronize": (_int, [_vp]),
+    "pq_bvh_from_leaves": (_int, [_vp, _u32, _vp, _u32, _c.POINTER(_u32), _c.POINTER(_u32)]),
+    "pq_trimesh_create": (_int, [_vp, _u32, _vp, _u32, _vp, _u32, _c.POINTER(_vp)]),
+    "pq_trimesh_destroy": (_int, [_vp]),
+    "pq_trimesh_info": (_int, [_vp, _c.POINTER(_u32), _c.POINTER(_u32), _c.POINTER(_u32)]),
+    "pq_trimesh_export_bvh": (_int, [_vp, _vp, _u32]),
+    "pq_trimesh_cast_local_rays": (_int, [_vp, _vp, _u64, _f32, _int, _vp, _vp]),
+    "pq_trimesh_cast_local_rays_counted": (_int, [_vp, _vp, _u64, _f32, _int, _vp, _vp, _vp]),
+    "pq_trimesh_cast_local_rays_host": (_int, [_vp, _vp, _u64, _f32, _int, _vp]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class NativeError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[pq error {code}] {msg}")
+        self.code = code
+
+
+class Unsupported(NativeError):
+    """parry's ``Unsupported`` (src/query/error.rs:146-147)."""
+
+
+def lib() -> ctypes.CDLL:
+    """Loads the native library once; raises if it has not been built.
+
+    torch (when installed) is imported first so that the process has a single
+    HIP runtime: torch's bundled libamdhip64.so.7 then also serves this
+    library, whose dependency has the same soname, and device pointers and
+    streams can be shared with torch tensors.
+    """
+    global _lib
+    with _lock:
+        if _lib is None:
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"{LIB_PATH} is missing: build it with `python -m parry_amd.build` "
+                    "(the HIP library is required; there is no fallback)"
+                )
+            l = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                f = getattr(l, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = l
+    return _lib
+
+
+def last_error() -> str:
+    l = lib()
+    n = l.pq_last_error(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    l.pq_last_error(buf, n + 1)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int) -> None:
+    if rc != PQ_OK:
+        msg = last_error()
+        if rc == PQ_ERR_UNSUPPORTED:
+            raise Unsupported(rc, msg)
+        raise NativeError(rc, msg)

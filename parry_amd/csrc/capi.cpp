@@ -1,0 +1,242 @@
+// capi.cpp -- extern "C" entry points declared in include/parry_mi355x.h.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pq_internal.h"
+
+namespace pq {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg) {
+    set_error(msg);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    std::string m = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+    return fail(e == hipErrorOutOfMemory ? PQ_ERR_OOM : PQ_ERR_HIP, m);
+}
+
+namespace {
+
+// LDS budget of the ray kernel's per-lane stack (256 lanes x depth-1 entries x 4 B).
+constexpr uint32_t kMaxRayStack = 64;
+
+int upload(void **dptr, const void *src, size_t bytes) {
+    PQ_HIP_CHECK(hipMalloc(dptr, bytes));
+    hipError_t e = hipMemcpy(*dptr, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(*dptr);
+        *dptr = nullptr;
+        return hip_fail(e, "hipMemcpy(host->device)");
+    }
+    return PQ_OK;
+}
+
+}  // namespace
+}  // namespace pq
+
+using namespace pq;
+
+extern "C" {
+
+size_t pq_last_error(char *buf, size_t cap) {
+    const std::string &m = g_last_error;
+    if (buf && cap) {
+        size_t k = m.size() < cap - 1 ? m.size() : cap - 1;
+        std::memcpy(buf, m.data(), k);
+        buf[k] = 0;
+    }
+    return m.size();
+}
+
+int pq_device_malloc(void **ptr, size_t bytes) {
+    if (!ptr) return fail(PQ_ERR_INVALID, "pq_device_malloc: null ptr");
+    PQ_HIP_CHECK(hipMalloc(ptr, bytes ? bytes : 1));
+    return PQ_OK;
+}
+
+int pq_device_free(void *ptr) {
+    if (ptr) PQ_HIP_CHECK(hipFree(ptr));
+    return PQ_OK;
+}
+
+int pq_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return PQ_OK;
+    PQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return PQ_OK;
+    PQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    PQ_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_stream_synchronize(void *stream) {
+    PQ_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_bvh_from_leaves(const float *aabbs, uint32_t n, pq_bvh_node_wide *out, uint32_t capacity,
+                       uint32_t *num_nodes, uint32_t *depth) {
+    if (n && !aabbs) return fail(PQ_ERR_INVALID, "pq_bvh_from_leaves: null aabbs");
+    if (!num_nodes) return fail(PQ_ERR_INVALID, "pq_bvh_from_leaves: null num_nodes");
+    std::vector<LeafBox> leaves(n);
+    std::memcpy(leaves.data(), aabbs, sizeof(LeafBox) * (size_t)n);
+    std::vector<pq_bvh_node_wide> nodes;
+    uint32_t d = 0;
+    build_binned_bvh(leaves, nodes, &d);
+    *num_nodes = (uint32_t)nodes.size();
+    if (depth) *depth = d;
+    if (capacity < nodes.size())
+        return fail(PQ_ERR_CAPACITY, "pq_bvh_from_leaves: need " + std::to_string(nodes.size()) + " records");
+    if (!nodes.empty()) {
+        if (!out) return fail(PQ_ERR_INVALID, "pq_bvh_from_leaves: null output");
+        std::memcpy(out, nodes.data(), nodes.size() * sizeof(pq_bvh_node_wide));
+    }
+    return PQ_OK;
+}
+
+int pq_trimesh_create(const float *vertices, uint32_t num_vertices, const uint32_t *indices, uint32_t num_triangles,
+                      const pq_bvh_node_wide *nodes, uint32_t num_nodes, pq_trimesh **out) {
+    if (!out) return fail(PQ_ERR_INVALID, "pq_trimesh_create: null output handle");
+    *out = nullptr;
+    if (num_triangles == 0 || !indices) return fail(PQ_ERR_INVALID, "pq_trimesh_create: empty index buffer");
+    if (!vertices || num_vertices == 0) return fail(PQ_ERR_INVALID, "pq_trimesh_create: empty vertex buffer");
+    for (uint64_t k = 0; k < 3ull * num_triangles; ++k)
+        if (indices[k] >= num_vertices)
+            return fail(PQ_ERR_INVALID, "pq_trimesh_create: vertex index " + std::to_string(indices[k]) +
+                                            " out of range at position " + std::to_string(k));
+
+    auto *m = new pq_trimesh();
+    uint32_t depth = 0;
+    if (nodes) {
+        bool ok = false;
+        depth = bvh_depth(nodes, num_nodes, num_triangles, &ok);
+        if (!ok) {
+            delete m;
+            return fail(PQ_ERR_INVALID, "pq_trimesh_create: the supplied Bvh is not a well-formed tree over the triangles");
+        }
+        m->host_nodes.assign(nodes, nodes + num_nodes);
+    } else {
+        std::vector<LeafBox> leaves(num_triangles);
+        for (uint32_t t = 0; t < num_triangles; ++t) {
+            const float *a = vertices + 3 * (size_t)indices[3 * t];
+            const float *b = vertices + 3 * (size_t)indices[3 * t + 1];
+            const float *c = vertices + 3 * (size_t)indices[3 * t + 2];
+            // Triangle::local_aabb (aabb_triangle.rs:16-30): min(min(a,b),c).
+            for (int d = 0; d < 3; ++d) {
+                leaves[t].mins[d] = std::fmin(std::fmin(a[d], b[d]), c[d]);
+                leaves[t].maxs[d] = std::fmax(std::fmax(a[d], b[d]), c[d]);
+            }
+        }
+        build_binned_bvh(leaves, m->host_nodes, &depth);
+    }
+    if (depth > kMaxRayStack + 1) {
+        delete m;
+        return fail(PQ_ERR_UNSUPPORTED, "pq_trimesh_create: Bvh depth " + std::to_string(depth) +
+                                            " exceeds the ray kernel's stack budget (" +
+                                            std::to_string(kMaxRayStack + 1) + " levels)");
+    }
+
+    // Device copy: leaves renumbered to the order the depth-first tree visits
+    // them, triangle records gathered in that order.
+    std::vector<pq_bvh_node_wide> dev_nodes = m->host_nodes;
+    std::vector<TriRecord> recs;
+    recs.reserve(num_triangles);
+    for (auto &w : dev_nodes) {
+        pq_bvh_node *side[2] = {&w.left, &w.right};
+        for (pq_bvh_node *c : side) {
+            if (leaf_count(*c) == 1) {
+                uint32_t t = c->children;
+                TriRecord r{};
+                std::memcpy(r.a, vertices + 3 * (size_t)indices[3 * t], 12);
+                std::memcpy(r.b, vertices + 3 * (size_t)indices[3 * t + 1], 12);
+                std::memcpy(r.c, vertices + 3 * (size_t)indices[3 * t + 2], 12);
+                r.tri_id = t;
+                c->children = (uint32_t)recs.size();
+                recs.push_back(r);
+            }
+        }
+    }
+    int rc = upload(&m->d_nodes, dev_nodes.data(), dev_nodes.size() * sizeof(pq_bvh_node_wide));
+    if (rc == PQ_OK) rc = upload(&m->d_tris, recs.data(), recs.size() * sizeof(TriRecord));
+    if (rc != PQ_OK) {
+        pq_trimesh_destroy(m);
+        return rc;
+    }
+    m->dev.nodes = (const pq_bvh_node_wide *)m->d_nodes;
+    m->dev.tris = (const TriRecord *)m->d_tris;
+    m->dev.num_nodes = (uint32_t)dev_nodes.size();
+    m->dev.num_triangles = num_triangles;
+    m->dev.depth = depth;
+    *out = m;
+    return PQ_OK;
+}
+
+int pq_trimesh_destroy(pq_trimesh *m) {
+    if (!m) return PQ_OK;
+    if (m->d_nodes) (void)hipFree(m->d_nodes);
+    if (m->d_tris) (void)hipFree(m->d_tris);
+    delete m;
+    return PQ_OK;
+}
+
+int pq_trimesh_info(const pq_trimesh *m, uint32_t *num_nodes, uint32_t *depth, uint32_t *num_triangles) {
+    if (!m) return fail(PQ_ERR_INVALID, "pq_trimesh_info: null mesh");
+    if (num_nodes) *num_nodes = m->dev.num_nodes;
+    if (depth) *depth = m->dev.depth;
+    if (num_triangles) *num_triangles = m->dev.num_triangles;
+    return PQ_OK;
+}
+
+int pq_trimesh_export_bvh(const pq_trimesh *m, pq_bvh_node_wide *out, uint32_t capacity) {
+    if (!m || !out) return fail(PQ_ERR_INVALID, "pq_trimesh_export_bvh: null argument");
+    if (capacity < m->host_nodes.size())
+        return fail(PQ_ERR_CAPACITY, "pq_trimesh_export_bvh: need " + std::to_string(m->host_nodes.size()) + " records");
+    std::memcpy(out, m->host_nodes.data(), m->host_nodes.size() * sizeof(pq_bvh_node_wide));
+    return PQ_OK;
+}
+
+int pq_trimesh_cast_local_rays(const pq_trimesh *m, const pq_ray *rays, uint64_t n, float max_toi, int solid,
+                               pq_ray_hit *hits, void *stream) {
+    (void)solid;  // TriMesh parts are triangles, which ignore `solid` (ray_triangle.rs:49-62)
+    if (!m) return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays: null mesh");
+    if (n && (!rays || !hits)) return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays: null buffer");
+    PQ_HIP_CHECK(launch_cast_rays(m->dev, rays, n, max_toi, hits, nullptr, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_trimesh_cast_local_rays_counted(const pq_trimesh *m, const pq_ray *rays, uint64_t n, float max_toi, int solid,
+                                       pq_ray_hit *hits, uint32_t *counts, void *stream) {
+    (void)solid;
+    if (!m) return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays_counted: null mesh");
+    if (n && (!rays || !hits || !counts)) return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays_counted: null buffer");
+    PQ_HIP_CHECK(launch_cast_rays(m->dev, rays, n, max_toi, hits, counts, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_trimesh_cast_local_rays_host(const pq_trimesh *m, const pq_ray *rays, uint64_t n, float max_toi, int solid,
+                                    pq_ray_hit *hits) {
+    if (!m) return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays_host: null mesh");
+    if (n == 0) return PQ_OK;
+    if (!rays || !hits) return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays_host: null buffer");
+    void *dr = nullptr, *dh = nullptr;
+    int rc = pq_device_malloc(&dr, n * sizeof(pq_ray));
+    if (rc == PQ_OK) rc = pq_device_malloc(&dh, n * sizeof(pq_ray_hit));
+    if (rc == PQ_OK) rc = pq_memcpy_htod(dr, rays, n * sizeof(pq_ray), nullptr);
+    if (rc == PQ_OK) rc = pq_trimesh_cast_local_rays(m, (const pq_ray *)dr, n, max_toi, solid, (pq_ray_hit *)dh, nullptr);
+    if (rc == PQ_OK) rc = pq_memcpy_dtoh(hits, dh, n * sizeof(pq_ray_hit), nullptr);
+    pq_device_free(dr);
+    pq_device_free(dh);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// pq_internal.h -- types shared by the host runtime and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/parry_mi355x.h"
+
+namespace pq {
+
+// ---- error plumbing -------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define PQ_HIP_CHECK(expr)                                   \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return ::pq::hip_fail(_e, #expr); \
+    } while (0)
+
+// ---- Bvh ------------------------------------------------------------------
+// Leaf AABB input of the binned builder.
+struct LeafBox {
+    float mins[3];
+    float maxs[3];
+};
+
+// Builds parry's binned Bvh (bvh_binned_build.rs:39-176 followed by the
+// depth-first relayout of bvh_refit.rs:170-320) over `leaves`.  Output records
+// are in depth-first preorder with leaf `children` = leaf index; returns the
+// tree depth in wide-node levels through *depth.
+void build_binned_bvh(const std::vector<LeafBox> &leaves, std::vector<pq_bvh_node_wide> &out, uint32_t *depth);
+
+// Depth (wide-node levels) of a depth-first tree; 0 if malformed.
+uint32_t bvh_depth(const pq_bvh_node_wide *nodes, uint32_t num_nodes, uint32_t num_leaves, bool *ok);
+
+inline uint32_t leaf_count(const pq_bvh_node &n) { return n.data & 0x3fffffffu; }
+inline bool is_leaf(const pq_bvh_node &n) { return leaf_count(n) == 1; }
+
+// ---- device-side TriMesh layout ------------------------------------------
+// Triangle record, 48 B = three 16-B loads: vertices a, b, c and the
+// triangle's index in the caller's index buffer.  Records are stored in the
+// order in which the depth-first Bvh visits its leaves, so sibling leaves share
+// cache lines; Bvh leaf `children` index this array.
+struct alignas(16) TriRecord {
+    float a[3];
+    float b[3];
+    float c[3];
+    uint32_t tri_id;
+    uint32_t pad[2];
+};
+static_assert(sizeof(TriRecord) == 48, "TriRecord is 48 bytes");
+static_assert(sizeof(pq_bvh_node_wide) == 64, "wide node is 64 bytes");
+
+struct DeviceMesh {
+    const pq_bvh_node_wide *nodes;  // device, leaf children -> TriRecord index
+    const TriRecord *tris;          // device
+    uint32_t num_nodes;
+    uint32_t num_triangles;
+    uint32_t depth;
+};
+
+// Launches the ray-cast kernel (ray_trimesh.hip).
+// `counts` (nullable, device, 2 per ray): wide-node records fetched, triangles tested.
+hipError_t launch_cast_rays(const DeviceMesh &m, const pq_ray *rays, uint64_t n, float max_toi, pq_ray_hit *hits,
+                            uint32_t *counts, hipStream_t stream);
+
+}  // namespace pq
+
+struct pq_trimesh {
+    pq::DeviceMesh dev;
+    std::vector<pq_bvh_node_wide> host_nodes;  // leaf children = triangle index
+    void *d_nodes = nullptr;
+    void *d_tris = nullptr;
+};

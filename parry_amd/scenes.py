@@ -1,0 +1,134 @@
+"""Seeded synthetic workloads for the collision-query hot path.
+
+These generators build the inputs that the benchmarks and the parity tests
+share: noise-displaced icospheres with ray batches aimed into them, and (in
+later sections) convex shape populations.  Everything is numpy, seeded with
+``numpy.random.Generator(PCG64(seed))``, computed in float64 and rounded to
+float32 once at the end, so the same seed always gives bit-identical float32
+inputs on every machine.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+__all__ = [
+    "icosahedron",
+    "icosphere",
+    "noisy_icosphere",
+    "rays_into_unit_ball",
+    "load_obj",
+]
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def icosahedron() -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosahedron (12 vertices, 20 faces) with outward counter-clockwise winding."""
+    phi = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array(
+        [
+            [-1, phi, 0], [1, phi, 0], [-1, -phi, 0], [1, -phi, 0],
+            [0, -1, phi], [0, 1, phi], [0, -1, -phi], [0, 1, -phi],
+            [phi, 0, -1], [phi, 0, 1], [-phi, 0, -1], [-phi, 0, 1],
+        ],
+        dtype=np.float64,
+    )
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array(
+        [
+            [0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+            [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+            [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+            [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1],
+        ],
+        dtype=np.int64,
+    )
+    # Orient every face outward (normal . centroid > 0).
+    n = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    c = v[f].mean(axis=1)
+    flip = (n * c).sum(axis=1) < 0
+    f[flip] = f[flip][:, [0, 2, 1]]
+    return v, f
+
+
+def icosphere(subdivisions: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere: midpoint subdivision with shared-edge deduplication.
+
+    Level s has 10*4^s + 2 vertices and 20*4^s triangles (s=6: 40,962 / 81,920).
+    Each triangle (a, b, c) becomes (a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca),
+    so the outward winding is preserved.  Returns float64 vertices on the unit
+    sphere and int64 indices.
+    """
+    v, f = icosahedron()
+    for _ in range(subdivisions):
+        nv = v.shape[0]
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]], axis=0)
+        key = np.sort(e, axis=1)
+        uniq, inv = np.unique(key[:, 0] * (nv + 1) + key[:, 1], return_inverse=True)
+        a = uniq // (nv + 1)
+        b = uniq % (nv + 1)
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        mid_idx = inv.reshape(3, -1).T + nv  # columns: ab, bc, ca
+        ab, bc, ca = mid_idx[:, 0], mid_idx[:, 1], mid_idx[:, 2]
+        A, B, C = f[:, 0], f[:, 1], f[:, 2]
+        f = np.concatenate(
+            [
+                np.stack([A, ab, ca], axis=1),
+                np.stack([B, bc, ab], axis=1),
+                np.stack([C, ca, bc], axis=1),
+                np.stack([ab, bc, ca], axis=1),
+            ],
+            axis=0,
+        )
+        v = np.concatenate([v, mid], axis=0)
+    return v, f
+
+
+def noisy_icosphere(subdivisions: int, seed: int, radius: float = 1.0, noise: float = 0.05):
+    """Icosphere of the given level, each vertex scaled radially by 1 + U[-noise, noise].
+
+    Returns (vertices float32 [nv,3], indices uint32 [nt,3]).
+    """
+    v, f = icosphere(subdivisions)
+    r = _rng(seed).uniform(-noise, noise, size=v.shape[0])
+    v = v * (radius * (1.0 + r))[:, None]
+    return np.ascontiguousarray(v, dtype=np.float32), np.ascontiguousarray(f, dtype=np.uint32)
+
+
+def _unit_vectors(rng: np.random.Generator, n: int) -> np.ndarray:
+    g = rng.standard_normal(size=(n, 3))
+    nrm = np.linalg.norm(g, axis=1, keepdims=True)
+    nrm[nrm == 0] = 1.0
+    return g / nrm
+
+
+def rays_into_unit_ball(n: int, seed: int, origin_radius: float = 3.0) -> np.ndarray:
+    """Rays with origins uniform on a sphere of radius ``origin_radius`` aimed at
+    points uniform in the unit ball; directions are normalised.
+
+    Returns float32 [n, 6] laid out as parry's ``Ray`` (origin xyz, dir xyz).
+    """
+    rng = _rng(seed)
+    o = _unit_vectors(rng, n) * origin_radius
+    t = _unit_vectors(rng, n) * (rng.uniform(0.0, 1.0, size=(n, 1)) ** (1.0 / 3.0))
+    d = t - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.ascontiguousarray(np.concatenate([o, d], axis=1), dtype=np.float32)
+
+
+def load_obj(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """Minimal Wavefront OBJ reader (``v`` and triangular ``f`` records, 1-based)."""
+    vs, fs = [], []
+    with open(path) as fh:
+        for line in fh:
+            p = line.split()
+            if not p:
+                continue
+            if p[0] == "v":
+                vs.append([float(x) for x in p[1:4]])
+            elif p[0] == "f":
+                fs.append([int(x.split("/")[0]) - 1 for x in p[1:4]])
+    return np.asarray(vs, dtype=np.float32), np.asarray(fs, dtype=np.uint32)

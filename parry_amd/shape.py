@@ -1,0 +1,172 @@
+"""Shapes backed by device-resident records (parry3d ``src/shape``).
+
+``TriMesh`` mirrors parry3d's ``TriMesh`` (src/shape/trimesh.rs:519-528) for
+the ray-cast path: construction builds the same binned Bvh
+(trimesh.rs:1159-1171) and uploads it; ``RayCast`` methods
+(src/query/ray/ray.rs:359-411) run on the GPU in batches.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _native
+from .query import Ray, RayIntersection, RAY_DTYPE, RAY_HIT_DTYPE, FeatureId
+
+BVH_NODE_WIDE_DTYPE = np.dtype(
+    [
+        ("left_mins", "<f4", 3), ("left_children", "<u4"), ("left_maxs", "<f4", 3), ("left_data", "<u4"),
+        ("right_mins", "<f4", 3), ("right_children", "<u4"), ("right_maxs", "<f4", 3), ("right_data", "<u4"),
+    ]
+)
+assert BVH_NODE_WIDE_DTYPE.itemsize == 64
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class TriMeshBuilderError(ValueError):
+    """parry's TriMeshBuilderError (src/shape/trimesh.rs)."""
+
+
+class TriMesh:
+    """A triangle mesh whose Bvh and triangles live in GPU memory.
+
+    ``vertices``: (nv, 3) float32; ``indices``: (nt, 3) uint32.  ``bvh_nodes``
+    optionally supplies parry's Bvh as depth-first ``BVH_NODE_WIDE_DTYPE``
+    records (e.g. exported from a Rust ``Bvh``); by default the binned Bvh is
+    built here exactly as ``TriMesh::new`` builds it.
+    """
+
+    def __init__(self, vertices, indices, bvh_nodes: Optional[np.ndarray] = None):
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+        if i.shape[0] == 0:
+            raise TriMeshBuilderError("EmptyIndices")
+        self._vertices = v
+        self._indices = i
+        lib = _native.lib()
+        h = ctypes.c_void_p()
+        if bvh_nodes is not None:
+            nodes = np.ascontiguousarray(bvh_nodes, dtype=BVH_NODE_WIDE_DTYPE)
+            rc = lib.pq_trimesh_create(_ptr(v), v.shape[0], _ptr(i), i.shape[0], _ptr(nodes), nodes.shape[0], ctypes.byref(h))
+        else:
+            rc = lib.pq_trimesh_create(_ptr(v), v.shape[0], _ptr(i), i.shape[0], None, 0, ctypes.byref(h))
+        _native.check(rc)
+        self._h = h
+        nn, d, nt = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _native.check(lib.pq_trimesh_info(h, ctypes.byref(nn), ctypes.byref(d), ctypes.byref(nt)))
+        self._num_nodes, self._depth = nn.value, d.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _native.lib().pq_trimesh_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # -- accessors -------------------------------------------------------
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def vertices(self) -> np.ndarray:
+        return self._vertices
+
+    def indices(self) -> np.ndarray:
+        return self._indices
+
+    def num_triangles(self) -> int:
+        return self._indices.shape[0]
+
+    @property
+    def bvh_depth(self) -> int:
+        return self._depth
+
+    def bvh_nodes(self) -> np.ndarray:
+        """The Bvh as depth-first BvhNodeWide records (leaf children = triangle index)."""
+        out = np.zeros(self._num_nodes, dtype=BVH_NODE_WIDE_DTYPE)
+        _native.check(_native.lib().pq_trimesh_export_bvh(self._h, _ptr(out), out.shape[0]))
+        return out
+
+    # -- RayCast ---------------------------------------------------------
+    def cast_local_rays_and_get_normal(self, rays, max_time_of_impact: float, solid: bool, out=None, stream=None):
+        """Batched ``RayCast::cast_local_ray_and_get_normal``.
+
+        ``rays`` is either a CUDA ``torch.Tensor`` of shape (n, 6) float32 (origin,
+        dir) -- then the result is a (n, 5) float32 CUDA tensor of
+        ``pq_ray_hit`` records produced asynchronously on ``stream`` (default:
+        torch's current stream); see ``split_hits`` -- or a numpy array, for
+        which the call copies in and out and returns a ``RAY_HIT_DTYPE``
+        structured array.
+        """
+        lib = _native.lib()
+        if isinstance(rays, np.ndarray):
+            r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+            hits = np.zeros(r.shape[0], dtype=RAY_HIT_DTYPE)
+            _native.check(
+                lib.pq_trimesh_cast_local_rays_host(self._h, _ptr(r), r.shape[0], float(max_time_of_impact), int(bool(solid)), _ptr(hits))
+            )
+            return hits
+        import torch
+
+        if not (rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous() and rays.shape[-1] == 6):
+            raise ValueError("rays must be a contiguous CUDA float32 tensor of shape (n, 6)")
+        n = rays.shape[0]
+        if out is None:
+            out = torch.empty((n, 5), dtype=torch.float32, device=rays.device)
+        elif not (out.is_cuda and out.dtype == torch.float32 and out.is_contiguous() and out.numel() >= 5 * n):
+            raise ValueError("out must be a contiguous CUDA float32 tensor with room for (n, 5)")
+        s = stream if stream is not None else torch.cuda.current_stream(rays.device)
+        _native.check(
+            lib.pq_trimesh_cast_local_rays(
+                self._h, rays.data_ptr(), n, float(max_time_of_impact), int(bool(solid)), out.data_ptr(), ctypes.c_void_p(s.cuda_stream)
+            )
+        )
+        return out
+
+    def cast_local_rays_counted(self, rays, max_time_of_impact: float, solid: bool = True, stream=None):
+        """Instrumented batch cast on CUDA tensors: returns (hits (n, 5) f32, counts (n, 2) int32).
+
+        counts[:, 0] = Bvh wide-node records fetched, counts[:, 1] = triangles tested.
+        """
+        import torch
+
+        if not (rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous() and rays.shape[-1] == 6):
+            raise ValueError("rays must be a contiguous CUDA float32 tensor of shape (n, 6)")
+        n = rays.shape[0]
+        out = torch.empty((n, 5), dtype=torch.float32, device=rays.device)
+        counts = torch.empty((n, 2), dtype=torch.int32, device=rays.device)
+        s = stream if stream is not None else torch.cuda.current_stream(rays.device)
+        _native.check(
+            _native.lib().pq_trimesh_cast_local_rays_counted(
+                self._h, rays.data_ptr(), n, float(max_time_of_impact), int(bool(solid)), out.data_ptr(),
+                counts.data_ptr(), ctypes.c_void_p(s.cuda_stream)
+            )
+        )
+        return out, counts
+
+    def cast_local_ray_and_get_normal(self, ray: Ray, max_time_of_impact: float, solid: bool) -> Optional[RayIntersection]:
+        """Single-ray ``RayCast::cast_local_ray_and_get_normal`` (ray.rs:393-403 in local space)."""
+        r = np.asarray([list(ray.origin) + list(ray.dir)], dtype=np.float32)
+        h = self.cast_local_rays_and_get_normal(r, max_time_of_impact, solid)[0]
+        if int(h["feature"]) == _native.PQ_NO_HIT:
+            return None
+        return RayIntersection(float(h["time_of_impact"]), tuple(float(x) for x in h["normal"]), FeatureId.face(int(h["feature"])))
+
+    def cast_local_ray(self, ray: Ray, max_time_of_impact: float, solid: bool) -> Optional[float]:
+        hit = self.cast_local_ray_and_get_normal(ray, max_time_of_impact, solid)
+        return None if hit is None else hit.time_of_impact
+
+
+def split_hits(hits):
+    """Views of a (n, 5) float32 ``pq_ray_hit`` tensor: (toi, normal, feature as int32; -1 = None)."""
+    import torch
+
+    feat = hits.view(torch.int32)[:, 4]
+    return hits[:, 0], hits[:, 1:4], feat

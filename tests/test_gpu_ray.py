@@ -1,0 +1,171 @@
+"""GPU parity of the TriMesh ray-cast kernel against the oracle (through the C ABI).
+
+Bar: hit flags and FeatureId indices bit-exact with the f32 oracle (parry3d);
+time of impact and normal within 1e-5 relative (they are expected to be
+bit-identical, since the kernel evaluates parry's exact f32 operation
+sequence); the error against the f64 oracle (parry3d-f64) is printed.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from parry_amd import RAY_HIT_DTYPE, TriMesh, scenes
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-5  # relative tolerance for toi / normal (f32)
+NO_HIT = 0xFFFFFFFF
+
+
+def _cast_gpu_host(mesh, rays, max_toi):
+    return mesh.cast_local_rays_and_get_normal(rays, max_toi, True)
+
+
+def _assert_parity(hits, ref, label):
+    feat = hits["feature"]
+    assert np.array_equal(feat, ref["feature"]), (
+        f"{label}: {np.count_nonzero(feat != ref['feature'])} feature mismatches"
+    )
+    hit = feat != NO_HIT
+    toi_g, toi_r = hits["time_of_impact"][hit], ref["toi"][hit]
+    rel = np.abs(toi_g - toi_r) / np.maximum(np.abs(toi_r), 1e-30)
+    assert (rel <= TOL).all(), f"{label}: toi rel err {rel.max()}"
+    nerr = np.abs(hits["normal"][hit] - ref["normal"][hit]).max(initial=0.0)
+    assert nerr <= TOL, f"{label}: normal err {nerr}"
+    return int(np.count_nonzero(toi_g != toi_r)), int(np.count_nonzero(hits["normal"][hit] != ref["normal"][hit]))
+
+
+def test_c0_full_size_parity_and_f64_error(orc):
+    """The 81,920-triangle icosphere with 1,048,576 rays (the ray-cast benchmark workload)."""
+    v, f = scenes.noisy_icosphere(6, seed=1)
+    rays = scenes.rays_into_unit_ball(1 << 20, seed=2)
+    print("case: C0 icosphere s=6 seed=1, rays seed=2 n=1048576", flush=True)
+    mesh = TriMesh(v, f)
+    ref_mesh = orc.OracleTriMesh(v, f)
+    assert mesh.bvh_nodes().tobytes() == ref_mesh.bvh_nodes().tobytes()
+    hits = _cast_gpu_host(mesh, rays, 10.0)
+    ref = ref_mesh.cast_rays(rays, 10.0)
+    ntoi, nnrm = _assert_parity(hits, ref, "C0")
+    print(f"C0: toi bit-differences {ntoi}, normal bit-differences {nnrm}")
+    assert ntoi == 0 and nnrm == 0
+    # error against parry3d-f64 (stated, not asserted beyond sanity)
+    r64 = orc.OracleTriMesh(v.astype(np.float64), f, f64=True).cast_rays(rays.astype(np.float64), 10.0)
+    same = r64["feature"] == hits["feature"]
+    hit = same & (hits["feature"] != NO_HIT)
+    rel = np.abs(hits["time_of_impact"][hit] - r64["toi"][hit]) / np.abs(r64["toi"][hit])
+    nerr = np.abs(hits["normal"][hit] - r64["normal"][hit]).max()
+    print(f"C0 vs f64 oracle: feature agreement {same.mean():.7f}, toi max rel err {rel.max():.3e}, "
+          f"normal max abs err {nerr:.3e}")
+    assert same.mean() > 0.9999 and rel.max() < 1e-5
+
+
+@pytest.mark.parametrize("level", [0, 1, 3, 5, 7])
+def test_icosphere_levels(orc, level):
+    v, f = scenes.noisy_icosphere(level, seed=100 + level)
+    rays = scenes.rays_into_unit_ball(50000, seed=200 + level)
+    print(f"case: icosphere s={level}", flush=True)
+    hits = _cast_gpu_host(TriMesh(v, f), rays, 10.0)
+    _assert_parity(hits, orc.OracleTriMesh(v, f).cast_rays(rays, 10.0), f"s={level}")
+
+
+def test_bunny_reference_mesh(orc):
+    v, f = scenes.load_obj(os.path.join(GOLDEN, "low_poly_bunny.obj"))
+    rng = np.random.default_rng(3)
+    c = (v.min(0) + v.max(0)) / 2
+    o = rng.standard_normal((100000, 3))
+    o = o / np.linalg.norm(o, axis=1, keepdims=True) * 5 + c
+    t = c + rng.uniform(-1, 1, (100000, 3)) * (v.max(0) - v.min(0)) / 2
+    d = (t - o) / np.linalg.norm(t - o, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    # rays from inside the mesh too (hit back faces)
+    rays[::7, :3] = c
+    print("case: bunny", flush=True)
+    hits = _cast_gpu_host(TriMesh(v, f), rays, 100.0)
+    ref = orc.OracleTriMesh(v, f).cast_rays(rays, 100.0)
+    _assert_parity(hits, ref, "bunny")
+    assert (hits["feature"][::7] >= f.shape[0]).mean() > 0.5  # back faces seen from inside
+
+
+def test_edge_cases(orc):
+    v, f = scenes.noisy_icosphere(3, seed=9)
+    rng = np.random.default_rng(4)
+    n = 20000
+    rays = scenes.rays_into_unit_ball(n, seed=5)
+    # axis-aligned directions (zero components take the slab's inside test)
+    rays[0:2000, 3:] = 0.0
+    rays[0:2000, 3 + (np.arange(2000) % 3)] = np.where(np.arange(2000) % 2 == 0, 1.0, -1.0)
+    rays[0:2000, :3] = rng.uniform(-1.5, 1.5, (2000, 3))
+    # rays through vertices (ties between adjacent triangles)
+    vi = rng.integers(0, v.shape[0], 2000)
+    o = v[vi] * 3.0
+    d = v[vi] - o
+    rays[2000:4000, :3] = o
+    rays[2000:4000, 3:] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    # rays pointing away (miss), unnormalised and zero directions
+    rays[4000:5000, 3:] = -rays[4000:5000, :3]
+    rays[5000:6000, 3:] *= 3.7
+    rays[6000:6010, 3:] = 0.0
+    mesh = TriMesh(v, f)
+    om = orc.OracleTriMesh(v, f)
+    print("case: edge cases", flush=True)
+    for max_toi in (10.0, 2.5, 0.0):
+        hits = _cast_gpu_host(mesh, rays, max_toi)
+        _assert_parity(hits, om.cast_rays(rays, max_toi), f"edge max_toi={max_toi}")
+    assert (hits["feature"] == NO_HIT).all()  # max_toi = 0 never hits (strict <)
+
+
+def test_small_trees_and_duplicates(orc):
+    tri = np.array([[0, 0, 0], [1, 0, 0.2], [0, 1, 0.1], [2, 0, 0], [3, 0, 0], [2, 1, 0.5]], np.float32)
+    rays = scenes.rays_into_unit_ball(5000, seed=8)
+    rays[:, :3] = np.array([0.3, 0.3, 3.0]) + rays[:, :3] * 0.2
+    rays[:, 3:] = np.array([0.1, 0.05, -1.0]) + rays[:, 3:] * 0.3
+    for idx in ([[0, 1, 2]], [[0, 1, 2], [3, 4, 5]], [[0, 1, 2], [0, 1, 2]], [[0, 1, 2], [3, 4, 5], [0, 2, 1]]):
+        f = np.array(idx, np.uint32)
+        print(f"case: small tree {idx}", flush=True)
+        hits = _cast_gpu_host(TriMesh(tri, f), rays, 100.0)
+        _assert_parity(hits, orc.OracleTriMesh(tri, f).cast_rays(rays, 100.0), f"small {idx}")
+    # duplicated triangles in a deep tree: ties resolved by visit order
+    v, f = scenes.noisy_icosphere(4, seed=12)
+    f = np.concatenate([f, f[::3], f[1::5, ::-1]], 0)
+    rays = scenes.rays_into_unit_ball(50000, seed=13)
+    hits = _cast_gpu_host(TriMesh(v, f), rays, 10.0)
+    _assert_parity(hits, orc.OracleTriMesh(v, f).cast_rays(rays, 10.0), "duplicates")
+
+
+def test_traversal_counts_match_oracle(orc):
+    """Per-ray wide-node loads and triangle tests equal the oracle's: the GPU
+    traversal visits exactly parry's nodes in parry's order."""
+    import torch
+
+    v, f = scenes.noisy_icosphere(5, seed=21)
+    rays = scenes.rays_into_unit_ball(100000, seed=22)
+    print("case: traversal counts s=5", flush=True)
+    hits, counts = TriMesh(v, f).cast_local_rays_counted(torch.from_numpy(rays).cuda(), 10.0)
+    torch.cuda.synchronize()
+    ref = orc.OracleTriMesh(v, f).cast_rays(rays, 10.0, per_ray_counts=True)
+    c = counts.cpu().numpy()
+    assert np.array_equal(c[:, 0], ref["w"].astype(np.int32))
+    assert np.array_equal(c[:, 1], ref["k"].astype(np.int32))
+    assert np.array_equal(hits.cpu().numpy().view(np.uint32)[:, 4], ref["feature"])
+
+
+def test_empty_batch_and_device_path(orc):
+    import torch
+
+    from parry_amd import split_hits
+
+    v, f = scenes.noisy_icosphere(2, seed=1)
+    mesh = TriMesh(v, f)
+    assert mesh.cast_local_rays_and_get_normal(np.zeros((0, 6), np.float32), 10.0, True).shape == (0,)
+    rays = scenes.rays_into_unit_ball(3000, seed=6)
+    dev = torch.from_numpy(rays).cuda()
+    out = mesh.cast_local_rays_and_get_normal(dev, 10.0, True)
+    torch.cuda.synchronize()
+    toi, nrm, feat = split_hits(out)
+    ref = orc.OracleTriMesh(v, f).cast_rays(rays, 10.0)
+    assert np.array_equal(feat.cpu().numpy().view(np.uint32), ref["feature"])
+    assert np.array_equal(toi.cpu().numpy(), ref["toi"])
+    hit = mesh.cast_local_ray_and_get_normal(__import__("parry_amd").Ray((0.0, 0.0, 3.0), (0.0, 0.0, -1.0)), 10.0, True)
+    assert hit is not None and hit.feature.kind == "Face" and 1.9 < hit.time_of_impact < 2.1

@@ -1,0 +1,192 @@
+"""Pins the CPU oracle's ray-cast path (TriMesh + Bvh::find_best) before it is trusted.
+
+Checks, each chosen so that a plausible restatement mistake fails one of them:
+  * reference known-answer tests: BvhNode::cast_ray toi == 5.0
+    (src/partitioning/bvh/bvh_tree.rs:1160-1170) and the single-triangle
+    front/back feature KAT behind src/query/ray/ray_trimesh.rs:187-212;
+  * closed forms for the slab and triangle tests (hit points, normals, signs);
+  * find_best == brute-force nearest triangle over every triangle (pins pruning,
+    strict-less acceptance, stack handling) on the reference's own test mesh
+    (assets/tests/low_poly_bunny.obj, copied to tests/golden) and icospheres;
+  * tie order: equal-toi leaves keep the first one visited (left child first);
+  * Bvh invariants of bvh_validation.rs:61-134,196-218 (leaf counts, box
+    containment, depth-first order) on the built trees;
+  * f32 vs f64 restatements agree to 1e-5 relative.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from parry_amd import scenes
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+MAXF = np.float32(3.4028235e38)
+
+
+def test_kat_bvh_node_cast_ray(orc):
+    # bvh_tree.rs:1160-1170: Aabb (5,-1,-1)-(6,1,1), ray from origin along +x -> 5.0
+    t = orc.aabb_cast_ray([5, -1, -1], [6, 1, 1], [0, 0, 0, 1, 0, 0], float(MAXF), True)
+    assert t == 5.0
+
+
+def test_slab_closed_forms(orc):
+    # inside the box: solid -> 0, hollow -> exit distance
+    assert orc.aabb_cast_ray([-1, -1, -1], [1, 1, 1], [0, 0, 0, 1, 0, 0], 100.0, True) == 0.0
+    assert orc.aabb_cast_ray([-1, -1, -1], [1, 1, 1], [0, 0, 0, 1, 0, 0], 100.0, False) == 1.0
+    # zero direction component outside the slab misses; inside it is ignored
+    assert orc.aabb_cast_ray([-1, -1, -1], [1, 1, 1], [-5, 2, 0, 1, 0, 0], 100.0, True) is None
+    assert orc.aabb_cast_ray([-1, -1, -1], [1, 1, 1], [-5, 0.5, 0, 1, 0, 0], 100.0, True) == 4.0
+    # beyond max_toi misses, pointing away misses
+    assert orc.aabb_cast_ray([-1, -1, -1], [1, 1, 1], [-5, 0, 0, 1, 0, 0], 3.0, True) is None
+    assert orc.aabb_cast_ray([-1, -1, -1], [1, 1, 1], [-5, 0, 0, -1, 0, 0], 100.0, True) is None
+    # unnormalised direction scales toi
+    assert orc.aabb_cast_ray([-1, -1, -1], [1, 1, 1], [-5, 0, 0, 2, 0, 0], 100.0, True) == 2.0
+
+
+def test_triangle_closed_forms(orc):
+    tri = [0, 0, 0, 1, 0, 0, 0, 1, 0]  # normal +z (ab x ac)
+    up = orc.triangle_cast_ray(tri, [0.25, 0.25, -1, 0, 0, 1], 1000.0)
+    down = orc.triangle_cast_ray(tri, [0.25, 0.25, 1, 0, 0, -1], 1000.0)
+    (u, ufid), (d, dfid) = up, down
+    assert u[0] == 1.0 and d[0] == 1.0
+    assert ufid == 1 and tuple(u[1:]) == (0.0, 0.0, -1.0)  # back face: normal opposes +z
+    assert dfid == 0 and tuple(d[1:]) == (0.0, 0.0, 1.0)
+    # outside the barycentric range, parallel, behind, beyond max_toi
+    assert orc.triangle_cast_ray(tri, [0.75, 0.75, 1, 0, 0, -1], 1000.0) is None
+    assert orc.triangle_cast_ray(tri, [0.25, 0.25, 1, 1, 0, 0], 1000.0) is None
+    assert orc.triangle_cast_ray(tri, [0.25, 0.25, 1, 0, 0, 1], 1000.0) is None
+    assert orc.triangle_cast_ray(tri, [0.25, 0.25, 1, 0, 0, -1], 0.5) is None
+    # the inclusive acceptance `toi <= max_toi` (ray_triangle.rs:57)
+    assert orc.triangle_cast_ray(tri, [0.25, 0.25, 1, 0, 0, -1], 1.0) is not None
+    # edges and vertices are inclusive
+    assert orc.triangle_cast_ray(tri, [0.5, 0.5, 1, 0, 0, -1], 10.0) is not None
+    assert orc.triangle_cast_ray(tri, [0.0, 0.0, 1, 0, 0, -1], 10.0) is not None
+    # slanted ray: toi = distance along dir to the plane z=0
+    d = np.array([0.3, -0.2, -1.0])
+    d /= np.linalg.norm(d)
+    o = np.array([0.1, 0.5, 2.0])
+    hit = orc.triangle_cast_ray(tri, list(o) + list(d), 10.0)
+    assert hit is not None and abs(hit[0][0] - 2.0 / -d[2]) < 1e-6
+
+
+def test_kat_trimesh_feature_remap(orc):
+    # ray_trimesh.rs:187-212 triangle; FeatureId remap of ray_trimesh.rs:25-33.
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    f = np.array([[0, 1, 2]], np.uint32)
+    m = orc.OracleTriMesh(v, f)
+    rays = np.array([[0.0, 0.0, -1.0, 0, 0, 1], [0.0, 0.0, 1.0, 0, 0, -1]], np.float32)
+    out = m.cast_rays(rays, 1000.0)
+    assert list(out["feature"]) == [1, 0]  # back face -> Face(0 + ntri), front -> Face(0)
+    assert list(out["toi"]) == [1.0, 1.0]
+
+
+def _random_rays(n, seed, center, radius):
+    rng = np.random.default_rng(seed)
+    o = rng.standard_normal((n, 3))
+    o = o / np.linalg.norm(o, axis=1, keepdims=True) * radius * 3 + center
+    t = rng.uniform(-1, 1, (n, 3)) * radius + center
+    d = t - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("which", ["bunny", "ico3", "ico5"])
+def test_find_best_equals_brute_force(orc, which):
+    if which == "bunny":
+        v, f = scenes.load_obj(os.path.join(GOLDEN, "low_poly_bunny.obj"))
+    else:
+        v, f = scenes.noisy_icosphere(int(which[3:]), seed=7)
+    center = (v.min(0) + v.max(0)) / 2
+    radius = float(np.linalg.norm(v.max(0) - v.min(0))) / 2
+    rays = _random_rays(4000, 11, center, radius)
+    m = orc.OracleTriMesh(v, f)
+    fast = m.cast_rays(rays, 1e4)
+    slow = m.cast_rays(rays, 1e4, brute=True)
+    hit_f = fast["feature"] != 0xFFFFFFFF
+    hit_s = slow["feature"] != 0xFFFFFFFF
+    assert (hit_f == hit_s).all()
+    assert hit_f.mean() > 0.15
+    # the nearest toi is unique regardless of ties; features agree away from ties
+    assert np.array_equal(fast["toi"][hit_f], slow["toi"][hit_s])
+    same = fast["feature"] == slow["feature"]
+    assert same.mean() > 0.999
+    assert np.array_equal(fast["normal"][same], slow["normal"][same])
+
+
+def test_tie_keeps_first_visited(orc):
+    # Two coincident triangles: a 2-leaf root, equal scores -> the left leaf
+    # (leaf 0) is tested first, and the right one's equal toi is not strictly
+    # better (bvh_traverse.rs:369-372, 379, 395).
+    v = np.array([[0, 0, 0], [1, 0, 0.2], [0, 1, 0.1]], np.float32)
+    f = np.array([[0, 1, 2], [0, 1, 2]], np.uint32)
+    m = orc.OracleTriMesh(v, f)
+    out = m.cast_rays(np.array([[0.2, 0.2, 5, 0, 0, -1]], np.float32), 100.0)
+    assert out["feature"][0] == 0
+    # Swapping the two triangles' positions swaps the winner: the choice is by
+    # visit order, not by geometry.
+    f_sw = np.array([[0, 2, 1], [0, 1, 2]], np.uint32)
+    out = orc.OracleTriMesh(v, f_sw).cast_rays(np.array([[0.2, 0.2, 5, 0, 0, -1]], np.float32), 100.0)
+    assert out["feature"][0] == 0 + 2  # leaf 0 is now seen from its back face -> Face(0 + ntri)
+
+
+def _check_tree(nodes, n_leaves):
+    lc = lambda d: d & 0x3FFFFFFF
+    assert nodes.shape[0] == max(1, n_leaves - 1)
+    seen = set()
+    # depth-first order (bvh_validation.rs:196-218)
+    stack, expect = [0], 0
+    while stack:
+        i = stack.pop()
+        assert i == expect
+        expect += 1
+        n = nodes[i]
+        if lc(n["right_data"]) != 1:
+            stack.append(int(n["right_children"]))
+        if lc(n["left_data"]) != 1:
+            stack.append(int(n["left_children"]))
+
+    def rec(i):
+        n = nodes[i]
+        total = 0
+        for side in ("left", "right"):
+            d = lc(int(n[side + "_data"]))
+            c = int(n[side + "_children"])
+            if d == 1:
+                assert c not in seen
+                seen.add(c)
+                total += 1
+            else:
+                ch = nodes[c]
+                sub = rec(c)
+                assert d == sub
+                for cs in ("left", "right"):
+                    assert (n[side + "_mins"] <= ch[cs + "_mins"]).all()
+                    assert (n[side + "_maxs"] >= ch[cs + "_maxs"]).all()
+                assert np.array_equal(n[side + "_mins"], np.minimum(ch["left_mins"], ch["right_mins"]))
+                assert np.array_equal(n[side + "_maxs"], np.maximum(ch["left_maxs"], ch["right_maxs"]))
+                total += sub
+        return total
+
+    if n_leaves >= 2:
+        assert rec(0) == n_leaves
+        assert seen == set(range(n_leaves))
+
+
+@pytest.mark.parametrize("level", [0, 1, 4, 6])
+def test_oracle_tree_invariants(orc, level):
+    v, f = scenes.noisy_icosphere(level, seed=1)
+    m = orc.OracleTriMesh(v, f)
+    _check_tree(m.bvh_nodes(), f.shape[0])
+
+
+def test_f32_matches_f64(orc):
+    v, f = scenes.noisy_icosphere(4, seed=2)
+    rays = scenes.rays_into_unit_ball(20000, seed=3)
+    a = orc.OracleTriMesh(v, f).cast_rays(rays, 10.0)
+    b = orc.OracleTriMesh(v.astype(np.float64), f, f64=True).cast_rays(rays.astype(np.float64), 10.0)
+    same = a["feature"] == b["feature"]
+    assert same.mean() > 0.999
+    rel = np.abs(a["toi"][same] - b["toi"][same]) / np.abs(b["toi"][same])
+    assert rel.max() < 1e-5
+    assert np.abs(a["normal"][same] - b["normal"][same]).max() < 1e-5
