@@ -117,6 +117,25 @@ int pq_trimesh_info(const pq_trimesh *mesh, uint32_t *num_nodes, uint32_t *depth
 /* Copies the mesh's Bvh (depth-first wide nodes, leaf `children` = triangle index) to host memory. */
 int pq_trimesh_export_bvh(const pq_trimesh *mesh, pq_bvh_node_wide *out_host, uint32_t capacity);
 
+/* Lane mapping of the ray kernel, chosen per mesh (no counterpart in parry;
+ * results are identical for every setting, only speed changes):
+ *   PQ_RAY_MAPPING_STATIC      one launch lane per ray;
+ *   PQ_RAY_MAPPING_PERSISTENT  resident waves pull `chunk` ray indices at a time
+ *                              from a per-stream queue in device memory and
+ *                              refill finished lanes every iteration.
+ * The persistent queue of a stream is allocated on the first persistent call
+ * on that stream: make that first call outside hipGraph capture. */
+#define PQ_RAY_MAPPING_STATIC 0u
+#define PQ_RAY_MAPPING_PERSISTENT 1u
+typedef struct pq_ray_tuning {
+    uint32_t mapping;       /* PQ_RAY_MAPPING_* */
+    uint32_t chunk;         /* persistent: ray indices claimed per queue access (0 = 128) */
+    uint32_t blocks_per_cu; /* persistent: resident 256-lane blocks per CU (0 = occupancy limit) */
+    uint32_t reserved;      /* must be 0 */
+} pq_ray_tuning;
+int pq_trimesh_set_tuning(pq_trimesh *mesh, const pq_ray_tuning *tuning);
+int pq_trimesh_get_tuning(const pq_trimesh *mesh, pq_ray_tuning *tuning);
+
 /* Batched RayCast::cast_local_ray_and_get_normal on a TriMesh
  * (src/query/ray/ray.rs:393-403 -> ray_trimesh.rs:16-35 -> Bvh::find_best,
  * bvh_traverse.rs:335-417).  One result per ray; bit-identical triangle/feature

@@ -39,6 +39,8 @@ SIGNATURES = {
     "pq_trimesh_create": (_int, [_vp, _u32, _vp, _u32, _vp, _u32, _c.POINTER(_vp)]),
     "pq_trimesh_destroy": (_int, [_vp]),
     "pq_trimesh_info": (_int, [_vp, _c.POINTER(_u32), _c.POINTER(_u32), _c.POINTER(_u32)]),
+    "pq_trimesh_set_tuning": (_int, [_vp, _vp]),
+    "pq_trimesh_get_tuning": (_int, [_vp, _vp]),
     "pq_trimesh_export_bvh": (_int, [_vp, _vp, _u32]),
     "pq_trimesh_cast_local_rays": (_int, [_vp, _vp, _u64, _f32, _int, _vp, _vp]),
     "pq_trimesh_cast_local_rays_counted": (_int, [_vp, _vp, _u64, _f32, _int, _vp, _vp, _vp]),
@@ -47,6 +49,16 @@ SIGNATURES = {
 
 _lock = threading.Lock()
 _lib = None
+
+
+PQ_RAY_MAPPING_STATIC = 0
+PQ_RAY_MAPPING_PERSISTENT = 1
+
+
+class RayTuning(ctypes.Structure):
+    """pq_ray_tuning (include/parry_mi355x.h)."""
+
+    _fields_ = [("mapping", _u32), ("chunk", _u32), ("blocks_per_cu", _u32), ("reserved", _u32)]
 
 
 class NativeError(RuntimeError):

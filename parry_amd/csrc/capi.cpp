@@ -1,7 +1,10 @@
 // capi.cpp -- extern "C" entry points declared in include/parry_mi355x.h.
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "pq_internal.h"
@@ -20,6 +23,30 @@ int fail(int code, const std::string &msg) {
 int hip_fail(hipError_t e, const char *what) {
     std::string m = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
     return fail(e == hipErrorOutOfMemory ? PQ_ERR_OOM : PQ_ERR_HIP, m);
+}
+
+hipError_t stream_queue(hipStream_t stream, unsigned long long **queue) {
+    static std::mutex mu;
+    static std::map<std::pair<int, hipStream_t>, unsigned long long *> queues;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_pair(dev, stream);
+    auto it = queues.find(key);
+    if (it != queues.end()) {
+        *queue = it->second;
+        return hipSuccess;
+    }
+    void *p = nullptr;
+    if ((e = hipMalloc(&p, 2 * sizeof(unsigned long long))) != hipSuccess) return e;
+    if ((e = hipMemset(p, 0, 2 * sizeof(unsigned long long))) != hipSuccess) {
+        (void)hipFree(p);
+        return e;
+    }
+    queues[key] = (unsigned long long *)p;
+    *queue = (unsigned long long *)p;
+    return hipSuccess;
 }
 
 namespace {
@@ -177,6 +204,7 @@ int pq_trimesh_create(const float *vertices, uint32_t num_vertices, const uint32
     m->dev.num_nodes = (uint32_t)dev_nodes.size();
     m->dev.num_triangles = num_triangles;
     m->dev.depth = depth;
+    m->dev.tuning = pq_ray_tuning{PQ_RAY_MAPPING_STATIC, 128, 0, 0};
     *out = m;
     return PQ_OK;
 }
@@ -194,6 +222,20 @@ int pq_trimesh_info(const pq_trimesh *m, uint32_t *num_nodes, uint32_t *depth, u
     if (num_nodes) *num_nodes = m->dev.num_nodes;
     if (depth) *depth = m->dev.depth;
     if (num_triangles) *num_triangles = m->dev.num_triangles;
+    return PQ_OK;
+}
+
+int pq_trimesh_set_tuning(pq_trimesh *m, const pq_ray_tuning *t) {
+    if (!m || !t) return fail(PQ_ERR_INVALID, "pq_trimesh_set_tuning: null argument");
+    if (t->mapping > PQ_RAY_MAPPING_PERSISTENT || t->reserved != 0 || t->chunk > (1u << 20))
+        return fail(PQ_ERR_INVALID, "pq_trimesh_set_tuning: invalid tuning");
+    m->dev.tuning = *t;
+    return PQ_OK;
+}
+
+int pq_trimesh_get_tuning(const pq_trimesh *m, pq_ray_tuning *t) {
+    if (!m || !t) return fail(PQ_ERR_INVALID, "pq_trimesh_get_tuning: null argument");
+    *t = m->dev.tuning;
     return PQ_OK;
 }
 

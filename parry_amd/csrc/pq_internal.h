@@ -62,7 +62,15 @@ struct DeviceMesh {
     uint32_t num_nodes;
     uint32_t num_triangles;
     uint32_t depth;
+    pq_ray_tuning tuning;
 };
+
+// Per-(device, stream) 16-byte work queue used by persistent kernels
+// (zeroed at allocation; every persistent kernel leaves it zeroed).
+hipError_t stream_queue(hipStream_t stream, unsigned long long **queue);
+
+// LDS stack entries the ray kernels need for a tree of `depth` wide levels.
+uint32_t ray_stack_entries(uint32_t depth);
 
 // Launches the ray-cast kernel (ray_trimesh.hip).
 // `counts` (nullable, device, 2 per ray): wide-node records fetched, triangles tested.

@@ -94,6 +94,19 @@ class TriMesh:
         _native.check(_native.lib().pq_trimesh_export_bvh(self._h, _ptr(out), out.shape[0]))
         return out
 
+    # -- tuning ----------------------------------------------------------
+    def set_tuning(self, mapping: str = "persistent", chunk: int = 128, blocks_per_cu: int = 0) -> None:
+        """Selects the ray kernel's lane mapping ("static" or "persistent"); results never change."""
+        m = {"static": _native.PQ_RAY_MAPPING_STATIC, "persistent": _native.PQ_RAY_MAPPING_PERSISTENT}[mapping]
+        t = _native.RayTuning(m, chunk, blocks_per_cu, 0)
+        _native.check(_native.lib().pq_trimesh_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self) -> dict:
+        t = _native.RayTuning()
+        _native.check(_native.lib().pq_trimesh_get_tuning(self._h, ctypes.byref(t)))
+        return {"mapping": "persistent" if t.mapping == _native.PQ_RAY_MAPPING_PERSISTENT else "static",
+                "chunk": t.chunk, "blocks_per_cu": t.blocks_per_cu}
+
     # -- RayCast ---------------------------------------------------------
     def cast_local_rays_and_get_normal(self, rays, max_time_of_impact: float, solid: bool, out=None, stream=None):
         """Batched ``RayCast::cast_local_ray_and_get_normal``.

@@ -19,7 +19,11 @@ TOL = 1e-5  # relative tolerance for toi / normal (f32)
 NO_HIT = 0xFFFFFFFF
 
 
-def _cast_gpu_host(mesh, rays, max_toi):
+MAPPINGS = ["static", "persistent"]
+
+
+def _cast_gpu_host(mesh, rays, max_toi, mapping="persistent"):
+    mesh.set_tuning(mapping)
     return mesh.cast_local_rays_and_get_normal(rays, max_toi, True)
 
 
@@ -41,32 +45,36 @@ def test_c0_full_size_parity_and_f64_error(orc):
     """The 81,920-triangle icosphere with 1,048,576 rays (the ray-cast benchmark workload)."""
     v, f = scenes.noisy_icosphere(6, seed=1)
     rays = scenes.rays_into_unit_ball(1 << 20, seed=2)
-    print("case: C0 icosphere s=6 seed=1, rays seed=2 n=1048576", flush=True)
     mesh = TriMesh(v, f)
     ref_mesh = orc.OracleTriMesh(v, f)
     assert mesh.bvh_nodes().tobytes() == ref_mesh.bvh_nodes().tobytes()
-    hits = _cast_gpu_host(mesh, rays, 10.0)
     ref = ref_mesh.cast_rays(rays, 10.0)
-    ntoi, nnrm = _assert_parity(hits, ref, "C0")
-    print(f"C0: toi bit-differences {ntoi}, normal bit-differences {nnrm}")
-    assert ntoi == 0 and nnrm == 0
+    for mapping in MAPPINGS:
+        print(f"case: C0 icosphere s=6 seed=1, rays seed=2 n=1048576, mapping={mapping}", flush=True)
+        hits = _cast_gpu_host(mesh, rays, 10.0, mapping)
+        ntoi, nnrm = _assert_parity(hits, ref, f"C0/{mapping}")
+        print(f"C0/{mapping}: toi bit-differences {ntoi}, normal bit-differences {nnrm}")
+        assert ntoi == 0 and nnrm == 0
     # error against parry3d-f64 (stated, not asserted beyond sanity)
     r64 = orc.OracleTriMesh(v.astype(np.float64), f, f64=True).cast_rays(rays.astype(np.float64), 10.0)
     same = r64["feature"] == hits["feature"]
     hit = same & (hits["feature"] != NO_HIT)
     rel = np.abs(hits["time_of_impact"][hit] - r64["toi"][hit]) / np.abs(r64["toi"][hit])
     nerr = np.abs(hits["normal"][hit] - r64["normal"][hit]).max()
-    print(f"C0 vs f64 oracle: feature agreement {same.mean():.7f}, toi max rel err {rel.max():.3e}, "
-          f"normal max abs err {nerr:.3e}")
-    assert same.mean() > 0.9999 and rel.max() < 1e-5
+    print(f"C0 vs f64 oracle: feature agreement {same.mean():.7f}, toi max rel err {rel.max():.3e} "
+          f"(99.99th pct {np.quantile(rel, 0.9999):.3e}), normal max abs err {nerr:.3e}")
+    # f32 rounding itself (parry3d vs parry3d-f64) is what is measured here:
+    # stated above, bounded only loosely.
+    assert same.mean() > 0.9999 and np.quantile(rel, 0.9999) < 1e-5 and rel.max() < 1e-3
 
 
+@pytest.mark.parametrize("mapping", MAPPINGS)
 @pytest.mark.parametrize("level", [0, 1, 3, 5, 7])
-def test_icosphere_levels(orc, level):
+def test_icosphere_levels(orc, level, mapping):
     v, f = scenes.noisy_icosphere(level, seed=100 + level)
     rays = scenes.rays_into_unit_ball(50000, seed=200 + level)
-    print(f"case: icosphere s={level}", flush=True)
-    hits = _cast_gpu_host(TriMesh(v, f), rays, 10.0)
+    print(f"case: icosphere s={level} mapping={mapping}", flush=True)
+    hits = _cast_gpu_host(TriMesh(v, f), rays, 10.0, mapping)
     _assert_parity(hits, orc.OracleTriMesh(v, f).cast_rays(rays, 10.0), f"s={level}")
 
 
@@ -109,11 +117,16 @@ def test_edge_cases(orc):
     rays[6000:6010, 3:] = 0.0
     mesh = TriMesh(v, f)
     om = orc.OracleTriMesh(v, f)
-    print("case: edge cases", flush=True)
-    for max_toi in (10.0, 2.5, 0.0):
-        hits = _cast_gpu_host(mesh, rays, max_toi)
-        _assert_parity(hits, om.cast_rays(rays, max_toi), f"edge max_toi={max_toi}")
-    assert (hits["feature"] == NO_HIT).all()  # max_toi = 0 never hits (strict <)
+    for mapping in MAPPINGS:
+        print(f"case: edge cases mapping={mapping}", flush=True)
+        for max_toi in (10.0, 2.5, 0.0):
+            hits = _cast_gpu_host(mesh, rays, max_toi, mapping)
+            _assert_parity(hits, om.cast_rays(rays, max_toi), f"edge max_toi={max_toi} {mapping}")
+        assert (hits["feature"] == NO_HIT).all()  # max_toi = 0 never hits (strict <)
+    # batch sizes around the persistent chunk and wave sizes
+    for n_small in (1, 63, 64, 65, 127, 129, 1000):
+        hits = _cast_gpu_host(mesh, rays[:n_small], 10.0, "persistent")
+        _assert_parity(hits, om.cast_rays(rays[:n_small], 10.0), f"n={n_small}")
 
 
 def test_small_trees_and_duplicates(orc):
@@ -123,9 +136,10 @@ def test_small_trees_and_duplicates(orc):
     rays[:, 3:] = np.array([0.1, 0.05, -1.0]) + rays[:, 3:] * 0.3
     for idx in ([[0, 1, 2]], [[0, 1, 2], [3, 4, 5]], [[0, 1, 2], [0, 1, 2]], [[0, 1, 2], [3, 4, 5], [0, 2, 1]]):
         f = np.array(idx, np.uint32)
-        print(f"case: small tree {idx}", flush=True)
-        hits = _cast_gpu_host(TriMesh(tri, f), rays, 100.0)
-        _assert_parity(hits, orc.OracleTriMesh(tri, f).cast_rays(rays, 100.0), f"small {idx}")
+        for mapping in MAPPINGS:
+            print(f"case: small tree {idx} {mapping}", flush=True)
+            hits = _cast_gpu_host(TriMesh(tri, f), rays, 100.0, mapping)
+            _assert_parity(hits, orc.OracleTriMesh(tri, f).cast_rays(rays, 100.0), f"small {idx}")
     # duplicated triangles in a deep tree: ties resolved by visit order
     v, f = scenes.noisy_icosphere(4, seed=12)
     f = np.concatenate([f, f[::3], f[1::5, ::-1]], 0)
@@ -141,14 +155,17 @@ def test_traversal_counts_match_oracle(orc):
 
     v, f = scenes.noisy_icosphere(5, seed=21)
     rays = scenes.rays_into_unit_ball(100000, seed=22)
-    print("case: traversal counts s=5", flush=True)
-    hits, counts = TriMesh(v, f).cast_local_rays_counted(torch.from_numpy(rays).cuda(), 10.0)
-    torch.cuda.synchronize()
     ref = orc.OracleTriMesh(v, f).cast_rays(rays, 10.0, per_ray_counts=True)
-    c = counts.cpu().numpy()
-    assert np.array_equal(c[:, 0], ref["w"].astype(np.int32))
-    assert np.array_equal(c[:, 1], ref["k"].astype(np.int32))
-    assert np.array_equal(hits.cpu().numpy().view(np.uint32)[:, 4], ref["feature"])
+    mesh = TriMesh(v, f)
+    for mapping in MAPPINGS:
+        print(f"case: traversal counts s=5 {mapping}", flush=True)
+        mesh.set_tuning(mapping)
+        hits, counts = mesh.cast_local_rays_counted(torch.from_numpy(rays).cuda(), 10.0)
+        torch.cuda.synchronize()
+        c = counts.cpu().numpy()
+        assert np.array_equal(c[:, 0], ref["w"].astype(np.int32))
+        assert np.array_equal(c[:, 1], ref["k"].astype(np.int32))
+        assert np.array_equal(hits.cpu().numpy().view(np.uint32)[:, 4], ref["feature"])
 
 
 def test_empty_batch_and_device_path(orc):
