@@ -153,6 +153,71 @@ int pq_trimesh_cast_local_rays_counted(const pq_trimesh *mesh, const pq_ray *ray
 int pq_trimesh_cast_local_rays_host(const pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays, float max_toi,
                                     int solid, pq_ray_hit *hits);
 
+/* ---- Shapes and poses ---------------------------------------------------- */
+
+/* parry3d shape types on the path (ShapeType, src/shape/shape.rs:36-94). */
+#define PQ_SHAPE_BALL 0u              /* params[0] = radius (src/shape/ball.rs:50-56) */
+#define PQ_SHAPE_CUBOID 1u            /* params[0..3] = half extents (src/shape/cuboid.rs:69-79) */
+#define PQ_SHAPE_CAPSULE 2u           /* params[0] = half height along y, params[1] = radius (capsule.rs:76-82) */
+#define PQ_SHAPE_CONE 3u              /* params[0] = half height along y, params[1] = radius (cone.rs:80-92) */
+#define PQ_SHAPE_CONVEX_POLYHEDRON 4u /* vertex range [first_vertex, +num_vertices) (convex_polyhedron.rs:172-185) */
+
+typedef struct pq_shape_desc {
+    uint32_t type;
+    uint32_t first_vertex;
+    uint32_t num_vertices;
+    uint32_t reserved; /* must be 0 */
+    float params[4];
+} pq_shape_desc;
+
+/* nalgebra Isometry3<f32> (#[repr(C)]: UnitQuaternion coords [i, j, k, w], then
+ * the translation), 28 bytes: a Rust `&[Isometry3<f32>]` can be passed as is. */
+typedef struct pq_isometry {
+    float rotation[4];
+    float translation[3];
+} pq_isometry;
+
+typedef struct pq_shapes pq_shapes;
+
+/* Uploads a shape set: `shapes` (num_shapes records) and the ConvexPolyhedron
+ * vertex pool (`vertices`, 3*num_vertices floats).  Host pointers, copied. */
+int pq_shapes_create(const pq_shape_desc *shapes, uint32_t num_shapes, const float *vertices,
+                     uint32_t num_vertices, pq_shapes **out);
+int pq_shapes_destroy(pq_shapes *shapes);
+
+/* Shape::compute_aabb(pos) for every shape (src/shape/shape.rs:369 and the
+ * per-type aabb functions of src/bounding_volume/aabb_*.rs).  poses: device,
+ * num_shapes pq_isometry; aabbs: device, num_shapes x {mins[3], maxs[3]}. */
+int pq_shapes_compute_aabbs(const pq_shapes *shapes, const pq_isometry *poses, float *aabbs, void *stream);
+
+/* ---- Broadphase Bvh ------------------------------------------------------- */
+
+typedef struct pq_bvh pq_bvh;
+
+/* Bvh::from_leaves(BvhBuildStrategy::Binned, aabbs) (bvh_tree.rs:1835-1837) built
+ * on the host from `aabbs` (host, n x {mins[3], maxs[3]}) and uploaded. */
+int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out);
+int pq_bvh_destroy(pq_bvh *bvh);
+int pq_bvh_info(const pq_bvh *bvh, uint32_t *num_leaves, uint32_t *num_nodes, uint32_t *depth);
+/* Copies the device tree to host memory (synchronises the default stream). */
+int pq_bvh_export(const pq_bvh *bvh, pq_bvh_node_wide *out_host, uint32_t capacity);
+
+/* One frame's tree update: Bvh::insert_or_update_partially(aabbs[i], i, 0.0) for
+ * every leaf i (bvh_insert.rs:209-231) followed by Bvh::refit (bvh_refit.rs:170-320;
+ * the depth-first layout is already in place, so only boxes, counts and change
+ * flags are recomputed).  aabbs: device, n x {mins[3], maxs[3]}.  Calls that
+ * update one Bvh must be ordered (one stream), as parry's `&mut self`. */
+int pq_bvh_update_and_refit(pq_bvh *bvh, const float *aabbs, void *stream);
+
+/* Bvh::traverse_bvtt_single_tree::<false> (bvh_traverse_bvtt.rs:19-204): every
+ * unordered pair of leaves whose AABBs overlap (closed intervals,
+ * BvhNode::intersects bvh_tree.rs:954-957), written as (i, j) with i < j, grouped
+ * by i in increasing order.  pairs: device, room for `capacity` pairs (2 u32 each);
+ * count: device u64 receiving the total number of pairs (also when it exceeds
+ * `capacity`, in which case only the first `capacity` are written).  The set
+ * equals parry's callback set; the order of the list is this function's. */
+int pq_bvh_self_pairs(pq_bvh *bvh, uint32_t *pairs, uint64_t capacity, uint64_t *count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

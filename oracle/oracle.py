@@ -148,3 +148,84 @@ def triangle_cast_ray(abc, ray, max_toi: float):
     fid = np.zeros(1, np.uint32)
     ok = lib().orc_triangle_cast_ray_f32(t.ctypes.data, r.ctypes.data, max_toi, out.ctypes.data, fid.ctypes.data)
     return (out, int(fid[0])) if ok else None
+
+
+# ---------------------------------------------------------------------------
+# Broadphase
+# ---------------------------------------------------------------------------
+
+_SIG.update({
+    "orc_polyhedra_aabbs_f32": (None, [_vp, _vp, _u32, _vp, _vp, _int]),
+    "orc_polyhedra_aabbs_f64": (None, [_vp, _vp, _u32, _vp, _vp, _int]),
+    "orc_bvh_create_f32": (_vp, [_vp, _u32]),
+    "orc_bvh_free_f32": (None, [_vp]),
+    "orc_bvh_num_nodes_f32": (_u32, [_vp]),
+    "orc_bvh_nodes_f32": (None, [_vp, _vp]),
+    "orc_bvh_update_refit_f32": (None, [_vp, _vp]),
+    "orc_bvh_self_pairs_f32": (None, [_vp, _vp, _u64, _vp, _vp]),
+    "orc_brute_pairs_f32": (None, [_vp, _u32, _vp, _u64, _vp]),
+})
+
+
+def polyhedra_aabbs(vertices, offsets, poses, f64: bool = False, threads: int | None = None) -> np.ndarray:
+    """ConvexPolyhedron::aabb(pose) for every shape (n, 6)."""
+    dt = np.float64 if f64 else np.float32
+    v = np.ascontiguousarray(vertices, dtype=dt).reshape(-1, 3)
+    o = np.ascontiguousarray(offsets, dtype=np.uint32)
+    p = np.ascontiguousarray(poses, dtype=dt).reshape(-1, 7)
+    n = o.shape[0] - 1
+    out = np.zeros((n, 6), dtype=dt)
+    f = lib().orc_polyhedra_aabbs_f64 if f64 else lib().orc_polyhedra_aabbs_f32
+    f(v.ctypes.data, o.ctypes.data, n, p.ctypes.data, out.ctypes.data, default_threads() if threads is None else threads)
+    return out
+
+
+class OracleBvh:
+    """parry3d Bvh (f32) restated: from_leaves(Binned), update+refit, self-BVTT."""
+
+    def __init__(self, aabbs):
+        a = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
+        self.n = a.shape[0]
+        self.h = lib().orc_bvh_create_f32(a.ctypes.data, self.n)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_bvh_free_f32(self.h)
+            self.h = None
+
+    def nodes(self) -> np.ndarray:
+        from parry_amd.shape import BVH_NODE_WIDE_DTYPE
+
+        out = np.zeros(lib().orc_bvh_num_nodes_f32(self.h), dtype=BVH_NODE_WIDE_DTYPE)
+        lib().orc_bvh_nodes_f32(self.h, out.ctypes.data)
+        return out
+
+    def update_and_refit(self, aabbs) -> None:
+        a = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
+        assert a.shape[0] == self.n
+        lib().orc_bvh_update_refit_f32(self.h, a.ctypes.data)
+
+    def self_pairs(self):
+        """(pairs uint32 (m, 2) in parry's callback order, wide records touched)."""
+        cnt = np.zeros(1, np.uint64)
+        touched = np.zeros(1, np.uint64)
+        lib().orc_bvh_self_pairs_f32(self.h, None, 0, cnt.ctypes.data, touched.ctypes.data)
+        out = np.zeros((int(cnt[0]), 2), np.uint32)
+        lib().orc_bvh_self_pairs_f32(self.h, out.ctypes.data, out.shape[0], cnt.ctypes.data, touched.ctypes.data)
+        return out, int(touched[0])
+
+
+def brute_pairs(aabbs) -> np.ndarray:
+    """Every pair i < j of closed-overlapping AABBs by exhaustive test (small n only)."""
+    a = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
+    cnt = np.zeros(1, np.uint64)
+    lib().orc_brute_pairs_f32(a.ctypes.data, a.shape[0], None, 0, cnt.ctypes.data)
+    out = np.zeros((int(cnt[0]), 2), np.uint32)
+    lib().orc_brute_pairs_f32(a.ctypes.data, a.shape[0], out.ctypes.data, out.shape[0], cnt.ctypes.data)
+    return out
+
+
+def pair_set(pairs) -> set:
+    p = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    lo, hi = np.minimum(p[:, 0], p[:, 1]), np.maximum(p[:, 0], p[:, 1])
+    return set(zip(lo.tolist(), hi.tolist()))

@@ -178,3 +178,121 @@ int orc_triangle_cast_ray_f32(const float *abc, const float *ray, float max_toi,
 }
 
 }  // extern "C"
+
+// ---- broadphase ------------------------------------------------------------
+
+struct OrcBvh {
+    Bvh<float> bvh;
+    std::vector<LeafSlot> slots;
+    uint32_t n;
+};
+
+static Aabb<float> aabb_at(const float *a, uint32_t i) {
+    return Aabb<float>{v3(a[6 * i], a[6 * i + 1], a[6 * i + 2]), v3(a[6 * i + 3], a[6 * i + 4], a[6 * i + 5])};
+}
+
+template <class T>
+static void polyhedra_aabbs(const T *verts, const uint32_t *offsets, uint32_t n, const T *poses, T *out, int nthreads) {
+    parallel_for(n, nthreads, 256, [&](uint64_t b, uint64_t e) {
+        std::vector<V3<T>> pts;
+        for (uint64_t i = b; i < e; ++i) {
+            pts.clear();
+            for (uint32_t k = offsets[i]; k < offsets[i + 1]; ++k)
+                pts.push_back(v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]));
+            const T *p = poses + 7 * i;
+            Iso<T> m{Quat<T>{p[0], p[1], p[2], p[3]}, v3(p[4], p[5], p[6])};
+            Aabb<T> a = point_cloud_aabb(m, pts.data(), pts.size());
+            T *o = out + 6 * i;
+            o[0] = a.mins.x; o[1] = a.mins.y; o[2] = a.mins.z;
+            o[3] = a.maxs.x; o[4] = a.maxs.y; o[5] = a.maxs.z;
+        }
+    });
+}
+
+extern "C" {
+
+void orc_polyhedra_aabbs_f32(const float *verts, const uint32_t *offsets, uint32_t n, const float *poses, float *out,
+                             int nthreads) {
+    polyhedra_aabbs<float>(verts, offsets, n, poses, out, nthreads);
+}
+void orc_polyhedra_aabbs_f64(const double *verts, const uint32_t *offsets, uint32_t n, const double *poses,
+                             double *out, int nthreads) {
+    polyhedra_aabbs<double>(verts, offsets, n, poses, out, nthreads);
+}
+
+// Bvh::from_leaves(Binned, aabbs) over n leaf AABBs (n*6 floats).
+void *orc_bvh_create_f32(const float *aabbs, uint32_t n) {
+    auto *h = new OrcBvh();
+    std::vector<Aabb<float>> leaves(n);
+    for (uint32_t i = 0; i < n; ++i) leaves[i] = aabb_at(aabbs, i);
+    h->bvh = bvh_from_leaves(leaves);
+    h->slots = leaf_slots(h->bvh, n);
+    h->n = n;
+    return h;
+}
+void orc_bvh_free_f32(void *h) { delete (OrcBvh *)h; }
+uint32_t orc_bvh_num_nodes_f32(void *h) { return (uint32_t)((OrcBvh *)h)->bvh.nodes.size(); }
+void orc_bvh_nodes_f32(void *hp, void *out) {
+    auto *h = (OrcBvh *)hp;
+    auto *o = (WideF32 *)out;
+    for (size_t i = 0; i < h->bvh.nodes.size(); ++i) {
+        const auto &w = h->bvh.nodes[i];
+        WideF32 r;
+        for (int d = 0; d < 3; ++d) {
+            r.lmins[d] = w.left.mins[d];
+            r.lmaxs[d] = w.left.maxs[d];
+            r.rmins[d] = w.right.mins[d];
+            r.rmaxs[d] = w.right.maxs[d];
+        }
+        r.lchildren = w.left.children;
+        r.ldata = w.left.data;
+        r.rchildren = w.right.children;
+        r.rdata = w.right.data;
+        o[i] = r;
+    }
+}
+
+// One broadphase frame's tree update: insert_or_update_partially(aabb_i, i, 0.0)
+// for every leaf (bvh_insert.rs:209-231) followed by refit (bvh_refit.rs:170-180).
+void orc_bvh_update_refit_f32(void *hp, const float *aabbs) {
+    auto *h = (OrcBvh *)hp;
+    for (uint32_t i = 0; i < h->n; ++i) update_leaf(h->bvh, h->slots[i], aabb_at(aabbs, i));
+    refit(h->bvh);
+    h->slots = leaf_slots(h->bvh, h->n);
+}
+
+// traverse_bvtt_single_tree::<false>; pairs in callback order (2 u32 each).
+// Returns 0 and sets *count to the number of pairs; fills `out` up to `cap` pairs.
+void orc_bvh_self_pairs_f32(void *hp, uint32_t *out, uint64_t cap, uint64_t *count, uint64_t *wide_touched) {
+    auto *h = (OrcBvh *)hp;
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    BvttStats st;
+    self_pairs(h->bvh, pairs, st);
+    *count = pairs.size();
+    if (wide_touched) *wide_touched = st.wide_touched;
+    for (uint64_t i = 0; i < pairs.size() && i < cap; ++i) {
+        out[2 * i] = pairs[i].first;
+        out[2 * i + 1] = pairs[i].second;
+    }
+}
+
+// Every unordered pair (i < j) of closed-overlapping AABBs, by exhaustive test
+// (O(n^2); the ground truth for small sets).
+void orc_brute_pairs_f32(const float *aabbs, uint32_t n, uint32_t *out, uint64_t cap, uint64_t *count) {
+    uint64_t c = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        Aabb<float> a = aabb_at(aabbs, i);
+        for (uint32_t j = i + 1; j < n; ++j) {
+            if (a.intersects(aabb_at(aabbs, j))) {
+                if (c < cap) {
+                    out[2 * c] = i;
+                    out[2 * c + 1] = j;
+                }
+                ++c;
+            }
+        }
+    }
+    *count = c;
+}
+
+}  // extern "C"

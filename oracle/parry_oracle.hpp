@@ -608,4 +608,161 @@ bool trimesh_cast_ray_brute(const TriMesh<T> &m, const Ray<T> &ray, T max_toi, R
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Isometries (nalgebra Isometry3 = UnitQuaternion + Translation), restated from
+// nalgebra 0.34's published implementation (not present in the reference tree,
+// hence "parity unpinned" at the last ulp; see the file header):
+//   q * v       : t = (q.v x v) * 2 ; r = (t * q.w + q.v x t) + v
+//   iso * p     : q * p + t
+//   q^-1        : conjugate (w, -x, -y, -z)
+//   q1 * q2     : Hamilton product in nalgebra's term order
+//   a.inv_mul(b): (q_a^-1 * (t_b - t_a), q_a^-1 * q_b)
+// ---------------------------------------------------------------------------
+template <class T> struct Quat {
+    T x, y, z, w;  // nalgebra storage order [i, j, k, w]
+};
+template <class T> struct Iso {
+    Quat<T> q;
+    V3<T> t;
+};
+template <class T> inline V3<T> rotate(const Quat<T> &q, V3<T> v) {
+    V3<T> qv{q.x, q.y, q.z};
+    V3<T> t = cross(qv, v) * T(2);
+    V3<T> c = cross(qv, t);
+    return (t * q.w + c) + v;
+}
+template <class T> inline Quat<T> conj(const Quat<T> &q) { return {-q.x, -q.y, -q.z, q.w}; }
+template <class T> inline V3<T> inv_rotate(const Quat<T> &q, V3<T> v) { return rotate(conj(q), v); }
+template <class T> inline V3<T> transform_point(const Iso<T> &m, V3<T> p) { return rotate(m.q, p) + m.t; }
+template <class T> inline Quat<T> qmul(const Quat<T> &a, const Quat<T> &b) {
+    Quat<T> r;
+    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+    r.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
+    r.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+    return r;
+}
+template <class T> inline Iso<T> inv_mul(const Iso<T> &a, const Iso<T> &b) {
+    Quat<T> ia = conj(a.q);
+    return Iso<T>{qmul(ia, b.q), rotate(ia, b.t - a.t)};
+}
+
+// point_cloud_aabb (aabb_utils.rs:66-86), the AABB of a ConvexPolyhedron
+// (aabb_convex_polyhedron.rs:8-10).
+template <class T> inline Aabb<T> point_cloud_aabb(const Iso<T> &m, const V3<T> *pts, size_t n) {
+    V3<T> w0 = transform_point(m, pts[0]);
+    Aabb<T> r{w0, w0};
+    for (size_t i = 1; i < n; ++i) {
+        V3<T> w = transform_point(m, pts[i]);
+        r.mins = vmin(r.mins, w);
+        r.maxs = vmax(r.maxs, w);
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Broadphase: leaf updates, refit and self-BVTT (bvh_insert.rs,
+// bvh_refit.rs, bvh_traverse_bvtt.rs).
+// ---------------------------------------------------------------------------
+struct LeafSlot {
+    uint32_t wide;
+    bool right;
+};
+
+// leaf_node_indices of a Bvh: where each leaf lives.
+template <class T> std::vector<LeafSlot> leaf_slots(const Bvh<T> &b, uint32_t n_leaves) {
+    std::vector<LeafSlot> s(n_leaves, LeafSlot{0, false});
+    for (uint32_t i = 0; i < b.nodes.size(); ++i) {
+        const Wide<T> &w = b.nodes[i];
+        if (w.left.leaf_count() == 1) s[w.left.children] = LeafSlot{i, false};
+        if (w.right.leaf_count() == 1) s[w.right.children] = LeafSlot{i, true};
+    }
+    return s;
+}
+
+// insert_or_update_partially with change_detection_margin = 0 (bvh_insert.rs:209-231).
+template <class T> void update_leaf(Bvh<T> &b, const LeafSlot &s, const Aabb<T> &a) {
+    Node<T> &n = s.right ? b.nodes[s.wide].right : b.nodes[s.wide].left;
+    n.mins = a.mins;
+    n.maxs = a.maxs;
+}
+
+struct BvttStats {
+    uint64_t wide_touched = 0;  // wide-node records read
+};
+
+template <class T> struct SelfBvtt {
+    const Bvh<T> &b;
+    std::vector<std::pair<uint32_t, uint32_t>> &out;
+    BvttStats &st;
+    std::vector<uint32_t> stack;
+
+    void single_subtree(const Node<T> &node, uint32_t subtree) {  // bvh_traverse_bvtt.rs:145-204
+        uint32_t curr = subtree;
+        for (;;) {
+            st.wide_touched++;
+            const Wide<T> &w = b.nodes[curr];
+            bool lc = node.intersects(w.left);
+            bool rc = node.intersects(w.right);
+            bool found = false;
+            if (lc) {
+                if (w.left.is_leaf()) out.push_back({node.children, w.left.children});
+                else { curr = w.left.children; found = true; }
+            }
+            if (rc) {
+                if (w.right.is_leaf()) out.push_back({node.children, w.right.children});
+                else if (!found) { curr = w.right.children; found = true; }
+                else stack.push_back(w.right.children);
+            }
+            if (!found) {
+                if (stack.empty()) return;
+                curr = stack.back();
+                stack.pop_back();
+            }
+        }
+    }
+    void dispatch(bool check, const Node<T> &a, const Node<T> &c) {
+        if (!check) return;
+        if (a.is_leaf() && c.is_leaf()) out.push_back({a.children, c.children});
+        else if (a.is_leaf()) single_subtree(a, c.children);
+        else if (c.is_leaf()) single_subtree(c, a.children);
+        else two_branches(a.children, c.children);
+    }
+    void two_branches(uint32_t x, uint32_t y) {  // bvh_traverse_bvtt.rs:89-142
+        st.wide_touched += 2;
+        const Wide<T> &n1 = b.nodes[x];
+        const Wide<T> &n2 = b.nodes[y];
+        bool ll = n1.left.intersects(n2.left);
+        bool lr = n1.left.intersects(n2.right);
+        bool rl = n1.right.intersects(n2.left);
+        bool rr = n1.right.intersects(n2.right);
+        dispatch(ll, n1.left, n2.left);
+        dispatch(lr, n1.left, n2.right);
+        dispatch(rl, n1.right, n2.left);
+        dispatch(rr, n1.right, n2.right);
+    }
+    void self_node(uint32_t id) {  // bvh_traverse_bvtt.rs:38-87
+        st.wide_touched++;
+        const Wide<T> &n = b.nodes[id];
+        bool lr = n.left.intersects(n.right);
+        if (!n.left.is_leaf()) self_node(n.left.children);
+        if (!n.right.is_leaf()) self_node(n.right.children);
+        if (lr) {
+            bool ll = n.left.is_leaf(), rl = n.right.is_leaf();
+            if (ll && rl) out.push_back({n.left.children, n.right.children});
+            else if (ll) single_subtree(n.left, n.right.children);
+            else if (rl) single_subtree(n.right, n.left.children);
+            else two_branches(n.left.children, n.right.children);
+        }
+    }
+};
+
+// traverse_bvtt_single_tree::<false> (bvh_traverse_bvtt.rs:19-31).
+template <class T>
+void self_pairs(const Bvh<T> &b, std::vector<std::pair<uint32_t, uint32_t>> &out, BvttStats &st) {
+    if (b.nodes.empty() || b.nodes[0].right.leaf_count() == 0) return;
+    SelfBvtt<T> v{b, out, st, {}};
+    v.self_node(0);
+}
+
 }  // namespace oracle

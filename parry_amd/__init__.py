@@ -4,7 +4,20 @@ The compute path is the HIP library ``libparry_mi355x.so`` (C ABI in
 include/parry_mi355x.h); this package is its Python host mirror.
 """
 from .query import Ray, RayIntersection, FeatureId, RAY_DTYPE, RAY_HIT_DTYPE
-from .shape import TriMesh, TriMeshBuilderError, BVH_NODE_WIDE_DTYPE, split_hits
+from .shape import (
+    TriMesh,
+    TriMeshBuilderError,
+    BVH_NODE_WIDE_DTYPE,
+    split_hits,
+    ShapeSet,
+    SHAPE_DESC_DTYPE,
+    SHAPE_BALL,
+    SHAPE_CUBOID,
+    SHAPE_CAPSULE,
+    SHAPE_CONE,
+    SHAPE_CONVEX_POLYHEDRON,
+)
+from .partitioning import Bvh
 from . import scenes
 
 __all__ = [
@@ -17,5 +30,8 @@ __all__ = [
     "TriMeshBuilderError",
     "BVH_NODE_WIDE_DTYPE",
     "split_hits",
+    "ShapeSet",
+    "SHAPE_DESC_DTYPE",
+    "Bvh",
     "scenes",
 ]

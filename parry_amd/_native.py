@@ -45,6 +45,15 @@ SIGNATURES = {
     "pq_trimesh_cast_local_rays": (_int, [_vp, _vp, _u64, _f32, _int, _vp, _vp]),
     "pq_trimesh_cast_local_rays_counted": (_int, [_vp, _vp, _u64, _f32, _int, _vp, _vp, _vp]),
     "pq_trimesh_cast_local_rays_host": (_int, [_vp, _vp, _u64, _f32, _int, _vp]),
+    "pq_shapes_create": (_int, [_vp, _u32, _vp, _u32, _c.POINTER(_vp)]),
+    "pq_shapes_destroy": (_int, [_vp]),
+    "pq_shapes_compute_aabbs": (_int, [_vp, _vp, _vp, _vp]),
+    "pq_bvh_create": (_int, [_vp, _u32, _c.POINTER(_vp)]),
+    "pq_bvh_destroy": (_int, [_vp]),
+    "pq_bvh_info": (_int, [_vp, _c.POINTER(_u32), _c.POINTER(_u32), _c.POINTER(_u32)]),
+    "pq_bvh_export": (_int, [_vp, _vp, _u32]),
+    "pq_bvh_update_and_refit": (_int, [_vp, _vp, _vp]),
+    "pq_bvh_self_pairs": (_int, [_vp, _vp, _u64, _vp, _vp]),
 }
 
 _lock = threading.Lock()

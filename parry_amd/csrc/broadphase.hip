@@ -1,0 +1,211 @@
+// broadphase.hip -- Bvh leaf update + refit and self-overlap pairs (gfx950).
+//
+// Refit (Bvh::insert_or_update_partially with margin 0 for every leaf, then
+// Bvh::refit, src/partitioning/bvh/bvh_insert.rs:209-231, bvh_refit.rs:170-320):
+// one lane per leaf writes the leaf's new box into its slot and climbs towards
+// the root.  Every wide node counts the arrivals of its two child slots; the
+// first arriver stops, the second one (which now sees both children) merges
+// them -- min/max, leaf counts, change flags, exactly BvhNode::merged
+// (bvh_tree.rs:610-618) -- writes the result into the parent's slot and climbs
+// on.  No lane ever waits, so the launch cannot stall; the second arriver
+// zeroes the counter, so counters are zero again for the next launch and no
+// clearing pass (memset) is needed.  Topology never changes, and the stored
+// layout is already depth-first, so parry's relayout is the identity.
+//
+// Pairs (Bvh::traverse_bvtt_single_tree::<false>, bvh_traverse_bvtt.rs:19-204):
+// parry reports every pair of leaves whose boxes overlap, which is the set
+// {(i, j) : i < j, intersects(i, j)} because every ancestor box contains its
+// descendants exactly.  One lane per leaf i walks the tree from the root with
+// an LDS stack and reports the leaves j > i it overlaps.  The list is written
+// without atomics: a counting pass, an exclusive scan of the per-leaf counts
+// (rocPRIM), and a writing pass that places leaf i's pairs at its offset.
+#include <cstring>
+
+#include <rocprim/device/device_scan.hpp>
+
+#include "pq_internal.h"
+
+namespace pq {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kFlagChanged = 1u, kFlagPending = 3u;
+
+__device__ __forceinline__ uint32_t ld_coherent(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_coherent_f(const float *p) {
+    return __uint_as_float(ld_coherent(reinterpret_cast<const uint32_t *>(p)));
+}
+
+// Loads a BvhNode (mins, children, maxs, data) written by another lane of the
+// same launch, bypassing the non-coherent per-CU cache.
+__device__ __forceinline__ void load_node_coherent(const pq_bvh_node *n, float mn[3], float mx[3], uint32_t *data) {
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = ld_coherent_f(&n->mins[d]);
+        mx[d] = ld_coherent_f(&n->maxs[d]);
+    }
+    *data = ld_coherent(&n->data);
+}
+
+__global__ __launch_bounds__(kBlock) void refit_kernel(pq_bvh_node_wide *__restrict__ nodes,
+                                                       const uint32_t *__restrict__ leaf_slot,
+                                                       const uint32_t *__restrict__ parent_slot,
+                                                       uint32_t *__restrict__ arrive, const float *__restrict__ aabbs,
+                                                       uint32_t n, int propagate) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t slot = leaf_slot[i];
+    uint32_t wide = slot >> 1;
+    pq_bvh_node *leaf = (slot & 1u) ? &nodes[wide].right : &nodes[wide].left;
+    const float *a = aabbs + 6 * (size_t)i;
+    leaf->mins[0] = a[0];
+    leaf->mins[1] = a[1];
+    leaf->mins[2] = a[2];
+    leaf->maxs[0] = a[3];
+    leaf->maxs[1] = a[4];
+    leaf->maxs[2] = a[5];
+    // resolve_pending_change (bvh_tree.rs:192-198)
+    const uint32_t d = leaf->data;
+    leaf->data = (d & 0x3fffffffu) | (((d >> 30) == kFlagPending ? kFlagChanged : 0u) << 30);
+    if (!propagate) return;  // a root with at most two leaves has no ancestor boxes
+    for (;;) {
+        __threadfence();
+        if (atomicAdd(&arrive[wide], 1u) == 0u) return;  // the sibling slot is not written yet
+        arrive[wide] = 0u;
+        __threadfence();
+        if (wide == 0) return;  // the root's own box is not stored
+        float lmn[3], lmx[3], rmn[3], rmx[3];
+        uint32_t ld, rd;
+        load_node_coherent(&nodes[wide].left, lmn, lmx, &ld);
+        load_node_coherent(&nodes[wide].right, rmn, rmx, &rd);
+        const uint32_t ps = parent_slot[wide];
+        pq_bvh_node *dst = (ps & 1u) ? &nodes[ps >> 1].right : &nodes[ps >> 1].left;
+        for (int k = 0; k < 3; ++k) {
+            dst->mins[k] = fminf(lmn[k], rmn[k]);
+            dst->maxs[k] = fmaxf(lmx[k], rmx[k]);
+        }
+        dst->data = ((ld & 0x3fffffffu) + (rd & 0x3fffffffu)) | (((ld >> 30) | (rd >> 30)) << 30);
+        wide = ps >> 1;
+    }
+}
+
+__device__ __forceinline__ bool overlaps(const float q[6], float4 mn, float4 mx) {
+    // BvhNode::intersects: mins <= other.maxs && maxs >= other.mins on every axis.
+    return (q[0] <= mx.x) & (q[1] <= mx.y) & (q[2] <= mx.z) & (q[3] >= mn.x) & (q[4] >= mn.y) & (q[5] >= mn.z);
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void self_pairs_kernel(const pq_bvh_node_wide *__restrict__ nodes,
+                                                            const uint32_t *__restrict__ leaf_slot, uint32_t n,
+                                                            uint32_t *__restrict__ counts,
+                                                            const unsigned long long *__restrict__ offsets,
+                                                            uint32_t *__restrict__ pairs, uint64_t capacity) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t *stk = lds_stack + threadIdx.x;
+    const float4 *node4 = reinterpret_cast<const float4 *>(nodes);
+    const uint32_t slot = leaf_slot[i];
+    const float4 a0 = node4[4 * (slot >> 1) + 2 * (slot & 1u)];
+    const float4 a1 = node4[4 * (slot >> 1) + 2 * (slot & 1u) + 1];
+    const float q[6] = {a0.x, a0.y, a0.z, a1.x, a1.y, a1.z};
+    uint64_t base = 0;
+    if (WRITE) base = offsets[i];
+    uint32_t found = 0;
+    uint32_t curr = 0, sp = 0;
+    for (;;) {
+        const float4 l0 = node4[4 * curr + 0], l1 = node4[4 * curr + 1];
+        const float4 r0 = node4[4 * curr + 2], r1 = node4[4 * curr + 3];
+        const uint32_t lc = __float_as_uint(l0.w), ld = __float_as_uint(l1.w);
+        const uint32_t rc = __float_as_uint(r0.w), rd = __float_as_uint(r1.w);
+        const bool lhit = overlaps(q, l0, l1);
+        const bool rhit = overlaps(q, r0, r1) & ((rd & 0x3fffffffu) != 0u);
+        bool next = false;
+        if (lhit) {
+            if ((ld & 0x3fffffffu) == 1u) {
+                if (lc > i) {
+                    if (WRITE && base + found < capacity) {
+                        pairs[2 * (base + found)] = i;
+                        pairs[2 * (base + found) + 1] = lc;
+                    }
+                    ++found;
+                }
+            } else {
+                curr = lc;
+                next = true;
+            }
+        }
+        if (rhit) {
+            if ((rd & 0x3fffffffu) == 1u) {
+                if (rc > i) {
+                    if (WRITE && base + found < capacity) {
+                        pairs[2 * (base + found)] = i;
+                        pairs[2 * (base + found) + 1] = rc;
+                    }
+                    ++found;
+                }
+            } else if (next) {
+                // at most depth-1 pending entries (each one level deeper than the
+                // one below it); the host sizes the LDS stack to that
+                stk[sp * kBlock] = rc;
+                ++sp;
+            } else {
+                curr = rc;
+                next = true;
+            }
+        }
+        if (!next) {
+            if (sp == 0) break;
+            --sp;
+            curr = stk[sp * kBlock];
+        }
+    }
+    if (!WRITE) counts[i] = found;
+}
+
+__global__ void copy_total_kernel(const unsigned long long *offsets, uint32_t n, unsigned long long *count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *count = offsets[n];
+}
+
+}  // namespace
+
+hipError_t launch_bvh_refit(const DeviceBvh &b, const float *aabbs, hipStream_t stream) {
+    if (b.num_leaves == 0) return hipSuccess;
+    hipLaunchKernelGGL(refit_kernel, dim3((b.num_leaves + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, b.nodes,
+                       b.leaf_slot, b.parent_slot, b.arrive, aabbs, b.num_leaves, b.num_leaves > 2 ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t bvh_scan_temp_bytes(uint32_t num_leaves, size_t *bytes) {
+    return rocprim::exclusive_scan(nullptr, *bytes, (const uint32_t *)nullptr, (unsigned long long *)nullptr,
+                                   0ull, (size_t)num_leaves + 1, rocprim::plus<unsigned long long>());
+}
+
+hipError_t launch_bvh_self_pairs(const DeviceBvh &b, uint32_t *pairs, uint64_t capacity, unsigned long long *count,
+                                 hipStream_t stream) {
+    const uint32_t n = b.num_leaves;
+    if (n < 2) {
+        // traverse_bvtt_single_tree returns at once (bvh_traverse_bvtt.rs:24-27).
+        hipLaunchKernelGGL(copy_total_kernel, dim3(1), dim3(64), 0, stream, b.pair_offsets, 0u, count);
+        return hipGetLastError();
+    }
+    const uint32_t cap = b.depth > 1 ? b.depth - 1 : 1;
+    const size_t lds = (size_t)cap * kBlock * sizeof(uint32_t);
+    const dim3 grid((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(self_pairs_kernel<false>, grid, dim3(kBlock), lds, stream, b.nodes, b.leaf_slot, n,
+                       b.pair_counts, b.pair_offsets, pairs, capacity);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t bytes = b.scan_temp_bytes;
+    e = rocprim::exclusive_scan(b.scan_temp, bytes, b.pair_counts, b.pair_offsets, 0ull, (size_t)n + 1,
+                                rocprim::plus<unsigned long long>(), stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(self_pairs_kernel<true>, grid, dim3(kBlock), lds, stream, b.nodes, b.leaf_slot, n,
+                       b.pair_counts, b.pair_offsets, pairs, capacity);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(copy_total_kernel, dim3(1), dim3(64), 0, stream, b.pair_offsets, n, count);
+    return hipGetLastError();
+}
+
+}  // namespace pq

@@ -1,0 +1,191 @@
+// capi_broadphase.cpp -- extern "C" entry points for shapes and the broadphase Bvh.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pq_internal.h"
+
+using namespace pq;
+
+namespace {
+
+int dev_alloc(pq_bvh *b, void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    b->allocations.push_back(*p);
+    return PQ_OK;
+}
+
+constexpr uint32_t kMaxBroadphaseDepth = 65;
+
+}  // namespace
+
+extern "C" {
+
+int pq_shapes_create(const pq_shape_desc *shapes, uint32_t n, const float *vertices, uint32_t num_vertices,
+                     pq_shapes **out) {
+    if (!out) return fail(PQ_ERR_INVALID, "pq_shapes_create: null output");
+    *out = nullptr;
+    if (n && !shapes) return fail(PQ_ERR_INVALID, "pq_shapes_create: null shapes");
+    if (num_vertices && !vertices) return fail(PQ_ERR_INVALID, "pq_shapes_create: null vertices");
+    std::vector<ShapeRecord> rec(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const pq_shape_desc &d = shapes[i];
+        if (d.reserved != 0) return fail(PQ_ERR_INVALID, "pq_shapes_create: reserved field must be 0");
+        switch (d.type) {
+            case PQ_SHAPE_BALL:
+                break;
+            case PQ_SHAPE_CONVEX_POLYHEDRON:
+                if (d.num_vertices == 0 || (uint64_t)d.first_vertex + d.num_vertices > num_vertices)
+                    return fail(PQ_ERR_INVALID, "pq_shapes_create: shape " + std::to_string(i) +
+                                                    " has an empty or out-of-range vertex range");
+                break;
+            case PQ_SHAPE_CUBOID:
+            case PQ_SHAPE_CAPSULE:
+            case PQ_SHAPE_CONE:
+                return fail(PQ_ERR_UNSUPPORTED, "pq_shapes_create: shape type " + std::to_string(d.type) +
+                                                    " is not supported by this build yet");
+            default:
+                return fail(PQ_ERR_INVALID, "pq_shapes_create: unknown shape type " + std::to_string(d.type));
+        }
+        rec[i].type = d.type;
+        rec[i].first_vertex = d.first_vertex;
+        rec[i].num_vertices = d.num_vertices;
+        rec[i].pad = 0;
+        std::memcpy(rec[i].param, d.params, sizeof(rec[i].param));
+    }
+    std::vector<float4> v4(num_vertices);
+    for (uint32_t k = 0; k < num_vertices; ++k)
+        v4[k] = make_float4(vertices[3 * k], vertices[3 * k + 1], vertices[3 * k + 2], 0.0f);
+    auto *s = new pq_shapes();
+    s->host_rec = rec;
+    hipError_t e = hipMalloc(&s->d_rec, rec.size() * sizeof(ShapeRecord) + 16);
+    if (e == hipSuccess) e = hipMalloc(&s->d_verts, v4.size() * sizeof(float4) + 16);
+    if (e == hipSuccess && n) e = hipMemcpy(s->d_rec, rec.data(), rec.size() * sizeof(ShapeRecord), hipMemcpyHostToDevice);
+    if (e == hipSuccess && num_vertices)
+        e = hipMemcpy(s->d_verts, v4.data(), v4.size() * sizeof(float4), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        pq_shapes_destroy(s);
+        return hip_fail(e, "pq_shapes_create upload");
+    }
+    s->dev = DeviceShapes{(const ShapeRecord *)s->d_rec, (const float4 *)s->d_verts, n, num_vertices};
+    *out = s;
+    return PQ_OK;
+}
+
+int pq_shapes_destroy(pq_shapes *s) {
+    if (!s) return PQ_OK;
+    if (s->d_rec) (void)hipFree(s->d_rec);
+    if (s->d_verts) (void)hipFree(s->d_verts);
+    delete s;
+    return PQ_OK;
+}
+
+int pq_shapes_compute_aabbs(const pq_shapes *s, const pq_isometry *poses, float *aabbs, void *stream) {
+    if (!s) return fail(PQ_ERR_INVALID, "pq_shapes_compute_aabbs: null shapes");
+    if (s->dev.n && (!poses || !aabbs)) return fail(PQ_ERR_INVALID, "pq_shapes_compute_aabbs: null buffer");
+    PQ_HIP_CHECK(launch_shape_aabbs(s->dev, (const float *)poses, aabbs, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out) {
+    if (!out) return fail(PQ_ERR_INVALID, "pq_bvh_create: null output");
+    *out = nullptr;
+    if (n && !aabbs) return fail(PQ_ERR_INVALID, "pq_bvh_create: null aabbs");
+    std::vector<LeafBox> leaves(n);
+    if (n) std::memcpy(leaves.data(), aabbs, sizeof(LeafBox) * (size_t)n);
+    std::vector<pq_bvh_node_wide> nodes;
+    uint32_t depth = 0;
+    build_binned_bvh(leaves, nodes, &depth);
+    if (depth > kMaxBroadphaseDepth)
+        return fail(PQ_ERR_UNSUPPORTED, "pq_bvh_create: Bvh depth " + std::to_string(depth) + " exceeds " +
+                                            std::to_string(kMaxBroadphaseDepth));
+    const uint32_t nn = (uint32_t)nodes.size();
+    std::vector<uint32_t> leaf_slot(n, 0), parent_slot(nn ? nn : 1, UINT32_MAX);
+    for (uint32_t w = 0; w < nn; ++w) {
+        const pq_bvh_node *side[2] = {&nodes[w].left, &nodes[w].right};
+        for (uint32_t s = 0; s < 2; ++s) {
+            const pq_bvh_node &c = *side[s];
+            const uint32_t lc = leaf_count(c);
+            if (lc == 1) leaf_slot[c.children] = 2 * w + s;
+            else if (lc > 1) parent_slot[c.children] = 2 * w + s;
+        }
+    }
+    auto *b = new pq_bvh();
+    DeviceBvh &d = b->dev;
+    d.num_leaves = n;
+    d.num_nodes = nn;
+    d.depth = depth;
+    int rc = PQ_OK;
+    size_t scan = 0;
+    hipError_t e = bvh_scan_temp_bytes(n, &scan);
+    if (e != hipSuccess) rc = hip_fail(e, "rocprim::exclusive_scan (size query)");
+    if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.nodes, nn * sizeof(pq_bvh_node_wide));
+    if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.leaf_slot, n * sizeof(uint32_t));
+    if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.parent_slot, parent_slot.size() * sizeof(uint32_t));
+    if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.arrive, parent_slot.size() * sizeof(uint32_t));
+    if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.pair_counts, ((size_t)n + 1) * sizeof(uint32_t));
+    if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.pair_offsets, ((size_t)n + 1) * sizeof(unsigned long long));
+    if (rc == PQ_OK) rc = dev_alloc(b, &d.scan_temp, scan);
+    d.scan_temp_bytes = scan;
+    if (rc == PQ_OK) {
+        e = hipSuccess;
+        if (nn) e = hipMemcpy(d.nodes, nodes.data(), nn * sizeof(pq_bvh_node_wide), hipMemcpyHostToDevice);
+        if (e == hipSuccess && n)
+            e = hipMemcpy(d.leaf_slot, leaf_slot.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(d.parent_slot, parent_slot.data(), parent_slot.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(d.arrive, 0, parent_slot.size() * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemset(d.pair_counts, 0, ((size_t)n + 1) * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemset(d.pair_offsets, 0, ((size_t)n + 1) * sizeof(unsigned long long));
+        if (e != hipSuccess) rc = hip_fail(e, "pq_bvh_create upload");
+    }
+    if (rc != PQ_OK) {
+        pq_bvh_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return PQ_OK;
+}
+
+int pq_bvh_destroy(pq_bvh *b) {
+    if (!b) return PQ_OK;
+    for (void *p : b->allocations) (void)hipFree(p);
+    delete b;
+    return PQ_OK;
+}
+
+int pq_bvh_info(const pq_bvh *b, uint32_t *num_leaves, uint32_t *num_nodes, uint32_t *depth) {
+    if (!b) return fail(PQ_ERR_INVALID, "pq_bvh_info: null bvh");
+    if (num_leaves) *num_leaves = b->dev.num_leaves;
+    if (num_nodes) *num_nodes = b->dev.num_nodes;
+    if (depth) *depth = b->dev.depth;
+    return PQ_OK;
+}
+
+int pq_bvh_export(const pq_bvh *b, pq_bvh_node_wide *out, uint32_t capacity) {
+    if (!b || !out) return fail(PQ_ERR_INVALID, "pq_bvh_export: null argument");
+    if (capacity < b->dev.num_nodes)
+        return fail(PQ_ERR_CAPACITY, "pq_bvh_export: need " + std::to_string(b->dev.num_nodes) + " records");
+    PQ_HIP_CHECK(hipDeviceSynchronize());
+    if (b->dev.num_nodes)
+        PQ_HIP_CHECK(hipMemcpy(out, b->dev.nodes, b->dev.num_nodes * sizeof(pq_bvh_node_wide), hipMemcpyDeviceToHost));
+    return PQ_OK;
+}
+
+int pq_bvh_update_and_refit(pq_bvh *b, const float *aabbs, void *stream) {
+    if (!b) return fail(PQ_ERR_INVALID, "pq_bvh_update_and_refit: null bvh");
+    if (b->dev.num_leaves && !aabbs) return fail(PQ_ERR_INVALID, "pq_bvh_update_and_refit: null aabbs");
+    PQ_HIP_CHECK(launch_bvh_refit(b->dev, aabbs, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_bvh_self_pairs(pq_bvh *b, uint32_t *pairs, uint64_t capacity, uint64_t *count, void *stream) {
+    if (!b || !count) return fail(PQ_ERR_INVALID, "pq_bvh_self_pairs: null argument");
+    if (capacity && !pairs) return fail(PQ_ERR_INVALID, "pq_bvh_self_pairs: null pairs buffer");
+    PQ_HIP_CHECK(launch_bvh_self_pairs(b->dev, pairs, capacity, (unsigned long long *)count, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+}  // extern "C"

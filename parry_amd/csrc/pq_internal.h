@@ -77,7 +77,60 @@ uint32_t ray_stack_entries(uint32_t depth);
 hipError_t launch_cast_rays(const DeviceMesh &m, const pq_ray *rays, uint64_t n, float max_toi, pq_ray_hit *hits,
                             uint32_t *counts, hipStream_t stream);
 
+// ---- shapes -------------------------------------------------------------
+// Device shape record, 32 B: type tag, vertex range (ConvexPolyhedron) and
+// up to four scalar parameters (see pq_shape_desc).
+struct alignas(16) ShapeRecord {
+    uint32_t type;
+    uint32_t first_vertex;
+    uint32_t num_vertices;
+    uint32_t pad;
+    float param[4];
+};
+static_assert(sizeof(ShapeRecord) == 32, "ShapeRecord is 32 bytes");
+
+struct DeviceShapes {
+    const ShapeRecord *rec;  // device
+    const float4 *verts;     // device, xyz + pad
+    uint32_t n;
+    uint32_t num_vertices;
+};
+
+hipError_t launch_shape_aabbs(const DeviceShapes &s, const float *poses, float *aabbs, hipStream_t stream);
+
+// ---- broadphase Bvh -----------------------------------------------------
+struct DeviceBvh {
+    pq_bvh_node_wide *nodes;  // device, depth-first
+    uint32_t *leaf_slot;      // device, per leaf: wide * 2 + (1 if right)
+    uint32_t *parent_slot;    // device, per wide node: parent wide * 2 + side (root: UINT32_MAX)
+    uint32_t *arrive;         // device, per wide node arrival counter (zero between launches)
+    uint32_t *pair_counts;    // device, n + 1
+    unsigned long long *pair_offsets;  // device, n + 1
+    void *scan_temp;
+    size_t scan_temp_bytes;
+    uint32_t num_leaves;
+    uint32_t num_nodes;
+    uint32_t depth;
+};
+
+hipError_t launch_bvh_refit(const DeviceBvh &b, const float *aabbs, hipStream_t stream);
+hipError_t launch_bvh_self_pairs(const DeviceBvh &b, uint32_t *pairs, uint64_t capacity, unsigned long long *count,
+                                 hipStream_t stream);
+hipError_t bvh_scan_temp_bytes(uint32_t num_leaves, size_t *bytes);
+
 }  // namespace pq
+
+struct pq_shapes {
+    pq::DeviceShapes dev;
+    std::vector<pq::ShapeRecord> host_rec;
+    void *d_rec = nullptr;
+    void *d_verts = nullptr;
+};
+
+struct pq_bvh {
+    pq::DeviceBvh dev;
+    std::vector<void *> allocations;
+};
 
 struct pq_trimesh {
     pq::DeviceMesh dev;

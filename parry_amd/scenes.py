@@ -132,3 +132,108 @@ def load_obj(path: str) -> tuple[np.ndarray, np.ndarray]:
             elif p[0] == "f":
                 fs.append([int(x.split("/")[0]) - 1 for x in p[1:4]])
     return np.asarray(vs, dtype=np.float32), np.asarray(fs, dtype=np.uint32)
+
+
+# ---------------------------------------------------------------------------
+# Convex shape populations (broadphase + contact workloads)
+# ---------------------------------------------------------------------------
+
+def random_rotations(n: int, rng: np.random.Generator) -> np.ndarray:
+    """Uniform random unit quaternions (Shoemake's method), float64 [n, 4] as (i, j, k, w)."""
+    u1, u2, u3 = rng.uniform(0.0, 1.0, size=(3, n))
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    q = np.stack(
+        [a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2), b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)],
+        axis=1,
+    )
+    return q
+
+
+def convex_point_cloud(rng: np.random.Generator, k: int) -> np.ndarray:
+    """k points on an ellipsoid (semi-axes U[0.5, 1.5]) jittered radially by U[-5%, +5%]."""
+    axes = rng.uniform(0.5, 1.5, size=3)
+    d = _unit_vectors(rng, k)
+    return d * axes * (1.0 + rng.uniform(-0.05, 0.05, size=(k, 1)))
+
+
+def convex_hull_vertices(pts: np.ndarray) -> np.ndarray:
+    """Vertices of the convex hull of ``pts`` (scipy's Qhull), in ascending input order."""
+    from scipy.spatial import ConvexHull
+
+    h = ConvexHull(pts)
+    return pts[np.sort(h.vertices)]
+
+
+def convex_polyhedra(n: int, seed: int, kmin: int = 8, kmax: int = 64):
+    """``n`` ConvexPolyhedron vertex sets: the hull of kmin..kmax (uniform) points
+    sampled as in ``convex_point_cloud``.  Returns (vertices float32 [V, 3], offsets uint32 [n+1])."""
+    rng = _rng(seed)
+    ks = rng.integers(kmin, kmax + 1, size=n)
+    verts, offs = [], [0]
+    for k in ks:
+        v = convex_hull_vertices(convex_point_cloud(rng, int(k)))
+        verts.append(v)
+        offs.append(offs[-1] + v.shape[0])
+    return np.ascontiguousarray(np.concatenate(verts, 0), dtype=np.float32), np.asarray(offs, dtype=np.uint32)
+
+
+def _rotated_half_extents(verts: np.ndarray, offs: np.ndarray, quats: np.ndarray) -> np.ndarray:
+    """Half extents of each shape's AABB under its rotation (float64, for calibration only)."""
+    n = offs.shape[0] - 1
+    out = np.zeros((n, 3))
+    for i in range(n):
+        p = verts[offs[i]:offs[i + 1]].astype(np.float64)
+        x, y, z, w = quats[i]
+        R = np.array(
+            [
+                [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+            ]
+        )
+        q = p @ R.T
+        out[i] = (q.max(0) - q.min(0)) / 2
+    return out
+
+
+def calibrate_cube_side(half_extents: np.ndarray, target_overlaps: float, rng: np.random.Generator,
+                        samples: int = 200000) -> float:
+    """Side L of the cube in which uniform centres give ``target_overlaps`` expected AABB
+    overlaps per shape: for two centres uniform on [0, L] and summed half-width s,
+    P(|x_i - x_j| <= s) = 2s/L - s^2/L^2; the expectation over shape pairs is
+    estimated from ``samples`` random pairs and L is found by bisection."""
+    n = half_extents.shape[0]
+    i = rng.integers(0, n, samples)
+    j = rng.integers(0, n, samples)
+    s = half_extents[i] + half_extents[j]
+
+    def overlaps(L):
+        p = np.clip(2 * s / L - (s / L) ** 2, 0.0, 1.0)
+        return (n - 1) * p.prod(axis=1).mean()
+
+    lo, hi = float(s.max()), 1e6
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        if overlaps(mid) > target_overlaps:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def polyhedra_scene(n: int, seed: int, target_overlaps: float = 8.0):
+    """The convex-polyhedra broadphase workload: ``n`` ConvexPolyhedra (8-64 hull
+    points on jittered ellipsoids), uniform random rotations and centres uniform
+    in a cube calibrated for ``target_overlaps`` mean AABB overlaps per shape.
+
+    Returns dict(vertices float32 [V,3], offsets uint32 [n+1], poses float32 [n,7]
+    as parry Isometry3 (i, j, k, w, tx, ty, tz), side float).
+    """
+    verts, offs = convex_polyhedra(n, seed)
+    rng = _rng(seed + 1)
+    q = random_rotations(n, rng)
+    he = _rotated_half_extents(verts, offs, q)
+    L = calibrate_cube_side(he, target_overlaps, rng)
+    t = rng.uniform(0.0, L, size=(n, 3))
+    poses = np.ascontiguousarray(np.concatenate([q, t], 1), dtype=np.float32)
+    return {"vertices": verts, "offsets": offs, "poses": poses, "side": L}

@@ -183,3 +183,73 @@ def split_hits(hits):
 
     feat = hits.view(torch.int32)[:, 4]
     return hits[:, 0], hits[:, 1:4], feat
+
+
+# ---------------------------------------------------------------------------
+# Shape sets (ConvexPolyhedron, Ball, ...) for the broadphase and contacts
+# ---------------------------------------------------------------------------
+
+SHAPE_BALL = 0
+SHAPE_CUBOID = 1
+SHAPE_CAPSULE = 2
+SHAPE_CONE = 3
+SHAPE_CONVEX_POLYHEDRON = 4
+
+SHAPE_DESC_DTYPE = np.dtype(
+    [("type", "<u4"), ("first_vertex", "<u4"), ("num_vertices", "<u4"), ("reserved", "<u4"), ("params", "<f4", 4)]
+)
+assert SHAPE_DESC_DTYPE.itemsize == 32
+
+
+class ShapeSet:
+    """A device-resident set of shapes (pq_shapes): type tags, parameters and the
+    ConvexPolyhedron vertex pool."""
+
+    def __init__(self, descs: np.ndarray, vertices: np.ndarray):
+        d = np.ascontiguousarray(descs, dtype=SHAPE_DESC_DTYPE)
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().pq_shapes_create(_ptr(d), d.shape[0], _ptr(v), v.shape[0], ctypes.byref(h)))
+        self._h = h
+        self.descs = d
+        self.vertices = v
+
+    @classmethod
+    def convex_polyhedra(cls, vertices: np.ndarray, offsets: np.ndarray) -> "ShapeSet":
+        """One ConvexPolyhedron per range offsets[i]..offsets[i+1] of ``vertices``."""
+        offs = np.asarray(offsets, dtype=np.uint32)
+        n = offs.shape[0] - 1
+        d = np.zeros(n, dtype=SHAPE_DESC_DTYPE)
+        d["type"] = SHAPE_CONVEX_POLYHEDRON
+        d["first_vertex"] = offs[:-1]
+        d["num_vertices"] = offs[1:] - offs[:-1]
+        return cls(d, vertices)
+
+    def __len__(self) -> int:
+        return self.descs.shape[0]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _native.lib().pq_shapes_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def compute_aabbs(self, poses, out=None, stream=None):
+        """Shape::compute_aabb for every shape: poses CUDA (n, 7) f32 -> CUDA (n, 6) f32."""
+        import torch
+
+        n = len(self)
+        if out is None:
+            out = torch.empty((n, 6), dtype=torch.float32, device=poses.device)
+        s = stream if stream is not None else torch.cuda.current_stream(poses.device)
+        _native.check(
+            _native.lib().pq_shapes_compute_aabbs(self._h, poses.data_ptr(), out.data_ptr(), ctypes.c_void_p(s.cuda_stream))
+        )
+        return out
