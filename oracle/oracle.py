@@ -229,3 +229,44 @@ def pair_set(pairs) -> set:
     p = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
     lo, hi = np.minimum(p[:, 0], p[:, 1]), np.maximum(p[:, 0], p[:, 1])
     return set(zip(lo.tolist(), hi.tolist()))
+
+
+# ---------------------------------------------------------------------------
+# Contacts
+# ---------------------------------------------------------------------------
+
+_SIG.update({
+    "orc_contact_pairs_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _int]),
+    "orc_contact_pairs_f64": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f64, _vp, _vp, _vp, _vp, _int]),
+})
+
+SHAPE_TRIANGLE_TEST_ONLY = 5
+
+
+def contact_pairs(descs, vertices, poses, pairs, prediction: float, init_dirs=None, f64: bool = False,
+                  threads: int | None = None):
+    """query::contact on every pair; returns dict(status int32 [m], out [m,13], iters uint32 [m,2]).
+
+    out = point1, point2, normal1, normal2 (world frame), dist.  status: 1
+    contact, 0 None, -1 pair type not handled by the restatement.
+    """
+    from parry_amd.shape import SHAPE_DESC_DTYPE
+
+    dt = np.float64 if f64 else np.float32
+    d = np.ascontiguousarray(descs, dtype=SHAPE_DESC_DTYPE)
+    v = np.ascontiguousarray(vertices, dtype=dt).reshape(-1, 3)
+    p = np.ascontiguousarray(poses, dtype=dt).reshape(-1, 7)
+    pr = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    m = pr.shape[0]
+    out = np.zeros((m, 13), dtype=dt)
+    status = np.zeros(m, np.int32)
+    iters = np.zeros((m, 2), np.uint32)
+    seeds = None
+    if init_dirs is not None:
+        seeds = np.ascontiguousarray(init_dirs, dtype=dt).reshape(-1, 3)
+        assert seeds.shape[0] == m
+    f = lib().orc_contact_pairs_f64 if f64 else lib().orc_contact_pairs_f32
+    f(d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data, pr.ctypes.data, m,
+      prediction, None if seeds is None else seeds.ctypes.data, out.ctypes.data, status.ctypes.data,
+      iters.ctypes.data, default_threads() if threads is None else threads)
+    return {"status": status, "out": out, "iters": iters}

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "parry_oracle.hpp"
+#include "parry_oracle_contact.hpp"
 
 using namespace oracle;
 
@@ -293,6 +294,86 @@ void orc_brute_pairs_f32(const float *aabbs, uint32_t n, uint32_t *out, uint64_t
         }
     }
     *count = c;
+}
+
+}  // extern "C"
+
+// ---- contacts ----------------------------------------------------------------
+
+namespace {
+
+struct ShapeDesc {  // same 32-byte layout as pq_shape_desc
+    uint32_t type, first_vertex, num_vertices, reserved;
+    float params[4];
+};
+
+template <class T>
+void contact_pairs(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uint32_t nverts, const T *poses,
+                   const uint32_t *pairs, uint64_t m, T prediction, const T *init_dirs, T *out, int32_t *status,
+                   uint32_t *iters, int nthreads) {
+    std::vector<V3<T>> pts(nverts);
+    for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
+    std::vector<Shape<T>> shapes(nshapes);
+    for (uint32_t i = 0; i < nshapes; ++i) {
+        shapes[i].type = descs[i].type;
+        shapes[i].pts = pts.data() + descs[i].first_vertex;
+        shapes[i].n = descs[i].num_vertices;
+        for (int k = 0; k < 4; ++k) shapes[i].p[k] = (T)descs[i].params[k];
+    }
+    parallel_for(m, nthreads, 64, [&](uint64_t b, uint64_t e) {
+        for (uint64_t k = b; k < e; ++k) {
+            uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+            const T *a = poses + 7 * (size_t)i, *c = poses + 7 * (size_t)j;
+            Iso<T> p1{Quat<T>{a[0], a[1], a[2], a[3]}, v3(a[4], a[5], a[6])};
+            Iso<T> p2{Quat<T>{c[0], c[1], c[2], c[3]}, v3(c[4], c[5], c[6])};
+            V3<T> seed;
+            const V3<T> *sp = nullptr;
+            if (init_dirs) {
+                seed = v3(init_dirs[3 * k], init_dirs[3 * k + 1], init_dirs[3 * k + 2]);
+                sp = &seed;
+            }
+            Contact<T> ct{};
+            ContactStats st;
+            int r = contact(p1, shapes[i], p2, shapes[j], prediction, ct, sp, &st);
+            status[k] = r;
+            T *o = out + 13 * k;
+            if (r == 1) {
+                o[0] = ct.point1.x; o[1] = ct.point1.y; o[2] = ct.point1.z;
+                o[3] = ct.point2.x; o[4] = ct.point2.y; o[5] = ct.point2.z;
+                o[6] = ct.normal1.x; o[7] = ct.normal1.y; o[8] = ct.normal1.z;
+                o[9] = ct.normal2.x; o[10] = ct.normal2.y; o[11] = ct.normal2.z;
+                o[12] = ct.dist;
+            } else {
+                for (int q = 0; q < 13; ++q) o[q] = T(0);
+            }
+            if (iters) {
+                iters[2 * k] = st.gjk_iters;
+                iters[2 * k + 1] = st.used_epa ? st.epa_iters + 1 : 0;
+            }
+        }
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+// query::contact for every (i, j) in `pairs` (m pairs).  out: 13 per pair
+// (point1, point2, normal1, normal2, dist, world frame); status: 1 contact,
+// 0 None, -1 shape pair not handled; iters (nullable): GJK iterations and
+// EPA iterations + 1 (0 when EPA did not run).  init_dirs (nullable): GJK seed
+// directions in shape 1's local frame, replacing parry's default seed.
+void orc_contact_pairs_f32(const void *descs, uint32_t nshapes, const float *verts, uint32_t nverts,
+                           const float *poses, const uint32_t *pairs, uint64_t m, float prediction,
+                           const float *init_dirs, float *out, int32_t *status, uint32_t *iters, int nthreads) {
+    contact_pairs<float>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, init_dirs, out,
+                         status, iters, nthreads);
+}
+void orc_contact_pairs_f64(const void *descs, uint32_t nshapes, const double *verts, uint32_t nverts,
+                           const double *poses, const uint32_t *pairs, uint64_t m, double prediction,
+                           const double *init_dirs, double *out, int32_t *status, uint32_t *iters, int nthreads) {
+    contact_pairs<double>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, init_dirs,
+                          out, status, iters, nthreads);
 }
 
 }  // extern "C"

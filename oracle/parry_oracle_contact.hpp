@@ -1,0 +1,910 @@
+// parry_oracle_contact.hpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of parry3d's narrowphase contact path for support-map shapes:
+// support maps, CSO points, the 3D Voronoi simplex, GJK, EPA and
+// contact_support_map_support_map, plus query::contact's frame handling.
+// Template over T (float = parry3d, double = parry3d-f64); compiled with
+// -ffp-contract=off.  Citations are dimforge/parry 0.25.3 path:line.
+//
+// Third-party behaviour restated from published sources (unpinned at the ulp):
+//   nalgebra Unit::try_new_and_get(v, m): |v|^2 > m^2 ? (v/|v|, |v|) : None
+//   nalgebra try_normalize(m): |v| <= m ? None : v/|v|
+//   nalgebra orthonormal_subspace_basis([v]) first vector: a x v with
+//     a = normalize(|v.x| > |v.y| ? (v.z, 0, -v.x) : (0, -v.z, v.y))
+//   approx relative_eq!(x, 0, epsilon = e)  <=>  |x| <= e (finite x)
+//   std BinaryHeap push/pop (sift_up / sift_down_to_bottom + sift_up)
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "parry_oracle.hpp"
+
+namespace oracle {
+
+template <class T> struct Eps;
+template <> struct Eps<float> {
+    static constexpr float eps = 1.1920929e-07f;  // f32::EPSILON (src/lib.rs:102)
+};
+template <> struct Eps<double> {
+    static constexpr double eps = 2.220446049250313e-16;
+};
+
+// TRIANGLE (pts[0..3]) is test-only: it lets the reference's triangle EPA
+// known-answer test (crates/parry3d/tests/geometry/epa3.rs:25-59) run here.
+enum ShapeType : uint32_t { BALL = 0, CUBOID = 1, CAPSULE = 2, CONE = 3, CONVEX_POLYHEDRON = 4, TRIANGLE = 5 };
+
+template <class T> struct Shape {
+    uint32_t type;
+    const V3<T> *pts;  // ConvexPolyhedron vertices
+    uint32_t n;
+    T p[4];  // Ball: r; Cuboid: half extents; Capsule: half_height, radius; Cone: half_height, radius
+};
+
+template <class T> inline T copy_sign(T from, T to) { return std::copysign(to, from); }
+
+// SupportMap::local_support_point for the shapes on the path.
+template <class T> V3<T> local_support_point(const Shape<T> &s, V3<T> dir) {
+    switch (s.type) {
+        case CONVEX_POLYHEDRON: {  // point_cloud_support_point.rs:5-19: first strict maximum
+            uint32_t best = 0;
+            T best_dot = dot(s.pts[0], dir);
+            for (uint32_t i = 1; i < s.n; ++i) {
+                T d = dot(s.pts[i], dir);
+                if (d > best_dot) {
+                    best_dot = d;
+                    best = i;
+                }
+            }
+            return s.pts[best];
+        }
+        case TRIANGLE: {  // triangle.rs:697-715
+            T d1 = dot(s.pts[0], dir), d2 = dot(s.pts[1], dir), d3 = dot(s.pts[2], dir);
+            if (d1 > d2) return d1 > d3 ? s.pts[0] : s.pts[2];
+            return d2 > d3 ? s.pts[1] : s.pts[2];
+        }
+        case BALL: {  // ball.rs:242-249: Unit::new_normalize(dir) * radius
+            return normalize(dir) * s.p[0];
+        }
+        case CUBOID: {  // cuboid.rs:454-456, wops.rs:30-35: sign bit of dir copied onto half extents
+            return v3(copy_sign(dir.x, s.p[0]), copy_sign(dir.y, s.p[1]), copy_sign(dir.z, s.p[2]));
+        }
+        case CAPSULE: {  // capsule.rs:407-418, segment a = (0,-h,0), b = (0,h,0)
+            V3<T> d;
+            T sq = dot(dir, dir);
+            if (sq > T(0)) d = dir / std::sqrt(sq);
+            else d = v3(T(0), T(1), T(0));
+            V3<T> a = v3(T(0), -s.p[0], T(0)), b = v3(T(0), s.p[0], T(0));
+            if (dot(d, a) > dot(d, b)) return a + d * s.p[1];
+            return b + d * s.p[1];
+        }
+        case CONE: {  // cone.rs:162-182
+            V3<T> v = dir;
+            v.y = T(0);
+            T nrm = norm(v);
+            v = v / nrm;
+            if (nrm == T(0)) {
+                v = v3(T(0), copy_sign(dir.y, s.p[0]), T(0));
+            } else {
+                v = v * s.p[1];
+                v.y = -s.p[0];
+                if (dot(dir, v) < dir.y * s.p[0]) v = v3(T(0), s.p[0], T(0));
+            }
+            return v;
+        }
+    }
+    return v3(T(0), T(0), T(0));
+}
+
+// SupportMap::support_point (support_map.rs:380-383).
+template <class T> V3<T> support_point(const Iso<T> &m, const Shape<T> &s, V3<T> dir) {
+    V3<T> local_dir = inv_rotate(m.q, dir);
+    return transform_point(m, local_support_point(s, local_dir));
+}
+
+template <class T> struct CSO {  // cso_point.rs:12-89
+    V3<T> point, orig1, orig2;
+};
+template <class T> CSO<T> cso_from_shapes(const Iso<T> &pos12, const Shape<T> &g1, const Shape<T> &g2, V3<T> dir) {
+    V3<T> sp1 = local_support_point(g1, dir);
+    V3<T> sp2 = support_point(pos12, g2, -dir);
+    return CSO<T>{sp1 - sp2, sp1, sp2};
+}
+
+// relative_eq!(p, origin) with approx's defaults (epsilon = max_relative = EPS).
+template <class T> inline bool rel_eq_origin(V3<T> p) {
+    const T e = Eps<T>::eps;
+    return std::fabs(p.x) <= e && std::fabs(p.y) <= e && std::fabs(p.z) <= e;
+}
+
+// ---- point projections with location ---------------------------------------
+
+enum LocKind { ON_VERTEX, ON_EDGE, ON_FACE, ON_SOLID };
+template <class T> struct Loc {
+    LocKind kind;
+    uint32_t id;  // vertex / edge / face index
+    T bc[3];      // barycentric coordinates (edge: 2, face: 3)
+};
+
+// Segment (point_segment.rs:45-85), projecting pt.
+template <class T> V3<T> project_segment(V3<T> a, V3<T> b, V3<T> pt, Loc<T> &loc) {
+    V3<T> ab = b - a;
+    V3<T> ap = pt - a;
+    T ab_ap = dot(ab, ap);
+    T sqnab = dot(ab, ab);
+    if (ab_ap <= T(0)) {
+        loc = Loc<T>{ON_VERTEX, 0, {T(1), T(0), T(0)}};
+        return a;
+    } else if (ab_ap >= sqnab) {
+        loc = Loc<T>{ON_VERTEX, 1, {T(1), T(0), T(0)}};
+        return b;
+    }
+    T u = ab_ap / sqnab;
+    loc = Loc<T>{ON_EDGE, 0, {T(1) - u, u, T(0)}};
+    return a + ab * u;
+}
+
+// Triangle (point_triangle.rs:54-291), solid = true.  *inside = relative_eq!(proj, pt).
+template <class T> V3<T> project_triangle(V3<T> a, V3<T> b, V3<T> c, V3<T> pt, Loc<T> &loc, bool *inside) {
+    V3<T> ab = b - a, ac = c - a, ap = pt - a;
+    T ab_ap = dot(ab, ap), ac_ap = dot(ac, ap);
+    auto res = [&](V3<T> proj) {
+        *inside = rel_eq_origin(proj - pt);
+        return proj;
+    };
+    if (ab_ap <= T(0) && ac_ap <= T(0)) {
+        loc = Loc<T>{ON_VERTEX, 0, {}};
+        return res(a);
+    }
+    V3<T> bp = pt - b;
+    T ab_bp = dot(ab, bp), ac_bp = dot(ac, bp);
+    if (ab_bp >= T(0) && ac_bp <= ab_bp) {
+        loc = Loc<T>{ON_VERTEX, 1, {}};
+        return res(b);
+    }
+    V3<T> cp = pt - c;
+    T ab_cp = dot(ab, cp), ac_cp = dot(ac, cp);
+    if (ac_cp >= T(0) && ab_cp <= ac_cp) {
+        loc = Loc<T>{ON_VERTEX, 2, {}};
+        return res(c);
+    }
+    V3<T> bc = c - b;
+    // stable_check_edges_voronoi (dim3)
+    V3<T> n = cross(ab, ac);
+    T vc = dot(n, cross(ab, ap));
+    if (vc < T(0) && ab_ap >= T(0) && ab_bp <= T(0)) {
+        T v = ab_ap / dot(ab, ab);
+        loc = Loc<T>{ON_EDGE, 0, {T(1) - v, v, T(0)}};
+        return res(a + ab * v);
+    }
+    T vb = -dot(n, cross(ac, cp));
+    if (vb < T(0) && ac_ap >= T(0) && ac_cp <= T(0)) {
+        T w = ac_ap / dot(ac, ac);
+        loc = Loc<T>{ON_EDGE, 2, {T(1) - w, w, T(0)}};
+        return res(a + ac * w);
+    }
+    T va = dot(n, cross(bc, bp));
+    if (va < T(0) && ac_bp - ab_bp >= T(0) && ab_cp - ac_cp >= T(0)) {
+        T w = dot(bc, bp) / dot(bc, bc);
+        loc = Loc<T>{ON_EDGE, 1, {T(1) - w, w, T(0)}};
+        return res(b + bc * w);
+    }
+    uint32_t clockwise = dot(n, ap) >= T(0) ? 0 : 1;
+    if (va + vb + vc != T(0)) {
+        T denom = T(1) / (va + vb + vc);
+        T v = vb * denom;
+        T w = vc * denom;
+        loc = Loc<T>{ON_FACE, clockwise, {T(1) - v - w, v, w}};
+        return res((a + ab * v) + ac * w);
+    }
+    loc = Loc<T>{ON_SOLID, 0, {}};
+    *inside = true;
+    return pt;
+}
+
+// TrianglePointLocation::barycentric_coordinates (triangle.rs:121-148).
+template <class T> bool tri_bcoords(const Loc<T> &l, T out[3]) {
+    out[0] = out[1] = out[2] = T(0);
+    switch (l.kind) {
+        case ON_VERTEX: out[l.id] = T(1); return true;
+        case ON_EDGE: {
+            int i0 = l.id == 1 ? 1 : 0, i1 = l.id == 0 ? 1 : 2;
+            out[i0] = l.bc[0];
+            out[i1] = l.bc[1];
+            return true;
+        }
+        case ON_FACE: out[0] = l.bc[0]; out[1] = l.bc[1]; out[2] = l.bc[2]; return true;
+        default: return false;
+    }
+}
+
+// Tetrahedron (point_tetrahedron.rs:28-340), solid = true.
+template <class T> V3<T> project_tetrahedron(V3<T> a, V3<T> b, V3<T> c, V3<T> d, V3<T> pt, Loc<T> &loc) {
+    V3<T> ab = b - a, ac = c - a, ad = d - a, ap = pt - a;
+    T ap_ab = dot(ap, ab), ap_ac = dot(ap, ac), ap_ad = dot(ap, ad);
+    if (ap_ab <= T(0) && ap_ac <= T(0) && ap_ad <= T(0)) { loc = Loc<T>{ON_VERTEX, 0, {}}; return a; }
+    V3<T> bc = c - b, bd = d - b, bp = pt - b;
+    T bp_bc = dot(bp, bc), bp_bd = dot(bp, bd), bp_ab = dot(bp, ab);
+    if (bp_bc <= T(0) && bp_bd <= T(0) && bp_ab >= T(0)) { loc = Loc<T>{ON_VERTEX, 1, {}}; return b; }
+    V3<T> cd = d - c, cp = pt - c;
+    T cp_ac = dot(cp, ac), cp_bc = dot(cp, bc), cp_cd = dot(cp, cd);
+    if (cp_cd <= T(0) && cp_bc >= T(0) && cp_ac >= T(0)) { loc = Loc<T>{ON_VERTEX, 2, {}}; return c; }
+    V3<T> dp = pt - d;
+    T dp_cd = dot(dp, cd), dp_bd = dot(dp, bd), dp_ad = dot(dp, ad);
+    if (dp_ad >= T(0) && dp_bd >= T(0) && dp_cd >= T(0)) { loc = Loc<T>{ON_VERTEX, 3, {}}; return d; }
+
+    // check_edge: returns true (and the projection) if pt is in edge i's region.
+    auto check_edge = [&](uint32_t i, V3<T> ea, V3<T> nabc, V3<T> nabd, V3<T> eap, V3<T> eab, T eap_ab, T ebp_ab,
+                          T &dabc, T &dabd, V3<T> &out) -> bool {
+        T ab_ab = eap_ab - ebp_ab;
+        V3<T> ap_x_ab = cross(eap, eab);
+        dabc = dot(ap_x_ab, nabc);
+        dabd = dot(ap_x_ab, nabd);
+        if (ab_ab != T(0) && dabc >= T(0) && dabd >= T(0) && eap_ab >= T(0) && eap_ab <= ab_ab) {
+            T u = eap_ab / ab_ab;
+            loc = Loc<T>{ON_EDGE, i, {T(1) - u, u, T(0)}};
+            out = ea + eab * u;
+            return true;
+        }
+        return false;
+    };
+    V3<T> out;
+    V3<T> nabc = cross(ab, ac), nabd = cross(ab, ad);
+    T dabc, dabd, dacd, dacb, dadb, dadc, dbca, dbcd, dbdc, dbda, dcda, dcdb;
+    if (check_edge(0, a, nabc, nabd, ap, ab, ap_ab, bp_ab, dabc, dabd, out)) return out;
+    V3<T> nacd = cross(ac, ad);
+    if (check_edge(1, a, nacd, -nabc, ap, ac, ap_ac, cp_ac, dacd, dacb, out)) return out;
+    if (check_edge(2, a, -nabd, -nacd, ap, ad, ap_ad, dp_ad, dadb, dadc, out)) return out;
+    V3<T> nbcd = cross(bc, bd);
+    if (check_edge(3, b, nabc, nbcd, bp, bc, bp_bc, cp_bc, dbca, dbcd, out)) return out;
+    if (check_edge(4, b, -nbcd, nabd, bp, bd, bp_bd, dp_bd, dbdc, dbda, out)) return out;
+    if (check_edge(5, c, nacd, nbcd, cp, cd, cp_cd, dp_cd, dcda, dcdb, out)) return out;
+
+    auto check_face = [&](uint32_t i, V3<T> fa, V3<T> fb, V3<T> fc, V3<T> fap, V3<T> fbp, V3<T> fcp, V3<T> fab,
+                          V3<T> fac, V3<T> fad, T d1, T d2, T d3, V3<T> &res) -> bool {
+        if (d1 < T(0) && d2 < T(0) && d3 < T(0)) {
+            V3<T> n = cross(fab, fac);
+            if (dot(n, fad) * dot(n, fap) < T(0)) {
+                T nn = norm(n);
+                if (nn <= Eps<T>::eps) return false;  // try_normalize(DEFAULT_EPSILON) failed
+                V3<T> normal = n / nn;
+                T vc = dot(normal, cross(fap, fbp));
+                T va = dot(normal, cross(fbp, fcp));
+                T vb = dot(normal, cross(fcp, fap));
+                T denom = va + vb + vc;
+                T inv = T(1) / denom;
+                T bc0 = va * inv, bc1 = vb * inv, bc2 = vc * inv;
+                loc = Loc<T>{ON_FACE, i, {bc0, bc1, bc2}};
+                res = (fa * bc0 + fb * bc1) + fc * bc2;
+                return true;
+            }
+        }
+        return false;
+    };
+    if (check_face(0, a, b, c, ap, bp, cp, ab, ac, ad, dabc, dbca, dacb, out)) return out;
+    if (check_face(1, a, b, d, ap, bp, dp, ab, ad, ac, dadb, dabd, dbda, out)) return out;
+    if (check_face(2, a, c, d, ap, cp, dp, ac, ad, ab, dacd, dcda, dadc, out)) return out;
+    if (check_face(3, b, c, d, bp, cp, dp, bc, bd, -ab, dbcd, dcdb, dbdc, out)) return out;
+    loc = Loc<T>{ON_SOLID, 0, {}};
+    return pt;
+}
+
+// ---- VoronoiSimplex (voronoi_simplex3.rs:13-351) ----------------------------
+
+template <class T> struct Simplex {
+    size_t prev_vertices[4] = {0, 1, 2, 3};
+    T prev_proj[3] = {0, 0, 0};
+    size_t prev_dim = 0;
+    CSO<T> vertices[4] = {};
+    T proj[3] = {0, 0, 0};
+    size_t dim = 0;
+
+    void swap(size_t i1, size_t i2) {
+        std::swap(vertices[i1], vertices[i2]);
+        std::swap(prev_vertices[i1], prev_vertices[i2]);
+    }
+    void reset(const CSO<T> &pt) {
+        dim = 0;
+        prev_dim = 0;
+        vertices[0] = pt;
+    }
+    bool add_point(const CSO<T> &pt) {
+        const T eps_tol = Eps<T>::eps * T(10);
+        prev_dim = dim;
+        for (int i = 0; i < 3; ++i) prev_proj[i] = proj[i];
+        for (size_t i = 0; i < 4; ++i) prev_vertices[i] = i;
+        if (dim == 0) {
+            V3<T> d = vertices[0].point - pt.point;
+            if (dot(d, d) < eps_tol) return false;
+        } else if (dim == 1) {
+            V3<T> ab = vertices[1].point - vertices[0].point;
+            V3<T> ac = pt.point - vertices[0].point;
+            V3<T> c = cross(ab, ac);
+            if (dot(c, c) < eps_tol) return false;
+        } else {  // dim == 2
+            V3<T> ab = vertices[1].point - vertices[0].point;
+            V3<T> ac = vertices[2].point - vertices[0].point;
+            V3<T> ap = pt.point - vertices[0].point;
+            V3<T> n = normalize(cross(ab, ac));
+            if (std::fabs(dot(n, ap)) < eps_tol) return false;
+        }
+        dim += 1;
+        vertices[dim] = pt;
+        return true;
+    }
+
+    V3<T> project_origin_and_reduce() {
+        const V3<T> o = v3(T(0), T(0), T(0));
+        if (dim == 0) {
+            proj[0] = T(1);
+            return vertices[0].point;
+        } else if (dim == 1) {
+            Loc<T> l;
+            V3<T> p = project_segment(vertices[0].point, vertices[1].point, o, l);
+            if (l.kind == ON_VERTEX) {
+                if (l.id == 1) swap(0, 1);
+                proj[0] = T(1);
+                dim = 0;
+            } else {
+                proj[0] = l.bc[0];
+                proj[1] = l.bc[1];
+            }
+            return p;
+        } else if (dim == 2) {
+            Loc<T> l;
+            bool inside;
+            V3<T> p = project_triangle(vertices[0].point, vertices[1].point, vertices[2].point, o, l, &inside);
+            if (l.kind == ON_VERTEX) {
+                swap(0, l.id);
+                proj[0] = T(1);
+                dim = 0;
+            } else if (l.kind == ON_EDGE) {
+                if (l.id == 0) {
+                    proj[0] = l.bc[0];
+                    proj[1] = l.bc[1];
+                } else if (l.id == 1) {
+                    swap(0, 2);
+                    proj[0] = l.bc[1];
+                    proj[1] = l.bc[0];
+                } else {
+                    swap(1, 2);
+                    proj[0] = l.bc[0];
+                    proj[1] = l.bc[1];
+                }
+                dim = 1;
+            } else if (l.kind == ON_FACE) {
+                proj[0] = l.bc[0];
+                proj[1] = l.bc[1];
+                proj[2] = l.bc[2];
+            }
+            return p;
+        } else {
+            Loc<T> l;
+            V3<T> p = project_tetrahedron(vertices[0].point, vertices[1].point, vertices[2].point, vertices[3].point,
+                                          o, l);
+            if (l.kind == ON_VERTEX) {
+                swap(0, l.id);
+                proj[0] = T(1);
+                dim = 0;
+            } else if (l.kind == ON_EDGE) {
+                switch (l.id) {
+                    case 0: break;
+                    case 1: swap(1, 2); break;
+                    case 2: swap(1, 3); break;
+                    case 3: swap(0, 2); break;
+                    case 4: swap(0, 3); break;
+                    case 5: swap(0, 2); swap(1, 3); break;
+                }
+                if (l.id == 3 || l.id == 4) {
+                    proj[0] = l.bc[1];
+                    proj[1] = l.bc[0];
+                } else {
+                    proj[0] = l.bc[0];
+                    proj[1] = l.bc[1];
+                }
+                dim = 1;
+            } else if (l.kind == ON_FACE) {
+                switch (l.id) {
+                    case 0:
+                        proj[0] = l.bc[0]; proj[1] = l.bc[1]; proj[2] = l.bc[2];
+                        break;
+                    case 1:
+                        vertices[2] = vertices[3];
+                        proj[0] = l.bc[0]; proj[1] = l.bc[1]; proj[2] = l.bc[2];
+                        break;
+                    case 2:
+                        vertices[1] = vertices[3];
+                        proj[0] = l.bc[0]; proj[1] = l.bc[2]; proj[2] = l.bc[1];
+                        break;
+                    case 3:
+                        vertices[0] = vertices[3];
+                        proj[0] = l.bc[2]; proj[1] = l.bc[0]; proj[2] = l.bc[1];
+                        break;
+                }
+                dim = 2;
+            }
+            return p;
+        }
+    }
+};
+
+// ---- GJK (gjk.rs:353-453, 797-818) -----------------------------------------
+
+enum GjkKind { GJK_INTERSECTION, GJK_CLOSEST_POINTS, GJK_NO_INTERSECTION };
+template <class T> struct GjkResult {
+    GjkKind kind;
+    V3<T> p1, p2, n;
+};
+
+template <class T> void gjk_result(const Simplex<T> &s, bool prev, V3<T> &p1, V3<T> &p2) {
+    p1 = p2 = v3(T(0), T(0), T(0));
+    if (prev) {
+        for (size_t i = 0; i < s.prev_dim + 1; ++i) {
+            T c = s.prev_proj[i];
+            const CSO<T> &pt = s.vertices[s.prev_vertices[i]];
+            p1 = p1 + pt.orig1 * c;
+            p2 = p2 + pt.orig2 * c;
+        }
+    } else {
+        for (size_t i = 0; i < s.dim + 1; ++i) {
+            T c = s.proj[i];
+            const CSO<T> &pt = s.vertices[i];
+            p1 = p1 + pt.orig1 * c;
+            p2 = p2 + pt.orig2 * c;
+        }
+    }
+}
+
+template <class T>
+GjkResult<T> gjk_closest_points(const Iso<T> &pos12, const Shape<T> &g1, const Shape<T> &g2, T max_dist,
+                                Simplex<T> &simplex, uint32_t *iters) {
+    const T eps_tol = Eps<T>::eps * T(10);
+    const T eps_rel = std::sqrt(eps_tol);
+    V3<T> proj = simplex.project_origin_and_reduce();
+    V3<T> old_dir;
+    {
+        T sq = dot(proj, proj);
+        if (!(sq > T(0))) return GjkResult<T>{GJK_INTERSECTION, {}, {}, {}};
+        old_dir = -(proj / std::sqrt(sq));
+    }
+    T max_bound = Consts<T>::max_value;
+    V3<T> dir;
+    uint32_t niter = 0;
+    if (iters) *iters = 0;
+    for (;;) {
+        T old_max_bound = max_bound;
+        V3<T> mp = -proj;
+        T sq = dot(mp, mp);
+        if (sq > eps_tol * eps_tol) {
+            T nrm = std::sqrt(sq);
+            dir = mp / nrm;
+            max_bound = nrm;
+        } else {
+            return GjkResult<T>{GJK_INTERSECTION, {}, {}, {}};
+        }
+        if (max_bound >= old_max_bound) {
+            GjkResult<T> r{GJK_CLOSEST_POINTS, {}, {}, old_dir};
+            gjk_result(simplex, true, r.p1, r.p2);
+            return r;
+        }
+        CSO<T> cso = cso_from_shapes(pos12, g1, g2, dir);
+        T min_bound = -dot(dir, cso.point);
+        if (!std::isfinite(min_bound)) return GjkResult<T>{GJK_NO_INTERSECTION, {}, {}, v3(T(1), T(0), T(0))};
+        if (min_bound > max_dist) {
+            return GjkResult<T>{GJK_NO_INTERSECTION, {}, {}, dir};
+        } else if (max_bound - min_bound <= eps_rel * max_bound) {
+            GjkResult<T> r{GJK_CLOSEST_POINTS, {}, {}, dir};
+            gjk_result(simplex, false, r.p1, r.p2);
+            return r;
+        }
+        if (!simplex.add_point(cso)) {
+            GjkResult<T> r{GJK_CLOSEST_POINTS, {}, {}, dir};
+            gjk_result(simplex, false, r.p1, r.p2);
+            return r;
+        }
+        old_dir = dir;
+        proj = simplex.project_origin_and_reduce();
+        if (simplex.dim == 3) {
+            if (min_bound >= eps_tol) {
+                GjkResult<T> r{GJK_CLOSEST_POINTS, {}, {}, old_dir};
+                gjk_result(simplex, true, r.p1, r.p2);
+                return r;
+            }
+            return GjkResult<T>{GJK_INTERSECTION, {}, {}, {}};
+        }
+        niter += 1;
+        if (iters) *iters = niter;
+        if (niter == 100) return GjkResult<T>{GJK_NO_INTERSECTION, {}, {}, v3(T(1), T(0), T(0))};
+    }
+}
+
+// ---- EPA (epa3.rs:17-675) ----------------------------------------------------
+
+template <class T> struct EpaFaceId {
+    size_t id;
+    T neg_dist;
+};
+// Ord of FaceId (epa3.rs:40-53): by neg_dist; NaN compares Equal.
+template <class T> inline int face_cmp(const EpaFaceId<T> &a, const EpaFaceId<T> &b) {
+    if (a.neg_dist < b.neg_dist) return -1;
+    if (a.neg_dist > b.neg_dist) return 1;
+    return 0;
+}
+
+// std::collections::BinaryHeap (max-heap) with the standard library's sift order.
+template <class T> struct Heap {
+    std::vector<EpaFaceId<T>> d;
+    size_t sift_up(size_t start, size_t pos) {
+        EpaFaceId<T> e = d[pos];
+        while (pos > start) {
+            size_t parent = (pos - 1) / 2;
+            if (face_cmp(e, d[parent]) <= 0) break;
+            d[pos] = d[parent];
+            pos = parent;
+        }
+        d[pos] = e;
+        return pos;
+    }
+    void push(const EpaFaceId<T> &x) {
+        d.push_back(x);
+        sift_up(0, d.size() - 1);
+    }
+    void sift_down_to_bottom(size_t pos) {
+        size_t end = d.size();
+        size_t start = pos;
+        EpaFaceId<T> e = d[pos];
+        size_t child = 2 * pos + 1;
+        while (child + 2 <= end) {  // child <= end - 2
+            if (face_cmp(d[child], d[child + 1]) <= 0) child += 1;
+            d[pos] = d[child];
+            pos = child;
+            child = 2 * pos + 1;
+        }
+        if (child == end - 1) {
+            d[pos] = d[child];
+            pos = child;
+        }
+        d[pos] = e;
+        sift_up(start, pos);
+    }
+    bool pop(EpaFaceId<T> &out) {
+        if (d.empty()) return false;
+        EpaFaceId<T> item = d.back();
+        d.pop_back();
+        if (!d.empty()) {
+            std::swap(item, d[0]);
+            sift_down_to_bottom(0);
+        }
+        out = item;
+        return true;
+    }
+};
+
+template <class T> struct EpaFace {
+    size_t pts[3];
+    size_t adj[3];
+    V3<T> normal;
+    T bcoords[3];
+    bool deleted;
+};
+
+template <class T> struct Epa {
+    std::vector<CSO<T>> vertices;
+    std::vector<EpaFace<T>> faces;
+    std::vector<std::pair<size_t, size_t>> silhouette;  // (face_id, opp_pt_id)
+    Heap<T> heap;
+
+    static bool ccw_face_normal(V3<T> a, V3<T> b, V3<T> c, V3<T> &n) {  // ccw_face_normal.rs:19-27
+        V3<T> r = cross(b - a, c - a);
+        T sq = dot(r, r);
+        const T e = Eps<T>::eps;
+        if (sq > e * e) {
+            n = r / std::sqrt(sq);
+            return true;
+        }
+        return false;
+    }
+    EpaFace<T> face_with_proj(const T bc[3], const size_t pts[3], const size_t adj[3]) {
+        EpaFace<T> f;
+        for (int i = 0; i < 3; ++i) {
+            f.pts[i] = pts[i];
+            f.adj[i] = adj[i];
+            f.bcoords[i] = bc[i];
+        }
+        if (!ccw_face_normal(vertices[pts[0]].point, vertices[pts[1]].point, vertices[pts[2]].point, f.normal))
+            f.normal = v3(T(0), T(0), T(0));
+        f.deleted = false;
+        return f;
+    }
+    // Face::new (epa3.rs:96-118): (face, proj_inside).
+    std::pair<EpaFace<T>, bool> face_new(const size_t pts[3], const size_t adj[3]) {
+        Loc<T> l;
+        bool inside;
+        V3<T> p = project_triangle(vertices[pts[0]].point, vertices[pts[1]].point, vertices[pts[2]].point,
+                                   v3(T(0), T(0), T(0)), l, &inside);
+        if (l.kind == ON_VERTEX || l.kind == ON_EDGE) {
+            T bc[3];
+            tri_bcoords(l, bc);
+            const T eps_tol = Eps<T>::eps * T(100);
+            bool in = inside || dot(p, p) < eps_tol * eps_tol;
+            return {face_with_proj(bc, pts, adj), in};
+        } else if (l.kind == ON_FACE) {
+            return {face_with_proj(l.bc, pts, adj), true};
+        }
+        const T zero[3] = {T(0), T(0), T(0)};
+        return {face_with_proj(zero, pts, adj), false};
+    }
+    void closest_points_of(const EpaFace<T> &f, V3<T> &p1, V3<T> &p2) const {
+        const CSO<T> &a = vertices[f.pts[0]], &b = vertices[f.pts[1]], &c = vertices[f.pts[2]];
+        p1 = (a.orig1 * f.bcoords[0] + b.orig1 * f.bcoords[1]) + c.orig1 * f.bcoords[2];
+        p2 = (a.orig2 * f.bcoords[0] + b.orig2 * f.bcoords[1]) + c.orig2 * f.bcoords[2];
+    }
+    static size_t next_ccw_pt_id(const EpaFace<T> &f, size_t id) {
+        if (f.pts[0] == id) return 1;
+        if (f.pts[1] == id) return 2;
+        return 0;
+    }
+    bool can_be_seen_by(const EpaFace<T> &f, size_t point, size_t opp) const {
+        const T eps_tol = Eps<T>::eps * T(10);
+        V3<T> p0 = vertices[f.pts[opp]].point;
+        V3<T> p1 = vertices[f.pts[(opp + 1) % 3]].point;
+        V3<T> p2 = vertices[f.pts[(opp + 2) % 3]].point;
+        V3<T> pt = vertices[point].point;
+        if (dot(pt - p0, f.normal) >= -eps_tol) return true;
+        // Triangle::is_affinely_dependent (triangle.rs:534-546)
+        const T E = Eps<T>::eps * T(100);
+        V3<T> c = cross(p2 - p1, pt - p1);
+        return dot(c, c) <= E * E;
+    }
+    void compute_silhouette(size_t point, size_t id, size_t opp) {
+        if (faces[id].deleted) return;
+        if (!can_be_seen_by(faces[id], point, opp)) {
+            silhouette.push_back({id, opp});
+        } else {
+            faces[id].deleted = true;
+            size_t a1 = (opp + 2) % 3, a2 = opp;
+            size_t adj1 = faces[id].adj[a1], adj2 = faces[id].adj[a2];
+            size_t o1 = next_ccw_pt_id(faces[adj1], faces[id].pts[a1]);
+            size_t o2 = next_ccw_pt_id(faces[adj2], faces[id].pts[a2]);
+            compute_silhouette(point, adj1, o1);
+            compute_silhouette(point, adj2, o2);
+        }
+    }
+
+    // EPA::closest_points (epa3.rs:428-651).
+    bool closest_points(const Iso<T> &pos12, const Shape<T> &g1, const Shape<T> &g2, const Simplex<T> &simplex,
+                        V3<T> &p1, V3<T> &p2, V3<T> &n, uint32_t *iters) {
+        const T eps = Eps<T>::eps;
+        const T eps_tol = eps * T(100);
+        vertices.clear();
+        faces.clear();
+        heap.d.clear();
+        silhouette.clear();
+        if (iters) *iters = 0;
+        for (size_t i = 0; i < simplex.dim + 1; ++i) vertices.push_back(simplex.vertices[i]);
+        auto fid_new = [&](size_t id, T neg_dist, EpaFaceId<T> &out) -> bool {
+            if (neg_dist > Eps<T>::eps * T(10)) return false;
+            out = EpaFaceId<T>{id, neg_dist};
+            return true;
+        };
+        if (simplex.dim == 0) {
+            p1 = p2 = v3(T(0), T(0), T(0));
+            n = v3(T(0), T(1), T(0));
+            return true;
+        } else if (simplex.dim == 3) {
+            V3<T> dp1 = vertices[1].point - vertices[0].point;
+            V3<T> dp2 = vertices[2].point - vertices[0].point;
+            V3<T> dp3 = vertices[3].point - vertices[0].point;
+            if (dot(cross(dp1, dp2), dp3) > T(0)) std::swap(vertices[1], vertices[2]);
+            const size_t P[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
+            const size_t A[4][3] = {{3, 1, 2}, {3, 2, 0}, {0, 1, 3}, {2, 1, 0}};
+            bool inside[4];
+            for (int k = 0; k < 4; ++k) {
+                auto fr = face_new(P[k], A[k]);
+                faces.push_back(fr.first);
+                inside[k] = fr.second;
+            }
+            for (int k = 0; k < 4; ++k) {
+                if (inside[k]) {
+                    T dist = dot(faces[k].normal, vertices[k].point);
+                    EpaFaceId<T> f;
+                    if (!fid_new((size_t)k, -dist, f)) return false;
+                    heap.push(f);
+                }
+            }
+            if (!(inside[0] || inside[1] || inside[2] || inside[3])) return false;
+        } else {
+            if (simplex.dim == 1) {
+                V3<T> dpt = vertices[1].point - vertices[0].point;
+                V3<T> a = std::fabs(dpt.x) > std::fabs(dpt.y) ? v3(dpt.z, T(0), -dpt.x) : v3(T(0), -dpt.z, dpt.y);
+                a = a / norm(a);
+                V3<T> dir = cross(a, dpt);
+                vertices.push_back(cso_from_shapes(pos12, g1, g2, dir));
+            }
+            const size_t p1i[3] = {0, 1, 2}, p2i[3] = {0, 2, 1};
+            const size_t a1[3] = {1, 1, 1}, a2[3] = {0, 0, 0};
+            faces.push_back(face_new(p1i, a1).first);
+            faces.push_back(face_new(p2i, a2).first);
+            EpaFaceId<T> f;
+            if (!fid_new(0, T(0), f)) return false;
+            heap.push(f);
+            if (!fid_new(1, T(0), f)) return false;
+            heap.push(f);
+        }
+        if (heap.d.empty()) return false;
+        uint32_t niter = 0;
+        T max_dist = Consts<T>::max_value;
+        EpaFaceId<T> best_face_id = heap.d[0];
+        T old_dist = T(0);
+        EpaFaceId<T> face_id;
+        while (heap.pop(face_id)) {
+            EpaFace<T> face = faces[face_id.id];
+            if (face.deleted) continue;
+            CSO<T> cso = cso_from_shapes(pos12, g1, g2, face.normal);
+            size_t support_point_id = vertices.size();
+            vertices.push_back(cso);
+            T candidate_max_dist = dot(cso.point, face.normal);
+            if (candidate_max_dist < max_dist) {
+                best_face_id = face_id;
+                max_dist = candidate_max_dist;
+            }
+            T curr_dist = -face_id.neg_dist;
+            if (max_dist - curr_dist < eps_tol ||
+                (std::fabs(curr_dist - old_dist) < eps && candidate_max_dist < max_dist)) {
+                const EpaFace<T> &bf = faces[best_face_id.id];
+                closest_points_of(bf, p1, p2);
+                n = bf.normal;
+                return true;
+            }
+            old_dist = curr_dist;
+            faces[face_id.id].deleted = true;
+            size_t o1 = next_ccw_pt_id(faces[face.adj[0]], face.pts[0]);
+            size_t o2 = next_ccw_pt_id(faces[face.adj[1]], face.pts[1]);
+            size_t o3 = next_ccw_pt_id(faces[face.adj[2]], face.pts[2]);
+            compute_silhouette(support_point_id, face.adj[0], o1);
+            compute_silhouette(support_point_id, face.adj[1], o2);
+            compute_silhouette(support_point_id, face.adj[2], o3);
+            size_t first_new_face_id = faces.size();
+            if (silhouette.empty()) return false;
+            for (const auto &edge : silhouette) {
+                if (!faces[edge.first].deleted) {
+                    size_t new_face_id = faces.size();
+                    size_t opp = edge.second;
+                    size_t pt_id1 = faces[edge.first].pts[(opp + 2) % 3];
+                    size_t pt_id2 = faces[edge.first].pts[(opp + 1) % 3];
+                    const size_t pts[3] = {pt_id1, pt_id2, support_point_id};
+                    const size_t adj[3] = {edge.first, new_face_id + 1, new_face_id - 1};
+                    auto nf = face_new(pts, adj);
+                    faces[edge.first].adj[(opp + 1) % 3] = new_face_id;
+                    faces.push_back(nf.first);
+                    if (nf.second) {
+                        V3<T> pt = vertices[faces[new_face_id].pts[0]].point;
+                        T dist = dot(faces[new_face_id].normal, pt);
+                        if (dist < curr_dist) {
+                            closest_points_of(face, p1, p2);
+                            n = face.normal;
+                            return true;
+                        }
+                        EpaFaceId<T> f;
+                        if (!fid_new(new_face_id, -dist, f)) return false;
+                        heap.push(f);
+                    }
+                }
+            }
+            if (first_new_face_id == faces.size()) return false;
+            faces[first_new_face_id].adj[2] = faces.size() - 1;
+            faces.back().adj[1] = first_new_face_id;
+            silhouette.clear();
+            niter += 1;
+            if (iters) *iters = niter;
+            if (niter > 100) break;
+        }
+        const EpaFace<T> &bf = faces[best_face_id.id];
+        closest_points_of(bf, p1, p2);
+        n = bf.normal;
+        return true;
+    }
+};
+
+// ---- contact (contact_support_map_support_map.rs, contact_ball_ball.rs,
+//      contact_shape_shape.rs) ---------------------------------------------
+
+template <class T> struct Contact {
+    V3<T> point1, point2, normal1, normal2;
+    T dist;
+};
+
+struct ContactStats {
+    uint32_t gjk_iters = 0;
+    uint32_t epa_iters = 0;
+    bool used_epa = false;
+};
+
+// contact_support_map_support_map_with_params (contact_support_map_support_map.rs:40-77).
+// init_dir: nullptr for parry's default seed.
+template <class T>
+GjkResult<T> contact_sm_sm_with_params(const Iso<T> &pos12, const Shape<T> &g1, const Shape<T> &g2, T prediction,
+                                       Simplex<T> &simplex, const V3<T> *init_dir, ContactStats *st) {
+    V3<T> dir;
+    if (init_dir) {
+        dir = *init_dir;
+    } else {
+        V3<T> t = pos12.t;
+        T sq = dot(t, t);
+        const T e = Eps<T>::eps;
+        if (sq > e * e) dir = t / std::sqrt(sq);
+        else dir = v3(T(1), T(0), T(0));
+    }
+    simplex.reset(cso_from_shapes(pos12, g1, g2, dir));
+    uint32_t gi = 0;
+    GjkResult<T> c = gjk_closest_points(pos12, g1, g2, prediction, simplex, &gi);
+    if (st) st->gjk_iters = gi;
+    if (c.kind != GJK_INTERSECTION) return c;
+    Epa<T> epa;
+    V3<T> p1, p2, n;
+    uint32_t ei = 0;
+    if (st) st->used_epa = true;
+    bool ok = epa.closest_points(pos12, g1, g2, simplex, p1, p2, n, &ei);
+    if (st) st->epa_iters = ei;
+    if (ok) return GjkResult<T>{GJK_CLOSEST_POINTS, p1, p2, n};
+    return GjkResult<T>{GJK_NO_INTERSECTION, {}, {}, v3(T(1), T(0), T(0))};
+}
+
+// contact_support_map_support_map (contact_support_map_support_map.rs:10-32), local frame of shape 1.
+template <class T>
+bool contact_sm_sm(const Iso<T> &pos12, const Shape<T> &g1, const Shape<T> &g2, T prediction, Contact<T> &out,
+                   const V3<T> *init_dir, ContactStats *st) {
+    Simplex<T> simplex;
+    GjkResult<T> r = contact_sm_sm_with_params(pos12, g1, g2, prediction, simplex, init_dir, st);
+    if (r.kind != GJK_CLOSEST_POINTS) return false;
+    V3<T> point1 = r.p1, point2_1 = r.p2, normal1 = r.n;
+    out.dist = dot(point2_1 - point1, normal1);
+    out.point1 = point1;
+    out.point2 = inv_rotate(pos12.q, point2_1 - pos12.t);
+    out.normal1 = normal1;
+    out.normal2 = inv_rotate(pos12.q, -normal1);
+    return true;
+}
+
+// contact_ball_ball (contact_ball_ball.rs:9-42).
+template <class T> bool contact_ball_ball(const Iso<T> &pos12, T r1, T r2, T prediction, Contact<T> &out) {
+    V3<T> c = pos12.t;
+    T d2 = dot(c, c);
+    T sum = r1 + r2;
+    T sum_err = sum + prediction;
+    if (!(d2 < sum_err * sum_err)) return false;
+    V3<T> n1 = d2 != T(0) ? c / norm(c) : v3(T(1), T(0), T(0));
+    V3<T> n2 = -inv_rotate(pos12.q, n1);
+    out.point1 = n1 * r1;
+    out.point2 = n2 * r2;
+    out.normal1 = n1;
+    out.normal2 = n2;
+    out.dist = std::sqrt(d2) - sum;
+    return true;
+}
+
+// query::contact (contact_shape_shape.rs:123-138) for the shape pairs handled
+// here (Ball-Ball and support-map pairs without a Ball).  Returns 1 on
+// contact, 0 for None, -1 for pairs not handled by this restatement.
+template <class T>
+int contact(const Iso<T> &pos1, const Shape<T> &g1, const Iso<T> &pos2, const Shape<T> &g2, T prediction,
+            Contact<T> &out, const V3<T> *init_dir, ContactStats *st) {
+    Iso<T> pos12 = inv_mul(pos1, pos2);
+    bool ok;
+    if (g1.type == BALL && g2.type == BALL) {
+        ok = contact_ball_ball(pos12, g1.p[0], g2.p[0], prediction, out);
+    } else if (g1.type == BALL || g2.type == BALL) {
+        return -1;
+    } else {
+        ok = contact_sm_sm(pos12, g1, g2, prediction, out, init_dir, st);
+    }
+    if (!ok) return 0;
+    // Contact::transform_by_mut (contact.rs:171-176)
+    out.point1 = transform_point(pos1, out.point1);
+    out.point2 = transform_point(pos2, out.point2);
+    out.normal1 = rotate(pos1.q, out.normal1);
+    out.normal2 = rotate(pos2.q, out.normal2);
+    return 1;
+}
+
+}  // namespace oracle
