@@ -218,6 +218,45 @@ int pq_bvh_update_and_refit(pq_bvh *bvh, const float *aabbs, void *stream);
  * equals parry's callback set; the order of the list is this function's. */
 int pq_bvh_self_pairs(pq_bvh *bvh, uint32_t *pairs, uint64_t capacity, uint64_t *count, void *stream);
 
+/* ---- Narrowphase contacts ------------------------------------------------ */
+
+/* parry3d::query::Contact (src/query/contact/contact.rs:71-105) in world space,
+ * plus `has_contact`: 1 = Some(contact), 0 = None (no contact within
+ * `prediction`), PQ_CONTACT_UNSUPPORTED = the pair's shape types are not
+ * handled by this build. */
+#define PQ_CONTACT_UNSUPPORTED 0xFFFFFFFFu
+typedef struct pq_contact {
+    float point1[3];
+    float point2[3];
+    float normal1[3];
+    float normal2[3];
+    float dist;
+    uint32_t has_contact;
+} pq_contact;
+
+/* Device workspace bytes pq_contact_pairs needs for `num_pairs` pairs. */
+size_t pq_contact_workspace_bytes(uint64_t num_pairs);
+
+/* Batched query::contact(pos1, g1, pos2, g2, prediction)
+ * (src/query/contact/contact_shape_shape.rs:123-138) over shape pairs.
+ *   shapes:    the shape set; poses: device, one pq_isometry per shape;
+ *   pairs:     device, num_pairs x (i, j) shape indices;
+ *   init_dirs: device, nullable; num_pairs x 3 floats: the GJK seed direction
+ *              (unit, in shape i's local frame) passed as `init_dir` to
+ *              contact_support_map_support_map_with_params
+ *              (contact_support_map_support_map.rs:40-60); NULL = parry's
+ *              default seed normalize(pos12.translation), else +x;
+ *   out:       device, num_pairs pq_contact;
+ *   iters:     device, nullable, 2 u32 per pair: GJK iterations and EPA
+ *              iterations (0 when EPA did not run);
+ *   workspace: device, nullable, pq_contact_workspace_bytes(num_pairs) bytes;
+ *              NULL = a library-owned per-stream buffer, grown on demand (the
+ *              growth allocates: give a workspace under hipGraph capture).
+ * Asynchronous on `stream`. */
+int pq_contact_pairs(const pq_shapes *shapes, const pq_isometry *poses, const uint32_t *pairs, uint64_t num_pairs,
+                     float prediction, const float *init_dirs, pq_contact *out, uint32_t *iters, void *workspace,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif

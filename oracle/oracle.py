@@ -241,6 +241,20 @@ _SIG.update({
 })
 
 SHAPE_TRIANGLE_TEST_ONLY = 5
+_SIG["orc_shape_aabbs_f32"] = (None, [_vp, _u32, _vp, _u32, _vp, _vp])
+
+
+def shape_aabbs(descs, vertices, poses) -> np.ndarray:
+    """Shape::compute_aabb(pose) for every shape of any type on the path (n, 6) f32."""
+    from parry_amd.shape import SHAPE_DESC_DTYPE
+
+    d = np.ascontiguousarray(descs, dtype=SHAPE_DESC_DTYPE)
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    p = np.ascontiguousarray(poses, dtype=np.float32).reshape(-1, 7)
+    out = np.zeros((d.shape[0], 6), np.float32)
+    lib().orc_shape_aabbs_f32(d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data,
+                              out.ctypes.data)
+    return out
 
 
 def contact_pairs(descs, vertices, poses, pairs, prediction: float, init_dirs=None, f64: bool = False,

@@ -307,12 +307,8 @@ struct ShapeDesc {  // same 32-byte layout as pq_shape_desc
     float params[4];
 };
 
-template <class T>
-void contact_pairs(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uint32_t nverts, const T *poses,
-                   const uint32_t *pairs, uint64_t m, T prediction, const T *init_dirs, T *out, int32_t *status,
-                   uint32_t *iters, int nthreads) {
-    std::vector<V3<T>> pts(nverts);
-    for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
+template <class T> std::vector<Shape<T>> make_shapes(const ShapeDesc *descs, uint32_t nshapes,
+                                                     const std::vector<V3<T>> &pts) {
     std::vector<Shape<T>> shapes(nshapes);
     for (uint32_t i = 0; i < nshapes; ++i) {
         shapes[i].type = descs[i].type;
@@ -320,6 +316,31 @@ void contact_pairs(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uin
         shapes[i].n = descs[i].num_vertices;
         for (int k = 0; k < 4; ++k) shapes[i].p[k] = (T)descs[i].params[k];
     }
+    return shapes;
+}
+
+template <class T>
+void shape_aabbs(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uint32_t nverts, const T *poses, T *out) {
+    std::vector<V3<T>> pts(nverts);
+    for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
+    std::vector<Shape<T>> shapes = make_shapes(descs, nshapes, pts);
+    for (uint32_t i = 0; i < nshapes; ++i) {
+        const T *a = poses + 7 * (size_t)i;
+        Iso<T> m{Quat<T>{a[0], a[1], a[2], a[3]}, v3(a[4], a[5], a[6])};
+        Aabb<T> b = shape_aabb(m, shapes[i]);
+        T *o = out + 6 * (size_t)i;
+        o[0] = b.mins.x; o[1] = b.mins.y; o[2] = b.mins.z;
+        o[3] = b.maxs.x; o[4] = b.maxs.y; o[5] = b.maxs.z;
+    }
+}
+
+template <class T>
+void contact_pairs(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uint32_t nverts, const T *poses,
+                   const uint32_t *pairs, uint64_t m, T prediction, const T *init_dirs, T *out, int32_t *status,
+                   uint32_t *iters, int nthreads) {
+    std::vector<V3<T>> pts(nverts);
+    for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
+    std::vector<Shape<T>> shapes = make_shapes(descs, nshapes, pts);
     parallel_for(m, nthreads, 64, [&](uint64_t b, uint64_t e) {
         for (uint64_t k = b; k < e; ++k) {
             uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
@@ -357,6 +378,12 @@ void contact_pairs(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uin
 }  // namespace
 
 extern "C" {
+
+// Shape::compute_aabb(pose) for every shape (n x 6).
+void orc_shape_aabbs_f32(const void *descs, uint32_t n, const float *verts, uint32_t nverts, const float *poses,
+                         float *out) {
+    shape_aabbs<float>((const ShapeDesc *)descs, n, verts, nverts, poses, out);
+}
 
 // query::contact for every (i, j) in `pairs` (m pairs).  out: 13 per pair
 // (point1, point2, normal1, normal2, dist, world frame); status: 1 contact,

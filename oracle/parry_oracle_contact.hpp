@@ -806,6 +806,51 @@ template <class T> struct Epa {
     }
 };
 
+// ---- Shape::compute_aabb for every type on the path ---------------------------
+// nalgebra UnitQuaternion::to_rotation_matrix (restated, unpinned at the ulp):
+// rows (ww+ii-jj-kk, ij-wk, wj+ik), (wk+ij, ww-ii+jj-kk, jk-wi),
+// (ik-wj, wi+jk, ww-ii-jj+kk) with ij = i*j*2 etc.; Matrix3 * v accumulates
+// (r0*v0 + r1*v1) + r2*v2.
+template <class T> Aabb<T> shape_aabb(const Iso<T> &m, const Shape<T> &s) {
+    switch (s.type) {
+        case CONVEX_POLYHEDRON:
+            return point_cloud_aabb(m, s.pts, s.n);
+        case BALL: {  // aabb_ball.rs:7-27
+            T r = s.p[0];
+            return Aabb<T>{m.t + v3(-r, -r, -r), m.t + v3(r, r, r)};
+        }
+        case CUBOID: {  // aabb_cuboid.rs:9-14, isometry_ops.rs:14-19
+            T i = m.q.x, j = m.q.y, k = m.q.z, w = m.q.w;
+            T ww = w * w, ii = i * i, jj = j * j, kk = k * k;
+            T ij = i * j * T(2), wk = w * k * T(2), wj = w * j * T(2), ik = i * k * T(2), jk = j * k * T(2),
+              wi = w * i * T(2);
+            T R[3][3] = {{ww + ii - jj - kk, ij - wk, wj + ik},
+                         {wk + ij, ww - ii + jj - kk, jk - wi},
+                         {ik - wj, wi + jk, ww - ii - jj + kk}};
+            V3<T> he = v3(s.p[0], s.p[1], s.p[2]), ws;
+            for (int a = 0; a < 3; ++a)
+                ws[a] = (std::fabs(R[a][0]) * he.x + std::fabs(R[a][1]) * he.y) + std::fabs(R[a][2]) * he.z;
+            return Aabb<T>{m.t - ws, m.t + ws};  // Aabb::from_half_extents
+        }
+        case CAPSULE: {  // aabb_capsule.rs:8-20
+            V3<T> a = transform_point(m, v3(T(0), -s.p[0], T(0))), b = transform_point(m, v3(T(0), s.p[0], T(0)));
+            T r = s.p[1];
+            return Aabb<T>{vmin(a, b) - v3(r, r, r), vmax(a, b) + v3(r, r, r)};
+        }
+        default: {  // support_map_aabb (aabb_utils.rs:10-31)
+            Aabb<T> out;
+            for (int d = 0; d < 3; ++d) {
+                V3<T> basis = v3(T(0), T(0), T(0));
+                basis[d] = T(1);
+                out.maxs[d] = support_point(m, s, basis)[d];
+                basis[d] = T(-1);
+                out.mins[d] = support_point(m, s, basis)[d];
+            }
+            return out;
+        }
+    }
+}
+
 // ---- contact (contact_support_map_support_map.rs, contact_ball_ball.rs,
 //      contact_shape_shape.rs) ---------------------------------------------
 

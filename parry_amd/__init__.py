@@ -3,7 +3,18 @@
 The compute path is the HIP library ``libparry_mi355x.so`` (C ABI in
 include/parry_mi355x.h); this package is its Python host mirror.
 """
-from .query import Ray, RayIntersection, FeatureId, RAY_DTYPE, RAY_HIT_DTYPE
+from .query import (
+    Ray,
+    RayIntersection,
+    FeatureId,
+    RAY_DTYPE,
+    RAY_HIT_DTYPE,
+    Contact,
+    CONTACT_DTYPE,
+    CONTACT_UNSUPPORTED,
+    contact_pairs,
+    contact_workspace_bytes,
+)
 from .shape import (
     TriMesh,
     TriMeshBuilderError,
@@ -34,4 +45,8 @@ __all__ = [
     "SHAPE_DESC_DTYPE",
     "Bvh",
     "scenes",
+    "Contact",
+    "CONTACT_DTYPE",
+    "contact_pairs",
+    "contact_workspace_bytes",
 ]

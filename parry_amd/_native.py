@@ -54,6 +54,8 @@ SIGNATURES = {
     "pq_bvh_export": (_int, [_vp, _vp, _u32]),
     "pq_bvh_update_and_refit": (_int, [_vp, _vp, _vp]),
     "pq_bvh_self_pairs": (_int, [_vp, _vp, _u64, _vp, _vp]),
+    "pq_contact_workspace_bytes": (_c.c_size_t, [_u64]),
+    "pq_contact_pairs": (_int, [_vp, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lock = threading.Lock()

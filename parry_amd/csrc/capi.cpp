@@ -49,6 +49,27 @@ hipError_t stream_queue(hipStream_t stream, unsigned long long **queue) {
     return hipSuccess;
 }
 
+hipError_t stream_workspace(hipStream_t stream, size_t bytes, void **ptr) {
+    static std::mutex mu;
+    static std::map<std::pair<int, hipStream_t>, std::pair<void *, size_t>> spaces;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    auto &slot = spaces[std::make_pair(dev, stream)];
+    if (slot.second < bytes) {
+        if (slot.first) {
+            if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+            (void)hipFree(slot.first);
+            slot = {nullptr, 0};
+        }
+        if ((e = hipMalloc(&slot.first, bytes)) != hipSuccess) return e;
+        slot.second = bytes;
+    }
+    *ptr = slot.first;
+    return hipSuccess;
+}
+
 namespace {
 
 // LDS budget of the ray kernel's per-lane stack (256 lanes x depth-1 entries x 4 B).

@@ -34,17 +34,15 @@ int pq_shapes_create(const pq_shape_desc *shapes, uint32_t n, const float *verti
         if (d.reserved != 0) return fail(PQ_ERR_INVALID, "pq_shapes_create: reserved field must be 0");
         switch (d.type) {
             case PQ_SHAPE_BALL:
+            case PQ_SHAPE_CUBOID:
+            case PQ_SHAPE_CAPSULE:
+            case PQ_SHAPE_CONE:
                 break;
             case PQ_SHAPE_CONVEX_POLYHEDRON:
                 if (d.num_vertices == 0 || (uint64_t)d.first_vertex + d.num_vertices > num_vertices)
                     return fail(PQ_ERR_INVALID, "pq_shapes_create: shape " + std::to_string(i) +
                                                     " has an empty or out-of-range vertex range");
                 break;
-            case PQ_SHAPE_CUBOID:
-            case PQ_SHAPE_CAPSULE:
-            case PQ_SHAPE_CONE:
-                return fail(PQ_ERR_UNSUPPORTED, "pq_shapes_create: shape type " + std::to_string(d.type) +
-                                                    " is not supported by this build yet");
             default:
                 return fail(PQ_ERR_INVALID, "pq_shapes_create: unknown shape type " + std::to_string(d.type));
         }
@@ -185,6 +183,20 @@ int pq_bvh_self_pairs(pq_bvh *b, uint32_t *pairs, uint64_t capacity, uint64_t *c
     if (!b || !count) return fail(PQ_ERR_INVALID, "pq_bvh_self_pairs: null argument");
     if (capacity && !pairs) return fail(PQ_ERR_INVALID, "pq_bvh_self_pairs: null pairs buffer");
     PQ_HIP_CHECK(launch_bvh_self_pairs(b->dev, pairs, capacity, (unsigned long long *)count, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+size_t pq_contact_workspace_bytes(uint64_t num_pairs) { return contact_workspace_bytes(num_pairs); }
+
+int pq_contact_pairs(const pq_shapes *s, const pq_isometry *poses, const uint32_t *pairs, uint64_t m, float prediction,
+                     const float *init_dirs, pq_contact *out, uint32_t *iters, void *workspace, void *stream) {
+    if (!s) return fail(PQ_ERR_INVALID, "pq_contact_pairs: null shapes");
+    if (m == 0) return PQ_OK;
+    if (!poses || !pairs || !out) return fail(PQ_ERR_INVALID, "pq_contact_pairs: null buffer");
+    void *ws = workspace;
+    if (!ws) PQ_HIP_CHECK(stream_workspace((hipStream_t)stream, contact_workspace_bytes(m), &ws));
+    PQ_HIP_CHECK(launch_contacts(s->dev, (const float *)poses, pairs, m, prediction, init_dirs, out, iters, ws,
+                                 (hipStream_t)stream));
     return PQ_OK;
 }
 

@@ -1,0 +1,496 @@
+// contact_gjk.h -- device-side support maps, simplex projections and GJK (gfx950).
+//
+// These functions evaluate parry3d's f32 algorithms in parry's operation order
+// (the file must be compiled with -ffp-contract=off):
+//   * support maps: ConvexPolyhedron (first strict maximum of p.dir,
+//     src/utils/point_cloud_support_point.rs:5-19), Ball (ball.rs:242-249),
+//     Cuboid (cuboid.rs:454-456 + utils/wops.rs:30-35), Capsule
+//     (capsule.rs:407-418), Cone (cone.rs:162-182), and the posed support
+//     point of shape 2 (support_map.rs:380-383);
+//   * CSOPoint::from_shapes (src/query/gjk/cso_point.rs:69-78);
+//   * closest points of the origin on a segment / triangle / tetrahedron with
+//     their Voronoi location (src/query/point/point_segment.rs:45-85,
+//     point_triangle.rs:54-291, point_tetrahedron.rs:28-340);
+//   * the 3D Voronoi simplex (src/query/gjk/voronoi_simplex3.rs:13-351);
+//   * gjk::closest_points with exact_dist = true (src/query/gjk/gjk.rs:353-453)
+//     and its result() (gjk.rs:797-818).
+// One lane evaluates one pair; all state lives in registers.
+#pragma once
+
+#include "pq_internal.h"
+#include "pq_math.h"
+
+namespace pq {
+
+constexpr float kEps = 1.1920929e-07f;      // f32::EPSILON = DEFAULT_EPSILON (src/lib.rs:102)
+constexpr float kEpsTol = kEps * 10.0f;     // gjk::eps_tol (gjk.rs:141-144)
+constexpr float kFltMax = 3.40282347e+38f;  // f32::MAX
+
+struct ShapeRef {
+    uint32_t type;
+    uint32_t n;
+    const float4 *v;
+    float p0, p1, p2;
+};
+
+__device__ __forceinline__ ShapeRef load_shape(const ShapeRecord *rec, const float4 *verts, uint32_t i) {
+    const ShapeRecord r = rec[i];
+    return ShapeRef{r.type, r.num_vertices, verts + r.first_vertex, r.param[0], r.param[1], r.param[2]};
+}
+
+__device__ __forceinline__ f3 local_support(const ShapeRef &s, f3 d) {
+    if (s.type == PQ_SHAPE_CONVEX_POLYHEDRON) {
+        const float4 p0 = s.v[0];
+        f3 best = f3{p0.x, p0.y, p0.z};
+        float best_dot = dot3(best, d);
+        for (uint32_t k = 1; k < s.n; ++k) {
+            const float4 p = s.v[k];
+            const f3 c = f3{p.x, p.y, p.z};
+            const float dd = dot3(c, d);
+            if (dd > best_dot) {
+                best_dot = dd;
+                best = c;
+            }
+        }
+        return best;
+    }
+    if (s.type == PQ_SHAPE_BALL) {
+        const float l = sqrtf(norm_sq3(d));
+        return mul3(div3(d, l), s.p0);
+    }
+    if (s.type == PQ_SHAPE_CUBOID) return f3{copysignf(s.p0, d.x), copysignf(s.p1, d.y), copysignf(s.p2, d.z)};
+    if (s.type == PQ_SHAPE_CAPSULE) {
+        const float sq = norm_sq3(d);
+        f3 u = sq > 0.0f ? div3(d, sqrtf(sq)) : f3{0.0f, 1.0f, 0.0f};
+        const f3 a = f3{0.0f, -s.p0, 0.0f}, b = f3{0.0f, s.p0, 0.0f};
+        return dot3(u, a) > dot3(u, b) ? add3(a, mul3(u, s.p1)) : add3(b, mul3(u, s.p1));
+    }
+    // PQ_SHAPE_CONE
+    f3 v = f3{d.x, 0.0f, d.z};
+    const float nrm = sqrtf(norm_sq3(v));
+    v = div3(v, nrm);
+    if (nrm == 0.0f) return f3{0.0f, copysignf(s.p0, d.y), 0.0f};
+    v = mul3(v, s.p1);
+    v.y = -s.p0;
+    if (dot3(d, v) < d.y * s.p0) return f3{0.0f, s.p0, 0.0f};
+    return v;
+}
+
+struct Cso {
+    f3 p, o1, o2;
+};
+
+// CSOPoint::from_shapes: s1(dir) - pos12 * s2(pos12^-1 * -dir).
+__device__ __forceinline__ Cso cso_from_shapes(const iso &pos12, const ShapeRef &g1, const ShapeRef &g2, f3 dir) {
+    const f3 a = local_support(g1, dir);
+    const f3 b = iso_point(pos12, local_support(g2, qrot_inv(pos12.q, neg3(dir))));
+    return Cso{sub3(a, b), a, b};
+}
+
+// relative_eq!(proj, origin) with approx's default tolerances.
+__device__ __forceinline__ bool near_origin(f3 p) {
+    return (fabsf(p.x) <= kEps) & (fabsf(p.y) <= kEps) & (fabsf(p.z) <= kEps);
+}
+
+enum : uint32_t { LOC_VERTEX = 0, LOC_EDGE = 1, LOC_FACE = 2, LOC_SOLID = 3 };
+struct Loc {
+    uint32_t kind, id;
+    float b0, b1, b2;
+};
+
+__device__ __forceinline__ f3 seg_origin(f3 a, f3 b, Loc &l) {
+    const f3 ab = sub3(b, a);
+    const f3 ap = neg3(a);  // origin - a
+    const float ab_ap = dot3(ab, ap);
+    const float sqnab = norm_sq3(ab);
+    if (ab_ap <= 0.0f) {
+        l = Loc{LOC_VERTEX, 0, 1.0f, 0.0f, 0.0f};
+        return a;
+    }
+    if (ab_ap >= sqnab) {
+        l = Loc{LOC_VERTEX, 1, 1.0f, 0.0f, 0.0f};
+        return b;
+    }
+    const float u = ab_ap / sqnab;
+    l = Loc{LOC_EDGE, 0, 1.0f - u, u, 0.0f};
+    return add3(a, mul3(ab, u));
+}
+
+// Triangle projection of the origin (solid); *inside = relative_eq!(proj, origin).
+__device__ __forceinline__ f3 tri_origin(f3 a, f3 b, f3 c, Loc &l, bool &inside) {
+    const f3 ab = sub3(b, a), ac = sub3(c, a), ap = neg3(a);
+    const float ab_ap = dot3(ab, ap), ac_ap = dot3(ac, ap);
+    if (ab_ap <= 0.0f && ac_ap <= 0.0f) {
+        l = Loc{LOC_VERTEX, 0, 0, 0, 0};
+        inside = near_origin(a);
+        return a;
+    }
+    const f3 bp = neg3(b);
+    const float ab_bp = dot3(ab, bp), ac_bp = dot3(ac, bp);
+    if (ab_bp >= 0.0f && ac_bp <= ab_bp) {
+        l = Loc{LOC_VERTEX, 1, 0, 0, 0};
+        inside = near_origin(b);
+        return b;
+    }
+    const f3 cp = neg3(c);
+    const float ab_cp = dot3(ab, cp), ac_cp = dot3(ac, cp);
+    if (ac_cp >= 0.0f && ab_cp <= ac_cp) {
+        l = Loc{LOC_VERTEX, 2, 0, 0, 0};
+        inside = near_origin(c);
+        return c;
+    }
+    const f3 bc = sub3(c, b);
+    const f3 n = cross3(ab, ac);
+    const float vc = dot3(n, cross3(ab, ap));
+    if (vc < 0.0f && ab_ap >= 0.0f && ab_bp <= 0.0f) {
+        const float v = ab_ap / norm_sq3(ab);
+        l = Loc{LOC_EDGE, 0, 1.0f - v, v, 0.0f};
+        const f3 r = add3(a, mul3(ab, v));
+        inside = near_origin(r);
+        return r;
+    }
+    const float vb = -dot3(n, cross3(ac, cp));
+    if (vb < 0.0f && ac_ap >= 0.0f && ac_cp <= 0.0f) {
+        const float w = ac_ap / norm_sq3(ac);
+        l = Loc{LOC_EDGE, 2, 1.0f - w, w, 0.0f};
+        const f3 r = add3(a, mul3(ac, w));
+        inside = near_origin(r);
+        return r;
+    }
+    const float va = dot3(n, cross3(bc, bp));
+    if (va < 0.0f && ac_bp - ab_bp >= 0.0f && ab_cp - ac_cp >= 0.0f) {
+        const float w = dot3(bc, bp) / norm_sq3(bc);
+        l = Loc{LOC_EDGE, 1, 1.0f - w, w, 0.0f};
+        const f3 r = add3(b, mul3(bc, w));
+        inside = near_origin(r);
+        return r;
+    }
+    const uint32_t side = dot3(n, ap) >= 0.0f ? 0u : 1u;
+    if (va + vb + vc != 0.0f) {
+        const float denom = 1.0f / (va + vb + vc);
+        const float v = vb * denom, w = vc * denom;
+        l = Loc{LOC_FACE, side, 1.0f - v - w, v, w};
+        const f3 r = add3(add3(a, mul3(ab, v)), mul3(ac, w));
+        inside = near_origin(r);
+        return r;
+    }
+    l = Loc{LOC_SOLID, 0, 0, 0, 0};
+    inside = true;
+    return f3{0.0f, 0.0f, 0.0f};
+}
+
+// Tetrahedron projection of the origin (solid).
+__device__ __noinline__ f3 tet_origin(f3 a, f3 b, f3 c, f3 d, Loc &l) {
+    const f3 ab = sub3(b, a), ac = sub3(c, a), ad = sub3(d, a), ap = neg3(a);
+    const float ap_ab = dot3(ap, ab), ap_ac = dot3(ap, ac), ap_ad = dot3(ap, ad);
+    if (ap_ab <= 0.0f && ap_ac <= 0.0f && ap_ad <= 0.0f) {
+        l = Loc{LOC_VERTEX, 0, 0, 0, 0};
+        return a;
+    }
+    const f3 bc = sub3(c, b), bd = sub3(d, b), bp = neg3(b);
+    const float bp_bc = dot3(bp, bc), bp_bd = dot3(bp, bd), bp_ab = dot3(bp, ab);
+    if (bp_bc <= 0.0f && bp_bd <= 0.0f && bp_ab >= 0.0f) {
+        l = Loc{LOC_VERTEX, 1, 0, 0, 0};
+        return b;
+    }
+    const f3 cd = sub3(d, c), cp = neg3(c);
+    const float cp_ac = dot3(cp, ac), cp_bc = dot3(cp, bc), cp_cd = dot3(cp, cd);
+    if (cp_cd <= 0.0f && cp_bc >= 0.0f && cp_ac >= 0.0f) {
+        l = Loc{LOC_VERTEX, 2, 0, 0, 0};
+        return c;
+    }
+    const f3 dp = neg3(d);
+    const float dp_cd = dot3(dp, cd), dp_bd = dot3(dp, bd), dp_ad = dot3(dp, ad);
+    if (dp_ad >= 0.0f && dp_bd >= 0.0f && dp_cd >= 0.0f) {
+        l = Loc{LOC_VERTEX, 3, 0, 0, 0};
+        return d;
+    }
+    // Edge regions; each test also yields the two signed volumes the face
+    // tests reuse.
+    f3 r;
+#define PQ_EDGE(I, EA, NABC, NABD, EAP, EAB, EAP_AB, EBP_AB, D1, D2)                                 \
+    float D1, D2;                                                                                    \
+    {                                                                                                \
+        const float ab_ab = (EAP_AB) - (EBP_AB);                                                     \
+        const f3 x = cross3(EAP, EAB);                                                               \
+        D1 = dot3(x, NABC);                                                                          \
+        D2 = dot3(x, NABD);                                                                          \
+        if (ab_ab != 0.0f && D1 >= 0.0f && D2 >= 0.0f && (EAP_AB) >= 0.0f && (EAP_AB) <= ab_ab) {    \
+            const float u = (EAP_AB) / ab_ab;                                                        \
+            l = Loc{LOC_EDGE, I, 1.0f - u, u, 0.0f};                                                 \
+            return add3(EA, mul3(EAB, u));                                                           \
+        }                                                                                            \
+    }
+    const f3 nabc = cross3(ab, ac), nabd = cross3(ab, ad);
+    PQ_EDGE(0, a, nabc, nabd, ap, ab, ap_ab, bp_ab, dabc, dabd)
+    const f3 nacd = cross3(ac, ad);
+    PQ_EDGE(1, a, nacd, neg3(nabc), ap, ac, ap_ac, cp_ac, dacd, dacb)
+    PQ_EDGE(2, a, neg3(nabd), neg3(nacd), ap, ad, ap_ad, dp_ad, dadb, dadc)
+    const f3 nbcd = cross3(bc, bd);
+    PQ_EDGE(3, b, nabc, nbcd, bp, bc, bp_bc, cp_bc, dbca, dbcd)
+    PQ_EDGE(4, b, neg3(nbcd), nabd, bp, bd, bp_bd, dp_bd, dbdc, dbda)
+    PQ_EDGE(5, c, nacd, nbcd, cp, cd, cp_cd, dp_cd, dcda, dcdb)
+#undef PQ_EDGE
+#define PQ_FACE(I, FA, FB, FC, FAP, FBP, FCP, FAB, FAC, FAD, D1, D2, D3)                            \
+    if (D1 < 0.0f && D2 < 0.0f && D3 < 0.0f) {                                                      \
+        const f3 n = cross3(FAB, FAC);                                                              \
+        if (dot3(n, FAD) * dot3(n, FAP) < 0.0f) {                                                   \
+            const float nn = sqrtf(norm_sq3(n));                                                    \
+            if (!(nn <= kEps)) {                                                                    \
+                const f3 nu = div3(n, nn);                                                          \
+                const float vc = dot3(nu, cross3(FAP, FBP));                                        \
+                const float va = dot3(nu, cross3(FBP, FCP));                                        \
+                const float vb = dot3(nu, cross3(FCP, FAP));                                        \
+                const float inv = 1.0f / ((va + vb) + vc);                                          \
+                const float b0 = va * inv, b1 = vb * inv, b2 = vc * inv;                            \
+                l = Loc{LOC_FACE, I, b0, b1, b2};                                                   \
+                return add3(add3(mul3(FA, b0), mul3(FB, b1)), mul3(FC, b2));                        \
+            }                                                                                       \
+        }                                                                                           \
+    }
+    PQ_FACE(0, a, b, c, ap, bp, cp, ab, ac, ad, dabc, dbca, dacb)
+    PQ_FACE(1, a, b, d, ap, bp, dp, ab, ad, ac, dadb, dabd, dbda)
+    PQ_FACE(2, a, c, d, ap, cp, dp, ac, ad, ab, dacd, dcda, dadc)
+    PQ_FACE(3, b, c, d, bp, cp, dp, bc, bd, neg3(ab), dbcd, dcdb, dbdc)
+#undef PQ_FACE
+    (void)r;
+    l = Loc{LOC_SOLID, 0, 0, 0, 0};
+    return f3{0.0f, 0.0f, 0.0f};
+}
+
+// VoronoiSimplex: vertices, barycentric coordinates of the last projection and
+// the previous simplex (as a permutation of the current vertex slots).
+struct Simplex {
+    Cso v[4];
+    float proj[3];
+    float prev_proj[3];
+    uint32_t perm;  // prev_vertices, 4 x 8 bits
+    uint32_t dim, prev_dim;
+};
+
+__device__ __forceinline__ uint32_t perm_get(uint32_t perm, uint32_t i) { return (perm >> (8 * i)) & 0xffu; }
+__device__ __forceinline__ uint32_t perm_swap(uint32_t perm, uint32_t i, uint32_t j) {
+    const uint32_t a = perm_get(perm, i), b = perm_get(perm, j);
+    perm &= ~((0xffu << (8 * i)) | (0xffu << (8 * j)));
+    return perm | (b << (8 * i)) | (a << (8 * j));
+}
+constexpr uint32_t kPermIdentity = 0x03020100u;
+
+__device__ __forceinline__ void sx_swap(Simplex &s, uint32_t i, uint32_t j) {
+    const Cso t = s.v[i];
+    s.v[i] = s.v[j];
+    s.v[j] = t;
+    s.perm = perm_swap(s.perm, i, j);
+}
+
+__device__ __forceinline__ void sx_reset(Simplex &s, const Cso &pt) {
+    s.dim = 0;
+    s.prev_dim = 0;
+    s.v[0] = pt;
+    s.perm = kPermIdentity;
+    s.proj[0] = s.proj[1] = s.proj[2] = 0.0f;
+    s.prev_proj[0] = s.prev_proj[1] = s.prev_proj[2] = 0.0f;
+}
+
+// VoronoiSimplex::add_point (voronoi_simplex3.rs:57-92).
+__device__ __forceinline__ bool sx_add(Simplex &s, const Cso &pt) {
+    s.prev_dim = s.dim;
+    s.prev_proj[0] = s.proj[0];
+    s.prev_proj[1] = s.proj[1];
+    s.prev_proj[2] = s.proj[2];
+    s.perm = kPermIdentity;
+    if (s.dim == 0) {
+        if (norm_sq3(sub3(s.v[0].p, pt.p)) < kEpsTol) return false;
+    } else if (s.dim == 1) {
+        const f3 ab = sub3(s.v[1].p, s.v[0].p), ac = sub3(pt.p, s.v[0].p);
+        if (norm_sq3(cross3(ab, ac)) < kEpsTol) return false;
+    } else {
+        const f3 ab = sub3(s.v[1].p, s.v[0].p), ac = sub3(s.v[2].p, s.v[0].p), ap = sub3(pt.p, s.v[0].p);
+        const f3 c = cross3(ab, ac);
+        const f3 n = div3(c, sqrtf(norm_sq3(c)));
+        if (fabsf(dot3(n, ap)) < kEpsTol) return false;
+    }
+    s.dim += 1;
+    s.v[s.dim] = pt;
+    return true;
+}
+
+// VoronoiSimplex::project_origin_and_reduce (voronoi_simplex3.rs:123-279).
+__device__ __noinline__ f3 sx_project(Simplex &s) {
+    if (s.dim == 0) {
+        s.proj[0] = 1.0f;
+        return s.v[0].p;
+    }
+    Loc l;
+    if (s.dim == 1) {
+        const f3 p = seg_origin(s.v[0].p, s.v[1].p, l);
+        if (l.kind == LOC_VERTEX) {
+            if (l.id == 1) sx_swap(s, 0, 1);
+            s.proj[0] = 1.0f;
+            s.dim = 0;
+        } else {
+            s.proj[0] = l.b0;
+            s.proj[1] = l.b1;
+        }
+        return p;
+    }
+    if (s.dim == 2) {
+        bool inside;
+        const f3 p = tri_origin(s.v[0].p, s.v[1].p, s.v[2].p, l, inside);
+        if (l.kind == LOC_VERTEX) {
+            sx_swap(s, 0, l.id);
+            s.proj[0] = 1.0f;
+            s.dim = 0;
+        } else if (l.kind == LOC_EDGE) {
+            if (l.id == 0) {
+                s.proj[0] = l.b0;
+                s.proj[1] = l.b1;
+            } else if (l.id == 1) {
+                sx_swap(s, 0, 2);
+                s.proj[0] = l.b1;
+                s.proj[1] = l.b0;
+            } else {
+                sx_swap(s, 1, 2);
+                s.proj[0] = l.b0;
+                s.proj[1] = l.b1;
+            }
+            s.dim = 1;
+        } else if (l.kind == LOC_FACE) {
+            s.proj[0] = l.b0;
+            s.proj[1] = l.b1;
+            s.proj[2] = l.b2;
+        }
+        return p;
+    }
+    const f3 p = tet_origin(s.v[0].p, s.v[1].p, s.v[2].p, s.v[3].p, l);
+    if (l.kind == LOC_VERTEX) {
+        sx_swap(s, 0, l.id);
+        s.proj[0] = 1.0f;
+        s.dim = 0;
+    } else if (l.kind == LOC_EDGE) {
+        switch (l.id) {
+            case 1: sx_swap(s, 1, 2); break;
+            case 2: sx_swap(s, 1, 3); break;
+            case 3: sx_swap(s, 0, 2); break;
+            case 4: sx_swap(s, 0, 3); break;
+            case 5: sx_swap(s, 0, 2); sx_swap(s, 1, 3); break;
+            default: break;
+        }
+        const bool flip = l.id == 3 || l.id == 4;
+        s.proj[0] = flip ? l.b1 : l.b0;
+        s.proj[1] = flip ? l.b0 : l.b1;
+        s.dim = 1;
+    } else if (l.kind == LOC_FACE) {
+        if (l.id == 0) {
+            s.proj[0] = l.b0; s.proj[1] = l.b1; s.proj[2] = l.b2;
+        } else if (l.id == 1) {
+            s.v[2] = s.v[3];
+            s.proj[0] = l.b0; s.proj[1] = l.b1; s.proj[2] = l.b2;
+        } else if (l.id == 2) {
+            s.v[1] = s.v[3];
+            s.proj[0] = l.b0; s.proj[1] = l.b2; s.proj[2] = l.b1;
+        } else {
+            s.v[0] = s.v[3];
+            s.proj[0] = l.b2; s.proj[1] = l.b0; s.proj[2] = l.b1;
+        }
+        s.dim = 2;
+    }
+    return p;
+}
+
+// gjk::result (gjk.rs:797-818): points from the current or previous simplex.
+__device__ __forceinline__ void sx_points(const Simplex &s, bool prev, f3 &p1, f3 &p2) {
+    p1 = p2 = f3{0.0f, 0.0f, 0.0f};
+    const uint32_t k = prev ? s.prev_dim : s.dim;
+    for (uint32_t i = 0; i <= k; ++i) {
+        const float c = prev ? s.prev_proj[i] : s.proj[i];
+        const Cso &pt = s.v[prev ? perm_get(s.perm, i) : i];
+        p1 = add3(p1, mul3(pt.o1, c));
+        p2 = add3(p2, mul3(pt.o2, c));
+    }
+}
+
+enum : uint32_t { GJK_INTERSECTION = 0, GJK_CLOSEST = 1, GJK_NO_INTERSECTION = 2 };
+
+struct GjkOut {
+    uint32_t kind;
+    uint32_t iters;
+    f3 p1, p2, n;
+};
+
+// gjk::closest_points (gjk.rs:353-453) with exact_dist = true.
+__device__ __noinline__ GjkOut gjk_closest_points(const iso &pos12, const ShapeRef &g1, const ShapeRef &g2,
+                                                  float max_dist, Simplex &s) {
+    const float eps_rel = sqrtf(kEpsTol);
+    GjkOut out{GJK_INTERSECTION, 0, f3{0, 0, 0}, f3{0, 0, 0}, f3{0, 0, 0}};
+    f3 proj = sx_project(s);
+    f3 old_dir;
+    {
+        const float sq = norm_sq3(proj);
+        if (!(sq > 0.0f)) return out;
+        old_dir = neg3(div3(proj, sqrtf(sq)));
+    }
+    float max_bound = kFltMax;
+    for (uint32_t niter = 0;;) {
+        const float old_max_bound = max_bound;
+        const f3 mp = neg3(proj);
+        const float sq = norm_sq3(mp);
+        if (!(sq > kEpsTol * kEpsTol)) {
+            out.kind = GJK_INTERSECTION;
+            return out;
+        }
+        const float nrm = sqrtf(sq);
+        const f3 dir = div3(mp, nrm);
+        max_bound = nrm;
+        if (max_bound >= old_max_bound) {
+            out.kind = GJK_CLOSEST;
+            out.n = old_dir;
+            sx_points(s, true, out.p1, out.p2);
+            return out;
+        }
+        const Cso c = cso_from_shapes(pos12, g1, g2, dir);
+        const float min_bound = -dot3(dir, c.p);
+        if (!isfinite(min_bound)) {  // parry asserts here (gjk.rs:407)
+            out.kind = GJK_NO_INTERSECTION;
+            return out;
+        }
+        if (min_bound > max_dist) {
+            out.kind = GJK_NO_INTERSECTION;
+            out.n = dir;
+            return out;
+        }
+        if (max_bound - min_bound <= eps_rel * max_bound) {
+            out.kind = GJK_CLOSEST;
+            out.n = dir;
+            sx_points(s, false, out.p1, out.p2);
+            return out;
+        }
+        if (!sx_add(s, c)) {
+            out.kind = GJK_CLOSEST;
+            out.n = dir;
+            sx_points(s, false, out.p1, out.p2);
+            return out;
+        }
+        old_dir = dir;
+        proj = sx_project(s);
+        if (s.dim == 3) {
+            if (min_bound >= kEpsTol) {
+                out.kind = GJK_CLOSEST;
+                out.n = old_dir;
+                sx_points(s, true, out.p1, out.p2);
+            } else {
+                out.kind = GJK_INTERSECTION;
+            }
+            return out;
+        }
+        ++niter;
+        out.iters = niter;
+        if (niter == 100) {
+            out.kind = GJK_NO_INTERSECTION;
+            out.n = f3{1.0f, 0.0f, 0.0f};
+            return out;
+        }
+    }
+}
+
+}  // namespace pq

@@ -118,6 +118,17 @@ hipError_t launch_bvh_self_pairs(const DeviceBvh &b, uint32_t *pairs, uint64_t c
                                  hipStream_t stream);
 hipError_t bvh_scan_temp_bytes(uint32_t num_leaves, size_t *bytes);
 
+// ---- contacts ------------------------------------------------------------
+// Waves (and 438 KB global-memory polytope buffers) of the EPA pass that
+// re-runs pairs whose polytope outgrew the LDS capacity.
+constexpr uint32_t kEpaGlobalWaves = 32;
+size_t contact_workspace_bytes(uint64_t num_pairs);
+hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
+                           float prediction, const float *seeds, pq_contact *out, uint32_t *iters, void *workspace,
+                           hipStream_t stream);
+// Per-(device, stream) workspace owned by the library, grown on demand.
+hipError_t stream_workspace(hipStream_t stream, size_t bytes, void **ptr);
+
 }  // namespace pq
 
 struct pq_shapes {

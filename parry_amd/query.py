@@ -45,3 +45,69 @@ class RayIntersection:
     time_of_impact: float
     normal: Vec3
     feature: FeatureId
+
+
+# ---------------------------------------------------------------------------
+# Contacts
+# ---------------------------------------------------------------------------
+
+# pq_contact (include/parry_mi355x.h): parry's Contact in world space + has_contact.
+CONTACT_DTYPE = np.dtype(
+    [("point1", "<f4", 3), ("point2", "<f4", 3), ("normal1", "<f4", 3), ("normal2", "<f4", 3), ("dist", "<f4"),
+     ("has_contact", "<u4")]
+)
+assert CONTACT_DTYPE.itemsize == 56
+CONTACT_UNSUPPORTED = 0xFFFFFFFF
+
+
+@dataclass(frozen=True)
+class Contact:
+    """parry3d ``Contact`` (src/query/contact/contact.rs:71-105)."""
+
+    point1: Vec3
+    point2: Vec3
+    normal1: Vec3
+    normal2: Vec3
+    dist: float
+
+
+def contact_pairs(shapes, poses, pairs, prediction: float, init_dirs=None, with_iters: bool = False, out=None,
+                  workspace=None, stream=None):
+    """Batched ``query::contact`` (contact_shape_shape.rs:123-138) on CUDA tensors.
+
+    ``shapes``: a ``ShapeSet``; ``poses``: (n, 7) f32 (quaternion i j k w,
+    translation); ``pairs``: (m, 2) int32/uint32; ``init_dirs``: optional (m, 3)
+    f32 GJK seeds in shape 1's local frame.  Returns a (m, 14) f32 tensor of
+    ``pq_contact`` records (view ``has_contact`` with ``.view(torch.int32)[:, 13]``)
+    and, if ``with_iters``, a (m, 2) int32 tensor of GJK / EPA iterations.
+    """
+    import ctypes
+
+    import torch
+
+    from . import _native
+
+    m = pairs.shape[0]
+    dev = poses.device
+    if out is None:
+        out = torch.empty((m, 14), dtype=torch.float32, device=dev)
+    iters = torch.empty((m, 2), dtype=torch.int32, device=dev) if with_iters else None
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    for t in (poses, pairs) + ((init_dirs,) if init_dirs is not None else ()):
+        if not (t.is_cuda and t.is_contiguous()):
+            raise ValueError("contact_pairs: tensors must be contiguous CUDA tensors")
+    _native.check(
+        _native.lib().pq_contact_pairs(
+            shapes.handle, poses.data_ptr(), pairs.data_ptr(), m, float(prediction),
+            None if init_dirs is None else init_dirs.data_ptr(), out.data_ptr(),
+            None if iters is None else iters.data_ptr(), None if workspace is None else workspace.data_ptr(),
+            ctypes.c_void_p(s.cuda_stream),
+        )
+    )
+    return (out, iters) if with_iters else out
+
+
+def contact_workspace_bytes(num_pairs: int) -> int:
+    from . import _native
+
+    return int(_native.lib().pq_contact_workspace_bytes(num_pairs))

@@ -237,3 +237,42 @@ def polyhedra_scene(n: int, seed: int, target_overlaps: float = 8.0):
     t = rng.uniform(0.0, L, size=(n, 3))
     poses = np.ascontiguousarray(np.concatenate([q, t], 1), dtype=np.float32)
     return {"vertices": verts, "offsets": offs, "poses": poses, "side": L}
+
+
+MIXED_TYPES = ("convex_polyhedron", "cuboid", "ball", "capsule", "cone")
+
+
+def mixed_shapes(n: int, seed: int, kmin: int = 8, kmax: int = 64):
+    """``n`` shapes with types uniform over ConvexPolyhedron (kmin..kmax hull
+    points on a jittered ellipsoid, semi-axes U[0.5, 1.5]), Cuboid (half extents
+    U[0.5, 1.5]^3), Ball (radius U[0.5, 1.5]), Capsule (half height U[0.5, 1.5],
+    radius U[0.25, 0.75]) and Cone (half height and radius U[0.5, 1.5]).
+
+    Returns (descs SHAPE_DESC_DTYPE [n], vertices float32 [V, 3]).
+    """
+    from .shape import SHAPE_DESC_DTYPE
+
+    rng = _rng(seed)
+    kinds = rng.integers(0, 5, size=n)
+    descs = np.zeros(n, dtype=SHAPE_DESC_DTYPE)
+    verts = []
+    nv = 0
+    type_code = {0: 4, 1: 1, 2: 0, 3: 2, 4: 3}  # -> PQ_SHAPE_* codes
+    for i, k in enumerate(kinds):
+        descs[i]["type"] = type_code[int(k)]
+        if k == 0:
+            v = convex_hull_vertices(convex_point_cloud(rng, int(rng.integers(kmin, kmax + 1))))
+            descs[i]["first_vertex"] = nv
+            descs[i]["num_vertices"] = v.shape[0]
+            verts.append(v)
+            nv += v.shape[0]
+        elif k == 1:
+            descs[i]["params"][:3] = rng.uniform(0.5, 1.5, 3)
+        elif k == 2:
+            descs[i]["params"][0] = rng.uniform(0.5, 1.5)
+        elif k == 3:
+            descs[i]["params"][:2] = [rng.uniform(0.5, 1.5), rng.uniform(0.25, 0.75)]
+        else:
+            descs[i]["params"][:2] = rng.uniform(0.5, 1.5, 2)
+    v = np.concatenate(verts, 0) if verts else np.zeros((0, 3))
+    return descs, np.ascontiguousarray(v, dtype=np.float32)

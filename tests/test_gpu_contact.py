@@ -1,0 +1,158 @@
+"""GPU parity of batched query::contact against the oracle (through the C ABI).
+
+Bar: has_contact identical to the f32 oracle (parry3d) on every pair;
+point1, point2, normal1, normal2 and dist within 1e-5 relative (|a-b| <=
+1e-5 * max(1, |b|)) of the f32 oracle; the number of bit-identical records and
+the error against the f64 oracle (parry3d-f64) are printed.
+"""
+import numpy as np
+import pytest
+
+from parry_amd import SHAPE_DESC_DTYPE, Bvh, ShapeSet, contact_pairs, scenes
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+UNSUP = 0xFFFFFFFF
+
+
+def _run_gpu(shapes, poses, pairs, prediction, seeds=None):
+    import torch
+
+    p = torch.from_numpy(np.ascontiguousarray(poses, np.float32)).cuda()
+    pr = torch.from_numpy(np.ascontiguousarray(pairs, np.uint32).view(np.int32)).cuda()
+    sd = None if seeds is None else torch.from_numpy(np.ascontiguousarray(seeds, np.float32)).cuda()
+    out, iters = contact_pairs(shapes, p, pr, prediction, init_dirs=sd, with_iters=True)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    return o[:, :13], o.view(np.uint32)[:, 13], iters.cpu().numpy()
+
+
+def _compare(label, gpu, has, ref):
+    st = ref["status"]
+    handled = st >= 0
+    assert (has[~handled] == UNSUP).all()
+    assert np.array_equal(has[handled].astype(np.int64), st[handled].astype(np.int64)), (
+        f"{label}: {np.count_nonzero(has[handled] != st[handled])} has_contact mismatches")
+    c = (st == 1)
+    err = np.abs(gpu[c] - ref["out"][c]) / np.maximum(1.0, np.abs(ref["out"][c]))
+    bad = (err > TOL).any(axis=1)
+    exact = np.all(gpu[c] == ref["out"][c], axis=1)
+    print(f"{label}: {c.sum()} contacts, {exact.sum()} bit-identical, max rel err {err.max(initial=0):.3e}")
+    assert not bad.any(), f"{label}: {bad.sum()} contacts beyond {TOL}"
+    return c
+
+
+def test_c1_pairs_full_size(orc):
+    import torch
+
+    print("case: C1 polyhedra n=65536 seed=3, broadphase pairs, prediction 0.01", flush=True)
+    sc = scenes.polyhedra_scene(65536, seed=3)
+    shapes = ShapeSet.convex_polyhedra(sc["vertices"], sc["offsets"])
+    poses = torch.from_numpy(sc["poses"]).cuda()
+    aabbs = shapes.compute_aabbs(poses)
+    pairs = Bvh(aabbs.cpu().numpy()).self_pairs().cpu().numpy().view(np.uint32)
+    gpu, has, iters = _run_gpu(shapes, sc["poses"], pairs, 0.01)
+    ref = orc.contact_pairs(shapes.descs, sc["vertices"], sc["poses"], pairs, 0.01)
+    c = _compare("C1", gpu, has, ref)
+    assert np.array_equal(iters[:, 0], ref["iters"][:, 0].astype(np.int32))
+    epa = ref["iters"][:, 1] > 0
+    print(f"C1: {pairs.shape[0]} pairs, {c.mean():.3f} in contact, {epa.mean():.3f} through EPA, "
+          f"mean GJK iterations {iters[:, 0].mean():.3f}")
+    # error vs parry3d-f64
+    r64 = orc.contact_pairs(shapes.descs, sc["vertices"], sc["poses"], pairs, 0.01, f64=True)
+    both = c & (r64["status"] == 1)
+    d = np.abs(gpu[both, 12] - r64["out"][both, 12])
+    print(f"C1 vs f64 oracle: has_contact agreement {(r64['status'] == ref['status']).mean():.6f}, "
+          f"dist abs err max {d.max():.3e} / 99.9% {np.quantile(d, 0.999):.3e}")
+
+
+def _mixed_pairs(n, seed, spread):
+    descs, verts = scenes.mixed_shapes(n, seed)
+    rng = np.random.default_rng(seed + 1)
+    q = scenes.random_rotations(n, rng)
+    t = rng.uniform(0, spread, (n, 3))
+    poses = np.concatenate([q, t], 1).astype(np.float32)
+    return descs, verts, poses
+
+
+def test_mixed_support_map_pairs(orc):
+    descs, verts, poses = _mixed_pairs(4000, 7, 18.0)
+    shapes = ShapeSet(descs, verts)
+    import torch
+
+    aabbs = shapes.compute_aabbs(torch.from_numpy(poses).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(aabbs.cpu().numpy(), orc.shape_aabbs(descs, verts, poses))
+    pairs = Bvh(aabbs.cpu().numpy()).self_pairs().cpu().numpy().view(np.uint32)
+    print(f"case: mixed shapes n=4000, {pairs.shape[0]} pairs", flush=True)
+    gpu, has, _ = _run_gpu(shapes, poses, pairs, 0.01)
+    ref = orc.contact_pairs(descs, verts, poses, pairs, 0.01)
+    _compare("mixed", gpu, has, ref)
+
+
+def test_kat_and_deep_penetration(orc):
+    # Cuboid(2,1,1) KAT (epa3.rs:6-23) plus deep capsule / cone / cuboid overlaps,
+    # whose curved or flat polytopes take many EPA iterations.
+    d = np.zeros(6, SHAPE_DESC_DTYPE)
+    d["type"] = [1, 1, 2, 2, 3, 3]
+    d["params"][0] = [2, 1, 1, 0]
+    d["params"][1] = [2, 1, 1, 0]
+    d["params"][2] = [1.0, 0.5, 0, 0]
+    d["params"][3] = [0.7, 0.6, 0, 0]
+    d["params"][4] = [1.0, 0.8, 0, 0]
+    d["params"][5] = [0.9, 1.1, 0, 0]
+    shapes = ShapeSet(d, np.zeros((0, 3), np.float32))
+    rng = np.random.default_rng(5)
+    poses, pairs = [], []
+    for t in ((3.5, 0, 0), (0, 0.2, 0)):
+        poses += [[0, 0, 0, 1, *t], [0, 0, 0, 1, 0, 0, 0]]
+    for _ in range(400):
+        q = scenes.random_rotations(2, rng)
+        off = rng.standard_normal(3) * rng.uniform(0.0, 0.8)
+        poses += [list(q[0]) + [0, 0, 0], list(q[1]) + list(off)]
+    poses = np.asarray(poses, np.float32)
+    # re-map: every pose row gets its own shape index via a repeated shape table
+    k = poses.shape[0]
+    kinds = np.concatenate([[0, 1, 0, 1], rng.integers(0, 6, k - 4)])
+    dd = d[kinds]
+    shapes = ShapeSet(dd, np.zeros((0, 3), np.float32))
+    pairs = np.arange(k, dtype=np.uint32).reshape(-1, 2)
+    print(f"case: KAT + deep penetration, {pairs.shape[0]} pairs", flush=True)
+    gpu, has, iters = _run_gpu(shapes, poses, pairs, 10.0)
+    ref = orc.contact_pairs(dd, np.zeros((0, 3), np.float32), poses, pairs, 10.0)
+    _compare("deep", gpu, has, ref)
+    assert gpu[0, 12] == np.float32(-0.5) and tuple(gpu[0, 6:9]) == (-1, 0, 0)
+    assert gpu[1, 12] == np.float32(-1.8) and tuple(gpu[1, 6:9]) == (0, -1, 0)
+    print(f"deep: max EPA iterations {iters[:, 1].max()}")
+
+
+def test_seeded_gjk_matches_oracle(orc):
+    descs, verts, poses = _mixed_pairs(3000, 11, 16.0)
+    keep = descs["type"] != 0  # no Balls (their support-map pairs go through projections)
+    descs = descs[keep]
+    poses = poses[keep]
+    shapes = ShapeSet(descs, verts)
+    aabbs = orc.shape_aabbs(descs, verts, poses)
+    pairs, _ = orc.OracleBvh(aabbs).self_pairs()
+    rng = np.random.default_rng(3)
+    seeds = rng.standard_normal((pairs.shape[0], 3))
+    seeds = (seeds / np.linalg.norm(seeds, axis=1, keepdims=True)).astype(np.float32)
+    print(f"case: seeded GJK, {pairs.shape[0]} pairs", flush=True)
+    gpu, has, iters = _run_gpu(shapes, poses, pairs, 0.01, seeds)
+    ref = orc.contact_pairs(descs, verts, poses, pairs, 0.01, init_dirs=seeds)
+    _compare("seeded", gpu, has, ref)
+    assert np.array_equal(iters[:, 0], ref["iters"][:, 0].astype(np.int32))
+
+
+def test_ball_ball_and_empty(orc):
+    d = np.zeros(3, SHAPE_DESC_DTYPE)
+    d["type"] = 0
+    d["params"][:, 0] = [0.75, 0.5, 1.0]
+    shapes = ShapeSet(d, np.zeros((0, 3), np.float32))
+    poses = np.array([[0, 0, 0, 1, 0, 0, 0], [0, 0, 0, 1, 1.0, 0.5, 0], [0, 0.6, 0, 0.8, 0, 0, 0]], np.float32)
+    pairs = np.array([[0, 1], [1, 0], [0, 2], [2, 2]], np.uint32)
+    gpu, has, _ = _run_gpu(shapes, poses, pairs, 0.01)
+    ref = orc.contact_pairs(d, np.zeros((0, 3), np.float32), poses, pairs, 0.01)
+    _compare("balls", gpu, has, ref)
+    gpu, has, _ = _run_gpu(shapes, poses, np.zeros((0, 2), np.uint32), 0.01)
+    assert gpu.shape[0] == 0
