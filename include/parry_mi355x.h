@@ -257,6 +257,15 @@ int pq_contact_pairs(const pq_shapes *shapes, const pq_isometry *poses, const ui
                      float prediction, const float *init_dirs, pq_contact *out, uint32_t *iters, void *workspace,
                      void *stream);
 
+/* Same as pq_contact_pairs for the first min(capacity, *num_pairs) pairs, where
+ * num_pairs is a device u64 (e.g. the count written by pq_bvh_self_pairs), so
+ * a broadphase + narrowphase frame runs without a host round trip.  Records
+ * beyond the count are not written.  workspace: pq_contact_workspace_bytes(capacity). */
+int pq_contact_pairs_device_count(const pq_shapes *shapes, const pq_isometry *poses, const uint32_t *pairs,
+                                  uint64_t capacity, const uint64_t *num_pairs, float prediction,
+                                  const float *init_dirs, pq_contact *out, uint32_t *iters, void *workspace,
+                                  void *stream);
+
 #ifdef __cplusplus
 }
 #endif

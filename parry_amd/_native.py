@@ -56,6 +56,7 @@ SIGNATURES = {
     "pq_bvh_self_pairs": (_int, [_vp, _vp, _u64, _vp, _vp]),
     "pq_contact_workspace_bytes": (_c.c_size_t, [_u64]),
     "pq_contact_pairs": (_int, [_vp, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp]),
+    "pq_contact_pairs_device_count": (_int, [_vp, _vp, _vp, _u64, _vp, _f32, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lock = threading.Lock()

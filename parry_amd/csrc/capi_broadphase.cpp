@@ -195,8 +195,22 @@ int pq_contact_pairs(const pq_shapes *s, const pq_isometry *poses, const uint32_
     if (!poses || !pairs || !out) return fail(PQ_ERR_INVALID, "pq_contact_pairs: null buffer");
     void *ws = workspace;
     if (!ws) PQ_HIP_CHECK(stream_workspace((hipStream_t)stream, contact_workspace_bytes(m), &ws));
-    PQ_HIP_CHECK(launch_contacts(s->dev, (const float *)poses, pairs, m, prediction, init_dirs, out, iters, ws,
-                                 (hipStream_t)stream));
+    PQ_HIP_CHECK(launch_contacts(s->dev, (const float *)poses, pairs, m, nullptr, prediction, init_dirs, out, iters,
+                                 ws, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_contact_pairs_device_count(const pq_shapes *s, const pq_isometry *poses, const uint32_t *pairs,
+                                  uint64_t capacity, const uint64_t *num_pairs, float prediction,
+                                  const float *init_dirs, pq_contact *out, uint32_t *iters, void *workspace,
+                                  void *stream) {
+    if (!s || !num_pairs) return fail(PQ_ERR_INVALID, "pq_contact_pairs_device_count: null argument");
+    if (capacity == 0) return PQ_OK;
+    if (!poses || !pairs || !out) return fail(PQ_ERR_INVALID, "pq_contact_pairs_device_count: null buffer");
+    void *ws = workspace;
+    if (!ws) PQ_HIP_CHECK(stream_workspace((hipStream_t)stream, contact_workspace_bytes(capacity), &ws));
+    PQ_HIP_CHECK(launch_contacts(s->dev, (const float *)poses, pairs, capacity, num_pairs, prediction, init_dirs, out,
+                                 iters, ws, (hipStream_t)stream));
     return PQ_OK;
 }
 

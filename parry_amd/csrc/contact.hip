@@ -83,10 +83,14 @@ __global__ void reset_counters_kernel(uint32_t *counters) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
     const ShapeRecord *__restrict__ rec, const float4 *__restrict__ verts, const float *__restrict__ poses,
-    const uint32_t *__restrict__ pairs, uint64_t m, float prediction, const float *__restrict__ seeds,
-    DevContact *__restrict__ out, uint32_t *__restrict__ iters, EpaItem *__restrict__ queue,
-    uint32_t *__restrict__ counters) {
+    const uint32_t *__restrict__ pairs, uint64_t m, const unsigned long long *__restrict__ m_dev, float prediction,
+    const float *__restrict__ seeds, DevContact *__restrict__ out, uint32_t *__restrict__ iters,
+    EpaItem *__restrict__ queue, uint32_t *__restrict__ counters) {
     const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (m_dev) {
+        const uint64_t md = *m_dev;
+        if (md < m) m = md;
+    }
     bool need_epa = false;
     Simplex s;
     uint32_t pair_id = 0;
@@ -600,8 +604,8 @@ size_t contact_workspace_bytes(uint64_t num_pairs) {
 }
 
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
-                           float prediction, const float *seeds, pq_contact *out, uint32_t *iters, void *workspace,
-                           hipStream_t stream) {
+                           const uint64_t *m_dev, float prediction, const float *seeds, pq_contact *out,
+                           uint32_t *iters, void *workspace, hipStream_t stream) {
     if (m == 0) return hipSuccess;
     const size_t align = 256;
     auto up = [&](size_t b) { return (b + align - 1) / align * align; };
@@ -612,7 +616,8 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
     EpaLarge *scratch = (EpaLarge *)(ws + up(16) + up(m * sizeof(EpaItem)) + up(m * sizeof(uint32_t)));
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters);
     hipLaunchKernelGGL(contact_gjk_kernel, dim3((uint32_t)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
-                       s.rec, s.verts, poses, pairs, m, prediction, seeds, (DevContact *)out, iters, queue, counters);
+                       s.rec, s.verts, poses, pairs, m, (const unsigned long long *)m_dev, prediction, seeds,
+                       (DevContact *)out, iters, queue, counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     int dev = 0, cus = 0;

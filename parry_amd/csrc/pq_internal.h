@@ -123,9 +123,11 @@ hipError_t bvh_scan_temp_bytes(uint32_t num_leaves, size_t *bytes);
 // re-runs pairs whose polytope outgrew the LDS capacity.
 constexpr uint32_t kEpaGlobalWaves = 32;
 size_t contact_workspace_bytes(uint64_t num_pairs);
+// m_dev (nullable, device): the number of pairs actually present (<= m), e.g.
+// the count written by pq_bvh_self_pairs, so a frame needs no host round trip.
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
-                           float prediction, const float *seeds, pq_contact *out, uint32_t *iters, void *workspace,
-                           hipStream_t stream);
+                           const uint64_t *m_dev, float prediction, const float *seeds, pq_contact *out,
+                           uint32_t *iters, void *workspace, hipStream_t stream);
 // Per-(device, stream) workspace owned by the library, grown on demand.
 hipError_t stream_workspace(hipStream_t stream, size_t bytes, void **ptr);
 
