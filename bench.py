@@ -177,6 +177,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--skip-contact", action="store_true", help="measure only the ray-cast line")
+    ap.add_argument("--contact-only", action="store_true", help="profiling aid: run only the contact leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -193,6 +194,9 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.contact_only:
+        print(json.dumps({"contact": contact_leg(args, log)}), flush=True)
+        return 0
     v, f = scenes.noisy_icosphere(6, seed=1)
     rays = scenes.rays_into_unit_ball(1 << 20, seed=2)
     n = rays.shape[0]

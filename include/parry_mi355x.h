@@ -234,6 +234,16 @@ typedef struct pq_contact {
     uint32_t has_contact;
 } pq_contact;
 
+/* Kernel mapping of the contact path (no counterpart in parry; results are
+ * identical for every setting): lanes that run one pair's EPA together, 64
+ * (one wave per pair), 16 or 8.  Process-wide. */
+typedef struct pq_contact_tuning {
+    uint32_t epa_lanes_per_pair;
+    uint32_t reserved[3]; /* must be 0 */
+} pq_contact_tuning;
+int pq_contact_set_tuning(const pq_contact_tuning *tuning);
+int pq_contact_get_tuning(pq_contact_tuning *tuning);
+
 /* Device workspace bytes pq_contact_pairs needs for `num_pairs` pairs. */
 size_t pq_contact_workspace_bytes(uint64_t num_pairs);
 

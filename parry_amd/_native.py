@@ -55,6 +55,8 @@ SIGNATURES = {
     "pq_bvh_update_and_refit": (_int, [_vp, _vp, _vp]),
     "pq_bvh_self_pairs": (_int, [_vp, _vp, _u64, _vp, _vp]),
     "pq_contact_workspace_bytes": (_c.c_size_t, [_u64]),
+    "pq_contact_set_tuning": (_int, [_vp]),
+    "pq_contact_get_tuning": (_int, [_vp]),
     "pq_contact_pairs": (_int, [_vp, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp]),
     "pq_contact_pairs_device_count": (_int, [_vp, _vp, _vp, _u64, _vp, _f32, _vp, _vp, _vp, _vp, _vp]),
 }
@@ -71,6 +73,16 @@ class RayTuning(ctypes.Structure):
     """pq_ray_tuning (include/parry_mi355x.h)."""
 
     _fields_ = [("mapping", _u32), ("chunk", _u32), ("blocks_per_cu", _u32), ("reserved", _u32)]
+
+
+class ContactTuning(ctypes.Structure):
+    """pq_contact_tuning (include/parry_mi355x.h)."""
+
+    _fields_ = [("epa_lanes_per_pair", _u32), ("reserved", _u32 * 3)]
+
+
+def set_contact_tuning(epa_lanes_per_pair: int) -> None:
+    check(lib().pq_contact_set_tuning(ctypes.byref(ContactTuning(epa_lanes_per_pair))))
 
 
 class NativeError(RuntimeError):

@@ -188,6 +188,21 @@ int pq_bvh_self_pairs(pq_bvh *b, uint32_t *pairs, uint64_t capacity, uint64_t *c
 
 size_t pq_contact_workspace_bytes(uint64_t num_pairs) { return contact_workspace_bytes(num_pairs); }
 
+int pq_contact_set_tuning(const pq_contact_tuning *t) {
+    if (!t) return fail(PQ_ERR_INVALID, "pq_contact_set_tuning: null tuning");
+    const uint32_t g = t->epa_lanes_per_pair;
+    if ((g != 64 && g != 16 && g != 8) || t->reserved[0] || t->reserved[1] || t->reserved[2])
+        return fail(PQ_ERR_INVALID, "pq_contact_set_tuning: epa_lanes_per_pair must be 64, 16 or 8");
+    g_epa_lanes_per_pair = (int)g;
+    return PQ_OK;
+}
+
+int pq_contact_get_tuning(pq_contact_tuning *t) {
+    if (!t) return fail(PQ_ERR_INVALID, "pq_contact_get_tuning: null tuning");
+    *t = pq_contact_tuning{(uint32_t)g_epa_lanes_per_pair, {0, 0, 0}};
+    return PQ_OK;
+}
+
 int pq_contact_pairs(const pq_shapes *s, const pq_isometry *poses, const uint32_t *pairs, uint64_t m, float prediction,
                      const float *init_dirs, pq_contact *out, uint32_t *iters, void *workspace, void *stream) {
     if (!s) return fail(PQ_ERR_INVALID, "pq_contact_pairs: null shapes");

@@ -173,42 +173,59 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Stage 2/3: EPA, one wave per pair
+// Stage 2/3: EPA, G lanes per pair
 // ---------------------------------------------------------------------------
-struct EpaFace {
-    uint16_t pts[3];
-    uint16_t adj[3];
+// The polytope lives in LDS in compact form: CSO points, faces (vertex and
+// neighbour indices, deleted flag, unit normal), the face heap and the
+// silhouette stacks.  The shape-space points orig1/orig2 of each CSO vertex
+// are only needed for the final witness points, so they go to a per-group
+// global buffer; a face's barycentric coordinates are recomputed (the same
+// projection as Face::new) when its witness points are built.
+// ConvexPolyhedra with at most this many vertices are staged in LDS for the
+// EPA support queries; larger ones are read from global memory.
+constexpr int kStagedVerts = 64;
+
+template <class I> struct EpaFaceT {
+    I pts[3];
+    I adj[3];
+    uint8_t deleted;
     float n[3];
-    float bc[3];
-    uint32_t deleted;
 };
 
-template <int V, int F, int H, int S, int K> struct EpaState {
+template <int V, int F, int H, int S, int K, class I> struct EpaState {
     static constexpr int kV = V, kF = F, kH = H, kS = S, kK = K;
-    float vert[V][9];
-    EpaFace face[F];
-    uint32_t heap_id[H];
+    using Idx = I;
+    using Face = EpaFaceT<I>;
+    float vp[V][3];
+    Face face[F];
+    I heap_id[H];
     float heap_nd[H];
     uint32_t sil[S];  // face | opp << 16
     uint32_t stk[K];  // face | opp << 16
+    float sv[2][kStagedVerts][3];  // both ConvexPolyhedra's vertices (local frames)
 };
-// LDS variant: room for parry's 100-iteration cap in vertices; faces, heap and
-// stacks sized for the common case (overflow re-runs in global memory).
-using EpaSmall = EpaState<112, 224, 224, 64, 192>;
-using EpaLarge = EpaState<112, 8192, 8192, 2048, 8192>;
+// LDS variant (~4.8 KB per pair) sized for the common case; a pair that
+// outgrows it re-runs with the large buffers in global memory, which have room
+// for parry's 100-iteration cap.
+using EpaSmall = EpaState<32, 96, 96, 32, 96, uint8_t>;
+using EpaLarge = EpaState<112, 8192, 8192, 2048, 8192, uint16_t>;
+static_assert(sizeof(EpaFaceT<uint8_t>) == 20, "compact LDS face record");
+constexpr uint32_t kOrigFloats = 6;  // orig1, orig2 per CSO vertex
 
-__device__ __forceinline__ f3 wave_support(const ShapeRef &s, f3 d) {
+// ConvexPolyhedron support point computed by a group of G consecutive lanes:
+// parry takes the first strict maximum of p.d over the vertex list; if vertex
+// 0's product is NaN nothing ever beats it, otherwise the result is the lowest
+// index among the largest non-NaN products, which is what the group reduction
+// (largest value, then lowest index) returns.
+template <int G> __device__ __forceinline__ f3 group_support(const ShapeRef &s, f3 d) {
     if (s.type != PQ_SHAPE_CONVEX_POLYHEDRON) return local_support(s, d);
-    // First strict maximum of p.d over the vertex list, as a wave reduction:
-    // if vertex 0's product is NaN nothing ever beats it; otherwise the result
-    // is the lowest index among the largest non-NaN products.
     const float4 p0 = s.v[0];
     const float d0 = dot3(f3{p0.x, p0.y, p0.z}, d);
     if (d0 != d0) return f3{p0.x, p0.y, p0.z};
-    const uint32_t lane = __lane_id();
+    const uint32_t sub = __lane_id() % G;
     float best = -INFINITY;
     uint32_t bi = 0xffffffffu;
-    for (uint32_t k = lane; k < s.n; k += 64) {
+    for (uint32_t k = sub; k < s.n; k += G) {
         const float4 p = s.v[k];
         const float dd = dot3(f3{p.x, p.y, p.z}, d);
         if (dd == dd && (bi == 0xffffffffu || dd > best)) {
@@ -216,9 +233,9 @@ __device__ __forceinline__ f3 wave_support(const ShapeRef &s, f3 d) {
             bi = k;
         }
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-        const float ob = __shfl_xor(best, off);
-        const uint32_t oi = __shfl_xor(bi, off);
+    for (int off = G / 2; off >= 1; off >>= 1) {
+        const float ob = __shfl_xor(best, off, G);
+        const uint32_t oi = __shfl_xor(bi, off, G);
         if (ob > best || (ob == best && oi < bi)) {
             best = ob;
             bi = oi;
@@ -228,22 +245,74 @@ __device__ __forceinline__ f3 wave_support(const ShapeRef &s, f3 d) {
     return f3{p.x, p.y, p.z};
 }
 
-__device__ __forceinline__ Cso wave_cso(const iso &pos12, const ShapeRef &g1, const ShapeRef &g2, f3 dir) {
-    const f3 a = wave_support(g1, dir);
-    const f3 b = iso_point(pos12, wave_support(g2, qrot_inv(pos12.q, neg3(dir))));
+// The same reduction over vertices staged in LDS (sv = nullptr: not staged).
+template <int G> __device__ __forceinline__ f3 group_support_staged(const ShapeRef &s, const float (*sv)[3], f3 d) {
+    if (s.type != PQ_SHAPE_CONVEX_POLYHEDRON || sv == nullptr) return group_support<G>(s, d);
+    const f3 p0 = f3{sv[0][0], sv[0][1], sv[0][2]};
+    const float d0 = dot3(p0, d);
+    if (d0 != d0) return p0;
+    const uint32_t sub = __lane_id() % G;
+    float best = -INFINITY;
+    uint32_t bi = 0xffffffffu;
+    for (uint32_t k = sub; k < s.n; k += G) {
+        const float dd = dot3(f3{sv[k][0], sv[k][1], sv[k][2]}, d);
+        if (dd == dd && (bi == 0xffffffffu || dd > best)) {
+            best = dd;
+            bi = k;
+        }
+    }
+    for (int off = G / 2; off >= 1; off >>= 1) {
+        const float ob = __shfl_xor(best, off, G);
+        const uint32_t oi = __shfl_xor(bi, off, G);
+        if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+        }
+    }
+    return f3{sv[bi][0], sv[bi][1], sv[bi][2]};
+}
+
+struct Staged {
+    const float (*s1)[3];
+    const float (*s2)[3];
+};
+
+template <int G>
+__device__ __forceinline__ Cso group_cso(const iso &pos12, const ShapeRef &g1, const ShapeRef &g2, const Staged &st,
+                                         f3 dir) {
+    const f3 a = group_support_staged<G>(g1, st.s1, dir);
+    const f3 b = iso_point(pos12, group_support_staged<G>(g2, st.s2, qrot_inv(pos12.q, neg3(dir))));
     return Cso{sub3(a, b), a, b};
 }
 
-template <class W> __device__ __forceinline__ f3 vpt(const W *w, uint32_t i) {
-    return f3{w->vert[i][0], w->vert[i][1], w->vert[i][2]};
-}
-template <class W> __device__ __forceinline__ void vset(W *w, uint32_t i, const Cso &c) {
-    w->vert[i][0] = c.p.x; w->vert[i][1] = c.p.y; w->vert[i][2] = c.p.z;
-    w->vert[i][3] = c.o1.x; w->vert[i][4] = c.o1.y; w->vert[i][5] = c.o1.z;
-    w->vert[i][6] = c.o2.x; w->vert[i][7] = c.o2.y; w->vert[i][8] = c.o2.z;
+// Copies a ConvexPolyhedron's vertices into LDS (group-cooperative); returns
+// the staged array or nullptr if the shape is not staged.
+template <int G, class W> __device__ const float (*stage_shape(W *w, int slot, const ShapeRef &s))[3] {
+    if (s.type != PQ_SHAPE_CONVEX_POLYHEDRON || s.n > (uint32_t)kStagedVerts) return nullptr;
+    const uint32_t sub = __lane_id() % G;
+    for (uint32_t k = sub; k < s.n; k += G) {
+        const float4 p = s.v[k];
+        w->sv[slot][k][0] = p.x;
+        w->sv[slot][k][1] = p.y;
+        w->sv[slot][k][2] = p.z;
+    }
+    return w->sv[slot];
 }
 
-// BinaryHeap<FaceId> (max-heap on neg_dist; equal keys compare Equal).
+template <class W> __device__ __forceinline__ f3 vpt(const W *w, uint32_t i) {
+    return f3{w->vp[i][0], w->vp[i][1], w->vp[i][2]};
+}
+template <class W> __device__ __forceinline__ void vset(W *w, float *orig, uint32_t i, const Cso &c) {
+    w->vp[i][0] = c.p.x;
+    w->vp[i][1] = c.p.y;
+    w->vp[i][2] = c.p.z;
+    float *o = orig + kOrigFloats * i;
+    o[0] = c.o1.x; o[1] = c.o1.y; o[2] = c.o1.z;
+    o[3] = c.o2.x; o[4] = c.o2.y; o[5] = c.o2.z;
+}
+
+// BinaryHeap<FaceId> (max-heap on neg_dist; equal keys compare Equal), with
+// std's sift_up / sift_down_to_bottom so equal keys pop in parry's order.
 template <class W> struct Heap {
     W *w;
     uint32_t len;
@@ -258,13 +327,13 @@ template <class W> struct Heap {
             w->heap_nd[pos] = w->heap_nd[parent];
             pos = parent;
         }
-        w->heap_id[pos] = eid;
+        w->heap_id[pos] = (typename W::Idx)eid;
         w->heap_nd[pos] = end;
         return pos;
     }
     __device__ bool push(uint32_t id, float nd) {
         if (len >= (uint32_t)W::kH) return false;
-        w->heap_id[len] = id;
+        w->heap_id[len] = (typename W::Idx)id;
         w->heap_nd[len] = nd;
         ++len;
         sift_up(0, len - 1);
@@ -287,7 +356,7 @@ template <class W> struct Heap {
             w->heap_nd[pos] = w->heap_nd[child];
             pos = child;
         }
-        w->heap_id[pos] = eid;
+        w->heap_id[pos] = (typename W::Idx)eid;
         w->heap_nd[pos] = end_nd;
         sift_up(start, pos);
     }
@@ -299,7 +368,7 @@ template <class W> struct Heap {
         if (len > 0) {
             const uint32_t t_id = w->heap_id[0];
             const float t_nd = w->heap_nd[0];
-            w->heap_id[0] = iid;
+            w->heap_id[0] = (typename W::Idx)iid;
             w->heap_nd[0] = ind;
             iid = t_id;
             ind = t_nd;
@@ -313,36 +382,46 @@ template <class W> struct Heap {
 
 enum : uint32_t { EPA_OK = 0, EPA_NONE = 1, EPA_OVERFLOW = 2 };
 
-// Face::new (epa3.rs:96-118): fills face f; returns proj_inside.
-template <class W>
-__device__ bool face_new(W *w, uint32_t f, uint32_t a, uint32_t b, uint32_t c, uint32_t j0, uint32_t j1, uint32_t j2) {
-    EpaFace &F = w->face[f];
-    F.pts[0] = (uint16_t)a; F.pts[1] = (uint16_t)b; F.pts[2] = (uint16_t)c;
-    F.adj[0] = (uint16_t)j0; F.adj[1] = (uint16_t)j1; F.adj[2] = (uint16_t)j2;
-    F.deleted = 0;
-    const f3 pa = vpt(w, a), pb = vpt(w, b), pc = vpt(w, c);
+// Barycentric coordinates of Face::new's projection (epa3.rs:96-118,
+// TrianglePointLocation::barycentric_coordinates); returns proj_inside.
+__device__ __forceinline__ bool face_bcoords(f3 pa, f3 pb, f3 pc, float bc[3]) {
     Loc l;
     bool inside;
     const f3 proj = tri_origin(pa, pb, pc, l, inside);
-    float bc[3] = {0.0f, 0.0f, 0.0f};
-    bool in;
+    bc[0] = bc[1] = bc[2] = 0.0f;
+    const float e = kEps * 100.0f;
     if (l.kind == LOC_VERTEX) {
-        bc[l.id] = 1.0f;
-        const float e = kEps * 100.0f;
-        in = inside || norm_sq3(proj) < e * e;
-    } else if (l.kind == LOC_EDGE) {
-        const uint32_t i0 = l.id == 1 ? 1u : 0u, i1 = l.id == 0 ? 1u : 2u;
-        bc[i0] = l.b0;
-        bc[i1] = l.b1;
-        const float e = kEps * 100.0f;
-        in = inside || norm_sq3(proj) < e * e;
-    } else if (l.kind == LOC_FACE) {
-        bc[0] = l.b0; bc[1] = l.b1; bc[2] = l.b2;
-        in = true;
-    } else {
-        in = false;
+        bc[0] = l.id == 0 ? 1.0f : 0.0f;
+        bc[1] = l.id == 1 ? 1.0f : 0.0f;
+        bc[2] = l.id == 2 ? 1.0f : 0.0f;
+        return inside || norm_sq3(proj) < e * e;
     }
-    F.bc[0] = bc[0]; F.bc[1] = bc[1]; F.bc[2] = bc[2];
+    if (l.kind == LOC_EDGE) {  // edges (0,1), (1,2), (0,2)
+        bc[0] = l.id == 1 ? 0.0f : l.b0;
+        bc[1] = l.id == 0 ? l.b1 : (l.id == 1 ? l.b0 : 0.0f);
+        bc[2] = l.id == 0 ? 0.0f : l.b1;
+        return inside || norm_sq3(proj) < e * e;
+    }
+    if (l.kind == LOC_FACE) {
+        bc[0] = l.b0;
+        bc[1] = l.b1;
+        bc[2] = l.b2;
+        return true;
+    }
+    return false;
+}
+
+// Face::new: fills face f; returns proj_inside.
+template <class W>
+__device__ bool face_new(W *w, uint32_t f, uint32_t a, uint32_t b, uint32_t c, uint32_t j0, uint32_t j1, uint32_t j2) {
+    using I = typename W::Idx;
+    typename W::Face &F = w->face[f];
+    F.pts[0] = (I)a; F.pts[1] = (I)b; F.pts[2] = (I)c;
+    F.adj[0] = (I)j0; F.adj[1] = (I)j1; F.adj[2] = (I)j2;
+    F.deleted = 0;
+    const f3 pa = vpt(w, a), pb = vpt(w, b), pc = vpt(w, c);
+    float bc[3];
+    const bool in = face_bcoords(pa, pb, pc, bc);
     // ccw_face_normal (utils/ccw_face_normal.rs:19-27), zero if degenerate
     const f3 r = cross3(sub3(pb, pa), sub3(pc, pa));
     const float sq = norm_sq3(r);
@@ -356,16 +435,19 @@ template <class W> __device__ __forceinline__ f3 fnormal(const W *w, uint32_t f)
     return f3{w->face[f].n[0], w->face[f].n[1], w->face[f].n[2]};
 }
 
-template <class W> __device__ void face_points(const W *w, uint32_t f, f3 &p1, f3 &p2) {
-    const EpaFace &F = w->face[f];
+// Face::closest_points (epa3.rs:120-130).
+template <class W> __device__ void face_points(const W *w, const float *orig, uint32_t f, f3 &p1, f3 &p2) {
+    const typename W::Face &F = w->face[f];
+    float bc[3];
+    face_bcoords(vpt(w, F.pts[0]), vpt(w, F.pts[1]), vpt(w, F.pts[2]), bc);
     f3 o1[3], o2[3];
     for (int q = 0; q < 3; ++q) {
-        const uint32_t v = F.pts[q];
-        o1[q] = f3{w->vert[v][3], w->vert[v][4], w->vert[v][5]};
-        o2[q] = f3{w->vert[v][6], w->vert[v][7], w->vert[v][8]};
+        const float *o = orig + kOrigFloats * F.pts[q];
+        o1[q] = f3{o[0], o[1], o[2]};
+        o2[q] = f3{o[3], o[4], o[5]};
     }
-    p1 = add3(add3(mul3(o1[0], F.bc[0]), mul3(o1[1], F.bc[1])), mul3(o1[2], F.bc[2]));
-    p2 = add3(add3(mul3(o2[0], F.bc[0]), mul3(o2[1], F.bc[1])), mul3(o2[2], F.bc[2]));
+    p1 = add3(add3(mul3(o1[0], bc[0]), mul3(o1[1], bc[1])), mul3(o1[2], bc[2]));
+    p2 = add3(add3(mul3(o2[0], bc[0]), mul3(o2[1], bc[1])), mul3(o2[2], bc[2]));
 }
 
 template <class W> __device__ __forceinline__ uint32_t next_ccw(const W *w, uint32_t f, uint32_t id) {
@@ -376,7 +458,7 @@ template <class W> __device__ __forceinline__ uint32_t next_ccw(const W *w, uint
 
 // Face::can_be_seen_by (epa3.rs:147-163).
 template <class W> __device__ bool seen_by(const W *w, uint32_t f, uint32_t point, uint32_t opp) {
-    const EpaFace &F = w->face[f];
+    const typename W::Face &F = w->face[f];
     const f3 p0 = vpt(w, F.pts[opp]);
     const f3 p1 = vpt(w, F.pts[(opp + 1) % 3]);
     const f3 p2 = vpt(w, F.pts[(opp + 2) % 3]);
@@ -386,31 +468,45 @@ template <class W> __device__ bool seen_by(const W *w, uint32_t f, uint32_t poin
     return norm_sq3(cross3(sub3(p2, p1), sub3(pt, p1))) <= E * E;  // Triangle::is_affinely_dependent
 }
 
-// EPA::closest_points (epa3.rs:428-651) for one pair, executed by a whole wave.
-template <class W>
-__device__ uint32_t epa_run(W *w, const iso &pos12, const ShapeRef &g1, const ShapeRef &g2, const EpaItem &it,
-                            f3 &p1, f3 &p2, f3 &n, uint32_t &epa_iters) {
+// EPA::closest_points (epa3.rs:428-651) for one pair, executed by a group of G lanes.
+template <int G, class W>
+__device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef &g1, const ShapeRef &g2,
+                            const EpaItem *it, f3 &p1, f3 &p2, f3 &n, uint32_t &epa_iters) {
     const float eps_tol = kEps * 100.0f;
+    Staged st{stage_shape<G>(w, 0, g1), stage_shape<G>(w, 1, g2)};
+    // The group's lanes wrote the staged vertices; make them visible to the
+    // whole wave before any lane reads another lane's entries.
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     uint32_t nv = 0, nf = 0;
     Heap<W> heap{w, 0};
     epa_iters = 0;
-    for (uint32_t i = 0; i <= it.dim; ++i) {
-        for (int q = 0; q < 9; ++q) w->vert[i][q] = it.v[i][q];
+    const uint32_t dim = it->dim;
+    for (uint32_t i = 0; i <= dim; ++i) {
+        w->vp[i][0] = it->v[i][0];
+        w->vp[i][1] = it->v[i][1];
+        w->vp[i][2] = it->v[i][2];
+        for (int q = 0; q < 6; ++q) orig[kOrigFloats * i + q] = it->v[i][3 + q];
     }
-    nv = it.dim + 1;
-    if (it.dim == 0) {
+    nv = dim + 1;
+    if (dim == 0) {
         p1 = p2 = f3{0.0f, 0.0f, 0.0f};
         n = f3{0.0f, 1.0f, 0.0f};
         return EPA_OK;
-    } else if (it.dim == 3) {
+    } else if (dim == 3) {
         const f3 dp1 = sub3(vpt(w, 1), vpt(w, 0));
         const f3 dp2 = sub3(vpt(w, 2), vpt(w, 0));
         const f3 dp3 = sub3(vpt(w, 3), vpt(w, 0));
         if (dot3(cross3(dp1, dp2), dp3) > 0.0f) {
-            for (int q = 0; q < 9; ++q) {
-                const float t = w->vert[1][q];
-                w->vert[1][q] = w->vert[2][q];
-                w->vert[2][q] = t;
+            for (int q = 0; q < 3; ++q) {
+                const float t = w->vp[1][q];
+                w->vp[1][q] = w->vp[2][q];
+                w->vp[2][q] = t;
+            }
+            for (int q = 0; q < 6; ++q) {
+                const float t = orig[kOrigFloats * 1 + q];
+                orig[kOrigFloats * 1 + q] = orig[kOrigFloats * 2 + q];
+                orig[kOrigFloats * 2 + q] = t;
             }
         }
         const bool in0 = face_new(w, 0, 0, 1, 2, 3, 1, 2);
@@ -419,6 +515,7 @@ __device__ uint32_t epa_run(W *w, const iso &pos12, const ShapeRef &g1, const Sh
         const bool in3 = face_new(w, 3, 0, 3, 1, 2, 1, 0);
         nf = 4;
         const bool ins[4] = {in0, in1, in2, in3};
+#pragma unroll
         for (uint32_t f = 0; f < 4; ++f) {
             if (ins[f]) {
                 const float dist = dot3(fnormal(w, f), vpt(w, f));
@@ -428,12 +525,12 @@ __device__ uint32_t epa_run(W *w, const iso &pos12, const ShapeRef &g1, const Sh
         }
         if (!(in0 || in1 || in2 || in3)) return EPA_NONE;
     } else {
-        if (it.dim == 1) {
+        if (dim == 1) {
             // orthonormal_subspace_basis([dpt]): first vector a x dpt
             const f3 dpt = sub3(vpt(w, 1), vpt(w, 0));
             f3 a = fabsf(dpt.x) > fabsf(dpt.y) ? f3{dpt.z, 0.0f, -dpt.x} : f3{0.0f, -dpt.z, dpt.y};
             a = div3(a, sqrtf(norm_sq3(a)));
-            vset(w, 2, wave_cso(pos12, g1, g2, cross3(a, dpt)));
+            vset(w, orig, 2, group_cso<G>(pos12, g1, g2, st, cross3(a, dpt)));
             nv = 3;
         }
         face_new(w, 0, 0, 1, 2, 1, 1, 1);
@@ -451,12 +548,12 @@ __device__ uint32_t epa_run(W *w, const iso &pos12, const ShapeRef &g1, const Sh
     float fnd;
     while (heap.pop(fid, fnd)) {
         if (w->face[fid].deleted) continue;
-        const EpaFace face = w->face[fid];
+        const typename W::Face face = w->face[fid];
         const f3 fn = f3{face.n[0], face.n[1], face.n[2]};
-        const Cso c = wave_cso(pos12, g1, g2, fn);
+        const Cso c = group_cso<G>(pos12, g1, g2, st, fn);
         if (nv >= (uint32_t)W::kV) return EPA_OVERFLOW;
         const uint32_t sp = nv;
-        vset(w, sp, c);
+        vset(w, orig, sp, c);
         ++nv;
         const float cand = dot3(c.p, fn);
         if (cand < max_dist) {
@@ -465,7 +562,7 @@ __device__ uint32_t epa_run(W *w, const iso &pos12, const ShapeRef &g1, const Sh
         }
         const float curr_dist = -fnd;
         if (max_dist - curr_dist < eps_tol || (fabsf(curr_dist - old_dist) < kEps && cand < max_dist)) {
-            face_points(w, best_id, p1, p2);
+            face_points(w, orig, best_id, p1, p2);
             n = fnormal(w, best_id);
             epa_iters = niter + 1;
             return EPA_OK;
@@ -507,12 +604,12 @@ __device__ uint32_t epa_run(W *w, const iso &pos12, const ShapeRef &g1, const Sh
             const uint32_t pt1 = w->face[ef].pts[(opp + 2) % 3];
             const uint32_t pt2 = w->face[ef].pts[(opp + 1) % 3];
             const bool in = face_new(w, new_id, pt1, pt2, sp, ef, new_id + 1, new_id - 1);
-            w->face[ef].adj[(opp + 1) % 3] = (uint16_t)new_id;
+            w->face[ef].adj[(opp + 1) % 3] = (typename W::Idx)new_id;
             ++nf;
             if (in) {
                 const float dist = dot3(fnormal(w, new_id), vpt(w, w->face[new_id].pts[0]));
                 if (dist < curr_dist) {
-                    face_points(w, fid, p1, p2);  // `face` is unchanged: its points and bcoords
+                    face_points(w, orig, fid, p1, p2);  // the popped face: its points are unchanged
                     n = fn;
                     epa_iters = niter + 1;
                     return EPA_OK;
@@ -522,30 +619,30 @@ __device__ uint32_t epa_run(W *w, const iso &pos12, const ShapeRef &g1, const Sh
             }
         }
         if (first_new == nf) return EPA_NONE;
-        w->face[first_new].adj[2] = (uint16_t)(nf - 1);
-        w->face[nf - 1].adj[1] = (uint16_t)first_new;
+        w->face[first_new].adj[2] = (typename W::Idx)(nf - 1);
+        w->face[nf - 1].adj[1] = (typename W::Idx)first_new;
         ++niter;
         if (niter > 100) break;
     }
     epa_iters = niter + 1;
-    face_points(w, best_id, p1, p2);
+    face_points(w, orig, best_id, p1, p2);
     n = fnormal(w, best_id);
     return EPA_OK;
 }
 
-template <class W>
-__device__ void epa_item(W *w, const ShapeRecord *rec, const float4 *verts, const float *poses,
-                         const uint32_t *pairs, const EpaItem &it, DevContact *out, uint32_t *iters,
+template <int G, class W>
+__device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4 *verts, const float *poses,
+                         const uint32_t *pairs, const EpaItem *it, DevContact *out, uint32_t *iters,
                          uint32_t *overflow_list, uint32_t *counters, uint32_t slot) {
-    const uint32_t k = it.pair;
+    const uint32_t k = it->pair;
     const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
     const iso pos1 = load_iso(poses + 7 * (size_t)i), pos2 = load_iso(poses + 7 * (size_t)j);
     const iso pos12 = iso_inv_mul(pos1, pos2);
     const ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
     f3 p1, p2, n;
     uint32_t ei = 0;
-    const uint32_t st = epa_run(w, pos12, g1, g2, it, p1, p2, n, ei);
-    if (__lane_id() != 0) return;
+    const uint32_t st = epa_run<G>(w, orig, pos12, g1, g2, it, p1, p2, n, ei);
+    if (__lane_id() % G != 0) return;
     if (st == EPA_OVERFLOW) {
         if (overflow_list) {
             overflow_list[atomicAdd(&counters[1], 1u)] = slot;
@@ -564,20 +661,26 @@ __device__ void epa_item(W *w, const ShapeRecord *rec, const float4 *verts, cons
     if (iters) iters[2 * k + 1] = ei;
 }
 
+// Stage 2: G lanes per pair, 64/G pairs per wave, one LDS polytope per group.
+template <int G>
 __global__ __launch_bounds__(64) void epa_lds_kernel(const ShapeRecord *__restrict__ rec,
                                                      const float4 *__restrict__ verts, const float *__restrict__ poses,
                                                      const uint32_t *__restrict__ pairs,
                                                      const EpaItem *__restrict__ queue, DevContact *__restrict__ out,
                                                      uint32_t *__restrict__ iters, uint32_t *overflow_list,
-                                                     uint32_t *counters) {
-    __shared__ EpaSmall w;
+                                                     uint32_t *counters, float *orig_scratch) {
+    constexpr int kGroups = 64 / G;
+    __shared__ EpaSmall w[kGroups];
+    const uint32_t group = threadIdx.x / G;
+    const uint32_t gid = blockIdx.x * kGroups + group;
+    float *orig = orig_scratch + (size_t)gid * EpaSmall::kV * kOrigFloats;
     const uint32_t count = __hip_atomic_load(&counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
-        const EpaItem it = queue[q];
-        epa_item(&w, rec, verts, poses, pairs, it, out, iters, overflow_list, counters, q);
-    }
+    const uint32_t stride = gridDim.x * kGroups;
+    for (uint32_t q = gid; q < count; q += stride)
+        epa_item<G>(&w[group], orig, rec, verts, poses, pairs, queue + q, out, iters, overflow_list, counters, q);
 }
 
+// Stage 3: one wave per overflowed pair, polytope in global memory.
 __global__ __launch_bounds__(64) void epa_global_kernel(const ShapeRecord *__restrict__ rec,
                                                         const float4 *__restrict__ verts,
                                                         const float *__restrict__ poses,
@@ -585,22 +688,47 @@ __global__ __launch_bounds__(64) void epa_global_kernel(const ShapeRecord *__res
                                                         const EpaItem *__restrict__ queue,
                                                         DevContact *__restrict__ out, uint32_t *__restrict__ iters,
                                                         const uint32_t *overflow_list, uint32_t *counters,
-                                                        EpaLarge *scratch) {
+                                                        EpaLarge *scratch, float *orig_scratch) {
     EpaLarge *w = scratch + blockIdx.x;
+    float *orig = orig_scratch + (size_t)blockIdx.x * EpaLarge::kV * kOrigFloats;
     const uint32_t count = __hip_atomic_load(&counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
-        const EpaItem it = queue[overflow_list[q]];
-        epa_item(w, rec, verts, poses, pairs, it, out, iters, nullptr, counters, q);
-    }
+    for (uint32_t q = blockIdx.x; q < count; q += gridDim.x)
+        epa_item<64>(w, orig, rec, verts, poses, pairs, queue + overflow_list[q], out, iters, nullptr, counters, q);
+}
+
+template <int G> hipError_t launch_epa_lds(const DeviceShapes &s, const float *poses, const uint32_t *pairs,
+                                           uint64_t m, const EpaItem *queue, DevContact *out, uint32_t *iters,
+                                           uint32_t *overflow, uint32_t *counters, float *orig, uint64_t max_groups,
+                                           hipStream_t stream) {
+    hipError_t e;
+    int dev = 0, cus = 0, per_cu = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, epa_lds_kernel<G>, 64, 0)) != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    constexpr uint64_t kGroups = 64 / G;
+    uint64_t waves = (uint64_t)cus * (uint64_t)per_cu;
+    const uint64_t need = (m + kGroups - 1) / kGroups;
+    if (waves > need) waves = need;
+    if (waves * kGroups > max_groups) waves = max_groups / kGroups;
+    hipLaunchKernelGGL(epa_lds_kernel<G>, dim3((uint32_t)waves), dim3(64), 0, stream, s.rec, s.verts, poses, pairs,
+                       queue, out, iters, overflow, counters, orig);
+    return hipGetLastError();
 }
 
 }  // namespace
+
+// Resident EPA groups the workspace provides orig1/orig2 buffers for.
+constexpr uint64_t kEpaMaxGroups = 256ull * 16 * 8;
+
+int g_epa_lanes_per_pair = 16;
 
 size_t contact_workspace_bytes(uint64_t num_pairs) {
     const size_t align = 256;
     auto up = [&](size_t b) { return (b + align - 1) / align * align; };
     return up(16) + up(num_pairs * sizeof(EpaItem)) + up(num_pairs * sizeof(uint32_t)) +
-           up(kEpaGlobalWaves * sizeof(EpaLarge));
+           up(kEpaGlobalWaves * sizeof(EpaLarge)) + up(kEpaGlobalWaves * EpaLarge::kV * kOrigFloats * sizeof(float)) +
+           up(kEpaMaxGroups * EpaSmall::kV * kOrigFloats * sizeof(float));
 }
 
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
@@ -611,25 +739,39 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
     auto up = [&](size_t b) { return (b + align - 1) / align * align; };
     char *ws = (char *)workspace;
     uint32_t *counters = (uint32_t *)ws;
-    EpaItem *queue = (EpaItem *)(ws + up(16));
-    uint32_t *overflow = (uint32_t *)(ws + up(16) + up(m * sizeof(EpaItem)));
-    EpaLarge *scratch = (EpaLarge *)(ws + up(16) + up(m * sizeof(EpaItem)) + up(m * sizeof(uint32_t)));
+    ws += up(16);
+    EpaItem *queue = (EpaItem *)ws;
+    ws += up(m * sizeof(EpaItem));
+    uint32_t *overflow = (uint32_t *)ws;
+    ws += up(m * sizeof(uint32_t));
+    EpaLarge *scratch = (EpaLarge *)ws;
+    ws += up(kEpaGlobalWaves * sizeof(EpaLarge));
+    float *orig_large = (float *)ws;
+    ws += up(kEpaGlobalWaves * EpaLarge::kV * kOrigFloats * sizeof(float));
+    float *orig_small = (float *)ws;
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters);
     hipLaunchKernelGGL(contact_gjk_kernel, dim3((uint32_t)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
                        s.rec, s.verts, poses, pairs, m, (const unsigned long long *)m_dev, prediction, seeds,
                        (DevContact *)out, iters, queue, counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    int dev = 0, cus = 0;
-    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    uint64_t waves = (uint64_t)cus * 8;
-    if (waves > m) waves = m;
-    hipLaunchKernelGGL(epa_lds_kernel, dim3((uint32_t)waves), dim3(64), 0, stream, s.rec, s.verts, poses, pairs,
-                       queue, (DevContact *)out, iters, overflow, counters);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    switch (g_epa_lanes_per_pair) {
+        case 8:
+            e = launch_epa_lds<8>(s, poses, pairs, m, queue, (DevContact *)out, iters, overflow, counters, orig_small,
+                                  kEpaMaxGroups, stream);
+            break;
+        case 16:
+            e = launch_epa_lds<16>(s, poses, pairs, m, queue, (DevContact *)out, iters, overflow, counters,
+                                   orig_small, kEpaMaxGroups, stream);
+            break;
+        default:
+            e = launch_epa_lds<64>(s, poses, pairs, m, queue, (DevContact *)out, iters, overflow, counters,
+                                   orig_small, kEpaMaxGroups, stream);
+            break;
+    }
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(epa_global_kernel, dim3(kEpaGlobalWaves), dim3(64), 0, stream, s.rec, s.verts, poses, pairs,
-                       queue, (DevContact *)out, iters, overflow, counters, scratch);
+                       queue, (DevContact *)out, iters, overflow, counters, scratch, orig_large);
     return hipGetLastError();
 }
 

@@ -89,7 +89,7 @@ __device__ __forceinline__ Cso cso_from_shapes(const iso &pos12, const ShapeRef 
 
 // relative_eq!(proj, origin) with approx's default tolerances.
 __device__ __forceinline__ bool near_origin(f3 p) {
-    return (fabsf(p.x) <= kEps) & (fabsf(p.y) <= kEps) & (fabsf(p.z) <= kEps);
+    return ((unsigned)(fabsf(p.x) <= kEps) & (unsigned)(fabsf(p.y) <= kEps) & (unsigned)(fabsf(p.z) <= kEps)) != 0u;
 }
 
 enum : uint32_t { LOC_VERTEX = 0, LOC_EDGE = 1, LOC_FACE = 2, LOC_SOLID = 3 };

@@ -128,6 +128,8 @@ size_t contact_workspace_bytes(uint64_t num_pairs);
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
                            const uint64_t *m_dev, float prediction, const float *seeds, pq_contact *out,
                            uint32_t *iters, void *workspace, hipStream_t stream);
+// Lanes that run one pair's EPA together (64, 16 or 8), set by pq_contact_set_tuning.
+extern int g_epa_lanes_per_pair;
 // Per-(device, stream) workspace owned by the library, grown on demand.
 hipError_t stream_workspace(hipStream_t stream, size_t bytes, void **ptr);
 
