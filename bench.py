@@ -74,6 +74,119 @@ def cpu_baseline(v, f, rays, gpu_feat):
     return rec, {"feature_mismatches_vs_oracle": mism, "rays_checked": n}
 
 
+def batch_leg(args, log):
+    """64 noisy icospheres at levels 3-7 (uniform), 2^X rays each with X ~ U[10, 20]
+    (seed 2): one step = one batched launch over every mesh's rays, replayed
+    from a captured hipGraph after the autotuner picked the lane mapping.
+    Compared with 64 back-to-back single-mesh launches of the same rays."""
+    import numpy as np
+    import torch
+
+    from parry_amd import TriMesh, TriMeshBatch, scenes
+
+    spec = scenes.uneven_mesh_batch(64, seed=2)
+    meshes, rays, hosts = [], [], []
+    for level, nr, mseed, rseed in spec:
+        v, f = scenes.noisy_icosphere(level, seed=mseed)
+        meshes.append(TriMesh(v, f))
+        hosts.append((v, f))
+        rays.append(scenes.rays_into_unit_ball(nr, seed=rseed))
+    counts = [r.shape[0] for r in rays]
+    n = int(sum(counts))
+    ntri = int(sum(f.shape[0] for _, f in hosts))
+    batch = TriMeshBatch(meshes, counts)
+    stream = torch.cuda.current_stream()
+    d_rays = torch.from_numpy(np.concatenate(rays)).cuda()
+    out = torch.empty((n, 5), dtype=torch.float32, device="cuda")
+    _, cnt = batch.cast_local_rays_counted(d_rays, MAX_TOI)
+    sum_w = int(cnt[:, 0].sum(dtype=torch.int64).item())
+    sum_k = int(cnt[:, 1].sum(dtype=torch.int64).item())
+    alg_bytes = n * 44 + 64 * sum_w + 48 * sum_k
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3, e0.elapsed_time(e1) / steps
+
+    # 64 separate single-mesh launches (static mapping) on slices of the same buffers
+    offs = batch.offsets.astype(np.int64)
+    views = [(m, d_rays[offs[i]:offs[i + 1]], out[offs[i]:offs[i + 1]]) for i, m in enumerate(meshes)]
+    for m in meshes:
+        m.set_tuning("static")
+
+    def per_mesh():
+        for m, r, o in views:
+            if r.shape[0]:
+                m.cast_local_rays_and_get_normal(r, MAX_TOI, True, out=o, stream=stream)
+
+    sep_wall, sep_gpu = timed(per_mesh, max(1, args.steps // 2))
+    ref = out.clone()
+    modes = {}
+    for mapping, chunk in (("static", 128), ("persistent", 128)):
+        batch.set_tuning(mapping, chunk)
+        modes[f"{mapping}/{chunk}"] = timed(
+            lambda: batch.cast_local_rays_and_get_normal(d_rays, MAX_TOI, True, out=out, stream=stream), args.steps)
+    tuning, tune_ms = batch.autotune(d_rays, MAX_TOI, out=out)
+    graph = batch.capture(d_rays, MAX_TOI, True, out)
+    wall_ms, gpu_ms = timed(lambda: graph.launch(stream), args.steps)
+    identical = bool(torch.equal(out.view(torch.int32), ref.view(torch.int32)))
+    rec = {
+        "metric": "Mrays/s",
+        "value": round(n / (wall_ms * 1e-3) / 1e6, 3),
+        "unit": "Mrays/s",
+        "ms_per_step": round(wall_ms, 4),
+        "gpu_ms_per_step": round(gpu_ms, 4),
+        "workload": f"64 noisy icospheres, levels 3-7 ({ntri} tris), 2^U(10,20) rays each ({n} rays), seed 2; "
+                    "one step = one hipGraph replay of the batched launch (autotuned mapping)",
+        "rays": n,
+        "triangles": ntri,
+        "tuning": tuning,
+        "autotune_ms": round(tune_ms, 4),
+        "mode_ms": {k: round(v[1], 4) for k, v in modes.items()},
+        "separate_launches": {"wall_ms": round(sep_wall, 4), "gpu_ms": round(sep_gpu, 4),
+                              "value": round(n / (sep_wall * 1e-3) / 1e6, 3)},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "cast_rays_batch_*_kernel",
+            "achieved": round(alg_bytes / (gpu_ms * 1e-3) / 1e9, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(alg_bytes / (gpu_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes_per_launch": alg_bytes,
+        },
+        "parity": {"identical_to_separate_launches": identical},
+    }
+    if not args.no_cpu:
+        from oracle import oracle
+
+        cores = oracle.default_threads()
+        take = 1 << 14
+        om = [oracle.OracleTriMesh(v, f) for v, f in hosts]
+        t0 = time.perf_counter()
+        got = 0
+        mism = 0
+        host_out = out.cpu().numpy().view(np.uint32)[:, 4]
+        for i, (o, r) in enumerate(zip(om, rays)):
+            k = min(take, r.shape[0])
+            res = o.cast_rays(r[:k], MAX_TOI, threads=cores)
+            mism += int(np.count_nonzero(res["feature"] != host_out[offs[i]:offs[i] + k]))
+            got += k
+        t_cpu = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(got / t_cpu / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
+                               "sample": f"first min(16384, n_i) rays of each of the 64 meshes ({got} rays), "
+                                         f"{cores} threads"}
+        rec["parity"]["feature_mismatches_vs_oracle_sample"] = mism
+    return rec
+
+
 def contact_leg(args, log):
     """65,536 ConvexPolyhedra (8-64 hull points), mean ~8 AABB overlaps per shape:
     one step = leaf AABBs + Bvh refit + overlapping pairs + query::contact
@@ -178,6 +291,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--skip-contact", action="store_true", help="measure only the ray-cast line")
     ap.add_argument("--contact-only", action="store_true", help="profiling aid: run only the contact leg")
+    ap.add_argument("--skip-batch", action="store_true", help="skip the batched multi-mesh leg")
+    ap.add_argument("--batch-only", action="store_true", help="profiling aid: run only the batched multi-mesh leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -196,6 +311,9 @@ def main() -> int:
     torch.cuda.set_device(dev)
     if args.contact_only:
         print(json.dumps({"contact": contact_leg(args, log)}), flush=True)
+        return 0
+    if args.batch_only:
+        print(json.dumps({"batch": batch_leg(args, log)}), flush=True)
         return 0
     v, f = scenes.noisy_icosphere(6, seed=1)
     rays = scenes.rays_into_unit_ball(1 << 20, seed=2)
@@ -273,6 +391,9 @@ def main() -> int:
         rec["parity"] = parity
     else:
         rec["cpu_baseline"] = None
+    if not args.skip_batch:
+        log("batched multi-mesh leg ...")
+        rec["batch"] = batch_leg(args, log)
     if not args.skip_contact:
         log("contact leg (broadphase + narrowphase frame) ...")
         rec["contact"] = contact_leg(args, log)

@@ -153,6 +153,41 @@ int pq_trimesh_cast_local_rays_counted(const pq_trimesh *mesh, const pq_ray *ray
 int pq_trimesh_cast_local_rays_host(const pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays, float max_toi,
                                     int solid, pq_ray_hit *hits);
 
+/* ---- Batched ray casts over many TriMeshes (no counterpart in parry: the
+ *      batching of many RayCast calls) --------------------------------------- */
+
+typedef struct pq_trimesh_batch pq_trimesh_batch;
+typedef struct pq_graph pq_graph;
+
+/* Groups meshes for one launch.  ray_offsets (host, num_meshes + 1, first 0,
+ * non-decreasing): the rays of mesh i are [ray_offsets[i], ray_offsets[i+1]) of
+ * the concatenated ray array passed to the cast calls.  The meshes must
+ * outlive the batch. */
+int pq_trimesh_batch_create(const pq_trimesh *const *meshes, uint32_t num_meshes, const uint64_t *ray_offsets,
+                            pq_trimesh_batch **out);
+int pq_trimesh_batch_destroy(pq_trimesh_batch *batch);
+int pq_trimesh_batch_set_tuning(pq_trimesh_batch *batch, const pq_ray_tuning *tuning);
+int pq_trimesh_batch_get_tuning(const pq_trimesh_batch *batch, pq_ray_tuning *tuning);
+/* Every ray of every mesh: per-ray results exactly as pq_trimesh_cast_local_rays
+ * on the ray's own mesh.  Device pointers; asynchronous on `stream`.  The
+ * persistent mapping uses a work queue owned by the batch: launches on one
+ * batch must be ordered (one stream or a graph). */
+int pq_trimesh_batch_cast_local_rays(const pq_trimesh_batch *batch, const pq_ray *rays, float max_toi, int solid,
+                                     pq_ray_hit *hits, void *stream);
+int pq_trimesh_batch_cast_local_rays_counted(const pq_trimesh_batch *batch, const pq_ray *rays, float max_toi,
+                                             int solid, pq_ray_hit *hits, uint32_t *counts, void *stream);
+/* Times the lane mappings (static; persistent with several chunk sizes and
+ * residencies) on these rays, keeps the fastest in the batch and returns it
+ * with its time per launch.  Synchronises `stream`. */
+int pq_trimesh_batch_autotune(pq_trimesh_batch *batch, const pq_ray *rays, float max_toi, pq_ray_hit *hits,
+                              void *stream, pq_ray_tuning *chosen, float *chosen_ms);
+/* Captures one batched cast (with the batch's current tuning) into a hipGraph;
+ * replay it with pq_graph_launch.  stream NULL: an internal capture stream. */
+int pq_trimesh_batch_capture(const pq_trimesh_batch *batch, const pq_ray *rays, float max_toi, int solid,
+                             pq_ray_hit *hits, void *stream, pq_graph **out);
+int pq_graph_launch(pq_graph *graph, void *stream);
+int pq_graph_destroy(pq_graph *graph);
+
 /* ---- Shapes and poses ---------------------------------------------------- */
 
 /* parry3d shape types on the path (ShapeType, src/shape/shape.rs:36-94). */

@@ -17,6 +17,8 @@ from .query import (
 )
 from .shape import (
     TriMesh,
+    TriMeshBatch,
+    Graph,
     TriMeshBuilderError,
     BVH_NODE_WIDE_DTYPE,
     split_hits,
@@ -38,6 +40,8 @@ __all__ = [
     "RAY_DTYPE",
     "RAY_HIT_DTYPE",
     "TriMesh",
+    "TriMeshBatch",
+    "Graph",
     "TriMeshBuilderError",
     "BVH_NODE_WIDE_DTYPE",
     "split_hits",

@@ -45,6 +45,16 @@ SIGNATURES = {
     "pq_trimesh_cast_local_rays": (_int, [_vp, _vp, _u64, _f32, _int, _vp, _vp]),
     "pq_trimesh_cast_local_rays_counted": (_int, [_vp, _vp, _u64, _f32, _int, _vp, _vp, _vp]),
     "pq_trimesh_cast_local_rays_host": (_int, [_vp, _vp, _u64, _f32, _int, _vp]),
+    "pq_trimesh_batch_create": (_int, [_vp, _u32, _vp, _c.POINTER(_vp)]),
+    "pq_trimesh_batch_destroy": (_int, [_vp]),
+    "pq_trimesh_batch_set_tuning": (_int, [_vp, _vp]),
+    "pq_trimesh_batch_get_tuning": (_int, [_vp, _vp]),
+    "pq_trimesh_batch_cast_local_rays": (_int, [_vp, _vp, _f32, _int, _vp, _vp]),
+    "pq_trimesh_batch_cast_local_rays_counted": (_int, [_vp, _vp, _f32, _int, _vp, _vp, _vp]),
+    "pq_trimesh_batch_autotune": (_int, [_vp, _vp, _f32, _vp, _vp, _vp, _c.POINTER(_f32)]),
+    "pq_trimesh_batch_capture": (_int, [_vp, _vp, _f32, _int, _vp, _vp, _c.POINTER(_vp)]),
+    "pq_graph_launch": (_int, [_vp, _vp]),
+    "pq_graph_destroy": (_int, [_vp]),
     "pq_shapes_create": (_int, [_vp, _u32, _vp, _u32, _c.POINTER(_vp)]),
     "pq_shapes_destroy": (_int, [_vp]),
     "pq_shapes_compute_aabbs": (_int, [_vp, _vp, _vp, _vp]),

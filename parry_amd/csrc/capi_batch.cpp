@@ -1,0 +1,198 @@
+// capi_batch.cpp -- batched multi-mesh ray casts, their tuner and hipGraph capture.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pq_internal.h"
+
+using namespace pq;
+
+struct pq_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int batch_alloc(pq_trimesh_batch *b, void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    b->allocations.push_back(*p);
+    return PQ_OK;
+}
+
+int check_tuning(const pq_ray_tuning *t) {
+    if (!t) return fail(PQ_ERR_INVALID, "null tuning");
+    if (t->mapping > PQ_RAY_MAPPING_PERSISTENT || t->reserved != 0 || t->chunk > (1u << 20))
+        return fail(PQ_ERR_INVALID, "invalid ray tuning");
+    return PQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pq_trimesh_batch_create(const pq_trimesh *const *meshes, uint32_t num_meshes, const uint64_t *ray_offsets,
+                            pq_trimesh_batch **out) {
+    if (!out) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_create: null output");
+    *out = nullptr;
+    if (num_meshes == 0 || !meshes || !ray_offsets)
+        return fail(PQ_ERR_INVALID, "pq_trimesh_batch_create: empty batch");
+    if (ray_offsets[0] != 0) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_create: ray_offsets[0] must be 0");
+    std::vector<BatchMesh> bm(num_meshes);
+    uint32_t max_depth = 1;
+    for (uint32_t i = 0; i < num_meshes; ++i) {
+        if (!meshes[i]) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_create: null mesh " + std::to_string(i));
+        if (ray_offsets[i + 1] < ray_offsets[i])
+            return fail(PQ_ERR_INVALID, "pq_trimesh_batch_create: ray_offsets must not decrease");
+        const DeviceMesh &m = meshes[i]->dev;
+        // A root whose right child has no leaf holds the mesh's single triangle.
+        const bool single = (meshes[i]->host_nodes[0].right.data & 0x3fffffffu) == 0u;
+        bm[i] = BatchMesh{m.nodes, m.tris, m.num_triangles, single ? 1u : 0u};
+        max_depth = std::max(max_depth, m.depth);
+    }
+    auto *b = new pq_trimesh_batch();
+    DeviceBatch &d = b->dev;
+    d.num_meshes = num_meshes;
+    d.max_depth = max_depth;
+    d.num_rays = ray_offsets[num_meshes];
+    d.tuning = pq_ray_tuning{PQ_RAY_MAPPING_STATIC, 128, 0, 0};
+    int rc = batch_alloc(b, (void **)&d.meshes, num_meshes * sizeof(BatchMesh));
+    if (rc == PQ_OK) rc = batch_alloc(b, (void **)&d.offsets, (num_meshes + 1) * sizeof(unsigned long long));
+    if (rc == PQ_OK) rc = batch_alloc(b, (void **)&d.queue, 2 * sizeof(unsigned long long));
+    if (rc == PQ_OK) {
+        hipError_t e = hipMemcpy((void *)d.meshes, bm.data(), num_meshes * sizeof(BatchMesh), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy((void *)d.offsets, ray_offsets, (num_meshes + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(d.queue, 0, 2 * sizeof(unsigned long long));
+        if (e != hipSuccess) rc = hip_fail(e, "pq_trimesh_batch_create upload");
+    }
+    if (rc != PQ_OK) {
+        pq_trimesh_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return PQ_OK;
+}
+
+int pq_trimesh_batch_destroy(pq_trimesh_batch *b) {
+    if (!b) return PQ_OK;
+    for (void *p : b->allocations) (void)hipFree(p);
+    delete b;
+    return PQ_OK;
+}
+
+int pq_trimesh_batch_set_tuning(pq_trimesh_batch *b, const pq_ray_tuning *t) {
+    if (!b) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_set_tuning: null batch");
+    int rc = check_tuning(t);
+    if (rc != PQ_OK) return rc;
+    b->dev.tuning = *t;
+    return PQ_OK;
+}
+
+int pq_trimesh_batch_get_tuning(const pq_trimesh_batch *b, pq_ray_tuning *t) {
+    if (!b || !t) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_get_tuning: null argument");
+    *t = b->dev.tuning;
+    return PQ_OK;
+}
+
+int pq_trimesh_batch_cast_local_rays(const pq_trimesh_batch *b, const pq_ray *rays, float max_toi, int solid,
+                                     pq_ray_hit *hits, void *stream) {
+    (void)solid;
+    if (!b) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_cast_local_rays: null batch");
+    if (b->dev.num_rays && (!rays || !hits)) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_cast_local_rays: null buffer");
+    PQ_HIP_CHECK(launch_cast_rays_batch(b->dev, rays, max_toi, hits, nullptr, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_trimesh_batch_cast_local_rays_counted(const pq_trimesh_batch *b, const pq_ray *rays, float max_toi, int solid,
+                                             pq_ray_hit *hits, uint32_t *counts, void *stream) {
+    (void)solid;
+    if (!b) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_cast_local_rays_counted: null batch");
+    if (b->dev.num_rays && (!rays || !hits || !counts))
+        return fail(PQ_ERR_INVALID, "pq_trimesh_batch_cast_local_rays_counted: null buffer");
+    PQ_HIP_CHECK(launch_cast_rays_batch(b->dev, rays, max_toi, hits, counts, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_trimesh_batch_autotune(pq_trimesh_batch *b, const pq_ray *rays, float max_toi, pq_ray_hit *hits, void *stream,
+                              pq_ray_tuning *chosen, float *chosen_ms) {
+    if (!b || !rays || !hits) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_autotune: null argument");
+    const hipStream_t s = (hipStream_t)stream;
+    const pq_ray_tuning candidates[] = {
+        {PQ_RAY_MAPPING_STATIC, 128, 0, 0},     {PQ_RAY_MAPPING_PERSISTENT, 64, 0, 0},
+        {PQ_RAY_MAPPING_PERSISTENT, 128, 0, 0}, {PQ_RAY_MAPPING_PERSISTENT, 256, 0, 0},
+        {PQ_RAY_MAPPING_PERSISTENT, 512, 0, 0}, {PQ_RAY_MAPPING_PERSISTENT, 128, 4, 0},
+    };
+    hipEvent_t e0, e1;
+    PQ_HIP_CHECK(hipEventCreate(&e0));
+    PQ_HIP_CHECK(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    pq_ray_tuning best = candidates[0];
+    for (const pq_ray_tuning &c : candidates) {
+        b->dev.tuning = c;
+        PQ_HIP_CHECK(launch_cast_rays_batch(b->dev, rays, max_toi, hits, nullptr, s));  // warm-up
+        PQ_HIP_CHECK(hipEventRecord(e0, s));
+        for (int k = 0; k < 3; ++k) PQ_HIP_CHECK(launch_cast_rays_batch(b->dev, rays, max_toi, hits, nullptr, s));
+        PQ_HIP_CHECK(hipEventRecord(e1, s));
+        PQ_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        PQ_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        ms /= 3.0f;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->dev.tuning = best;
+    if (chosen) *chosen = best;
+    if (chosen_ms) *chosen_ms = best_ms;
+    return PQ_OK;
+}
+
+int pq_trimesh_batch_capture(const pq_trimesh_batch *b, const pq_ray *rays, float max_toi, int solid,
+                             pq_ray_hit *hits, void *stream, pq_graph **out) {
+    (void)solid;
+    if (!b || !out || !rays || !hits) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_capture: null argument");
+    *out = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    hipStream_t own = nullptr;
+    if (!s) {
+        PQ_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+        s = own;
+    }
+    auto *g = new pq_graph();
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t le = launch_cast_rays_batch(b->dev, rays, max_toi, hits, nullptr, s);
+        e = hipStreamEndCapture(s, &g->graph);
+        if (e == hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (own) (void)hipStreamDestroy(own);
+    if (e != hipSuccess) {
+        pq_graph_destroy(g);
+        return hip_fail(e, "pq_trimesh_batch_capture");
+    }
+    *out = g;
+    return PQ_OK;
+}
+
+int pq_graph_launch(pq_graph *g, void *stream) {
+    if (!g || !g->exec) return fail(PQ_ERR_INVALID, "pq_graph_launch: null graph");
+    PQ_HIP_CHECK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_graph_destroy(pq_graph *g) {
+    if (!g) return PQ_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return PQ_OK;
+}
+
+}  // extern "C"

@@ -65,6 +65,27 @@ struct DeviceMesh {
     pq_ray_tuning tuning;
 };
 
+// One mesh of a batch, as the batched kernels see it.
+struct BatchMesh {
+    const pq_bvh_node_wide *nodes;
+    const TriRecord *tris;
+    uint32_t ntri;
+    uint32_t single_leaf;
+};
+
+struct DeviceBatch {
+    const BatchMesh *meshes;               // device
+    const unsigned long long *offsets;     // device, num_meshes + 1 ray offsets
+    unsigned long long *queue;             // device, 2 x u64 (persistent mapping)
+    uint32_t num_meshes;
+    uint32_t max_depth;
+    uint64_t num_rays;
+    pq_ray_tuning tuning;
+};
+
+hipError_t launch_cast_rays_batch(const DeviceBatch &b, const pq_ray *rays, float max_toi, pq_ray_hit *hits,
+                                  uint32_t *counts, hipStream_t stream);
+
 // Per-(device, stream) 16-byte work queue used by persistent kernels
 // (zeroed at allocation; every persistent kernel leaves it zeroed).
 hipError_t stream_queue(hipStream_t stream, unsigned long long **queue);
@@ -144,6 +165,11 @@ struct pq_shapes {
 
 struct pq_bvh {
     pq::DeviceBvh dev;
+    std::vector<void *> allocations;
+};
+
+struct pq_trimesh_batch {
+    pq::DeviceBatch dev;
     std::vector<void *> allocations;
 };
 

@@ -371,6 +371,142 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_persistent_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched casts over many meshes in one launch.  Rays are concatenated mesh by
+// mesh; ray i belongs to the mesh whose [offset, next offset) range holds it
+// (binary search over the offsets, which stay in cache).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t mesh_of(const unsigned long long *offsets, uint32_t nmesh, uint64_t i) {
+    uint32_t lo = 0, hi = nmesh;  // offsets[lo] <= i < offsets[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (offsets[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void cast_one(const BatchMesh &bm, const LaneRay &r, float max_toi, uint32_t *stk,
+                                         Traversal &t, uint32_t &n_wide, uint32_t &n_tri) {
+    const float4 *node4 = reinterpret_cast<const float4 *>(bm.nodes);
+    const float4 *tri4 = reinterpret_cast<const float4 *>(bm.tris);
+    start_traversal(t, max_toi);
+    n_wide = n_tri = 0;
+    if (bm.single_leaf) {
+        const float4 q0 = node4[0], q1 = node4[1];
+        n_wide = 1;
+        if (slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi) < max_toi)
+            visit_leaf<true>(tri4, __float_as_uint(q0.w), r, t, n_tri);
+        return;
+    }
+    while (!find_best_step<COUNT>(node4, tri4, r, t, stk, n_wide, n_tri)) {
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kRayBlock) void cast_rays_batch_kernel(const BatchMesh *__restrict__ meshes,
+                                                                    const unsigned long long *__restrict__ offsets,
+                                                                    uint32_t nmesh, const pq_ray *__restrict__ rays,
+                                                                    uint64_t n, float max_toi,
+                                                                    pq_ray_hit *__restrict__ hits,
+                                                                    uint32_t *__restrict__ counts) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint64_t i = (uint64_t)blockIdx.x * kRayBlock + threadIdx.x;
+    if (i >= n) return;
+    const BatchMesh bm = meshes[mesh_of(offsets, nmesh, i)];
+    const LaneRay r = load_ray(rays, i);
+    Traversal t;
+    uint32_t n_wide, n_tri;
+    cast_one<COUNT>(bm, r, max_toi, lds_stack + threadIdx.x, t, n_wide, n_tri);
+    if (COUNT) {
+        counts[2 * i] = n_wide;
+        counts[2 * i + 1] = n_tri;
+    }
+    write_hit(hits, i, t, reinterpret_cast<const float4 *>(bm.tris), bm.ntri);
+}
+
+// Persistent variant: one device queue over the concatenated rays; finished
+// lanes take the next rays every iteration, whatever mesh they belong to.
+template <bool COUNT>
+__global__ __launch_bounds__(kRayBlock) void cast_rays_batch_persistent_kernel(
+    const BatchMesh *__restrict__ meshes, const unsigned long long *__restrict__ offsets, uint32_t nmesh,
+    const pq_ray *__restrict__ rays, uint64_t n, float max_toi, pq_ray_hit *__restrict__ hits,
+    uint32_t *__restrict__ counts, unsigned long long *__restrict__ queue, uint32_t chunk) {
+    extern __shared__ uint32_t lds_stack[];
+    uint32_t *stk = lds_stack + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    uint64_t q_next = 0, q_end = 0;
+    bool drained = false, active = false;
+    uint64_t ray_id = 0;
+    LaneRay r{};
+    BatchMesh bm{};
+    Traversal t;
+    start_traversal(t, max_toi);
+    uint32_t n_wide = 0, n_tri = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        if (idle != 0 && !drained) {
+            if (q_next >= q_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&queue[0], (unsigned long long)chunk);
+                base = __shfl(base, 0);
+                q_next = base;
+                q_end = base + chunk < n ? base + chunk : n;
+                drained = base >= n;
+            }
+            if (!drained) {
+                const uint64_t avail = q_end - q_next;
+                const uint32_t rank = lanes_below(idle);
+                if (!active && rank < avail) {
+                    ray_id = q_next + rank;
+                    bm = meshes[mesh_of(offsets, nmesh, ray_id)];
+                    r = load_ray(rays, ray_id);
+                    start_traversal(t, max_toi);
+                    n_wide = n_tri = 0;
+                    active = true;
+                    if (bm.single_leaf) {  // one-triangle mesh: a single test
+                        const float4 *node4 = reinterpret_cast<const float4 *>(bm.nodes);
+                        const float4 q0 = node4[0], q1 = node4[1];
+                        n_wide = 1;
+                        if (slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi) < max_toi)
+                            visit_leaf<true>(reinterpret_cast<const float4 *>(bm.tris), __float_as_uint(q0.w), r, t,
+                                             n_tri);
+                    }
+                }
+                const uint64_t took = (uint64_t)__popcll(idle);
+                q_next += took < avail ? took : avail;
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            const bool done = bm.single_leaf ||
+                              find_best_step<COUNT>(reinterpret_cast<const float4 *>(bm.nodes),
+                                                    reinterpret_cast<const float4 *>(bm.tris), r, t, stk, n_wide, n_tri);
+            if (done) {
+                if (COUNT) {
+                    counts[2 * ray_id] = n_wide;
+                    counts[2 * ray_id + 1] = n_tri;
+                }
+                write_hit(hits, ray_id, t, reinterpret_cast<const float4 *>(bm.tris), bm.ntri);
+                active = false;
+            }
+        }
+    }
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t waves = gridDim.x * (kRayBlock / 64);
+        const unsigned long long done = atomicAdd(&queue[1], 1ull);
+        if (done == waves - 1) {
+            queue[0] = 0;
+            queue[1] = 0;
+        }
+    }
+}
+
 }  // namespace
 
 uint32_t ray_stack_entries(uint32_t depth) { return depth > 1 ? depth - 1 : 1; }
@@ -415,6 +551,50 @@ hipError_t launch_cast_rays(const DeviceMesh &m, const pq_ray *rays, uint64_t n,
     else
         hipLaunchKernelGGL(cast_rays_kernel<false>, dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream, m.nodes,
                            m.tris, m.num_triangles, rays, n, max_toi, hits, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_cast_rays_batch(const DeviceBatch &b, const pq_ray *rays, float max_toi, pq_ray_hit *hits,
+                                  uint32_t *counts, hipStream_t stream) {
+    const uint64_t n = b.num_rays;
+    if (n == 0) return hipSuccess;
+    const size_t lds = (size_t)ray_stack_entries(b.max_depth) * kRayBlock * sizeof(uint32_t);
+    if (b.tuning.mapping == PQ_RAY_MAPPING_PERSISTENT) {
+        hipError_t e;
+        int dev = 0, cus = 0, per_cu = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+        if (counts)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cast_rays_batch_persistent_kernel<true>,
+                                                             kRayBlock, lds);
+        else
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cast_rays_batch_persistent_kernel<false>,
+                                                             kRayBlock, lds);
+        if (e != hipSuccess) return e;
+        if (b.tuning.blocks_per_cu > 0 && (int)b.tuning.blocks_per_cu < per_cu) per_cu = (int)b.tuning.blocks_per_cu;
+        if (per_cu < 1) per_cu = 1;
+        const uint32_t chunk = b.tuning.chunk ? b.tuning.chunk : 128;
+        const uint64_t want = (n + chunk - 1) / chunk;
+        uint64_t blocks = (uint64_t)cus * (uint64_t)per_cu;
+        if (blocks * (kRayBlock / 64) > want) blocks = (want + (kRayBlock / 64) - 1) / (kRayBlock / 64);
+        if (blocks < 1) blocks = 1;
+        if (counts)
+            hipLaunchKernelGGL(cast_rays_batch_persistent_kernel<true>, dim3((uint32_t)blocks), dim3(kRayBlock), lds,
+                               stream, b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts, b.queue,
+                               chunk);
+        else
+            hipLaunchKernelGGL(cast_rays_batch_persistent_kernel<false>, dim3((uint32_t)blocks), dim3(kRayBlock), lds,
+                               stream, b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts, b.queue,
+                               chunk);
+        return hipGetLastError();
+    }
+    const uint64_t blocks = (n + kRayBlock - 1) / kRayBlock;
+    if (counts)
+        hipLaunchKernelGGL(cast_rays_batch_kernel<true>, dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream,
+                           b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts);
+    else
+        hipLaunchKernelGGL(cast_rays_batch_kernel<false>, dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream,
+                           b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts);
     return hipGetLastError();
 }
 

@@ -119,6 +119,23 @@ def rays_into_unit_ball(n: int, seed: int, origin_radius: float = 3.0) -> np.nda
     return np.ascontiguousarray(np.concatenate([o, d], axis=1), dtype=np.float32)
 
 
+def uneven_mesh_batch(num_meshes: int, seed: int, levels=(3, 7), log2_rays=(10.0, 20.0)):
+    """The uneven multi-mesh ray workload: per mesh a subdivision level uniform
+    over ``levels`` (inclusive) and 2^X rays with X uniform in ``log2_rays``
+    (continuous, rounded to the nearest integer count).  64 meshes at the
+    default ranges give about 10 M rays over about 5.6 M triangles.
+
+    Returns a list of (level, num_rays, mesh_seed, ray_seed) tuples; build the
+    meshes with ``noisy_icosphere(level, mesh_seed)`` and the rays with
+    ``rays_into_unit_ball(num_rays, ray_seed)``.
+    """
+    rng = _rng(seed)
+    lv = rng.integers(levels[0], levels[1] + 1, size=num_meshes)
+    x = rng.uniform(log2_rays[0], log2_rays[1], size=num_meshes)
+    counts = np.rint(np.exp2(x)).astype(np.int64)
+    return [(int(lv[i]), int(counts[i]), seed * 1000 + 2 * i + 1, seed * 1000 + 2 * i + 2) for i in range(num_meshes)]
+
+
 def load_obj(path: str) -> tuple[np.ndarray, np.ndarray]:
     """Minimal Wavefront OBJ reader (``v`` and triangular ``f`` records, 1-based)."""
     vs, fs = [], []

@@ -185,6 +185,135 @@ def split_hits(hits):
     return hits[:, 0], hits[:, 1:4], feat
 
 
+class TriMeshBatch:
+    """Many ``TriMesh`` ray-cast batches of uneven sizes in one launch.
+
+    ``ray_counts[i]`` rays are cast against ``meshes[i]``; the rays of all
+    meshes are concatenated in mesh order.  Each ray's result is exactly what
+    ``meshes[i].cast_local_rays_and_get_normal`` returns for it.  The meshes
+    are kept alive by the batch.
+    """
+
+    def __init__(self, meshes, ray_counts):
+        self._meshes = list(meshes)
+        counts = np.asarray(ray_counts, dtype=np.uint64).reshape(-1)
+        if counts.shape[0] != len(self._meshes):
+            raise ValueError("one ray count per mesh")
+        self.offsets = np.zeros(len(self._meshes) + 1, dtype=np.uint64)
+        np.cumsum(counts, out=self.offsets[1:])
+        handles = (ctypes.c_void_p * len(self._meshes))(*[m.handle.value for m in self._meshes])
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().pq_trimesh_batch_create(handles, len(self._meshes), _ptr(self.offsets), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _native.lib().pq_trimesh_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    @property
+    def num_rays(self) -> int:
+        return int(self.offsets[-1])
+
+    def set_tuning(self, mapping: str = "persistent", chunk: int = 128, blocks_per_cu: int = 0) -> None:
+        m = {"static": _native.PQ_RAY_MAPPING_STATIC, "persistent": _native.PQ_RAY_MAPPING_PERSISTENT}[mapping]
+        t = _native.RayTuning(m, chunk, blocks_per_cu, 0)
+        _native.check(_native.lib().pq_trimesh_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self) -> dict:
+        t = _native.RayTuning()
+        _native.check(_native.lib().pq_trimesh_batch_get_tuning(self._h, ctypes.byref(t)))
+        return {"mapping": "persistent" if t.mapping == _native.PQ_RAY_MAPPING_PERSISTENT else "static",
+                "chunk": t.chunk, "blocks_per_cu": t.blocks_per_cu}
+
+    def _check_rays(self, rays, out):
+        import torch
+
+        n = self.num_rays
+        if not (rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous() and tuple(rays.shape) == (n, 6)):
+            raise ValueError(f"rays must be a contiguous CUDA float32 tensor of shape ({n}, 6)")
+        if out is None:
+            out = torch.empty((n, 5), dtype=torch.float32, device=rays.device)
+        elif not (out.is_cuda and out.dtype == torch.float32 and out.is_contiguous() and out.numel() >= 5 * n):
+            raise ValueError("out must be a contiguous CUDA float32 tensor with room for (n, 5)")
+        return out
+
+    @staticmethod
+    def _stream(rays, stream):
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream(rays.device)
+        return ctypes.c_void_p(s.cuda_stream)
+
+    def cast_local_rays_and_get_normal(self, rays, max_time_of_impact: float, solid: bool, out=None, stream=None):
+        """(n, 5) float32 ``pq_ray_hit`` records for the concatenated rays, asynchronously on ``stream``."""
+        out = self._check_rays(rays, out)
+        _native.check(_native.lib().pq_trimesh_batch_cast_local_rays(
+            self._h, rays.data_ptr(), float(max_time_of_impact), int(bool(solid)), out.data_ptr(),
+            self._stream(rays, stream)))
+        return out
+
+    def cast_local_rays_counted(self, rays, max_time_of_impact: float, solid: bool = True, stream=None):
+        import torch
+
+        out = self._check_rays(rays, None)
+        counts = torch.empty((self.num_rays, 2), dtype=torch.int32, device=rays.device)
+        _native.check(_native.lib().pq_trimesh_batch_cast_local_rays_counted(
+            self._h, rays.data_ptr(), float(max_time_of_impact), int(bool(solid)), out.data_ptr(), counts.data_ptr(),
+            self._stream(rays, stream)))
+        return out, counts
+
+    def autotune(self, rays, max_time_of_impact: float, out=None, stream=None):
+        """Times the lane mappings on these rays and keeps the fastest; returns (tuning, ms per launch)."""
+        out = self._check_rays(rays, out)
+        t = _native.RayTuning()
+        ms = ctypes.c_float()
+        _native.check(_native.lib().pq_trimesh_batch_autotune(
+            self._h, rays.data_ptr(), float(max_time_of_impact), out.data_ptr(), self._stream(rays, stream),
+            ctypes.byref(t), ctypes.byref(ms)))
+        return self.tuning(), float(ms.value)
+
+    def capture(self, rays, max_time_of_impact: float, solid: bool, out, stream=None) -> "Graph":
+        """Captures one batched cast into a replayable hipGraph writing ``out``."""
+        out = self._check_rays(rays, out)
+        g = ctypes.c_void_p()
+        _native.check(_native.lib().pq_trimesh_batch_capture(
+            self._h, rays.data_ptr(), float(max_time_of_impact), int(bool(solid)), out.data_ptr(),
+            self._stream(rays, stream), ctypes.byref(g)))
+        return Graph(g, keep=(self, rays, out))
+
+
+class Graph:
+    """A captured hipGraph (``pq_graph``); ``launch`` replays it on a stream."""
+
+    def __init__(self, handle: ctypes.c_void_p, keep=()):
+        self._h = handle
+        self._keep = keep
+
+    def launch(self, stream=None) -> None:
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _native.check(_native.lib().pq_graph_launch(self._h, ctypes.c_void_p(s.cuda_stream)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _native.lib().pq_graph_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
 # ---------------------------------------------------------------------------
 # Shape sets (ConvexPolyhedron, Ball, ...) for the broadphase and contacts
 # ---------------------------------------------------------------------------
