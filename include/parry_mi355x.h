@@ -283,14 +283,20 @@ int pq_contact_get_tuning(pq_contact_tuning *tuning);
 size_t pq_contact_workspace_bytes(uint64_t num_pairs);
 
 /* Batched query::contact(pos1, g1, pos2, g2, prediction)
- * (src/query/contact/contact_shape_shape.rs:123-138) over shape pairs.
+ * (src/query/contact/contact_shape_shape.rs:123-138) over shape pairs, with
+ * DefaultQueryDispatcher::contact's routing (default_query_dispatcher.rs:302-356):
+ * Ball-Ball closed form (contact_ball_ball.rs:9-42); Ball with Cuboid, Capsule,
+ * Cone or ConvexPolyhedron by projecting the ball centre
+ * (contact_ball_convex_polyhedron.rs:12-63); every other pair by GJK/EPA
+ * (contact_support_map_support_map.rs:10-77).
  *   shapes:    the shape set; poses: device, one pq_isometry per shape;
  *   pairs:     device, num_pairs x (i, j) shape indices;
  *   init_dirs: device, nullable; num_pairs x 3 floats: the GJK seed direction
  *              (unit, in shape i's local frame) passed as `init_dir` to
  *              contact_support_map_support_map_with_params
  *              (contact_support_map_support_map.rs:40-60); NULL = parry's
- *              default seed normalize(pos12.translation), else +x;
+ *              default seed normalize(pos12.translation), else +x.  Ignored
+ *              for pairs with a Ball (parry takes no seed there);
  *   out:       device, num_pairs pq_contact;
  *   iters:     device, nullable, 2 u32 per pair: GJK iterations and EPA
  *              iterations (0 when EPA did not run);

@@ -32,7 +32,9 @@ template <> struct Eps<double> {
 
 // TRIANGLE (pts[0..3]) is test-only: it lets the reference's triangle EPA
 // known-answer test (crates/parry3d/tests/geometry/epa3.rs:25-59) run here.
-enum ShapeType : uint32_t { BALL = 0, CUBOID = 1, CAPSULE = 2, CONE = 3, CONVEX_POLYHEDRON = 4, TRIANGLE = 5 };
+// ORIGIN is parry's ConstantOrigin (special_support_maps.rs:32-54), the second
+// operand of GJK/EPA point projections.
+enum ShapeType : uint32_t { BALL = 0, CUBOID = 1, CAPSULE = 2, CONE = 3, CONVEX_POLYHEDRON = 4, TRIANGLE = 5, ORIGIN = 6 };
 
 template <class T> struct Shape {
     uint32_t type;
@@ -78,6 +80,8 @@ template <class T> V3<T> local_support_point(const Shape<T> &s, V3<T> dir) {
             if (dot(d, a) > dot(d, b)) return a + d * s.p[1];
             return b + d * s.p[1];
         }
+        case ORIGIN:  // special_support_maps.rs:46-48
+            return v3(T(0), T(0), T(0));
         case CONE: {  // cone.rs:162-182
             V3<T> v = dir;
             v.y = T(0);
@@ -96,8 +100,10 @@ template <class T> V3<T> local_support_point(const Shape<T> &s, V3<T> dir) {
     return v3(T(0), T(0), T(0));
 }
 
-// SupportMap::support_point (support_map.rs:380-383).
+// SupportMap::support_point (support_map.rs:380-383); ConstantOrigin
+// overrides it with m's translation (special_support_maps.rs:36-38).
 template <class T> V3<T> support_point(const Iso<T> &m, const Shape<T> &s, V3<T> dir) {
+    if (s.type == ORIGIN) return m.t;
     V3<T> local_dir = inv_rotate(m.q, dir);
     return transform_point(m, local_support_point(s, local_dir));
 }
@@ -928,9 +934,284 @@ template <class T> bool contact_ball_ball(const Iso<T> &pos12, T r1, T r2, T pre
     return true;
 }
 
-// query::contact (contact_shape_shape.rs:123-138) for the shape pairs handled
-// here (Ball-Ball and support-map pairs without a Ball).  Returns 1 on
-// contact, 0 for None, -1 for pairs not handled by this restatement.
+// ---- Ball against a convex shape (contact_ball_convex_polyhedron.rs:12-63) ----
+// Point projections with solid = false and the feature ids the contact needs.
+
+// Isometry::inverse: (q^-1, q^-1 * -t).
+template <class T> inline Iso<T> iso_inverse(const Iso<T> &m) {
+    Quat<T> qi = conj(m.q);
+    return Iso<T>{qi, rotate(qi, -m.t)};
+}
+
+// approx relative_eq!(a, b) with epsilon = max_relative = EPS, per component.
+template <class T> inline bool rel_eq(T a, T b) {
+    if (a == b) return true;
+    if (std::isinf(a) || std::isinf(b)) return false;
+    const T e = Eps<T>::eps;
+    T d = std::fabs(a - b);
+    if (d <= e) return true;
+    T fa = std::fabs(a), fb = std::fabs(b);
+    T largest = fb > fa ? fb : fa;
+    return d <= largest * e;
+}
+template <class T> inline bool rel_eq(V3<T> a, V3<T> b) { return rel_eq(a.x, b.x) && rel_eq(a.y, b.y) && rel_eq(a.z, b.z); }
+
+// nalgebra Unit::try_new_and_get(v, m): |v|^2 > m^2.
+template <class T> inline bool try_unit(V3<T> v, T m, V3<T> &u, T &len) {
+    T sq = dot(v, v);
+    if (!(sq > m * m)) return false;
+    len = std::sqrt(sq);
+    u = v / len;
+    return true;
+}
+
+template <class T> struct PointProj {
+    bool inside;
+    V3<T> point;
+};
+// Cuboid feature of Aabb::project_local_point_and_get_feature (point_aabb.rs:60-132).
+enum CuboidFeature : uint32_t { CF_UNKNOWN = 0, CF_VERTEX = 1, CF_EDGE = 2, CF_FACE = 3 };
+
+// Segment::project_local_point (point_segment.rs:45-85) with its relative_eq inside flag.
+template <class T> PointProj<T> segment_point(V3<T> a, V3<T> b, V3<T> pt) {
+    Loc<T> loc;
+    V3<T> p = project_segment(a, b, pt, loc);
+    return PointProj<T>{rel_eq(p, pt), p};
+}
+
+// Cuboid (point_cuboid.rs:14-22) -> Aabb(-he, he) (point_aabb.rs:9-132), solid = false.
+template <class T> PointProj<T> cuboid_point_feature(const Shape<T> &s, V3<T> pt, uint32_t &fkind, uint32_t &fid) {
+    V3<T> mins = v3(-s.p[0], -s.p[1], -s.p[2]), maxs = v3(s.p[0], s.p[1], s.p[2]);
+    V3<T> mins_pt = mins - pt, pt_maxs = pt - maxs;
+    V3<T> zero = v3(T(0), T(0), T(0));
+    V3<T> shift = vmax(mins_pt, zero) - vmax(pt_maxs, zero);
+    bool inside = shift.x == T(0) && shift.y == T(0) && shift.z == T(0);
+    V3<T> ls;
+    if (!inside) {
+        ls = pt + shift;
+    } else {
+        T best = -std::numeric_limits<T>::max();
+        bool is_mins = false;
+        int best_id = 0;
+        for (int i = 0; i < 3; ++i) {
+            T a = mins_pt[i], b = pt_maxs[i];
+            if (a < b) {
+                if (pt_maxs[i] > best) {
+                    best_id = i;
+                    is_mins = false;
+                    best = b;
+                }
+            } else if (a > best) {
+                best_id = i;
+                is_mins = true;
+                best = a;
+            }
+        }
+        shift = zero;
+        shift[best_id] = is_mins ? best : -best;
+        ls = pt + shift;
+    }
+    int nzero = 0, last_zero = 0, last_nz = 0;
+    for (int i = 0; i < 3; ++i) {
+        if (shift[i] == T(0)) {
+            ++nzero;
+            last_zero = i;
+        } else {
+            last_nz = i;
+        }
+    }
+    const T e = Eps<T>::eps;
+    V3<T> center = (mins + maxs) * T(0.5);
+    fkind = CF_UNKNOWN;
+    fid = 0;
+    if (nzero == 3) {
+        for (int i = 0; i < 3; ++i) {
+            if (ls[i] > maxs[i] - e) {
+                fkind = CF_FACE;
+                fid = (uint32_t)i;
+                break;
+            }
+            if (ls[i] <= mins[i] + e) {
+                fkind = CF_FACE;
+                fid = (uint32_t)(i + 3);
+                break;
+            }
+        }
+    } else if (nzero == 2) {
+        fkind = CF_FACE;
+        fid = ls[last_nz] < center[last_nz] ? (uint32_t)(last_nz + 3) : (uint32_t)last_nz;
+    } else {
+        uint32_t id = 0;
+        for (int i = 0; i < 3; ++i)
+            if (ls[i] < center[i]) id |= 1u << i;
+        if (nzero == 0) {
+            fkind = CF_VERTEX;
+            fid = id;
+        } else {
+            fkind = CF_EDGE;
+            fid = (id << 2) | (uint32_t)last_zero;
+        }
+    }
+    return PointProj<T>{inside, ls};
+}
+
+// Cuboid::feature_normal (cuboid.rs:401-448).
+template <class T> bool cuboid_feature_normal(uint32_t fkind, uint32_t id, V3<T> &n) {
+    n = v3(T(0), T(0), T(0));
+    if (fkind == CF_FACE) {
+        if (id < 3) n[id] = T(1);
+        else n[id - 3] = T(-1);
+        return true;
+    }
+    if (fkind == CF_EDGE) {
+        uint32_t edge = id & 3u, f1 = (edge + 1) % 3, f2 = (edge + 2) % 3, signs = id >> 2;
+        n[f1] = (signs & (1u << f1)) ? T(-1) : T(1);
+        n[f2] = (signs & (1u << f2)) ? T(-1) : T(1);
+        n = n / norm(n);
+        return true;
+    }
+    if (fkind == CF_VERTEX) {
+        for (int i = 0; i < 3; ++i) n[i] = (id & (1u << i)) ? T(-1) : T(1);
+        n = n / norm(n);
+        return true;
+    }
+    return false;
+}
+
+// Capsule::project_local_point (point_capsule.rs:9-50), solid = false.
+template <class T> PointProj<T> capsule_point(const Shape<T> &s, V3<T> pt) {
+    V3<T> a = v3(T(0), -s.p[0], T(0)), b = v3(T(0), s.p[0], T(0));
+    const T r = s.p[1];
+    PointProj<T> proj = segment_point(a, b, pt);
+    V3<T> dproj = pt - proj.point;
+    V3<T> dir;
+    T dist;
+    if (try_unit(dproj, Eps<T>::eps, dir, dist)) return PointProj<T>{dist <= r, proj.point + dir * r};
+    V3<T> sd;
+    T sl;
+    if (try_unit(b - a, Eps<T>::eps, sd, sl)) {
+        // WBasis::orthonormal_basis()[0] (utils/wops.rs:96-110)
+        T sign = std::copysign(T(1), sd.z);
+        T aa = T(-1) / (sign + sd.z);
+        T bb = sd.x * sd.y * aa;
+        V3<T> e0 = v3(T(1) + sign * sd.x * sd.x * aa, sign * bb, -sign * sd.x);
+        return PointProj<T>{true, proj.point + e0 * r};
+    }
+    return PointProj<T>{true, proj.point + v3(T(0), r, T(0))};
+}
+
+// Cone::project_local_point (point_cone.rs:8-63), solid = false.
+template <class T> PointProj<T> cone_point(const Shape<T> &s, V3<T> pt) {
+    const T h = s.p[0], r = s.p[1];
+    T dx = pt.x, dz = pt.z;
+    T planar = std::sqrt(dx * dx + dz * dz);  // Vector2::normalize_mut
+    dx = dx / planar;
+    dz = dz / planar;
+    if (planar <= Eps<T>::eps) {
+        dx = T(1);
+        dz = T(0);
+    }
+    V3<T> on_basis = v3(pt.x, -h, pt.z);
+    if (pt.y < -h && planar <= r) return PointProj<T>{false, on_basis};
+    V3<T> circle = v3(dx * r, -h, dz * r);
+    V3<T> apex = v3(T(0), h, T(0));
+    V3<T> side_dir = circle - apex;
+    PointProj<T> proj = segment_point(apex, circle, pt);
+    V3<T> apex_to_basis = v3(T(0), T(-2) * h, T(0));
+    if (pt.y >= -h && pt.y <= h && dot(cross(side_dir, pt - apex), cross(side_dir, apex_to_basis)) >= T(0)) {
+        V3<T> d1 = proj.point - pt, d2 = on_basis - pt;
+        if (dot(d1, d1) > dot(d2, d2)) return PointProj<T>{true, on_basis};
+        proj.inside = true;
+        return proj;
+    }
+    return proj;
+}
+
+// local_point_projection_on_support_map (point_support_map.rs:17-51), solid = false.
+// Returns false where parry reaches unreachable!() (GJK ending without a result).
+template <class T> bool support_map_point(const Shape<T> &s, V3<T> pt, PointProj<T> &out, ContactStats *st) {
+    Shape<T> origin{ORIGIN, nullptr, 0, {T(0), T(0), T(0), T(0)}};
+    const Quat<T> id{T(0), T(0), T(0), T(1)};
+    Iso<T> m{id, -pt}, m_inv{id, pt};
+    V3<T> dir;
+    T len;
+    if (!try_unit(pt, Eps<T>::eps, dir, len)) dir = v3(T(1), T(0), T(0));
+    Simplex<T> simplex;
+    simplex.reset(cso_from_shapes(m_inv, s, origin, dir));
+    Iso<T> pos12 = iso_inverse(m);
+    uint32_t gi = 0;
+    GjkResult<T> g = gjk_closest_points(pos12, s, origin, std::numeric_limits<T>::max(), simplex, &gi);
+    if (st) st->gjk_iters = gi;
+    if (g.kind == GJK_CLOSEST_POINTS) {
+        out = PointProj<T>{false, g.p1};
+        return true;
+    }
+    if (g.kind != GJK_INTERSECTION) return false;
+    Epa<T> epa;
+    V3<T> p1, p2, n;
+    uint32_t ei = 0;
+    if (st) st->used_epa = true;
+    bool ok = epa.closest_points(pos12, s, origin, simplex, p1, p2, n, &ei);
+    if (st) st->epa_iters = ei;
+    out = PointProj<T>{true, ok ? p1 : pt};
+    return true;
+}
+
+// contact_convex_polyhedron_ball (contact_ball_convex_polyhedron.rs:27-63) in
+// shape 1's frame.  Returns 1 (contact), 0 (None), -1 (unreachable!() in parry).
+template <class T>
+int contact_convex_ball(const Iso<T> &pos12, const Shape<T> &g1, T radius, T prediction, Contact<T> &out,
+                        ContactStats *st) {
+    V3<T> c = pos12.t;
+    PointProj<T> proj;
+    uint32_t fkind = CF_UNKNOWN, fid = 0;
+    switch (g1.type) {
+        case CUBOID: proj = cuboid_point_feature(g1, c, fkind, fid); break;
+        case CAPSULE: proj = capsule_point(g1, c); break;
+        case CONE: proj = cone_point(g1, c); break;
+        case CONVEX_POLYHEDRON:
+            if (!support_map_point(g1, c, proj, st)) return -1;
+            break;
+        default: return -1;
+    }
+    T dist;
+    V3<T> n1, dir1;
+    T len;
+    if (try_unit(proj.point - c, Eps<T>::eps, dir1, len)) {
+        if (proj.inside) {
+            dist = -len - radius;
+            n1 = dir1;
+        } else {
+            dist = len - radius;
+            n1 = -dir1;
+        }
+    } else {
+        // Only a Cuboid has a feature normal here: a ConvexPolyhedron's feature is
+        // Unknown whenever this branch is taken (point_support_map.rs:66-72 uses the
+        // same threshold), Capsule and Cone have none (shape.rs:425-431).
+        dist = -radius;
+        V3<T> u;
+        T l;
+        if (!(g1.type == CUBOID && cuboid_feature_normal(fkind, fid, n1))) {
+            if (try_unit(proj.point, Eps<T>::eps, u, l)) n1 = u;
+            else n1 = v3(T(0), T(1), T(0));
+        }
+    }
+    if (!(dist <= prediction)) return 0;
+    V3<T> n2 = inv_rotate(pos12.q, -n1);
+    out.point1 = proj.point;
+    out.point2 = n2 * radius;
+    out.normal1 = n1;
+    out.normal2 = n2;
+    out.dist = dist;
+    return 1;
+}
+
+// query::contact (contact_shape_shape.rs:123-138) with the dispatch of
+// DefaultQueryDispatcher::contact (default_query_dispatcher.rs:302-356) for the
+// shapes held here.  Returns 1 on contact, 0 for None, -1 for pairs not handled
+// by this restatement (a Ball with the test-only Triangle) or where parry
+// reaches unreachable!().
 template <class T>
 int contact(const Iso<T> &pos1, const Shape<T> &g1, const Iso<T> &pos2, const Shape<T> &g2, T prediction,
             Contact<T> &out, const V3<T> *init_dir, ContactStats *st) {
@@ -938,8 +1219,21 @@ int contact(const Iso<T> &pos1, const Shape<T> &g1, const Iso<T> &pos2, const Sh
     bool ok;
     if (g1.type == BALL && g2.type == BALL) {
         ok = contact_ball_ball(pos12, g1.p[0], g2.p[0], prediction, out);
-    } else if (g1.type == BALL || g2.type == BALL) {
+    } else if ((g1.type == BALL && g2.type == TRIANGLE) || (g1.type == TRIANGLE && g2.type == BALL)) {
         return -1;
+    } else if (g1.type == BALL) {
+        // contact_ball_convex_polyhedron: the flipped contact of (shape 2, ball) under pos12^-1
+        int r = contact_convex_ball(iso_inverse(pos12), g2, g1.p[0], prediction, out, st);
+        if (r < 0) return -1;
+        ok = r == 1;
+        if (ok) {
+            std::swap(out.point1, out.point2);
+            std::swap(out.normal1, out.normal2);
+        }
+    } else if (g2.type == BALL) {
+        int r = contact_convex_ball(pos12, g1, g2.p[0], prediction, out, st);
+        if (r < 0) return -1;
+        ok = r == 1;
     } else {
         ok = contact_sm_sm(pos12, g1, g2, prediction, out, init_dir, st);
     }

@@ -4,6 +4,11 @@
 // DefaultQueryDispatcher::contact (src/query/default_query_dispatcher.rs:302-356)
 // for the shapes held here, then
 //   * Ball-Ball: the closed form of contact_ball_ball.rs:9-42;
+//   * Ball with a Cuboid, Capsule or Cone: the closed-form projection of the
+//     ball centre (point_aabb.rs, point_capsule.rs, point_cone.rs) and
+//     contact_ball_convex_polyhedron.rs:12-63; with a ConvexPolyhedron, GJK
+//     of the shape against ConstantOrigin (point_support_map.rs:17-51), the
+//     centre inside going to the EPA queue like an intersecting pair;
 //   * support-map pairs: contact_support_map_support_map_with_params
 //     (src/query/contact/contact_support_map_support_map.rs:40-77) -- seed
 //     direction (caller-supplied or parry's normalize(pos12.t) / +x), then GJK
@@ -37,11 +42,17 @@ struct DevContact {
 };
 static_assert(sizeof(DevContact) == sizeof(pq_contact), "pq_contact layout");
 
-// Queued EPA work item: pair index and GJK's final simplex.
+// Queued EPA work item: pair index, what the EPA solves and GJK's final simplex.
+enum : uint32_t {
+    kPairSupportMaps = 0,  // contact_support_map_support_map on (shape 1, shape 2)
+    kBallSecond = 1,       // (ConvexPolyhedron, Ball): project the ball centre
+    kBallFirst = 2,        // (Ball, ConvexPolyhedron): the same, flipped
+};
 struct EpaItem {
     uint32_t pair;
     uint32_t dim;
-    uint32_t pad[2];
+    uint32_t mode;
+    uint32_t pad;
     float v[4][9];  // point, orig1, orig2
 };
 
@@ -52,6 +63,37 @@ __device__ __forceinline__ void finish_contact(const iso &pos1, const iso &pos2,
     const f3 p2 = qrot_inv(pos12.q, sub3(p2_1, pos12.t));
     const f3 n2 = qrot_inv(pos12.q, neg3(n1));
     // Contact::transform_by_mut
+    const f3 w1 = iso_point(pos1, p1), w2 = iso_point(pos2, p2);
+    const f3 m1 = qrot(pos1.q, n1), m2 = qrot(pos2.q, n2);
+    c.p1[0] = w1.x; c.p1[1] = w1.y; c.p1[2] = w1.z;
+    c.p2[0] = w2.x; c.p2[1] = w2.y; c.p2[2] = w2.z;
+    c.n1[0] = m1.x; c.n1[1] = m1.y; c.n1[2] = m1.z;
+    c.n2[0] = m2.x; c.n2[1] = m2.y; c.n2[2] = m2.z;
+    c.dist = dist;
+    c.has = 1;
+}
+
+// Builds the Ball-vs-convex contact (contact_ball_convex_polyhedron.rs:27-63)
+// and maps it to world space; pb = pose of the ball in the convex shape's frame.
+__device__ void finish_ball_contact(const iso &pos1, const iso &pos2, const iso &pb, uint32_t conv_type,
+                                    float radius, float prediction, const PointProj &proj, uint32_t fk, uint32_t fid,
+                                    uint32_t mode, DevContact &c) {
+    f3 p1, p2, n1, n2;
+    float dist;
+    if (!convex_ball_contact(pb, conv_type, radius, prediction, proj, fk, fid, p1, p2, n1, n2, dist)) {
+        for (int q = 0; q < 3; ++q) c.p1[q] = c.p2[q] = c.n1[q] = c.n2[q] = 0.0f;
+        c.dist = 0.0f;
+        c.has = 0;
+        return;
+    }
+    if (mode == kBallFirst) {  // Contact::flipped (contact.rs:156-166)
+        f3 t = p1;
+        p1 = p2;
+        p2 = t;
+        t = n1;
+        n1 = n2;
+        n2 = t;
+    }
     const f3 w1 = iso_point(pos1, p1), w2 = iso_point(pos2, p2);
     const f3 m1 = qrot(pos1.q, n1), m2 = qrot(pos2.q, n2);
     c.p1[0] = w1.x; c.p1[1] = w1.y; c.p1[2] = w1.z;
@@ -92,6 +134,7 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
         if (md < m) m = md;
     }
     bool need_epa = false;
+    uint32_t ball_mode = kPairSupportMaps;
     Simplex s;
     uint32_t pair_id = 0;
     if (k < m) {
@@ -124,7 +167,46 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
             }
             out[k] = c;
         } else if (g1.type == PQ_SHAPE_BALL || g2.type == PQ_SHAPE_BALL) {
-            out[k] = no_contact(kUnsupportedPair);
+            // contact_ball_convex_polyhedron / contact_convex_polyhedron_ball
+            // (default_query_dispatcher.rs:326-335): a Ball first is the flipped
+            // contact of (shape 2, ball) under pos12^-1.
+            ball_mode = g1.type == PQ_SHAPE_BALL ? kBallFirst : kBallSecond;
+            const iso pb = ball_mode == kBallFirst ? iso_inverse(pos12) : pos12;
+            const ShapeRef &cv = ball_mode == kBallFirst ? g2 : g1;
+            const float radius = ball_mode == kBallFirst ? g1.p0 : g2.p0;
+            const f3 ctr = pb.t;
+            PointProj proj{false, f3{0.0f, 0.0f, 0.0f}};
+            uint32_t fk = CF_UNKNOWN, fid = 0;
+            bool have = true;
+            if (cv.type == PQ_SHAPE_CUBOID) {
+                proj = cuboid_point_feature(cv, ctr, fk, fid);
+            } else if (cv.type == PQ_SHAPE_CAPSULE) {
+                proj = capsule_point(cv, ctr);
+            } else if (cv.type == PQ_SHAPE_CONE) {
+                proj = cone_point(cv, ctr);
+            } else {
+                // local_point_projection_on_support_map (point_support_map.rs:17-51)
+                const ShapeRef org = origin_shape();
+                f3 dir;
+                float l;
+                if (!try_unit(ctr, dir, l)) dir = f3{1.0f, 0.0f, 0.0f};
+                sx_reset(s, cso_from_shapes(point_proj_seed_iso(ctr), cv, org, dir));
+                const GjkOut r = gjk_closest_points(point_proj_iso(ctr), cv, org, kFltMax, s);
+                gi = r.iters;
+                if (r.kind == GJK_CLOSEST) {
+                    proj = PointProj{false, r.p1};
+                } else if (r.kind == GJK_INTERSECTION) {
+                    need_epa = true;
+                    have = false;
+                } else {
+                    out[k] = no_contact(0);  // parry: unreachable!() in gjk::project_origin
+                    have = false;
+                }
+            }
+            if (have) {
+                finish_ball_contact(pos1, pos2, pb, cv.type, radius, prediction, proj, fk, fid, ball_mode, c);
+                out[k] = c;
+            }
         } else {
             f3 dir;
             if (seeds) {
@@ -162,6 +244,7 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
             EpaItem &it = queue[base + lanes_below(mask)];
             it.pair = pair_id;
             it.dim = s.dim;
+            it.mode = ball_mode;
             for (int q = 0; q < 4; ++q) {
                 const Cso &v = s.v[q];
                 it.v[q][0] = v.p.x; it.v[q][1] = v.p.y; it.v[q][2] = v.p.z;
@@ -281,7 +364,9 @@ template <int G>
 __device__ __forceinline__ Cso group_cso(const iso &pos12, const ShapeRef &g1, const ShapeRef &g2, const Staged &st,
                                          f3 dir) {
     const f3 a = group_support_staged<G>(g1, st.s1, dir);
-    const f3 b = iso_point(pos12, group_support_staged<G>(g2, st.s2, qrot_inv(pos12.q, neg3(dir))));
+    const f3 b = g2.type == kShapeOrigin
+                     ? pos12.t
+                     : iso_point(pos12, group_support_staged<G>(g2, st.s2, qrot_inv(pos12.q, neg3(dir))));
     return Cso{sub3(a, b), a, b};
 }
 
@@ -632,8 +717,8 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
 
 template <int G, class W>
 __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4 *verts, const float *poses,
-                         const uint32_t *pairs, const EpaItem *it, DevContact *out, uint32_t *iters,
-                         uint32_t *overflow_list, uint32_t *counters, uint32_t slot) {
+                         const uint32_t *pairs, float prediction, const EpaItem *it, DevContact *out,
+                         uint32_t *iters, uint32_t *overflow_list, uint32_t *counters, uint32_t slot) {
     const uint32_t k = it->pair;
     const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
     const iso pos1 = load_iso(poses + 7 * (size_t)i), pos2 = load_iso(poses + 7 * (size_t)j);
@@ -641,6 +726,30 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
     const ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
     f3 p1, p2, n;
     uint32_t ei = 0;
+    const uint32_t mode = it->mode;
+    if (mode != kPairSupportMaps) {
+        // EPA::project_origin of the ball centre on the ConvexPolyhedron
+        // (point_support_map.rs:39-49), then the Ball-vs-convex contact.
+        const iso pb = mode == kBallFirst ? iso_inverse(pos12) : pos12;
+        const ShapeRef &cv = mode == kBallFirst ? g2 : g1;
+        const float radius = mode == kBallFirst ? g1.p0 : g2.p0;
+        const uint32_t st = epa_run<G>(w, orig, point_proj_iso(pb.t), cv, origin_shape(), it, p1, p2, n, ei);
+        if (__lane_id() % G != 0) return;
+        if (st == EPA_OVERFLOW) {
+            if (overflow_list) {
+                overflow_list[atomicAdd(&counters[1], 1u)] = slot;
+                return;
+            }
+            out[k] = no_contact(kUnsupportedPair);  // cannot happen with the large buffers
+            return;
+        }
+        const PointProj proj{true, st == EPA_OK ? p1 : pb.t};
+        DevContact c;
+        finish_ball_contact(pos1, pos2, pb, cv.type, radius, prediction, proj, CF_UNKNOWN, 0, mode, c);
+        out[k] = c;
+        if (iters) iters[2 * k + 1] = ei;
+        return;
+    }
     const uint32_t st = epa_run<G>(w, orig, pos12, g1, g2, it, p1, p2, n, ei);
     if (__lane_id() % G != 0) return;
     if (st == EPA_OVERFLOW) {
@@ -665,7 +774,7 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
 template <int G>
 __global__ __launch_bounds__(64) void epa_lds_kernel(const ShapeRecord *__restrict__ rec,
                                                      const float4 *__restrict__ verts, const float *__restrict__ poses,
-                                                     const uint32_t *__restrict__ pairs,
+                                                     const uint32_t *__restrict__ pairs, float prediction,
                                                      const EpaItem *__restrict__ queue, DevContact *__restrict__ out,
                                                      uint32_t *__restrict__ iters, uint32_t *overflow_list,
                                                      uint32_t *counters, float *orig_scratch) {
@@ -677,14 +786,15 @@ __global__ __launch_bounds__(64) void epa_lds_kernel(const ShapeRecord *__restri
     const uint32_t count = __hip_atomic_load(&counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t stride = gridDim.x * kGroups;
     for (uint32_t q = gid; q < count; q += stride)
-        epa_item<G>(&w[group], orig, rec, verts, poses, pairs, queue + q, out, iters, overflow_list, counters, q);
+        epa_item<G>(&w[group], orig, rec, verts, poses, pairs, prediction, queue + q, out, iters, overflow_list,
+                    counters, q);
 }
 
 // Stage 3: one wave per overflowed pair, polytope in global memory.
 __global__ __launch_bounds__(64) void epa_global_kernel(const ShapeRecord *__restrict__ rec,
                                                         const float4 *__restrict__ verts,
                                                         const float *__restrict__ poses,
-                                                        const uint32_t *__restrict__ pairs,
+                                                        const uint32_t *__restrict__ pairs, float prediction,
                                                         const EpaItem *__restrict__ queue,
                                                         DevContact *__restrict__ out, uint32_t *__restrict__ iters,
                                                         const uint32_t *overflow_list, uint32_t *counters,
@@ -693,11 +803,12 @@ __global__ __launch_bounds__(64) void epa_global_kernel(const ShapeRecord *__res
     float *orig = orig_scratch + (size_t)blockIdx.x * EpaLarge::kV * kOrigFloats;
     const uint32_t count = __hip_atomic_load(&counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (uint32_t q = blockIdx.x; q < count; q += gridDim.x)
-        epa_item<64>(w, orig, rec, verts, poses, pairs, queue + overflow_list[q], out, iters, nullptr, counters, q);
+        epa_item<64>(w, orig, rec, verts, poses, pairs, prediction, queue + overflow_list[q], out, iters, nullptr,
+                     counters, q);
 }
 
 template <int G> hipError_t launch_epa_lds(const DeviceShapes &s, const float *poses, const uint32_t *pairs,
-                                           uint64_t m, const EpaItem *queue, DevContact *out, uint32_t *iters,
+                                           float prediction, uint64_t m, const EpaItem *queue, DevContact *out, uint32_t *iters,
                                            uint32_t *overflow, uint32_t *counters, float *orig, uint64_t max_groups,
                                            hipStream_t stream) {
     hipError_t e;
@@ -712,7 +823,7 @@ template <int G> hipError_t launch_epa_lds(const DeviceShapes &s, const float *p
     if (waves > need) waves = need;
     if (waves * kGroups > max_groups) waves = max_groups / kGroups;
     hipLaunchKernelGGL(epa_lds_kernel<G>, dim3((uint32_t)waves), dim3(64), 0, stream, s.rec, s.verts, poses, pairs,
-                       queue, out, iters, overflow, counters, orig);
+                       prediction, queue, out, iters, overflow, counters, orig);
     return hipGetLastError();
 }
 
@@ -757,21 +868,21 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
     if (e != hipSuccess) return e;
     switch (g_epa_lanes_per_pair) {
         case 8:
-            e = launch_epa_lds<8>(s, poses, pairs, m, queue, (DevContact *)out, iters, overflow, counters, orig_small,
+            e = launch_epa_lds<8>(s, poses, pairs, prediction, m, queue, (DevContact *)out, iters, overflow, counters, orig_small,
                                   kEpaMaxGroups, stream);
             break;
         case 16:
-            e = launch_epa_lds<16>(s, poses, pairs, m, queue, (DevContact *)out, iters, overflow, counters,
+            e = launch_epa_lds<16>(s, poses, pairs, prediction, m, queue, (DevContact *)out, iters, overflow, counters,
                                    orig_small, kEpaMaxGroups, stream);
             break;
         default:
-            e = launch_epa_lds<64>(s, poses, pairs, m, queue, (DevContact *)out, iters, overflow, counters,
+            e = launch_epa_lds<64>(s, poses, pairs, prediction, m, queue, (DevContact *)out, iters, overflow, counters,
                                    orig_small, kEpaMaxGroups, stream);
             break;
     }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(epa_global_kernel, dim3(kEpaGlobalWaves), dim3(64), 0, stream, s.rec, s.verts, poses, pairs,
-                       queue, (DevContact *)out, iters, overflow, counters, scratch, orig_large);
+                       prediction, queue, (DevContact *)out, iters, overflow, counters, scratch, orig_large);
     return hipGetLastError();
 }
 

@@ -25,6 +25,9 @@ namespace pq {
 constexpr float kEps = 1.1920929e-07f;      // f32::EPSILON = DEFAULT_EPSILON (src/lib.rs:102)
 constexpr float kEpsTol = kEps * 10.0f;     // gjk::eps_tol (gjk.rs:141-144)
 constexpr float kFltMax = 3.40282347e+38f;  // f32::MAX
+// parry's ConstantOrigin (src/query/gjk/special_support_maps.rs:32-54): the
+// second operand of GJK/EPA point projections.  Internal, never in a shape set.
+constexpr uint32_t kShapeOrigin = 6u;
 
 struct ShapeRef {
     uint32_t type;
@@ -59,6 +62,7 @@ __device__ __forceinline__ f3 local_support(const ShapeRef &s, f3 d) {
         return mul3(div3(d, l), s.p0);
     }
     if (s.type == PQ_SHAPE_CUBOID) return f3{copysignf(s.p0, d.x), copysignf(s.p1, d.y), copysignf(s.p2, d.z)};
+    if (s.type == kShapeOrigin) return f3{0.0f, 0.0f, 0.0f};
     if (s.type == PQ_SHAPE_CAPSULE) {
         const float sq = norm_sq3(d);
         f3 u = sq > 0.0f ? div3(d, sqrtf(sq)) : f3{0.0f, 1.0f, 0.0f};
@@ -80,10 +84,11 @@ struct Cso {
     f3 p, o1, o2;
 };
 
-// CSOPoint::from_shapes: s1(dir) - pos12 * s2(pos12^-1 * -dir).
+// CSOPoint::from_shapes: s1(dir) - pos12 * s2(pos12^-1 * -dir); ConstantOrigin's
+// support_point is pos12's translation (special_support_maps.rs:36-38).
 __device__ __forceinline__ Cso cso_from_shapes(const iso &pos12, const ShapeRef &g1, const ShapeRef &g2, f3 dir) {
     const f3 a = local_support(g1, dir);
-    const f3 b = iso_point(pos12, local_support(g2, qrot_inv(pos12.q, neg3(dir))));
+    const f3 b = g2.type == kShapeOrigin ? pos12.t : iso_point(pos12, local_support(g2, qrot_inv(pos12.q, neg3(dir))));
     return Cso{sub3(a, b), a, b};
 }
 
@@ -491,6 +496,244 @@ __device__ __noinline__ GjkOut gjk_closest_points(const iso &pos12, const ShapeR
             return out;
         }
     }
+}
+
+
+// ---------------------------------------------------------------------------
+// Ball against a convex shape (contact_ball_convex_polyhedron.rs:12-63): the
+// point projections (solid = false) of the ball centre and the contact built
+// from them.  A ConvexPolyhedron is projected by GJK (and EPA when the centre
+// is inside) against ConstantOrigin; the caller runs those.
+// ---------------------------------------------------------------------------
+struct PointProj {
+    bool inside;
+    f3 point;
+};
+enum : uint32_t { CF_UNKNOWN = 0, CF_VERTEX = 1, CF_EDGE = 2, CF_FACE = 3 };
+
+// approx relative_eq!(a, b) with epsilon = max_relative = f32::EPSILON.
+__device__ __forceinline__ bool rel_eq1(float a, float b) {
+    if (a == b) return true;
+    if (isinf(a) || isinf(b)) return false;
+    const float d = fabsf(a - b);
+    if (d <= kEps) return true;
+    const float fa = fabsf(a), fb = fabsf(b);
+    const float largest = fb > fa ? fb : fa;
+    return d <= largest * kEps;
+}
+__device__ __forceinline__ bool rel_eq3(f3 a, f3 b) { return rel_eq1(a.x, b.x) && rel_eq1(a.y, b.y) && rel_eq1(a.z, b.z); }
+
+// Unit::try_new_and_get(v, EPS): |v|^2 > EPS^2.
+__device__ __forceinline__ bool try_unit(f3 v, f3 &u, float &len) {
+    const float sq = norm_sq3(v);
+    if (!(sq > kEps * kEps)) return false;
+    len = sqrtf(sq);
+    u = div3(v, len);
+    return true;
+}
+
+__device__ __forceinline__ float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+__device__ __forceinline__ void set_comp(f3 &v, int i, float x) {
+    if (i == 0) v.x = x;
+    else if (i == 1) v.y = x;
+    else v.z = x;
+}
+
+// Segment::project_local_point (point_segment.rs:45-85) with relative_eq!(proj, pt).
+__device__ __forceinline__ PointProj segment_point(f3 a, f3 b, f3 pt) {
+    const f3 ab = sub3(b, a), ap = sub3(pt, a);
+    const float ab_ap = dot3(ab, ap), sqnab = norm_sq3(ab);
+    f3 p;
+    if (ab_ap <= 0.0f) p = a;
+    else if (ab_ap >= sqnab) p = b;
+    else p = add3(a, mul3(ab, ab_ap / sqnab));
+    return PointProj{rel_eq3(p, pt), p};
+}
+
+// Aabb(-he, he)::project_local_point_and_get_feature (point_aabb.rs:9-132).
+__device__ PointProj cuboid_point_feature(const ShapeRef &s, f3 pt, uint32_t &fk, uint32_t &fid) {
+    const f3 maxs = f3{s.p0, s.p1, s.p2}, mins = neg3(maxs);
+    const f3 mins_pt = sub3(mins, pt), pt_maxs = sub3(pt, maxs);
+    const f3 zero = f3{0.0f, 0.0f, 0.0f};
+    f3 shift = sub3(max3(mins_pt, zero), max3(pt_maxs, zero));
+    const bool inside = shift.x == 0.0f && shift.y == 0.0f && shift.z == 0.0f;
+    if (inside) {
+        float best = -kFltMax;
+        bool is_mins = false;
+        int best_id = 0;
+        for (int i = 0; i < 3; ++i) {
+            const float a = comp(mins_pt, i), b = comp(pt_maxs, i);
+            if (a < b) {
+                if (b > best) {
+                    best_id = i;
+                    is_mins = false;
+                    best = b;
+                }
+            } else if (a > best) {
+                best_id = i;
+                is_mins = true;
+                best = a;
+            }
+        }
+        shift = zero;
+        set_comp(shift, best_id, is_mins ? best : -best);
+    }
+    const f3 ls = add3(pt, shift);
+    int nzero = 0, last_zero = 0, last_nz = 0;
+    for (int i = 0; i < 3; ++i) {
+        if (comp(shift, i) == 0.0f) {
+            ++nzero;
+            last_zero = i;
+        } else {
+            last_nz = i;
+        }
+    }
+    const f3 center = mul3(add3(mins, maxs), 0.5f);
+    fk = CF_UNKNOWN;
+    fid = 0;
+    if (nzero == 3) {
+        for (int i = 0; i < 3; ++i) {
+            if (comp(ls, i) > comp(maxs, i) - kEps) {
+                fk = CF_FACE;
+                fid = (uint32_t)i;
+                break;
+            }
+            if (comp(ls, i) <= comp(mins, i) + kEps) {
+                fk = CF_FACE;
+                fid = (uint32_t)(i + 3);
+                break;
+            }
+        }
+    } else if (nzero == 2) {
+        fk = CF_FACE;
+        fid = comp(ls, last_nz) < comp(center, last_nz) ? (uint32_t)(last_nz + 3) : (uint32_t)last_nz;
+    } else {
+        uint32_t id = 0;
+        for (int i = 0; i < 3; ++i)
+            if (comp(ls, i) < comp(center, i)) id |= 1u << i;
+        if (nzero == 0) {
+            fk = CF_VERTEX;
+            fid = id;
+        } else {
+            fk = CF_EDGE;
+            fid = (id << 2) | (uint32_t)last_zero;
+        }
+    }
+    return PointProj{inside, ls};
+}
+
+// Cuboid::feature_normal (cuboid.rs:401-448).
+__device__ bool cuboid_feature_normal(uint32_t fk, uint32_t id, f3 &n) {
+    n = f3{0.0f, 0.0f, 0.0f};
+    if (fk == CF_FACE) {
+        if (id < 3) set_comp(n, (int)id, 1.0f);
+        else set_comp(n, (int)id - 3, -1.0f);
+        return true;
+    }
+    if (fk == CF_EDGE) {
+        const uint32_t edge = id & 3u, f1 = (edge + 1) % 3, f2 = (edge + 2) % 3, signs = id >> 2;
+        set_comp(n, (int)f1, (signs & (1u << f1)) ? -1.0f : 1.0f);
+        set_comp(n, (int)f2, (signs & (1u << f2)) ? -1.0f : 1.0f);
+        n = div3(n, sqrtf(norm_sq3(n)));
+        return true;
+    }
+    if (fk == CF_VERTEX) {
+        for (int i = 0; i < 3; ++i) set_comp(n, i, (id & (1u << i)) ? -1.0f : 1.0f);
+        n = div3(n, sqrtf(norm_sq3(n)));
+        return true;
+    }
+    return false;
+}
+
+// Capsule::project_local_point (point_capsule.rs:9-50), solid = false.
+__device__ PointProj capsule_point(const ShapeRef &s, f3 pt) {
+    const f3 a = f3{0.0f, -s.p0, 0.0f}, b = f3{0.0f, s.p0, 0.0f};
+    const float r = s.p1;
+    const PointProj proj = segment_point(a, b, pt);
+    f3 dir;
+    float dist;
+    if (try_unit(sub3(pt, proj.point), dir, dist)) return PointProj{dist <= r, add3(proj.point, mul3(dir, r))};
+    f3 sd;
+    float sl;
+    if (try_unit(sub3(b, a), sd, sl)) {
+        // WBasis::orthonormal_basis()[0] (utils/wops.rs:96-110)
+        const float sign = copysignf(1.0f, sd.z);
+        const float aa = -1.0f / (sign + sd.z);
+        const float bb = sd.x * sd.y * aa;
+        const f3 e0 = f3{1.0f + sign * sd.x * sd.x * aa, sign * bb, -sign * sd.x};
+        return PointProj{true, add3(proj.point, mul3(e0, r))};
+    }
+    return PointProj{true, add3(proj.point, f3{0.0f, r, 0.0f})};
+}
+
+// Cone::project_local_point (point_cone.rs:8-63), solid = false.
+__device__ PointProj cone_point(const ShapeRef &s, f3 pt) {
+    const float h = s.p0, r = s.p1;
+    float dx = pt.x, dz = pt.z;
+    const float planar = sqrtf(dx * dx + dz * dz);
+    dx = dx / planar;
+    dz = dz / planar;
+    if (planar <= kEps) {
+        dx = 1.0f;
+        dz = 0.0f;
+    }
+    const f3 on_basis = f3{pt.x, -h, pt.z};
+    if (pt.y < -h && planar <= r) return PointProj{false, on_basis};
+    const f3 circle = f3{dx * r, -h, dz * r};
+    const f3 apex = f3{0.0f, h, 0.0f};
+    const f3 side_dir = sub3(circle, apex);
+    PointProj proj = segment_point(apex, circle, pt);
+    const f3 apex_to_basis = f3{0.0f, -2.0f * h, 0.0f};
+    if (pt.y >= -h && pt.y <= h &&
+        dot3(cross3(side_dir, sub3(pt, apex)), cross3(side_dir, apex_to_basis)) >= 0.0f) {
+        const f3 d1 = sub3(proj.point, pt), d2 = sub3(on_basis, pt);
+        if (norm_sq3(d1) > norm_sq3(d2)) return PointProj{true, on_basis};
+        proj.inside = true;
+    }
+    return proj;
+}
+
+// The isometries of local_point_projection_on_support_map (point_support_map.rs:26-29):
+// the initial CSO point uses m_inv = (I, c), GJK/EPA run on m.inverse() with m = (I, -c).
+__device__ __forceinline__ iso point_proj_seed_iso(f3 c) { return iso{quat{0.0f, 0.0f, 0.0f, 1.0f}, c}; }
+__device__ __forceinline__ iso point_proj_iso(f3 c) {
+    return iso_inverse(iso{quat{0.0f, 0.0f, 0.0f, 1.0f}, neg3(c)});
+}
+__device__ __forceinline__ ShapeRef origin_shape() { return ShapeRef{kShapeOrigin, 0, nullptr, 0.0f, 0.0f, 0.0f}; }
+
+// contact_convex_polyhedron_ball (contact_ball_convex_polyhedron.rs:27-63) in the
+// convex shape's frame: pos12 maps the ball's frame into the convex shape's.
+// Returns false for None.
+__device__ bool convex_ball_contact(const iso &pos12, uint32_t conv_type, float radius, float prediction,
+                                    const PointProj &proj, uint32_t fk, uint32_t fid, f3 &p1, f3 &p2, f3 &n1, f3 &n2,
+                                    float &dist) {
+    const f3 c = pos12.t;
+    f3 dir1;
+    float len;
+    if (try_unit(sub3(proj.point, c), dir1, len)) {
+        if (proj.inside) {
+            dist = -len - radius;
+            n1 = dir1;
+        } else {
+            dist = len - radius;
+            n1 = neg3(dir1);
+        }
+    } else {
+        // Only a Cuboid has a feature normal here: a ConvexPolyhedron's feature is
+        // Unknown whenever this branch is taken (same threshold, point_support_map.rs:66-72);
+        // Capsule and Cone have none (shape.rs:425-431).
+        dist = -radius;
+        if (!(conv_type == PQ_SHAPE_CUBOID && cuboid_feature_normal(fk, fid, n1))) {
+            f3 u;
+            float l;
+            n1 = try_unit(proj.point, u, l) ? u : f3{0.0f, 1.0f, 0.0f};
+        }
+    }
+    if (!(dist <= prediction)) return false;
+    n2 = qrot_inv(pos12.q, neg3(n1));
+    p2 = mul3(n2, radius);
+    p1 = proj.point;
+    return true;
 }
 
 }  // namespace pq

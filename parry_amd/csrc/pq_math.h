@@ -64,6 +64,11 @@ __host__ __device__ __forceinline__ iso iso_inv_mul(const iso &a, const iso &b) 
     return iso{qmul(ia, b.q), qrot(ia, sub3(b.t, a.t))};
 }
 
+// Isometry::inverse: (q^-1, q^-1 * -t).
+__host__ __device__ __forceinline__ iso iso_inverse(const iso &m) {
+    const quat qi = qconj(m.q);
+    return iso{qi, qrot(qi, neg3(m.t))};
+}
 __device__ __forceinline__ iso load_iso(const float *p) {
     return iso{quat{p[0], p[1], p[2], p[3]}, f3{p[4], p[5], p[6]}};
 }

@@ -156,3 +156,47 @@ def test_ball_ball_and_empty(orc):
     _compare("balls", gpu, has, ref)
     gpu, has, _ = _run_gpu(shapes, poses, np.zeros((0, 2), np.uint32), 0.01)
     assert gpu.shape[0] == 0
+
+
+def test_ball_convex_pairs(orc):
+    """Ball with Cuboid / Capsule / Cone / ConvexPolyhedron in both orders, centres
+    inside and outside (polyhedra inside go through EPA on ConstantOrigin), plus
+    centres exactly on a cuboid face, corner and a capsule axis."""
+    descs, verts = scenes.mixed_shapes(3000, seed=31)
+    rng = np.random.default_rng(32)
+    balls = np.nonzero(descs["type"] == 0)[0]
+    others = np.nonzero(descs["type"] != 0)[0]
+    m = min(len(balls), len(others))
+    i, j = rng.permutation(balls)[:m], rng.permutation(others)[:m]
+    q = scenes.random_rotations(len(descs), rng)
+    t = np.zeros((len(descs), 3))
+    t[j] = rng.uniform(-2.0, 2.0, size=(m, 3))
+    poses = np.concatenate([q, t], 1).astype(np.float32)
+    pairs = np.stack([i, j], axis=1).astype(np.uint32)
+    pairs[::2] = pairs[::2, ::-1]
+    # degenerate centres: on a cuboid face / corner, on a capsule axis
+    extra_d = np.zeros(6, SHAPE_DESC_DTYPE)
+    extra_d["type"] = [1, 0, 1, 0, 2, 0]
+    extra_d["params"][:, :3] = [[1.0, 0.5, 0.5], [0.25, 0, 0], [1.0, 0.5, 0.5], [0.25, 0, 0], [1.0, 0.5, 0],
+                                [0.25, 0, 0]]
+    ident = [0, 0, 0, 1]
+    extra_p = np.array([ident + [0, 0, 0], ident + [1.0, 0.25, -0.125], ident + [0, 0, 0], ident + [-1.0, 0.5, 0.5],
+                        ident + [0, 0, 0], ident + [0, 0.25, 0]], np.float32)
+    base = len(descs)
+    descs = np.concatenate([descs, extra_d])
+    poses = np.concatenate([poses, extra_p])
+    pairs = np.concatenate([pairs, base + np.array([[0, 1], [3, 2], [4, 5]], np.uint32)])
+    shapes = ShapeSet(descs, verts)
+    print(f"case: ball-convex pairs, {pairs.shape[0]} pairs", flush=True)
+    gpu, has, iters = _run_gpu(shapes, poses, pairs, 0.01)
+    ref = orc.contact_pairs(descs, verts, poses, pairs, 0.01)
+    assert (ref["status"] >= 0).all()
+    c = _compare("ball-convex", gpu, has, ref)
+    assert c.sum() > 100
+    assert np.array_equal(iters[:, 0], ref["iters"][:, 0].astype(np.int32))
+    poly = (descs["type"][pairs[:, 0]] == 4) | (descs["type"][pairs[:, 1]] == 4)
+    epa = ref["iters"][:, 1] > 0
+    print(f"ball-convex: {c.sum()} contacts, {(poly & epa).sum()} polyhedron centres through EPA")
+    assert (poly & epa).sum() > 5
+    assert gpu[-3, 12] == np.float32(-0.25) and tuple(gpu[-3, 6:9]) == (1.0, 0.0, 0.0)
+    assert gpu[-1, 12] == np.float32(-0.75)
