@@ -187,6 +187,105 @@ def batch_leg(args, log):
     return rec
 
 
+def learned_leg(args, log):
+    """Learned GJK warm start vs the plain GJK on 1,048,576 mixed convex pairs
+    (ConvexPolyhedron 8-64 points, Cuboid, Ball, Capsule, Cone; the overlapping
+    AABB pairs of a 262,144-shape scene, seed 3), prediction 0.01.  One step =
+    query::contact on every pair; the learned step adds the fused seed kernel.
+    A "perfect seed" run (the plain GJK's own final direction as the seed)
+    bounds what any seed can gain."""
+    import numpy as np
+    import torch
+
+    from parry_amd import Bvh, ShapeSet, contact_pairs, contact_workspace_bytes, scenes
+    from parry_amd.learned import WEIGHTS_FILE, WarmStartModel, gjk_directions, support_map_pairs
+
+    if not os.path.exists(WEIGHTS_FILE):
+        return {"skipped": f"no trained parameters at {os.path.relpath(WEIGHTS_FILE, ROOT)}"}
+    sc = scenes.mixed_scene(262144, seed=3)
+    shapes = ShapeSet(sc["descs"], sc["vertices"])
+    poses = torch.from_numpy(sc["poses"]).cuda()
+    pairs = Bvh(shapes.compute_aabbs(poses).cpu().numpy()).self_pairs()
+    pairs = pairs[: 1 << 20].contiguous()
+    m = pairs.shape[0]
+    net = WarmStartModel.pretrained()
+    desc = shapes.descriptors()
+    stream = torch.cuda.current_stream()
+    out = torch.empty((m, 14), dtype=torch.float32, device="cuda")
+    ws = torch.empty(contact_workspace_bytes(m), dtype=torch.uint8, device="cuda")
+    seeds = torch.empty((m, 3), dtype=torch.float32, device="cuda")
+
+    def run(seed_fn, steps):
+        for _ in range(args.warmup):
+            seed_fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        t_seed = t_contact = 0.0
+        for _ in range(steps):
+            ev[0].record(stream)
+            s = seed_fn()
+            ev[1].record(stream)
+            contact_pairs(shapes, poses, pairs, 0.01, init_dirs=s, out=out, workspace=ws)
+            ev[2].record(stream)
+            torch.cuda.synchronize()
+            t_seed += ev[0].elapsed_time(ev[1])
+            t_contact += ev[1].elapsed_time(ev[2])
+        return t_seed / steps, t_contact / steps
+
+    steps = max(3, args.steps // 2)
+    plain_ms = run(lambda: None, steps)
+    learned_ms = run(lambda: net.seeds(poses, pairs, desc, out=seeds), steps)
+    # perfect seeds: the plain GJK's own directions (zero for pairs with a Ball)
+    perfect, valid = gjk_directions(shapes, poses, pairs)
+    perfect[~valid] = 0.0
+    keep = torch.from_numpy(support_map_pairs(sc["descs"], pairs.cpu().numpy().view(np.uint32))).cuda()
+    perfect[~keep] = 0.0
+    perfect_ms = run(lambda: perfect, steps)
+
+    def stats(s):
+        c, it = contact_pairs(shapes, poses, pairs, 0.01, init_dirs=s, with_iters=True, workspace=ws)
+        it = it.cpu().numpy()
+        return c.cpu().numpy(), it
+
+    cp, ip = stats(None)
+    cl, il = stats(net.seeds(poses, pairs, desc))
+    _, iq = stats(perfect)
+    hp, hl = cp.view(np.uint32)[:, 13], cl.view(np.uint32)[:, 13]
+    both = (hp == 1) & (hl == 1)
+    derr = np.abs(cp[both, 12] - cl[both, 12]) / np.maximum(1.0, np.abs(cp[both, 12]))
+    km = keep.cpu().numpy()
+
+    def rate(ms):
+        return round(m / ((ms[0] + ms[1]) * 1e-3), 1)
+
+    return {
+        "metric": "contact pairs/s, learned warm start vs plain GJK",
+        "workload": f"{m} overlapping-AABB pairs of 262144 mixed convex shapes (5 types uniform), seed 3, "
+                    "prediction 0.01",
+        "pairs": m,
+        "seeded_pairs": int(km.sum()),
+        "plain": {"pairs_per_s": rate(plain_ms), "contact_ms": round(plain_ms[1], 4)},
+        "learned": {"pairs_per_s": rate(learned_ms), "seed_kernel_ms": round(learned_ms[0], 4),
+                    "contact_ms": round(learned_ms[1], 4)},
+        "perfect_seed_bound": {"pairs_per_s": round(m / (perfect_ms[1] * 1e-3), 1),
+                               "contact_ms": round(perfect_ms[1], 4)},
+        "speedup_learned_vs_plain": round(rate(plain_ms) and rate(learned_ms) / rate(plain_ms), 4),
+        "mean_gjk_iterations_seeded_pairs": {"plain": round(float(ip[km, 0].mean()), 4),
+                                             "learned": round(float(il[km, 0].mean()), 4),
+                                             "perfect_seed": round(float(iq[km, 0].mean()), 4)},
+        "mean_epa_iterations_seeded_pairs": {"plain": round(float(ip[km, 1].mean()), 4),
+                                             "learned": round(float(il[km, 1].mean()), 4)},
+        "agreement_with_plain": {"has_contact_mismatches": int((hp != hl).sum()),
+                                 "dist_rel_diff_p99": float(np.quantile(derr, 0.99)) if derr.size else 0.0,
+                                 "dist_rel_diff_max": float(derr.max()) if derr.size else 0.0,
+                                 "within_1e-5": float((derr <= 1e-5).mean()) if derr.size else 1.0},
+        "mlp_roofline": {"bound": "mfma", "flops_per_pair": 2 * (128 * 64 + 2 * 128 * 128 + 16 * 128),
+                         "achieved_tflops": round(m * 2 * (128 * 64 + 2 * 128 * 128 + 16 * 128)
+                                                  / (learned_ms[0] * 1e-3) / 1e12, 2),
+                         "peak_tflops": 2500.0},
+    }
+
+
 def contact_leg(args, log):
     """65,536 ConvexPolyhedra (8-64 hull points), mean ~8 AABB overlaps per shape:
     one step = leaf AABBs + Bvh refit + overlapping pairs + query::contact
@@ -292,6 +391,8 @@ def main() -> int:
     ap.add_argument("--skip-contact", action="store_true", help="measure only the ray-cast line")
     ap.add_argument("--contact-only", action="store_true", help="profiling aid: run only the contact leg")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched multi-mesh leg")
+    ap.add_argument("--skip-learned", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--learned-only", action="store_true", help="profiling aid: run only the learned leg")
     ap.add_argument("--batch-only", action="store_true", help="profiling aid: run only the batched multi-mesh leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -311,6 +412,9 @@ def main() -> int:
     torch.cuda.set_device(dev)
     if args.contact_only:
         print(json.dumps({"contact": contact_leg(args, log)}), flush=True)
+        return 0
+    if args.learned_only:
+        print(json.dumps({"learned": learned_leg(args, log)}), flush=True)
         return 0
     if args.batch_only:
         print(json.dumps({"batch": batch_leg(args, log)}), flush=True)
@@ -394,6 +498,9 @@ def main() -> int:
     if not args.skip_batch:
         log("batched multi-mesh leg ...")
         rec["batch"] = batch_leg(args, log)
+    if not args.skip_learned:
+        log("learned warm-start leg ...")
+        rec["learned"] = learned_leg(args, log)
     if not args.skip_contact:
         log("contact leg (broadphase + narrowphase frame) ...")
         rec["contact"] = contact_leg(args, log)

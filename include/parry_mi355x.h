@@ -317,6 +317,63 @@ int pq_contact_pairs_device_count(const pq_shapes *shapes, const pq_isometry *po
                                   const float *init_dirs, pq_contact *out, uint32_t *iters, void *workspace,
                                   void *stream);
 
+/* ---- Learned GJK warm start (no counterpart in parry) ------------------------
+ * A 44-128-128-128-3 MLP (ReLU, bf16 weights on MFMA with f32 accumulation)
+ * predicts the GJK seed direction that pq_contact_pairs takes as `init_dirs`
+ * (the init_dir slot of contact_support_map_support_map_with_params,
+ * contact_support_map_support_map.rs:52-60).  The seed only starts the simplex:
+ * GJK still stops at parry's tolerance.  A zero seed means parry's default seed,
+ * and a seeded GJK that reaches the 100-iteration cap is re-run with the
+ * default seed.
+ *   descriptors: 16 floats per shape (pq_shape_descriptors);
+ *   features:    PQ_MLP_FEATURE_STRIDE floats per pair: the rotation matrix
+ *                (row-major) and translation of pos_i^-1 * pos_j, then the
+ *                descriptors of shapes i and j, then 4 zeros;
+ *   parameters:  PQ_MLP_NUM_PARAMS floats: W1 (128x44, row-major), b1, W2
+ *                (128x128), b2, W3, b3, W4 (3x128), b4.
+ * All buffers except `params` / `epoch_losses` are device pointers. */
+#define PQ_MLP_NUM_PARAMS 39171u
+#define PQ_MLP_FEATURES 44u
+#define PQ_MLP_FEATURE_STRIDE 48u
+#define PQ_SHAPE_DESCRIPTOR_FLOATS 16u
+typedef struct pq_mlp pq_mlp;
+typedef struct pq_mlp_train_config {
+    uint32_t epochs;
+    uint32_t batch;    /* multiple of 32, at most 65536 */
+    float lr;          /* Adam learning rate, cosine-decayed to lr_final */
+    float lr_final;
+    float beta1, beta2, eps;
+    uint32_t reserved; /* must be 0 */
+    uint64_t shuffle_seed;
+} pq_mlp_train_config;
+
+/* params: host, PQ_MLP_NUM_PARAMS floats, or NULL for a He initialisation from `seed`. */
+int pq_mlp_create(const float *params, uint64_t seed, pq_mlp **out);
+int pq_mlp_destroy(pq_mlp *net);
+int pq_mlp_get_params(const pq_mlp *net, float *params);
+int pq_mlp_set_params(pq_mlp *net, const float *params); /* also resets the Adam state */
+int pq_shape_descriptors(const pq_shapes *shapes, float *descriptors, void *stream);
+int pq_pair_features(const pq_isometry *poses, const uint32_t *pairs, uint64_t num_pairs, const float *descriptors,
+                     float *features, void *stream);
+/* Raw network outputs (n x 3) for feature rows. */
+int pq_mlp_forward(const pq_mlp *net, const float *features, uint64_t n, float *out, void *stream);
+/* Features + network + normalisation fused: unit seeds (num_pairs x 3; zero where
+ * the output is degenerate), ready for pq_contact_pairs' init_dirs. */
+int pq_contact_seeds(const pq_mlp *net, const pq_isometry *poses, const uint32_t *pairs, uint64_t num_pairs,
+                     const float *descriptors, float *seeds, void *stream);
+/* Loss gradient over feature rows `rows[0..count)` with unit labels (3 per row):
+ * grads (PQ_MLP_NUM_PARAMS floats), loss_sum (one double: the summed 1 - cos). */
+int pq_mlp_gradients(pq_mlp *net, const float *features, const float *labels, const uint32_t *rows, uint32_t count,
+                     float *grads, double *loss_sum, void *stream);
+/* Adam training on n rows; epoch_losses (host, nullable): mean loss per epoch.
+ * Synchronises `stream`. */
+int pq_mlp_train(pq_mlp *net, const float *features, const float *labels, uint64_t n, const pq_mlp_train_config *cfg,
+                 double *epoch_losses, void *stream);
+/* normal1 of each contact rotated into shape i's frame (zero where has_contact != 1):
+ * the plain GJK's direction, the label the network is trained on. */
+int pq_contact_local_normals(const pq_contact *contacts, const pq_isometry *poses, const uint32_t *pairs,
+                             uint64_t num_pairs, float *normals, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

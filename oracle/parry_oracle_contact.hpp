@@ -876,19 +876,29 @@ struct ContactStats {
 template <class T>
 GjkResult<T> contact_sm_sm_with_params(const Iso<T> &pos12, const Shape<T> &g1, const Shape<T> &g2, T prediction,
                                        Simplex<T> &simplex, const V3<T> *init_dir, ContactStats *st) {
-    V3<T> dir;
-    if (init_dir) {
-        dir = *init_dir;
-    } else {
+    const T e = Eps<T>::eps;
+    V3<T> dflt;
+    {
         V3<T> t = pos12.t;
         T sq = dot(t, t);
-        const T e = Eps<T>::eps;
-        if (sq > e * e) dir = t / std::sqrt(sq);
-        else dir = v3(T(1), T(0), T(0));
+        if (sq > e * e) dflt = t / std::sqrt(sq);
+        else dflt = v3(T(1), T(0), T(0));
     }
+    // A given seed replaces the default; a zero (or NaN) seed means the default, and
+    // a seeded run that reaches the 100-iteration cap is repeated unseeded (the
+    // warm-start contract of this build; parry takes any Unit direction).
+    bool seeded = init_dir && dot(*init_dir, *init_dir) > e * e;
+    V3<T> dir = seeded ? *init_dir : dflt;
     simplex.reset(cso_from_shapes(pos12, g1, g2, dir));
     uint32_t gi = 0;
     GjkResult<T> c = gjk_closest_points(pos12, g1, g2, prediction, simplex, &gi);
+    if (seeded && gi == 100) {
+        uint32_t gi2 = 0;
+        simplex = Simplex<T>();
+        simplex.reset(cso_from_shapes(pos12, g1, g2, dflt));
+        c = gjk_closest_points(pos12, g1, g2, prediction, simplex, &gi2);
+        gi += gi2;
+    }
     if (st) st->gjk_iters = gi;
     if (c.kind != GJK_INTERSECTION) return c;
     Epa<T> epa;

@@ -69,6 +69,17 @@ SIGNATURES = {
     "pq_contact_get_tuning": (_int, [_vp]),
     "pq_contact_pairs": (_int, [_vp, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp]),
     "pq_contact_pairs_device_count": (_int, [_vp, _vp, _vp, _u64, _vp, _f32, _vp, _vp, _vp, _vp, _vp]),
+    "pq_mlp_create": (_int, [_vp, _u64, _c.POINTER(_vp)]),
+    "pq_mlp_destroy": (_int, [_vp]),
+    "pq_mlp_get_params": (_int, [_vp, _vp]),
+    "pq_mlp_set_params": (_int, [_vp, _vp]),
+    "pq_shape_descriptors": (_int, [_vp, _vp, _vp]),
+    "pq_pair_features": (_int, [_vp, _vp, _u64, _vp, _vp, _vp]),
+    "pq_mlp_forward": (_int, [_vp, _vp, _u64, _vp, _vp]),
+    "pq_contact_seeds": (_int, [_vp, _vp, _vp, _u64, _vp, _vp, _vp]),
+    "pq_mlp_gradients": (_int, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp]),
+    "pq_mlp_train": (_int, [_vp, _vp, _vp, _u64, _vp, _vp, _vp]),
+    "pq_contact_local_normals": (_int, [_vp, _vp, _vp, _u64, _vp, _vp]),
 }
 
 _lock = threading.Lock()
@@ -93,6 +104,13 @@ class ContactTuning(ctypes.Structure):
 
 def set_contact_tuning(epa_lanes_per_pair: int) -> None:
     check(lib().pq_contact_set_tuning(ctypes.byref(ContactTuning(epa_lanes_per_pair))))
+
+
+class MlpTrainConfig(_c.Structure):
+    """pq_mlp_train_config."""
+
+    _fields_ = [("epochs", _u32), ("batch", _u32), ("lr", _f32), ("lr_final", _f32), ("beta1", _f32),
+                ("beta2", _f32), ("eps", _f32), ("reserved", _u32), ("shuffle_seed", _u64)]
 
 
 class NativeError(RuntimeError):

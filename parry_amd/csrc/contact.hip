@@ -56,52 +56,15 @@ struct EpaItem {
     float v[4][9];  // point, orig1, orig2
 };
 
-__device__ __forceinline__ void finish_contact(const iso &pos1, const iso &pos2, const iso &pos12, f3 p1, f3 p2_1,
-                                               f3 n1, DevContact &c) {
-    // contact_support_map_support_map (contact_support_map_support_map.rs:18-26)
-    const float dist = dot3(sub3(p2_1, p1), n1);
-    const f3 p2 = qrot_inv(pos12.q, sub3(p2_1, pos12.t));
-    const f3 n2 = qrot_inv(pos12.q, neg3(n1));
-    // Contact::transform_by_mut
-    const f3 w1 = iso_point(pos1, p1), w2 = iso_point(pos2, p2);
-    const f3 m1 = qrot(pos1.q, n1), m2 = qrot(pos2.q, n2);
-    c.p1[0] = w1.x; c.p1[1] = w1.y; c.p1[2] = w1.z;
-    c.p2[0] = w2.x; c.p2[1] = w2.y; c.p2[2] = w2.z;
-    c.n1[0] = m1.x; c.n1[1] = m1.y; c.n1[2] = m1.z;
-    c.n2[0] = m2.x; c.n2[1] = m2.y; c.n2[2] = m2.z;
-    c.dist = dist;
-    c.has = 1;
-}
-
-// Builds the Ball-vs-convex contact (contact_ball_convex_polyhedron.rs:27-63)
-// and maps it to world space; pb = pose of the ball in the convex shape's frame.
-__device__ void finish_ball_contact(const iso &pos1, const iso &pos2, const iso &pb, uint32_t conv_type,
-                                    float radius, float prediction, const PointProj &proj, uint32_t fk, uint32_t fid,
-                                    uint32_t mode, DevContact &c) {
-    f3 p1, p2, n1, n2;
-    float dist;
-    if (!convex_ball_contact(pb, conv_type, radius, prediction, proj, fk, fid, p1, p2, n1, n2, dist)) {
-        for (int q = 0; q < 3; ++q) c.p1[q] = c.p2[q] = c.n1[q] = c.n2[q] = 0.0f;
-        c.dist = 0.0f;
-        c.has = 0;
-        return;
-    }
-    if (mode == kBallFirst) {  // Contact::flipped (contact.rs:156-166)
-        f3 t = p1;
-        p1 = p2;
-        p2 = t;
-        t = n1;
-        n1 = n2;
-        n2 = t;
-    }
-    const f3 w1 = iso_point(pos1, p1), w2 = iso_point(pos2, p2);
-    const f3 m1 = qrot(pos1.q, n1), m2 = qrot(pos2.q, n2);
-    c.p1[0] = w1.x; c.p1[1] = w1.y; c.p1[2] = w1.z;
-    c.p2[0] = w2.x; c.p2[1] = w2.y; c.p2[2] = w2.z;
-    c.n1[0] = m1.x; c.n1[1] = m1.y; c.n1[2] = m1.z;
-    c.n2[0] = m2.x; c.n2[1] = m2.y; c.n2[2] = m2.z;
-    c.dist = dist;
-    c.has = 1;
+// Writes a contact record straight to its output slot (a local record copied
+// out afterwards would be kept in scratch memory).
+__device__ __forceinline__ void store_contact(DevContact *c, f3 w1, f3 w2, f3 m1, f3 m2, float dist) {
+    c->p1[0] = w1.x; c->p1[1] = w1.y; c->p1[2] = w1.z;
+    c->p2[0] = w2.x; c->p2[1] = w2.y; c->p2[2] = w2.z;
+    c->n1[0] = m1.x; c->n1[1] = m1.y; c->n1[2] = m1.z;
+    c->n2[0] = m2.x; c->n2[1] = m2.y; c->n2[2] = m2.z;
+    c->dist = dist;
+    c->has = 1;
 }
 
 __device__ __forceinline__ DevContact no_contact(uint32_t has) {
@@ -110,6 +73,32 @@ __device__ __forceinline__ DevContact no_contact(uint32_t has) {
     c.dist = 0.0f;
     c.has = has;
     return c;
+}
+
+__device__ __forceinline__ void finish_contact(const iso &pos1, const iso &pos2, const iso &pos12, f3 p1, f3 p2_1,
+                                               f3 n1, DevContact *o) {
+    // contact_support_map_support_map (contact_support_map_support_map.rs:18-26)
+    const float dist = dot3(sub3(p2_1, p1), n1);
+    const f3 p2 = qrot_inv(pos12.q, sub3(p2_1, pos12.t));
+    const f3 n2 = qrot_inv(pos12.q, neg3(n1));
+    // Contact::transform_by_mut
+    store_contact(o, iso_point(pos1, p1), iso_point(pos2, p2), qrot(pos1.q, n1), qrot(pos2.q, n2), dist);
+}
+
+// Builds the Ball-vs-convex contact (contact_ball_convex_polyhedron.rs:27-63)
+// and maps it to world space; pb = pose of the ball in the convex shape's frame.
+__device__ __forceinline__ void finish_ball_contact(const iso &pos1, const iso &pos2, const iso &pb, uint32_t conv_type,
+                                                    float radius, float prediction, const PointProj &proj, uint32_t fk,
+                                                    uint32_t fid, uint32_t mode, DevContact *o) {
+    f3 p1, p2, n1, n2;
+    float dist;
+    if (!convex_ball_contact(pb, conv_type, radius, prediction, proj, fk, fid, p1, p2, n1, n2, dist)) {
+        *o = no_contact(0);
+        return;
+    }
+    const bool flip = mode == kBallFirst;  // Contact::flipped (contact.rs:156-166)
+    const f3 q1 = sel2(flip, p2, p1), q2 = sel2(flip, p1, p2), m1 = sel2(flip, n2, n1), m2 = sel2(flip, n1, n2);
+    store_contact(o, iso_point(pos1, q1), iso_point(pos2, q2), qrot(pos1.q, m1), qrot(pos2.q, m2), dist);
 }
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
@@ -143,8 +132,20 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
         const iso pos1 = load_iso(poses + 7 * (size_t)i), pos2 = load_iso(poses + 7 * (size_t)j);
         const iso pos12 = iso_inv_mul(pos1, pos2);
         const ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
-        DevContact c = no_contact(0);
+        DevContact *const c = out + k;
         uint32_t gi = 0;
+        // What GJK runs on, if anything: a support-map pair, or a ConvexPolyhedron
+        // against ConstantOrigin (the ball-centre projection).  One call site below.
+        bool run_gjk = false;
+        iso gjk_pos{}, seed_pos{};
+        ShapeRef ga{}, gb{};
+        float max_dist = prediction;
+        f3 dir{1.0f, 0.0f, 0.0f}, dflt{1.0f, 0.0f, 0.0f};
+        bool seeded = false;
+        // Ball pairs: the pose of the ball in the convex shape's frame, the shape, the radius.
+        iso pb{};
+        ShapeRef cv{};
+        float radius = 0.0f;
         if (g1.type == PQ_SHAPE_BALL && g2.type == PQ_SHAPE_BALL) {
             // contact_ball_ball
             const float r1 = g1.p0, r2 = g2.p0;
@@ -156,73 +157,83 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
                 const f3 n1 = d2 != 0.0f ? div3(c21, sqrtf(norm_sq3(c21))) : f3{1.0f, 0.0f, 0.0f};
                 const f3 n2 = neg3(qrot_inv(pos12.q, n1));
                 const f3 a = mul3(n1, r1), b = mul3(n2, r2);
-                const f3 w1 = iso_point(pos1, a), w2 = iso_point(pos2, b);
-                const f3 m1 = qrot(pos1.q, n1), m2 = qrot(pos2.q, n2);
-                c.p1[0] = w1.x; c.p1[1] = w1.y; c.p1[2] = w1.z;
-                c.p2[0] = w2.x; c.p2[1] = w2.y; c.p2[2] = w2.z;
-                c.n1[0] = m1.x; c.n1[1] = m1.y; c.n1[2] = m1.z;
-                c.n2[0] = m2.x; c.n2[1] = m2.y; c.n2[2] = m2.z;
-                c.dist = sqrtf(d2) - sum;
-                c.has = 1;
+                store_contact(c, iso_point(pos1, a), iso_point(pos2, b), qrot(pos1.q, n1), qrot(pos2.q, n2),
+                              sqrtf(d2) - sum);
+            } else {
+                *c = no_contact(0);
             }
-            out[k] = c;
         } else if (g1.type == PQ_SHAPE_BALL || g2.type == PQ_SHAPE_BALL) {
             // contact_ball_convex_polyhedron / contact_convex_polyhedron_ball
             // (default_query_dispatcher.rs:326-335): a Ball first is the flipped
             // contact of (shape 2, ball) under pos12^-1.
-            ball_mode = g1.type == PQ_SHAPE_BALL ? kBallFirst : kBallSecond;
-            const iso pb = ball_mode == kBallFirst ? iso_inverse(pos12) : pos12;
-            const ShapeRef &cv = ball_mode == kBallFirst ? g2 : g1;
-            const float radius = ball_mode == kBallFirst ? g1.p0 : g2.p0;
+            if (g1.type == PQ_SHAPE_BALL) {
+                ball_mode = kBallFirst;
+                pb = iso_inverse(pos12);
+                cv = g2;
+                radius = g1.p0;
+            } else {
+                ball_mode = kBallSecond;
+                pb = pos12;
+                cv = g1;
+                radius = g2.p0;
+            }
             const f3 ctr = pb.t;
             PointProj proj{false, f3{0.0f, 0.0f, 0.0f}};
             uint32_t fk = CF_UNKNOWN, fid = 0;
-            bool have = true;
-            if (cv.type == PQ_SHAPE_CUBOID) {
-                proj = cuboid_point_feature(cv, ctr, fk, fid);
-            } else if (cv.type == PQ_SHAPE_CAPSULE) {
-                proj = capsule_point(cv, ctr);
-            } else if (cv.type == PQ_SHAPE_CONE) {
-                proj = cone_point(cv, ctr);
-            } else {
+            if (cv.type == PQ_SHAPE_CONVEX_POLYHEDRON) {
                 // local_point_projection_on_support_map (point_support_map.rs:17-51)
-                const ShapeRef org = origin_shape();
-                f3 dir;
+                run_gjk = true;
+                ga = cv;
+                gb = origin_shape();
+                seed_pos = point_proj_seed_iso(ctr);
+                gjk_pos = point_proj_iso(ctr);
+                max_dist = kFltMax;
                 float l;
                 if (!try_unit(ctr, dir, l)) dir = f3{1.0f, 0.0f, 0.0f};
-                sx_reset(s, cso_from_shapes(point_proj_seed_iso(ctr), cv, org, dir));
-                const GjkOut r = gjk_closest_points(point_proj_iso(ctr), cv, org, kFltMax, s);
-                gi = r.iters;
-                if (r.kind == GJK_CLOSEST) {
-                    proj = PointProj{false, r.p1};
-                } else if (r.kind == GJK_INTERSECTION) {
-                    need_epa = true;
-                    have = false;
-                } else {
-                    out[k] = no_contact(0);  // parry: unreachable!() in gjk::project_origin
-                    have = false;
-                }
-            }
-            if (have) {
+            } else {
+                if (cv.type == PQ_SHAPE_CUBOID) proj = cuboid_point_feature(cv, ctr, fk, fid);
+                else if (cv.type == PQ_SHAPE_CAPSULE) proj = capsule_point(cv, ctr);
+                else proj = cone_point(cv, ctr);
                 finish_ball_contact(pos1, pos2, pb, cv.type, radius, prediction, proj, fk, fid, ball_mode, c);
-                out[k] = c;
             }
         } else {
-            f3 dir;
+            run_gjk = true;
+            ga = g1;
+            gb = g2;
+            seed_pos = gjk_pos = pos12;
+            // parry's default seed: normalize(pos12.t), else +x (contact_support_map_support_map.rs:52-60)
+            const float tsq = norm_sq3(pos12.t);
+            dflt = tsq > kEps * kEps ? div3(pos12.t, sqrtf(tsq)) : f3{1.0f, 0.0f, 0.0f};
+            dir = dflt;
             if (seeds) {
-                dir = f3{seeds[3 * k], seeds[3 * k + 1], seeds[3 * k + 2]};
-            } else {
-                const float sq = norm_sq3(pos12.t);
-                dir = sq > kEps * kEps ? div3(pos12.t, sqrtf(sq)) : f3{1.0f, 0.0f, 0.0f};
+                const f3 sd = f3{seeds[3 * k], seeds[3 * k + 1], seeds[3 * k + 2]};
+                seeded = norm_sq3(sd) > kEps * kEps;  // a zero (or NaN) seed means the default
+                if (seeded) dir = sd;
             }
-            sx_reset(s, cso_from_shapes(pos12, g1, g2, dir));
-            const GjkOut r = gjk_closest_points(pos12, g1, g2, prediction, s);
-            gi = r.iters;
-            if (r.kind == GJK_CLOSEST) {
+        }
+        if (run_gjk) {
+            GjkOut r;
+            for (int attempt = 0;; ++attempt) {
+                sx_reset(s, cso_from_shapes(seed_pos, ga, gb, dir));
+                r = gjk_closest_points(gjk_pos, ga, gb, max_dist, s);
+                gi += r.iters;
+                // a seeded run that reaches the cap falls back to the unseeded GJK
+                if (!(seeded && attempt == 0 && r.iters == kGjkMaxIters)) break;
+                dir = dflt;
+            }
+            if (ball_mode != kPairSupportMaps) {
+                if (r.kind == GJK_CLOSEST) {
+                    finish_ball_contact(pos1, pos2, pb, cv.type, radius, prediction, PointProj{false, r.p1},
+                                        CF_UNKNOWN, 0, ball_mode, c);
+                } else if (r.kind == GJK_INTERSECTION) {
+                    need_epa = true;
+                } else {
+                    *c = no_contact(0);  // parry: unreachable!() in gjk::project_origin
+                }
+            } else if (r.kind == GJK_CLOSEST) {
                 finish_contact(pos1, pos2, pos12, r.p1, r.p2, r.n, c);
-                out[k] = c;
             } else if (r.kind == GJK_NO_INTERSECTION) {
-                out[k] = c;
+                *c = no_contact(0);
             } else {
                 need_epa = true;
             }
@@ -245,6 +256,7 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
             it.pair = pair_id;
             it.dim = s.dim;
             it.mode = ball_mode;
+#pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const Cso &v = s.v[q];
                 it.v[q][0] = v.p.x; it.v[q][1] = v.p.y; it.v[q][2] = v.p.z;
@@ -291,6 +303,10 @@ template <int V, int F, int H, int S, int K, class I> struct EpaState {
 // outgrows it re-runs with the large buffers in global memory, which have room
 // for parry's 100-iteration cap.
 using EpaSmall = EpaState<32, 96, 96, 32, 96, uint8_t>;
+// One wave per pair, room for parry's 100 iterations (the vertex count) and for
+// the faces a long run creates: ~31 KB and ~81 KB of LDS.
+using EpaMedium = EpaState<112, 768, 768, 128, 512, uint16_t>;
+using EpaBig = EpaState<112, 2048, 2048, 128, 2048, uint16_t>;
 using EpaLarge = EpaState<112, 8192, 8192, 2048, 8192, uint16_t>;
 static_assert(sizeof(EpaFaceT<uint8_t>) == 20, "compact LDS face record");
 constexpr uint32_t kOrigFloats = 6;  // orig1, orig2 per CSO vertex
@@ -718,7 +734,7 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
 template <int G, class W>
 __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4 *verts, const float *poses,
                          const uint32_t *pairs, float prediction, const EpaItem *it, DevContact *out,
-                         uint32_t *iters, uint32_t *overflow_list, uint32_t *counters, uint32_t slot) {
+                         uint32_t *iters, uint32_t *overflow_list, uint32_t *overflow_count, uint32_t slot) {
     const uint32_t k = it->pair;
     const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
     const iso pos1 = load_iso(poses + 7 * (size_t)i), pos2 = load_iso(poses + 7 * (size_t)j);
@@ -730,23 +746,26 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
     if (mode != kPairSupportMaps) {
         // EPA::project_origin of the ball centre on the ConvexPolyhedron
         // (point_support_map.rs:39-49), then the Ball-vs-convex contact.
-        const iso pb = mode == kBallFirst ? iso_inverse(pos12) : pos12;
-        const ShapeRef &cv = mode == kBallFirst ? g2 : g1;
-        const float radius = mode == kBallFirst ? g1.p0 : g2.p0;
+        iso pb = pos12;
+        ShapeRef cv = g1;
+        float radius = g2.p0;
+        if (mode == kBallFirst) {
+            pb = iso_inverse(pos12);
+            cv = g2;
+            radius = g1.p0;
+        }
         const uint32_t st = epa_run<G>(w, orig, point_proj_iso(pb.t), cv, origin_shape(), it, p1, p2, n, ei);
         if (__lane_id() % G != 0) return;
         if (st == EPA_OVERFLOW) {
             if (overflow_list) {
-                overflow_list[atomicAdd(&counters[1], 1u)] = slot;
+                overflow_list[atomicAdd(overflow_count, 1u)] = slot;
                 return;
             }
             out[k] = no_contact(kUnsupportedPair);  // cannot happen with the large buffers
             return;
         }
         const PointProj proj{true, st == EPA_OK ? p1 : pb.t};
-        DevContact c;
-        finish_ball_contact(pos1, pos2, pb, cv.type, radius, prediction, proj, CF_UNKNOWN, 0, mode, c);
-        out[k] = c;
+        finish_ball_contact(pos1, pos2, pb, cv.type, radius, prediction, proj, CF_UNKNOWN, 0, mode, out + k);
         if (iters) iters[2 * k + 1] = ei;
         return;
     }
@@ -754,135 +773,167 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
     if (__lane_id() % G != 0) return;
     if (st == EPA_OVERFLOW) {
         if (overflow_list) {
-            overflow_list[atomicAdd(&counters[1], 1u)] = slot;
+            overflow_list[atomicAdd(overflow_count, 1u)] = slot;
             return;
         }
         out[k] = no_contact(kUnsupportedPair);  // cannot happen with the large buffers
         return;
     }
     if (st == EPA_OK) {
-        DevContact c;
-        finish_contact(pos1, pos2, pos12, p1, p2, n, c);
-        out[k] = c;
+        finish_contact(pos1, pos2, pos12, p1, p2, n, out + k);
     } else {
         out[k] = no_contact(0);  // EPA failed: NoIntersection (contact_support_map_support_map.rs:75-76)
     }
     if (iters) iters[2 * k + 1] = ei;
 }
 
-// Stage 2: G lanes per pair, 64/G pairs per wave, one LDS polytope per group.
-template <int G>
-__global__ __launch_bounds__(64) void epa_lds_kernel(const ShapeRecord *__restrict__ rec,
-                                                     const float4 *__restrict__ verts, const float *__restrict__ poses,
-                                                     const uint32_t *__restrict__ pairs, float prediction,
-                                                     const EpaItem *__restrict__ queue, DevContact *__restrict__ out,
-                                                     uint32_t *__restrict__ iters, uint32_t *overflow_list,
-                                                     uint32_t *counters, float *orig_scratch) {
+// EPA tiers.  Each reads a list of queued pairs (the queue itself for the first
+// tier) and sends the pairs whose polytope outgrows its buffers to the next
+// tier's list: G lanes per pair with a small LDS polytope, then one wave per
+// pair with medium and big LDS polytopes, then a global-memory polytope.
+template <int G, class W>
+__global__ __launch_bounds__(64) void epa_tier_kernel(const ShapeRecord *__restrict__ rec,
+                                                      const float4 *__restrict__ verts, const float *__restrict__ poses,
+                                                      const uint32_t *__restrict__ pairs, float prediction,
+                                                      const EpaItem *__restrict__ queue, const uint32_t *in_list,
+                                                      const uint32_t *in_count, DevContact *__restrict__ out,
+                                                      uint32_t *__restrict__ iters, uint32_t *out_list,
+                                                      uint32_t *out_count, float *orig_scratch) {
     constexpr int kGroups = 64 / G;
-    __shared__ EpaSmall w[kGroups];
+    __shared__ W w[kGroups];
     const uint32_t group = threadIdx.x / G;
     const uint32_t gid = blockIdx.x * kGroups + group;
-    float *orig = orig_scratch + (size_t)gid * EpaSmall::kV * kOrigFloats;
-    const uint32_t count = __hip_atomic_load(&counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    float *orig = orig_scratch + (size_t)gid * W::kV * kOrigFloats;
+    const uint32_t count = __hip_atomic_load(in_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t stride = gridDim.x * kGroups;
-    for (uint32_t q = gid; q < count; q += stride)
-        epa_item<G>(&w[group], orig, rec, verts, poses, pairs, prediction, queue + q, out, iters, overflow_list,
-                    counters, q);
+    for (uint32_t q = gid; q < count; q += stride) {
+        const uint32_t slot = in_list ? in_list[q] : q;
+        epa_item<G>(&w[group], orig, rec, verts, poses, pairs, prediction, queue + slot, out, iters, out_list,
+                    out_count, slot);
+    }
 }
 
-// Stage 3: one wave per overflowed pair, polytope in global memory.
+// Last tier: one wave per pair, polytope in global memory.
 __global__ __launch_bounds__(64) void epa_global_kernel(const ShapeRecord *__restrict__ rec,
                                                         const float4 *__restrict__ verts,
                                                         const float *__restrict__ poses,
                                                         const uint32_t *__restrict__ pairs, float prediction,
                                                         const EpaItem *__restrict__ queue,
                                                         DevContact *__restrict__ out, uint32_t *__restrict__ iters,
-                                                        const uint32_t *overflow_list, uint32_t *counters,
+                                                        const uint32_t *in_list, const uint32_t *in_count,
                                                         EpaLarge *scratch, float *orig_scratch) {
     EpaLarge *w = scratch + blockIdx.x;
     float *orig = orig_scratch + (size_t)blockIdx.x * EpaLarge::kV * kOrigFloats;
-    const uint32_t count = __hip_atomic_load(&counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t count = __hip_atomic_load(in_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (uint32_t q = blockIdx.x; q < count; q += gridDim.x)
-        epa_item<64>(w, orig, rec, verts, poses, pairs, prediction, queue + overflow_list[q], out, iters, nullptr,
-                     counters, q);
+        epa_item<64>(w, orig, rec, verts, poses, pairs, prediction, queue + in_list[q], out, iters, nullptr, nullptr,
+                     q);
 }
 
-template <int G> hipError_t launch_epa_lds(const DeviceShapes &s, const float *poses, const uint32_t *pairs,
-                                           float prediction, uint64_t m, const EpaItem *queue, DevContact *out, uint32_t *iters,
-                                           uint32_t *overflow, uint32_t *counters, float *orig, uint64_t max_groups,
-                                           hipStream_t stream) {
+template <int G, class W>
+hipError_t launch_epa_tier(const DeviceShapes &s, const float *poses, const uint32_t *pairs, float prediction,
+                           uint64_t max_items, const EpaItem *queue, const uint32_t *in_list, const uint32_t *in_count,
+                           DevContact *out, uint32_t *iters, uint32_t *out_list, uint32_t *out_count, float *orig,
+                           uint64_t max_groups, hipStream_t stream) {
     hipError_t e;
     int dev = 0, cus = 0, per_cu = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, epa_lds_kernel<G>, 64, 0)) != hipSuccess) return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, epa_tier_kernel<G, W>, 64, 0)) != hipSuccess)
+        return e;
     if (per_cu < 1) per_cu = 1;
     constexpr uint64_t kGroups = 64 / G;
     uint64_t waves = (uint64_t)cus * (uint64_t)per_cu;
-    const uint64_t need = (m + kGroups - 1) / kGroups;
+    const uint64_t need = (max_items + kGroups - 1) / kGroups;
     if (waves > need) waves = need;
     if (waves * kGroups > max_groups) waves = max_groups / kGroups;
-    hipLaunchKernelGGL(epa_lds_kernel<G>, dim3((uint32_t)waves), dim3(64), 0, stream, s.rec, s.verts, poses, pairs,
-                       prediction, queue, out, iters, overflow, counters, orig);
+    if (waves < 1) waves = 1;
+    hipLaunchKernelGGL((epa_tier_kernel<G, W>), dim3((uint32_t)waves), dim3(64), 0, stream, s.rec, s.verts, poses,
+                       pairs, prediction, queue, in_list, in_count, out, iters, out_list, out_count, orig);
     return hipGetLastError();
 }
 
 }  // namespace
 
-// Resident EPA groups the workspace provides orig1/orig2 buffers for.
+// Resident EPA groups the workspace provides orig1/orig2 buffers for, per tier.
 constexpr uint64_t kEpaMaxGroups = 256ull * 16 * 8;
+constexpr uint64_t kEpaMediumWaves = 256ull * 8;
+constexpr uint64_t kEpaBigWaves = 256ull * 2;
 
 int g_epa_lanes_per_pair = 16;
 
-size_t contact_workspace_bytes(uint64_t num_pairs) {
+namespace {
+struct ContactWorkspace {
+    uint32_t *counters;  // queue length and the three overflow-list lengths
+    EpaItem *queue;
+    uint32_t *lists[3];
+    EpaLarge *large;
+    float *orig_large, *orig_small, *orig_medium, *orig_big;
+    size_t bytes;
+};
+
+ContactWorkspace carve_workspace(void *base, uint64_t m) {
     const size_t align = 256;
     auto up = [&](size_t b) { return (b + align - 1) / align * align; };
-    return up(16) + up(num_pairs * sizeof(EpaItem)) + up(num_pairs * sizeof(uint32_t)) +
-           up(kEpaGlobalWaves * sizeof(EpaLarge)) + up(kEpaGlobalWaves * EpaLarge::kV * kOrigFloats * sizeof(float)) +
-           up(kEpaMaxGroups * EpaSmall::kV * kOrigFloats * sizeof(float));
+    char *ws = (char *)base;
+    ContactWorkspace w{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *p = ws ? ws + off : nullptr;
+        off += up(bytes);
+        return p;
+    };
+    w.counters = (uint32_t *)take(16);
+    w.queue = (EpaItem *)take(m * sizeof(EpaItem));
+    for (int k = 0; k < 3; ++k) w.lists[k] = (uint32_t *)take(m * sizeof(uint32_t));
+    w.large = (EpaLarge *)take(kEpaGlobalWaves * sizeof(EpaLarge));
+    w.orig_large = (float *)take(kEpaGlobalWaves * EpaLarge::kV * kOrigFloats * sizeof(float));
+    w.orig_small = (float *)take(kEpaMaxGroups * EpaSmall::kV * kOrigFloats * sizeof(float));
+    w.orig_medium = (float *)take(kEpaMediumWaves * EpaMedium::kV * kOrigFloats * sizeof(float));
+    w.orig_big = (float *)take(kEpaBigWaves * EpaBig::kV * kOrigFloats * sizeof(float));
+    w.bytes = off;
+    return w;
 }
+}  // namespace
+
+size_t contact_workspace_bytes(uint64_t num_pairs) { return carve_workspace(nullptr, num_pairs).bytes; }
 
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
                            const uint64_t *m_dev, float prediction, const float *seeds, pq_contact *out,
                            uint32_t *iters, void *workspace, hipStream_t stream) {
     if (m == 0) return hipSuccess;
-    const size_t align = 256;
-    auto up = [&](size_t b) { return (b + align - 1) / align * align; };
-    char *ws = (char *)workspace;
-    uint32_t *counters = (uint32_t *)ws;
-    ws += up(16);
-    EpaItem *queue = (EpaItem *)ws;
-    ws += up(m * sizeof(EpaItem));
-    uint32_t *overflow = (uint32_t *)ws;
-    ws += up(m * sizeof(uint32_t));
-    EpaLarge *scratch = (EpaLarge *)ws;
-    ws += up(kEpaGlobalWaves * sizeof(EpaLarge));
-    float *orig_large = (float *)ws;
-    ws += up(kEpaGlobalWaves * EpaLarge::kV * kOrigFloats * sizeof(float));
-    float *orig_small = (float *)ws;
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters);
+    const ContactWorkspace w = carve_workspace(workspace, m);
+    DevContact *o = (DevContact *)out;
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, w.counters);
     hipLaunchKernelGGL(contact_gjk_kernel, dim3((uint32_t)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
-                       s.rec, s.verts, poses, pairs, m, (const unsigned long long *)m_dev, prediction, seeds,
-                       (DevContact *)out, iters, queue, counters);
+                       s.rec, s.verts, poses, pairs, m, (const unsigned long long *)m_dev, prediction, seeds, o, iters,
+                       w.queue, w.counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    uint32_t *const c = w.counters;
     switch (g_epa_lanes_per_pair) {
         case 8:
-            e = launch_epa_lds<8>(s, poses, pairs, prediction, m, queue, (DevContact *)out, iters, overflow, counters, orig_small,
-                                  kEpaMaxGroups, stream);
+            e = launch_epa_tier<8, EpaSmall>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
+                                             w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
             break;
         case 16:
-            e = launch_epa_lds<16>(s, poses, pairs, prediction, m, queue, (DevContact *)out, iters, overflow, counters,
-                                   orig_small, kEpaMaxGroups, stream);
+            e = launch_epa_tier<16, EpaSmall>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
+                                              w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
             break;
         default:
-            e = launch_epa_lds<64>(s, poses, pairs, prediction, m, queue, (DevContact *)out, iters, overflow, counters,
-                                   orig_small, kEpaMaxGroups, stream);
+            e = launch_epa_tier<64, EpaSmall>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
+                                              w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
             break;
     }
     if (e != hipSuccess) return e;
+    e = launch_epa_tier<64, EpaMedium>(s, poses, pairs, prediction, m, w.queue, w.lists[0], c + 1, o, iters,
+                                       w.lists[1], c + 2, w.orig_medium, kEpaMediumWaves, stream);
+    if (e != hipSuccess) return e;
+    e = launch_epa_tier<64, EpaBig>(s, poses, pairs, prediction, m, w.queue, w.lists[1], c + 2, o, iters, w.lists[2],
+                                    c + 3, w.orig_big, kEpaBigWaves, stream);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(epa_global_kernel, dim3(kEpaGlobalWaves), dim3(64), 0, stream, s.rec, s.verts, poses, pairs,
-                       prediction, queue, (DevContact *)out, iters, overflow, counters, scratch, orig_large);
+                       prediction, w.queue, o, iters, w.lists[2], c + 3, w.large, w.orig_large);
     return hipGetLastError();
 }
 

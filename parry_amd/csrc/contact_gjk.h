@@ -25,6 +25,7 @@ namespace pq {
 constexpr float kEps = 1.1920929e-07f;      // f32::EPSILON = DEFAULT_EPSILON (src/lib.rs:102)
 constexpr float kEpsTol = kEps * 10.0f;     // gjk::eps_tol (gjk.rs:141-144)
 constexpr float kFltMax = 3.40282347e+38f;  // f32::MAX
+constexpr uint32_t kGjkMaxIters = 100;      // gjk.rs:449-451
 // parry's ConstantOrigin (src/query/gjk/special_support_maps.rs:32-54): the
 // second operand of GJK/EPA point projections.  Internal, never in a shape set.
 constexpr uint32_t kShapeOrigin = 6u;
@@ -184,8 +185,19 @@ __device__ __forceinline__ f3 tri_origin(f3 a, f3 b, f3 c, Loc &l, bool &inside)
     return f3{0.0f, 0.0f, 0.0f};
 }
 
-// Tetrahedron projection of the origin (solid).
-__device__ __noinline__ f3 tet_origin(f3 a, f3 b, f3 c, f3 d, Loc &l) {
+// Tetrahedron projection of the origin (solid).  Not inlined (it is large and
+// runs on the last GJK iterations only); inputs and outputs pass in registers.
+struct TetOut {
+    f3 p;
+    Loc l;
+};
+__device__ __forceinline__ f3 tet_origin_impl(f3 a, f3 b, f3 c, f3 d, Loc &l);
+__device__ __noinline__ TetOut tet_origin(f3 a, f3 b, f3 c, f3 d) {
+    TetOut o;
+    o.p = tet_origin_impl(a, b, c, d, o.l);
+    return o;
+}
+__device__ __forceinline__ f3 tet_origin_impl(f3 a, f3 b, f3 c, f3 d, Loc &l) {
     const f3 ab = sub3(b, a), ac = sub3(c, a), ad = sub3(d, a), ap = neg3(a);
     const float ap_ab = dot3(ap, ab), ap_ac = dot3(ap, ac), ap_ad = dot3(ap, ad);
     if (ap_ab <= 0.0f && ap_ac <= 0.0f && ap_ad <= 0.0f) {
@@ -281,10 +293,39 @@ __device__ __forceinline__ uint32_t perm_swap(uint32_t perm, uint32_t i, uint32_
 }
 constexpr uint32_t kPermIdentity = 0x03020100u;
 
+// Slot access by a run-time index through selects, so the simplex stays in
+// registers (an indexed private array would live in scratch memory).
+// (Selecting whole structs with ?: would select an address: component-wise
+// scalar selects keep everything in registers.)
+__device__ __forceinline__ float sel4(float a, float b, float c, float d, uint32_t i) {
+    return i == 0 ? a : (i == 1 ? b : (i == 2 ? c : d));
+}
+__device__ __forceinline__ f3 sel4(const f3 &a, const f3 &b, const f3 &c, const f3 &d, uint32_t i) {
+    return f3{sel4(a.x, b.x, c.x, d.x, i), sel4(a.y, b.y, c.y, d.y, i), sel4(a.z, b.z, c.z, d.z, i)};
+}
+__device__ __forceinline__ Cso sx_get(const Simplex &s, uint32_t i) {
+    return Cso{sel4(s.v[0].p, s.v[1].p, s.v[2].p, s.v[3].p, i), sel4(s.v[0].o1, s.v[1].o1, s.v[2].o1, s.v[3].o1, i),
+               sel4(s.v[0].o2, s.v[1].o2, s.v[2].o2, s.v[3].o2, i)};
+}
+__device__ __forceinline__ f3 sel2(bool c, const f3 &a, const f3 &b) {
+    return f3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z};
+}
+__device__ __forceinline__ Cso csel(bool c, const Cso &a, const Cso &b) {
+    return Cso{sel2(c, a.p, b.p), sel2(c, a.o1, b.o1), sel2(c, a.o2, b.o2)};
+}
+// Every slot is rewritten (with a value select), so no store goes through a
+// run-time-selected address.
+__device__ __forceinline__ void sx_set(Simplex &s, uint32_t i, const Cso &c) {
+    s.v[0] = csel(i == 0, c, s.v[0]);
+    s.v[1] = csel(i == 1, c, s.v[1]);
+    s.v[2] = csel(i == 2, c, s.v[2]);
+    s.v[3] = csel(i == 3, c, s.v[3]);
+}
+
 __device__ __forceinline__ void sx_swap(Simplex &s, uint32_t i, uint32_t j) {
-    const Cso t = s.v[i];
-    s.v[i] = s.v[j];
-    s.v[j] = t;
+    const Cso a = sx_get(s, i), b = sx_get(s, j);
+    sx_set(s, i, b);
+    sx_set(s, j, a);
     s.perm = perm_swap(s.perm, i, j);
 }
 
@@ -316,12 +357,12 @@ __device__ __forceinline__ bool sx_add(Simplex &s, const Cso &pt) {
         if (fabsf(dot3(n, ap)) < kEpsTol) return false;
     }
     s.dim += 1;
-    s.v[s.dim] = pt;
+    sx_set(s, s.dim, pt);
     return true;
 }
 
 // VoronoiSimplex::project_origin_and_reduce (voronoi_simplex3.rs:123-279).
-__device__ __noinline__ f3 sx_project(Simplex &s) {
+__device__ __forceinline__ f3 sx_project(Simplex &s) {
     if (s.dim == 0) {
         s.proj[0] = 1.0f;
         return s.v[0].p;
@@ -367,7 +408,9 @@ __device__ __noinline__ f3 sx_project(Simplex &s) {
         }
         return p;
     }
-    const f3 p = tet_origin(s.v[0].p, s.v[1].p, s.v[2].p, s.v[3].p, l);
+    const TetOut to = tet_origin(s.v[0].p, s.v[1].p, s.v[2].p, s.v[3].p);
+    const f3 p = to.p;
+    l = to.l;
     if (l.kind == LOC_VERTEX) {
         sx_swap(s, 0, l.id);
         s.proj[0] = 1.0f;
@@ -386,16 +429,13 @@ __device__ __noinline__ f3 sx_project(Simplex &s) {
         s.proj[1] = flip ? l.b0 : l.b1;
         s.dim = 1;
     } else if (l.kind == LOC_FACE) {
-        if (l.id == 0) {
-            s.proj[0] = l.b0; s.proj[1] = l.b1; s.proj[2] = l.b2;
-        } else if (l.id == 1) {
-            s.v[2] = s.v[3];
+        // face 0 keeps (v0, v1, v2); faces 1, 2, 3 move v3 into slot 2, 1, 0
+        if (l.id != 0) sx_set(s, 3 - l.id, s.v[3]);
+        if (l.id <= 1) {
             s.proj[0] = l.b0; s.proj[1] = l.b1; s.proj[2] = l.b2;
         } else if (l.id == 2) {
-            s.v[1] = s.v[3];
             s.proj[0] = l.b0; s.proj[1] = l.b2; s.proj[2] = l.b1;
         } else {
-            s.v[0] = s.v[3];
             s.proj[0] = l.b2; s.proj[1] = l.b0; s.proj[2] = l.b1;
         }
         s.dim = 2;
@@ -406,10 +446,12 @@ __device__ __noinline__ f3 sx_project(Simplex &s) {
 // gjk::result (gjk.rs:797-818): points from the current or previous simplex.
 __device__ __forceinline__ void sx_points(const Simplex &s, bool prev, f3 &p1, f3 &p2) {
     p1 = p2 = f3{0.0f, 0.0f, 0.0f};
-    const uint32_t k = prev ? s.prev_dim : s.dim;
-    for (uint32_t i = 0; i <= k; ++i) {
+    const uint32_t k = prev ? s.prev_dim : s.dim;  // at most 2 here
+#pragma unroll
+    for (uint32_t i = 0; i < 3; ++i) {
+        if (i > k) break;
         const float c = prev ? s.prev_proj[i] : s.proj[i];
-        const Cso &pt = s.v[prev ? perm_get(s.perm, i) : i];
+        const Cso pt = sx_get(s, prev ? perm_get(s.perm, i) : i);
         p1 = add3(p1, mul3(pt.o1, c));
         p2 = add3(p2, mul3(pt.o2, c));
     }
@@ -424,7 +466,7 @@ struct GjkOut {
 };
 
 // gjk::closest_points (gjk.rs:353-453) with exact_dist = true.
-__device__ __noinline__ GjkOut gjk_closest_points(const iso &pos12, const ShapeRef &g1, const ShapeRef &g2,
+__device__ __forceinline__ GjkOut gjk_closest_points(const iso &pos12, const ShapeRef &g1, const ShapeRef &g2,
                                                   float max_dist, Simplex &s) {
     const float eps_rel = sqrtf(kEpsTol);
     GjkOut out{GJK_INTERSECTION, 0, f3{0, 0, 0}, f3{0, 0, 0}, f3{0, 0, 0}};
@@ -490,7 +532,7 @@ __device__ __noinline__ GjkOut gjk_closest_points(const iso &pos12, const ShapeR
         }
         ++niter;
         out.iters = niter;
-        if (niter == 100) {
+        if (niter == kGjkMaxIters) {
             out.kind = GJK_NO_INTERSECTION;
             out.n = f3{1.0f, 0.0f, 0.0f};
             return out;

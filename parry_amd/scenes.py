@@ -293,3 +293,59 @@ def mixed_shapes(n: int, seed: int, kmin: int = 8, kmax: int = 64):
             descs[i]["params"][:2] = rng.uniform(0.5, 1.5, 2)
     v = np.concatenate(verts, 0) if verts else np.zeros((0, 3))
     return descs, np.ascontiguousarray(v, dtype=np.float32)
+
+
+def _mixed_half_extents(descs, verts, quats):
+    """World AABB half extents of each mixed shape under its rotation (float64,
+    for the cube calibration only)."""
+    q = np.asarray(quats, np.float64)
+    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    R = np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1),
+    ], 1)  # (n, 3, 3)
+    t = descs["type"]
+    p = descs["params"].astype(np.float64)
+    he = np.zeros((len(descs), 3))
+    # Cuboid: |R| he; Ball: r; Capsule: |R e_y| h + r; Cone: |R e_y| h + r sqrt(1 - (R e_y)^2)
+    cub = t == 1
+    he[cub] = np.einsum("nij,nj->ni", np.abs(R[cub]), p[cub, :3])
+    ball = t == 0
+    he[ball] = p[ball, :1]
+    cap = t == 2
+    he[cap] = np.abs(R[cap, :, 1]) * p[cap, :1] + p[cap, 1:2]
+    cone = t == 3
+    ay = R[cone, :, 1]
+    he[cone] = np.abs(ay) * p[cone, :1] + p[cone, 1:2] * np.sqrt(np.clip(1 - ay * ay, 0, 1))
+    poly = np.nonzero(t == 4)[0]
+    if poly.size:
+        first = descs["first_vertex"][poly].astype(np.int64)
+        cnt = descs["num_vertices"][poly].astype(np.int64)
+        owner = np.repeat(np.arange(poly.size), cnt)
+        idx = np.concatenate([np.arange(f, f + c) for f, c in zip(first, cnt)])
+        v = verts[idx].astype(np.float64)
+        wv = np.einsum("nij,nj->ni", R[poly][owner], v)
+        starts = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+        mx = np.maximum.reduceat(wv, starts, axis=0)
+        mn = np.minimum.reduceat(wv, starts, axis=0)
+        he[poly] = (mx - mn) / 2
+    return he
+
+
+def mixed_scene(n: int, seed: int, target_overlaps: float = 8.0):
+    """``n`` mixed convex shapes (``mixed_shapes``) with uniform random rotations
+    and centres uniform in a cube calibrated for ``target_overlaps`` mean AABB
+    overlaps per shape (the learned warm-start and dynamic-scene workloads).
+
+    Returns dict(descs, vertices, poses float32 [n, 7] (i j k w tx ty tz), side).
+    """
+    descs, verts = mixed_shapes(n, seed)
+    rng = _rng(seed + 1)
+    q = random_rotations(n, rng)
+    he = _mixed_half_extents(descs, verts, q)
+    L = calibrate_cube_side(he, target_overlaps, rng)
+    t = rng.uniform(0.0, L, size=(n, 3))
+    poses = np.ascontiguousarray(np.concatenate([q, t], 1), dtype=np.float32)
+    return {"descs": descs, "vertices": verts, "poses": poses, "side": L}
+

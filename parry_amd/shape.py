@@ -370,6 +370,17 @@ class ShapeSet:
     def handle(self):
         return self._h
 
+    def descriptors(self, out=None, stream=None):
+        """(n, 16) f32 CUDA tensor of shape descriptors (the warm-start network's per-shape input)."""
+        import torch
+
+        n = len(self)
+        if out is None:
+            out = torch.empty((n, 16), dtype=torch.float32, device="cuda")
+        s = stream if stream is not None else torch.cuda.current_stream(out.device)
+        _native.check(_native.lib().pq_shape_descriptors(self._h, out.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
+        return out
+
     def compute_aabbs(self, poses, out=None, stream=None):
         """Shape::compute_aabb for every shape: poses CUDA (n, 7) f32 -> CUDA (n, 6) f32."""
         import torch
