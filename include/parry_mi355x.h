@@ -374,6 +374,40 @@ int pq_mlp_train(pq_mlp *net, const float *features, const float *labels, uint64
 int pq_contact_local_normals(const pq_contact *contacts, const pq_isometry *poses, const uint32_t *pairs,
                              uint64_t num_pairs, float *normals, void *stream);
 
+/* Same as pq_contact_seeds for the first min(capacity, *num_pairs) pairs, the
+ * count being a device u64 (e.g. written by pq_bvh_self_pairs). */
+int pq_contact_seeds_device_count(const pq_mlp *net, const pq_isometry *poses, const uint32_t *pairs,
+                                  uint64_t capacity, const uint64_t *num_pairs, const float *descriptors, float *seeds,
+                                  void *stream);
+
+/* ---- Simulation frames (no counterpart in parry's query path) ----------------
+ * Rigid motion: t += v dt; q = normalize(dq * q), dq the rotation by |w| dt about
+ * the world-frame angular velocity w.  linvel, angvel: device, 3 floats per shape. */
+int pq_integrate_poses(pq_isometry *poses, const float *linvel, const float *angvel, uint32_t n, float dt,
+                       void *stream);
+/* Per-pair warm-start cache: (i, j) -> last contact normal1 in shape i's frame,
+ * the next frame's GJK seed (parry's manifolds reuse their previous normal the
+ * same way, contact_manifolds_pfm_pfm.rs:67).  A device buffer of
+ * pq_warm_cache_bytes(capacity) bytes, capacity a power of two well above the
+ * pair count; cleared by a kernel (capturable in a graph). */
+size_t pq_warm_cache_bytes(uint32_t capacity);
+/* Zeroes `bytes` (a multiple of 4) of device memory with a kernel (no memset node
+ * when captured). */
+int pq_device_zero(void *ptr, size_t bytes, void *stream);
+int pq_warm_cache_clear(void *cache, uint32_t capacity, void *stream);
+/* Stores normal1 of every pair with has_contact == 1 (first min(pair_capacity, *num_pairs) pairs). */
+int pq_warm_cache_store(void *cache, uint32_t capacity, const uint32_t *pairs, const uint64_t *num_pairs,
+                        uint64_t pair_capacity, const pq_contact *contacts, const pq_isometry *poses, void *stream);
+/* Overwrites seeds[k] with the cached normal of pair k where there is one;
+ * hits (device u32, nullable) counts them. */
+int pq_warm_cache_apply(const void *cache, uint32_t capacity, const uint32_t *pairs, const uint64_t *num_pairs,
+                        uint64_t pair_capacity, float *seeds, uint32_t *hits, void *stream);
+/* Records the library calls made on `stream` between the two calls into a
+ * hipGraph (replay with pq_graph_launch).  The calls must not allocate: pass
+ * workspaces, and capacities large enough. */
+int pq_stream_begin_capture(void *stream);
+int pq_stream_end_capture(void *stream, pq_graph **out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,15 @@ SIGNATURES = {
     "pq_mlp_gradients": (_int, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp]),
     "pq_mlp_train": (_int, [_vp, _vp, _vp, _u64, _vp, _vp, _vp]),
     "pq_contact_local_normals": (_int, [_vp, _vp, _vp, _u64, _vp, _vp]),
+    "pq_contact_seeds_device_count": (_int, [_vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp]),
+    "pq_integrate_poses": (_int, [_vp, _vp, _vp, _u32, _f32, _vp]),
+    "pq_warm_cache_bytes": (_c.c_size_t, [_u32]),
+    "pq_device_zero": (_int, [_vp, _c.c_size_t, _vp]),
+    "pq_warm_cache_clear": (_int, [_vp, _u32, _vp]),
+    "pq_warm_cache_store": (_int, [_vp, _u32, _vp, _vp, _u64, _vp, _vp, _vp]),
+    "pq_warm_cache_apply": (_int, [_vp, _u32, _vp, _vp, _u64, _vp, _vp, _vp]),
+    "pq_stream_begin_capture": (_int, [_vp]),
+    "pq_stream_end_capture": (_int, [_vp, _c.POINTER(_vp)]),
 }
 
 _lock = threading.Lock()

@@ -8,11 +8,6 @@
 
 using namespace pq;
 
-struct pq_graph {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
-
 namespace {
 
 int batch_alloc(pq_trimesh_batch *b, void **p, size_t bytes) {

@@ -169,7 +169,17 @@ int pq_contact_seeds(const pq_mlp *net, const pq_isometry *poses, const uint32_t
                      const float *descriptors, float *seeds, void *stream) {
     if (!net || (num_pairs && (!poses || !pairs || !descriptors || !seeds)))
         return fail(PQ_ERR_INVALID, "pq_contact_seeds: null argument");
-    PQ_HIP_CHECK(launch_seeds(net->dev, (const float *)poses, pairs, num_pairs, descriptors, seeds,
+    PQ_HIP_CHECK(launch_seeds(net->dev, (const float *)poses, pairs, num_pairs, nullptr, descriptors, seeds,
+                              (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_contact_seeds_device_count(const pq_mlp *net, const pq_isometry *poses, const uint32_t *pairs,
+                                  uint64_t capacity, const uint64_t *num_pairs, const float *descriptors, float *seeds,
+                                  void *stream) {
+    if (!net || !num_pairs || (capacity && (!poses || !pairs || !descriptors || !seeds)))
+        return fail(PQ_ERR_INVALID, "pq_contact_seeds_device_count: null argument");
+    PQ_HIP_CHECK(launch_seeds(net->dev, (const float *)poses, pairs, capacity, num_pairs, descriptors, seeds,
                               (hipStream_t)stream));
     return PQ_OK;
 }

@@ -280,9 +280,14 @@ __global__ __launch_bounds__(kInferBlock) void seeds_kernel(const uint4 *__restr
                                                             const float *__restrict__ bias,
                                                             const float *__restrict__ poses,
                                                             const uint32_t *__restrict__ pairs, uint64_t m,
+                                                            const unsigned long long *__restrict__ m_dev,
                                                             const float *__restrict__ desc, float *__restrict__ seeds) {
     __shared__ uint4 lds_frag[kFwdFrags * 64];
     __shared__ float lds_bias[kBiasCount];
+    if (m_dev) {  // pair count produced on the device (e.g. by the broadphase), capped by m
+        const uint64_t md = *m_dev;
+        if (md < m) m = md;
+    }
     stage_weights(fw, bias, lds_frag, lds_bias);
     const uint32_t l = lane_id(), g = l >> 4, p = l & 15;
     const uint64_t waves = (uint64_t)gridDim.x * (kInferBlock / 64);
@@ -695,13 +700,13 @@ static hipError_t infer_grid(uint64_t m, uint32_t *blocks) {
 }
 
 hipError_t launch_seeds(const DeviceMlp &net, const float *poses, const uint32_t *pairs, uint64_t m,
-                        const float *desc, float *seeds, hipStream_t stream) {
+                        const uint64_t *m_dev, const float *desc, float *seeds, hipStream_t stream) {
     if (m == 0) return hipSuccess;
     uint32_t blocks = 0;
     hipError_t e = infer_grid(m, &blocks);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(seeds_kernel, dim3(blocks), dim3(kInferBlock), 0, stream, net.fw, net.bias, poses, pairs, m,
-                       desc, seeds);
+                       (const unsigned long long *)m_dev, desc, seeds);
     return hipGetLastError();
 }
 

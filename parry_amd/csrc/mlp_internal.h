@@ -64,8 +64,9 @@ struct TrainWork {
 hipError_t launch_descriptors(const DeviceShapes &s, float *out, hipStream_t stream);
 hipError_t launch_features(const float *poses, const uint32_t *pairs, uint64_t m, const float *desc, float *out,
                            hipStream_t stream);
+// m_dev (nullable, device u64): the pair count, capped by m.
 hipError_t launch_seeds(const DeviceMlp &net, const float *poses, const uint32_t *pairs, uint64_t m,
-                        const float *desc, float *seeds, hipStream_t stream);
+                        const uint64_t *m_dev, const float *desc, float *seeds, hipStream_t stream);
 hipError_t launch_forward_rows(const DeviceMlp &net, const float *X, uint64_t n, float *out, hipStream_t stream);
 hipError_t launch_pack(const DeviceMlp &net, hipStream_t stream);
 // One batch: forward, loss, backward; Adam step (adam != null) or gradient into grad_out.

@@ -168,6 +168,11 @@ struct pq_bvh {
     std::vector<void *> allocations;
 };
 
+struct pq_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct pq_trimesh_batch {
     pq::DeviceBatch dev;
     std::vector<void *> allocations;
