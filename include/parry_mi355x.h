@@ -270,11 +270,17 @@ typedef struct pq_contact {
 } pq_contact;
 
 /* Kernel mapping of the contact path (no counterpart in parry; results are
- * identical for every setting): lanes that run one pair's EPA together, 64
- * (one wave per pair), 16 or 8.  Process-wide. */
+ * identical for every setting).  Process-wide.
+ *   epa_lanes_per_pair: lanes that run one pair's EPA together in the first EPA
+ *       tier: 64 (one wave per pair), 32, 16, 8 or 4;
+ *   epa_polytope: the first tier's polytope buffers: 0 = 32 vertices / 96 faces
+ *       with the ConvexPolyhedra's vertices staged in LDS, 1 = the same without
+ *       staging, 2 = 24 vertices / 64 faces without staging (less LDS per pair,
+ *       more pairs resident; larger polytopes move on to one-wave-per-pair tiers). */
 typedef struct pq_contact_tuning {
     uint32_t epa_lanes_per_pair;
-    uint32_t reserved[3]; /* must be 0 */
+    uint32_t epa_polytope;
+    uint32_t reserved[2]; /* must be 0 */
 } pq_contact_tuning;
 int pq_contact_set_tuning(const pq_contact_tuning *tuning);
 int pq_contact_get_tuning(pq_contact_tuning *tuning);

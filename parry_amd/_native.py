@@ -108,11 +108,17 @@ class RayTuning(ctypes.Structure):
 class ContactTuning(ctypes.Structure):
     """pq_contact_tuning (include/parry_mi355x.h)."""
 
-    _fields_ = [("epa_lanes_per_pair", _u32), ("reserved", _u32 * 3)]
+    _fields_ = [("epa_lanes_per_pair", _u32), ("epa_polytope", _u32), ("reserved", _u32 * 2)]
 
 
-def set_contact_tuning(epa_lanes_per_pair: int) -> None:
-    check(lib().pq_contact_set_tuning(ctypes.byref(ContactTuning(epa_lanes_per_pair))))
+def set_contact_tuning(epa_lanes_per_pair: int, epa_polytope: int = 0) -> None:
+    check(lib().pq_contact_set_tuning(ctypes.byref(ContactTuning(epa_lanes_per_pair, epa_polytope))))
+
+
+def get_contact_tuning() -> tuple:
+    t = ContactTuning()
+    check(lib().pq_contact_get_tuning(ctypes.byref(t)))
+    return int(t.epa_lanes_per_pair), int(t.epa_polytope)
 
 
 class MlpTrainConfig(_c.Structure):

@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
                 } else if (r.kind == GJK_INTERSECTION) {
                     need_epa = true;
                 } else {
-                    *c = no_contact(0);  // parry: unreachable!() in gjk::project_origin
+                    *c = no_contact(kUnsupportedPair);  // parry: unreachable!() in gjk::project_origin
                 }
             } else if (r.kind == GJK_CLOSEST) {
                 finish_contact(pos1, pos2, pos12, r.p1, r.p2, r.n, c);
@@ -287,8 +287,8 @@ template <class I> struct EpaFaceT {
     float n[3];
 };
 
-template <int V, int F, int H, int S, int K, class I> struct EpaState {
-    static constexpr int kV = V, kF = F, kH = H, kS = S, kK = K;
+template <int V, int F, int H, int S, int K, class I, int SV> struct EpaState {
+    static constexpr int kV = V, kF = F, kH = H, kS = S, kK = K, kSV = SV;
     using Idx = I;
     using Face = EpaFaceT<I>;
     float vp[V][3];
@@ -297,17 +297,25 @@ template <int V, int F, int H, int S, int K, class I> struct EpaState {
     float heap_nd[H];
     uint32_t sil[S];  // face | opp << 16
     uint32_t stk[K];  // face | opp << 16
-    float sv[2][kStagedVerts][3];  // both ConvexPolyhedra's vertices (local frames)
+    float fdist[S];   // per silhouette edge: the new face's distance (if proj_inside)
+    I nid[S];         // per silhouette edge: the new face's id
+    uint8_t fin[S];   // per silhouette edge: the new face's proj_inside
+    uint8_t vis[F];   // per face: can_be_seen_by the newest vertex, one bit per opposite vertex
+    float sv[2][SV > 0 ? SV : 1][3];  // both ConvexPolyhedra's vertices (local frames), if SV > 0
 };
 // LDS variant (~4.8 KB per pair) sized for the common case; a pair that
 // outgrows it re-runs with the large buffers in global memory, which have room
 // for parry's 100-iteration cap.
-using EpaSmall = EpaState<32, 96, 96, 32, 96, uint8_t>;
+using EpaSmall = EpaState<32, 96, 96, 32, 96, uint8_t, kStagedVerts>;
+// The same without staged support vertices, and a smaller one: less LDS per pair,
+// more pairs resident per CU (selected by pq_contact_tuning.epa_polytope).
+using EpaSmallUnstaged = EpaState<32, 96, 96, 32, 96, uint8_t, 0>;
+using EpaTiny = EpaState<24, 64, 64, 24, 64, uint8_t, 0>;
 // One wave per pair, room for parry's 100 iterations (the vertex count) and for
 // the faces a long run creates: ~31 KB and ~81 KB of LDS.
-using EpaMedium = EpaState<112, 768, 768, 128, 512, uint16_t>;
-using EpaBig = EpaState<112, 2048, 2048, 128, 2048, uint16_t>;
-using EpaLarge = EpaState<112, 8192, 8192, 2048, 8192, uint16_t>;
+using EpaMedium = EpaState<112, 768, 768, 128, 512, uint16_t, kStagedVerts>;
+using EpaBig = EpaState<112, 2048, 2048, 128, 2048, uint16_t, kStagedVerts>;
+using EpaLarge = EpaState<112, 8192, 8192, 2048, 8192, uint16_t, 0>;
 static_assert(sizeof(EpaFaceT<uint8_t>) == 20, "compact LDS face record");
 constexpr uint32_t kOrigFloats = 6;  // orig1, orig2 per CSO vertex
 
@@ -389,7 +397,7 @@ __device__ __forceinline__ Cso group_cso(const iso &pos12, const ShapeRef &g1, c
 // Copies a ConvexPolyhedron's vertices into LDS (group-cooperative); returns
 // the staged array or nullptr if the shape is not staged.
 template <int G, class W> __device__ const float (*stage_shape(W *w, int slot, const ShapeRef &s))[3] {
-    if (s.type != PQ_SHAPE_CONVEX_POLYHEDRON || s.n > (uint32_t)kStagedVerts) return nullptr;
+    if (W::kSV == 0 || s.type != PQ_SHAPE_CONVEX_POLYHEDRON || s.n > (uint32_t)W::kSV) return nullptr;
     const uint32_t sub = __lane_id() % G;
     for (uint32_t k = sub; k < s.n; k += G) {
         const float4 p = s.v[k];
@@ -670,6 +678,32 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
         }
         old_dist = curr_dist;
         w->face[fid].deleted = 1;
+        // Face::can_be_seen_by (epa3.rs:153-166) of every live face from the new
+        // vertex, for each of the three opposite vertices, by the group's lanes in
+        // parallel; the depth-first walk below only reads the bits.
+        {
+            const uint32_t sub = __lane_id() % G;
+            const f3 pt = vpt(w, sp);
+            const float E = kEps * 100.0f;
+            for (uint32_t f = sub; f < nf; f += G) {
+                if (w->face[f].deleted) continue;
+                const typename W::Face &F = w->face[f];
+                const f3 q0 = vpt(w, F.pts[0]), q1 = vpt(w, F.pts[1]), q2 = vpt(w, F.pts[2]);
+                const f3 fnn = f3{F.n[0], F.n[1], F.n[2]};
+                const f3 qs[3] = {q0, q1, q2};
+                uint32_t bits = 0;
+#pragma unroll
+                for (int opp = 0; opp < 3; ++opp) {
+                    const f3 p0 = qs[opp], p1 = qs[(opp + 1) % 3], p2 = qs[(opp + 2) % 3];
+                    const bool seen = dot3(sub3(pt, p0), fnn) >= -kEpsTol ||
+                                      norm_sq3(cross3(sub3(p2, p1), sub3(pt, p1))) <= E * E;  // is_affinely_dependent
+                    bits |= (seen ? 1u : 0u) << opp;
+                }
+                w->vis[f] = (uint8_t)bits;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
         // compute_silhouette for the three neighbours, depth first (explicit stack).
         uint32_t ns = 0, sk = 0;
         for (int q = 2; q >= 0; --q) {
@@ -680,7 +714,7 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
             const uint32_t e = w->stk[--sk];
             const uint32_t id = e & 0xffffu, opp = e >> 16;
             if (w->face[id].deleted) continue;
-            if (!seen_by(w, id, sp, opp)) {
+            if (!((w->vis[id] >> opp) & 1u)) {
                 if (ns >= (uint32_t)W::kS) return EPA_OVERFLOW;
                 w->sil[ns++] = e;
             } else {
@@ -696,29 +730,47 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
         }
         const uint32_t first_new = nf;
         if (ns == 0) return EPA_NONE;
-        for (uint32_t q = 0; q < ns; ++q) {
-            const uint32_t e = w->sil[q];
-            const uint32_t ef = e & 0xffffu, opp = e >> 16;
-            if (w->face[ef].deleted) continue;
-            if (nf >= (uint32_t)W::kF) return EPA_OVERFLOW;
-            const uint32_t new_id = nf;
-            const uint32_t pt1 = w->face[ef].pts[(opp + 2) % 3];
-            const uint32_t pt2 = w->face[ef].pts[(opp + 1) % 3];
-            const bool in = face_new(w, new_id, pt1, pt2, sp, ef, new_id + 1, new_id - 1);
-            w->face[ef].adj[(opp + 1) % 3] = (typename W::Idx)new_id;
-            ++nf;
-            if (in) {
-                const float dist = dot3(fnormal(w, new_id), vpt(w, w->face[new_id].pts[0]));
-                if (dist < curr_dist) {
-                    face_points(w, orig, fid, p1, p2);  // the popped face: its points are unchanged
-                    n = fn;
-                    epa_iters = niter + 1;
-                    return EPA_OK;
-                }
-                if (-dist > kEpsTol) return EPA_NONE;
-                if (!heap.push(new_id, -dist)) return EPA_OVERFLOW;
+        // The new faces (epa3.rs:598-626): the silhouette edges whose face is still
+        // live get consecutive ids in silhouette order; the faces are built by the
+        // group's lanes in parallel (Face::new reads vertex positions only and each
+        // edge rewrites its own adjacency slot), then the in-order pass applies
+        // parry's early exits and heap pushes.
+        uint32_t n_alive = 0;
+        for (uint32_t q = 0; q < ns; ++q) n_alive += w->face[w->sil[q] & 0xffffu].deleted ? 0u : 1u;
+        if (first_new + n_alive > (uint32_t)W::kF) return EPA_OVERFLOW;
+        {
+            const uint32_t sub = __lane_id() % G;
+            for (uint32_t q = sub; q < ns; q += G) {
+                const uint32_t e = w->sil[q];
+                const uint32_t ef = e & 0xffffu, opp = e >> 16;
+                if (w->face[ef].deleted) continue;
+                uint32_t rank = 0;
+                for (uint32_t r = 0; r < q; ++r) rank += w->face[w->sil[r] & 0xffffu].deleted ? 0u : 1u;
+                const uint32_t new_id = first_new + rank;
+                const uint32_t pt1 = w->face[ef].pts[(opp + 2) % 3];
+                const uint32_t pt2 = w->face[ef].pts[(opp + 1) % 3];
+                const bool in = face_new(w, new_id, pt1, pt2, sp, ef, new_id + 1, new_id - 1);
+                w->face[ef].adj[(opp + 1) % 3] = (typename W::Idx)new_id;
+                w->fin[q] = in ? 1 : 0;
+                w->nid[q] = (typename W::Idx)new_id;
+                w->fdist[q] = in ? dot3(fnormal(w, new_id), vpt(w, pt1)) : 0.0f;
             }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
+        for (uint32_t q = 0; q < ns; ++q) {
+            if (w->face[w->sil[q] & 0xffffu].deleted || !w->fin[q]) continue;
+            const float dist = w->fdist[q];
+            if (dist < curr_dist) {
+                face_points(w, orig, fid, p1, p2);  // the popped face: its points are unchanged
+                n = fn;
+                epa_iters = niter + 1;
+                return EPA_OK;
+            }
+            if (-dist > kEpsTol) return EPA_NONE;
+            if (!heap.push(w->nid[q], -dist)) return EPA_OVERFLOW;
+        }
+        nf = first_new + n_alive;
         if (first_new == nf) return EPA_NONE;
         w->face[first_new].adj[2] = (typename W::Idx)(nf - 1);
         w->face[nf - 1].adj[1] = (typename W::Idx)first_new;
@@ -861,6 +913,7 @@ constexpr uint64_t kEpaMediumWaves = 256ull * 8;
 constexpr uint64_t kEpaBigWaves = 256ull * 2;
 
 int g_epa_lanes_per_pair = 16;
+int g_epa_polytope = 0;
 
 namespace {
 struct ContactWorkspace {
@@ -898,6 +951,25 @@ ContactWorkspace carve_workspace(void *base, uint64_t m) {
 
 size_t contact_workspace_bytes(uint64_t num_pairs) { return carve_workspace(nullptr, num_pairs).bytes; }
 
+namespace {
+// First EPA tier: G lanes per pair with the selected small polytope.
+template <int G>
+hipError_t launch_small(const DeviceShapes &s, const float *poses, const uint32_t *pairs, float prediction, uint64_t m,
+                        const ContactWorkspace &w, uint32_t *c, DevContact *o, uint32_t *iters, hipStream_t stream) {
+    switch (g_epa_polytope) {
+        case 1:
+            return launch_epa_tier<G, EpaSmallUnstaged>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
+                                                        w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
+        case 2:
+            return launch_epa_tier<G, EpaTiny>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
+                                               w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
+        default:
+            return launch_epa_tier<G, EpaSmall>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
+                                                w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
+    }
+}
+}  // namespace
+
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
                            const uint64_t *m_dev, float prediction, const float *seeds, pq_contact *out,
                            uint32_t *iters, void *workspace, hipStream_t stream) {
@@ -912,18 +984,11 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
     if (e != hipSuccess) return e;
     uint32_t *const c = w.counters;
     switch (g_epa_lanes_per_pair) {
-        case 8:
-            e = launch_epa_tier<8, EpaSmall>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
-                                             w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
-            break;
-        case 16:
-            e = launch_epa_tier<16, EpaSmall>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
-                                              w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
-            break;
-        default:
-            e = launch_epa_tier<64, EpaSmall>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
-                                              w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
-            break;
+        case 4: e = launch_small<4>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
+        case 8: e = launch_small<8>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
+        case 16: e = launch_small<16>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
+        case 32: e = launch_small<32>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
+        default: e = launch_small<64>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
     }
     if (e != hipSuccess) return e;
     e = launch_epa_tier<64, EpaMedium>(s, poses, pairs, prediction, m, w.queue, w.lists[0], c + 1, o, iters,

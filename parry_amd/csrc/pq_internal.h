@@ -151,6 +151,7 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
                            uint32_t *iters, void *workspace, hipStream_t stream);
 // Lanes that run one pair's EPA together (64, 16 or 8), set by pq_contact_set_tuning.
 extern int g_epa_lanes_per_pair;
+extern int g_epa_polytope;
 // Per-(device, stream) workspace owned by the library, grown on demand.
 hipError_t stream_workspace(hipStream_t stream, size_t bytes, void **ptr);
 
