@@ -36,6 +36,19 @@ HBM_PEAK_GBS = 8000.0
 MAX_TOI = 10.0
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's readings), or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh)[kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -466,6 +479,8 @@ def main() -> int:
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched multi-mesh leg")
     ap.add_argument("--skip-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--skip-dynamic", action="store_true", help="skip the dynamic-scene leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="profiling aid: only copy BYTES of device memory 4 times (counter calibration)")
     ap.add_argument("--dynamic-only", action="store_true", help="profiling aid: run only the dynamic leg")
     ap.add_argument("--learned-only", action="store_true", help="profiling aid: run only the learned leg")
     ap.add_argument("--batch-only", action="store_true", help="profiling aid: run only the batched multi-mesh leg")
@@ -485,6 +500,14 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "copies": 4}), flush=True)
+        return 0
     if args.contact_only:
         print(json.dumps({"contact": contact_leg(args, log)}), flush=True)
         return 0
@@ -557,7 +580,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic("cast_rays_kernel"),
             "kernel": "cast_rays_kernel",
             "kernel_avg_ms": round(kern_ms, 5),
             "algorithmic_bytes_per_launch": alg_bytes,

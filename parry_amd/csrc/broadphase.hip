@@ -2,15 +2,14 @@
 //
 // Refit (Bvh::insert_or_update_partially with margin 0 for every leaf, then
 // Bvh::refit, src/partitioning/bvh/bvh_insert.rs:209-231, bvh_refit.rs:170-320):
-// one lane per leaf writes the leaf's new box into its slot and climbs towards
-// the root.  Every wide node counts the arrivals of its two child slots; the
-// first arriver stops, the second one (which now sees both children) merges
-// them -- min/max, leaf counts, change flags, exactly BvhNode::merged
-// (bvh_tree.rs:610-618) -- writes the result into the parent's slot and climbs
-// on.  No lane ever waits, so the launch cannot stall; the second arriver
-// zeroes the counter, so counters are zero again for the next launch and no
-// clearing pass (memset) is needed.  Topology never changes, and the stored
-// layout is already depth-first, so parry's relayout is the identity.
+// one launch writes every leaf's new box into its slot; then the wide nodes are
+// refitted level by level, a level being the nodes of one height above the
+// leaves (computed once, when the tree is built), so each node merges two
+// child records that earlier levels finished -- min/max, leaf counts, change
+// flags, exactly BvhNode::merged (bvh_tree.rs:610-618).  Wide levels get one
+// launch each; the narrow top levels run in one workgroup with a barrier
+// between levels.  Topology never changes, and the stored layout is already
+// depth-first, so parry's relayout is the identity.
 //
 // Pairs (Bvh::traverse_bvtt_single_tree::<false>, bvh_traverse_bvtt.rs:19-204):
 // parry reports every pair of leaves whose boxes overlap, which is the set
@@ -48,15 +47,16 @@ __device__ __forceinline__ void load_node_coherent(const pq_bvh_node *n, float m
     *data = ld_coherent(&n->data);
 }
 
-__global__ __launch_bounds__(kBlock) void refit_kernel(pq_bvh_node_wide *__restrict__ nodes,
-                                                       const uint32_t *__restrict__ leaf_slot,
-                                                       const uint32_t *__restrict__ parent_slot,
-                                                       uint32_t *__restrict__ arrive, const float *__restrict__ aabbs,
-                                                       uint32_t n, int propagate) {
+// Leaf update (insert_or_update_partially with margin 0, bvh_insert.rs:209-231):
+// the leaf's record takes its new box and resolves a pending change flag
+// (bvh_tree.rs:192-198).
+__global__ __launch_bounds__(kBlock) void refit_leaves_kernel(pq_bvh_node_wide *__restrict__ nodes,
+                                                              const uint32_t *__restrict__ leaf_slot,
+                                                              const float *__restrict__ aabbs, uint32_t n) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t slot = leaf_slot[i];
-    uint32_t wide = slot >> 1;
+    const uint32_t wide = slot >> 1;
     pq_bvh_node *leaf = (slot & 1u) ? &nodes[wide].right : &nodes[wide].left;
     const float *a = aabbs + 6 * (size_t)i;
     leaf->mins[0] = a[0];
@@ -65,28 +65,61 @@ __global__ __launch_bounds__(kBlock) void refit_kernel(pq_bvh_node_wide *__restr
     leaf->maxs[0] = a[3];
     leaf->maxs[1] = a[4];
     leaf->maxs[2] = a[5];
-    // resolve_pending_change (bvh_tree.rs:192-198)
     const uint32_t d = leaf->data;
     leaf->data = (d & 0x3fffffffu) | (((d >> 30) == kFlagPending ? kFlagChanged : 0u) << 30);
-    if (!propagate) return;  // a root with at most two leaves has no ancestor boxes
-    for (;;) {
-        __threadfence();
-        if (atomicAdd(&arrive[wide], 1u) == 0u) return;  // the sibling slot is not written yet
-        arrive[wide] = 0u;
-        __threadfence();
-        if (wide == 0) return;  // the root's own box is not stored
-        float lmn[3], lmx[3], rmn[3], rmx[3];
-        uint32_t ld, rd;
-        load_node_coherent(&nodes[wide].left, lmn, lmx, &ld);
-        load_node_coherent(&nodes[wide].right, rmn, rmx, &rd);
-        const uint32_t ps = parent_slot[wide];
-        pq_bvh_node *dst = (ps & 1u) ? &nodes[ps >> 1].right : &nodes[ps >> 1].left;
+}
+
+// The record of wide node w in its parent: the union of w's two child records
+// (bvh_refit.rs:170-320; min/max are exact, so any order gives parry's boxes),
+// leaf counts summed, change flags or-ed.
+__device__ __forceinline__ void refit_node(pq_bvh_node_wide *nodes, const uint32_t *parent_slot, uint32_t w,
+                                           bool coherent) {
+    float lmn[3], lmx[3], rmn[3], rmx[3];
+    uint32_t ld, rd;
+    if (coherent) {
+        load_node_coherent(&nodes[w].left, lmn, lmx, &ld);
+        load_node_coherent(&nodes[w].right, rmn, rmx, &rd);
+    } else {
+        const pq_bvh_node l = nodes[w].left, r = nodes[w].right;
         for (int k = 0; k < 3; ++k) {
-            dst->mins[k] = fminf(lmn[k], rmn[k]);
-            dst->maxs[k] = fmaxf(lmx[k], rmx[k]);
+            lmn[k] = l.mins[k];
+            lmx[k] = l.maxs[k];
+            rmn[k] = r.mins[k];
+            rmx[k] = r.maxs[k];
         }
-        dst->data = ((ld & 0x3fffffffu) + (rd & 0x3fffffffu)) | (((ld >> 30) | (rd >> 30)) << 30);
-        wide = ps >> 1;
+        ld = l.data;
+        rd = r.data;
+    }
+    const uint32_t ps = parent_slot[w];
+    pq_bvh_node *dst = (ps & 1u) ? &nodes[ps >> 1].right : &nodes[ps >> 1].left;
+    for (int k = 0; k < 3; ++k) {
+        dst->mins[k] = fminf(lmn[k], rmn[k]);
+        dst->maxs[k] = fmaxf(lmx[k], rmx[k]);
+    }
+    dst->data = ((ld & 0x3fffffffu) + (rd & 0x3fffffffu)) | (((ld >> 30) | (rd >> 30)) << 30);
+}
+
+// One level of the refit: the nodes of one height, in parallel.
+__global__ __launch_bounds__(kBlock) void refit_level_kernel(pq_bvh_node_wide *__restrict__ nodes,
+                                                             const uint32_t *__restrict__ parent_slot,
+                                                             const uint32_t *__restrict__ ids, uint32_t count) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < count) refit_node(nodes, parent_slot, ids[k], false);
+}
+
+// The narrow top levels in one workgroup, a barrier between levels.
+constexpr uint32_t kTopBlock = 1024;
+constexpr uint32_t kTopLevelMax = 4096;  // levels at most this wide go to the one-workgroup kernel
+__global__ __launch_bounds__(kTopBlock) void refit_top_kernel(pq_bvh_node_wide *nodes,
+                                                              const uint32_t *__restrict__ parent_slot,
+                                                              const uint32_t *__restrict__ ids,
+                                                              const uint32_t *__restrict__ level_offsets,
+                                                              uint32_t first_level, uint32_t last_level) {
+    for (uint32_t h = first_level; h <= last_level; ++h) {
+        const uint32_t begin = level_offsets[h - 1], end = level_offsets[h];
+        for (uint32_t k = begin + threadIdx.x; k < end; k += kTopBlock) refit_node(nodes, parent_slot, ids[k], true);
+        __threadfence();
+        __syncthreads();
     }
 }
 
@@ -172,9 +205,23 @@ __global__ void copy_total_kernel(const unsigned long long *offsets, uint32_t n,
 
 hipError_t launch_bvh_refit(const DeviceBvh &b, const float *aabbs, hipStream_t stream) {
     if (b.num_leaves == 0) return hipSuccess;
-    hipLaunchKernelGGL(refit_kernel, dim3((b.num_leaves + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, b.nodes,
-                       b.leaf_slot, b.parent_slot, b.arrive, aabbs, b.num_leaves, b.num_leaves > 2 ? 1 : 0);
-    return hipGetLastError();
+    hipLaunchKernelGGL(refit_leaves_kernel, dim3((b.num_leaves + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
+                       b.nodes, b.leaf_slot, aabbs, b.num_leaves);
+    hipError_t e = hipGetLastError();
+    uint32_t h = 1;
+    for (; e == hipSuccess && h <= b.num_levels; ++h) {
+        const uint32_t begin = b.level_offsets_host[h - 1], count = b.level_offsets_host[h] - begin;
+        if (count <= kTopLevelMax) break;
+        hipLaunchKernelGGL(refit_level_kernel, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, b.nodes,
+                           b.parent_slot, b.level_nodes + begin, count);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && h <= b.num_levels) {
+        hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, b.nodes, b.parent_slot,
+                           b.level_nodes, b.level_offsets, h, b.num_levels);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 hipError_t bvh_scan_temp_bytes(uint32_t num_leaves, size_t *bytes) {

@@ -1,4 +1,5 @@
 // capi_broadphase.cpp -- extern "C" entry points for shapes and the broadphase Bvh.
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -109,8 +110,33 @@ int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out) {
             else if (lc > 1) parent_slot[c.children] = 2 * w + s;
         }
     }
+    // Heights above the leaves: depth-first order puts children after parents, so
+    // one reverse sweep suffices.  The root's own box is not stored, so it is not
+    // refitted; trees with at most two leaves have no other wide node.
+    std::vector<uint32_t> height(nn, 0);
+    uint32_t max_h = 0;
+    for (uint32_t w = nn; w-- > 0;) {
+        uint32_t h = 1;
+        const pq_bvh_node *side[2] = {&nodes[w].left, &nodes[w].right};
+        for (const pq_bvh_node *c : side)
+            if (leaf_count(*c) > 1) h = std::max(h, height[c->children] + 1);
+        height[w] = h;
+        if (w != 0) max_h = std::max(max_h, h);
+    }
+    // offs[h] = number of nodes of height <= h: level h occupies [offs[h-1], offs[h]).
+    std::vector<uint32_t> offs(max_h + 1, 0), level_nodes(nn > 0 ? nn - 1 : 0);
+    for (uint32_t w = 1; w < nn; ++w) ++offs[height[w]];
+    for (uint32_t h = 1; h <= max_h; ++h) offs[h] += offs[h - 1];
+    {
+        std::vector<uint32_t> fill(offs);  // next free position of each level: starts at offs[h-1]
+        for (uint32_t h = max_h; h >= 1; --h) fill[h] = offs[h - 1];
+        for (uint32_t w = 1; w < nn; ++w) level_nodes[fill[height[w]]++] = w;
+    }
     auto *b = new pq_bvh();
     DeviceBvh &d = b->dev;
+    b->level_offsets = offs;
+    d.num_levels = max_h;
+    d.level_offsets_host = b->level_offsets.data();
     d.num_leaves = n;
     d.num_nodes = nn;
     d.depth = depth;
@@ -121,10 +147,11 @@ int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out) {
     if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.nodes, nn * sizeof(pq_bvh_node_wide));
     if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.leaf_slot, n * sizeof(uint32_t));
     if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.parent_slot, parent_slot.size() * sizeof(uint32_t));
-    if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.arrive, parent_slot.size() * sizeof(uint32_t));
     if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.pair_counts, ((size_t)n + 1) * sizeof(uint32_t));
     if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.pair_offsets, ((size_t)n + 1) * sizeof(unsigned long long));
     if (rc == PQ_OK) rc = dev_alloc(b, &d.scan_temp, scan);
+    if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.level_nodes, level_nodes.size() * sizeof(uint32_t));
+    if (rc == PQ_OK) rc = dev_alloc(b, (void **)&d.level_offsets, offs.size() * sizeof(uint32_t));
     d.scan_temp_bytes = scan;
     if (rc == PQ_OK) {
         e = hipSuccess;
@@ -134,7 +161,11 @@ int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out) {
         if (e == hipSuccess)
             e = hipMemcpy(d.parent_slot, parent_slot.data(), parent_slot.size() * sizeof(uint32_t),
                           hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemset(d.arrive, 0, parent_slot.size() * sizeof(uint32_t));
+        if (e == hipSuccess && !level_nodes.empty())
+            e = hipMemcpy(d.level_nodes, level_nodes.data(), level_nodes.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(d.level_offsets, offs.data(), offs.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemset(d.pair_counts, 0, ((size_t)n + 1) * sizeof(uint32_t));
         if (e == hipSuccess) e = hipMemset(d.pair_offsets, 0, ((size_t)n + 1) * sizeof(unsigned long long));
         if (e != hipSuccess) rc = hip_fail(e, "pq_bvh_create upload");

@@ -124,7 +124,6 @@ struct DeviceBvh {
     pq_bvh_node_wide *nodes;  // device, depth-first
     uint32_t *leaf_slot;      // device, per leaf: wide * 2 + (1 if right)
     uint32_t *parent_slot;    // device, per wide node: parent wide * 2 + side (root: UINT32_MAX)
-    uint32_t *arrive;         // device, per wide node arrival counter (zero between launches)
     uint32_t *pair_counts;    // device, n + 1
     unsigned long long *pair_offsets;  // device, n + 1
     void *scan_temp;
@@ -132,6 +131,12 @@ struct DeviceBvh {
     uint32_t num_leaves;
     uint32_t num_nodes;
     uint32_t depth;
+    // Refit order: the non-root wide nodes grouped by height above the leaves
+    // (a node's children all have smaller heights).
+    uint32_t *level_nodes;             // device, the nodes level by level
+    uint32_t *level_offsets;           // device, num_levels + 1
+    const uint32_t *level_offsets_host;  // the same offsets on the host
+    uint32_t num_levels;
 };
 
 hipError_t launch_bvh_refit(const DeviceBvh &b, const float *aabbs, hipStream_t stream);
@@ -167,6 +172,7 @@ struct pq_shapes {
 struct pq_bvh {
     pq::DeviceBvh dev;
     std::vector<void *> allocations;
+    std::vector<uint32_t> level_offsets;  // host copy backing dev.level_offsets_host
 };
 
 struct pq_graph {
