@@ -275,8 +275,11 @@ typedef struct pq_contact {
  *       tier: 64 (one wave per pair), 32, 16, 8 or 4;
  *   epa_polytope: the first tier's polytope buffers: 0 = 32 vertices / 96 faces
  *       with the ConvexPolyhedra's vertices staged in LDS, 1 = the same without
- *       staging, 2 = 24 vertices / 64 faces without staging (less LDS per pair,
- *       more pairs resident; larger polytopes move on to one-wave-per-pair tiers). */
+ *       staging, 2 = 24 vertices / 64 faces, 3 = 32 vertices / 160 faces,
+ *       4 = 48 vertices / 224 faces, all three without staging (less LDS per pair
+ *       means more pairs resident; a pair whose polytope outgrows the buffers
+ *       moves on to the one-wave-per-pair tiers).
+ * The default is 16 lanes per pair with polytope 1. */
 typedef struct pq_contact_tuning {
     uint32_t epa_lanes_per_pair;
     uint32_t epa_polytope;

@@ -111,7 +111,7 @@ class ContactTuning(ctypes.Structure):
     _fields_ = [("epa_lanes_per_pair", _u32), ("epa_polytope", _u32), ("reserved", _u32 * 2)]
 
 
-def set_contact_tuning(epa_lanes_per_pair: int, epa_polytope: int = 0) -> None:
+def set_contact_tuning(epa_lanes_per_pair: int, epa_polytope: int = 1) -> None:
     check(lib().pq_contact_set_tuning(ctypes.byref(ContactTuning(epa_lanes_per_pair, epa_polytope))))
 
 

@@ -30,23 +30,6 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kFlagChanged = 1u, kFlagPending = 3u;
 
-__device__ __forceinline__ uint32_t ld_coherent(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float ld_coherent_f(const float *p) {
-    return __uint_as_float(ld_coherent(reinterpret_cast<const uint32_t *>(p)));
-}
-
-// Loads a BvhNode (mins, children, maxs, data) written by another lane of the
-// same launch, bypassing the non-coherent per-CU cache.
-__device__ __forceinline__ void load_node_coherent(const pq_bvh_node *n, float mn[3], float mx[3], uint32_t *data) {
-    for (int d = 0; d < 3; ++d) {
-        mn[d] = ld_coherent_f(&n->mins[d]);
-        mx[d] = ld_coherent_f(&n->maxs[d]);
-    }
-    *data = ld_coherent(&n->data);
-}
-
 // Leaf update (insert_or_update_partially with margin 0, bvh_insert.rs:209-231):
 // the leaf's record takes its new box and resolves a pending change flag
 // (bvh_tree.rs:192-198).
@@ -72,31 +55,15 @@ __global__ __launch_bounds__(kBlock) void refit_leaves_kernel(pq_bvh_node_wide *
 // The record of wide node w in its parent: the union of w's two child records
 // (bvh_refit.rs:170-320; min/max are exact, so any order gives parry's boxes),
 // leaf counts summed, change flags or-ed.
-__device__ __forceinline__ void refit_node(pq_bvh_node_wide *nodes, const uint32_t *parent_slot, uint32_t w,
-                                           bool coherent) {
-    float lmn[3], lmx[3], rmn[3], rmx[3];
-    uint32_t ld, rd;
-    if (coherent) {
-        load_node_coherent(&nodes[w].left, lmn, lmx, &ld);
-        load_node_coherent(&nodes[w].right, rmn, rmx, &rd);
-    } else {
-        const pq_bvh_node l = nodes[w].left, r = nodes[w].right;
-        for (int k = 0; k < 3; ++k) {
-            lmn[k] = l.mins[k];
-            lmx[k] = l.maxs[k];
-            rmn[k] = r.mins[k];
-            rmx[k] = r.maxs[k];
-        }
-        ld = l.data;
-        rd = r.data;
-    }
+__device__ __forceinline__ void refit_node(pq_bvh_node_wide *nodes, const uint32_t *parent_slot, uint32_t w) {
+    const pq_bvh_node l = nodes[w].left, r = nodes[w].right;
     const uint32_t ps = parent_slot[w];
     pq_bvh_node *dst = (ps & 1u) ? &nodes[ps >> 1].right : &nodes[ps >> 1].left;
     for (int k = 0; k < 3; ++k) {
-        dst->mins[k] = fminf(lmn[k], rmn[k]);
-        dst->maxs[k] = fmaxf(lmx[k], rmx[k]);
+        dst->mins[k] = fminf(l.mins[k], r.mins[k]);
+        dst->maxs[k] = fmaxf(l.maxs[k], r.maxs[k]);
     }
-    dst->data = ((ld & 0x3fffffffu) + (rd & 0x3fffffffu)) | (((ld >> 30) | (rd >> 30)) << 30);
+    dst->data = ((l.data & 0x3fffffffu) + (r.data & 0x3fffffffu)) | (((l.data >> 30) | (r.data >> 30)) << 30);
 }
 
 // One level of the refit: the nodes of one height, in parallel.
@@ -104,10 +71,14 @@ __global__ __launch_bounds__(kBlock) void refit_level_kernel(pq_bvh_node_wide *_
                                                              const uint32_t *__restrict__ parent_slot,
                                                              const uint32_t *__restrict__ ids, uint32_t count) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k < count) refit_node(nodes, parent_slot, ids[k], false);
+    if (k < count) refit_node(nodes, parent_slot, ids[k]);
 }
 
-// The narrow top levels in one workgroup, a barrier between levels.
+// The narrow top levels in one workgroup, a barrier between levels.  The
+// workgroup runs on one CU, whose waves share its L1, so the barrier's
+// workgroup-scope release/acquire is enough for one level to see the records
+// the previous level wrote; nothing has to be written back to or invalidated
+// in the L2 between levels.
 constexpr uint32_t kTopBlock = 1024;
 constexpr uint32_t kTopLevelMax = 4096;  // levels at most this wide go to the one-workgroup kernel
 __global__ __launch_bounds__(kTopBlock) void refit_top_kernel(pq_bvh_node_wide *nodes,
@@ -117,8 +88,7 @@ __global__ __launch_bounds__(kTopBlock) void refit_top_kernel(pq_bvh_node_wide *
                                                               uint32_t first_level, uint32_t last_level) {
     for (uint32_t h = first_level; h <= last_level; ++h) {
         const uint32_t begin = level_offsets[h - 1], end = level_offsets[h];
-        for (uint32_t k = begin + threadIdx.x; k < end; k += kTopBlock) refit_node(nodes, parent_slot, ids[k], true);
-        __threadfence();
+        for (uint32_t k = begin + threadIdx.x; k < end; k += kTopBlock) refit_node(nodes, parent_slot, ids[k]);
         __syncthreads();
     }
 }

@@ -222,10 +222,10 @@ size_t pq_contact_workspace_bytes(uint64_t num_pairs) { return contact_workspace
 int pq_contact_set_tuning(const pq_contact_tuning *t) {
     if (!t) return fail(PQ_ERR_INVALID, "pq_contact_set_tuning: null tuning");
     const uint32_t g = t->epa_lanes_per_pair;
-    if ((g != 64 && g != 32 && g != 16 && g != 8 && g != 4) || t->epa_polytope > 2 || t->reserved[0] ||
+    if ((g != 64 && g != 32 && g != 16 && g != 8 && g != 4) || t->epa_polytope > 4 || t->reserved[0] ||
         t->reserved[1])
         return fail(PQ_ERR_INVALID,
-                    "pq_contact_set_tuning: epa_lanes_per_pair must be 64, 32, 16, 8 or 4 and epa_polytope 0, 1 or 2");
+                    "pq_contact_set_tuning: epa_lanes_per_pair must be 64, 32, 16, 8 or 4 and epa_polytope 0 to 4");
     g_epa_lanes_per_pair = (int)g;
     g_epa_polytope = (int)t->epa_polytope;
     return PQ_OK;

@@ -105,8 +105,11 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// counters[0..3]: EPA queue length and the three overflow-list lengths;
+// counters[4..7]: the next list entry each EPA tier hands out.
+constexpr uint32_t kNumCounters = 8;
 __global__ void reset_counters_kernel(uint32_t *counters) {
-    if (threadIdx.x < 4) counters[threadIdx.x] = 0u;
+    if (threadIdx.x < kNumCounters) counters[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -297,9 +300,6 @@ template <int V, int F, int H, int S, int K, class I, int SV> struct EpaState {
     float heap_nd[H];
     uint32_t sil[S];  // face | opp << 16
     uint32_t stk[K];  // face | opp << 16
-    float fdist[S];   // per silhouette edge: the new face's distance (if proj_inside)
-    I nid[S];         // per silhouette edge: the new face's id
-    uint8_t fin[S];   // per silhouette edge: the new face's proj_inside
     uint8_t vis[F];   // per face: can_be_seen_by the newest vertex, one bit per opposite vertex
     float sv[2][SV > 0 ? SV : 1][3];  // both ConvexPolyhedra's vertices (local frames), if SV > 0
 };
@@ -311,6 +311,10 @@ using EpaSmall = EpaState<32, 96, 96, 32, 96, uint8_t, kStagedVerts>;
 // more pairs resident per CU (selected by pq_contact_tuning.epa_polytope).
 using EpaSmallUnstaged = EpaState<32, 96, 96, 32, 96, uint8_t, 0>;
 using EpaTiny = EpaState<24, 64, 64, 24, 64, uint8_t, 0>;
+// More faces and heap entries for the same vertices, and a larger polytope
+// still on byte ids: fewer pairs move on to the one-wave-per-pair tiers.
+using EpaSmallWide = EpaState<32, 160, 160, 32, 96, uint8_t, 0>;
+using EpaSmallLarge = EpaState<48, 224, 224, 48, 128, uint8_t, 0>;
 // One wave per pair, room for parry's 100 iterations (the vertex count) and for
 // the faces a long run creates: ~31 KB and ~81 KB of LDS.
 using EpaMedium = EpaState<112, 768, 768, 128, 512, uint16_t, kStagedVerts>;
@@ -440,6 +444,13 @@ template <class W> struct Heap {
         w->heap_nd[pos] = end;
         return pos;
     }
+    // Places an element at position pos (the heap's current end plus the pushes
+    // before it) and sifts it up; the caller advances len.
+    __device__ void insert_at(uint32_t pos, uint32_t id, float nd) {
+        w->heap_id[pos] = (typename W::Idx)id;
+        w->heap_nd[pos] = nd;
+        sift_up(0, pos);
+    }
     __device__ bool push(uint32_t id, float nd) {
         if (len >= (uint32_t)W::kH) return false;
         w->heap_id[len] = (typename W::Idx)id;
@@ -557,6 +568,18 @@ template <class W> __device__ void face_points(const W *w, const float *orig, ui
     }
     p1 = add3(add3(mul3(o1[0], bc[0]), mul3(o1[1], bc[1])), mul3(o1[2], bc[2]));
     p2 = add3(add3(mul3(o2[0], bc[0]), mul3(o2[1], bc[1])), mul3(o2[2], bc[2]));
+}
+
+// Element k (0, 1 or 2) of three values, by selects rather than an indexed array.
+__device__ __forceinline__ uint32_t pick3(uint32_t k, uint32_t a, uint32_t b, uint32_t c) {
+    return k == 0 ? a : (k == 1 ? b : c);
+}
+
+// Face::next_ccw_pt_id (epa3.rs:135-151) on a face record held in registers.
+template <class Fc> __device__ __forceinline__ uint32_t ccw_index(const Fc &f, uint32_t id) {
+    if (f.pts[0] == id) return 1;
+    if (f.pts[1] == id) return 2;
+    return 0;
 }
 
 template <class W> __device__ __forceinline__ uint32_t next_ccw(const W *w, uint32_t f, uint32_t id) {
@@ -685,9 +708,11 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
             const uint32_t sub = __lane_id() % G;
             const f3 pt = vpt(w, sp);
             const float E = kEps * 100.0f;
+            // Deleted faces get bits too (the walk never reads them): with no
+            // branch in the body, the loads of consecutive faces overlap.
+#pragma unroll 2
             for (uint32_t f = sub; f < nf; f += G) {
-                if (w->face[f].deleted) continue;
-                const typename W::Face &F = w->face[f];
+                const typename W::Face F = w->face[f];
                 const f3 q0 = vpt(w, F.pts[0]), q1 = vpt(w, F.pts[1]), q2 = vpt(w, F.pts[2]);
                 const f3 fnn = f3{F.n[0], F.n[1], F.n[2]};
                 const f3 qs[3] = {q0, q1, q2};
@@ -695,80 +720,137 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
 #pragma unroll
                 for (int opp = 0; opp < 3; ++opp) {
                     const f3 p0 = qs[opp], p1 = qs[(opp + 1) % 3], p2 = qs[(opp + 2) % 3];
-                    const bool seen = dot3(sub3(pt, p0), fnn) >= -kEpsTol ||
-                                      norm_sq3(cross3(sub3(p2, p1), sub3(pt, p1))) <= E * E;  // is_affinely_dependent
-                    bits |= (seen ? 1u : 0u) << opp;
+                    const bool front = dot3(sub3(pt, p0), fnn) >= -kEpsTol;
+                    const bool flat = norm_sq3(cross3(sub3(p2, p1), sub3(pt, p1))) <= E * E;  // is_affinely_dependent
+                    bits |= (uint32_t)(front | flat) << opp;
                 }
                 w->vis[f] = (uint8_t)bits;
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
         }
-        // compute_silhouette for the three neighbours, depth first (explicit stack).
-        uint32_t ns = 0, sk = 0;
-        for (int q = 2; q >= 0; --q) {
-            const uint32_t adj = face.adj[q];
-            w->stk[sk++] = adj | (next_ccw(w, adj, face.pts[q]) << 16);
+        // compute_silhouette (epa3.rs:653-675) for the three neighbours in order
+        // adj[0], adj[1], adj[2]; a visible face recurses into adj[(opp + 2) % 3],
+        // then adj[opp].  The recursion is an explicit stack of face | opp << 16
+        // whose top entry lives in a register.  After a visible face, the face
+        // visited next (its first far neighbour) is loaded together with the
+        // second one's points and carried in registers, so a step costs one
+        // round trip to LDS.
+        uint32_t ns = 0, sk = 0, depth = 0, top = 0;
+        typename W::Face cf = w->face[face.adj[0]];
+        uint32_t cvis = w->vis[face.adj[0]];
+        uint32_t cur = face.adj[0] | (ccw_index(cf, face.pts[0]) << 16);
+        {
+            const uint32_t o2 = next_ccw(w, face.adj[2], face.pts[2]);
+            const uint32_t o1 = next_ccw(w, face.adj[1], face.pts[1]);
+            w->stk[0] = face.adj[2] | (o2 << 16);
+            top = face.adj[1] | (o1 << 16);
+            sk = 1;
+            depth = 2;
         }
-        while (sk > 0) {
-            const uint32_t e = w->stk[--sk];
-            const uint32_t id = e & 0xffffu, opp = e >> 16;
-            if (w->face[id].deleted) continue;
-            if (!((w->vis[id] >> opp) & 1u)) {
-                if (ns >= (uint32_t)W::kS) return EPA_OVERFLOW;
-                w->sil[ns++] = e;
-            } else {
-                w->face[id].deleted = 1;
-                const uint32_t a1 = (opp + 2) % 3, a2 = opp;
-                const uint32_t adj1 = w->face[id].adj[a1], adj2 = w->face[id].adj[a2];
-                const uint32_t o1 = next_ccw(w, adj1, w->face[id].pts[a1]);
-                const uint32_t o2 = next_ccw(w, adj2, w->face[id].pts[a2]);
-                if (sk + 2 > (uint32_t)W::kK) return EPA_OVERFLOW;
-                w->stk[sk++] = adj2 | (o2 << 16);
-                w->stk[sk++] = adj1 | (o1 << 16);
+        for (;;) {
+            const uint32_t id = cur & 0xffffu, opp = cur >> 16;
+            bool from_stack = true;
+            if (!cf.deleted) {
+                if (!((cvis >> opp) & 1u)) {
+                    if (ns >= (uint32_t)W::kS) return EPA_OVERFLOW;
+                    w->sil[ns++] = cur;
+                } else {
+                    w->face[id].deleted = 1;
+                    // a1 = (opp + 2) % 3 and a2 = opp, selected by value so the
+                    // record stays in registers
+                    const uint32_t adj1 = pick3(opp, cf.adj[2], cf.adj[0], cf.adj[1]);
+                    const uint32_t adj2 = pick3(opp, cf.adj[0], cf.adj[1], cf.adj[2]);
+                    const uint32_t pt1 = pick3(opp, cf.pts[2], cf.pts[0], cf.pts[1]);
+                    const uint32_t pt2 = pick3(opp, cf.pts[0], cf.pts[1], cf.pts[2]);
+                    const typename W::Face f1 = w->face[adj1];
+                    const uint32_t v1 = w->vis[adj1];
+                    const uint32_t o2 = next_ccw(w, adj2, pt2);
+                    const uint32_t o1 = ccw_index(f1, pt1);
+                    if (depth > 0) {
+                        if (sk >= (uint32_t)W::kK) return EPA_OVERFLOW;
+                        w->stk[sk++] = top;
+                    }
+                    top = adj2 | (o2 << 16);
+                    ++depth;
+                    cur = adj1 | (o1 << 16);
+                    cf = f1;
+                    if (adj1 == id) cf.deleted = 1;  // a face adjacent to itself (degenerate ring)
+                    cvis = v1;
+                    from_stack = false;
+                }
+            }
+            if (from_stack) {
+                if (depth == 0) break;
+                cur = top;
+                if (--depth > 0) top = w->stk[--sk];
+                cf = w->face[cur & 0xffffu];
+                cvis = w->vis[cur & 0xffffu];
             }
         }
         const uint32_t first_new = nf;
         if (ns == 0) return EPA_NONE;
-        // The new faces (epa3.rs:598-626): the silhouette edges whose face is still
-        // live get consecutive ids in silhouette order; the faces are built by the
-        // group's lanes in parallel (Face::new reads vertex positions only and each
-        // edge rewrites its own adjacency slot), then the in-order pass applies
-        // parry's early exits and heap pushes.
+        // The new faces (epa3.rs:598-626), G silhouette edges at a time, one per
+        // lane.  The edges whose face is still live get consecutive ids in
+        // silhouette order (a ballot gives each its rank), and each lane builds
+        // its edge's face (Face::new reads vertex positions only, and each edge
+        // rewrites its own adjacency slot).  Parry's loop then either returns at
+        // the first new face that ends EPA -- its heap pushes before that would
+        // be discarded anyway -- or pushes every new face in silhouette order;
+        // the lane holding an edge's face does that edge's push.
         uint32_t n_alive = 0;
-        for (uint32_t q = 0; q < ns; ++q) n_alive += w->face[w->sil[q] & 0xffffu].deleted ? 0u : 1u;
-        if (first_new + n_alive > (uint32_t)W::kF) return EPA_OVERFLOW;
         {
             const uint32_t sub = __lane_id() % G;
-            for (uint32_t q = sub; q < ns; q += G) {
-                const uint32_t e = w->sil[q];
-                const uint32_t ef = e & 0xffffu, opp = e >> 16;
-                if (w->face[ef].deleted) continue;
-                uint32_t rank = 0;
-                for (uint32_t r = 0; r < q; ++r) rank += w->face[w->sil[r] & 0xffffu].deleted ? 0u : 1u;
-                const uint32_t new_id = first_new + rank;
-                const uint32_t pt1 = w->face[ef].pts[(opp + 2) % 3];
-                const uint32_t pt2 = w->face[ef].pts[(opp + 1) % 3];
-                const bool in = face_new(w, new_id, pt1, pt2, sp, ef, new_id + 1, new_id - 1);
-                w->face[ef].adj[(opp + 1) % 3] = (typename W::Idx)new_id;
-                w->fin[q] = in ? 1 : 0;
-                w->nid[q] = (typename W::Idx)new_id;
-                w->fdist[q] = in ? dot3(fnormal(w, new_id), vpt(w, pt1)) : 0.0f;
+            const uint32_t gbase = __lane_id() - sub;
+            const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+            const uint64_t below = (1ull << sub) - 1ull;
+            for (uint32_t c = 0; c < ns; c += G) {
+                const uint32_t q = c + sub;
+                uint32_t e = 0;
+                bool alive = false;
+                if (q < ns) {
+                    e = w->sil[q];
+                    alive = !w->face[e & 0xffffu].deleted;
+                }
+                const uint64_t am = (__ballot(alive) >> gbase) & gmask;
+                if (first_new + n_alive + (uint32_t)__popcll(am) > (uint32_t)W::kF) return EPA_OVERFLOW;
+                const uint32_t new_id = first_new + n_alive + (uint32_t)__popcll(am & below);
+                bool in = false;
+                float dist = 0.0f;
+                if (alive) {
+                    const uint32_t ef = e & 0xffffu, opp = e >> 16;
+                    const uint32_t pt1 = w->face[ef].pts[(opp + 2) % 3];
+                    const uint32_t pt2 = w->face[ef].pts[(opp + 1) % 3];
+                    in = face_new(w, new_id, pt1, pt2, sp, ef, new_id + 1, new_id - 1);
+                    w->face[ef].adj[(opp + 1) % 3] = (typename W::Idx)new_id;
+                    if (in) dist = dot3(fnormal(w, new_id), vpt(w, pt1));
+                }
+                n_alive += (uint32_t)__popcll(am);
+                const bool pushed = alive && in;
+                const bool closer = pushed && dist < curr_dist;
+                const bool stop = closer || (pushed && -dist > kEpsTol);
+                const uint64_t stop_m = (__ballot(stop) >> gbase) & gmask;
+                const uint64_t closer_m = (__ballot(closer) >> gbase) & gmask;
+                if (stop_m != 0) {
+                    if ((closer_m >> __builtin_ctzll(stop_m)) & 1ull) {
+                        face_points(w, orig, fid, p1, p2);  // the popped face: its points are unchanged
+                        n = fn;
+                        epa_iters = niter + 1;
+                        return EPA_OK;
+                    }
+                    return EPA_NONE;  // FaceId::new -> None
+                }
+                const uint64_t push_m = (__ballot(pushed) >> gbase) & gmask;
+                const uint32_t n_push = (uint32_t)__popcll(push_m);
+                if (heap.len + n_push > (uint32_t)W::kH) return EPA_OVERFLOW;
+                for (uint64_t m = push_m; m != 0; m &= m - 1) {
+                    if (sub == (uint32_t)__builtin_ctzll(m))
+                        heap.insert_at(heap.len + (uint32_t)__popcll(push_m & below), new_id, -dist);
+                }
+                heap.len += n_push;
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
-        }
-        for (uint32_t q = 0; q < ns; ++q) {
-            if (w->face[w->sil[q] & 0xffffu].deleted || !w->fin[q]) continue;
-            const float dist = w->fdist[q];
-            if (dist < curr_dist) {
-                face_points(w, orig, fid, p1, p2);  // the popped face: its points are unchanged
-                n = fn;
-                epa_iters = niter + 1;
-                return EPA_OK;
-            }
-            if (-dist > kEpsTol) return EPA_NONE;
-            if (!heap.push(w->nid[q], -dist)) return EPA_OVERFLOW;
         }
         nf = first_new + n_alive;
         if (first_new == nf) return EPA_NONE;
@@ -848,20 +930,30 @@ __global__ __launch_bounds__(64) void epa_tier_kernel(const ShapeRecord *__restr
                                                       const float4 *__restrict__ verts, const float *__restrict__ poses,
                                                       const uint32_t *__restrict__ pairs, float prediction,
                                                       const EpaItem *__restrict__ queue, const uint32_t *in_list,
-                                                      const uint32_t *in_count, DevContact *__restrict__ out,
-                                                      uint32_t *__restrict__ iters, uint32_t *out_list,
-                                                      uint32_t *out_count, float *orig_scratch) {
+                                                      const uint32_t *in_count, uint32_t *next,
+                                                      DevContact *__restrict__ out, uint32_t *__restrict__ iters,
+                                                      uint32_t *out_list, uint32_t *out_count, float *orig_scratch) {
     constexpr int kGroups = 64 / G;
     __shared__ W w[kGroups];
     const uint32_t group = threadIdx.x / G;
     const uint32_t gid = blockIdx.x * kGroups + group;
     float *orig = orig_scratch + (size_t)gid * W::kV * kOrigFloats;
     const uint32_t count = __hip_atomic_load(in_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t stride = gridDim.x * kGroups;
-    for (uint32_t q = gid; q < count; q += stride) {
-        const uint32_t slot = in_list ? in_list[q] : q;
-        epa_item<G>(&w[group], orig, rec, verts, poses, pairs, prediction, queue + slot, out, iters, out_list,
-                    out_count, slot);
+    // The wave takes its next kGroups list entries from the tier's counter
+    // whenever all its groups are done, so no wave is left with a long run of
+    // long pairs.  The fetch happens where the whole wave is converged and the
+    // exit test is wave-uniform: no cross-lane exchange under divergence.
+    for (;;) {
+        uint32_t base = 0;
+        if (__lane_id() == 0) base = atomicAdd(next, (uint32_t)kGroups);
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (base >= count) break;
+        const uint32_t q = base + group;
+        if (q < count) {
+            const uint32_t slot = in_list ? in_list[q] : q;
+            epa_item<G>(&w[group], orig, rec, verts, poses, pairs, prediction, queue + slot, out, iters, out_list,
+                        out_count, slot);
+        }
     }
 }
 
@@ -873,20 +965,25 @@ __global__ __launch_bounds__(64) void epa_global_kernel(const ShapeRecord *__res
                                                         const EpaItem *__restrict__ queue,
                                                         DevContact *__restrict__ out, uint32_t *__restrict__ iters,
                                                         const uint32_t *in_list, const uint32_t *in_count,
-                                                        EpaLarge *scratch, float *orig_scratch) {
+                                                        uint32_t *next, EpaLarge *scratch, float *orig_scratch) {
     EpaLarge *w = scratch + blockIdx.x;
     float *orig = orig_scratch + (size_t)blockIdx.x * EpaLarge::kV * kOrigFloats;
     const uint32_t count = __hip_atomic_load(in_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (uint32_t q = blockIdx.x; q < count; q += gridDim.x)
+    for (;;) {
+        uint32_t q = 0;
+        if (__lane_id() == 0) q = atomicAdd(next, 1u);
+        q = __builtin_amdgcn_readfirstlane(q);
+        if (q >= count) break;
         epa_item<64>(w, orig, rec, verts, poses, pairs, prediction, queue + in_list[q], out, iters, nullptr, nullptr,
                      q);
+    }
 }
 
 template <int G, class W>
 hipError_t launch_epa_tier(const DeviceShapes &s, const float *poses, const uint32_t *pairs, float prediction,
                            uint64_t max_items, const EpaItem *queue, const uint32_t *in_list, const uint32_t *in_count,
-                           DevContact *out, uint32_t *iters, uint32_t *out_list, uint32_t *out_count, float *orig,
-                           uint64_t max_groups, hipStream_t stream) {
+                           uint32_t *next, DevContact *out, uint32_t *iters, uint32_t *out_list, uint32_t *out_count,
+                           float *orig, uint64_t max_groups, hipStream_t stream) {
     hipError_t e;
     int dev = 0, cus = 0, per_cu = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
@@ -901,7 +998,7 @@ hipError_t launch_epa_tier(const DeviceShapes &s, const float *poses, const uint
     if (waves * kGroups > max_groups) waves = max_groups / kGroups;
     if (waves < 1) waves = 1;
     hipLaunchKernelGGL((epa_tier_kernel<G, W>), dim3((uint32_t)waves), dim3(64), 0, stream, s.rec, s.verts, poses,
-                       pairs, prediction, queue, in_list, in_count, out, iters, out_list, out_count, orig);
+                       pairs, prediction, queue, in_list, in_count, next, out, iters, out_list, out_count, orig);
     return hipGetLastError();
 }
 
@@ -910,14 +1007,20 @@ hipError_t launch_epa_tier(const DeviceShapes &s, const float *poses, const uint
 // Resident EPA groups the workspace provides orig1/orig2 buffers for, per tier.
 constexpr uint64_t kEpaMaxGroups = 256ull * 16 * 8;
 constexpr uint64_t kEpaMediumWaves = 256ull * 8;
+// orig1/orig2 rows per first-tier group: the most vertices of any first-tier polytope.
+constexpr uint64_t kSmallMaxVertices = 48;
+static_assert(EpaSmall::kV <= kSmallMaxVertices && EpaSmallUnstaged::kV <= kSmallMaxVertices &&
+                  EpaTiny::kV <= kSmallMaxVertices && EpaSmallWide::kV <= kSmallMaxVertices &&
+                  EpaSmallLarge::kV <= kSmallMaxVertices,
+              "first-tier orig buffer too small");
 constexpr uint64_t kEpaBigWaves = 256ull * 2;
 
 int g_epa_lanes_per_pair = 16;
-int g_epa_polytope = 0;
+int g_epa_polytope = 1;
 
 namespace {
 struct ContactWorkspace {
-    uint32_t *counters;  // queue length and the three overflow-list lengths
+    uint32_t *counters;  // list lengths and the tiers' next entries (reset_counters_kernel)
     EpaItem *queue;
     uint32_t *lists[3];
     EpaLarge *large;
@@ -936,12 +1039,12 @@ ContactWorkspace carve_workspace(void *base, uint64_t m) {
         off += up(bytes);
         return p;
     };
-    w.counters = (uint32_t *)take(16);
+    w.counters = (uint32_t *)take(kNumCounters * sizeof(uint32_t));
     w.queue = (EpaItem *)take(m * sizeof(EpaItem));
     for (int k = 0; k < 3; ++k) w.lists[k] = (uint32_t *)take(m * sizeof(uint32_t));
     w.large = (EpaLarge *)take(kEpaGlobalWaves * sizeof(EpaLarge));
     w.orig_large = (float *)take(kEpaGlobalWaves * EpaLarge::kV * kOrigFloats * sizeof(float));
-    w.orig_small = (float *)take(kEpaMaxGroups * EpaSmall::kV * kOrigFloats * sizeof(float));
+    w.orig_small = (float *)take(kEpaMaxGroups * kSmallMaxVertices * kOrigFloats * sizeof(float));
     w.orig_medium = (float *)take(kEpaMediumWaves * EpaMedium::kV * kOrigFloats * sizeof(float));
     w.orig_big = (float *)take(kEpaBigWaves * EpaBig::kV * kOrigFloats * sizeof(float));
     w.bytes = off;
@@ -958,13 +1061,19 @@ hipError_t launch_small(const DeviceShapes &s, const float *poses, const uint32_
                         const ContactWorkspace &w, uint32_t *c, DevContact *o, uint32_t *iters, hipStream_t stream) {
     switch (g_epa_polytope) {
         case 1:
-            return launch_epa_tier<G, EpaSmallUnstaged>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
+            return launch_epa_tier<G, EpaSmallUnstaged>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, o, iters,
                                                         w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
         case 2:
-            return launch_epa_tier<G, EpaTiny>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
+            return launch_epa_tier<G, EpaTiny>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, o, iters,
                                                w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
+        case 3:
+            return launch_epa_tier<G, EpaSmallWide>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, o, iters,
+                                                    w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
+        case 4:
+            return launch_epa_tier<G, EpaSmallLarge>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, o, iters,
+                                                     w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
         default:
-            return launch_epa_tier<G, EpaSmall>(s, poses, pairs, prediction, m, w.queue, nullptr, c, o, iters,
+            return launch_epa_tier<G, EpaSmall>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, o, iters,
                                                 w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
     }
 }
@@ -991,14 +1100,14 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
         default: e = launch_small<64>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
     }
     if (e != hipSuccess) return e;
-    e = launch_epa_tier<64, EpaMedium>(s, poses, pairs, prediction, m, w.queue, w.lists[0], c + 1, o, iters,
+    e = launch_epa_tier<64, EpaMedium>(s, poses, pairs, prediction, m, w.queue, w.lists[0], c + 1, c + 5, o, iters,
                                        w.lists[1], c + 2, w.orig_medium, kEpaMediumWaves, stream);
     if (e != hipSuccess) return e;
-    e = launch_epa_tier<64, EpaBig>(s, poses, pairs, prediction, m, w.queue, w.lists[1], c + 2, o, iters, w.lists[2],
-                                    c + 3, w.orig_big, kEpaBigWaves, stream);
+    e = launch_epa_tier<64, EpaBig>(s, poses, pairs, prediction, m, w.queue, w.lists[1], c + 2, c + 6, o, iters,
+                                    w.lists[2], c + 3, w.orig_big, kEpaBigWaves, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(epa_global_kernel, dim3(kEpaGlobalWaves), dim3(64), 0, stream, s.rec, s.verts, poses, pairs,
-                       prediction, w.queue, o, iters, w.lists[2], c + 3, w.large, w.orig_large);
+                       prediction, w.queue, o, iters, w.lists[2], c + 3, c + 7, w.large, w.orig_large);
     return hipGetLastError();
 }
 

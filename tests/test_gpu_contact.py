@@ -90,9 +90,9 @@ def test_mixed_support_map_pairs(orc):
     _compare("mixed", gpu, has, ref)
 
 
-def test_kat_and_deep_penetration(orc):
-    # Cuboid(2,1,1) KAT (epa3.rs:6-23) plus deep capsule / cone / cuboid overlaps,
-    # whose curved or flat polytopes take many EPA iterations.
+def _deep_case():
+    """Cuboid(2,1,1) KAT (epa3.rs:6-23) plus deep capsule / cone / cuboid overlaps,
+    whose curved or flat polytopes take many EPA iterations: (descs, poses, pairs)."""
     d = np.zeros(6, SHAPE_DESC_DTYPE)
     d["type"] = [1, 1, 2, 2, 3, 3]
     d["params"][0] = [2, 1, 1, 0]
@@ -101,7 +101,6 @@ def test_kat_and_deep_penetration(orc):
     d["params"][3] = [0.7, 0.6, 0, 0]
     d["params"][4] = [1.0, 0.8, 0, 0]
     d["params"][5] = [0.9, 1.1, 0, 0]
-    shapes = ShapeSet(d, np.zeros((0, 3), np.float32))
     rng = np.random.default_rng(5)
     poses, pairs = [], []
     for t in ((3.5, 0, 0), (0, 0.2, 0)):
@@ -115,8 +114,13 @@ def test_kat_and_deep_penetration(orc):
     k = poses.shape[0]
     kinds = np.concatenate([[0, 1, 0, 1], rng.integers(0, 6, k - 4)])
     dd = d[kinds]
-    shapes = ShapeSet(dd, np.zeros((0, 3), np.float32))
     pairs = np.arange(k, dtype=np.uint32).reshape(-1, 2)
+    return dd, poses, pairs
+
+
+def test_kat_and_deep_penetration(orc):
+    dd, poses, pairs = _deep_case()
+    shapes = ShapeSet(dd, np.zeros((0, 3), np.float32))
     print(f"case: KAT + deep penetration, {pairs.shape[0]} pairs", flush=True)
     gpu, has, iters = _run_gpu(shapes, poses, pairs, 10.0)
     ref = orc.contact_pairs(dd, np.zeros((0, 3), np.float32), poses, pairs, 10.0)
@@ -200,3 +204,35 @@ def test_ball_convex_pairs(orc):
     assert (poly & epa).sum() > 5
     assert gpu[-3, 12] == np.float32(-0.25) and tuple(gpu[-3, 6:9]) == (1.0, 0.0, 0.0)
     assert gpu[-1, 12] == np.float32(-0.75)
+
+
+def test_every_epa_tuning_matches_oracle(orc):
+    """Each first-tier EPA mapping (lanes per pair x polytope buffers) gives the
+    oracle's contacts and byte-identical records, on mixed broadphase pairs and
+    on the deep-penetration set (whose long runs reach the wave-per-pair tiers)."""
+    from parry_amd import _native
+
+    sc = scenes.mixed_scene(40000, seed=21)
+    descs, verts, poses = sc["descs"], sc["vertices"], sc["poses"]
+    pairs, _ = orc.OracleBvh(orc.shape_aabbs(descs, verts, poses)).self_pairs()
+    dd, dposes, dpairs = _deep_case()
+    cases = [("mixed", ShapeSet(descs, verts), descs, verts, poses, pairs, 0.01),
+             ("deep", ShapeSet(dd, np.zeros((0, 3), np.float32)), dd, np.zeros((0, 3), np.float32), dposes,
+              dpairs, 10.0)]
+    refs = [orc.contact_pairs(c[2], c[3], c[4], c[5], c[6]) for c in cases]
+    saved = _native.get_contact_tuning()
+    first = [None, None]
+    try:
+        for g in (64, 32, 16, 8, 4):
+            for poly in range(5):
+                _native.set_contact_tuning(g, poly)
+                for k, (label, shapes, _, _, p, pr, pred) in enumerate(cases):
+                    print(f"case: EPA tuning G={g} polytope={poly}, {label} ({pr.shape[0]} pairs)", flush=True)
+                    gpu, has, _ = _run_gpu(shapes, p, pr, pred)
+                    _compare(f"{label} G={g} polytope={poly}", gpu, has, refs[k])
+                    rec = np.concatenate([gpu.view(np.uint32), has[:, None]], 1)
+                    if first[k] is None:
+                        first[k] = rec
+                    assert np.array_equal(rec, first[k]), f"{label}: records differ between EPA tunings"
+    finally:
+        _native.set_contact_tuning(*saved)
