@@ -106,8 +106,9 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 }
 
 // counters[0..3]: EPA queue length and the three overflow-list lengths;
-// counters[4..7]: the next list entry each EPA tier hands out.
-constexpr uint32_t kNumCounters = 8;
+// counters[4..7]: the next list entry each EPA tier hands out;
+// counters[8]: continuation records taken by the first tier.
+constexpr uint32_t kNumCounters = 9;
 __global__ void reset_counters_kernel(uint32_t *counters) {
     if (threadIdx.x < kNumCounters) counters[threadIdx.x] = 0u;
 }
@@ -300,6 +301,8 @@ template <int V, int F, int H, int S, int K, class I, int SV> struct EpaState {
     float heap_nd[H];
     uint32_t sil[S];  // face | opp << 16
     uint32_t stk[K];  // face | opp << 16
+    I dl[S];          // faces the current iteration deleted (undone if it cannot commit)
+    uint32_t xfer;    // a value one lane passes to its group
     uint8_t vis[F];   // per face: can_be_seen_by the newest vertex, one bit per opposite vertex
     float sv[2][SV > 0 ? SV : 1][3];  // both ConvexPolyhedra's vertices (local frames), if SV > 0
 };
@@ -531,6 +534,15 @@ __device__ __forceinline__ bool face_bcoords(f3 pa, f3 pb, f3 pc, float bc[3]) {
     return false;
 }
 
+// ccw_face_normal (utils/ccw_face_normal.rs:19-27), zero if degenerate.
+__device__ __forceinline__ f3 ccw_normal(f3 pa, f3 pb, f3 pc) {
+    const f3 r = cross3(sub3(pb, pa), sub3(pc, pa));
+    const float sq = norm_sq3(r);
+    f3 n = f3{0.0f, 0.0f, 0.0f};
+    if (sq > kEps * kEps) n = div3(r, sqrtf(sq));
+    return n;
+}
+
 // Face::new: fills face f; returns proj_inside.
 template <class W>
 __device__ bool face_new(W *w, uint32_t f, uint32_t a, uint32_t b, uint32_t c, uint32_t j0, uint32_t j1, uint32_t j2) {
@@ -542,11 +554,7 @@ __device__ bool face_new(W *w, uint32_t f, uint32_t a, uint32_t b, uint32_t c, u
     const f3 pa = vpt(w, a), pb = vpt(w, b), pc = vpt(w, c);
     float bc[3];
     const bool in = face_bcoords(pa, pb, pc, bc);
-    // ccw_face_normal (utils/ccw_face_normal.rs:19-27), zero if degenerate
-    const f3 r = cross3(sub3(pb, pa), sub3(pc, pa));
-    const float sq = norm_sq3(r);
-    f3 n = f3{0.0f, 0.0f, 0.0f};
-    if (sq > kEps * kEps) n = div3(r, sqrtf(sq));
+    const f3 n = ccw_normal(pa, pb, pc);
     F.n[0] = n.x; F.n[1] = n.y; F.n[2] = n.z;
     return in;
 }
@@ -600,10 +608,136 @@ template <class W> __device__ bool seen_by(const W *w, uint32_t f, uint32_t poin
     return norm_sq3(cross3(sub3(p2, p1), sub3(pt, p1))) <= E * E;  // Triangle::is_affinely_dependent
 }
 
+// A first-tier EPA polytope at the start of an iteration, handed to the
+// wave-per-pair tier so that the pair continues where it stopped instead of
+// starting over (byte ids: at most 48 vertices, 224 faces, 224 heap entries).
+constexpr int kContV = 48, kContF = 224, kContH = 224;
+constexpr uint32_t kNoCont = 0xffffffffu;
+struct EpaCont {
+    uint32_t nv, nf, len, best_id, niter, pad[3];
+    float max_dist, old_dist, padf[2];
+    float vp[kContV][3];
+    float orig[kContV][6];
+    uint32_t face[kContF][2];  // pts | deleted << 24, adj
+    uint32_t heap_id[kContH];
+    float heap_nd[kContH];
+};
+// Where a tier writes the polytopes it hands on (cap 0: it hands on none).
+struct ContSink {
+    EpaCont *recs;
+    uint32_t *count;
+    uint32_t cap;
+};
+struct EpaScalars {
+    uint32_t nv, nf, best_id, niter;
+    float max_dist, old_dist;
+};
+
+// Writes the group's polytope to a continuation record; returns its index or
+// kNoCont (no room, or a polytope type the record cannot hold).
+template <int G, class W>
+__device__ uint32_t epa_export(W *w, const float *orig, const ContSink &sink, const EpaScalars &x, uint32_t len) {
+    if constexpr (sizeof(typename W::Idx) == 1 && W::kV <= kContV && W::kF <= kContF && W::kH <= kContH) {
+        if (sink.cap == 0) return kNoCont;
+        const uint32_t sub = __lane_id() % G;
+        if (sub == 0) w->xfer = atomicAdd(sink.count, 1u);
+        // orig rows were written to global memory by this wave; the group's
+        // lanes read each other's rows below
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t ci = w->xfer;
+        if (ci >= sink.cap) return kNoCont;
+        EpaCont *r = sink.recs + ci;
+        if (sub == 0) {
+            r->nv = x.nv;
+            r->nf = x.nf;
+            r->len = len;
+            r->best_id = x.best_id;
+            r->niter = x.niter;
+            r->max_dist = x.max_dist;
+            r->old_dist = x.old_dist;
+        }
+        for (uint32_t k = sub; k < x.nv; k += G) {
+            for (int q = 0; q < 3; ++q) r->vp[k][q] = w->vp[k][q];
+            for (int q = 0; q < 6; ++q) r->orig[k][q] = orig[kOrigFloats * k + q];
+        }
+        for (uint32_t k = sub; k < x.nf; k += G) {
+            const typename W::Face &F = w->face[k];
+            r->face[k][0] = (uint32_t)F.pts[0] | ((uint32_t)F.pts[1] << 8) | ((uint32_t)F.pts[2] << 16) |
+                            ((uint32_t)F.deleted << 24);
+            r->face[k][1] = (uint32_t)F.adj[0] | ((uint32_t)F.adj[1] << 8) | ((uint32_t)F.adj[2] << 16);
+        }
+        for (uint32_t k = sub; k < len; k += G) {
+            r->heap_id[k] = w->heap_id[k];
+            r->heap_nd[k] = w->heap_nd[k];
+        }
+        return ci;
+    } else {
+        return kNoCont;
+    }
+}
+
+// Loads a continuation record into the group's polytope; face normals are
+// recomputed from the vertices exactly as Face::new computed them.
+template <int G, class W>
+__device__ void epa_import(W *w, float *orig, const EpaCont *r, EpaScalars &x, uint32_t &len) {
+    if constexpr (W::kV >= kContV && W::kF >= kContF && W::kH >= kContH) {
+        const uint32_t sub = __lane_id() % G;
+        x.nv = r->nv;
+        x.nf = r->nf;
+        len = r->len;
+        x.best_id = r->best_id;
+        x.niter = r->niter;
+        x.max_dist = r->max_dist;
+        x.old_dist = r->old_dist;
+        for (uint32_t k = sub; k < x.nv; k += G) {
+            for (int q = 0; q < 3; ++q) w->vp[k][q] = r->vp[k][q];
+            for (int q = 0; q < 6; ++q) orig[kOrigFloats * k + q] = r->orig[k][q];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        using I = typename W::Idx;
+        for (uint32_t k = sub; k < x.nf; k += G) {
+            const uint32_t a = r->face[k][0], b = r->face[k][1];
+            typename W::Face &F = w->face[k];
+            F.pts[0] = (I)(a & 0xffu);
+            F.pts[1] = (I)((a >> 8) & 0xffu);
+            F.pts[2] = (I)((a >> 16) & 0xffu);
+            F.deleted = (uint8_t)(a >> 24);
+            F.adj[0] = (I)(b & 0xffu);
+            F.adj[1] = (I)((b >> 8) & 0xffu);
+            F.adj[2] = (I)((b >> 16) & 0xffu);
+            const f3 nn = ccw_normal(vpt(w, F.pts[0]), vpt(w, F.pts[1]), vpt(w, F.pts[2]));
+            F.n[0] = nn.x;
+            F.n[1] = nn.y;
+            F.n[2] = nn.z;
+        }
+        for (uint32_t k = sub; k < len; k += G) {
+            w->heap_id[k] = (I)r->heap_id[k];
+            w->heap_nd[k] = r->heap_nd[k];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // EPA::closest_points (epa3.rs:428-651) for one pair, executed by a group of G lanes.
+//
+// Every iteration is checkpointable: the heap's top is only peeked, the new
+// vertex goes to the first free slot without being counted, the best face and
+// distances are kept aside, and the faces the iteration deletes are listed.
+// Only when the polytope is known to have room for the iteration's new faces
+// and heap entries is the iteration committed -- the top popped (nothing reads
+// the heap between parry's pop and its pushes, so the heap ends up the same),
+// the vertex counted and the deletions kept.  A polytope that runs out of room
+// is therefore still the one the iteration started with: the deletions are
+// undone and, if `sink` has room, the polytope is written out for the next
+// tier to continue (`cont` gets the record's index).  `resume` starts from such
+// a record instead of the GJK simplex.
 template <int G, class W>
 __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef &g1, const ShapeRef &g2,
-                            const EpaItem *it, f3 &p1, f3 &p2, f3 &n, uint32_t &epa_iters) {
+                            const EpaItem *it, const EpaCont *resume, const ContSink &sink, f3 &p1, f3 &p2, f3 &n,
+                            uint32_t &epa_iters, uint32_t &cont) {
     const float eps_tol = kEps * 100.0f;
     Staged st{stage_shape<G>(w, 0, g1), stage_shape<G>(w, 1, g2)};
     // The group's lanes wrote the staged vertices; make them visible to the
@@ -613,93 +747,118 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
     uint32_t nv = 0, nf = 0;
     Heap<W> heap{w, 0};
     epa_iters = 0;
-    const uint32_t dim = it->dim;
-    for (uint32_t i = 0; i <= dim; ++i) {
-        w->vp[i][0] = it->v[i][0];
-        w->vp[i][1] = it->v[i][1];
-        w->vp[i][2] = it->v[i][2];
-        for (int q = 0; q < 6; ++q) orig[kOrigFloats * i + q] = it->v[i][3 + q];
-    }
-    nv = dim + 1;
-    if (dim == 0) {
-        p1 = p2 = f3{0.0f, 0.0f, 0.0f};
-        n = f3{0.0f, 1.0f, 0.0f};
-        return EPA_OK;
-    } else if (dim == 3) {
-        const f3 dp1 = sub3(vpt(w, 1), vpt(w, 0));
-        const f3 dp2 = sub3(vpt(w, 2), vpt(w, 0));
-        const f3 dp3 = sub3(vpt(w, 3), vpt(w, 0));
-        if (dot3(cross3(dp1, dp2), dp3) > 0.0f) {
-            for (int q = 0; q < 3; ++q) {
-                const float t = w->vp[1][q];
-                w->vp[1][q] = w->vp[2][q];
-                w->vp[2][q] = t;
-            }
-            for (int q = 0; q < 6; ++q) {
-                const float t = orig[kOrigFloats * 1 + q];
-                orig[kOrigFloats * 1 + q] = orig[kOrigFloats * 2 + q];
-                orig[kOrigFloats * 2 + q] = t;
-            }
-        }
-        const bool in0 = face_new(w, 0, 0, 1, 2, 3, 1, 2);
-        const bool in1 = face_new(w, 1, 1, 3, 2, 3, 2, 0);
-        const bool in2 = face_new(w, 2, 0, 2, 3, 0, 1, 3);
-        const bool in3 = face_new(w, 3, 0, 3, 1, 2, 1, 0);
-        nf = 4;
-        const bool ins[4] = {in0, in1, in2, in3};
-#pragma unroll
-        for (uint32_t f = 0; f < 4; ++f) {
-            if (ins[f]) {
-                const float dist = dot3(fnormal(w, f), vpt(w, f));
-                if (-dist > kEpsTol) return EPA_NONE;  // FaceId::new -> None
-                heap.push(f, -dist);
-            }
-        }
-        if (!(in0 || in1 || in2 || in3)) return EPA_NONE;
+    cont = kNoCont;
+    uint32_t best_id = 0, niter = 0;
+    float max_dist = kFltMax, old_dist = 0.0f;
+    if (resume) {
+        EpaScalars x;
+        epa_import<G>(w, orig, resume, x, heap.len);
+        nv = x.nv;
+        nf = x.nf;
+        best_id = x.best_id;
+        niter = x.niter;
+        max_dist = x.max_dist;
+        old_dist = x.old_dist;
     } else {
-        if (dim == 1) {
-            // orthonormal_subspace_basis([dpt]): first vector a x dpt
-            const f3 dpt = sub3(vpt(w, 1), vpt(w, 0));
-            f3 a = fabsf(dpt.x) > fabsf(dpt.y) ? f3{dpt.z, 0.0f, -dpt.x} : f3{0.0f, -dpt.z, dpt.y};
-            a = div3(a, sqrtf(norm_sq3(a)));
-            vset(w, orig, 2, group_cso<G>(pos12, g1, g2, st, cross3(a, dpt)));
-            nv = 3;
+        const uint32_t dim = it->dim;
+        for (uint32_t i = 0; i <= dim; ++i) {
+            w->vp[i][0] = it->v[i][0];
+            w->vp[i][1] = it->v[i][1];
+            w->vp[i][2] = it->v[i][2];
+            for (int q = 0; q < 6; ++q) orig[kOrigFloats * i + q] = it->v[i][3 + q];
         }
-        face_new(w, 0, 0, 1, 2, 1, 1, 1);
-        face_new(w, 1, 0, 2, 1, 0, 0, 0);
-        nf = 2;
-        heap.push(0, 0.0f);
-        heap.push(1, 0.0f);
+        nv = dim + 1;
+        if (dim == 0) {
+            p1 = p2 = f3{0.0f, 0.0f, 0.0f};
+            n = f3{0.0f, 1.0f, 0.0f};
+            return EPA_OK;
+        } else if (dim == 3) {
+            const f3 dp1 = sub3(vpt(w, 1), vpt(w, 0));
+            const f3 dp2 = sub3(vpt(w, 2), vpt(w, 0));
+            const f3 dp3 = sub3(vpt(w, 3), vpt(w, 0));
+            if (dot3(cross3(dp1, dp2), dp3) > 0.0f) {
+                for (int q = 0; q < 3; ++q) {
+                    const float t = w->vp[1][q];
+                    w->vp[1][q] = w->vp[2][q];
+                    w->vp[2][q] = t;
+                }
+                for (int q = 0; q < 6; ++q) {
+                    const float t = orig[kOrigFloats * 1 + q];
+                    orig[kOrigFloats * 1 + q] = orig[kOrigFloats * 2 + q];
+                    orig[kOrigFloats * 2 + q] = t;
+                }
+            }
+            const bool in0 = face_new(w, 0, 0, 1, 2, 3, 1, 2);
+            const bool in1 = face_new(w, 1, 1, 3, 2, 3, 2, 0);
+            const bool in2 = face_new(w, 2, 0, 2, 3, 0, 1, 3);
+            const bool in3 = face_new(w, 3, 0, 3, 1, 2, 1, 0);
+            nf = 4;
+            const bool ins[4] = {in0, in1, in2, in3};
+    #pragma unroll
+            for (uint32_t f = 0; f < 4; ++f) {
+                if (ins[f]) {
+                    const float dist = dot3(fnormal(w, f), vpt(w, f));
+                    if (-dist > kEpsTol) return EPA_NONE;  // FaceId::new -> None
+                    heap.push(f, -dist);
+                }
+            }
+            if (!(in0 || in1 || in2 || in3)) return EPA_NONE;
+        } else {
+            if (dim == 1) {
+                // orthonormal_subspace_basis([dpt]): first vector a x dpt
+                const f3 dpt = sub3(vpt(w, 1), vpt(w, 0));
+                f3 a = fabsf(dpt.x) > fabsf(dpt.y) ? f3{dpt.z, 0.0f, -dpt.x} : f3{0.0f, -dpt.z, dpt.y};
+                a = div3(a, sqrtf(norm_sq3(a)));
+                vset(w, orig, 2, group_cso<G>(pos12, g1, g2, st, cross3(a, dpt)));
+                nv = 3;
+            }
+            face_new(w, 0, 0, 1, 2, 1, 1, 1);
+            face_new(w, 1, 0, 2, 1, 0, 0, 0);
+            nf = 2;
+            heap.push(0, 0.0f);
+            heap.push(1, 0.0f);
+        }
+        if (heap.len == 0) return EPA_NONE;
+        best_id = w->heap_id[0];
     }
-    if (heap.len == 0) return EPA_NONE;
-    uint32_t best_id = w->heap_id[0];
-    float max_dist = kFltMax;
-    float old_dist = 0.0f;
-    uint32_t niter = 0;
-    uint32_t fid;
-    float fnd;
-    while (heap.pop(fid, fnd)) {
-        if (w->face[fid].deleted) continue;
+    // Hands the polytope as it was at the start of this iteration to the next tier.
+    auto overflow = [&](uint32_t ndel, uint32_t fid) -> uint32_t {
+        for (uint32_t k = 0; k < ndel; ++k) w->face[w->dl[k]].deleted = 0;
+        w->face[fid].deleted = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        cont = epa_export<G>(w, orig, sink, EpaScalars{nv, nf, best_id, niter, max_dist, old_dist}, heap.len);
+        return EPA_OVERFLOW;
+    };
+    while (heap.len > 0) {
+        const uint32_t fid = w->heap_id[0];
+        const float fnd = w->heap_nd[0];
+        if (w->face[fid].deleted) {
+            uint32_t pid;
+            float pnd;
+            heap.pop(pid, pnd);
+            continue;
+        }
         const typename W::Face face = w->face[fid];
         const f3 fn = f3{face.n[0], face.n[1], face.n[2]};
         const Cso c = group_cso<G>(pos12, g1, g2, st, fn);
-        if (nv >= (uint32_t)W::kV) return EPA_OVERFLOW;
-        const uint32_t sp = nv;
+        if (nv >= (uint32_t)W::kV) return overflow(0, fid);
+        const uint32_t sp = nv;  // counted when the iteration commits
         vset(w, orig, sp, c);
-        ++nv;
         const float cand = dot3(c.p, fn);
-        if (cand < max_dist) {
-            best_id = fid;
-            max_dist = cand;
+        uint32_t best_t = best_id;
+        float max_t = max_dist;
+        if (cand < max_t) {
+            best_t = fid;
+            max_t = cand;
         }
         const float curr_dist = -fnd;
-        if (max_dist - curr_dist < eps_tol || (fabsf(curr_dist - old_dist) < kEps && cand < max_dist)) {
-            face_points(w, orig, best_id, p1, p2);
-            n = fnormal(w, best_id);
+        if (max_t - curr_dist < eps_tol || (fabsf(curr_dist - old_dist) < kEps && cand < max_t)) {
+            face_points(w, orig, best_t, p1, p2);
+            n = fnormal(w, best_t);
             epa_iters = niter + 1;
             return EPA_OK;
         }
-        old_dist = curr_dist;
         w->face[fid].deleted = 1;
         // Face::can_be_seen_by (epa3.rs:153-166) of every live face from the new
         // vertex, for each of the three opposite vertices, by the group's lanes in
@@ -735,8 +894,8 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
         // whose top entry lives in a register.  After a visible face, the face
         // visited next (its first far neighbour) is loaded together with the
         // second one's points and carried in registers, so a step costs one
-        // round trip to LDS.
-        uint32_t ns = 0, sk = 0, depth = 0, top = 0;
+        // round trip to LDS.  Deleted faces are listed in dl for an undo.
+        uint32_t ns = 0, sk = 0, depth = 0, top = 0, ndel = 0;
         typename W::Face cf = w->face[face.adj[0]];
         uint32_t cvis = w->vis[face.adj[0]];
         uint32_t cur = face.adj[0] | (ccw_index(cf, face.pts[0]) << 16);
@@ -753,9 +912,11 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
             bool from_stack = true;
             if (!cf.deleted) {
                 if (!((cvis >> opp) & 1u)) {
-                    if (ns >= (uint32_t)W::kS) return EPA_OVERFLOW;
+                    if (ns >= (uint32_t)W::kS) return overflow(ndel, fid);
                     w->sil[ns++] = cur;
                 } else {
+                    if (ndel >= (uint32_t)W::kS) return overflow(ndel, fid);
+                    w->dl[ndel++] = (typename W::Idx)id;
                     w->face[id].deleted = 1;
                     // a1 = (opp + 2) % 3 and a2 = opp, selected by value so the
                     // record stays in registers
@@ -768,7 +929,7 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
                     const uint32_t o2 = next_ccw(w, adj2, pt2);
                     const uint32_t o1 = ccw_index(f1, pt1);
                     if (depth > 0) {
-                        if (sk >= (uint32_t)W::kK) return EPA_OVERFLOW;
+                        if (sk >= (uint32_t)W::kK) return overflow(ndel, fid);
                         w->stk[sk++] = top;
                     }
                     top = adj2 | (o2 << 16);
@@ -790,6 +951,30 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
         }
         const uint32_t first_new = nf;
         if (ns == 0) return EPA_NONE;
+        const uint32_t sub = __lane_id() % G;
+        const uint32_t gbase = __lane_id() - sub;
+        const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+        const uint64_t below = (1ull << sub) - 1ull;
+        // Room for the new faces and their heap entries (at most one per live
+        // silhouette edge), checked before anything is committed.
+        uint32_t n_live = 0;
+        for (uint32_t c0 = 0; c0 < ns; c0 += G) {
+            const uint32_t q = c0 + sub;
+            const bool alive = q < ns && !w->face[w->sil[q] & 0xffffu].deleted;
+            n_live += (uint32_t)__popcll((__ballot(alive) >> gbase) & gmask);
+        }
+        if (first_new + n_live > (uint32_t)W::kF || heap.len - 1 + n_live > (uint32_t)W::kH)
+            return overflow(ndel, fid);
+        // Commit: parry popped this face before all of the above.
+        {
+            uint32_t pid;
+            float pnd;
+            heap.pop(pid, pnd);
+        }
+        ++nv;
+        best_id = best_t;
+        max_dist = max_t;
+        old_dist = curr_dist;
         // The new faces (epa3.rs:598-626), G silhouette edges at a time, one per
         // lane.  The edges whose face is still live get consecutive ids in
         // silhouette order (a ballot gives each its rank), and each lane builds
@@ -799,59 +984,50 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
         // be discarded anyway -- or pushes every new face in silhouette order;
         // the lane holding an edge's face does that edge's push.
         uint32_t n_alive = 0;
-        {
-            const uint32_t sub = __lane_id() % G;
-            const uint32_t gbase = __lane_id() - sub;
-            const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-            const uint64_t below = (1ull << sub) - 1ull;
-            for (uint32_t c = 0; c < ns; c += G) {
-                const uint32_t q = c + sub;
-                uint32_t e = 0;
-                bool alive = false;
-                if (q < ns) {
-                    e = w->sil[q];
-                    alive = !w->face[e & 0xffffu].deleted;
-                }
-                const uint64_t am = (__ballot(alive) >> gbase) & gmask;
-                if (first_new + n_alive + (uint32_t)__popcll(am) > (uint32_t)W::kF) return EPA_OVERFLOW;
-                const uint32_t new_id = first_new + n_alive + (uint32_t)__popcll(am & below);
-                bool in = false;
-                float dist = 0.0f;
-                if (alive) {
-                    const uint32_t ef = e & 0xffffu, opp = e >> 16;
-                    const uint32_t pt1 = w->face[ef].pts[(opp + 2) % 3];
-                    const uint32_t pt2 = w->face[ef].pts[(opp + 1) % 3];
-                    in = face_new(w, new_id, pt1, pt2, sp, ef, new_id + 1, new_id - 1);
-                    w->face[ef].adj[(opp + 1) % 3] = (typename W::Idx)new_id;
-                    if (in) dist = dot3(fnormal(w, new_id), vpt(w, pt1));
-                }
-                n_alive += (uint32_t)__popcll(am);
-                const bool pushed = alive && in;
-                const bool closer = pushed && dist < curr_dist;
-                const bool stop = closer || (pushed && -dist > kEpsTol);
-                const uint64_t stop_m = (__ballot(stop) >> gbase) & gmask;
-                const uint64_t closer_m = (__ballot(closer) >> gbase) & gmask;
-                if (stop_m != 0) {
-                    if ((closer_m >> __builtin_ctzll(stop_m)) & 1ull) {
-                        face_points(w, orig, fid, p1, p2);  // the popped face: its points are unchanged
-                        n = fn;
-                        epa_iters = niter + 1;
-                        return EPA_OK;
-                    }
-                    return EPA_NONE;  // FaceId::new -> None
-                }
-                const uint64_t push_m = (__ballot(pushed) >> gbase) & gmask;
-                const uint32_t n_push = (uint32_t)__popcll(push_m);
-                if (heap.len + n_push > (uint32_t)W::kH) return EPA_OVERFLOW;
-                for (uint64_t m = push_m; m != 0; m &= m - 1) {
-                    if (sub == (uint32_t)__builtin_ctzll(m))
-                        heap.insert_at(heap.len + (uint32_t)__popcll(push_m & below), new_id, -dist);
-                }
-                heap.len += n_push;
+        for (uint32_t c0 = 0; c0 < ns; c0 += G) {
+            const uint32_t q = c0 + sub;
+            uint32_t e = 0;
+            bool alive = false;
+            if (q < ns) {
+                e = w->sil[q];
+                alive = !w->face[e & 0xffffu].deleted;
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
+            const uint64_t am = (__ballot(alive) >> gbase) & gmask;
+            const uint32_t new_id = first_new + n_alive + (uint32_t)__popcll(am & below);
+            bool in = false;
+            float dist = 0.0f;
+            if (alive) {
+                const uint32_t ef = e & 0xffffu, opp = e >> 16;
+                const uint32_t pt1 = w->face[ef].pts[(opp + 2) % 3];
+                const uint32_t pt2 = w->face[ef].pts[(opp + 1) % 3];
+                in = face_new(w, new_id, pt1, pt2, sp, ef, new_id + 1, new_id - 1);
+                w->face[ef].adj[(opp + 1) % 3] = (typename W::Idx)new_id;
+                if (in) dist = dot3(fnormal(w, new_id), vpt(w, pt1));
+            }
+            n_alive += (uint32_t)__popcll(am);
+            const bool pushed = alive && in;
+            const bool closer = pushed && dist < curr_dist;
+            const bool stop = closer || (pushed && -dist > kEpsTol);
+            const uint64_t stop_m = (__ballot(stop) >> gbase) & gmask;
+            const uint64_t closer_m = (__ballot(closer) >> gbase) & gmask;
+            if (stop_m != 0) {
+                if ((closer_m >> __builtin_ctzll(stop_m)) & 1ull) {
+                    face_points(w, orig, fid, p1, p2);  // the popped face: its points are unchanged
+                    n = fn;
+                    epa_iters = niter + 1;
+                    return EPA_OK;
+                }
+                return EPA_NONE;  // FaceId::new -> None
+            }
+            const uint64_t push_m = (__ballot(pushed) >> gbase) & gmask;
+            for (uint64_t m = push_m; m != 0; m &= m - 1) {
+                if (sub == (uint32_t)__builtin_ctzll(m))
+                    heap.insert_at(heap.len + (uint32_t)__popcll(push_m & below), new_id, -dist);
+            }
+            heap.len += (uint32_t)__popcll(push_m);
         }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         nf = first_new + n_alive;
         if (first_new == nf) return EPA_NONE;
         w->face[first_new].adj[2] = (typename W::Idx)(nf - 1);
@@ -867,16 +1043,28 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
 
 template <int G, class W>
 __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4 *verts, const float *poses,
-                         const uint32_t *pairs, float prediction, const EpaItem *it, DevContact *out,
-                         uint32_t *iters, uint32_t *overflow_list, uint32_t *overflow_count, uint32_t slot) {
+                         const uint32_t *pairs, float prediction, const EpaItem *it, const EpaCont *resume,
+                         const ContSink &sink, DevContact *out, uint32_t *iters, uint32_t *overflow_list,
+                         uint32_t *overflow_cont, uint32_t *overflow_count, uint32_t slot) {
     const uint32_t k = it->pair;
     const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
     const iso pos1 = load_iso(poses + 7 * (size_t)i), pos2 = load_iso(poses + 7 * (size_t)j);
     const iso pos12 = iso_inv_mul(pos1, pos2);
     const ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
     f3 p1, p2, n;
-    uint32_t ei = 0;
+    uint32_t ei = 0, cont = kNoCont;
     const uint32_t mode = it->mode;
+    // A pair that outgrows this tier's polytope goes to the next tier's list,
+    // with its continuation record if it has one.
+    auto hand_on = [&]() {
+        if (overflow_list) {
+            const uint32_t pos = atomicAdd(overflow_count, 1u);
+            overflow_list[pos] = slot;
+            if (overflow_cont) overflow_cont[pos] = cont;
+            return;
+        }
+        out[k] = no_contact(kUnsupportedPair);  // cannot happen with the large buffers
+    };
     if (mode != kPairSupportMaps) {
         // EPA::project_origin of the ball centre on the ConvexPolyhedron
         // (point_support_map.rs:39-49), then the Ball-vs-convex contact.
@@ -888,14 +1076,11 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
             cv = g2;
             radius = g1.p0;
         }
-        const uint32_t st = epa_run<G>(w, orig, point_proj_iso(pb.t), cv, origin_shape(), it, p1, p2, n, ei);
+        const uint32_t st =
+            epa_run<G>(w, orig, point_proj_iso(pb.t), cv, origin_shape(), it, resume, sink, p1, p2, n, ei, cont);
         if (__lane_id() % G != 0) return;
         if (st == EPA_OVERFLOW) {
-            if (overflow_list) {
-                overflow_list[atomicAdd(overflow_count, 1u)] = slot;
-                return;
-            }
-            out[k] = no_contact(kUnsupportedPair);  // cannot happen with the large buffers
+            hand_on();
             return;
         }
         const PointProj proj{true, st == EPA_OK ? p1 : pb.t};
@@ -903,14 +1088,10 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
         if (iters) iters[2 * k + 1] = ei;
         return;
     }
-    const uint32_t st = epa_run<G>(w, orig, pos12, g1, g2, it, p1, p2, n, ei);
+    const uint32_t st = epa_run<G>(w, orig, pos12, g1, g2, it, resume, sink, p1, p2, n, ei, cont);
     if (__lane_id() % G != 0) return;
     if (st == EPA_OVERFLOW) {
-        if (overflow_list) {
-            overflow_list[atomicAdd(overflow_count, 1u)] = slot;
-            return;
-        }
-        out[k] = no_contact(kUnsupportedPair);  // cannot happen with the large buffers
+        hand_on();
         return;
     }
     if (st == EPA_OK) {
@@ -925,14 +1106,18 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
 // tier) and sends the pairs whose polytope outgrows its buffers to the next
 // tier's list: G lanes per pair with a small LDS polytope, then one wave per
 // pair with medium and big LDS polytopes, then a global-memory polytope.
+// At most 168 VGPRs (three waves per SIMD): the first tier's small polytopes
+// leave room in LDS for that many waves, and its latency-bound code needs them.
 template <int G, class W>
-__global__ __launch_bounds__(64) void epa_tier_kernel(const ShapeRecord *__restrict__ rec,
+__global__ __launch_bounds__(64, 3) void epa_tier_kernel(const ShapeRecord *__restrict__ rec,
                                                       const float4 *__restrict__ verts, const float *__restrict__ poses,
                                                       const uint32_t *__restrict__ pairs, float prediction,
                                                       const EpaItem *__restrict__ queue, const uint32_t *in_list,
                                                       const uint32_t *in_count, uint32_t *next,
-                                                      DevContact *__restrict__ out, uint32_t *__restrict__ iters,
-                                                      uint32_t *out_list, uint32_t *out_count, float *orig_scratch) {
+                                                      const uint32_t *in_cont, const EpaCont *cont_recs,
+                                                      ContSink sink, DevContact *__restrict__ out,
+                                                      uint32_t *__restrict__ iters, uint32_t *out_list,
+                                                      uint32_t *out_cont, uint32_t *out_count, float *orig_scratch) {
     constexpr int kGroups = 64 / G;
     __shared__ W w[kGroups];
     const uint32_t group = threadIdx.x / G;
@@ -951,8 +1136,10 @@ __global__ __launch_bounds__(64) void epa_tier_kernel(const ShapeRecord *__restr
         const uint32_t q = base + group;
         if (q < count) {
             const uint32_t slot = in_list ? in_list[q] : q;
-            epa_item<G>(&w[group], orig, rec, verts, poses, pairs, prediction, queue + slot, out, iters, out_list,
-                        out_count, slot);
+            const uint32_t ci = in_cont ? in_cont[q] : kNoCont;
+            epa_item<G>(&w[group], orig, rec, verts, poses, pairs, prediction, queue + slot,
+                        ci != kNoCont ? cont_recs + ci : nullptr, sink, out, iters, out_list, out_cont, out_count,
+                        slot);
         }
     }
 }
@@ -974,16 +1161,17 @@ __global__ __launch_bounds__(64) void epa_global_kernel(const ShapeRecord *__res
         if (__lane_id() == 0) q = atomicAdd(next, 1u);
         q = __builtin_amdgcn_readfirstlane(q);
         if (q >= count) break;
-        epa_item<64>(w, orig, rec, verts, poses, pairs, prediction, queue + in_list[q], out, iters, nullptr, nullptr,
-                     q);
+        epa_item<64>(w, orig, rec, verts, poses, pairs, prediction, queue + in_list[q], nullptr, ContSink{nullptr, nullptr, 0},
+                     out, iters, nullptr, nullptr, nullptr, q);
     }
 }
 
 template <int G, class W>
 hipError_t launch_epa_tier(const DeviceShapes &s, const float *poses, const uint32_t *pairs, float prediction,
                            uint64_t max_items, const EpaItem *queue, const uint32_t *in_list, const uint32_t *in_count,
-                           uint32_t *next, DevContact *out, uint32_t *iters, uint32_t *out_list, uint32_t *out_count,
-                           float *orig, uint64_t max_groups, hipStream_t stream) {
+                           uint32_t *next, const uint32_t *in_cont, const EpaCont *cont_recs, ContSink sink,
+                           DevContact *out, uint32_t *iters, uint32_t *out_list, uint32_t *out_cont,
+                           uint32_t *out_count, float *orig, uint64_t max_groups, hipStream_t stream) {
     hipError_t e;
     int dev = 0, cus = 0, per_cu = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
@@ -998,7 +1186,8 @@ hipError_t launch_epa_tier(const DeviceShapes &s, const float *poses, const uint
     if (waves * kGroups > max_groups) waves = max_groups / kGroups;
     if (waves < 1) waves = 1;
     hipLaunchKernelGGL((epa_tier_kernel<G, W>), dim3((uint32_t)waves), dim3(64), 0, stream, s.rec, s.verts, poses,
-                       pairs, prediction, queue, in_list, in_count, next, out, iters, out_list, out_count, orig);
+                       pairs, prediction, queue, in_list, in_count, next, in_cont, cont_recs, sink, out, iters,
+                       out_list, out_cont, out_count, orig);
     return hipGetLastError();
 }
 
@@ -1023,10 +1212,23 @@ struct ContactWorkspace {
     uint32_t *counters;  // list lengths and the tiers' next entries (reset_counters_kernel)
     EpaItem *queue;
     uint32_t *lists[3];
+    uint32_t *cont_list;  // per entry of lists[0]: its continuation record or kNoCont
+    EpaCont *conts;
+    uint32_t cont_cap;
     EpaLarge *large;
     float *orig_large, *orig_small, *orig_medium, *orig_big;
     size_t bytes;
 };
+
+// Continuation records: one per 48 pairs (overflowing pairs are about 1% of
+// mixed-shape pairs), at least 256 and at most 32768.  A pair that finds them
+// all taken restarts in the next tier instead.
+uint64_t cont_capacity(uint64_t m) {
+    uint64_t c = m / 48;
+    if (c < 256) c = 256;
+    if (c > 32768) c = 32768;
+    return c;
+}
 
 ContactWorkspace carve_workspace(void *base, uint64_t m) {
     const size_t align = 256;
@@ -1042,6 +1244,9 @@ ContactWorkspace carve_workspace(void *base, uint64_t m) {
     w.counters = (uint32_t *)take(kNumCounters * sizeof(uint32_t));
     w.queue = (EpaItem *)take(m * sizeof(EpaItem));
     for (int k = 0; k < 3; ++k) w.lists[k] = (uint32_t *)take(m * sizeof(uint32_t));
+    w.cont_list = (uint32_t *)take(m * sizeof(uint32_t));
+    w.cont_cap = (uint32_t)cont_capacity(m);
+    w.conts = (EpaCont *)take((size_t)w.cont_cap * sizeof(EpaCont));
     w.large = (EpaLarge *)take(kEpaGlobalWaves * sizeof(EpaLarge));
     w.orig_large = (float *)take(kEpaGlobalWaves * EpaLarge::kV * kOrigFloats * sizeof(float));
     w.orig_small = (float *)take(kEpaMaxGroups * kSmallMaxVertices * kOrigFloats * sizeof(float));
@@ -1055,26 +1260,24 @@ ContactWorkspace carve_workspace(void *base, uint64_t m) {
 size_t contact_workspace_bytes(uint64_t num_pairs) { return carve_workspace(nullptr, num_pairs).bytes; }
 
 namespace {
-// First EPA tier: G lanes per pair with the selected small polytope.
+// First EPA tier: G lanes per pair with the selected small polytope; it hands
+// overflowing pairs on with continuation records.
+template <int G, class W>
+hipError_t launch_first(const DeviceShapes &s, const float *poses, const uint32_t *pairs, float prediction, uint64_t m,
+                        const ContactWorkspace &w, uint32_t *c, DevContact *o, uint32_t *iters, hipStream_t stream) {
+    return launch_epa_tier<G, W>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, nullptr, nullptr,
+                                 ContSink{w.conts, c + 8, w.cont_cap}, o, iters, w.lists[0], w.cont_list, c + 1,
+                                 w.orig_small, kEpaMaxGroups, stream);
+}
 template <int G>
 hipError_t launch_small(const DeviceShapes &s, const float *poses, const uint32_t *pairs, float prediction, uint64_t m,
                         const ContactWorkspace &w, uint32_t *c, DevContact *o, uint32_t *iters, hipStream_t stream) {
     switch (g_epa_polytope) {
-        case 1:
-            return launch_epa_tier<G, EpaSmallUnstaged>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, o, iters,
-                                                        w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
-        case 2:
-            return launch_epa_tier<G, EpaTiny>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, o, iters,
-                                               w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
-        case 3:
-            return launch_epa_tier<G, EpaSmallWide>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, o, iters,
-                                                    w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
-        case 4:
-            return launch_epa_tier<G, EpaSmallLarge>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, o, iters,
-                                                     w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
-        default:
-            return launch_epa_tier<G, EpaSmall>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, o, iters,
-                                                w.lists[0], c + 1, w.orig_small, kEpaMaxGroups, stream);
+        case 1: return launch_first<G, EpaSmallUnstaged>(s, poses, pairs, prediction, m, w, c, o, iters, stream);
+        case 2: return launch_first<G, EpaTiny>(s, poses, pairs, prediction, m, w, c, o, iters, stream);
+        case 3: return launch_first<G, EpaSmallWide>(s, poses, pairs, prediction, m, w, c, o, iters, stream);
+        case 4: return launch_first<G, EpaSmallLarge>(s, poses, pairs, prediction, m, w, c, o, iters, stream);
+        default: return launch_first<G, EpaSmall>(s, poses, pairs, prediction, m, w, c, o, iters, stream);
     }
 }
 }  // namespace
@@ -1100,11 +1303,13 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
         default: e = launch_small<64>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
     }
     if (e != hipSuccess) return e;
-    e = launch_epa_tier<64, EpaMedium>(s, poses, pairs, prediction, m, w.queue, w.lists[0], c + 1, c + 5, o, iters,
-                                       w.lists[1], c + 2, w.orig_medium, kEpaMediumWaves, stream);
+    const ContSink none{nullptr, nullptr, 0};
+    e = launch_epa_tier<64, EpaMedium>(s, poses, pairs, prediction, m, w.queue, w.lists[0], c + 1, c + 5, w.cont_list,
+                                       w.conts, none, o, iters, w.lists[1], nullptr, c + 2, w.orig_medium,
+                                       kEpaMediumWaves, stream);
     if (e != hipSuccess) return e;
-    e = launch_epa_tier<64, EpaBig>(s, poses, pairs, prediction, m, w.queue, w.lists[1], c + 2, c + 6, o, iters,
-                                    w.lists[2], c + 3, w.orig_big, kEpaBigWaves, stream);
+    e = launch_epa_tier<64, EpaBig>(s, poses, pairs, prediction, m, w.queue, w.lists[1], c + 2, c + 6, nullptr, nullptr,
+                                    none, o, iters, w.lists[2], nullptr, c + 3, w.orig_big, kEpaBigWaves, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(epa_global_kernel, dim3(kEpaGlobalWaves), dim3(64), 0, stream, s.rec, s.verts, poses, pairs,
                        prediction, w.queue, o, iters, w.lists[2], c + 3, c + 7, w.large, w.orig_large);
