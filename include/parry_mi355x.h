@@ -153,6 +153,35 @@ int pq_trimesh_cast_local_rays_counted(const pq_trimesh *mesh, const pq_ray *ray
 int pq_trimesh_cast_local_rays_host(const pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays, float max_toi,
                                     int solid, pq_ray_hit *hits);
 
+/* Batched TriMesh::cast_local_ray_with_culling (src/query/ray/ray_trimesh.rs:152-178):
+ * the same traversal, skipping the triangles RayCullingMode::check rejects
+ * (ray_trimesh.rs:58-65), with the same feature remap.  Device pointers;
+ * asynchronous on `stream`. */
+#define PQ_CULL_BACKFACES 1u  /* RayCullingMode::IgnoreBackfaces */
+#define PQ_CULL_FRONTFACES 2u /* RayCullingMode::IgnoreFrontfaces */
+int pq_trimesh_cast_local_rays_with_culling(const pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays,
+                                            float max_toi, uint32_t culling, pq_ray_hit *hits, void *stream);
+
+/* PointProjection (src/query/point/point_query.rs:65-77) plus the index of
+ * the triangle projected on: FeatureId::Face(triangle) of
+ * TriMesh::project_local_point_and_get_feature (point_composite_shape.rs:171-186,
+ * a mesh without pseudo-normals). */
+typedef struct pq_point_projection {
+    float point[3];
+    uint32_t is_inside;
+    uint32_t triangle;
+} pq_point_projection;
+
+/* Batched PointQuery::project_local_point on a TriMesh
+ * (point_composite_shape.rs:164-168 -> CompositeShapeRef::project_local_point,
+ * :59-80 -> Bvh::find_best with the AABB distance as node cost, bvh_queries.rs:213-231).
+ * points: device, 3 floats per point.  solid as in parry (it changes the
+ * result only for points whose projection falls on a degenerate triangle's
+ * face region).  With solid = 0 the triangle is FeatureId::Face of
+ * project_local_point_and_get_feature.  Device pointers; asynchronous. */
+int pq_trimesh_project_local_points(const pq_trimesh *mesh, const float *points, uint64_t num_points, int solid,
+                                    pq_point_projection *out, void *stream);
+
 /* ---- Batched ray casts over many TriMeshes (no counterpart in parry: the
  *      batching of many RayCast calls) --------------------------------------- */
 

@@ -35,6 +35,10 @@ _SIG = {
     "orc_cast_rays_brute_f32": (None, [_vp, _vp, _u64, _f32, _vp, _vp, _vp, _int]),
     "orc_aabb_cast_ray_f32": (_int, [_vp, _vp, _vp, _f32, _int, _vp]),
     "orc_triangle_cast_ray_f32": (_int, [_vp, _vp, _f32, _vp, _vp]),
+    "orc_cast_rays_culling_f32": (None, [_vp, _vp, _u64, _f32, _int, _vp, _vp, _vp, _vp, _int, _int]),
+    "orc_project_points_f32": (None, [_vp, _vp, _u64, _int, _vp, _vp, _vp, _vp, _int, _int]),
+    "orc_project_points_f64": (None, [_vp, _vp, _u64, _int, _vp, _vp, _vp, _vp, _int, _int]),
+    "orc_triangle_project_point_f32": (_int, [_vp, _vp, _int, _vp]),
 }
 
 _lib = None
@@ -130,6 +134,49 @@ class OracleTriMesh:
                 out["w"], out["k"] = w, k
         out["W"], out["K"], out["max_stack"] = int(cnt[0]), int(cnt[1]), int(cnt[2])
         return out
+
+    def cast_rays_with_culling(self, rays: np.ndarray, max_toi: float, culling: int, threads: int | None = None,
+                               brute: bool = False):
+        """TriMesh::cast_local_ray_with_culling (ray_trimesh.rs:152-178), f32: culling 1 =
+        IgnoreBackfaces, 2 = IgnoreFrontfaces.  Returns dict(feature, toi, normal, W, K): W and K are
+        the Bvh wide records fetched and the triangles tested, summed over the rays."""
+        assert not self.f64 and culling in (1, 2)
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = r.shape[0]
+        fid = np.zeros(n, np.uint32)
+        toi = np.zeros(n, np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        cnt = np.zeros(3, np.uint64)
+        th = default_threads() if threads is None else threads
+        lib().orc_cast_rays_culling_f32(self.h, r.ctypes.data, n, max_toi, int(culling), fid.ctypes.data,
+                                        toi.ctypes.data, nrm.ctypes.data, cnt.ctypes.data, th, int(brute))
+        return {"feature": fid, "toi": toi, "normal": nrm, "W": int(cnt[0]), "K": int(cnt[1])}
+
+    def project_points(self, points: np.ndarray, solid: bool, threads: int | None = None, brute: bool = False):
+        """TriMesh::project_local_point (point_composite_shape.rs:59-80, 164-168).
+        Returns dict(point [n,3], is_inside u32 [n], triangle u32 [n], W, K): W and K are the Bvh wide
+        records fetched and the triangles tested, summed over the points (0 with brute)."""
+        dt = np.float64 if self.f64 else np.float32
+        p = np.ascontiguousarray(points, dtype=dt).reshape(-1, 3)
+        n = p.shape[0]
+        proj = np.zeros((n, 3), dt)
+        inside = np.zeros(n, np.uint32)
+        tri = np.zeros(n, np.uint32)
+        cnt = np.zeros(2, np.uint64)
+        th = default_threads() if threads is None else threads
+        f = lib().orc_project_points_f64 if self.f64 else lib().orc_project_points_f32
+        f(self.h, p.ctypes.data, n, int(solid), proj.ctypes.data, inside.ctypes.data, tri.ctypes.data,
+          cnt.ctypes.data, th, int(brute))
+        return {"point": proj, "is_inside": inside, "triangle": tri, "W": int(cnt[0]), "K": int(cnt[1])}
+
+
+def triangle_project_point(abc, pt, solid: bool):
+    """Triangle::project_local_point (point_triangle.rs:54-291), f32: (point, is_inside)."""
+    t = np.ascontiguousarray(abc, np.float32).reshape(9)
+    q = np.ascontiguousarray(pt, np.float32).reshape(3)
+    out = np.zeros(3, np.float32)
+    ins = lib().orc_triangle_project_point_f32(t.ctypes.data, q.ctypes.data, int(solid), out.ctypes.data)
+    return out, bool(ins)
 
 
 def aabb_cast_ray(mins, maxs, ray, max_toi: float, solid: bool):

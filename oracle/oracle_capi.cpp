@@ -14,6 +14,7 @@
 
 #include "parry_oracle.hpp"
 #include "parry_oracle_contact.hpp"
+#include "parry_oracle_point.hpp"
 
 using namespace oracle;
 
@@ -50,7 +51,7 @@ template <class T> TriMesh<T> *make_mesh(const T *verts, uint32_t nv, const uint
 template <class T>
 void cast_rays(const TriMesh<T> *m, const T *rays, uint64_t n, T max_toi, int solid, uint32_t *fid, T *toi,
                T *normal, uint64_t *counters, uint32_t *per_ray_w, uint32_t *per_ray_k, int nthreads,
-               int brute) {
+               int brute, int culling = 0) {
     std::atomic<uint64_t> w_sum{0}, k_sum{0};
     std::atomic<uint32_t> max_stack{0};
     parallel_for(n, nthreads, 1024, [&](uint64_t b, uint64_t e) {
@@ -59,8 +60,8 @@ void cast_rays(const TriMesh<T> *m, const T *rays, uint64_t n, T max_toi, int so
             Ray<T> r{v3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), v3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5])};
             RayHit<T> h{};
             uint64_t w0 = c.wide_loads, k0 = c.leaf_tests;
-            bool hit = brute ? trimesh_cast_ray_brute(*m, r, max_toi, &h)
-                             : trimesh_cast_ray(*m, r, max_toi, solid != 0, &h, &c);
+            bool hit = brute ? trimesh_cast_ray_brute(*m, r, max_toi, &h, culling)
+                             : trimesh_cast_ray(*m, r, max_toi, solid != 0, &h, &c, culling);
             if (per_ray_w) per_ray_w[i] = (uint32_t)(c.wide_loads - w0);
             if (per_ray_k) per_ray_k[i] = (uint32_t)(c.leaf_tests - k0);
             if (hit) {
@@ -85,6 +86,32 @@ void cast_rays(const TriMesh<T> *m, const T *rays, uint64_t n, T max_toi, int so
         counters[0] = w_sum.load();
         counters[1] = k_sum.load();
         counters[2] = max_stack.load();
+    }
+}
+
+template <class T>
+void project_points(const TriMesh<T> *m, const T *pts, uint64_t n, int solid, T *proj, uint32_t *inside,
+                    uint32_t *tri, int nthreads, int brute, uint64_t *counters) {
+    std::atomic<uint64_t> w_sum{0}, k_sum{0};
+    parallel_for(n, nthreads, 1024, [&](uint64_t b, uint64_t e) {
+        RayCounters c;
+        for (uint64_t i = b; i < e; ++i) {
+            V3<T> p = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+            PointHit<T> h{};
+            bool ok = brute ? trimesh_project_point_brute(*m, p, solid != 0, &h)
+                            : trimesh_project_point(*m, p, solid != 0, &h, &c);
+            proj[3 * i] = ok ? h.point.x : T(0);
+            proj[3 * i + 1] = ok ? h.point.y : T(0);
+            proj[3 * i + 2] = ok ? h.point.z : T(0);
+            inside[i] = ok && h.inside ? 1u : 0u;
+            tri[i] = ok ? h.tri : UINT32_MAX;
+        }
+        w_sum += c.wide_loads;
+        k_sum += c.leaf_tests;
+    });
+    if (counters) {
+        counters[0] = w_sum.load();
+        counters[1] = k_sum.load();
     }
 }
 
@@ -148,6 +175,28 @@ void orc_cast_rays_f64(void *m, const double *rays, uint64_t n, double max_toi, 
                        double *toi, double *normal, uint64_t *counters, int nthreads) {
     cast_rays<double>((TriMesh<double> *)m, rays, n, max_toi, solid, fid, toi, normal, counters, nullptr, nullptr,
                       nthreads, 0);
+}
+void orc_cast_rays_culling_f32(void *m, const float *rays, uint64_t n, float max_toi, int culling, uint32_t *fid,
+                               float *toi, float *normal, uint64_t *counters, int nthreads, int brute) {
+    cast_rays<float>((TriMesh<float> *)m, rays, n, max_toi, 0, fid, toi, normal, counters, nullptr, nullptr, nthreads,
+                     brute, culling);
+}
+void orc_project_points_f32(void *m, const float *pts, uint64_t n, int solid, float *proj, uint32_t *inside,
+                            uint32_t *tri, uint64_t *counters, int nthreads, int brute) {
+    project_points<float>((TriMesh<float> *)m, pts, n, solid, proj, inside, tri, nthreads, brute, counters);
+}
+void orc_project_points_f64(void *m, const double *pts, uint64_t n, int solid, double *proj, uint32_t *inside,
+                            uint32_t *tri, uint64_t *counters, int nthreads, int brute) {
+    project_points<double>((TriMesh<double> *)m, pts, n, solid, proj, inside, tri, nthreads, brute, counters);
+}
+int orc_triangle_project_point_f32(const float *abc, const float *pt, int solid, float *out) {
+    bool ins = false;
+    V3<float> p = triangle_project_point(v3(abc[0], abc[1], abc[2]), v3(abc[3], abc[4], abc[5]),
+                                         v3(abc[6], abc[7], abc[8]), v3(pt[0], pt[1], pt[2]), solid != 0, &ins);
+    out[0] = p.x;
+    out[1] = p.y;
+    out[2] = p.z;
+    return ins ? 1 : 0;
 }
 void orc_cast_rays_brute_f32(void *m, const float *rays, uint64_t n, float max_toi, uint32_t *fid, float *toi,
                              float *normal, int nthreads) {

@@ -493,12 +493,24 @@ struct RayCounters {
     uint32_t max_stack = 0;
 };
 
+// RayCullingMode::check (ray_trimesh.rs:58-65) on the triangle's scaled
+// normal (ab x ac, triangle.rs:324-328): culling 1 = IgnoreBackfaces keeps
+// n.dir < 0, 2 = IgnoreFrontfaces keeps n.dir > 0, 0 keeps every triangle.
+template <class T> inline bool culling_keeps(int culling, V3<T> a, V3<T> b, V3<T> c, V3<T> dir) {
+    if (culling == 0) return true;
+    T d = dot(cross(b - a, c - a), dir);
+    return culling == 1 ? d < T(0) : d > T(0);
+}
+
 // TriMesh::cast_local_ray_and_get_normal (ray_trimesh.rs:16-35) through
 // CompositeShapeRef (ray_composite_shape.rs:43-62) and Bvh::find_best
-// (bvh_traverse.rs:335-417).  Returns false for None.
+// (bvh_traverse.rs:335-417).  Returns false for None.  With culling != 0 it
+// is TriMesh::cast_local_ray_with_culling (ray_trimesh.rs:152-178): the same
+// traversal over TriMeshWithCulling, whose parts that fail the culling test
+// are None (ray_trimesh.rs:97-127), and the same feature remap.
 template <class T>
 bool trimesh_cast_ray(const TriMesh<T> &m, const Ray<T> &ray, T max_toi, bool solid, RayHit<T> *out,
-                      RayCounters *cnt) {
+                      RayCounters *cnt, int culling = 0) {
     (void)solid;  // triangles ignore `solid`; Bvh nodes always use solid = true
     const auto &nodes = m.bvh.nodes;
     if (nodes.empty()) return false;
@@ -506,6 +518,7 @@ bool trimesh_cast_ray(const TriMesh<T> &m, const Ray<T> &ray, T max_toi, bool so
         if (cnt) cnt->leaf_tests++;
         V3<T> a, b, c;
         m.tri(prim, a, b, c);
+        if (!culling_keeps(culling, a, b, c, ray.dir)) return false;
         return triangle_cast(a, b, c, ray, best, h);
     };
     uint32_t best_id = UINT32_MAX;
@@ -586,7 +599,7 @@ bool trimesh_cast_ray(const TriMesh<T> &m, const Ray<T> &ray, T max_toi, bool so
 
 // Brute-force nearest hit over all triangles (test helper: pins find_best).
 template <class T>
-bool trimesh_cast_ray_brute(const TriMesh<T> &m, const Ray<T> &ray, T max_toi, RayHit<T> *out) {
+bool trimesh_cast_ray_brute(const TriMesh<T> &m, const Ray<T> &ray, T max_toi, RayHit<T> *out, int culling = 0) {
     bool have = false;
     T best = max_toi;
     uint32_t bid = 0;
@@ -595,7 +608,7 @@ bool trimesh_cast_ray_brute(const TriMesh<T> &m, const Ray<T> &ray, T max_toi, R
         V3<T> a, b, c;
         m.tri(i, a, b, c);
         RayHit<T> h;
-        if (triangle_cast(a, b, c, ray, best, &h) && h.toi < best) {
+        if (culling_keeps(culling, a, b, c, ray.dir) && triangle_cast(a, b, c, ray, best, &h) && h.toi < best) {
             best = h.toi;
             bv = h;
             bid = i;

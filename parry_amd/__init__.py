@@ -22,6 +22,7 @@ from .shape import (
     TriMeshBuilderError,
     BVH_NODE_WIDE_DTYPE,
     split_hits,
+    split_projections,
     ShapeSet,
     SHAPE_DESC_DTYPE,
     SHAPE_BALL,
@@ -45,6 +46,7 @@ __all__ = [
     "TriMeshBuilderError",
     "BVH_NODE_WIDE_DTYPE",
     "split_hits",
+    "split_projections",
     "ShapeSet",
     "SHAPE_DESC_DTYPE",
     "Bvh",

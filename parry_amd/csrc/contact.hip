@@ -953,7 +953,7 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
         if (ns == 0) return EPA_NONE;
         const uint32_t sub = __lane_id() % G;
         const uint32_t gbase = __lane_id() - sub;
-        const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+        const uint64_t gmask = G >= 64 ? ~0ull : ((1ull << (G & 63)) - 1ull);
         const uint64_t below = (1ull << sub) - 1ull;
         // Room for the new faces and their heap entries (at most one per live
         // silhouette edge), checked before anything is committed.
@@ -1106,10 +1106,13 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
 // tier) and sends the pairs whose polytope outgrows its buffers to the next
 // tier's list: G lanes per pair with a small LDS polytope, then one wave per
 // pair with medium and big LDS polytopes, then a global-memory polytope.
-// At most 168 VGPRs (three waves per SIMD): the first tier's small polytopes
-// leave room in LDS for that many waves, and its latency-bound code needs them.
+// At most 168 VGPRs (three waves per SIMD) where a wave's polytopes take at
+// most 16 KB of LDS, which leaves room for that many waves: the first tier's
+// latency-bound code needs them.  Larger polytopes are limited by LDS anyway,
+// and a wave per pair (G = 64) keeps its registers.
+template <int G, class W> constexpr int epa_min_waves() { return G < 64 && (64 / G) * sizeof(W) <= 16384 ? 3 : 1; }
 template <int G, class W>
-__global__ __launch_bounds__(64, 3) void epa_tier_kernel(const ShapeRecord *__restrict__ rec,
+__global__ __launch_bounds__(64, (epa_min_waves<G, W>())) void epa_tier_kernel(const ShapeRecord *__restrict__ rec,
                                                       const float4 *__restrict__ verts, const float *__restrict__ poses,
                                                       const uint32_t *__restrict__ pairs, float prediction,
                                                       const EpaItem *__restrict__ queue, const uint32_t *in_list,

@@ -81,6 +81,10 @@ struct TriHit {
 
 // local_ray_intersection_with_triangle (ray_triangle.rs:70-152) followed by
 // the `toi <= max_toi` acceptance of Triangle::cast_local_ray_and_get_normal.
+// CULL 1 / 2 first drops the triangles RayCullingMode::IgnoreBackfaces /
+// IgnoreFrontfaces skips (ray_trimesh.rs:58-65): its test is the sign of the
+// scaled normal (ab x ac) dotted with the ray direction, which is `d` below.
+template <int CULL = 0>
 __device__ __forceinline__ bool ray_triangle(const float4 q0, const float4 q1, const float4 q2, const LaneRay &r,
                                              float max_toi, TriHit &h) {
     const float ax = q0.x, ay = q0.y, az = q0.z;
@@ -93,6 +97,8 @@ __device__ __forceinline__ bool ray_triangle(const float4 q0, const float4 q1, c
     const float ny = abz * acx - abx * acz;
     const float nz = abx * acy - aby * acx;
     float d = (nx * r.dx + ny * r.dy) + nz * r.dz;
+    if (CULL == 1 && !(d < 0.0f)) return false;
+    if (CULL == 2 && !(d > 0.0f)) return false;
     if (d == 0.0f) return false;
     const float apx = r.ox - ax, apy = r.oy - ay, apz = r.oz - az;
     const float t = (apx * nx + apy * ny) + apz * nz;
@@ -167,12 +173,12 @@ __device__ __forceinline__ void start_traversal(Traversal &t, float max_toi) {
 }
 
 // Tests one leaf (a triangle record) against the current best.
-template <bool COUNT>
+template <bool COUNT, int CULL = 0>
 __device__ __forceinline__ void visit_leaf(const float4 *tri4, uint32_t leaf, const LaneRay &r, Traversal &t,
                                            uint32_t &n_tri) {
     TriHit h;
     if (COUNT) ++n_tri;
-    if (ray_triangle(tri4[3 * leaf], tri4[3 * leaf + 1], tri4[3 * leaf + 2], r, t.best_cost, h) &&
+    if (ray_triangle<CULL>(tri4[3 * leaf], tri4[3 * leaf + 1], tri4[3 * leaf + 2], r, t.best_cost, h) &&
         h.toi < t.best_cost) {
         t.best = h;
         t.best_cost = h.toi;
@@ -186,7 +192,7 @@ __device__ __forceinline__ void visit_leaf(const float4 *tri4, uint32_t leaf, co
 // LDS stack (stride kRayBlock); it never holds more than depth-1 entries, the
 // most a root-to-leaf path can leave pending (each pending entry sits one
 // level deeper than the one below it), and the host sizes it to that.
-template <bool COUNT>
+template <bool COUNT, int CULL = 0>
 __device__ __forceinline__ bool find_best_step(const float4 *node4, const float4 *tri4, const LaneRay &r,
                                                Traversal &t, uint32_t *stk, uint32_t &n_wide, uint32_t &n_tri) {
     if (COUNT) ++n_wide;
@@ -213,7 +219,7 @@ __device__ __forceinline__ bool find_best_step(const float4 *node4, const float4
     bool found_next = false;
     if (ls < t.best_cost && ls != FLT_MAX) {
         if (node_is_leaf(ld)) {
-            visit_leaf<COUNT>(tri4, lc, r, t, n_tri);
+            visit_leaf<COUNT, CULL>(tri4, lc, r, t, n_tri);
         } else {
             t.curr = lc;
             found_next = true;
@@ -221,7 +227,7 @@ __device__ __forceinline__ bool find_best_step(const float4 *node4, const float4
     }
     if (rs < t.best_cost && rs != FLT_MAX) {
         if (node_is_leaf(rd)) {
-            visit_leaf<COUNT>(tri4, rc, r, t, n_tri);
+            visit_leaf<COUNT, CULL>(tri4, rc, r, t, n_tri);
         } else if (found_next) {
             stk[t.sp * kRayBlock] = rc;
             ++t.sp;
@@ -259,8 +265,9 @@ __device__ __forceinline__ void write_hit(pq_ray_hit *hits, uint64_t i, const Tr
 // Static mapping; also serves trees whose root holds a single leaf
 // (bvh_traverse.rs:348-359).  COUNT = true additionally writes, per ray, the
 // number of wide-node records fetched and of triangles tested (the traffic
-// model of the roofline figure).
-template <bool COUNT>
+// model of the roofline figure).  CULL selects the culling variant
+// (TriMesh::cast_local_ray_with_culling, ray_trimesh.rs:152-178).
+template <bool COUNT, int CULL = 0>
 __global__ __launch_bounds__(kRayBlock) void cast_rays_kernel(const pq_bvh_node_wide *__restrict__ nodes,
                                                               const TriRecord *__restrict__ tris, uint32_t ntri,
                                                               const pq_ray *__restrict__ rays, uint64_t n,
@@ -282,9 +289,9 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_kernel(const pq_bvh_node_
         const float4 q0 = node4[0], q1 = node4[1];
         n_wide = 1;
         if (slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi) < max_toi)
-            visit_leaf<true>(tri4, __float_as_uint(q0.w), r, t, n_tri);
+            visit_leaf<true, CULL>(tri4, __float_as_uint(q0.w), r, t, n_tri);
     } else {
-        while (!find_best_step<COUNT>(node4, tri4, r, t, stk, n_wide, n_tri)) {
+        while (!find_best_step<COUNT, CULL>(node4, tri4, r, t, stk, n_wide, n_tri)) {
         }
     }
     if (COUNT) {
@@ -510,6 +517,20 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_batch_persistent_kernel(
 }  // namespace
 
 uint32_t ray_stack_entries(uint32_t depth) { return depth > 1 ? depth - 1 : 1; }
+
+hipError_t launch_cast_rays_culling(const DeviceMesh &m, const pq_ray *rays, uint64_t n, float max_toi,
+                                    uint32_t culling, pq_ray_hit *hits, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const size_t lds = (size_t)ray_stack_entries(m.depth) * kRayBlock * sizeof(uint32_t);
+    const uint64_t blocks = (n + kRayBlock - 1) / kRayBlock;
+    if (culling == 1)
+        hipLaunchKernelGGL((cast_rays_kernel<false, 1>), dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream, m.nodes,
+                           m.tris, m.num_triangles, rays, n, max_toi, hits, nullptr);
+    else
+        hipLaunchKernelGGL((cast_rays_kernel<false, 2>), dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream, m.nodes,
+                           m.tris, m.num_triangles, rays, n, max_toi, hits, nullptr);
+    return hipGetLastError();
+}
 
 hipError_t launch_cast_rays(const DeviceMesh &m, const pq_ray *rays, uint64_t n, float max_toi, pq_ray_hit *hits,
                             uint32_t *counts, hipStream_t stream) {

@@ -164,6 +164,47 @@ class TriMesh:
         )
         return out, counts
 
+    def cast_local_rays_with_culling(self, rays, max_time_of_impact: float, culling: str, out=None, stream=None):
+        """Batched ``TriMesh::cast_local_ray_with_culling`` (ray_trimesh.rs:152-178) on CUDA tensors.
+
+        ``culling`` is "ignore_backfaces" or "ignore_frontfaces" (``RayCullingMode``);
+        returns (n, 5) float32 ``pq_ray_hit`` records (see ``split_hits``).
+        """
+        import torch
+
+        modes = {"ignore_backfaces": 1, "ignore_frontfaces": 2}
+        if culling not in modes:
+            raise ValueError(f"culling must be one of {sorted(modes)}")
+        if not (rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous() and rays.shape[-1] == 6):
+            raise ValueError("rays must be a contiguous CUDA float32 tensor of shape (n, 6)")
+        n = rays.shape[0]
+        if out is None:
+            out = torch.empty((n, 5), dtype=torch.float32, device=rays.device)
+        s = stream if stream is not None else torch.cuda.current_stream(rays.device)
+        _native.check(_native.lib().pq_trimesh_cast_local_rays_with_culling(
+            self._h, rays.data_ptr(), n, float(max_time_of_impact), modes[culling], out.data_ptr(),
+            ctypes.c_void_p(s.cuda_stream)))
+        return out
+
+    def project_local_points(self, points, solid: bool, out=None, stream=None):
+        """Batched ``PointQuery::project_local_point`` (point_composite_shape.rs:164-168).
+
+        ``points``: contiguous CUDA float32 tensor (n, 3).  Returns a (n, 5) float32
+        tensor of ``pq_point_projection`` records: point xyz, then is_inside and the
+        triangle index as uint32 (see ``split_projections``).
+        """
+        import torch
+
+        if not (points.is_cuda and points.dtype == torch.float32 and points.is_contiguous() and points.shape[-1] == 3):
+            raise ValueError("points must be a contiguous CUDA float32 tensor of shape (n, 3)")
+        n = points.shape[0]
+        if out is None:
+            out = torch.empty((n, 5), dtype=torch.float32, device=points.device)
+        s = stream if stream is not None else torch.cuda.current_stream(points.device)
+        _native.check(_native.lib().pq_trimesh_project_local_points(
+            self._h, points.data_ptr(), n, int(bool(solid)), out.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
+        return out
+
     def cast_local_ray_and_get_normal(self, ray: Ray, max_time_of_impact: float, solid: bool) -> Optional[RayIntersection]:
         """Single-ray ``RayCast::cast_local_ray_and_get_normal`` (ray.rs:393-403 in local space)."""
         r = np.asarray([list(ray.origin) + list(ray.dir)], dtype=np.float32)
@@ -183,6 +224,14 @@ def split_hits(hits):
 
     feat = hits.view(torch.int32)[:, 4]
     return hits[:, 0], hits[:, 1:4], feat
+
+
+def split_projections(proj):
+    """Views of a (n, 5) float32 ``pq_point_projection`` tensor: (point, is_inside, triangle as int32)."""
+    import torch
+
+    iv = proj.view(torch.int32)
+    return proj[:, 0:3], iv[:, 3], iv[:, 4]
 
 
 class TriMeshBatch:
