@@ -200,6 +200,99 @@ def batch_leg(args, log):
     return rec
 
 
+def queries_leg(args, log):
+    """Two more TriMesh queries on the C0 mesh (noisy icosphere s=6, 81,920 triangles):
+    point projection (TriMesh::project_local_point, solid = false) of 1,048,576 points
+    uniform in [-1.5, 1.5]^3 (seed 6), and the C0 rays cast with back faces culled
+    (TriMesh::cast_local_ray_with_culling); one step = one launch over every point
+    (ray).  Algorithmic bytes per query: 12 (point) + 20 (record), or 24 (ray) + 20
+    (hit), plus 64 per wide node fetched and 48 per triangle tested, the counts
+    coming from the kernels' instrumented launch."""
+    import numpy as np
+    import torch
+
+    from parry_amd import TriMesh, scenes, split_hits, split_projections
+
+    v, f = scenes.noisy_icosphere(6, seed=1)
+    mesh = TriMesh(v, f)
+    stream = torch.cuda.current_stream()
+    n = 1 << 20
+    pts = np.random.default_rng(6).uniform(-1.5, 1.5, (n, 3)).astype(np.float32)
+    rays = scenes.rays_into_unit_ball(n, seed=2)
+    d_pts = torch.from_numpy(pts).cuda()
+    d_rays = torch.from_numpy(rays).cuda()
+    out = torch.empty((n, 5), dtype=torch.float32, device="cuda")
+    counts = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps * 1e3, e0.elapsed_time(e1) / args.steps
+
+    def roofline(kernel, per_query, gpu_ms):
+        w = int(counts[:, 0].sum(dtype=torch.int64).item())
+        k = int(counts[:, 1].sum(dtype=torch.int64).item())
+        alg = n * per_query + 64 * w + 48 * k
+        gbs = alg / (gpu_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": kernel, "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": alg,
+                "mean_wide_nodes": round(w / n, 3), "mean_triangles": round(k / n, 3)}
+
+    mesh.project_local_points(d_pts, False, out=out, counts=counts)
+    p_wall, p_gpu = timed(lambda: mesh.project_local_points(d_pts, False, out=out, stream=stream))
+    proj = out.cpu().numpy().copy()
+    rec_p = {"metric": "Mpoints/s", "value": round(n / (p_wall * 1e-3) / 1e6, 3), "unit": "Mpoints/s",
+             "ms_per_step": round(p_wall, 4), "gpu_ms_per_step": round(p_gpu, 4),
+             "workload": "TriMesh::project_local_point, solid=false: noisy icosphere s=6 (81920 tris) x 1048576 "
+                         "points uniform in [-1.5,1.5]^3, seed 6",
+             "roofline": roofline("project_points_kernel", 32, p_gpu)}
+    mesh.cast_local_rays_with_culling(d_rays, MAX_TOI, "ignore_backfaces", out=out, counts=counts)
+    c_wall, c_gpu = timed(lambda: mesh.cast_local_rays_with_culling(d_rays, MAX_TOI, "ignore_backfaces", out=out,
+                                                                    stream=stream))
+    hits = out.cpu().numpy().copy()
+    rec_c = {"metric": "Mrays/s", "value": round(n / (c_wall * 1e-3) / 1e6, 3), "unit": "Mrays/s",
+             "ms_per_step": round(c_wall, 4), "gpu_ms_per_step": round(c_gpu, 4),
+             "workload": "TriMesh::cast_local_ray_with_culling(IgnoreBackfaces): the C0 mesh and its 1048576 rays, "
+                         "max_toi=10",
+             "roofline": roofline("cast_rays_kernel<false, 1>", 44, c_gpu)}
+    if not args.no_cpu:
+        from oracle import oracle
+
+        cores = oracle.default_threads()
+        om = oracle.OracleTriMesh(v, f)
+
+        def cpu_time(fn):
+            fn()
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                r = fn()
+                ts.append(time.perf_counter() - t0)
+            return sorted(ts)[1], r
+
+        t, ref = cpu_time(lambda: om.project_points(pts, False, threads=cores))
+        rec_p["cpu_baseline"] = {"value": round(n / t / 1e6, 4), "unit": "Mpoints/s", "cores": cores, "kind": "port",
+                                 "sample": f"all {n} points, median of 3 runs on {cores} threads"}
+        rec_p["parity"] = {
+            "mismatching_records_vs_oracle": int(np.count_nonzero(
+                (proj[:, :3] != ref["point"]).any(axis=1) | (proj.view(np.uint32)[:, 3] != ref["is_inside"])
+                | (proj.view(np.uint32)[:, 4] != ref["triangle"])))}
+        t, ref = cpu_time(lambda: om.cast_rays_with_culling(rays, MAX_TOI, 1, threads=cores))
+        rec_c["cpu_baseline"] = {"value": round(n / t / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
+                                 "sample": f"all {n} rays, median of 3 runs on {cores} threads"}
+        rec_c["parity"] = {"feature_mismatches_vs_oracle": int(np.count_nonzero(
+            hits.view(np.uint32)[:, 4] != ref["feature"]))}
+    return {"project_points": rec_p, "culling": rec_c}
+
+
 def learned_leg(args, log):
     """Learned GJK warm start vs the plain GJK on 1,048,576 mixed convex pairs
     (ConvexPolyhedron 8-64 points, Cuboid, Ball, Capsule, Cone; the overlapping
@@ -479,6 +572,8 @@ def main() -> int:
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched multi-mesh leg")
     ap.add_argument("--skip-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--skip-dynamic", action="store_true", help="skip the dynamic-scene leg")
+    ap.add_argument("--skip-queries", action="store_true", help="skip the point-projection and culling leg")
+    ap.add_argument("--queries-only", action="store_true", help="profiling aid: run only the point/culling leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="profiling aid: only copy BYTES of device memory 4 times (counter calibration)")
     ap.add_argument("--dynamic-only", action="store_true", help="profiling aid: run only the dynamic leg")
@@ -519,6 +614,9 @@ def main() -> int:
         return 0
     if args.batch_only:
         print(json.dumps({"batch": batch_leg(args, log)}), flush=True)
+        return 0
+    if args.queries_only:
+        print(json.dumps({"queries": queries_leg(args, log)}), flush=True)
         return 0
     v, f = scenes.noisy_icosphere(6, seed=1)
     rays = scenes.rays_into_unit_ball(1 << 20, seed=2)
@@ -599,6 +697,9 @@ def main() -> int:
     if not args.skip_batch:
         log("batched multi-mesh leg ...")
         rec["batch"] = batch_leg(args, log)
+    if not args.skip_queries:
+        log("point projection and culling leg ...")
+        rec["queries"] = queries_leg(args, log)
     if not args.skip_learned:
         log("learned warm-start leg ...")
         rec["learned"] = learned_leg(args, log)

@@ -161,6 +161,11 @@ int pq_trimesh_cast_local_rays_host(const pq_trimesh *mesh, const pq_ray *rays, 
 #define PQ_CULL_FRONTFACES 2u /* RayCullingMode::IgnoreFrontfaces */
 int pq_trimesh_cast_local_rays_with_culling(const pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays,
                                             float max_toi, uint32_t culling, pq_ray_hit *hits, void *stream);
+/* Instrumented variant (no counterpart in parry), counts as for
+ * pq_trimesh_cast_local_rays_counted; a culled triangle counts as tested. */
+int pq_trimesh_cast_local_rays_with_culling_counted(const pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays,
+                                                    float max_toi, uint32_t culling, pq_ray_hit *hits,
+                                                    uint32_t *counts, void *stream);
 
 /* PointProjection (src/query/point/point_query.rs:65-77) plus the index of
  * the triangle projected on: FeatureId::Face(triangle) of
@@ -181,6 +186,11 @@ typedef struct pq_point_projection {
  * project_local_point_and_get_feature.  Device pointers; asynchronous. */
 int pq_trimesh_project_local_points(const pq_trimesh *mesh, const float *points, uint64_t num_points, int solid,
                                     pq_point_projection *out, void *stream);
+/* Instrumented variant (no counterpart in parry): also writes, per point, two
+ * u32 counts to `counts` (device, 2*num_points): the Bvh wide-node records
+ * fetched (iterations of find_best's loop) and the triangles tested. */
+int pq_trimesh_project_local_points_counted(const pq_trimesh *mesh, const float *points, uint64_t num_points,
+                                            int solid, pq_point_projection *out, uint32_t *counts, void *stream);
 
 /* ---- Batched ray casts over many TriMeshes (no counterpart in parry: the
  *      batching of many RayCast calls) --------------------------------------- */

@@ -293,7 +293,19 @@ int pq_trimesh_cast_local_rays_with_culling(const pq_trimesh *m, const pq_ray *r
         return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays_with_culling: culling must be PQ_CULL_BACKFACES or "
                                     "PQ_CULL_FRONTFACES");
     if (n && (!rays || !hits)) return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays_with_culling: null buffer");
-    PQ_HIP_CHECK(launch_cast_rays_culling(m->dev, rays, n, max_toi, culling, hits, (hipStream_t)stream));
+    PQ_HIP_CHECK(launch_cast_rays_culling(m->dev, rays, n, max_toi, culling, hits, nullptr, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_trimesh_cast_local_rays_with_culling_counted(const pq_trimesh *m, const pq_ray *rays, uint64_t n,
+                                                    float max_toi, uint32_t culling, pq_ray_hit *hits,
+                                                    uint32_t *counts, void *stream) {
+    if (!m) return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays_with_culling_counted: null mesh");
+    if (culling != PQ_CULL_BACKFACES && culling != PQ_CULL_FRONTFACES)
+        return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays_with_culling_counted: bad culling mode");
+    if (n && (!rays || !hits || !counts))
+        return fail(PQ_ERR_INVALID, "pq_trimesh_cast_local_rays_with_culling_counted: null buffer");
+    PQ_HIP_CHECK(launch_cast_rays_culling(m->dev, rays, n, max_toi, culling, hits, counts, (hipStream_t)stream));
     return PQ_OK;
 }
 
@@ -301,7 +313,16 @@ int pq_trimesh_project_local_points(const pq_trimesh *m, const float *points, ui
                                     pq_point_projection *out, void *stream) {
     if (!m) return fail(PQ_ERR_INVALID, "pq_trimesh_project_local_points: null mesh");
     if (n && (!points || !out)) return fail(PQ_ERR_INVALID, "pq_trimesh_project_local_points: null buffer");
-    PQ_HIP_CHECK(launch_project_points(m->dev, points, n, solid, out, (hipStream_t)stream));
+    PQ_HIP_CHECK(launch_project_points(m->dev, points, n, solid, out, nullptr, (hipStream_t)stream));
+    return PQ_OK;
+}
+
+int pq_trimesh_project_local_points_counted(const pq_trimesh *m, const float *points, uint64_t n, int solid,
+                                            pq_point_projection *out, uint32_t *counts, void *stream) {
+    if (!m) return fail(PQ_ERR_INVALID, "pq_trimesh_project_local_points_counted: null mesh");
+    if (n && (!points || !out || !counts))
+        return fail(PQ_ERR_INVALID, "pq_trimesh_project_local_points_counted: null buffer");
+    PQ_HIP_CHECK(launch_project_points(m->dev, points, n, solid, out, counts, (hipStream_t)stream));
     return PQ_OK;
 }
 

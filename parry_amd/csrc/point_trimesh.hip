@@ -115,11 +115,15 @@ __device__ __forceinline__ void visit_leaf(const float4 *tri4, uint32_t leaf, f3
     }
 }
 
+// COUNT = true also writes, per point, the wide-node records fetched and the
+// triangles tested (the traffic model of the roofline figure).
+template <bool COUNT>
 __global__ __launch_bounds__(kPtBlock) void project_points_kernel(const pq_bvh_node_wide *__restrict__ nodes,
                                                                   const TriRecord *__restrict__ tris,
                                                                   const float *__restrict__ points, uint64_t n,
                                                                   int solid_flag,
-                                                                  pq_point_projection *__restrict__ out) {
+                                                                  pq_point_projection *__restrict__ out,
+                                                                  uint32_t *__restrict__ counts) {
     extern __shared__ uint32_t lds_stack[];
     const uint64_t i = (uint64_t)blockIdx.x * kPtBlock + threadIdx.x;
     if (i >= n) return;
@@ -129,15 +133,20 @@ __global__ __launch_bounds__(kPtBlock) void project_points_kernel(const pq_bvh_n
     const bool solid = solid_flag != 0;
     const f3 pt{points[3 * i], points[3 * i + 1], points[3 * i + 2]};
     PtBest b{FLT_MAX, f3{0.0f, 0.0f, 0.0f}, false, 0u, false};
+    uint32_t n_wide = 0, n_tri = 0;
 
     const float4 root3 = node4[3];
     if ((__float_as_uint(root3.w) & 0x3fffffffu) == 0u) {  // partial root, bvh_traverse.rs:348-359
         const float4 q0 = node4[0], q1 = node4[1];
-        if (aabb_point_dist(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, pt) < FLT_MAX)
+        n_wide = 1;
+        if (aabb_point_dist(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, pt) < FLT_MAX) {
+            ++n_tri;
             visit_leaf(tri4, __float_as_uint(q0.w), pt, solid, b);
+        }
     } else {
         uint32_t curr = 0, sp = 0;
         for (;;) {
+            if (COUNT) ++n_wide;
             const float4 q0 = node4[4 * curr + 0];  // left mins, left children
             const float4 q1 = node4[4 * curr + 1];  // left maxs, left data
             const float4 q2 = node4[4 * curr + 2];  // right mins, right children
@@ -160,6 +169,7 @@ __global__ __launch_bounds__(kPtBlock) void project_points_kernel(const pq_bvh_n
             bool found_next = false;
             if (ls < b.cost && ls != FLT_MAX) {
                 if ((ld & 0x3fffffffu) == 1u) {
+                    if (COUNT) ++n_tri;
                     visit_leaf(tri4, lc, pt, solid, b);
                 } else {
                     curr = lc;
@@ -168,6 +178,7 @@ __global__ __launch_bounds__(kPtBlock) void project_points_kernel(const pq_bvh_n
             }
             if (rs < b.cost && rs != FLT_MAX) {
                 if ((rd & 0x3fffffffu) == 1u) {
+                    if (COUNT) ++n_tri;
                     visit_leaf(tri4, rc, pt, solid, b);
                 } else if (found_next) {
                     stk[sp * kPtBlock] = rc;
@@ -183,6 +194,10 @@ __global__ __launch_bounds__(kPtBlock) void project_points_kernel(const pq_bvh_n
                 curr = stk[sp * kPtBlock];
             }
         }
+    }
+    if (COUNT) {
+        counts[2 * i] = n_wide;
+        counts[2 * i + 1] = n_tri;
     }
     pq_point_projection o;
     if (b.have) {
@@ -202,13 +217,17 @@ __global__ __launch_bounds__(kPtBlock) void project_points_kernel(const pq_bvh_n
 }  // namespace
 
 hipError_t launch_project_points(const DeviceMesh &m, const float *points, uint64_t n, int solid,
-                                 pq_point_projection *out, hipStream_t stream) {
+                                 pq_point_projection *out, uint32_t *counts, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     // The traversal stack never holds more than depth - 1 entries (as for rays).
     const size_t lds = (size_t)ray_stack_entries(m.depth) * kPtBlock * sizeof(uint32_t);
     const uint64_t blocks = (n + kPtBlock - 1) / kPtBlock;
-    hipLaunchKernelGGL(project_points_kernel, dim3((uint32_t)blocks), dim3(kPtBlock), lds, stream, m.nodes, m.tris,
-                       points, n, solid, out);
+    if (counts)
+        hipLaunchKernelGGL(project_points_kernel<true>, dim3((uint32_t)blocks), dim3(kPtBlock), lds, stream, m.nodes,
+                           m.tris, points, n, solid, out, counts);
+    else
+        hipLaunchKernelGGL(project_points_kernel<false>, dim3((uint32_t)blocks), dim3(kPtBlock), lds, stream,
+                           m.nodes, m.tris, points, n, solid, out, counts);
     return hipGetLastError();
 }
 

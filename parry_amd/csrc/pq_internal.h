@@ -98,11 +98,13 @@ uint32_t ray_stack_entries(uint32_t depth);
 hipError_t launch_cast_rays(const DeviceMesh &m, const pq_ray *rays, uint64_t n, float max_toi, pq_ray_hit *hits,
                             uint32_t *counts, hipStream_t stream);
 // The culling variant (1 = ignore back faces, 2 = ignore front faces), static mapping.
+// `counts` as for launch_cast_rays.
 hipError_t launch_cast_rays_culling(const DeviceMesh &m, const pq_ray *rays, uint64_t n, float max_toi,
-                                    uint32_t culling, pq_ray_hit *hits, hipStream_t stream);
+                                    uint32_t culling, pq_ray_hit *hits, uint32_t *counts, hipStream_t stream);
 // Point projections onto the mesh (point_trimesh.hip); solid as in parry.
+// `counts` (nullable, device, 2 per point): wide-node records fetched, triangles tested.
 hipError_t launch_project_points(const DeviceMesh &m, const float *points, uint64_t n, int solid,
-                                 pq_point_projection *out, hipStream_t stream);
+                                 pq_point_projection *out, uint32_t *counts, hipStream_t stream);
 
 // ---- shapes -------------------------------------------------------------
 // Device shape record, 32 B: type tag, vertex range (ConvexPolyhedron) and

@@ -519,16 +519,22 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_batch_persistent_kernel(
 uint32_t ray_stack_entries(uint32_t depth) { return depth > 1 ? depth - 1 : 1; }
 
 hipError_t launch_cast_rays_culling(const DeviceMesh &m, const pq_ray *rays, uint64_t n, float max_toi,
-                                    uint32_t culling, pq_ray_hit *hits, hipStream_t stream) {
+                                    uint32_t culling, pq_ray_hit *hits, uint32_t *counts, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     const size_t lds = (size_t)ray_stack_entries(m.depth) * kRayBlock * sizeof(uint32_t);
-    const uint64_t blocks = (n + kRayBlock - 1) / kRayBlock;
-    if (culling == 1)
-        hipLaunchKernelGGL((cast_rays_kernel<false, 1>), dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream, m.nodes,
-                           m.tris, m.num_triangles, rays, n, max_toi, hits, nullptr);
+    const dim3 grid((uint32_t)((n + kRayBlock - 1) / kRayBlock)), block(kRayBlock);
+    if (culling == 1 && counts)
+        hipLaunchKernelGGL((cast_rays_kernel<true, 1>), grid, block, lds, stream, m.nodes, m.tris, m.num_triangles,
+                           rays, n, max_toi, hits, counts);
+    else if (culling == 1)
+        hipLaunchKernelGGL((cast_rays_kernel<false, 1>), grid, block, lds, stream, m.nodes, m.tris, m.num_triangles,
+                           rays, n, max_toi, hits, counts);
+    else if (counts)
+        hipLaunchKernelGGL((cast_rays_kernel<true, 2>), grid, block, lds, stream, m.nodes, m.tris, m.num_triangles,
+                           rays, n, max_toi, hits, counts);
     else
-        hipLaunchKernelGGL((cast_rays_kernel<false, 2>), dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream, m.nodes,
-                           m.tris, m.num_triangles, rays, n, max_toi, hits, nullptr);
+        hipLaunchKernelGGL((cast_rays_kernel<false, 2>), grid, block, lds, stream, m.nodes, m.tris, m.num_triangles,
+                           rays, n, max_toi, hits, counts);
     return hipGetLastError();
 }
 

@@ -164,11 +164,14 @@ class TriMesh:
         )
         return out, counts
 
-    def cast_local_rays_with_culling(self, rays, max_time_of_impact: float, culling: str, out=None, stream=None):
+    def cast_local_rays_with_culling(self, rays, max_time_of_impact: float, culling: str, out=None, stream=None,
+                                     counts=None):
         """Batched ``TriMesh::cast_local_ray_with_culling`` (ray_trimesh.rs:152-178) on CUDA tensors.
 
         ``culling`` is "ignore_backfaces" or "ignore_frontfaces" (``RayCullingMode``);
-        returns (n, 5) float32 ``pq_ray_hit`` records (see ``split_hits``).
+        returns (n, 5) float32 ``pq_ray_hit`` records (see ``split_hits``).  With a
+        (n, 2) int32 CUDA tensor ``counts``, the instrumented launch also writes each
+        ray's wide-node records fetched and triangles tested.
         """
         import torch
 
@@ -181,17 +184,25 @@ class TriMesh:
         if out is None:
             out = torch.empty((n, 5), dtype=torch.float32, device=rays.device)
         s = stream if stream is not None else torch.cuda.current_stream(rays.device)
-        _native.check(_native.lib().pq_trimesh_cast_local_rays_with_culling(
-            self._h, rays.data_ptr(), n, float(max_time_of_impact), modes[culling], out.data_ptr(),
-            ctypes.c_void_p(s.cuda_stream)))
+        lib = _native.lib()
+        if counts is None:
+            _native.check(lib.pq_trimesh_cast_local_rays_with_culling(
+                self._h, rays.data_ptr(), n, float(max_time_of_impact), modes[culling], out.data_ptr(),
+                ctypes.c_void_p(s.cuda_stream)))
+        else:
+            _native.check(lib.pq_trimesh_cast_local_rays_with_culling_counted(
+                self._h, rays.data_ptr(), n, float(max_time_of_impact), modes[culling], out.data_ptr(),
+                counts.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
         return out
 
-    def project_local_points(self, points, solid: bool, out=None, stream=None):
+    def project_local_points(self, points, solid: bool, out=None, stream=None, counts=None):
         """Batched ``PointQuery::project_local_point`` (point_composite_shape.rs:164-168).
 
         ``points``: contiguous CUDA float32 tensor (n, 3).  Returns a (n, 5) float32
         tensor of ``pq_point_projection`` records: point xyz, then is_inside and the
-        triangle index as uint32 (see ``split_projections``).
+        triangle index as uint32 (see ``split_projections``).  With a (n, 2) int32
+        CUDA tensor ``counts``, the instrumented launch also writes each point's
+        wide-node records fetched and triangles tested.
         """
         import torch
 
@@ -201,8 +212,14 @@ class TriMesh:
         if out is None:
             out = torch.empty((n, 5), dtype=torch.float32, device=points.device)
         s = stream if stream is not None else torch.cuda.current_stream(points.device)
-        _native.check(_native.lib().pq_trimesh_project_local_points(
-            self._h, points.data_ptr(), n, int(bool(solid)), out.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
+        lib = _native.lib()
+        if counts is None:
+            _native.check(lib.pq_trimesh_project_local_points(
+                self._h, points.data_ptr(), n, int(bool(solid)), out.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
+        else:
+            _native.check(lib.pq_trimesh_project_local_points_counted(
+                self._h, points.data_ptr(), n, int(bool(solid)), out.data_ptr(), counts.data_ptr(),
+                ctypes.c_void_p(s.cuda_stream)))
         return out
 
     def cast_local_ray_and_get_normal(self, ray: Ray, max_time_of_impact: float, solid: bool) -> Optional[RayIntersection]:

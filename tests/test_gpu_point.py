@@ -105,3 +105,29 @@ def test_culling_reference_kat():
     _, _, back = _cast_culled(mesh, rays, "ignore_backfaces")
     assert front[0] != miss and front[1] == miss
     assert back[0] == miss and back[1] != miss
+
+
+def test_traversal_counts_match_oracle(orc):
+    """The instrumented launches fetch exactly the oracle's wide records and test
+    exactly its triangles (summed over the queries)."""
+    import torch
+
+    v, f = scenes.noisy_icosphere(5, seed=1)
+    mesh, om = TriMesh(v, f), orc.OracleTriMesh(v, f)
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(-1.6, 1.6, (65536, 3)).astype(np.float32)
+    counts = torch.zeros((pts.shape[0], 2), dtype=torch.int32, device="cuda")
+    print("case: projection counts, 65536 points", flush=True)
+    mesh.project_local_points(torch.from_numpy(pts).cuda(), False, counts=counts)
+    torch.cuda.synchronize()
+    c = counts.cpu().numpy().astype(np.int64).sum(axis=0)
+    ref = om.project_points(pts, False)
+    assert (int(c[0]), int(c[1])) == (ref["W"], ref["K"])
+    rays = scenes.rays_into_unit_ball(65536, seed=12)
+    counts = torch.zeros((rays.shape[0], 2), dtype=torch.int32, device="cuda")
+    print("case: culling counts, 65536 rays", flush=True)
+    mesh.cast_local_rays_with_culling(torch.from_numpy(rays).cuda(), 10.0, "ignore_frontfaces", counts=counts)
+    torch.cuda.synchronize()
+    c = counts.cpu().numpy().astype(np.int64).sum(axis=0)
+    ref = om.cast_rays_with_culling(rays, 10.0, 2)
+    assert (int(c[0]), int(c[1])) == (ref["W"], ref["K"])
