@@ -235,6 +235,7 @@ int pq_graph_destroy(pq_graph *graph);
 #define PQ_SHAPE_CAPSULE 2u           /* params[0] = half height along y, params[1] = radius (capsule.rs:76-82) */
 #define PQ_SHAPE_CONE 3u              /* params[0] = half height along y, params[1] = radius (cone.rs:80-92) */
 #define PQ_SHAPE_CONVEX_POLYHEDRON 4u /* vertex range [first_vertex, +num_vertices) (convex_polyhedron.rs:172-185) */
+#define PQ_SHAPE_TRIANGLE 5u          /* vertices a, b, c = [first_vertex, +3) (src/shape/triangle.rs) */
 
 typedef struct pq_shape_desc {
     uint32_t type;
@@ -330,12 +331,42 @@ int pq_contact_get_tuning(pq_contact_tuning *tuning);
 /* Device workspace bytes pq_contact_pairs needs for `num_pairs` pairs. */
 size_t pq_contact_workspace_bytes(uint64_t num_pairs);
 
+/* ---- Contacts between a TriMesh and convex shapes ------------------------ */
+
+/* query::contact(mesh_pose, TriMesh, poses[q], shape q, prediction) for every
+ * shape q of a shape set: the composite dispatch (default_query_dispatcher.rs:343-346)
+ * to CompositeShapeRef::contact_with_shape (contact_composite_shape_shape.rs:14-42).
+ * Shape q's AABB in the mesh's frame, loosened by `prediction`, selects
+ * triangles in Bvh::leaves order (bvh_traverse.rs:1-68); each gets the contact
+ * dispatch as a Triangle; the smallest dist wins (the first on ties).  Results
+ * are in world space; normal1 and point1 belong to the mesh.
+ * pq_mesh_contact_create copies the mesh's triangles and the shapes into one
+ * device shape table; the mesh and shapes must outlive the handle. */
+typedef struct pq_mesh_contact pq_mesh_contact;
+int pq_mesh_contact_create(const pq_trimesh *mesh, const pq_shapes *shapes, pq_mesh_contact **out);
+int pq_mesh_contact_destroy(pq_mesh_contact *mc);
+/* Device workspace for up to `max_candidates` (triangle, shape) candidates per call. */
+size_t pq_mesh_contact_workspace_bytes(const pq_mesh_contact *mc, uint64_t max_candidates);
+/* Candidates are numbered shape by shape; a shape with a candidate numbered
+ * `max_candidates` or above gets has_contact = PQ_CONTACT_TRUNCATED (the shapes
+ * before it are complete); *num_candidates (device, nullable) receives the
+ * number the call needed. */
+#define PQ_CONTACT_TRUNCATED 0xFFFFFFFEu
+/*   mesh_pose: device, one pq_isometry; poses: device, one per shape;
+ *   out: device, one pq_contact per shape; triangles: device, nullable, the
+ *   index of the triangle in contact (PQ_NO_HIT for None).  Asynchronous on
+ *   `stream`; no allocation. */
+int pq_mesh_contact_shapes(const pq_mesh_contact *mc, const float *mesh_pose, const float *poses, float prediction,
+                           uint64_t max_candidates, pq_contact *out, uint32_t *triangles, uint64_t *num_candidates,
+                           void *workspace, void *stream);
+
 /* Batched query::contact(pos1, g1, pos2, g2, prediction)
  * (src/query/contact/contact_shape_shape.rs:123-138) over shape pairs, with
  * DefaultQueryDispatcher::contact's routing (default_query_dispatcher.rs:302-356):
  * Ball-Ball closed form (contact_ball_ball.rs:9-42); Ball with Cuboid, Capsule,
  * Cone or ConvexPolyhedron by projecting the ball centre
- * (contact_ball_convex_polyhedron.rs:12-63); every other pair by GJK/EPA
+ * (contact_ball_convex_polyhedron.rs:12-63) -- Ball with Triangle too, by the
+ * triangle's closed-form projection; every other pair by GJK/EPA
  * (contact_support_map_support_map.rs:10-77).
  *   shapes:    the shape set; poses: device, one pq_isometry per shape;
  *   pairs:     device, num_pairs x (i, j) shape indices;

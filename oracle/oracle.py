@@ -39,6 +39,8 @@ _SIG = {
     "orc_project_points_f32": (None, [_vp, _vp, _u64, _int, _vp, _vp, _vp, _vp, _int, _int]),
     "orc_project_points_f64": (None, [_vp, _vp, _u64, _int, _vp, _vp, _vp, _vp, _int, _int]),
     "orc_triangle_project_point_f32": (_int, [_vp, _vp, _int, _vp]),
+    "orc_trimesh_contact_shapes_f32": (None, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _f32, _vp, _vp, _vp, _vp, _int]),
+    "orc_trimesh_contact_shapes_f64": (None, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _f64, _vp, _vp, _vp, _vp, _int]),
 }
 
 _lib = None
@@ -168,6 +170,33 @@ class OracleTriMesh:
         f(self.h, p.ctypes.data, n, int(solid), proj.ctypes.data, inside.ctypes.data, tri.ctypes.data,
           cnt.ctypes.data, th, int(brute))
         return {"point": proj, "is_inside": inside, "triangle": tri, "W": int(cnt[0]), "K": int(cnt[1])}
+
+
+def _trimesh_contact_shapes(self, mesh_pose, descs, vertices, poses, prediction: float, threads: int | None = None):
+    """query::contact(mesh_pose, this TriMesh, poses[q], shape q) for every shape q
+    (contact_composite_shape_shape.rs:14-59).  Returns dict(status int32 [n] (1 contact,
+    0 None), out [n,13] (point1, point2, normal1, normal2, dist in world space),
+    triangle u32 [n], candidates: triangles tested in total)."""
+    from parry_amd.shape import SHAPE_DESC_DTYPE
+
+    dt = np.float64 if self.f64 else np.float32
+    d = np.ascontiguousarray(descs, dtype=SHAPE_DESC_DTYPE)
+    v = np.ascontiguousarray(vertices, dtype=dt).reshape(-1, 3)
+    p = np.ascontiguousarray(poses, dtype=dt).reshape(-1, 7)
+    mp = np.ascontiguousarray(mesh_pose, dtype=dt).reshape(7)
+    n = d.shape[0]
+    out = np.zeros((n, 13), dt)
+    status = np.zeros(n, np.int32)
+    tri = np.zeros(n, np.uint32)
+    cand = np.zeros(1, np.uint64)
+    th = default_threads() if threads is None else threads
+    f = lib().orc_trimesh_contact_shapes_f64 if self.f64 else lib().orc_trimesh_contact_shapes_f32
+    f(self.h, mp.ctypes.data, d.ctypes.data, n, v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data,
+      prediction, out.ctypes.data, status.ctypes.data, tri.ctypes.data, cand.ctypes.data, th)
+    return {"status": status, "out": out, "triangle": tri, "candidates": int(cand[0])}
+
+
+OracleTriMesh.contact_shapes = _trimesh_contact_shapes
 
 
 def triangle_project_point(abc, pt, solid: bool):

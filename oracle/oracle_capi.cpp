@@ -453,3 +453,55 @@ void orc_contact_pairs_f64(const void *descs, uint32_t nshapes, const double *ve
 }
 
 }  // extern "C"
+
+namespace {
+template <class T>
+void trimesh_contact_shapes(const TriMesh<T> *mesh, const T *mesh_pose, const ShapeDesc *descs, uint32_t nshapes,
+                            const T *verts, uint32_t nverts, const T *poses, T prediction, T *out, int32_t *status,
+                            uint32_t *tri, uint64_t *candidates, int nthreads) {
+    std::vector<V3<T>> pts(nverts);
+    for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
+    std::vector<Shape<T>> shapes = make_shapes(descs, nshapes, pts);
+    Iso<T> p1{Quat<T>{mesh_pose[0], mesh_pose[1], mesh_pose[2], mesh_pose[3]}, v3(mesh_pose[4], mesh_pose[5], mesh_pose[6])};
+    std::atomic<uint64_t> cand{0};
+    parallel_for(nshapes, nthreads, 64, [&](uint64_t b, uint64_t e) {
+        uint64_t local = 0;
+        for (uint64_t q = b; q < e; ++q) {
+            const T *a = poses + 7 * (size_t)q;
+            Iso<T> p2{Quat<T>{a[0], a[1], a[2], a[3]}, v3(a[4], a[5], a[6])};
+            Contact<T> ct{};
+            uint32_t t = UINT32_MAX;
+            int r = trimesh_contact_shape(*mesh, p1, shapes[q], p2, prediction, ct, &t, &local);
+            status[q] = r;
+            tri[q] = r ? t : UINT32_MAX;
+            T *o = out + 13 * q;
+            if (r) {
+                o[0] = ct.point1.x; o[1] = ct.point1.y; o[2] = ct.point1.z;
+                o[3] = ct.point2.x; o[4] = ct.point2.y; o[5] = ct.point2.z;
+                o[6] = ct.normal1.x; o[7] = ct.normal1.y; o[8] = ct.normal1.z;
+                o[9] = ct.normal2.x; o[10] = ct.normal2.y; o[11] = ct.normal2.z;
+                o[12] = ct.dist;
+            } else {
+                for (int k = 0; k < 13; ++k) o[k] = T(0);
+            }
+        }
+        cand += local;
+    });
+    if (candidates) *candidates = cand.load();
+}
+}  // namespace
+
+extern "C" {
+void orc_trimesh_contact_shapes_f32(void *m, const float *mesh_pose, const void *descs, uint32_t nshapes,
+                                    const float *verts, uint32_t nverts, const float *poses, float prediction,
+                                    float *out, int32_t *status, uint32_t *tri, uint64_t *candidates, int nthreads) {
+    trimesh_contact_shapes<float>((TriMesh<float> *)m, mesh_pose, (const ShapeDesc *)descs, nshapes, verts, nverts,
+                                  poses, prediction, out, status, tri, candidates, nthreads);
+}
+void orc_trimesh_contact_shapes_f64(void *m, const double *mesh_pose, const void *descs, uint32_t nshapes,
+                                    const double *verts, uint32_t nverts, const double *poses, double prediction,
+                                    double *out, int32_t *status, uint32_t *tri, uint64_t *candidates, int nthreads) {
+    trimesh_contact_shapes<double>((TriMesh<double> *)m, mesh_pose, (const ShapeDesc *)descs, nshapes, verts, nverts,
+                                   poses, prediction, out, status, tri, candidates, nthreads);
+}
+}  // extern "C"

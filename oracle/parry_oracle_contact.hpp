@@ -30,8 +30,9 @@ template <> struct Eps<double> {
     static constexpr double eps = 2.220446049250313e-16;
 };
 
-// TRIANGLE (pts[0..3]) is test-only: it lets the reference's triangle EPA
-// known-answer test (crates/parry3d/tests/geometry/epa3.rs:25-59) run here.
+// TRIANGLE (pts[0..3]) is parry's Triangle (src/shape/triangle.rs): a TriMesh
+// part in composite contacts, and the shape of the reference's triangle EPA
+// known-answer test (crates/parry3d/tests/geometry/epa3.rs:25-59).
 // ORIGIN is parry's ConstantOrigin (special_support_maps.rs:32-54), the second
 // operand of GJK/EPA point projections.
 enum ShapeType : uint32_t { BALL = 0, CUBOID = 1, CAPSULE = 2, CONE = 3, CONVEX_POLYHEDRON = 4, TRIANGLE = 5, ORIGIN = 6 };
@@ -838,6 +839,10 @@ template <class T> Aabb<T> shape_aabb(const Iso<T> &m, const Shape<T> &s) {
                 ws[a] = (std::fabs(R[a][0]) * he.x + std::fabs(R[a][1]) * he.y) + std::fabs(R[a][2]) * he.z;
             return Aabb<T>{m.t - ws, m.t + ws};  // Aabb::from_half_extents
         }
+        case TRIANGLE: {  // Triangle::aabb (aabb_triangle.rs:10-30): transformed vertices, a.min(b).min(c)
+            V3<T> a = transform_point(m, s.pts[0]), b = transform_point(m, s.pts[1]), c = transform_point(m, s.pts[2]);
+            return Aabb<T>{vmin(vmin(a, b), c), vmax(vmax(a, b), c)};
+        }
         case CAPSULE: {  // aabb_capsule.rs:8-20
             V3<T> a = transform_point(m, v3(T(0), -s.p[0], T(0))), b = transform_point(m, v3(T(0), s.p[0], T(0)));
             T r = s.p[1];
@@ -1169,6 +1174,53 @@ template <class T> bool support_map_point(const Shape<T> &s, V3<T> pt, PointProj
 
 // contact_convex_polyhedron_ball (contact_ball_convex_polyhedron.rs:27-63) in
 // shape 1's frame.  Returns 1 (contact), 0 (None), -1 (unreachable!() in parry).
+// Triangle::project_local_point_and_get_location (point_triangle.rs:54-291),
+// 3D, for any point.  *inside is PointProjection::is_inside: relative_eq!
+// (projection, point) in the Voronoi-region branches, true in the solid and
+// closest-edge fallbacks of a degenerate face.
+template <class T>
+V3<T> triangle_project_point(V3<T> a, V3<T> b, V3<T> c, V3<T> pt, bool solid, bool *inside) {
+    V3<T> ab = b - a, ac = c - a, ap = pt - a;
+    T ab_ap = dot(ab, ap), ac_ap = dot(ac, ap);
+    auto res = [&](V3<T> proj) {
+        *inside = rel_eq(proj, pt);
+        return proj;
+    };
+    if (ab_ap <= T(0) && ac_ap <= T(0)) return res(a);
+    V3<T> bp = pt - b;
+    T ab_bp = dot(ab, bp), ac_bp = dot(ac, bp);
+    if (ab_bp >= T(0) && ac_bp <= ab_bp) return res(b);
+    V3<T> cp = pt - c;
+    T ab_cp = dot(ab, cp), ac_cp = dot(ac, cp);
+    if (ac_cp >= T(0) && ab_cp <= ac_cp) return res(c);
+    V3<T> bc = c - b;
+    // stable_check_edges_voronoi (dim3)
+    V3<T> n = cross(ab, ac);
+    T vc = dot(n, cross(ab, ap));
+    if (vc < T(0) && ab_ap >= T(0) && ab_bp <= T(0)) return res(a + ab * (ab_ap / dot(ab, ab)));
+    T vb = -dot(n, cross(ac, cp));
+    if (vb < T(0) && ac_ap >= T(0) && ac_cp <= T(0)) return res(a + ac * (ac_ap / dot(ac, ac)));
+    T va = dot(n, cross(bc, bp));
+    if (va < T(0) && ac_bp - ab_bp >= T(0) && ab_cp - ac_cp >= T(0)) return res(b + bc * (dot(bc, bp) / dot(bc, bc)));
+    if (va + vb + vc != T(0)) {
+        T denom = T(1) / (va + vb + vc);
+        T v = vb * denom;
+        T w = vc * denom;
+        return res((a + ab * v) + ac * w);
+    }
+    *inside = true;
+    if (solid) return pt;
+    // Degenerate face, not solid: the closest of the three edges.
+    T v = ab_ap / (ab_ap - ab_bp);
+    T w = ac_ap / (ac_ap - ac_cp);
+    T u = (ac_bp - ab_bp) / (ac_bp - ab_bp + ab_cp - ac_cp);
+    T d_ab = dot(ap, ap) - (dot(ab, ab) * v * v);
+    T d_ac = dot(ap, ap) - (dot(ac, ac) * w * w);
+    T d_bc = dot(bp, bp) - (dot(bc, bc) * u * u);
+    if (d_ab < d_ac) return d_ab < d_bc ? a + ab * v : b + bc * u;
+    return d_ac < d_bc ? a + ac * w : b + bc * u;
+}
+
 template <class T>
 int contact_convex_ball(const Iso<T> &pos12, const Shape<T> &g1, T radius, T prediction, Contact<T> &out,
                         ContactStats *st) {
@@ -1182,6 +1234,12 @@ int contact_convex_ball(const Iso<T> &pos12, const Shape<T> &g1, T radius, T pre
         case CONVEX_POLYHEDRON:
             if (!support_map_point(g1, c, proj, st)) return -1;
             break;
+        case TRIANGLE: {  // project_local_point_and_get_feature: solid = true in 3D (point_triangle.rs:26-48)
+            bool inside = false;
+            proj.point = triangle_project_point(g1.pts[0], g1.pts[1], g1.pts[2], c, true, &inside);
+            proj.inside = inside;
+            break;
+        }
         default: return -1;
     }
     T dist;
@@ -1196,13 +1254,19 @@ int contact_convex_ball(const Iso<T> &pos12, const Shape<T> &g1, T radius, T pre
             n1 = -dir1;
         }
     } else {
-        // Only a Cuboid has a feature normal here: a ConvexPolyhedron's feature is
-        // Unknown whenever this branch is taken (point_support_map.rs:66-72 uses the
-        // same threshold), Capsule and Cone have none (shape.rs:425-431).
+        // A Cuboid's feature normal; a Triangle's normal() for any feature
+        // (triangle.rs:226-228, 626-628; shape.rs:919-928).  A ConvexPolyhedron's
+        // feature is Unknown whenever this branch is taken (point_support_map.rs:66-72
+        // uses the same threshold), Capsule and Cone have none (shape.rs:425-431).
         dist = -radius;
         V3<T> u;
         T l;
-        if (!(g1.type == CUBOID && cuboid_feature_normal(fkind, fid, n1))) {
+        bool have = g1.type == CUBOID && cuboid_feature_normal(fkind, fid, n1);
+        if (g1.type == TRIANGLE && try_unit(cross(g1.pts[1] - g1.pts[0], g1.pts[2] - g1.pts[0]), Eps<T>::eps, u, l)) {
+            n1 = u;
+            have = true;
+        }
+        if (!have) {
             if (try_unit(proj.point, Eps<T>::eps, u, l)) n1 = u;
             else n1 = v3(T(0), T(1), T(0));
         }
@@ -1219,9 +1283,8 @@ int contact_convex_ball(const Iso<T> &pos12, const Shape<T> &g1, T radius, T pre
 
 // query::contact (contact_shape_shape.rs:123-138) with the dispatch of
 // DefaultQueryDispatcher::contact (default_query_dispatcher.rs:302-356) for the
-// shapes held here.  Returns 1 on contact, 0 for None, -1 for pairs not handled
-// by this restatement (a Ball with the test-only Triangle) or where parry
-// reaches unreachable!().
+// shapes held here.  Returns 1 on contact, 0 for None, -1 where parry reaches
+// unreachable!().
 template <class T>
 int contact(const Iso<T> &pos1, const Shape<T> &g1, const Iso<T> &pos2, const Shape<T> &g2, T prediction,
             Contact<T> &out, const V3<T> *init_dir, ContactStats *st) {
@@ -1229,8 +1292,6 @@ int contact(const Iso<T> &pos1, const Shape<T> &g1, const Iso<T> &pos2, const Sh
     bool ok;
     if (g1.type == BALL && g2.type == BALL) {
         ok = contact_ball_ball(pos12, g1.p[0], g2.p[0], prediction, out);
-    } else if ((g1.type == BALL && g2.type == TRIANGLE) || (g1.type == TRIANGLE && g2.type == BALL)) {
-        return -1;
     } else if (g1.type == BALL) {
         // contact_ball_convex_polyhedron: the flipped contact of (shape 2, ball) under pos12^-1
         int r = contact_convex_ball(iso_inverse(pos12), g2, g1.p[0], prediction, out, st);

@@ -1,10 +1,12 @@
 // parry_oracle_point.hpp -- TEST INFRASTRUCTURE ONLY.
 //
-// CPU restatement of two more TriMesh queries of parry3d, for the GPU parity
+// CPU restatement of more TriMesh queries of parry3d, for the GPU parity
 // tests: point projection through the Bvh (TriMesh::project_local_point,
-// src/query/point/point_composite_shape.rs:59-80 and 164-168) and the ray
-// cast with back- or front-face culling (TriMesh::cast_local_ray_with_culling,
-// src/query/ray/ray_trimesh.rs:58-65, 131-178).  Same conventions as
+// src/query/point/point_composite_shape.rs:59-80 and 164-168), the ray cast
+// with back- or front-face culling (TriMesh::cast_local_ray_with_culling,
+// src/query/ray/ray_trimesh.rs:58-65, 131-178; in parry_oracle.hpp), and the
+// contact between a TriMesh and a convex shape (contact_composite_shape_shape,
+// src/query/contact/contact_composite_shape_shape.rs:14-59).  Same conventions as
 // parry_oracle.hpp: T = float is parry3d, T = double parry3d-f64, and every
 // expression keeps the reference's operation order.
 #pragma once
@@ -22,53 +24,6 @@ namespace oracle {
 // |max(mins - p, p - maxs, 0)|.
 template <class T> inline T aabb_point_distance(const Aabb<T> &b, V3<T> p) {
     return norm(vmax(vmax(b.mins - p, p - b.maxs), v3(T(0), T(0), T(0))));
-}
-
-// Triangle::project_local_point_and_get_location (point_triangle.rs:54-291),
-// 3D, for any point.  *inside is PointProjection::is_inside: relative_eq!
-// (projection, point) in the Voronoi-region branches, true in the solid and
-// closest-edge fallbacks of a degenerate face.
-template <class T>
-V3<T> triangle_project_point(V3<T> a, V3<T> b, V3<T> c, V3<T> pt, bool solid, bool *inside) {
-    V3<T> ab = b - a, ac = c - a, ap = pt - a;
-    T ab_ap = dot(ab, ap), ac_ap = dot(ac, ap);
-    auto res = [&](V3<T> proj) {
-        *inside = rel_eq(proj, pt);
-        return proj;
-    };
-    if (ab_ap <= T(0) && ac_ap <= T(0)) return res(a);
-    V3<T> bp = pt - b;
-    T ab_bp = dot(ab, bp), ac_bp = dot(ac, bp);
-    if (ab_bp >= T(0) && ac_bp <= ab_bp) return res(b);
-    V3<T> cp = pt - c;
-    T ab_cp = dot(ab, cp), ac_cp = dot(ac, cp);
-    if (ac_cp >= T(0) && ab_cp <= ac_cp) return res(c);
-    V3<T> bc = c - b;
-    // stable_check_edges_voronoi (dim3)
-    V3<T> n = cross(ab, ac);
-    T vc = dot(n, cross(ab, ap));
-    if (vc < T(0) && ab_ap >= T(0) && ab_bp <= T(0)) return res(a + ab * (ab_ap / dot(ab, ab)));
-    T vb = -dot(n, cross(ac, cp));
-    if (vb < T(0) && ac_ap >= T(0) && ac_cp <= T(0)) return res(a + ac * (ac_ap / dot(ac, ac)));
-    T va = dot(n, cross(bc, bp));
-    if (va < T(0) && ac_bp - ab_bp >= T(0) && ab_cp - ac_cp >= T(0)) return res(b + bc * (dot(bc, bp) / dot(bc, bc)));
-    if (va + vb + vc != T(0)) {
-        T denom = T(1) / (va + vb + vc);
-        T v = vb * denom;
-        T w = vc * denom;
-        return res((a + ab * v) + ac * w);
-    }
-    *inside = true;
-    if (solid) return pt;
-    // Degenerate face, not solid: the closest of the three edges.
-    T v = ab_ap / (ab_ap - ab_bp);
-    T w = ac_ap / (ac_ap - ac_cp);
-    T u = (ac_bp - ab_bp) / (ac_bp - ab_bp + ab_cp - ac_cp);
-    T d_ab = dot(ap, ap) - (dot(ab, ab) * v * v);
-    T d_ac = dot(ap, ap) - (dot(ac, ac) * w * w);
-    T d_bc = dot(bp, bp) - (dot(bc, bc) * u * u);
-    if (d_ab < d_ac) return d_ab < d_bc ? a + ab * v : b + bc * u;
-    return d_ac < d_bc ? a + ac * w : b + bc * u;
 }
 
 template <class T> struct PointHit {
@@ -178,6 +133,70 @@ template <class T> bool trimesh_project_point_brute(const TriMesh<T> &m, V3<T> p
             have = true;
         }
     }
+    return have;
+}
+
+// Bvh::leaves with an AABB-intersection check (bvh_traverse.rs:1-68, through
+// Bvh::intersect_aabb, bvh_queries.rs:203-205): leaves in depth-first order,
+// left subtree before right, a subtree skipped when its box misses.
+template <class T, class F> void bvh_leaves_intersecting(const Bvh<T> &bvh, const Aabb<T> &box, F &&emit) {
+    const auto &nodes = bvh.nodes;
+    if (nodes.empty()) return;
+    std::vector<const Node<T> *> stack;
+    const Node<T> *next = nullptr;
+    if (nodes[0].left.aabb().intersects(box)) next = &nodes[0].left;
+    if (nodes[0].right.leaf_count() > 0 && nodes[0].right.aabb().intersects(box)) stack.push_back(&nodes[0].right);
+    for (;;) {
+        if (!next) {
+            if (stack.empty()) return;
+            next = stack.back();
+            stack.pop_back();
+        }
+        const Node<T> *node = next;
+        next = nullptr;
+        if (node->is_leaf()) {
+            emit(node->children);
+            continue;
+        }
+        const Wide<T> &ch = nodes[node->children];
+        if (ch.left.aabb().intersects(box)) next = &ch.left;
+        if (ch.right.aabb().intersects(box)) {
+            if (!next) next = &ch.right;
+            else stack.push_back(&ch.right);
+        }
+    }
+}
+
+// query::contact(pos1, TriMesh, pos2, g2) for a convex g2: the composite
+// dispatch (default_query_dispatcher.rs:343-346) to
+// CompositeShapeRef::contact_with_shape (contact_composite_shape_shape.rs:14-42).
+// g2's AABB at pos12 loosened by `prediction` selects the triangles; each goes
+// through the contact dispatch as a Triangle with pos12 (a TriMesh part has no
+// pose of its own); the smallest dist wins, the first on ties; an Err or None
+// is skipped.  The result is in world space, as query::contact returns it.
+// Returns 1 with *tri set, or 0.
+template <class T>
+int trimesh_contact_shape(const TriMesh<T> &m, const Iso<T> &pos1, const Shape<T> &g2, const Iso<T> &pos2,
+                          T prediction, Contact<T> &out, uint32_t *tri, uint64_t *candidates) {
+    Iso<T> pos12 = inv_mul(pos1, pos2);
+    Aabb<T> box = shape_aabb(pos12, g2);
+    box.mins = box.mins + v3(-prediction, -prediction, -prediction);  // Aabb::loosened
+    box.maxs = box.maxs + v3(prediction, prediction, prediction);
+    int have = 0;
+    bvh_leaves_intersecting(m.bvh, box, [&](uint32_t prim) {
+        if (candidates) ++*candidates;
+        V3<T> p[3];
+        m.tri(prim, p[0], p[1], p[2]);
+        Shape<T> t{TRIANGLE, p, 3, {T(0), T(0), T(0), T(0)}};
+        Contact<T> c{};
+        ContactStats st;
+        if (contact(pos1, t, pos2, g2, prediction, c, static_cast<const V3<T> *>(nullptr), &st) != 1) return;
+        if (!have || c.dist < out.dist) {
+            out = c;
+            *tri = prim;
+            have = 1;
+        }
+    });
     return have;
 }
 

@@ -14,6 +14,8 @@ from .query import (
     CONTACT_UNSUPPORTED,
     contact_pairs,
     contact_workspace_bytes,
+    MeshContact,
+    PQ_CONTACT_TRUNCATED,
 )
 from .shape import (
     TriMesh,
@@ -30,6 +32,7 @@ from .shape import (
     SHAPE_CAPSULE,
     SHAPE_CONE,
     SHAPE_CONVEX_POLYHEDRON,
+    SHAPE_TRIANGLE,
 )
 from .partitioning import Bvh
 from . import scenes
@@ -55,4 +58,6 @@ __all__ = [
     "CONTACT_DTYPE",
     "contact_pairs",
     "contact_workspace_bytes",
+    "MeshContact",
+    "PQ_CONTACT_TRUNCATED",
 ]

@@ -47,6 +47,10 @@ SIGNATURES = {
     "pq_trimesh_cast_local_rays_host": (_int, [_vp, _vp, _u64, _f32, _int, _vp]),
     "pq_trimesh_cast_local_rays_with_culling": (_int, [_vp, _vp, _u64, _f32, _u32, _vp, _vp]),
     "pq_trimesh_project_local_points": (_int, [_vp, _vp, _u64, _int, _vp, _vp]),
+    "pq_mesh_contact_create": (_int, [_vp, _vp, _c.POINTER(_vp)]),
+    "pq_mesh_contact_destroy": (_int, [_vp]),
+    "pq_mesh_contact_workspace_bytes": (_c.c_size_t, [_vp, _u64]),
+    "pq_mesh_contact_shapes": (_int, [_vp, _vp, _vp, _f32, _u64, _vp, _vp, _vp, _vp, _vp]),
     "pq_trimesh_project_local_points_counted": (_int, [_vp, _vp, _u64, _int, _vp, _vp, _vp]),
     "pq_trimesh_cast_local_rays_with_culling_counted": (_int, [_vp, _vp, _u64, _f32, _u32, _vp, _vp, _vp]),
     "pq_trimesh_batch_create": (_int, [_vp, _u32, _vp, _c.POINTER(_vp)]),

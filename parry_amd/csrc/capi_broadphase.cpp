@@ -44,6 +44,11 @@ int pq_shapes_create(const pq_shape_desc *shapes, uint32_t n, const float *verti
                     return fail(PQ_ERR_INVALID, "pq_shapes_create: shape " + std::to_string(i) +
                                                     " has an empty or out-of-range vertex range");
                 break;
+            case PQ_SHAPE_TRIANGLE:
+                if (d.num_vertices != 3 || (uint64_t)d.first_vertex + 3 > num_vertices)
+                    return fail(PQ_ERR_INVALID, "pq_shapes_create: triangle " + std::to_string(i) +
+                                                    " needs 3 vertices in range");
+                break;
             default:
                 return fail(PQ_ERR_INVALID, "pq_shapes_create: unknown shape type " + std::to_string(d.type));
         }

@@ -87,12 +87,12 @@ __device__ __forceinline__ void finish_contact(const iso &pos1, const iso &pos2,
 
 // Builds the Ball-vs-convex contact (contact_ball_convex_polyhedron.rs:27-63)
 // and maps it to world space; pb = pose of the ball in the convex shape's frame.
-__device__ __forceinline__ void finish_ball_contact(const iso &pos1, const iso &pos2, const iso &pb, uint32_t conv_type,
-                                                    float radius, float prediction, const PointProj &proj, uint32_t fk,
-                                                    uint32_t fid, uint32_t mode, DevContact *o) {
+__device__ __forceinline__ void finish_ball_contact(const iso &pos1, const iso &pos2, const iso &pb, float radius,
+                                                    float prediction, const PointProj &proj, bool has_fn, f3 fnrm,
+                                                    uint32_t mode, DevContact *o) {
     f3 p1, p2, n1, n2;
     float dist;
-    if (!convex_ball_contact(pb, conv_type, radius, prediction, proj, fk, fid, p1, p2, n1, n2, dist)) {
+    if (!convex_ball_contact(pb, radius, prediction, proj, has_fn, fnrm, p1, p2, n1, n2, dist)) {
         *o = no_contact(0);
         return;
     }
@@ -195,10 +195,20 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
                 float l;
                 if (!try_unit(ctr, dir, l)) dir = f3{1.0f, 0.0f, 0.0f};
             } else {
-                if (cv.type == PQ_SHAPE_CUBOID) proj = cuboid_point_feature(cv, ctr, fk, fid);
-                else if (cv.type == PQ_SHAPE_CAPSULE) proj = capsule_point(cv, ctr);
-                else proj = cone_point(cv, ctr);
-                finish_ball_contact(pos1, pos2, pb, cv.type, radius, prediction, proj, fk, fid, ball_mode, c);
+                bool has_fn = false;
+                f3 fnrm{0.0f, 0.0f, 0.0f};
+                if (cv.type == PQ_SHAPE_CUBOID) {
+                    proj = cuboid_point_feature(cv, ctr, fk, fid);
+                    has_fn = cuboid_feature_normal(fk, fid, fnrm);
+                } else if (cv.type == PQ_SHAPE_TRIANGLE) {
+                    proj = triangle_point(cv, ctr);
+                    has_fn = triangle_normal(cv, fnrm);  // any feature (shape.rs:919-928)
+                } else if (cv.type == PQ_SHAPE_CAPSULE) {
+                    proj = capsule_point(cv, ctr);
+                } else {
+                    proj = cone_point(cv, ctr);
+                }
+                finish_ball_contact(pos1, pos2, pb, radius, prediction, proj, has_fn, fnrm, ball_mode, c);
             }
         } else {
             run_gjk = true;
@@ -227,8 +237,8 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
             }
             if (ball_mode != kPairSupportMaps) {
                 if (r.kind == GJK_CLOSEST) {
-                    finish_ball_contact(pos1, pos2, pb, cv.type, radius, prediction, PointProj{false, r.p1},
-                                        CF_UNKNOWN, 0, ball_mode, c);
+                    finish_ball_contact(pos1, pos2, pb, radius, prediction, PointProj{false, r.p1}, false,
+                                        f3{0.0f, 0.0f, 0.0f}, ball_mode, c);
                 } else if (r.kind == GJK_INTERSECTION) {
                     need_epa = true;
                 } else {
@@ -1084,7 +1094,7 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
             return;
         }
         const PointProj proj{true, st == EPA_OK ? p1 : pb.t};
-        finish_ball_contact(pos1, pos2, pb, cv.type, radius, prediction, proj, CF_UNKNOWN, 0, mode, out + k);
+        finish_ball_contact(pos1, pos2, pb, radius, prediction, proj, false, f3{0.0f, 0.0f, 0.0f}, mode, out + k);
         if (iters) iters[2 * k + 1] = ei;
         return;
     }

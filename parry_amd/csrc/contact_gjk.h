@@ -63,6 +63,12 @@ __device__ __forceinline__ f3 local_support(const ShapeRef &s, f3 d) {
         return mul3(div3(d, l), s.p0);
     }
     if (s.type == PQ_SHAPE_CUBOID) return f3{copysignf(s.p0, d.x), copysignf(s.p1, d.y), copysignf(s.p2, d.z)};
+    if (s.type == PQ_SHAPE_TRIANGLE) {  // triangle.rs:697-715
+        const float4 a = s.v[0], b = s.v[1], c = s.v[2];
+        const float d1 = dot3(f3{a.x, a.y, a.z}, d), d2 = dot3(f3{b.x, b.y, b.z}, d), d3 = dot3(f3{c.x, c.y, c.z}, d);
+        const float4 w = d1 > d2 ? (d1 > d3 ? a : c) : (d2 > d3 ? b : c);
+        return f3{w.x, w.y, w.z};
+    }
     if (s.type == kShapeOrigin) return f3{0.0f, 0.0f, 0.0f};
     if (s.type == PQ_SHAPE_CAPSULE) {
         const float sq = norm_sq3(d);
@@ -91,6 +97,76 @@ __device__ __forceinline__ Cso cso_from_shapes(const iso &pos12, const ShapeRef 
     const f3 a = local_support(g1, dir);
     const f3 b = g2.type == kShapeOrigin ? pos12.t : iso_point(pos12, local_support(g2, qrot_inv(pos12.q, neg3(dir))));
     return Cso{sub3(a, b), a, b};
+}
+
+// Shape::compute_aabb (src/shape/shape.rs:369) of record r at pose m.
+// ConvexPolyhedron: point_cloud_aabb (aabb_utils.rs:66-86) -- every vertex
+// posed and folded into a running min/max from vertex 0.  Triangle: the posed
+// vertices, a.min(b).min(c) (aabb_triangle.rs:10-30).  Ball: centre -/+ radius
+// (aabb_ball.rs:7-27).  Cuboid: centre -/+ |R| * half_extents (aabb_cuboid.rs:9-14,
+// utils/isometry_ops.rs:14-19) with R nalgebra's quaternion-to-matrix.  Capsule:
+// the posed segment's AABB grown by the radius (aabb_capsule.rs:8-20).  Cone:
+// support points along -/+ each axis (aabb_support_map.rs:12-14 -> aabb_utils.rs:10-31).
+__device__ __forceinline__ void shape_aabb(const ShapeRecord &r, const float4 *verts, const iso &m, f3 &lo, f3 &hi) {
+    if (r.type == PQ_SHAPE_CONVEX_POLYHEDRON) {
+        const float4 *v = verts + r.first_vertex;
+        float4 p0 = v[0];
+        lo = hi = iso_point(m, f3{p0.x, p0.y, p0.z});
+        for (uint32_t k = 1; k < r.num_vertices; ++k) {
+            const float4 p = v[k];
+            const f3 w = iso_point(m, f3{p.x, p.y, p.z});
+            lo = min3(lo, w);
+            hi = max3(hi, w);
+        }
+    } else if (r.type == PQ_SHAPE_TRIANGLE) {
+        const float4 *v = verts + r.first_vertex;
+        const f3 a = iso_point(m, f3{v[0].x, v[0].y, v[0].z});
+        const f3 b = iso_point(m, f3{v[1].x, v[1].y, v[1].z});
+        const f3 c = iso_point(m, f3{v[2].x, v[2].y, v[2].z});
+        lo = min3(min3(a, b), c);
+        hi = max3(max3(a, b), c);
+    } else if (r.type == PQ_SHAPE_BALL) {
+        const float rad = r.param[0];
+        lo = add3(m.t, f3{-rad, -rad, -rad});
+        hi = add3(m.t, f3{rad, rad, rad});
+    } else if (r.type == PQ_SHAPE_CUBOID) {
+        // UnitQuaternion::to_rotation_matrix, then abs(R) * half_extents.
+        const float qi = m.q.x, qj = m.q.y, qk = m.q.z, qw = m.q.w;
+        const float ww = qw * qw, ii = qi * qi, jj = qj * qj, kk = qk * qk;
+        const float ij = qi * qj * 2.0f, wk = qw * qk * 2.0f, wj = qw * qj * 2.0f;
+        const float ik = qi * qk * 2.0f, jk = qj * qk * 2.0f, wi = qw * qi * 2.0f;
+        const float R[3][3] = {{((ww + ii) - jj) - kk, ij - wk, wj + ik},
+                               {wk + ij, ((ww - ii) + jj) - kk, jk - wi},
+                               {ik - wj, wi + jk, ((ww - ii) - jj) + kk}};
+        const float he[3] = {r.param[0], r.param[1], r.param[2]};
+        float w[3];
+        for (int a = 0; a < 3; ++a)
+            w[a] = (fabsf(R[a][0]) * he[0] + fabsf(R[a][1]) * he[1]) + fabsf(R[a][2]) * he[2];
+        lo = sub3(m.t, f3{w[0], w[1], w[2]});
+        hi = add3(m.t, f3{w[0], w[1], w[2]});
+    } else if (r.type == PQ_SHAPE_CAPSULE) {
+        const f3 a = iso_point(m, f3{0.0f, -r.param[0], 0.0f});
+        const f3 b = iso_point(m, f3{0.0f, r.param[0], 0.0f});
+        const float rad = r.param[1];
+        lo = sub3(min3(a, b), f3{rad, rad, rad});
+        hi = add3(max3(a, b), f3{rad, rad, rad});
+    } else {  // PQ_SHAPE_CONE
+        const ShapeRef sr{r.type, 0, nullptr, r.param[0], r.param[1], r.param[2]};
+        float mn[3], mx[3];
+        for (int d = 0; d < 3; ++d) {
+            f3 basis = f3{0.0f, 0.0f, 0.0f};
+            if (d == 0) basis.x = 1.0f;
+            if (d == 1) basis.y = 1.0f;
+            if (d == 2) basis.z = 1.0f;
+            const f3 pmax = iso_point(m, local_support(sr, qrot_inv(m.q, basis)));
+            const f3 nb = f3{-basis.x, -basis.y, -basis.z};
+            const f3 pmin = iso_point(m, local_support(sr, qrot_inv(m.q, nb)));
+            mx[d] = d == 0 ? pmax.x : (d == 1 ? pmax.y : pmax.z);
+            mn[d] = d == 0 ? pmin.x : (d == 1 ? pmin.y : pmin.z);
+        }
+        lo = f3{mn[0], mn[1], mn[2]};
+        hi = f3{mx[0], mx[1], mx[2]};
+    }
 }
 
 // relative_eq!(proj, origin) with approx's default tolerances.
@@ -743,12 +819,94 @@ __device__ __forceinline__ iso point_proj_iso(f3 c) {
 }
 __device__ __forceinline__ ShapeRef origin_shape() { return ShapeRef{kShapeOrigin, 0, nullptr, 0.0f, 0.0f, 0.0f}; }
 
+// Triangle::normal (triangle.rs:226-228): the unit scaled normal, if not degenerate.
+__device__ __forceinline__ bool triangle_normal(const ShapeRef &t, f3 &n) {
+    const float4 a = t.v[0], b = t.v[1], c = t.v[2];
+    const f3 pa = f3{a.x, a.y, a.z};
+    float len;
+    return try_unit(cross3(sub3(f3{b.x, b.y, b.z}, pa), sub3(f3{c.x, c.y, c.z}, pa)), n, len);
+}
+
+// Triangle::project_local_point_and_get_location (point_triangle.rs:54-291), 3D,
+// for any point.  `inside` is PointProjection::is_inside: relative_eq!
+// (projection, point) in the Voronoi-region branches, true in a degenerate
+// face's fallbacks (the point itself when solid, else the closest edge).
+__device__ __forceinline__ f3 tri_project(f3 a, f3 b, f3 c, f3 pt, bool solid, bool &inside) {
+    const f3 ab = sub3(b, a), ac = sub3(c, a), ap = sub3(pt, a);
+    const float ab_ap = dot3(ab, ap), ac_ap = dot3(ac, ap);
+    if (ab_ap <= 0.0f && ac_ap <= 0.0f) {
+        inside = rel_eq3(a, pt);
+        return a;
+    }
+    const f3 bp = sub3(pt, b);
+    const float ab_bp = dot3(ab, bp), ac_bp = dot3(ac, bp);
+    if (ab_bp >= 0.0f && ac_bp <= ab_bp) {
+        inside = rel_eq3(b, pt);
+        return b;
+    }
+    const f3 cp = sub3(pt, c);
+    const float ab_cp = dot3(ab, cp), ac_cp = dot3(ac, cp);
+    if (ac_cp >= 0.0f && ab_cp <= ac_cp) {
+        inside = rel_eq3(c, pt);
+        return c;
+    }
+    const f3 bc = sub3(c, b);
+    // stable_check_edges_voronoi (dim3)
+    const f3 n = cross3(ab, ac);
+    const float vc = dot3(n, cross3(ab, ap));
+    f3 r;
+    if (vc < 0.0f && ab_ap >= 0.0f && ab_bp <= 0.0f) {
+        r = add3(a, mul3(ab, ab_ap / norm_sq3(ab)));
+        inside = rel_eq3(r, pt);
+        return r;
+    }
+    const float vb = -dot3(n, cross3(ac, cp));
+    if (vb < 0.0f && ac_ap >= 0.0f && ac_cp <= 0.0f) {
+        r = add3(a, mul3(ac, ac_ap / norm_sq3(ac)));
+        inside = rel_eq3(r, pt);
+        return r;
+    }
+    const float va = dot3(n, cross3(bc, bp));
+    if (va < 0.0f && ac_bp - ab_bp >= 0.0f && ab_cp - ac_cp >= 0.0f) {
+        r = add3(b, mul3(bc, dot3(bc, bp) / norm_sq3(bc)));
+        inside = rel_eq3(r, pt);
+        return r;
+    }
+    if (va + vb + vc != 0.0f) {
+        const float denom = 1.0f / (va + vb + vc);
+        const float v = vb * denom, w = vc * denom;
+        r = add3(add3(a, mul3(ab, v)), mul3(ac, w));
+        inside = rel_eq3(r, pt);
+        return r;
+    }
+    inside = true;
+    if (solid) return pt;
+    const float v = ab_ap / (ab_ap - ab_bp);
+    const float w = ac_ap / (ac_ap - ac_cp);
+    const float u = (ac_bp - ab_bp) / (ac_bp - ab_bp + ab_cp - ac_cp);
+    const float d_ab = norm_sq3(ap) - (norm_sq3(ab) * v * v);
+    const float d_ac = norm_sq3(ap) - (norm_sq3(ac) * w * w);
+    const float d_bc = norm_sq3(bp) - (norm_sq3(bc) * u * u);
+    if (d_ab < d_ac) return d_ab < d_bc ? add3(a, mul3(ab, v)) : add3(b, mul3(bc, u));
+    return d_ac < d_bc ? add3(a, mul3(ac, w)) : add3(b, mul3(bc, u));
+}
+
+// Triangle::project_local_point_and_get_feature: solid = true in 3D (point_triangle.rs:26-48).
+__device__ __forceinline__ PointProj triangle_point(const ShapeRef &t, f3 pt) {
+    const float4 a = t.v[0], b = t.v[1], c = t.v[2];
+    bool inside = false;
+    const f3 p = tri_project(f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, pt, true, inside);
+    return PointProj{inside, p};
+}
+
 // contact_convex_polyhedron_ball (contact_ball_convex_polyhedron.rs:27-63) in the
 // convex shape's frame: pos12 maps the ball's frame into the convex shape's.
+// `has_fn`/`fnrm`: the shape's feature_normal_at_point for the projection's
+// feature, when it has one (a Cuboid's face/edge/vertex normal, a Triangle's
+// normal); used only when the ball's centre lies on the projection.
 // Returns false for None.
-__device__ bool convex_ball_contact(const iso &pos12, uint32_t conv_type, float radius, float prediction,
-                                    const PointProj &proj, uint32_t fk, uint32_t fid, f3 &p1, f3 &p2, f3 &n1, f3 &n2,
-                                    float &dist) {
+__device__ bool convex_ball_contact(const iso &pos12, float radius, float prediction, const PointProj &proj,
+                                    bool has_fn, f3 fnrm, f3 &p1, f3 &p2, f3 &n1, f3 &n2, float &dist) {
     const f3 c = pos12.t;
     f3 dir1;
     float len;
@@ -761,11 +919,13 @@ __device__ bool convex_ball_contact(const iso &pos12, uint32_t conv_type, float 
             n1 = neg3(dir1);
         }
     } else {
-        // Only a Cuboid has a feature normal here: a ConvexPolyhedron's feature is
-        // Unknown whenever this branch is taken (same threshold, point_support_map.rs:66-72);
-        // Capsule and Cone have none (shape.rs:425-431).
+        // A ConvexPolyhedron's feature is Unknown whenever this branch is taken
+        // (same threshold, point_support_map.rs:66-72); Capsule and Cone have no
+        // feature normal (shape.rs:425-431).
         dist = -radius;
-        if (!(conv_type == PQ_SHAPE_CUBOID && cuboid_feature_normal(fk, fid, n1))) {
+        if (has_fn) {
+            n1 = fnrm;
+        } else {
             f3 u;
             float l;
             n1 = try_unit(proj.point, u, l) ? u : f3{0.0f, 1.0f, 0.0f};

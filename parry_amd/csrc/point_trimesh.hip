@@ -28,70 +28,6 @@ __device__ __forceinline__ float aabb_point_dist(float mnx, float mny, float mnz
     return sqrtf((sx * sx + sy * sy) + sz * sz);
 }
 
-// Triangle::project_local_point_and_get_location, 3D, for any point.
-// `inside` is PointProjection::is_inside: relative_eq!(projection, point) in
-// the Voronoi-region branches, true in a degenerate face's fallbacks.
-__device__ __forceinline__ f3 tri_project(f3 a, f3 b, f3 c, f3 pt, bool solid, bool &inside) {
-    const f3 ab = sub3(b, a), ac = sub3(c, a), ap = sub3(pt, a);
-    const float ab_ap = dot3(ab, ap), ac_ap = dot3(ac, ap);
-    if (ab_ap <= 0.0f && ac_ap <= 0.0f) {
-        inside = rel_eq3(a, pt);
-        return a;
-    }
-    const f3 bp = sub3(pt, b);
-    const float ab_bp = dot3(ab, bp), ac_bp = dot3(ac, bp);
-    if (ab_bp >= 0.0f && ac_bp <= ab_bp) {
-        inside = rel_eq3(b, pt);
-        return b;
-    }
-    const f3 cp = sub3(pt, c);
-    const float ab_cp = dot3(ab, cp), ac_cp = dot3(ac, cp);
-    if (ac_cp >= 0.0f && ab_cp <= ac_cp) {
-        inside = rel_eq3(c, pt);
-        return c;
-    }
-    const f3 bc = sub3(c, b);
-    // stable_check_edges_voronoi (dim3)
-    const f3 n = cross3(ab, ac);
-    const float vc = dot3(n, cross3(ab, ap));
-    f3 r;
-    if (vc < 0.0f && ab_ap >= 0.0f && ab_bp <= 0.0f) {
-        r = add3(a, mul3(ab, ab_ap / norm_sq3(ab)));
-        inside = rel_eq3(r, pt);
-        return r;
-    }
-    const float vb = -dot3(n, cross3(ac, cp));
-    if (vb < 0.0f && ac_ap >= 0.0f && ac_cp <= 0.0f) {
-        r = add3(a, mul3(ac, ac_ap / norm_sq3(ac)));
-        inside = rel_eq3(r, pt);
-        return r;
-    }
-    const float va = dot3(n, cross3(bc, bp));
-    if (va < 0.0f && ac_bp - ab_bp >= 0.0f && ab_cp - ac_cp >= 0.0f) {
-        r = add3(b, mul3(bc, dot3(bc, bp) / norm_sq3(bc)));
-        inside = rel_eq3(r, pt);
-        return r;
-    }
-    if (va + vb + vc != 0.0f) {
-        const float denom = 1.0f / (va + vb + vc);
-        const float v = vb * denom, w = vc * denom;
-        r = add3(add3(a, mul3(ab, v)), mul3(ac, w));
-        inside = rel_eq3(r, pt);
-        return r;
-    }
-    inside = true;
-    if (solid) return pt;
-    // Degenerate face, not solid: the closest of the three edges.
-    const float v = ab_ap / (ab_ap - ab_bp);
-    const float w = ac_ap / (ac_ap - ac_cp);
-    const float u = (ac_bp - ab_bp) / (ac_bp - ab_bp + ab_cp - ac_cp);
-    const float d_ab = norm_sq3(ap) - (norm_sq3(ab) * v * v);
-    const float d_ac = norm_sq3(ap) - (norm_sq3(ac) * w * w);
-    const float d_bc = norm_sq3(bp) - (norm_sq3(bc) * u * u);
-    if (d_ab < d_ac) return d_ab < d_bc ? add3(a, mul3(ab, v)) : add3(b, mul3(bc, u));
-    return d_ac < d_bc ? add3(a, mul3(ac, w)) : add3(b, mul3(bc, u));
-}
-
 struct PtBest {
     float cost;
     f3 p;

@@ -157,6 +157,14 @@ hipError_t bvh_scan_temp_bytes(uint32_t num_leaves, size_t *bytes);
 // re-runs pairs whose polytope outgrew the LDS capacity.
 constexpr uint32_t kEpaGlobalWaves = 32;
 size_t contact_workspace_bytes(uint64_t num_pairs);
+
+// TriMesh-against-shapes contacts (composite.hip).
+hipError_t build_mesh_shape_records(const DeviceMesh &m, ShapeRecord *rec, float4 *verts, hipStream_t stream);
+size_t mesh_contact_workspace_bytes(uint32_t ntri, uint32_t n, uint64_t capacity);
+hipError_t launch_mesh_contacts(const DeviceMesh &m, const DeviceShapes &combined, uint32_t n, const float *mesh_pose,
+                                const float *poses, float prediction, uint64_t capacity, pq_contact *out,
+                                uint32_t *triangles, unsigned long long *num_candidates, void *workspace,
+                                hipStream_t stream);
 // m_dev (nullable, device): the number of pairs actually present (<= m), e.g.
 // the count written by pq_bvh_self_pairs, so a frame needs no host round trip.
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
@@ -191,6 +199,14 @@ struct pq_graph {
 struct pq_trimesh_batch {
     pq::DeviceBatch dev;
     std::vector<void *> allocations;
+};
+
+struct pq_mesh_contact {
+    const pq_trimesh *mesh = nullptr;
+    pq::DeviceShapes combined{};  // the mesh's triangles (Bvh leaf record order), then the shapes
+    uint32_t num_shapes = 0;
+    void *d_rec = nullptr;
+    void *d_verts = nullptr;
 };
 
 struct pq_trimesh {

@@ -111,3 +111,70 @@ def contact_workspace_bytes(num_pairs: int) -> int:
     from . import _native
 
     return int(_native.lib().pq_contact_workspace_bytes(num_pairs))
+
+
+PQ_CONTACT_TRUNCATED = 0xFFFFFFFE
+
+
+class MeshContact:
+    """``query::contact`` between one ``TriMesh`` and every shape of a ``ShapeSet``
+    (contact_composite_shape_shape.rs:14-59, through DefaultQueryDispatcher::contact,
+    default_query_dispatcher.rs:343-346): per shape, the first contact with the
+    smallest dist among the mesh triangles its loosened AABB selects.
+    """
+
+    def __init__(self, mesh, shapes):
+        import ctypes
+
+        from . import _native
+
+        self.mesh, self.shapes = mesh, shapes  # kept alive: the handle points into both
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().pq_mesh_contact_create(mesh._h, shapes.handle, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        from . import _native
+
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _native.lib().pq_mesh_contact_destroy(self._h)
+            self._h = None
+
+    def workspace_bytes(self, max_candidates: int) -> int:
+        from . import _native
+
+        return int(_native.lib().pq_mesh_contact_workspace_bytes(self._h, int(max_candidates)))
+
+    def contact(self, mesh_pose, poses, prediction: float, max_candidates: int, out=None, triangles=None,
+                workspace=None, stream=None):
+        """``mesh_pose``: (7,) and ``poses``: (n, 7) float32 CUDA tensors (i j k w tx ty tz).
+        Returns (contacts (n, 14) f32 pq_contact records, triangles (n,) int32, the number of
+        candidates the call needed as a (1,) int64 tensor); all asynchronous on ``stream``.
+        """
+        import ctypes
+
+        import torch
+
+        from . import _native
+
+        n = poses.shape[0]
+        dev = poses.device
+        for t in (mesh_pose, poses):
+            if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+                raise ValueError("mesh_pose and poses must be contiguous CUDA float32 tensors")
+        if tuple(mesh_pose.shape) != (7,) or tuple(poses.shape) != (len(self.shapes), 7):
+            raise ValueError(f"mesh_pose must be (7,) and poses ({len(self.shapes)}, 7)")
+        if int(max_candidates) < 0:
+            raise ValueError("max_candidates must be non-negative")
+        if out is None:
+            out = torch.empty((n, 14), dtype=torch.float32, device=dev)
+        if triangles is None:
+            triangles = torch.empty(n, dtype=torch.int32, device=dev)
+        if workspace is None:
+            workspace = torch.empty(self.workspace_bytes(max_candidates), dtype=torch.uint8, device=dev)
+        needed = torch.zeros(1, dtype=torch.int64, device=dev)
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        _native.check(_native.lib().pq_mesh_contact_shapes(
+            self._h, mesh_pose.data_ptr(), poses.data_ptr(), float(prediction), int(max_candidates), out.data_ptr(),
+            triangles.data_ptr(), needed.data_ptr(), workspace.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
+        return out, triangles, needed

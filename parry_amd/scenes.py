@@ -349,3 +349,20 @@ def mixed_scene(n: int, seed: int, target_overlaps: float = 8.0):
     poses = np.ascontiguousarray(np.concatenate([q, t], 1), dtype=np.float32)
     return {"descs": descs, "vertices": verts, "poses": poses, "side": L}
 
+
+
+def shapes_around_mesh(n: int, seed: int, mesh_radius: float = 2.0, scale: float = 0.4,
+                       shell=(0.3, 0.8)):
+    """``mixed_shapes(n, seed)`` scaled by ``scale`` and placed around a mesh of
+    radius ``mesh_radius`` centred at the origin: random rotations, centres at
+    distance U[shell] x 2 mesh_radius along uniform random directions, so a part
+    of the shapes straddles the surface and a part stands clear of it.
+
+    Returns (descs, vertices float32 [V, 3], poses float32 [n, 7]).
+    """
+    descs, verts = mixed_shapes(n, seed)
+    rng = _rng(seed + 1)
+    q = random_rotations(n, rng)
+    t = _unit_vectors(rng, n) * rng.uniform(shell[0], shell[1], (n, 1)) * (2.0 * mesh_radius)
+    descs["params"] *= scale
+    return descs, (verts * scale).astype(np.float32), np.concatenate([q, t], 1).astype(np.float32)

@@ -389,6 +389,7 @@ SHAPE_CUBOID = 1
 SHAPE_CAPSULE = 2
 SHAPE_CONE = 3
 SHAPE_CONVEX_POLYHEDRON = 4
+SHAPE_TRIANGLE = 5
 
 SHAPE_DESC_DTYPE = np.dtype(
     [("type", "<u4"), ("first_vertex", "<u4"), ("num_vertices", "<u4"), ("reserved", "<u4"), ("params", "<f4", 4)]
