@@ -293,6 +293,89 @@ def queries_leg(args, log):
     return {"project_points": rec_p, "culling": rec_c}
 
 
+def mesh_contact_leg(args, log):
+    """query::contact between one TriMesh and many convex shapes
+    (contact_composite_shape_shape.rs:14-59): 262,144 mixed shapes (Ball, Cuboid,
+    Capsule, Cone, ConvexPolyhedron of 8-64 points; scale 0.15, seed 13) resting on
+    a noisy icosphere s=6 of radius 8 (81,920 triangles), centres at 0.96-1.04 x
+    the radius; prediction 0.01.  One step = one pq_mesh_contact_shapes call: the
+    Bvh candidate walk (count, scan, write), query::contact on every (triangle,
+    shape) candidate, and the per-shape selection."""
+    import numpy as np
+    import torch
+
+    from parry_amd import MeshContact, ShapeSet, TriMesh, scenes
+
+    v, f = scenes.noisy_icosphere(6, seed=1, radius=8.0)
+    n = 1 << 18
+    descs, verts, poses_np = scenes.shapes_around_mesh(n, seed=13, mesh_radius=8.0, scale=0.15, shell=(0.48, 0.52))
+    mesh = TriMesh(v, f)
+    shapes = ShapeSet(descs, verts)
+    mc = MeshContact(mesh, shapes)
+    stream = torch.cuda.current_stream()
+    mesh_pose = torch.tensor([0, 0, 0, 1, 0, 0, 0], dtype=torch.float32, device="cuda")
+    poses = torch.from_numpy(poses_np).cuda()
+    needed = torch.zeros(1, dtype=torch.int64, device="cuda")
+    out = torch.empty((n, 14), dtype=torch.float32, device="cuda")
+    tri = torch.empty(n, dtype=torch.int32, device="cuda")
+    ws0 = torch.empty(mc.workspace_bytes(0), dtype=torch.uint8, device="cuda")
+    mc.contact(mesh_pose, poses, 0.01, 0, out=out, triangles=tri, workspace=ws0, needed=needed)
+    cap = int(needed.item())
+    del ws0
+    ws = torch.empty(mc.workspace_bytes(cap), dtype=torch.uint8, device="cuda")
+
+    def step():
+        mc.contact(mesh_pose, poses, 0.01, cap, out=out, triangles=tri, workspace=ws, stream=stream, needed=needed)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        step()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    gpu_ms = ev[0].elapsed_time(ev[1]) / args.steps
+    o = out.cpu().numpy()
+    has = o.view(np.uint32)[:, 13]
+    got_tri = tri.cpu().numpy().view(np.uint32)
+    rec = {
+        "metric": "shape queries/s",
+        "value": round(n / wall, 1),
+        "unit": "shapes/s",
+        "ms_per_step": round(wall * 1e3, 4),
+        "gpu_ms_per_step": round(gpu_ms, 4),
+        "workload": "query::contact(TriMesh, shape) for 262144 mixed shapes (scale 0.15, seed 13) on a noisy "
+                    "icosphere s=6 of radius 8 (81920 tris), centres at 0.96-1.04 x radius, prediction 0.01",
+        "candidates": cap,
+        "mean_candidates_per_shape": round(cap / n, 3),
+        "contacts": int((has == 1).sum()),
+        "candidates_per_s": round(cap / wall, 1),
+    }
+    if not args.no_cpu:
+        from oracle import oracle
+
+        cores = oracle.default_threads()
+        om = oracle.OracleTriMesh(v, f)
+        t0 = time.perf_counter()
+        ref = om.contact_shapes(np.array([0, 0, 0, 1, 0, 0, 0], np.float32), descs, verts, poses_np, 0.01,
+                                threads=cores)
+        t_cpu = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(n / t_cpu, 1), "unit": "shapes/s", "cores": cores, "kind": "port",
+                               "sample": f"all {n} shapes, one run on {cores} threads (C++ restatement of parry3d f32)"}
+        c = ref["status"] == 1
+        rec["parity"] = {
+            "has_contact_mismatches": int((has.astype(np.int64) != ref["status"].astype(np.int64)).sum()),
+            "triangle_mismatches": int((got_tri[c] != ref["triangle"][c]).sum()),
+            "bit_identical_contacts": int(np.all(o[c, :13] == ref["out"][c], axis=1).sum()),
+            "candidates_oracle": int(ref["candidates"]),
+        }
+    return rec
+
+
 def learned_leg(args, log):
     """Learned GJK warm start vs the plain GJK on 1,048,576 mixed convex pairs
     (ConvexPolyhedron 8-64 points, Cuboid, Ball, Capsule, Cone; the overlapping
@@ -574,6 +657,8 @@ def main() -> int:
     ap.add_argument("--skip-dynamic", action="store_true", help="skip the dynamic-scene leg")
     ap.add_argument("--skip-queries", action="store_true", help="skip the point-projection and culling leg")
     ap.add_argument("--queries-only", action="store_true", help="profiling aid: run only the point/culling leg")
+    ap.add_argument("--skip-mesh-contact", action="store_true", help="skip the TriMesh-shape contact leg")
+    ap.add_argument("--mesh-contact-only", action="store_true", help="profiling aid: run only the mesh contact leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="profiling aid: only copy BYTES of device memory 4 times (counter calibration)")
     ap.add_argument("--dynamic-only", action="store_true", help="profiling aid: run only the dynamic leg")
@@ -617,6 +702,9 @@ def main() -> int:
         return 0
     if args.queries_only:
         print(json.dumps({"queries": queries_leg(args, log)}), flush=True)
+        return 0
+    if args.mesh_contact_only:
+        print(json.dumps({"mesh_contact": mesh_contact_leg(args, log)}), flush=True)
         return 0
     v, f = scenes.noisy_icosphere(6, seed=1)
     rays = scenes.rays_into_unit_ball(1 << 20, seed=2)
@@ -700,6 +788,9 @@ def main() -> int:
     if not args.skip_queries:
         log("point projection and culling leg ...")
         rec["queries"] = queries_leg(args, log)
+    if not args.skip_mesh_contact:
+        log("TriMesh-shape contact leg ...")
+        rec["mesh_contact"] = mesh_contact_leg(args, log)
     if not args.skip_learned:
         log("learned warm-start leg ...")
         rec["learned"] = learned_leg(args, log)

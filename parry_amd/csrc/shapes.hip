@@ -1,13 +1,15 @@
 // shapes.hip -- world-space AABBs of posed shapes (Shape::compute_aabb, src/shape/shape.rs:369).
 //
-// One lane per shape.  ConvexPolyhedron: point_cloud_aabb (aabb_utils.rs:66-86)
-// -- every vertex is transformed by the pose (quaternion rotation then
-// translation) and folded into a running min/max, starting from vertex 0.
-// Ball: centre -/+ radius (aabb_ball.rs:7-27).  Cuboid: centre -/+
-// |R| * half_extents (aabb_cuboid.rs:9-14, utils/isometry_ops.rs:14-19) with R
-// nalgebra's quaternion-to-matrix.  Capsule: the posed segment's AABB grown by
-// the radius (aabb_capsule.rs:8-20).  Cone: support points along -/+ each axis
-// (aabb_support_map.rs:12-14 -> aabb_utils.rs:10-31).
+// One lane per shape; the per-type rules are shape_aabb's (contact_gjk.h):
+// ConvexPolyhedron: point_cloud_aabb (aabb_utils.rs:66-86) -- every vertex is
+// transformed by the pose (quaternion rotation then translation) and folded
+// into a running min/max, starting from vertex 0.  Ball: centre -/+ radius
+// (aabb_ball.rs:7-27).  Cuboid: centre -/+ |R| * half_extents
+// (aabb_cuboid.rs:9-14, utils/isometry_ops.rs:14-19) with R nalgebra's
+// quaternion-to-matrix.  Capsule: the posed segment's AABB grown by the radius
+// (aabb_capsule.rs:8-20).  Cone: support points along -/+ each axis
+// (aabb_support_map.rs:12-14 -> aabb_utils.rs:10-31).  Triangle: the min/max of
+// the three posed vertices (aabb_triangle.rs:16-30).
 #include "contact_gjk.h"
 #include "pq_internal.h"
 #include "pq_math.h"

@@ -134,10 +134,14 @@ class MeshContact:
         self._h = h
 
     def __del__(self):
-        from . import _native
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                from . import _native
 
-        if getattr(self, "_h", None) is not None and self._h.value:
-            _native.lib().pq_mesh_contact_destroy(self._h)
+                _native.lib().pq_mesh_contact_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     def workspace_bytes(self, max_candidates: int) -> int:
@@ -146,7 +150,7 @@ class MeshContact:
         return int(_native.lib().pq_mesh_contact_workspace_bytes(self._h, int(max_candidates)))
 
     def contact(self, mesh_pose, poses, prediction: float, max_candidates: int, out=None, triangles=None,
-                workspace=None, stream=None):
+                workspace=None, stream=None, needed=None):
         """``mesh_pose``: (7,) and ``poses``: (n, 7) float32 CUDA tensors (i j k w tx ty tz).
         Returns (contacts (n, 14) f32 pq_contact records, triangles (n,) int32, the number of
         candidates the call needed as a (1,) int64 tensor); all asynchronous on ``stream``.
@@ -172,7 +176,8 @@ class MeshContact:
             triangles = torch.empty(n, dtype=torch.int32, device=dev)
         if workspace is None:
             workspace = torch.empty(self.workspace_bytes(max_candidates), dtype=torch.uint8, device=dev)
-        needed = torch.zeros(1, dtype=torch.int64, device=dev)
+        if needed is None:
+            needed = torch.zeros(1, dtype=torch.int64, device=dev)
         s = stream if stream is not None else torch.cuda.current_stream(dev)
         _native.check(_native.lib().pq_mesh_contact_shapes(
             self._h, mesh_pose.data_ptr(), poses.data_ptr(), float(prediction), int(max_candidates), out.data_ptr(),
