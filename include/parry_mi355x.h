@@ -331,6 +331,54 @@ int pq_contact_get_tuning(pq_contact_tuning *tuning);
 /* Device workspace bytes pq_contact_pairs needs for `num_pairs` pairs. */
 size_t pq_contact_workspace_bytes(uint64_t num_pairs);
 
+/* ---- Linear shape casts ---------------------------------------------------- */
+
+/* ShapeCastOptions (src/query/shape_cast/shape_cast.rs:105-146).  parry's
+ * default: max_time_of_impact = FLT_MAX, target_distance = 0,
+ * stop_at_penetration = 1, compute_impact_geometry_on_penetration = 1. */
+typedef struct pq_shape_cast_options {
+    float max_time_of_impact;
+    float target_distance;
+    uint32_t stop_at_penetration;
+    uint32_t compute_impact_geometry_on_penetration;
+} pq_shape_cast_options;
+
+/* Option<ShapeCastHit> (shape_cast.rs:36-70): status PQ_CAST_NONE is None,
+ * otherwise ShapeCastStatus (shape_cast.rs:12-33).  witness1 and normal1 are
+ * in shape 1's local frame, witness2 and normal2 in shape 2's. */
+#define PQ_CAST_NONE 0u
+#define PQ_CAST_OUT_OF_ITERATIONS 1u
+#define PQ_CAST_CONVERGED 2u
+#define PQ_CAST_FAILED 3u
+#define PQ_CAST_PENETRATING 4u
+typedef struct pq_shape_cast_hit {
+    float time_of_impact;
+    float witness1[3];
+    float witness2[3];
+    float normal1[3];
+    float normal2[3];
+    uint32_t status;
+} pq_shape_cast_hit;
+
+/* Batched query::cast_shapes(pos1, vel1, g1, pos2, vel2, g2, options)
+ * (shape_cast.rs:274-286) with DefaultQueryDispatcher::cast_shapes's routing
+ * (default_query_dispatcher.rs:434-540): Ball-Ball in closed form
+ * (shape_cast_ball_ball.rs:10-68); every other pair of the shape set by GJK's
+ * ray cast on the Minkowski difference (cast_shapes_support_map_support_map,
+ * shape_cast_support_map_support_map.rs:11-72; gjk.rs:632-795), with the
+ * impact geometry of penetrating pairs from contact_support_map_support_map.
+ *   poses: device, one pq_isometry per shape; vels: device, 3 floats (linear
+ *   velocity) per shape; pairs: device, num_pairs x (i, j);
+ *   out: device, one pq_shape_cast_hit per pair;
+ *   iters: device, nullable, GJK ray-cast iterations per pair (0 for Ball-Ball);
+ *   workspace: device, pq_cast_shapes_workspace_bytes(num_pairs) bytes, or NULL
+ *   to let the library allocate it on `stream`.  Asynchronous on `stream`. */
+int pq_cast_shapes_pairs(const pq_shapes *shapes, const pq_isometry *poses, const float *vels, const uint32_t *pairs,
+                         uint64_t num_pairs, const pq_shape_cast_options *options, pq_shape_cast_hit *out,
+                         uint32_t *iters, void *workspace, void *stream);
+/* Device workspace bytes pq_cast_shapes_pairs needs for `num_pairs` pairs. */
+size_t pq_cast_shapes_workspace_bytes(uint64_t num_pairs);
+
 /* ---- Contacts between a TriMesh and convex shapes ------------------------ */
 
 /* query::contact(mesh_pose, TriMesh, poses[q], shape q, prediction) for every

@@ -41,6 +41,10 @@ _SIG = {
     "orc_triangle_project_point_f32": (_int, [_vp, _vp, _int, _vp]),
     "orc_trimesh_contact_shapes_f32": (None, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _f32, _vp, _vp, _vp, _vp, _int]),
     "orc_trimesh_contact_shapes_f64": (None, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _f64, _vp, _vp, _vp, _vp, _int]),
+    "orc_cast_shapes_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, _u64, _f32, _f32, _int, _int, _vp, _vp, _vp,
+                                   _int]),
+    "orc_cast_shapes_f64": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, _u64, _f64, _f64, _int, _int, _vp, _vp, _vp,
+                                   _int]),
 }
 
 _lib = None
@@ -359,4 +363,35 @@ def contact_pairs(descs, vertices, poses, pairs, prediction: float, init_dirs=No
     f(d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data, pr.ctypes.data, m,
       prediction, None if seeds is None else seeds.ctypes.data, out.ctypes.data, status.ctypes.data,
       iters.ctypes.data, default_threads() if threads is None else threads)
+    return {"status": status, "out": out, "iters": iters}
+
+
+CAST_NONE, CAST_OUT_OF_ITERATIONS, CAST_CONVERGED, CAST_FAILED, CAST_PENETRATING = range(5)
+
+
+def cast_shapes(descs, vertices, poses, vels, pairs, max_toi: float = float("inf"), target_distance: float = 0.0,
+                stop_at_penetration: bool = True, compute_impact_geometry_on_penetration: bool = True,
+                f64: bool = False, threads: int | None = None):
+    """query::cast_shapes (shape_cast.rs:274-286) on every pair with per-shape linear
+    velocities; returns dict(status uint32 [m] (0 None, else ShapeCastStatus + 1),
+    out [m,13] (toi, witness1, witness2, normal1, normal2 in the shapes' local
+    frames), iters uint32 [m] (minkowski_ray_cast passes)).  max_toi = inf means
+    parry's default Real::MAX."""
+    from parry_amd.shape import SHAPE_DESC_DTYPE
+
+    dt = np.float64 if f64 else np.float32
+    d = np.ascontiguousarray(descs, dtype=SHAPE_DESC_DTYPE)
+    v = np.ascontiguousarray(vertices, dtype=dt).reshape(-1, 3)
+    p = np.ascontiguousarray(poses, dtype=dt).reshape(-1, 7)
+    vl = np.ascontiguousarray(vels, dtype=dt).reshape(-1, 3)
+    pr = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    m = pr.shape[0]
+    out = np.zeros((m, 13), dt)
+    status = np.zeros(m, np.uint32)
+    iters = np.zeros(m, np.uint32)
+    mt = float(np.finfo(dt).max) if not np.isfinite(max_toi) else max_toi
+    f = lib().orc_cast_shapes_f64 if f64 else lib().orc_cast_shapes_f32
+    f(d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data, vl.ctypes.data,
+      pr.ctypes.data, m, mt, target_distance, int(stop_at_penetration), int(compute_impact_geometry_on_penetration),
+      out.ctypes.data, status.ctypes.data, iters.ctypes.data, default_threads() if threads is None else threads)
     return {"status": status, "out": out, "iters": iters}

@@ -15,6 +15,7 @@
 #include "parry_oracle.hpp"
 #include "parry_oracle_contact.hpp"
 #include "parry_oracle_point.hpp"
+#include "parry_oracle_cast.hpp"
 
 using namespace oracle;
 
@@ -503,5 +504,64 @@ void orc_trimesh_contact_shapes_f64(void *m, const double *mesh_pose, const void
                                     double *out, int32_t *status, uint32_t *tri, uint64_t *candidates, int nthreads) {
     trimesh_contact_shapes<double>((TriMesh<double> *)m, mesh_pose, (const ShapeDesc *)descs, nshapes, verts, nverts,
                                    poses, prediction, out, status, tri, candidates, nthreads);
+}
+}  // extern "C"
+
+// ---- shape casts ---------------------------------------------------------------
+namespace {
+template <class T>
+void cast_pairs(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uint32_t nverts, const T *poses,
+                const T *vels, const uint32_t *pairs, uint64_t m, T max_toi, T target_distance, int stop,
+                int compute_geometry, T *out, uint32_t *status, uint32_t *iters, int nthreads) {
+    std::vector<V3<T>> pts(nverts);
+    for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
+    std::vector<Shape<T>> shapes = make_shapes(descs, nshapes, pts);
+    const CastOptions<T> o{max_toi, target_distance, stop != 0, compute_geometry != 0};
+    parallel_for(m, nthreads, 64, [&](uint64_t b, uint64_t e) {
+        for (uint64_t k = b; k < e; ++k) {
+            uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+            const T *a = poses + 7 * (size_t)i, *c = poses + 7 * (size_t)j;
+            Iso<T> p1{Quat<T>{a[0], a[1], a[2], a[3]}, v3(a[4], a[5], a[6])};
+            Iso<T> p2{Quat<T>{c[0], c[1], c[2], c[3]}, v3(c[4], c[5], c[6])};
+            V3<T> v1 = v3(vels[3 * i], vels[3 * i + 1], vels[3 * i + 2]);
+            V3<T> v2 = v3(vels[3 * j], vels[3 * j + 1], vels[3 * j + 2]);
+            CastHit<T> h{};
+            uint32_t it = 0;
+            bool ok = cast_shapes(p1, v1, shapes[i], p2, v2, shapes[j], o, h, &it);
+            T *q = out + 13 * k;
+            if (ok) {
+                q[0] = h.toi;
+                q[1] = h.witness1.x; q[2] = h.witness1.y; q[3] = h.witness1.z;
+                q[4] = h.witness2.x; q[5] = h.witness2.y; q[6] = h.witness2.z;
+                q[7] = h.normal1.x; q[8] = h.normal1.y; q[9] = h.normal1.z;
+                q[10] = h.normal2.x; q[11] = h.normal2.y; q[12] = h.normal2.z;
+                status[k] = h.status;
+            } else {
+                for (int r = 0; r < 13; ++r) q[r] = T(0);
+                status[k] = CAST_NONE;
+            }
+            if (iters) iters[k] = it;
+        }
+    });
+}
+}  // namespace
+
+extern "C" {
+// query::cast_shapes for every (i, j) in `pairs` with per-shape linear
+// velocities (n x 3) and one ShapeCastOptions.  out: 13 per pair (toi,
+// witness1, witness2, normal1, normal2; local frames); status: 0 None, else
+// ShapeCastStatus + 1; iters (nullable): minkowski_ray_cast loop passes.
+void orc_cast_shapes_f32(const void *descs, uint32_t nshapes, const float *verts, uint32_t nverts, const float *poses,
+                         const float *vels, const uint32_t *pairs, uint64_t m, float max_toi, float target_distance,
+                         int stop, int compute_geometry, float *out, uint32_t *status, uint32_t *iters, int nthreads) {
+    cast_pairs<float>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, vels, pairs, m, max_toi,
+                      target_distance, stop, compute_geometry, out, status, iters, nthreads);
+}
+void orc_cast_shapes_f64(const void *descs, uint32_t nshapes, const double *verts, uint32_t nverts,
+                         const double *poses, const double *vels, const uint32_t *pairs, uint64_t m, double max_toi,
+                         double target_distance, int stop, int compute_geometry, double *out, uint32_t *status,
+                         uint32_t *iters, int nthreads) {
+    cast_pairs<double>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, vels, pairs, m, max_toi,
+                       target_distance, stop, compute_geometry, out, status, iters, nthreads);
 }
 }  // extern "C"

@@ -16,6 +16,11 @@ from .query import (
     contact_workspace_bytes,
     MeshContact,
     PQ_CONTACT_TRUNCATED,
+    PQ_CAST_CONVERGED,
+    PQ_CAST_NONE,
+    PQ_CAST_PENETRATING,
+    cast_shapes_pairs,
+    cast_shapes_workspace_bytes,
 )
 from .shape import (
     TriMesh,
@@ -60,4 +65,9 @@ __all__ = [
     "contact_workspace_bytes",
     "MeshContact",
     "PQ_CONTACT_TRUNCATED",
+    "PQ_CAST_CONVERGED",
+    "PQ_CAST_NONE",
+    "PQ_CAST_PENETRATING",
+    "cast_shapes_pairs",
+    "cast_shapes_workspace_bytes",
 ]

@@ -73,6 +73,8 @@ SIGNATURES = {
     "pq_bvh_update_and_refit": (_int, [_vp, _vp, _vp]),
     "pq_bvh_self_pairs": (_int, [_vp, _vp, _u64, _vp, _vp]),
     "pq_contact_workspace_bytes": (_c.c_size_t, [_u64]),
+    "pq_cast_shapes_workspace_bytes": (_c.c_size_t, [_u64]),
+    "pq_cast_shapes_pairs": (_int, [_vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     "pq_contact_set_tuning": (_int, [_vp]),
     "pq_contact_get_tuning": (_int, [_vp]),
     "pq_contact_pairs": (_int, [_vp, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp]),
@@ -117,6 +119,13 @@ class ContactTuning(ctypes.Structure):
     """pq_contact_tuning (include/parry_mi355x.h)."""
 
     _fields_ = [("epa_lanes_per_pair", _u32), ("epa_polytope", _u32), ("reserved", _u32 * 2)]
+
+
+class ShapeCastOptions(ctypes.Structure):
+    """pq_shape_cast_options (include/parry_mi355x.h) = parry's ShapeCastOptions."""
+
+    _fields_ = [("max_time_of_impact", _f32), ("target_distance", _f32), ("stop_at_penetration", _u32),
+                ("compute_impact_geometry_on_penetration", _u32)]
 
 
 def set_contact_tuning(epa_lanes_per_pair: int, epa_polytope: int = 1) -> None:

@@ -47,7 +47,11 @@ enum : uint32_t {
     kPairSupportMaps = 0,  // contact_support_map_support_map on (shape 1, shape 2)
     kBallSecond = 1,       // (ConvexPolyhedron, Ball): project the ball centre
     kBallFirst = 2,        // (Ball, ConvexPolyhedron): the same, flipped
+    kPairSupportMapsLocal = 3,  // contact_support_map_support_map on any pair, Ball included,
+                                // results left in the shapes' local frames (the shape cast's
+                                // impact geometry, shape_cast_support_map_support_map.rs:42-46)
 };
+__device__ __forceinline__ bool is_ball_mode(uint32_t mode) { return mode == kBallFirst || mode == kBallSecond; }
 struct EpaItem {
     uint32_t pair;
     uint32_t dim;
@@ -76,11 +80,15 @@ __device__ __forceinline__ DevContact no_contact(uint32_t has) {
 }
 
 __device__ __forceinline__ void finish_contact(const iso &pos1, const iso &pos2, const iso &pos12, f3 p1, f3 p2_1,
-                                               f3 n1, DevContact *o) {
+                                               f3 n1, bool local, DevContact *o) {
     // contact_support_map_support_map (contact_support_map_support_map.rs:18-26)
     const float dist = dot3(sub3(p2_1, p1), n1);
     const f3 p2 = qrot_inv(pos12.q, sub3(p2_1, pos12.t));
     const f3 n2 = qrot_inv(pos12.q, neg3(n1));
+    if (local) {
+        store_contact(o, p1, p2, n1, n2, dist);
+        return;
+    }
     // Contact::transform_by_mut
     store_contact(o, iso_point(pos1, p1), iso_point(pos2, p2), qrot(pos1.q, n1), qrot(pos2.q, n2), dist);
 }
@@ -120,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
     const ShapeRecord *__restrict__ rec, const float4 *__restrict__ verts, const float *__restrict__ poses,
     const uint32_t *__restrict__ pairs, uint64_t m, const unsigned long long *__restrict__ m_dev, float prediction,
     const float *__restrict__ seeds, DevContact *__restrict__ out, uint32_t *__restrict__ iters,
-    EpaItem *__restrict__ queue, uint32_t *__restrict__ counters) {
+    EpaItem *__restrict__ queue, uint32_t *__restrict__ counters, uint32_t flags) {
     const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (m_dev) {
         const uint64_t md = *m_dev;
@@ -150,7 +158,8 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
         iso pb{};
         ShapeRef cv{};
         float radius = 0.0f;
-        if (g1.type == PQ_SHAPE_BALL && g2.type == PQ_SHAPE_BALL) {
+        const bool local = (flags & kContactLocalSupportMaps) != 0u;
+        if (!local && g1.type == PQ_SHAPE_BALL && g2.type == PQ_SHAPE_BALL) {
             // contact_ball_ball
             const float r1 = g1.p0, r2 = g2.p0;
             const f3 c21 = pos12.t;
@@ -166,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
             } else {
                 *c = no_contact(0);
             }
-        } else if (g1.type == PQ_SHAPE_BALL || g2.type == PQ_SHAPE_BALL) {
+        } else if (!local && (g1.type == PQ_SHAPE_BALL || g2.type == PQ_SHAPE_BALL)) {
             // contact_ball_convex_polyhedron / contact_convex_polyhedron_ball
             // (default_query_dispatcher.rs:326-335): a Ball first is the flipped
             // contact of (shape 2, ball) under pos12^-1.
@@ -212,6 +221,7 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
             }
         } else {
             run_gjk = true;
+            ball_mode = local ? kPairSupportMapsLocal : kPairSupportMaps;
             ga = g1;
             gb = g2;
             seed_pos = gjk_pos = pos12;
@@ -235,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
                 if (!(seeded && attempt == 0 && r.iters == kGjkMaxIters)) break;
                 dir = dflt;
             }
-            if (ball_mode != kPairSupportMaps) {
+            if (is_ball_mode(ball_mode)) {
                 if (r.kind == GJK_CLOSEST) {
                     finish_ball_contact(pos1, pos2, pb, radius, prediction, PointProj{false, r.p1}, false,
                                         f3{0.0f, 0.0f, 0.0f}, ball_mode, c);
@@ -245,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
                     *c = no_contact(kUnsupportedPair);  // parry: unreachable!() in gjk::project_origin
                 }
             } else if (r.kind == GJK_CLOSEST) {
-                finish_contact(pos1, pos2, pos12, r.p1, r.p2, r.n, c);
+                finish_contact(pos1, pos2, pos12, r.p1, r.p2, r.n, ball_mode == kPairSupportMapsLocal, c);
             } else if (r.kind == GJK_NO_INTERSECTION) {
                 *c = no_contact(0);
             } else {
@@ -1075,7 +1085,7 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
         }
         out[k] = no_contact(kUnsupportedPair);  // cannot happen with the large buffers
     };
-    if (mode != kPairSupportMaps) {
+    if (is_ball_mode(mode)) {
         // EPA::project_origin of the ball centre on the ConvexPolyhedron
         // (point_support_map.rs:39-49), then the Ball-vs-convex contact.
         iso pb = pos12;
@@ -1105,7 +1115,7 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
         return;
     }
     if (st == EPA_OK) {
-        finish_contact(pos1, pos2, pos12, p1, p2, n, out + k);
+        finish_contact(pos1, pos2, pos12, p1, p2, n, mode == kPairSupportMapsLocal, out + k);
     } else {
         out[k] = no_contact(0);  // EPA failed: NoIntersection (contact_support_map_support_map.rs:75-76)
     }
@@ -1297,14 +1307,14 @@ hipError_t launch_small(const DeviceShapes &s, const float *poses, const uint32_
 
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
                            const uint64_t *m_dev, float prediction, const float *seeds, pq_contact *out,
-                           uint32_t *iters, void *workspace, hipStream_t stream) {
+                           uint32_t *iters, void *workspace, hipStream_t stream, uint32_t flags) {
     if (m == 0) return hipSuccess;
     const ContactWorkspace w = carve_workspace(workspace, m);
     DevContact *o = (DevContact *)out;
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, w.counters);
     hipLaunchKernelGGL(contact_gjk_kernel, dim3((uint32_t)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
                        s.rec, s.verts, poses, pairs, m, (const unsigned long long *)m_dev, prediction, seeds, o, iters,
-                       w.queue, w.counters);
+                       w.queue, w.counters, flags);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     uint32_t *const c = w.counters;

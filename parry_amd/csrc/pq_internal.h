@@ -167,9 +167,18 @@ hipError_t launch_mesh_contacts(const DeviceMesh &m, const DeviceShapes &combine
                                 hipStream_t stream);
 // m_dev (nullable, device): the number of pairs actually present (<= m), e.g.
 // the count written by pq_bvh_self_pairs, so a frame needs no host round trip.
+// flags: kContactLocalSupportMaps runs contact_support_map_support_map on every
+// pair (Ball included, no closed forms) and leaves the results in the shapes'
+// local frames, as the shape cast's impact geometry needs.
+constexpr uint32_t kContactLocalSupportMaps = 1u;
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
                            const uint64_t *m_dev, float prediction, const float *seeds, pq_contact *out,
-                           uint32_t *iters, void *workspace, hipStream_t stream);
+                           uint32_t *iters, void *workspace, hipStream_t stream, uint32_t flags = 0u);
+// Linear shape casts (cast.hip).
+size_t cast_workspace_bytes(uint64_t m);
+hipError_t launch_casts(const DeviceShapes &s, const float *poses, const float *vels, const uint32_t *pairs, uint64_t m,
+                        const pq_shape_cast_options &o, pq_shape_cast_hit *out, uint32_t *iters, void *workspace,
+                        hipStream_t stream);
 // Lanes that run one pair's EPA together (64, 16 or 8), set by pq_contact_set_tuning.
 extern int g_epa_lanes_per_pair;
 extern int g_epa_polytope;

@@ -183,3 +183,54 @@ class MeshContact:
             self._h, mesh_pose.data_ptr(), poses.data_ptr(), float(prediction), int(max_candidates), out.data_ptr(),
             triangles.data_ptr(), needed.data_ptr(), workspace.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
         return out, triangles, needed
+
+
+# ShapeCastStatus (shape_cast.rs:12-33) numbered from 1; PQ_CAST_NONE is None.
+PQ_CAST_NONE, PQ_CAST_OUT_OF_ITERATIONS, PQ_CAST_CONVERGED, PQ_CAST_FAILED, PQ_CAST_PENETRATING = range(5)
+F32_MAX = 3.4028234663852886e38
+
+
+def cast_shapes_workspace_bytes(num_pairs: int) -> int:
+    from . import _native
+
+    return int(_native.lib().pq_cast_shapes_workspace_bytes(int(num_pairs)))
+
+
+def cast_shapes_pairs(shapes, poses, vels, pairs, max_time_of_impact: float = F32_MAX, target_distance: float = 0.0,
+                      stop_at_penetration: bool = True, compute_impact_geometry_on_penetration: bool = True,
+                      with_iters: bool = False, out=None, workspace=None, stream=None):
+    """Batched ``query::cast_shapes`` (shape_cast.rs:274-286) on CUDA tensors, with
+    ``ShapeCastOptions``' fields as keyword arguments (parry's defaults).
+
+    ``shapes``: a ``ShapeSet``; ``poses``: (n, 7) f32; ``vels``: (n, 3) f32 linear
+    velocities; ``pairs``: (m, 2) int32/uint32.  Returns a (m, 14) f32 tensor of
+    ``pq_shape_cast_hit`` records (toi, witness1, witness2, normal1, normal2 in the
+    shapes' local frames; view the status with ``.view(torch.int32)[:, 13]``) and,
+    if ``with_iters``, the (m,) int32 GJK ray-cast iterations.
+    """
+    import ctypes
+
+    import torch
+
+    from . import _native
+
+    m = pairs.shape[0]
+    dev = poses.device
+    for t in (poses, vels, pairs):
+        if not (t.is_cuda and t.is_contiguous()):
+            raise ValueError("cast_shapes_pairs: tensors must be contiguous CUDA tensors")
+    if poses.dtype != torch.float32 or vels.dtype != torch.float32:
+        raise ValueError("cast_shapes_pairs: poses and vels must be float32")
+    if tuple(poses.shape) != (len(shapes), 7) or tuple(vels.shape) != (len(shapes), 3):
+        raise ValueError(f"cast_shapes_pairs: poses must be ({len(shapes)}, 7) and vels ({len(shapes)}, 3)")
+    if out is None:
+        out = torch.empty((m, 14), dtype=torch.float32, device=dev)
+    iters = torch.empty(m, dtype=torch.int32, device=dev) if with_iters else None
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    opts = _native.ShapeCastOptions(float(max_time_of_impact), float(target_distance), int(bool(stop_at_penetration)),
+                                    int(bool(compute_impact_geometry_on_penetration)))
+    _native.check(_native.lib().pq_cast_shapes_pairs(
+        shapes.handle, poses.data_ptr(), vels.data_ptr(), pairs.data_ptr(), m, ctypes.byref(opts), out.data_ptr(),
+        None if iters is None else iters.data_ptr(), None if workspace is None else workspace.data_ptr(),
+        ctypes.c_void_p(s.cuda_stream)))
+    return (out, iters) if with_iters else out
