@@ -376,6 +376,81 @@ def mesh_contact_leg(args, log):
     return rec
 
 
+def cast_leg(args, log):
+    """query::cast_shapes (shape_cast.rs:274-286) with parry's default options on
+    1,048,576 pairs of mixed shapes in approach (scenes.approach_pairs, seed 7: Ball,
+    Cuboid, Capsule, Cone, ConvexPolyhedron of 8-64 points and Triangle; the second
+    shape 2-6 units away, closing at 0.5-3 units/s).  One step = one
+    pq_cast_shapes_pairs call: the GJK ray cast of every pair, then the impact
+    geometry (GJK/EPA contact) of the pairs that start within 1e-5 of touching.
+    Algorithmic bytes per pair: 8 (pair) + 2 x 28 (poses) + 2 x 12 (velocities) +
+    2 x 32 (shape records) + 12 per vertex of a ConvexPolyhedron or Triangle +
+    56 (hit record)."""
+    import numpy as np
+    import torch
+
+    from parry_amd import ShapeSet, cast_shapes_pairs, cast_shapes_workspace_bytes, scenes
+
+    n = 1 << 20
+    descs, verts, poses_np, vels_np, pairs_np = scenes.approach_pairs(n, 7)
+    shapes = ShapeSet(descs, verts)
+    poses = torch.from_numpy(poses_np).cuda()
+    vels = torch.from_numpy(vels_np).cuda()
+    pairs = torch.from_numpy(pairs_np.view(np.int32)).cuda()
+    out = torch.empty((n, 14), dtype=torch.float32, device="cuda")
+    ws = torch.empty(cast_shapes_workspace_bytes(n), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        cast_shapes_pairs(shapes, poses, vels, pairs, out=out, workspace=ws, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        step()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    gpu_ms = ev[0].elapsed_time(ev[1]) / args.steps
+    o = out.cpu().numpy()
+    st = o.view(np.uint32)[:, 13]
+    nv = np.where(np.isin(descs["type"], (4, 5)), descs["num_vertices"], 0).astype(np.int64)
+    alg = int((8 + 56 + 24 + 64 + 56) * n + 12 * (nv[pairs_np[:, 0]] + nv[pairs_np[:, 1]]).sum())
+    gbs = alg / (gpu_ms * 1e-3) / 1e9
+    rec = {
+        "metric": "shape casts/s",
+        "value": round(n / wall, 1),
+        "unit": "casts/s",
+        "ms_per_step": round(wall * 1e3, 4),
+        "gpu_ms_per_step": round(gpu_ms, 4),
+        "workload": "query::cast_shapes, default ShapeCastOptions: 1048576 mixed pairs in approach "
+                    "(scenes.approach_pairs seed 7, Triangles included)",
+        "hits": int((st != 0).sum()),
+        "penetrating": int((st == 4).sum()),
+        "roofline": {"bound": "latency (GJK ray cast, branchy scalar code)", "kernel": "cast_kernel + contact",
+                     "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes_per_launch": alg},
+    }
+    if not args.no_cpu:
+        from oracle import oracle
+
+        cores = oracle.default_threads()
+        t0 = time.perf_counter()
+        ref = oracle.cast_shapes(descs, verts, poses_np, vels_np, pairs_np, threads=cores)
+        t_cpu = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(n / t_cpu, 1), "unit": "casts/s", "cores": cores, "kind": "port",
+                               "sample": f"all {n} pairs, one run on {cores} threads (C++ restatement of parry3d f32)"}
+        h = ref["status"] != 0
+        rec["parity"] = {"status_mismatches": int((st != ref["status"]).sum()),
+                         "bit_identical_hits": int(np.all(o[h, :13] == ref["out"][h], axis=1).sum()),
+                         "hits_oracle": int(h.sum())}
+    return rec
+
+
 def learned_leg(args, log):
     """Learned GJK warm start vs the plain GJK on 1,048,576 mixed convex pairs
     (ConvexPolyhedron 8-64 points, Cuboid, Ball, Capsule, Cone; the overlapping
@@ -658,6 +733,8 @@ def main() -> int:
     ap.add_argument("--skip-queries", action="store_true", help="skip the point-projection and culling leg")
     ap.add_argument("--queries-only", action="store_true", help="profiling aid: run only the point/culling leg")
     ap.add_argument("--skip-mesh-contact", action="store_true", help="skip the TriMesh-shape contact leg")
+    ap.add_argument("--skip-cast", action="store_true", help="skip the shape-cast leg")
+    ap.add_argument("--cast-only", action="store_true", help="profiling aid: run only the shape-cast leg")
     ap.add_argument("--mesh-contact-only", action="store_true", help="profiling aid: run only the mesh contact leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="profiling aid: only copy BYTES of device memory 4 times (counter calibration)")
@@ -702,6 +779,9 @@ def main() -> int:
         return 0
     if args.queries_only:
         print(json.dumps({"queries": queries_leg(args, log)}), flush=True)
+        return 0
+    if args.cast_only:
+        print(json.dumps({"cast": cast_leg(args, log)}), flush=True)
         return 0
     if args.mesh_contact_only:
         print(json.dumps({"mesh_contact": mesh_contact_leg(args, log)}), flush=True)
@@ -788,6 +868,9 @@ def main() -> int:
     if not args.skip_queries:
         log("point projection and culling leg ...")
         rec["queries"] = queries_leg(args, log)
+    if not args.skip_cast:
+        log("shape-cast leg ...")
+        rec["cast"] = cast_leg(args, log)
     if not args.skip_mesh_contact:
         log("TriMesh-shape contact leg ...")
         rec["mesh_contact"] = mesh_contact_leg(args, log)

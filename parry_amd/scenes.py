@@ -366,3 +366,37 @@ def shapes_around_mesh(n: int, seed: int, mesh_radius: float = 2.0, scale: float
     t = _unit_vectors(rng, n) * rng.uniform(shell[0], shell[1], (n, 1)) * (2.0 * mesh_radius)
     descs["params"] *= scale
     return descs, (verts * scale).astype(np.float32), np.concatenate([q, t], 1).astype(np.float32)
+
+
+def approach_pairs(n: int, seed: int, spread=(2.0, 6.0), scale: float = 1.0, with_triangles: bool = True):
+    """``n`` shape-cast pairs (2q, 2q+1) of ``mixed_shapes(2n, seed)``, every fifth
+    shape replaced by a Triangle (vertices N(0, 0.7 scale)) when ``with_triangles``:
+    shape 2q at a random rotation and a position uniform in [-50, 50]^3, shape 2q+1
+    at a random rotation and U[spread] away along a uniform direction, moving back
+    along it (with N(0, 0.3) noise) at U[0.5, 3] units per second; every shape
+    also drifts at U[-0.3, 0.3]^3.
+
+    Returns (descs, vertices float32, poses float32 [2n, 7], vels float32 [2n, 3],
+    pairs uint32 [n, 2]).
+    """
+    rng = np.random.default_rng(seed)
+    descs, verts = mixed_shapes(2 * n, seed)
+    descs["params"] *= scale
+    verts = verts * scale
+    if with_triangles:  # every fifth shape becomes a Triangle
+        t = np.arange(0, 2 * n, 5)
+        tv = rng.normal(0.0, 0.7 * scale, (len(t), 3, 3)).astype(np.float32)
+        descs["type"][t] = 5
+        descs["first_vertex"][t] = verts.shape[0] + 3 * np.arange(len(t))
+        descs["num_vertices"][t] = 3
+        verts = np.concatenate([verts, tv.reshape(-1, 3)], 0)
+    poses = np.concatenate([random_rotations(2 * n, rng), np.zeros((2 * n, 3))], 1)
+    dirs = rng.normal(0, 1, (n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    poses[1::2, 4:] = dirs * rng.uniform(spread[0], spread[1], (n, 1))
+    poses[0::2, 4:] = rng.uniform(-50, 50, (n, 3))
+    poses[1::2, 4:] += poses[0::2, 4:]
+    vels = rng.uniform(-0.3, 0.3, (2 * n, 3))
+    vels[1::2] += (-dirs + rng.normal(0, 0.3, (n, 3))) * rng.uniform(0.5, 3.0, (n, 1))
+    pairs = np.stack([np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)], 1).astype(np.uint32)
+    return descs, verts.astype(np.float32), poses.astype(np.float32), vels.astype(np.float32), pairs

@@ -19,32 +19,6 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def _scene(n, seed, spread=(2.0, 6.0), scale=1.0, with_triangles=True):
-    """n pairs (2q, 2q+1) of mixed shapes; shape 2q+1 starts `spread` away and moves
-    roughly towards shape 2q."""
-    rng = np.random.default_rng(seed)
-    descs, verts = scenes.mixed_shapes(2 * n, seed)
-    descs["params"] *= scale
-    verts = verts * scale
-    if with_triangles:  # every fifth shape becomes a Triangle
-        t = np.arange(0, 2 * n, 5)
-        tv = rng.normal(0.0, 0.7 * scale, (len(t), 3, 3)).astype(np.float32)
-        descs["type"][t] = 5
-        descs["first_vertex"][t] = verts.shape[0] + 3 * np.arange(len(t))
-        descs["num_vertices"][t] = 3
-        verts = np.concatenate([verts, tv.reshape(-1, 3)], 0)
-    poses = np.concatenate([scenes.random_rotations(2 * n, rng), np.zeros((2 * n, 3))], 1)
-    dirs = rng.normal(0, 1, (n, 3))
-    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
-    poses[1::2, 4:] = dirs * rng.uniform(spread[0], spread[1], (n, 1))
-    poses[0::2, 4:] = rng.uniform(-50, 50, (n, 3))
-    poses[1::2, 4:] += poses[0::2, 4:]
-    vels = rng.uniform(-0.3, 0.3, (2 * n, 3))
-    vels[1::2] += (-dirs + rng.normal(0, 0.3, (n, 3))) * rng.uniform(0.5, 3.0, (n, 1))
-    pairs = np.stack([np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)], 1).astype(np.uint32)
-    return descs, verts.astype(np.float32), poses.astype(np.float32), vels.astype(np.float32), pairs
-
-
 def _run(shapes, poses, vels, pairs, **opts):
     import torch
 
@@ -79,7 +53,7 @@ def _opts_kw(max_toi=None, **kw):
 
 
 def test_mixed_casts_match_oracle(orc):
-    descs, verts, poses, vels, pairs = _scene(100000, 7)
+    descs, verts, poses, vels, pairs = scenes.approach_pairs(100000, 7)
     shapes = ShapeSet(descs, verts)
     cases = [
         ("defaults", {}),
@@ -90,17 +64,14 @@ def test_mixed_casts_match_oracle(orc):
     ]
     for label, kw in cases:
         print(f"case: casts, 100000 mixed pairs seed 7, {label}", flush=True)
-        ref_kw = dict(kw)
-        if "max_toi" in ref_kw:
-            ref_kw["max_toi"] = ref_kw.pop("max_toi")
-        ref = orc.cast_shapes(descs, verts, poses, vels, pairs, **ref_kw)
+        ref = orc.cast_shapes(descs, verts, poses, vels, pairs, **kw)
         h = _compare(label, _run(shapes, poses, vels, pairs, **_opts_kw(**kw)), ref)
         assert h.sum() > 10000
 
 
 def test_penetrating_pairs_match_oracle(orc):
     # shapes that start overlapping: toi 0 and the impact geometry from GJK/EPA
-    descs, verts, poses, vels, pairs = _scene(60000, 9, spread=(0.0, 1.2))
+    descs, verts, poses, vels, pairs = scenes.approach_pairs(60000, 9, spread=(0.0, 1.2))
     shapes = ShapeSet(descs, verts)
     for label, kw in (("defaults", {}), ("no stop at penetration", dict(stop_at_penetration=False)),
                       ("target 0.2, no stop", dict(target_distance=0.2, stop_at_penetration=False))):
@@ -131,33 +102,28 @@ def test_ball_ball_and_reference_cases(orc):
     idq = [0, 0, 0, 1]
     tri = np.array([[0.5, -0.5, 0.0], [-0.5, -0.5, 0.0], [-0.5, 0.5, 0.0]], np.float32)
     kd = np.zeros(6, SHAPE_DESC_DTYPE)
-    kd["type"] = [0, 1, 0, 0, 5, 1]
+    kd["type"] = [0, 1, 0, 0, 5, 0]
     kd["params"][0, 0] = 1.0
     kd["params"][1, :3] = 1.0
     kd["params"][2, 0] = 0.5
     kd["params"][3, 0] = 0.375
     kd["num_vertices"][4] = 3
-    kd["params"][5, :3] = 0.5
+    kd["params"][5, 0] = 0.5
     ks = ShapeSet(kd, tri)
     cases = [  # (pose1, vel1, shape1, pose2, vel2, shape2, expected toi or None)
         (idq + [1, 1, 1], [2, 2, 2], 0, idq + [0, 0, 0], [-1, 1, 1], 1, 0.0),
         (idq + [2, 2, 2], [-0.5, -0.5, -0.5], 0, idq + [0, 0, 0], [1, 1, 1], 1,
          (np.sqrt(np.float32(3.0)) - np.float32(1.0)) / np.linalg.norm(np.array([-1.5, -1.5, -1.5], np.float32))),
         (idq + [3, 3, 3], [2, 2, 2], 0, idq + [0, 0, 0], [-1, 1, 1], 1, None),
-        (idq + [0, 0, 0], [0, 10, 0], 2, idq + [0, 10, 0], [0, 0, 0], 2, np.float32(0.9)),
+        (idq + [0, 0, 0], [0, 10, 0], 2, idq + [0, 10, 0], [0, 0, 0], 5, np.float32(0.9)),
         (idq + [0, 0, 0], [0, 6.925e-42, 0], 3, idq + [11.5, 5.5, 0], [0, 0, 0], 4, None),
-        (idq + [0, 1.1, 0], [0, 0, 0], 5, idq + [0, 0, 0], [0, 0, 0], 5, None),
-        (idq + [0, 1.1, 0], [0, 1, 0], 5, idq + [0, 0, 0], [0, 0, 0], 5, None),
-        (idq + [0, 1.1, 0], [0, -1, 0], 5, idq + [0, 0, 0], [0, 0, 0], 5, 0.1),
     ]
     for c in cases:
-        # one shape set, poses and vels per case: shape i of the case at its pose
+        # one shape set; the case's two shapes at their poses and velocities
         p = np.zeros((6, 7), np.float32)
         p[:, 3] = 1.0
         v = np.zeros((6, 3), np.float32)
         i, j = c[2], c[5]
-        if i == j:  # two cuboids of the same record: use the pair (5, 1) shapes by value instead
-            continue
         p[i], v[i], p[j], v[j] = c[0], c[1], c[3], c[4]
         vals, st, _ = _run(ks, p, v, np.array([[i, j]], np.uint32))
         if c[6] is None:
@@ -165,7 +131,7 @@ def test_ball_ball_and_reference_cases(orc):
         else:
             assert st[0] != 0, c
             assert abs(vals[0, 0] - c[6]) <= 1e-6 * max(1.0, abs(c[6])), (c, vals[0, 0])
-    # the stacked cuboids need two cuboid records
+    # the stacked cuboids of still_objects_toi.rs
     sd = np.zeros(2, SHAPE_DESC_DTYPE)
     sd["type"] = 1
     sd["params"][:, :3] = 0.5
