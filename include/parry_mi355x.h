@@ -94,6 +94,19 @@ typedef struct pq_bvh_node_wide {
 int pq_bvh_from_leaves(const float *aabbs, uint32_t n, pq_bvh_node_wide *out, uint32_t capacity,
                        uint32_t *num_nodes, uint32_t *depth);
 
+/* Bvh::from_leaves(BvhBuildStrategy::Ploc, aabbs) (bvh_tree.rs:1891-1955,
+ * bvh_ploc_build.rs:10-93, utils/morton.rs:13-40) built on the GPU: Morton
+ * keys, a stable radix sort, the PLOC merge levels and refit's depth-first
+ * layout, identical bit for bit to parry's tree.
+ *   aabbs: device, n x {mins[3], maxs[3]}; out: device, `capacity` wide records
+ *   (n - 1 needed; 1 for n <= 2); workspace: device,
+ *   pq_bvh_ploc_workspace_bytes(n) bytes, or NULL for a library-held one.
+ * Synchronises `stream` (the host sizes each merge level); *num_nodes and
+ * *depth (nullable) are set on return. */
+size_t pq_bvh_ploc_workspace_bytes(uint32_t n);
+int pq_bvh_from_leaves_ploc(const float *aabbs, uint32_t n, pq_bvh_node_wide *out, uint32_t capacity,
+                            uint32_t *num_nodes, uint32_t *depth, void *workspace, void *stream);
+
 /* ---- TriMesh ray casts --------------------------------------------------- */
 
 typedef struct pq_trimesh pq_trimesh;
@@ -272,6 +285,10 @@ typedef struct pq_bvh pq_bvh;
 /* Bvh::from_leaves(BvhBuildStrategy::Binned, aabbs) (bvh_tree.rs:1835-1837) built
  * on the host from `aabbs` (host, n x {mins[3], maxs[3]}) and uploaded. */
 int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out);
+/* Bvh::from_leaves(BvhBuildStrategy::Ploc, aabbs) (bvh_tree.rs:1891-1955 with
+ * rebuild_range_ploc, bvh_ploc_build.rs:10-93, and refit's depth-first layout)
+ * built on the GPU, from `aabbs` in device memory, into a broadphase handle. */
+int pq_bvh_create_ploc(const float *aabbs, uint32_t n, pq_bvh **out, void *stream);
 int pq_bvh_destroy(pq_bvh *bvh);
 int pq_bvh_info(const pq_bvh *bvh, uint32_t *num_leaves, uint32_t *num_nodes, uint32_t *depth);
 /* Copies the device tree to host memory (synchronises the default stream). */

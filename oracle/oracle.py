@@ -41,6 +41,9 @@ _SIG = {
     "orc_triangle_project_point_f32": (_int, [_vp, _vp, _int, _vp]),
     "orc_trimesh_contact_shapes_f32": (None, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _f32, _vp, _vp, _vp, _vp, _int]),
     "orc_trimesh_contact_shapes_f64": (None, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _f64, _vp, _vp, _vp, _vp, _int]),
+    "orc_bvh_create_ploc_f32": (_vp, [_vp, _u32]),
+    "orc_ploc_keys_f32": (None, [_vp, _u32, _vp]),
+    "orc_morton_encode_unorm": (_u64, [_f64, _f64, _f64]),
     "orc_cast_shapes_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, _u64, _f32, _f32, _int, _int, _vp, _vp, _vp,
                                    _int]),
     "orc_cast_shapes_f64": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, _u64, _f64, _f64, _int, _int, _vp, _vp, _vp,
@@ -261,12 +264,14 @@ def polyhedra_aabbs(vertices, offsets, poses, f64: bool = False, threads: int | 
 
 
 class OracleBvh:
-    """parry3d Bvh (f32) restated: from_leaves(Binned), update+refit, self-BVTT."""
+    """parry3d Bvh (f32) restated: from_leaves (Binned, or Ploc with ploc=True),
+    update+refit, self-BVTT."""
 
-    def __init__(self, aabbs):
+    def __init__(self, aabbs, ploc: bool = False):
         a = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
         self.n = a.shape[0]
-        self.h = lib().orc_bvh_create_f32(a.ctypes.data, self.n)
+        create = lib().orc_bvh_create_ploc_f32 if ploc else lib().orc_bvh_create_f32
+        self.h = create(a.ctypes.data, self.n)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -395,3 +400,16 @@ def cast_shapes(descs, vertices, poses, vels, pairs, max_toi: float = float("inf
       pr.ctypes.data, m, mt, target_distance, int(stop_at_penetration), int(compute_impact_geometry_on_penetration),
       out.ctypes.data, status.ctypes.data, iters.ctypes.data, default_threads() if threads is None else threads)
     return {"status": status, "out": out, "iters": iters}
+
+
+def ploc_keys(aabbs) -> np.ndarray:
+    """The Morton keys the PLOC build sorts the leaves by (bvh_ploc_build.rs:12-21)."""
+    a = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
+    out = np.zeros(a.shape[0], np.uint64)
+    lib().orc_ploc_keys_f32(a.ctypes.data, a.shape[0], out.ctypes.data)
+    return out
+
+
+def morton_encode_unorm(x: float, y: float, z: float) -> int:
+    """morton_encode_u64_unorm (morton.rs:33-40)."""
+    return int(lib().orc_morton_encode_unorm(float(x), float(y), float(z)))

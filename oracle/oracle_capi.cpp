@@ -281,6 +281,24 @@ void *orc_bvh_create_f32(const float *aabbs, uint32_t n) {
     h->n = n;
     return h;
 }
+// Bvh::from_leaves(Ploc, aabbs) (bvh_tree.rs:1891-1955, bvh_ploc_build.rs:10-93).
+void *orc_bvh_create_ploc_f32(const float *aabbs, uint32_t n) {
+    auto *h = new OrcBvh();
+    std::vector<Aabb<float>> leaves(n);
+    for (uint32_t i = 0; i < n; ++i) leaves[i] = aabb_at(aabbs, i);
+    h->bvh = bvh_from_leaves_ploc(leaves);
+    h->slots = leaf_slots(h->bvh, n);
+    h->n = n;
+    return h;
+}
+// The Morton keys of the PLOC build (bvh_ploc_build.rs:12-21), in leaf order.
+void orc_ploc_keys_f32(const float *aabbs, uint32_t n, uint64_t *out) {
+    std::vector<Node<float>> leaves;
+    for (uint32_t i = 0; i < n; ++i) leaves.push_back(Node<float>::leaf(aabb_at(aabbs, i), i));
+    std::vector<uint64_t> k = ploc_keys(leaves);
+    for (uint32_t i = 0; i < n; ++i) out[i] = k[i];
+}
+uint64_t orc_morton_encode_unorm(double x, double y, double z) { return morton_encode_unorm(x, y, z); }
 void orc_bvh_free_f32(void *h) { delete (OrcBvh *)h; }
 uint32_t orc_bvh_num_nodes_f32(void *h) { return (uint32_t)((OrcBvh *)h)->bvh.nodes.size(); }
 void orc_bvh_nodes_f32(void *hp, void *out) {

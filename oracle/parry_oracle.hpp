@@ -357,6 +357,133 @@ template <class T> Bvh<T> bvh_from_leaves(const std::vector<Aabb<T>> &aabbs) {
     return bvh;
 }
 
+// ---- PLOC build (bvh_ploc_build.rs:10-93, utils/morton.rs:13-40) --------------
+
+// split_by_3_u64 and morton_encode_u64 (morton.rs:13-31): 21 bits per axis.
+inline uint64_t morton_split_by_3(uint32_t a) {
+    uint64_t x = (uint64_t)a & 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+// Rust's `f64 as u32`: truncation, saturating, NaN -> 0.
+inline uint32_t rust_as_u32(double f) {
+    if (!(f > 0.0)) return 0u;
+    if (f >= 4294967296.0) return UINT32_MAX;
+    return (uint32_t)f;
+}
+// morton_encode_u64_unorm (morton.rs:33-40): p * 2^21, each axis `as u32`.
+inline uint64_t morton_encode_unorm(double x, double y, double z) {
+    const double s = (double)(1u << 21);
+    return morton_split_by_3(rust_as_u32(x * s)) | morton_split_by_3(rust_as_u32(y * s)) << 1 |
+           morton_split_by_3(rust_as_u32(z * s)) << 2;
+}
+
+// Aabb::half_area (aabb.rs:473-476).
+template <class T> inline T half_area(const Aabb<T> &a) {
+    V3<T> e = a.maxs - a.mins;
+    return e.x * (e.y + e.z) + e.y * e.z;
+}
+
+// The Morton key of each leaf (bvh_ploc_build.rs:12-21): its centre relative to
+// the centres' AABB, times the inverse extents (in T), widened to f64.
+template <class T> std::vector<uint64_t> ploc_keys(const std::vector<Node<T>> &leaves) {
+    V3<T> c0 = leaves[0].center();
+    Aabb<T> cent{c0, c0};
+    for (size_t i = 1; i < leaves.size(); ++i) {
+        V3<T> c = leaves[i].center();
+        cent.mins = vmin(cent.mins, c);
+        cent.maxs = vmax(cent.maxs, c);
+    }
+    V3<T> e = cent.extents();
+    V3<T> inv = v3(T(1) / e.x, T(1) / e.y, T(1) / e.z);
+    std::vector<uint64_t> keys(leaves.size());
+    for (size_t i = 0; i < leaves.size(); ++i) {
+        V3<T> c = leaves[i].center() - cent.mins;
+        c = v3(c.x * inv.x, c.y * inv.y, c.z * inv.z);
+        keys[i] = morton_encode_unorm((double)c.x, (double)c.y, (double)c.z);
+    }
+    return keys;
+}
+
+// rebuild_range_ploc (bvh_ploc_build.rs:10-93): leaves stably sorted by Morton
+// key (sort_by_cached_key), then levels of merges: every node picks, within 16
+// places either side, the partner with the smallest half area of the merged box
+// (the first on ties); mutual picks merge (in order of the lower index, which
+// also takes the merged node's place), the others carry over.
+template <class T> void rebuild_ploc(std::vector<Wide<T>> &nodes, uint32_t target, std::vector<Node<T>> leaves) {
+    {
+        std::vector<uint64_t> keys = ploc_keys(leaves);
+        std::vector<size_t> order(leaves.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return keys[a] < keys[b]; });
+        std::vector<Node<T>> sorted(leaves.size());
+        for (size_t i = 0; i < order.size(); ++i) sorted[i] = leaves[order[i]];
+        leaves.swap(sorted);
+    }
+    const size_t R = 16;
+    std::vector<size_t> cand(leaves.size(), SIZE_MAX);
+    std::vector<Node<T>> next;
+    next.reserve(leaves.size());
+    while (leaves.size() > 1) {
+        const size_t L = leaves.size();
+        for (size_t i = 0; i < L; ++i) {
+            T best_sah = Consts<T>::max_value;
+            size_t best = SIZE_MAX;
+            const size_t lo = i >= R ? i - R : 0, hi = std::min(i + R, L - 1);
+            for (size_t k = lo; k <= hi; ++k) {
+                if (k == i) continue;
+                Aabb<T> m{vmin(leaves[i].mins, leaves[k].mins), vmax(leaves[i].maxs, leaves[k].maxs)};
+                T sah = half_area(m);
+                if (sah < best_sah) {
+                    best_sah = sah;
+                    best = k;
+                }
+            }
+            cand[i] = best;
+        }
+        for (size_t i = 0; i < L; ++i) {
+            const size_t k = cand[i];
+            if (cand[k] == i) {
+                if (i > k) continue;
+                Wide<T> w{leaves[i], leaves[k]};
+                if (L == 2) {
+                    nodes[target] = w;
+                } else {
+                    const uint32_t id = (uint32_t)nodes.size();
+                    nodes.push_back(w);
+                    next.push_back(w.merged(id));
+                }
+            } else {
+                next.push_back(leaves[i]);
+            }
+        }
+        leaves.swap(next);
+        next.clear();
+    }
+}
+
+// Bvh::from_iter with BvhBuildStrategy::Ploc (bvh_tree.rs:1891-1955).
+template <class T> Bvh<T> bvh_from_leaves_ploc(const std::vector<Aabb<T>> &aabbs) {
+    Bvh<T> bvh;
+    std::vector<Node<T>> leaves;
+    leaves.reserve(aabbs.size());
+    for (size_t i = 0; i < aabbs.size(); ++i) leaves.push_back(Node<T>::leaf(aabbs[i], (uint32_t)i));
+    if (leaves.empty()) return bvh;
+    if (leaves.size() <= 2) {
+        bvh.nodes.push_back(Wide<T>{leaves[0], leaves.size() == 2 ? leaves[1] : Node<T>::zeros()});
+        return bvh;
+    }
+    bvh.nodes.reserve(leaves.size());
+    bvh.nodes.push_back(Wide<T>{Node<T>::zeros(), Node<T>::zeros()});
+    rebuild_ploc(bvh.nodes, 0, std::move(leaves));
+    refit(bvh);
+    return bvh;
+}
+
 // Depth of the tree in wide-node levels (root = 1).
 template <class T> uint32_t bvh_depth(const Bvh<T> &bvh) {
     if (bvh.nodes.empty()) return 0;

@@ -39,7 +39,7 @@ from .shape import (
     SHAPE_CONVEX_POLYHEDRON,
     SHAPE_TRIANGLE,
 )
-from .partitioning import Bvh
+from .partitioning import Bvh, ploc_nodes
 from . import scenes
 
 __all__ = [
@@ -63,6 +63,7 @@ __all__ = [
     "CONTACT_DTYPE",
     "contact_pairs",
     "contact_workspace_bytes",
+    "ploc_nodes",
     "MeshContact",
     "PQ_CONTACT_TRUNCATED",
     "PQ_CAST_CONVERGED",

@@ -68,6 +68,9 @@ SIGNATURES = {
     "pq_shapes_compute_aabbs": (_int, [_vp, _vp, _vp, _vp]),
     "pq_bvh_create": (_int, [_vp, _u32, _c.POINTER(_vp)]),
     "pq_bvh_destroy": (_int, [_vp]),
+    "pq_bvh_create_ploc": (_int, [_vp, _u32, _c.POINTER(_vp), _vp]),
+    "pq_bvh_ploc_workspace_bytes": (_c.c_size_t, [_u32]),
+    "pq_bvh_from_leaves_ploc": (_int, [_vp, _u32, _vp, _u32, _c.POINTER(_u32), _c.POINTER(_u32), _vp, _vp]),
     "pq_bvh_info": (_int, [_vp, _c.POINTER(_u32), _c.POINTER(_u32), _c.POINTER(_u32)]),
     "pq_bvh_export": (_int, [_vp, _vp, _u32]),
     "pq_bvh_update_and_refit": (_int, [_vp, _vp, _vp]),

@@ -92,17 +92,12 @@ int pq_shapes_compute_aabbs(const pq_shapes *s, const pq_isometry *poses, float 
     return PQ_OK;
 }
 
-int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out) {
-    if (!out) return fail(PQ_ERR_INVALID, "pq_bvh_create: null output");
-    *out = nullptr;
-    if (n && !aabbs) return fail(PQ_ERR_INVALID, "pq_bvh_create: null aabbs");
-    std::vector<LeafBox> leaves(n);
-    if (n) std::memcpy(leaves.data(), aabbs, sizeof(LeafBox) * (size_t)n);
-    std::vector<pq_bvh_node_wide> nodes;
-    uint32_t depth = 0;
-    build_binned_bvh(leaves, nodes, &depth);
+namespace {
+// The broadphase handle of a depth-first tree (host copy `nodes`) over n leaves:
+// leaf and parent slots, and the refit schedule (non-root wide nodes by height).
+int bvh_handle_from_nodes(const std::vector<pq_bvh_node_wide> &nodes, uint32_t n, uint32_t depth, pq_bvh **out) {
     if (depth > kMaxBroadphaseDepth)
-        return fail(PQ_ERR_UNSUPPORTED, "pq_bvh_create: Bvh depth " + std::to_string(depth) + " exceeds " +
+        return fail(PQ_ERR_UNSUPPORTED, "Bvh depth " + std::to_string(depth) + " exceeds " +
                                             std::to_string(kMaxBroadphaseDepth));
     const uint32_t nn = (uint32_t)nodes.size();
     std::vector<uint32_t> leaf_slot(n, 0), parent_slot(nn ? nn : 1, UINT32_MAX);
@@ -173,7 +168,7 @@ int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out) {
             e = hipMemcpy(d.level_offsets, offs.data(), offs.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemset(d.pair_counts, 0, ((size_t)n + 1) * sizeof(uint32_t));
         if (e == hipSuccess) e = hipMemset(d.pair_offsets, 0, ((size_t)n + 1) * sizeof(unsigned long long));
-        if (e != hipSuccess) rc = hip_fail(e, "pq_bvh_create upload");
+        if (e != hipSuccess) rc = hip_fail(e, "Bvh upload");
     }
     if (rc != PQ_OK) {
         pq_bvh_destroy(b);
@@ -181,6 +176,62 @@ int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out) {
     }
     *out = b;
     return PQ_OK;
+}
+}  // namespace
+
+int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out) {
+    if (!out) return fail(PQ_ERR_INVALID, "pq_bvh_create: null output");
+    *out = nullptr;
+    if (n && !aabbs) return fail(PQ_ERR_INVALID, "pq_bvh_create: null aabbs");
+    std::vector<LeafBox> leaves(n);
+    if (n) std::memcpy(leaves.data(), aabbs, sizeof(LeafBox) * (size_t)n);
+    std::vector<pq_bvh_node_wide> nodes;
+    uint32_t depth = 0;
+    build_binned_bvh(leaves, nodes, &depth);
+    return bvh_handle_from_nodes(nodes, n, depth, out);
+}
+
+size_t pq_bvh_ploc_workspace_bytes(uint32_t n) { return ploc_workspace_bytes(n); }
+
+int pq_bvh_from_leaves_ploc(const float *aabbs, uint32_t n, pq_bvh_node_wide *out, uint32_t capacity,
+                            uint32_t *num_nodes, uint32_t *depth, void *workspace, void *stream) {
+    if (!num_nodes) return fail(PQ_ERR_INVALID, "pq_bvh_from_leaves_ploc: null num_nodes");
+    *num_nodes = n == 0 ? 0u : (n == 1 ? 1u : n - 1);
+    if (depth) *depth = 0;
+    if (n == 0) return PQ_OK;
+    if (!aabbs) return fail(PQ_ERR_INVALID, "pq_bvh_from_leaves_ploc: null aabbs");
+    if (capacity < *num_nodes)
+        return fail(PQ_ERR_CAPACITY, "pq_bvh_from_leaves_ploc: need " + std::to_string(*num_nodes) + " records");
+    if (!out) return fail(PQ_ERR_INVALID, "pq_bvh_from_leaves_ploc: null output");
+    void *ws = workspace;
+    if (!ws) PQ_HIP_CHECK(stream_workspace((hipStream_t)stream, ploc_workspace_bytes(n), &ws));
+    uint32_t d = 0;
+    PQ_HIP_CHECK(build_ploc_bvh(aabbs, n, out, &d, ws, (hipStream_t)stream));
+    if (depth) *depth = d;
+    return PQ_OK;
+}
+
+int pq_bvh_create_ploc(const float *aabbs, uint32_t n, pq_bvh **out, void *stream) {
+    if (!out) return fail(PQ_ERR_INVALID, "pq_bvh_create_ploc: null output");
+    *out = nullptr;
+    if (n && !aabbs) return fail(PQ_ERR_INVALID, "pq_bvh_create_ploc: null aabbs");
+    const uint32_t nn = n == 0 ? 0u : (n == 1 ? 1u : n - 1);
+    std::vector<pq_bvh_node_wide> nodes(nn);
+    uint32_t depth = 0;
+    if (n) {
+        pq_bvh_node_wide *d_nodes = nullptr;
+        hipError_t e = hipMalloc(&d_nodes, nn * sizeof(pq_bvh_node_wide));
+        if (e != hipSuccess) return hip_fail(e, "pq_bvh_create_ploc");
+        uint32_t got = 0;
+        int rc = pq_bvh_from_leaves_ploc(aabbs, n, d_nodes, nn, &got, &depth, nullptr, stream);
+        if (rc == PQ_OK) {
+            e = hipMemcpy(nodes.data(), d_nodes, nn * sizeof(pq_bvh_node_wide), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = hip_fail(e, "pq_bvh_create_ploc download");
+        }
+        (void)hipFree(d_nodes);
+        if (rc != PQ_OK) return rc;
+    }
+    return bvh_handle_from_nodes(nodes, n, depth, out);
 }
 
 int pq_bvh_destroy(pq_bvh *b) {

@@ -148,6 +148,11 @@ struct DeviceBvh {
 };
 
 hipError_t launch_bvh_refit(const DeviceBvh &b, const float *aabbs, hipStream_t stream);
+// Bvh::from_leaves(Ploc, aabbs) on the device (ploc.hip): n - 1 depth-first wide
+// records (one for n <= 2) into `out`; *depth set on return.  Synchronises `stream`.
+size_t ploc_workspace_bytes(uint32_t n);
+hipError_t build_ploc_bvh(const float *aabbs, uint32_t n, pq_bvh_node_wide *out, uint32_t *depth, void *workspace,
+                          hipStream_t stream);
 hipError_t launch_bvh_self_pairs(const DeviceBvh &b, uint32_t *pairs, uint64_t capacity, unsigned long long *count,
                                  hipStream_t stream);
 hipError_t bvh_scan_temp_bytes(uint32_t num_leaves, size_t *bytes);

@@ -37,8 +37,24 @@ class Bvh:
         self._count = None
 
     @classmethod
-    def from_leaves(cls, aabbs) -> "Bvh":
-        return cls(aabbs)
+    def from_leaves(cls, aabbs, strategy: str = "binned", stream=None) -> "Bvh":
+        """``Bvh::from_leaves(strategy, aabbs)`` (bvh_tree.rs:1835): "binned" builds on
+        the host from an array; "ploc" (BvhBuildStrategy::Ploc, bvh_ploc_build.rs:10-93)
+        builds on the GPU from a CUDA tensor (n, 6) f32."""
+        if strategy == "binned":
+            return cls(aabbs)
+        if strategy != "ploc":
+            raise ValueError(f"unknown build strategy {strategy!r}")
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().pq_bvh_create_ploc(aabbs.data_ptr(), aabbs.shape[0], ctypes.byref(h),
+                                                       _stream_ptr(stream, aabbs.device)))
+        self._h = h
+        nl, nn, d = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _native.check(_native.lib().pq_bvh_info(h, ctypes.byref(nl), ctypes.byref(nn), ctypes.byref(d)))
+        self.num_leaves, self.num_nodes, self.depth = nl.value, nn.value, d.value
+        self._count = None
+        return self
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -86,3 +102,23 @@ class Bvh:
             out = torch.empty((m, 2), dtype=torch.int32, device=dev)
             _native.check(lib.pq_bvh_self_pairs(self._h, out.data_ptr(), m, self._count.data_ptr(), sp))
         return out[:m]
+
+
+def ploc_nodes(aabbs, out=None, workspace=None, stream=None):
+    """``Bvh::from_leaves(BvhBuildStrategy::Ploc, aabbs)``'s depth-first wide nodes built
+    on the GPU (pq_bvh_from_leaves_ploc): ``aabbs`` a CUDA tensor (n, 6) f32.  Returns
+    (nodes uint8 CUDA tensor of n - 1 (1 for n <= 2) 64-byte records, depth); view
+    them with ``BVH_NODE_WIDE_DTYPE`` after a copy to the host.  Synchronises ``stream``."""
+    import torch
+
+    n = aabbs.shape[0]
+    if not (aabbs.is_cuda and aabbs.dtype == torch.float32 and aabbs.is_contiguous() and aabbs.shape[-1] == 6):
+        raise ValueError("ploc_nodes: aabbs must be a contiguous CUDA float32 tensor (n, 6)")
+    nn = 0 if n == 0 else max(1, n - 1)
+    if out is None:
+        out = torch.empty((max(nn, 1), 64), dtype=torch.uint8, device=aabbs.device)
+    got, depth = ctypes.c_uint32(), ctypes.c_uint32()
+    _native.check(_native.lib().pq_bvh_from_leaves_ploc(
+        aabbs.data_ptr(), n, out.data_ptr(), out.shape[0], ctypes.byref(got), ctypes.byref(depth),
+        None if workspace is None else workspace.data_ptr(), _stream_ptr(stream, aabbs.device)))
+    return out[:got.value], depth.value

@@ -18,6 +18,8 @@ import os
 import numpy as np
 import pytest
 
+from bvh_checks import check_tree
+
 from parry_amd import scenes
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -129,55 +131,11 @@ def test_tie_keeps_first_visited(orc):
     out = orc.OracleTriMesh(v, f_sw).cast_rays(np.array([[0.2, 0.2, 5, 0, 0, -1]], np.float32), 100.0)
     assert out["feature"][0] == 0 + 2  # leaf 0 is now seen from its back face -> Face(0 + ntri)
 
-
-def _check_tree(nodes, n_leaves):
-    lc = lambda d: d & 0x3FFFFFFF
-    assert nodes.shape[0] == max(1, n_leaves - 1)
-    seen = set()
-    # depth-first order (bvh_validation.rs:196-218)
-    stack, expect = [0], 0
-    while stack:
-        i = stack.pop()
-        assert i == expect
-        expect += 1
-        n = nodes[i]
-        if lc(n["right_data"]) != 1:
-            stack.append(int(n["right_children"]))
-        if lc(n["left_data"]) != 1:
-            stack.append(int(n["left_children"]))
-
-    def rec(i):
-        n = nodes[i]
-        total = 0
-        for side in ("left", "right"):
-            d = lc(int(n[side + "_data"]))
-            c = int(n[side + "_children"])
-            if d == 1:
-                assert c not in seen
-                seen.add(c)
-                total += 1
-            else:
-                ch = nodes[c]
-                sub = rec(c)
-                assert d == sub
-                for cs in ("left", "right"):
-                    assert (n[side + "_mins"] <= ch[cs + "_mins"]).all()
-                    assert (n[side + "_maxs"] >= ch[cs + "_maxs"]).all()
-                assert np.array_equal(n[side + "_mins"], np.minimum(ch["left_mins"], ch["right_mins"]))
-                assert np.array_equal(n[side + "_maxs"], np.maximum(ch["left_maxs"], ch["right_maxs"]))
-                total += sub
-        return total
-
-    if n_leaves >= 2:
-        assert rec(0) == n_leaves
-        assert seen == set(range(n_leaves))
-
-
 @pytest.mark.parametrize("level", [0, 1, 4, 6])
 def test_oracle_tree_invariants(orc, level):
     v, f = scenes.noisy_icosphere(level, seed=1)
     m = orc.OracleTriMesh(v, f)
-    _check_tree(m.bvh_nodes(), f.shape[0])
+    check_tree(m.bvh_nodes(), f.shape[0])
 
 
 def test_f32_matches_f64(orc):
