@@ -451,6 +451,57 @@ def cast_leg(args, log):
     return rec
 
 
+def ploc_leg(args, log):
+    """Bvh::from_leaves(BvhBuildStrategy::Ploc, aabbs) (bvh_ploc_build.rs:10-93) built
+    on the GPU from the 262,144 shape AABBs of the dynamic scene (scenes.mixed_scene
+    seed 3) and from 1,048,576 random boxes (centres U[-200, 200]^3, half extents
+    U[0.01, 1]^3, seed 12).  One step = one pq_bvh_from_leaves_ploc call (Morton
+    keys, radix sort, every merge level with its host readback, the depth-first
+    relayout), inputs and outputs in device memory."""
+    import numpy as np
+    import torch
+
+    from parry_amd import BVH_NODE_WIDE_DTYPE, ShapeSet, ploc_nodes, scenes
+    from parry_amd import _native
+
+    sc = scenes.mixed_scene(262144, 3)
+    shapes = ShapeSet(sc["descs"], sc["vertices"])
+    scene_aabbs = shapes.compute_aabbs(torch.from_numpy(sc["poses"]).cuda())
+    rng = np.random.default_rng(12)
+    c = rng.uniform(-200, 200, (1 << 20, 3)).astype(np.float32)
+    h = rng.uniform(0.01, 1.0, (1 << 20, 3)).astype(np.float32)
+    rand_aabbs = torch.from_numpy(np.concatenate([c - h, c + h], 1)).cuda()
+    out = {}
+    for name, a in (("mixed_scene_262144", scene_aabbs), ("random_1048576", rand_aabbs)):
+        n = a.shape[0]
+        nodes = torch.empty((n - 1, 64), dtype=torch.uint8, device="cuda")
+        ws = torch.empty(int(_native.lib().pq_bvh_ploc_workspace_bytes(n)), dtype=torch.uint8, device="cuda")
+        for _ in range(args.warmup):
+            ploc_nodes(a, out=nodes, workspace=ws)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, depth = ploc_nodes(a, out=nodes, workspace=ws)
+        wall = (time.perf_counter() - t0) / args.steps
+        rec = {"metric": "Bvh leaves built/s", "value": round(n / wall, 1), "unit": "leaves/s",
+               "ms_per_build": round(wall * 1e3, 4), "leaves": n, "depth": depth}
+        if not args.no_cpu:
+            from oracle import oracle
+
+            host = a.cpu().numpy()
+            t0 = time.perf_counter()
+            ob = oracle.OracleBvh(host, ploc=True)
+            t_cpu = time.perf_counter() - t0
+            ref = ob.nodes()
+            got = nodes.cpu().numpy().view(BVH_NODE_WIDE_DTYPE).reshape(-1)
+            rec["cpu_baseline"] = {"value": round(n / t_cpu, 1), "unit": "leaves/s", "cores": 1, "kind": "port",
+                                   "sample": f"the same {n} leaves, one build on one thread (C++ restatement of "
+                                             "parry3d's PLOC, which is single-threaded)"}
+            rec["parity"] = {"byte_identical_tree": bool(got.tobytes() == ref.tobytes())}
+        out[name] = rec
+    return out
+
+
 def learned_leg(args, log):
     """Learned GJK warm start vs the plain GJK on 1,048,576 mixed convex pairs
     (ConvexPolyhedron 8-64 points, Cuboid, Ball, Capsule, Cone; the overlapping
@@ -734,6 +785,8 @@ def main() -> int:
     ap.add_argument("--queries-only", action="store_true", help="profiling aid: run only the point/culling leg")
     ap.add_argument("--skip-mesh-contact", action="store_true", help="skip the TriMesh-shape contact leg")
     ap.add_argument("--skip-cast", action="store_true", help="skip the shape-cast leg")
+    ap.add_argument("--skip-ploc", action="store_true", help="skip the PLOC build leg")
+    ap.add_argument("--ploc-only", action="store_true", help="profiling aid: run only the PLOC build leg")
     ap.add_argument("--cast-only", action="store_true", help="profiling aid: run only the shape-cast leg")
     ap.add_argument("--mesh-contact-only", action="store_true", help="profiling aid: run only the mesh contact leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -779,6 +832,9 @@ def main() -> int:
         return 0
     if args.queries_only:
         print(json.dumps({"queries": queries_leg(args, log)}), flush=True)
+        return 0
+    if args.ploc_only:
+        print(json.dumps({"ploc": ploc_leg(args, log)}), flush=True)
         return 0
     if args.cast_only:
         print(json.dumps({"cast": cast_leg(args, log)}), flush=True)
@@ -868,6 +924,9 @@ def main() -> int:
     if not args.skip_queries:
         log("point projection and culling leg ...")
         rec["queries"] = queries_leg(args, log)
+    if not args.skip_ploc:
+        log("PLOC build leg ...")
+        rec["ploc"] = ploc_leg(args, log)
     if not args.skip_cast:
         log("shape-cast leg ...")
         rec["cast"] = cast_leg(args, log)

@@ -98,21 +98,50 @@ __global__ __launch_bounds__(kBlock) void centre_box_partial_kernel(const float 
     }
 }
 
+// Folds the partial boxes into partial[0] (one block; min and max are exact,
+// so the order does not matter).
+__global__ __launch_bounds__(kBlock) void centre_box_final_kernel(Box6 *__restrict__ partial, uint32_t nparts) {
+    __shared__ float s[6][kBlock];
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) {
+        const Box6 q = partial[p];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fminf(lo[d], q.lo[d]);
+            hi[d] = fmaxf(hi[d], q.hi[d]);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        s[d][threadIdx.x] = lo[d];
+        s[3 + d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int d = 0; d < 3; ++d) {
+                s[d][threadIdx.x] = fminf(s[d][threadIdx.x], s[d][threadIdx.x + w]);
+                s[3 + d][threadIdx.x] = fmaxf(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        Box6 b;
+        for (int d = 0; d < 3; ++d) {
+            b.lo[d] = s[d][0];
+            b.hi[d] = s[3 + d][0];
+        }
+        partial[0] = b;
+    }
+}
+
 // Morton key of every leaf (bvh_ploc_build.rs:12-21) and the identity permutation.
 __global__ __launch_bounds__(kBlock) void keys_kernel(const float *__restrict__ aabbs, uint32_t n,
-                                                      const Box6 *__restrict__ partial, uint32_t nparts,
+                                                      const Box6 *__restrict__ cent,
                                                       unsigned long long *__restrict__ keys,
                                                       uint32_t *__restrict__ idx) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    Box6 b = partial[0];
-    for (uint32_t p = 1; p < nparts; ++p) {
-        const Box6 q = partial[p];
-        for (int d = 0; d < 3; ++d) {
-            b.lo[d] = fminf(b.lo[d], q.lo[d]);
-            b.hi[d] = fmaxf(b.hi[d], q.hi[d]);
-        }
-    }
+    const Box6 b = *cent;
     const float *a = aabbs + 6 * (size_t)i;
     uint64_t key = 0;
     for (int d = 0; d < 3; ++d) {
@@ -367,8 +396,8 @@ hipError_t build_ploc_bvh(const float *aabbs, uint32_t n, pq_bvh_node_wide *out,
     hipError_t e;
     const uint32_t parts = std::min<uint32_t>(kReduceBlocks, blocks(n));
     hipLaunchKernelGGL(centre_box_partial_kernel, dim3(parts), dim3(kBlock), 0, stream, aabbs, n, w.partial);
-    hipLaunchKernelGGL(keys_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream, aabbs, n, w.partial, parts, w.keys,
-                       w.idx);
+    hipLaunchKernelGGL(centre_box_final_kernel, dim3(1), dim3(kBlock), 0, stream, w.partial, parts);
+    hipLaunchKernelGGL(keys_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream, aabbs, n, w.partial, w.keys, w.idx);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     size_t sb = w.sort_bytes;
     if ((e = rocprim::radix_sort_pairs(w.sort_temp, sb, w.keys, w.keys_sorted, w.idx, w.idx_sorted, (size_t)n, 0, 63,
