@@ -16,6 +16,7 @@
 #include "parry_oracle_contact.hpp"
 #include "parry_oracle_point.hpp"
 #include "parry_oracle_cast.hpp"
+#include "parry_oracle_manifold.hpp"
 
 using namespace oracle;
 
@@ -581,5 +582,68 @@ void orc_cast_shapes_f64(const void *descs, uint32_t nshapes, const double *vert
                          uint32_t *iters, int nthreads) {
     cast_pairs<double>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, vels, pairs, m, max_toi,
                        target_distance, stop, compute_geometry, out, status, iters, nthreads);
+}
+}  // extern "C"
+
+// ---- contact manifolds (pfm-pfm) ------------------------------------------------
+namespace {
+constexpr int kManifoldPoints = 25;
+template <class T>
+void manifolds_pfm(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uint32_t nverts, const T *poses,
+                   const uint32_t *pairs, uint64_t m, T prediction, int32_t *status, T *normals, uint32_t *counts,
+                   T *points, uint32_t *fids, int nthreads) {
+    std::vector<V3<T>> pts(nverts);
+    for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
+    std::vector<Shape<T>> shapes = make_shapes(descs, nshapes, pts);
+    parallel_for(m, nthreads, 64, [&](uint64_t b, uint64_t e) {
+        Manifold<T> mf;
+        for (uint64_t k = b; k < e; ++k) {
+            uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+            const T *a = poses + 7 * (size_t)i, *c = poses + 7 * (size_t)j;
+            Iso<T> p1{Quat<T>{a[0], a[1], a[2], a[3]}, v3(a[4], a[5], a[6])};
+            Iso<T> p2{Quat<T>{c[0], c[1], c[2], c[3]}, v3(c[4], c[5], c[6])};
+            int r = manifold_pfm_pfm(inv_mul(p1, p2), shapes[i], shapes[j], prediction, mf);
+            status[k] = r;
+            T *nrm = normals + 6 * k;
+            nrm[0] = mf.n1.x; nrm[1] = mf.n1.y; nrm[2] = mf.n1.z;
+            nrm[3] = mf.n2.x; nrm[4] = mf.n2.y; nrm[5] = mf.n2.z;
+            const size_t np = mf.points.size();
+            counts[k] = (uint32_t)np;
+            T *q = points + (size_t)k * kManifoldPoints * 7;
+            uint32_t *f = fids + (size_t)k * kManifoldPoints * 2;
+            for (size_t t = 0; t < (size_t)kManifoldPoints; ++t) {
+                if (t < np) {
+                    const auto &c2 = mf.points[t];
+                    q[7 * t + 0] = c2.p1.x; q[7 * t + 1] = c2.p1.y; q[7 * t + 2] = c2.p1.z;
+                    q[7 * t + 3] = c2.p2.x; q[7 * t + 4] = c2.p2.y; q[7 * t + 5] = c2.p2.z;
+                    q[7 * t + 6] = c2.dist;
+                    f[2 * t] = c2.fid1;
+                    f[2 * t + 1] = c2.fid2;
+                } else {
+                    for (int z = 0; z < 7; ++z) q[7 * t + z] = T(0);
+                    f[2 * t] = f[2 * t + 1] = 0u;
+                }
+            }
+        }
+    });
+}
+}  // namespace
+
+extern "C" {
+// contact_manifold_pfm_pfm on a fresh manifold for every (i, j) in `pairs`.
+// status: 1 ClosestPoints, 0 NoIntersection, -1 no feature map here; normals:
+// 6 per pair (local_n1, local_n2); counts: points per pair (at most 25);
+// points: 25 x 7 per pair (local_p1, local_p2, dist); fids: 25 x 2 per pair.
+void orc_manifolds_pfm_f32(const void *descs, uint32_t nshapes, const float *verts, uint32_t nverts,
+                           const float *poses, const uint32_t *pairs, uint64_t m, float prediction, int32_t *status,
+                           float *normals, uint32_t *counts, float *points, uint32_t *fids, int nthreads) {
+    manifolds_pfm<float>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, status,
+                         normals, counts, points, fids, nthreads);
+}
+void orc_manifolds_pfm_f64(const void *descs, uint32_t nshapes, const double *verts, uint32_t nverts,
+                           const double *poses, const uint32_t *pairs, uint64_t m, double prediction, int32_t *status,
+                           double *normals, uint32_t *counts, double *points, uint32_t *fids, int nthreads) {
+    manifolds_pfm<double>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, status,
+                          normals, counts, points, fids, nthreads);
 }
 }  // extern "C"

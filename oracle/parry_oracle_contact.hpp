@@ -35,7 +35,11 @@ template <> struct Eps<double> {
 // known-answer test (crates/parry3d/tests/geometry/epa3.rs:25-59).
 // ORIGIN is parry's ConstantOrigin (special_support_maps.rs:32-54), the second
 // operand of GJK/EPA point projections.
-enum ShapeType : uint32_t { BALL = 0, CUBOID = 1, CAPSULE = 2, CONE = 3, CONVEX_POLYHEDRON = 4, TRIANGLE = 5, ORIGIN = 6 };
+// SEGMENT (internal) is a Capsule's segment (0, -h, 0)-(0, h, 0), the feature map
+// contact manifolds use for a Capsule (shape.rs:855-857).
+enum ShapeType : uint32_t {
+    BALL = 0, CUBOID = 1, CAPSULE = 2, CONE = 3, CONVEX_POLYHEDRON = 4, TRIANGLE = 5, ORIGIN = 6, SEGMENT = 7
+};
 
 template <class T> struct Shape {
     uint32_t type;
@@ -83,6 +87,10 @@ template <class T> V3<T> local_support_point(const Shape<T> &s, V3<T> dir) {
         }
         case ORIGIN:  // special_support_maps.rs:46-48
             return v3(T(0), T(0), T(0));
+        case SEGMENT: {  // segment.rs:461-467
+            V3<T> a = v3(T(0), -s.p[0], T(0)), b = v3(T(0), s.p[0], T(0));
+            return dot(a, dir) > dot(b, dir) ? a : b;
+        }
         case CONE: {  // cone.rs:162-182
             V3<T> v = dir;
             v.y = T(0);
