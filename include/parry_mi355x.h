@@ -348,6 +348,53 @@ int pq_contact_get_tuning(pq_contact_tuning *tuning);
 /* Device workspace bytes pq_contact_pairs needs for `num_pairs` pairs. */
 size_t pq_contact_workspace_bytes(uint64_t num_pairs);
 
+/* ---- Contact manifolds between polygonal feature maps ------------------------ */
+
+/* TrackedContact (src/query/contact_manifolds/contact_manifold.rs:16-60): the
+ * points in the local frames of shape 1 and shape 2, the signed distance and the
+ * PackedFeatureId of each shape's feature (feature_id.rs:446-543; 0 = unknown). */
+typedef struct pq_tracked_contact {
+    float local_p1[3];
+    float local_p2[3];
+    float dist;
+    uint32_t fid1;
+    uint32_t fid2;
+} pq_tracked_contact;
+
+/* At most 25 points: 4 + 4 vertices inside the other face, 16 edge crossings, and
+ * GJK's own point. */
+#define PQ_MANIFOLD_MAX_POINTS 25
+/* ContactManifold (contact_manifold.rs:470-500) as contact_manifold_pfm_pfm leaves
+ * it.  status: 1 = GJK found closest points (the points are filled); 0 = no
+ * contact within the prediction (no points; local_n1 caches GJK's separating
+ * direction, as parry's manifold does); PQ_CONTACT_UNSUPPORTED = a shape has no
+ * polygonal feature map here. */
+typedef struct pq_contact_manifold {
+    float local_n1[3];
+    float local_n2[3];
+    uint32_t num_points;
+    uint32_t status;
+    pq_tracked_contact points[PQ_MANIFOLD_MAX_POINTS];
+} pq_contact_manifold;
+
+/* Batched contact_manifold_pfm_pfm(pos12, pfm1, border1, None, pfm2, border2, None,
+ * prediction, manifold) (contact_manifolds_pfm_pfm.rs:44-162) on a fresh manifold
+ * (no points, zero cached normal: the first contact of a pair, or every contact
+ * whose manifold update fails) for every (i, j) of `pairs`.  Feature maps
+ * (shape.rs as_polygonal_feature_map): Cuboid, Cone and Triangle are their own; a
+ * Capsule is its segment with the radius as border; Ball and ConvexPolyhedron
+ * have none here.  parry's dispatcher sends Cuboid-Cuboid, Capsule-Capsule and
+ * Cuboid-Triangle to other routines; this call runs the feature-map routine on
+ * whatever pairs it is given.
+ *   poses: device, one pq_isometry per shape; pairs: device, num_pairs x (i, j);
+ *   out: device, one pq_contact_manifold per pair; workspace: device,
+ *   pq_contact_manifolds_workspace_bytes(num_pairs) bytes, or NULL.  Asynchronous
+ *   on `stream`. */
+int pq_contact_manifolds_pfm_pairs(const pq_shapes *shapes, const pq_isometry *poses, const uint32_t *pairs,
+                                   uint64_t num_pairs, float prediction, pq_contact_manifold *out, void *workspace,
+                                   void *stream);
+size_t pq_contact_manifolds_workspace_bytes(uint64_t num_pairs);
+
 /* ---- Linear shape casts ---------------------------------------------------- */
 
 /* ShapeCastOptions (src/query/shape_cast/shape_cast.rs:105-146).  parry's

@@ -50,8 +50,27 @@ enum : uint32_t {
     kPairSupportMapsLocal = 3,  // contact_support_map_support_map on any pair, Ball included,
                                 // results left in the shapes' local frames (the shape cast's
                                 // impact geometry, shape_cast_support_map_support_map.rs:42-46)
+    kPairFeatureMaps = 4,       // the same on the pair's polygonal feature maps (a Capsule is its
+                                // segment), for contact manifolds (contact_manifolds_pfm_pfm.rs:66-74)
 };
 __device__ __forceinline__ bool is_ball_mode(uint32_t mode) { return mode == kBallFirst || mode == kBallSecond; }
+__device__ __forceinline__ bool is_local_mode(uint32_t mode) {
+    return mode == kPairSupportMapsLocal || mode == kPairFeatureMaps;
+}
+// A shape's polygonal feature map and border radius (shape.rs as_polygonal_feature_map):
+// Cuboid, Cone and Triangle are their own, a Capsule is its segment with the radius as
+// border.  False for shapes without one here (Ball; ConvexPolyhedron, whose map needs
+// face topology the shape records do not carry).
+__device__ __forceinline__ bool feature_map(ShapeRef &g, float &border) {
+    border = 0.0f;
+    if (g.type == PQ_SHAPE_CUBOID || g.type == PQ_SHAPE_CONE || g.type == PQ_SHAPE_TRIANGLE) return true;
+    if (g.type == PQ_SHAPE_CAPSULE) {
+        g.type = kShapeSegment;
+        border = g.p1;
+        return true;
+    }
+    return false;
+}
 struct EpaItem {
     uint32_t pair;
     uint32_t dim;
@@ -143,9 +162,17 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
         const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
         const iso pos1 = load_iso(poses + 7 * (size_t)i), pos2 = load_iso(poses + 7 * (size_t)j);
         const iso pos12 = iso_inv_mul(pos1, pos2);
-        const ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
+        ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
         DevContact *const c = out + k;
         uint32_t gi = 0;
+        const bool pfm = (flags & kContactFeatureMaps) != 0u;
+        bool unsupported = false;
+        float total_prediction = prediction;
+        if (pfm) {  // total_prediction = prediction + border1 + border2 (contact_manifolds_pfm_pfm.rs:65)
+            float br1 = 0.0f, br2 = 0.0f;
+            unsupported = !feature_map(g1, br1) || !feature_map(g2, br2);
+            total_prediction = prediction + br1 + br2;
+        }
         // What GJK runs on, if anything: a support-map pair, or a ConvexPolyhedron
         // against ConstantOrigin (the ball-centre projection).  One call site below.
         bool run_gjk = false;
@@ -158,8 +185,10 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
         iso pb{};
         ShapeRef cv{};
         float radius = 0.0f;
-        const bool local = (flags & kContactLocalSupportMaps) != 0u;
-        if (!local && g1.type == PQ_SHAPE_BALL && g2.type == PQ_SHAPE_BALL) {
+        const bool local = (flags & (kContactLocalSupportMaps | kContactFeatureMaps)) != 0u;
+        if (unsupported) {
+            *c = no_contact(kUnsupportedPair);
+        } else if (!local && g1.type == PQ_SHAPE_BALL && g2.type == PQ_SHAPE_BALL) {
             // contact_ball_ball
             const float r1 = g1.p0, r2 = g2.p0;
             const f3 c21 = pos12.t;
@@ -221,7 +250,8 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
             }
         } else {
             run_gjk = true;
-            ball_mode = local ? kPairSupportMapsLocal : kPairSupportMaps;
+            ball_mode = pfm ? kPairFeatureMaps : (local ? kPairSupportMapsLocal : kPairSupportMaps);
+            max_dist = total_prediction;
             ga = g1;
             gb = g2;
             seed_pos = gjk_pos = pos12;
@@ -255,9 +285,14 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
                     *c = no_contact(kUnsupportedPair);  // parry: unreachable!() in gjk::project_origin
                 }
             } else if (r.kind == GJK_CLOSEST) {
-                finish_contact(pos1, pos2, pos12, r.p1, r.p2, r.n, ball_mode == kPairSupportMapsLocal, c);
+                finish_contact(pos1, pos2, pos12, r.p1, r.p2, r.n, is_local_mode(ball_mode), c);
             } else if (r.kind == GJK_NO_INTERSECTION) {
                 *c = no_contact(0);
+                if (pfm) {  // the manifold caches GJK's separating direction (contact_manifolds_pfm_pfm.rs:148-151)
+                    c->n1[0] = r.n.x;
+                    c->n1[1] = r.n.y;
+                    c->n1[2] = r.n.z;
+                }
             } else {
                 need_epa = true;
             }
@@ -1070,10 +1105,15 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
     const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
     const iso pos1 = load_iso(poses + 7 * (size_t)i), pos2 = load_iso(poses + 7 * (size_t)j);
     const iso pos12 = iso_inv_mul(pos1, pos2);
-    const ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
+    ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
     f3 p1, p2, n;
     uint32_t ei = 0, cont = kNoCont;
     const uint32_t mode = it->mode;
+    if (mode == kPairFeatureMaps) {
+        float br;
+        (void)feature_map(g1, br);
+        (void)feature_map(g2, br);
+    }
     // A pair that outgrows this tier's polytope goes to the next tier's list,
     // with its continuation record if it has one.
     auto hand_on = [&]() {
@@ -1115,9 +1155,10 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
         return;
     }
     if (st == EPA_OK) {
-        finish_contact(pos1, pos2, pos12, p1, p2, n, mode == kPairSupportMapsLocal, out + k);
+        finish_contact(pos1, pos2, pos12, p1, p2, n, is_local_mode(mode), out + k);
     } else {
         out[k] = no_contact(0);  // EPA failed: NoIntersection (contact_support_map_support_map.rs:75-76)
+        if (mode == kPairFeatureMaps) out[k].n1[0] = 1.0f;  // ... of Vector::x_axis()
     }
     if (iters) iters[2 * k + 1] = ei;
 }

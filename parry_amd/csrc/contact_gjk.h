@@ -29,6 +29,9 @@ constexpr uint32_t kGjkMaxIters = 100;      // gjk.rs:449-451
 // parry's ConstantOrigin (src/query/gjk/special_support_maps.rs:32-54): the
 // second operand of GJK/EPA point projections.  Internal, never in a shape set.
 constexpr uint32_t kShapeOrigin = 6u;
+// A Capsule's segment (0, -h, 0)-(0, h, 0): the feature map contact manifolds use
+// for a Capsule (shape.rs:855-857).  Internal, never in a shape set.
+constexpr uint32_t kShapeSegment = 7u;
 
 struct ShapeRef {
     uint32_t type;
@@ -70,6 +73,10 @@ __device__ __forceinline__ f3 local_support(const ShapeRef &s, f3 d) {
         return f3{w.x, w.y, w.z};
     }
     if (s.type == kShapeOrigin) return f3{0.0f, 0.0f, 0.0f};
+    if (s.type == kShapeSegment) {  // segment.rs:461-467
+        const f3 a{0.0f, -s.p0, 0.0f}, b{0.0f, s.p0, 0.0f};
+        return dot3(a, d) > dot3(b, d) ? a : b;
+    }
     if (s.type == PQ_SHAPE_CAPSULE) {
         const float sq = norm_sq3(d);
         f3 u = sq > 0.0f ? div3(d, sqrtf(sq)) : f3{0.0f, 1.0f, 0.0f};
@@ -575,6 +582,7 @@ __device__ __forceinline__ GjkOut gjk_closest_points(const iso &pos12, const Sha
         const float min_bound = -dot3(dir, c.p);
         if (!isfinite(min_bound)) {  // parry asserts here (gjk.rs:407)
             out.kind = GJK_NO_INTERSECTION;
+            out.n = f3{1.0f, 0.0f, 0.0f};
             return out;
         }
         if (min_bound > max_dist) {

@@ -176,9 +176,18 @@ hipError_t launch_mesh_contacts(const DeviceMesh &m, const DeviceShapes &combine
 // pair (Ball included, no closed forms) and leaves the results in the shapes'
 // local frames, as the shape cast's impact geometry needs.
 constexpr uint32_t kContactLocalSupportMaps = 1u;
+// kContactFeatureMaps runs it on the pairs' polygonal feature maps (a Capsule is its
+// segment) with prediction + both border radii, local frames, and keeps GJK's
+// separating direction in normal1 of a pair without contact; a pair without
+// feature maps gets PQ_CONTACT_UNSUPPORTED.
+constexpr uint32_t kContactFeatureMaps = 2u;
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
                            const uint64_t *m_dev, float prediction, const float *seeds, pq_contact *out,
                            uint32_t *iters, void *workspace, hipStream_t stream, uint32_t flags = 0u);
+// Contact manifolds between polygonal feature maps (manifold.hip).
+size_t manifold_workspace_bytes(uint64_t m);
+hipError_t launch_manifolds(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
+                            float prediction, pq_contact_manifold *out, void *workspace, hipStream_t stream);
 // Linear shape casts (cast.hip).
 size_t cast_workspace_bytes(uint64_t m);
 hipError_t launch_casts(const DeviceShapes &s, const float *poses, const float *vels, const uint32_t *pairs, uint64_t m,

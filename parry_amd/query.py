@@ -234,3 +234,44 @@ def cast_shapes_pairs(shapes, poses, vels, pairs, max_time_of_impact: float = F3
         None if iters is None else iters.data_ptr(), None if workspace is None else workspace.data_ptr(),
         ctypes.c_void_p(s.cuda_stream)))
     return (out, iters) if with_iters else out
+
+
+# pq_contact_manifold: local_n1, local_n2, num_points, status, then 25 pq_tracked_contact.
+MANIFOLD_MAX_POINTS = 25
+TRACKED_CONTACT_DTYPE = np.dtype([("local_p1", "<f4", 3), ("local_p2", "<f4", 3), ("dist", "<f4"),
+                                  ("fid1", "<u4"), ("fid2", "<u4")])
+CONTACT_MANIFOLD_DTYPE = np.dtype([("local_n1", "<f4", 3), ("local_n2", "<f4", 3), ("num_points", "<u4"),
+                                   ("status", "<u4"), ("points", TRACKED_CONTACT_DTYPE, MANIFOLD_MAX_POINTS)])
+assert CONTACT_MANIFOLD_DTYPE.itemsize == 932
+
+
+def contact_manifolds_workspace_bytes(num_pairs: int) -> int:
+    from . import _native
+
+    return int(_native.lib().pq_contact_manifolds_workspace_bytes(int(num_pairs)))
+
+
+def contact_manifolds_pfm(shapes, poses, pairs, prediction: float, out=None, workspace=None, stream=None):
+    """Batched ``contact_manifold_pfm_pfm`` (contact_manifolds_pfm_pfm.rs:44-162) on fresh
+    manifolds, CUDA tensors in and out.  Returns a uint8 CUDA tensor of (m, 932)
+    ``pq_contact_manifold`` records; view them with ``CONTACT_MANIFOLD_DTYPE`` after a
+    copy to the host."""
+    import ctypes
+
+    import torch
+
+    from . import _native
+
+    m = pairs.shape[0]
+    for t in (poses, pairs):
+        if not (t.is_cuda and t.is_contiguous()):
+            raise ValueError("contact_manifolds_pfm: tensors must be contiguous CUDA tensors")
+    if tuple(poses.shape) != (len(shapes), 7) or poses.dtype != torch.float32:
+        raise ValueError(f"contact_manifolds_pfm: poses must be float32 ({len(shapes)}, 7)")
+    if out is None:
+        out = torch.empty((m, CONTACT_MANIFOLD_DTYPE.itemsize), dtype=torch.uint8, device=poses.device)
+    s = stream if stream is not None else torch.cuda.current_stream(poses.device)
+    _native.check(_native.lib().pq_contact_manifolds_pfm_pairs(
+        shapes.handle, poses.data_ptr(), pairs.data_ptr(), m, float(prediction), out.data_ptr(),
+        None if workspace is None else workspace.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
+    return out
