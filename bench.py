@@ -502,6 +502,78 @@ def ploc_leg(args, log):
     return out
 
 
+def manifold_leg(args, log):
+    """contact_manifold_pfm_pfm (contact_manifolds_pfm_pfm.rs:44-162) on fresh manifolds
+    for 1,048,576 pairs of Cuboid, Capsule, Cone and Triangle (scenes.feature_map_pairs,
+    seed 3, offsets N(0, 0.6^2)): one step = one pq_contact_manifolds_pfm_pairs call
+    (GJK/EPA on the feature maps, then the features and their clipping per pair).
+    Algorithmic bytes per pair: 8 (pair) + 2 x 28 (poses) + 2 x 32 (records) + 36 per
+    Triangle + 32 (manifold header) + 36 per manifold point written."""
+    import numpy as np
+    import torch
+
+    from parry_amd import CONTACT_MANIFOLD_DTYPE, ShapeSet, contact_manifolds_pfm, contact_manifolds_workspace_bytes
+    from parry_amd import scenes
+
+    n = 1 << 20
+    d, verts, poses_np, pairs_np = scenes.feature_map_pairs(n, 3)
+    shapes = ShapeSet(d, verts)
+    poses = torch.from_numpy(poses_np).cuda()
+    pairs = torch.from_numpy(pairs_np.view(np.int32)).cuda()
+    out = torch.empty((n, CONTACT_MANIFOLD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    ws = torch.empty(contact_manifolds_workspace_bytes(n), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        contact_manifolds_pfm(shapes, poses, pairs, 0.02, out=out, workspace=ws, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        step()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    gpu_ms = ev[0].elapsed_time(ev[1]) / args.steps
+    got = out.cpu().numpy().view(CONTACT_MANIFOLD_DTYPE).reshape(-1)
+    pts = int(got["num_points"].sum())
+    ntri = int((d["type"] == 5).sum())
+    alg = (8 + 56 + 64 + 32) * n + 36 * ntri + 36 * pts
+    gbs = alg / (gpu_ms * 1e-3) / 1e9
+    rec = {
+        "metric": "contact manifolds/s",
+        "value": round(n / wall, 1),
+        "unit": "manifolds/s",
+        "ms_per_step": round(wall * 1e3, 4),
+        "gpu_ms_per_step": round(gpu_ms, 4),
+        "workload": "contact_manifold_pfm_pfm on fresh manifolds, prediction 0.02: 1048576 pairs of Cuboid, "
+                    "Capsule, Cone and Triangle (scenes.feature_map_pairs seed 3)",
+        "manifolds_in_contact": int((got["status"] == 1).sum()),
+        "points": pts,
+        "roofline": {"bound": "latency (GJK/EPA and per-pair clipping)", "kernel": "contact kernels + manifold_kernel",
+                     "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes_per_launch": alg},
+    }
+    if not args.no_cpu:
+        from oracle import oracle
+
+        cores = oracle.default_threads()
+        t0 = time.perf_counter()
+        ref = oracle.manifolds_pfm(d, verts, poses_np, pairs_np, 0.02, threads=cores)
+        t_cpu = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(n / t_cpu, 1), "unit": "manifolds/s", "cores": cores, "kind": "port",
+                               "sample": f"all {n} pairs, one run on {cores} threads (C++ restatement of parry3d f32)"}
+        ok = ref["status"] == 1
+        same = (got["status"] == np.where(ref["status"] < 0, 0xFFFFFFFF, ref["status"]).astype(np.uint32)) & \
+               (got["num_points"] == ref["count"])
+        rec["parity"] = {"status_or_count_mismatches": int((~same).sum()), "manifolds_oracle": int(ok.sum())}
+    return rec
+
+
 def learned_leg(args, log):
     """Learned GJK warm start vs the plain GJK on 1,048,576 mixed convex pairs
     (ConvexPolyhedron 8-64 points, Cuboid, Ball, Capsule, Cone; the overlapping
@@ -786,6 +858,8 @@ def main() -> int:
     ap.add_argument("--skip-mesh-contact", action="store_true", help="skip the TriMesh-shape contact leg")
     ap.add_argument("--skip-cast", action="store_true", help="skip the shape-cast leg")
     ap.add_argument("--skip-ploc", action="store_true", help="skip the PLOC build leg")
+    ap.add_argument("--skip-manifold", action="store_true", help="skip the contact-manifold leg")
+    ap.add_argument("--manifold-only", action="store_true", help="profiling aid: run only the contact-manifold leg")
     ap.add_argument("--ploc-only", action="store_true", help="profiling aid: run only the PLOC build leg")
     ap.add_argument("--cast-only", action="store_true", help="profiling aid: run only the shape-cast leg")
     ap.add_argument("--mesh-contact-only", action="store_true", help="profiling aid: run only the mesh contact leg")
@@ -832,6 +906,9 @@ def main() -> int:
         return 0
     if args.queries_only:
         print(json.dumps({"queries": queries_leg(args, log)}), flush=True)
+        return 0
+    if args.manifold_only:
+        print(json.dumps({"manifold": manifold_leg(args, log)}), flush=True)
         return 0
     if args.ploc_only:
         print(json.dumps({"ploc": ploc_leg(args, log)}), flush=True)
@@ -924,6 +1001,9 @@ def main() -> int:
     if not args.skip_queries:
         log("point projection and culling leg ...")
         rec["queries"] = queries_leg(args, log)
+    if not args.skip_manifold:
+        log("contact-manifold leg ...")
+        rec["manifold"] = manifold_leg(args, log)
     if not args.skip_ploc:
         log("PLOC build leg ...")
         rec["ploc"] = ploc_leg(args, log)

@@ -400,3 +400,35 @@ def approach_pairs(n: int, seed: int, spread=(2.0, 6.0), scale: float = 1.0, wit
     vels[1::2] += (-dirs + rng.normal(0, 0.3, (n, 3))) * rng.uniform(0.5, 3.0, (n, 1))
     pairs = np.stack([np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)], 1).astype(np.uint32)
     return descs, verts.astype(np.float32), poses.astype(np.float32), vels.astype(np.float32), pairs
+
+
+def feature_map_pairs(n: int, seed: int, spread: float = 0.6, types=(1, 2, 3, 5)):
+    """``n`` contact-manifold pairs (2q, 2q+1) of shapes with polygonal feature maps:
+    types drawn uniformly from ``types`` (Cuboid half extents U[0.3, 1]^3; Capsule
+    and Cone half height and radius U[0.3, 1]; Triangle vertices N(0, 0.7); a Ball
+    radius U[0.3, 1] when 0 is among the types), random rotations, shape 2q
+    uniform in [-40, 40]^3 and shape 2q+1 offset from it by N(0, spread^2) per axis.
+
+    Returns (descs, vertices float32, poses float32 [2n, 7], pairs uint32 [n, 2]).
+    """
+    from .shape import SHAPE_DESC_DTYPE
+
+    rng = np.random.default_rng(seed)
+    t = rng.choice(types, size=2 * n)
+    d = np.zeros(2 * n, SHAPE_DESC_DTYPE)
+    d["type"] = t
+    cub = t == 1
+    d["params"][cub, :3] = rng.uniform(0.3, 1.0, (int(cub.sum()), 3))
+    cc = (t == 2) | (t == 3)
+    d["params"][cc, :2] = rng.uniform(0.3, 1.0, (int(cc.sum()), 2))
+    ball = t == 0
+    d["params"][ball, 0] = rng.uniform(0.3, 1.0, int(ball.sum()))
+    tri = np.nonzero(t == 5)[0]
+    d["first_vertex"][tri] = 3 * np.arange(len(tri))
+    d["num_vertices"][tri] = 3
+    verts = rng.normal(0, 0.7, (3 * len(tri), 3)).astype(np.float32)
+    poses = np.concatenate([random_rotations(2 * n, rng), np.zeros((2 * n, 3))], 1)
+    poses[0::2, 4:] = rng.uniform(-40, 40, (n, 3))
+    poses[1::2, 4:] = poses[0::2, 4:] + rng.normal(0, spread, (n, 3))
+    pairs = np.stack([np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)], 1).astype(np.uint32)
+    return d, verts, poses.astype(np.float32), pairs
