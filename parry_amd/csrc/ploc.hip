@@ -13,8 +13,9 @@
 //      (ids in order of the lower index, the last merge being the root, id 0)
 //      that takes the lower index's place, and the other nodes carry over in
 //      order.  One launch per step, with an exclusive scan giving each kept
-//      node its place and each merge its id; the host reads the level's size
-//      back to size the next one;
+//      node its place and each merge its id; the level's length stays on the
+//      device (a one-lane kernel ends each level), and the host reads it back
+//      every few levels to size the grids of the next ones;
 //   4. the depth-first relayout: positions are handed down from the root, the
 //      levels taken newest first (a node is always newer than its children):
 //      left child at parent + 1, right child after the left subtree's
@@ -172,8 +173,11 @@ __global__ __launch_bounds__(kBlock) void init_leaves_kernel(const float *__rest
 
 // Every node's merge candidate: the smallest half area of the merged box
 // (Aabb::half_area, aabb.rs:473-476) within kRadius places, the first on ties.
-__global__ __launch_bounds__(kBlock) void candidates_kernel(const pq_bvh_node *__restrict__ cur, uint32_t L,
+__global__ __launch_bounds__(kBlock) void candidates_kernel(const pq_bvh_node *__restrict__ cur,
+                                                            const uint32_t *__restrict__ level_len,
                                                             uint32_t *__restrict__ cand) {
+    const uint32_t L = *level_len;
+    if (L <= 1u || (uint64_t)blockIdx.x * kBlock >= L) return;  // whole blocks past the level (uniform exit)
     constexpr int kTile = kBlock + 2 * kRadius;
     __shared__ float lo[3][kTile], hi[3][kTile];
     const int64_t start = (int64_t)blockIdx.x * kBlock - kRadius;
@@ -212,11 +216,15 @@ __global__ __launch_bounds__(kBlock) void candidates_kernel(const pq_bvh_node *_
 
 // flags[i] = (kept << 32) | (merges here): i is kept unless it is the higher
 // index of a mutual pair; it opens a merge if it is the lower one.
-__global__ __launch_bounds__(kBlock) void flags_kernel(const uint32_t *__restrict__ cand, uint32_t L,
+// Launched over bound + 1 entries (bound >= L): the entries from L on are zero.
+__global__ __launch_bounds__(kBlock) void flags_kernel(const uint32_t *__restrict__ cand,
+                                                       const uint32_t *__restrict__ level_len, uint32_t bound,
                                                        unsigned long long *__restrict__ flags) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i > L) return;
-    if (i == L) {
+    if (i > bound) return;
+    const uint32_t L = *level_len;
+    if (L <= 1u) return;
+    if (i >= L) {
         flags[i] = 0ull;
         return;
     }
@@ -229,11 +237,15 @@ __global__ __launch_bounds__(kBlock) void flags_kernel(const uint32_t *__restric
 
 __global__ __launch_bounds__(kBlock) void merge_kernel(const pq_bvh_node *__restrict__ cur,
                                                        const uint32_t *__restrict__ cand,
-                                                       const unsigned long long *__restrict__ offs, uint32_t L,
-                                                       uint32_t base, pq_bvh_node *__restrict__ next,
+                                                       const unsigned long long *__restrict__ offs,
+                                                       const uint32_t *__restrict__ level_len,
+                                                       const uint32_t *__restrict__ next_id,
+                                                       pq_bvh_node *__restrict__ next,
                                                        pq_bvh_node_wide *__restrict__ wide) {
+    const uint32_t L = *level_len;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= L) return;
+    if (L <= 1u || i >= L) return;
+    const uint32_t base = *next_id;
     const uint32_t k = cand[i];
     const bool mutual = k < L && cand[k] == i;
     if (mutual && i > k) return;
@@ -258,6 +270,39 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(const pq_bvh_node *__rest
     m.children = id;
     m.data = ((a.data & kCountMask) + (b.data & kCountMask)) | (((a.data >> 30) | (b.data >> 30)) << 30);
     next[pos] = m;
+}
+
+// Ends a level: records the id range it created (the root's level records
+// nothing), advances the next free id and sets the next level's length.
+// `state`: [0] level length, [1] next free wide id, [2] levels recorded, [3] error.
+__global__ void finish_level_kernel(const unsigned long long *__restrict__ offs, uint32_t *__restrict__ state,
+                                    uint32_t *__restrict__ level_first, uint32_t *__restrict__ level_count) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t L = state[0];
+    if (L <= 1u) return;
+    const unsigned long long tot = offs[L];
+    const uint32_t kept = (uint32_t)(tot >> 32), merges = (uint32_t)(tot & 0xffffffffull);
+    if (merges == 0u || kept >= L) {  // cannot happen: a mutual pair always exists
+        state[3] = 1u;
+        state[0] = 1u;
+        return;
+    }
+    if (L != 2u) {
+        level_first[state[2]] = state[1];
+        level_count[state[2]] = merges;
+        state[1] += merges;
+        state[2] += 1u;
+    }
+    state[0] = kept;
+}
+
+__global__ void init_state_kernel(uint32_t *state, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        state[0] = n;
+        state[1] = 1u;
+        state[2] = 0u;
+        state[3] = 0u;
+    }
 }
 
 // Depth-first positions of the children of the wide nodes [first, first + count).
@@ -335,7 +380,7 @@ uint32_t blocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 struct PlocWorkspace {
     Box6 *partial;
     unsigned long long *keys, *keys_sorted, *flags, *offs;
-    uint32_t *idx, *idx_sorted, *cand, *pos, *level, *max_level;
+    uint32_t *idx, *idx_sorted, *cand, *pos, *level, *max_level, *state, *level_first, *level_count;
     pq_bvh_node *cur, *next;
     pq_bvh_node_wide *wide;
     void *sort_temp, *scan_temp;
@@ -363,6 +408,9 @@ PlocWorkspace carve(void *base, uint32_t n) {
     w.pos = (uint32_t *)take((size_t)n * 4);
     w.level = (uint32_t *)take((size_t)n * 4);
     w.max_level = (uint32_t *)take(4);
+    w.state = (uint32_t *)take(4 * 4);
+    w.level_first = (uint32_t *)take((size_t)n * 4);
+    w.level_count = (uint32_t *)take((size_t)n * 4);
     w.cur = (pq_bvh_node *)take((size_t)n * sizeof(pq_bvh_node));
     w.next = (pq_bvh_node *)take((size_t)n * sizeof(pq_bvh_node));
     w.wide = (pq_bvh_node_wide *)take((size_t)n * sizeof(pq_bvh_node_wide));
@@ -406,39 +454,58 @@ hipError_t build_ploc_bvh(const float *aabbs, uint32_t n, pq_bvh_node_wide *out,
     hipLaunchKernelGGL(init_leaves_kernel, dim3(blocks(n)), dim3(kBlock), 0, stream, aabbs, w.idx_sorted, n, w.cur);
     if ((e = hipGetLastError()) != hipSuccess) return e;
 
-    // Merge levels; `levels` keeps the id range each level created.
+    // Merge levels, their length kept on the device: the host issues kLevelBatch
+    // levels at a time with grids sized by the last length it read back, and the
+    // kernels of a level past the end return at once.
+    constexpr int kLevelBatch = 6;
+    hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(64), 0, stream, w.state, n);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    pq_bvh_node *cur = w.cur, *next = w.next;
+    uint32_t bound = n;
+    uint32_t host_state[4] = {n, 1u, 0u, 0u};
+    while (host_state[0] > 1u) {
+        for (int k = 0; k < kLevelBatch; ++k) {
+            hipLaunchKernelGGL(candidates_kernel, dim3(blocks(bound)), dim3(kBlock), 0, stream, cur, w.state, w.cand);
+            hipLaunchKernelGGL(flags_kernel, dim3(blocks((uint64_t)bound + 1)), dim3(kBlock), 0, stream, w.cand,
+                               w.state, bound, w.flags);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            size_t cb = w.scan_bytes;
+            if ((e = rocprim::exclusive_scan(w.scan_temp, cb, w.flags, w.offs, 0ull, (size_t)bound + 1,
+                                             rocprim::plus<unsigned long long>(), stream)) != hipSuccess)
+                return e;
+            hipLaunchKernelGGL(merge_kernel, dim3(blocks(bound)), dim3(kBlock), 0, stream, cur, w.cand, w.offs,
+                               w.state, w.state + 1, next, w.wide);
+            hipLaunchKernelGGL(finish_level_kernel, dim3(1), dim3(64), 0, stream, w.offs, w.state, w.level_first,
+                               w.level_count);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            pq_bvh_node *t = cur;
+            cur = next;
+            next = t;
+        }
+        if ((e = hipMemcpyAsync(host_state, w.state, sizeof(host_state), hipMemcpyDeviceToHost, stream)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+        bound = host_state[0];
+    }
+    if (host_state[3] != 0u || host_state[1] != n - 1) return hipErrorUnknown;
+    const uint32_t nlev = host_state[2];
     struct Range {
         uint32_t first, count;
     };
-    std::vector<Range> levels;
-    pq_bvh_node *cur = w.cur, *next = w.next;
-    uint32_t L = n, base = 1;
-    while (L > 1) {
-        hipLaunchKernelGGL(candidates_kernel, dim3(blocks(L)), dim3(kBlock), 0, stream, cur, L, w.cand);
-        hipLaunchKernelGGL(flags_kernel, dim3(blocks((uint64_t)L + 1)), dim3(kBlock), 0, stream, w.cand, L, w.flags);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        size_t cb = w.scan_bytes;
-        if ((e = rocprim::exclusive_scan(w.scan_temp, cb, w.flags, w.offs, 0ull, (size_t)L + 1,
-                                         rocprim::plus<unsigned long long>(), stream)) != hipSuccess)
-            return e;
-        hipLaunchKernelGGL(merge_kernel, dim3(blocks(L)), dim3(kBlock), 0, stream, cur, w.cand, w.offs, L, base, next,
-                           w.wide);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        unsigned long long tot = 0;
-        if ((e = hipMemcpyAsync(&tot, w.offs + L, sizeof(tot), hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
-        const uint32_t kept = (uint32_t)(tot >> 32), merges = (uint32_t)(tot & 0xffffffffull);
-        if (merges == 0 || kept >= L) return hipErrorUnknown;  // cannot happen: a mutual pair always exists
-        if (L != 2) {
-            levels.push_back(Range{base, merges});
-            base += merges;
+    std::vector<Range> levels(nlev);
+    {
+        std::vector<uint32_t> firsts(nlev), counts(nlev);
+        if (nlev) {
+            if ((e = hipMemcpyAsync(firsts.data(), w.level_first, nlev * 4, hipMemcpyDeviceToHost, stream)) !=
+                    hipSuccess ||
+                (e = hipMemcpyAsync(counts.data(), w.level_count, nlev * 4, hipMemcpyDeviceToHost, stream)) !=
+                    hipSuccess ||
+                (e = hipStreamSynchronize(stream)) != hipSuccess)
+                return e;
         }
-        L = kept;
-        pq_bvh_node *t = cur;
-        cur = next;
-        next = t;
+        for (uint32_t l = 0; l < nlev; ++l) levels[l] = Range{firsts[l], counts[l]};
     }
-    if (base != n - 1) return hipErrorUnknown;  // n - 1 wide nodes: the root and base - 1 others
 
     // Depth-first relayout, from the root down: newer levels first.
     hipLaunchKernelGGL(root_kernel, dim3(1), dim3(64), 0, stream, w.pos, w.level);
