@@ -16,11 +16,11 @@
 //      node its place and each merge its id; the level's length stays on the
 //      device (a one-lane kernel ends each level), and the host reads it back
 //      every few levels to size the grids of the next ones;
-//   4. the depth-first relayout: positions are handed down from the root, the
-//      levels taken newest first (a node is always newer than its children):
-//      left child at parent + 1, right child after the left subtree's
-//      leaf count - 1 wide nodes; then one scatter writes every wide node at its
-//      position with its children renamed and refit's flags (every slot
+//   4. the depth-first relayout: every wide node finds its position on its
+//      path to the root (parent links recorded by the merges): a left child
+//      sits right after its parent, a right child after the left subtree's
+//      leaf count - 1 wide nodes; then one scatter writes every wide node at
+//      its position with its children renamed and refit's flags (every slot
 //      `changed`, bvh_tree.rs:192-198).
 // Every float operation is parry's, in its order and precision, so the tree is
 // the same bit for bit.
@@ -241,7 +241,8 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(const pq_bvh_node *__rest
                                                        const uint32_t *__restrict__ level_len,
                                                        const uint32_t *__restrict__ next_id,
                                                        pq_bvh_node *__restrict__ next,
-                                                       pq_bvh_node_wide *__restrict__ wide) {
+                                                       pq_bvh_node_wide *__restrict__ wide,
+                                                       uint32_t *__restrict__ parent) {
     const uint32_t L = *level_len;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (L <= 1u || i >= L) return;
@@ -261,6 +262,9 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(const pq_bvh_node *__rest
     w.left = a;
     w.right = b;
     wide[id] = w;
+    // parent links of the wide children: parent wide id * 2 + side
+    if ((a.data & kCountMask) > 1u) parent[a.children] = 2u * id;
+    if ((b.data & kCountMask) > 1u) parent[b.children] = 2u * id + 1u;
     if (L == 2) return;
     pq_bvh_node m;  // BvhNode::merged (bvh_tree.rs:610-618, 200-204)
     for (int d = 0; d < 3; ++d) {
@@ -272,11 +276,10 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(const pq_bvh_node *__rest
     next[pos] = m;
 }
 
-// Ends a level: records the id range it created (the root's level records
-// nothing), advances the next free id and sets the next level's length.
-// `state`: [0] level length, [1] next free wide id, [2] levels recorded, [3] error.
-__global__ void finish_level_kernel(const unsigned long long *__restrict__ offs, uint32_t *__restrict__ state,
-                                    uint32_t *__restrict__ level_first, uint32_t *__restrict__ level_count) {
+// Ends a level: advances the next free wide id past the level's merges (the
+// root, merged last, is id 0) and sets the next level's length.
+// `state`: [0] level length, [1] next free wide id, [2] levels done, [3] error.
+__global__ void finish_level_kernel(const unsigned long long *__restrict__ offs, uint32_t *__restrict__ state) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const uint32_t L = state[0];
     if (L <= 1u) return;
@@ -287,12 +290,8 @@ __global__ void finish_level_kernel(const unsigned long long *__restrict__ offs,
         state[0] = 1u;
         return;
     }
-    if (L != 2u) {
-        level_first[state[2]] = state[1];
-        level_count[state[2]] = merges;
-        state[1] += merges;
-        state[2] += 1u;
-    }
+    if (L != 2u) state[1] += merges;
+    state[2] += 1u;
     state[0] = kept;
 }
 
@@ -305,26 +304,29 @@ __global__ void init_state_kernel(uint32_t *state, uint32_t n) {
     }
 }
 
-// Depth-first positions of the children of the wide nodes [first, first + count).
-__global__ __launch_bounds__(kBlock) void positions_kernel(const pq_bvh_node_wide *__restrict__ wide, uint32_t first,
-                                                           uint32_t count, uint32_t *__restrict__ pos,
-                                                           uint32_t *__restrict__ level,
+// Depth-first position and level of every wide node, from the path to the root:
+// a left child sits right after its parent, a right child after its parent and
+// the left subtree's wide nodes (its leaf count - 1, none for a leaf).
+__global__ __launch_bounds__(kBlock) void positions_kernel(const pq_bvh_node_wide *__restrict__ wide, uint32_t total,
+                                                           const uint32_t *__restrict__ parent,
+                                                           uint32_t *__restrict__ pos,
                                                            uint32_t *__restrict__ max_level) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= count) return;
-    const uint32_t w = first + t;
-    const pq_bvh_node_wide nd = wide[w];
-    const uint32_t p = pos[w], lv = level[w];
+    const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+    if (w >= total) return;
+    uint32_t p = 0, lv = 1, cur = w;
+    while (cur != 0u && lv <= total) {  // at most total - 1 steps up
+        const uint32_t link = parent[cur], par = link >> 1;
+        uint32_t skip = 0;
+        if (link & 1u) {
+            const uint32_t lc = wide[par].left.data & kCountMask;
+            skip = lc > 1u ? lc - 1u : 0u;
+        }
+        p += 1u + skip;
+        ++lv;
+        cur = par;
+    }
+    pos[w] = p;
     atomicMax(max_level, lv);
-    const uint32_t lc = nd.left.data & kCountMask, rc = nd.right.data & kCountMask;
-    if (lc > 1) {
-        pos[nd.left.children] = p + 1;
-        level[nd.left.children] = lv + 1;
-    }
-    if (rc > 1) {
-        pos[nd.right.children] = p + 1 + (lc > 1 ? lc - 1 : 0);
-        level[nd.right.children] = lv + 1;
-    }
 }
 
 __device__ __forceinline__ pq_bvh_node relaid(pq_bvh_node c, const uint32_t *pos) {
@@ -344,13 +346,6 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(const pq_bvh_node_wide 
     r.left = relaid(nd.left, pos);
     r.right = relaid(nd.right, pos);
     out[pos[w]] = r;
-}
-
-__global__ void root_kernel(uint32_t *pos, uint32_t *level) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        pos[0] = 0u;
-        level[0] = 1u;
-    }
 }
 
 // One or two leaves: no rebuild, no refit (bvh_tree.rs:1914-1932).
@@ -380,7 +375,7 @@ uint32_t blocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 struct PlocWorkspace {
     Box6 *partial;
     unsigned long long *keys, *keys_sorted, *flags, *offs;
-    uint32_t *idx, *idx_sorted, *cand, *pos, *level, *max_level, *state, *level_first, *level_count;
+    uint32_t *idx, *idx_sorted, *cand, *pos, *parent, *max_level, *state;
     pq_bvh_node *cur, *next;
     pq_bvh_node_wide *wide;
     void *sort_temp, *scan_temp;
@@ -406,11 +401,9 @@ PlocWorkspace carve(void *base, uint32_t n) {
     w.idx_sorted = (uint32_t *)take((size_t)n * 4);
     w.cand = (uint32_t *)take((size_t)n * 4);
     w.pos = (uint32_t *)take((size_t)n * 4);
-    w.level = (uint32_t *)take((size_t)n * 4);
+    w.parent = (uint32_t *)take((size_t)n * 4);
     w.max_level = (uint32_t *)take(4);
     w.state = (uint32_t *)take(4 * 4);
-    w.level_first = (uint32_t *)take((size_t)n * 4);
-    w.level_count = (uint32_t *)take((size_t)n * 4);
     w.cur = (pq_bvh_node *)take((size_t)n * sizeof(pq_bvh_node));
     w.next = (pq_bvh_node *)take((size_t)n * sizeof(pq_bvh_node));
     w.wide = (pq_bvh_node_wide *)take((size_t)n * sizeof(pq_bvh_node_wide));
@@ -474,9 +467,8 @@ hipError_t build_ploc_bvh(const float *aabbs, uint32_t n, pq_bvh_node_wide *out,
                                              rocprim::plus<unsigned long long>(), stream)) != hipSuccess)
                 return e;
             hipLaunchKernelGGL(merge_kernel, dim3(blocks(bound)), dim3(kBlock), 0, stream, cur, w.cand, w.offs,
-                               w.state, w.state + 1, next, w.wide);
-            hipLaunchKernelGGL(finish_level_kernel, dim3(1), dim3(64), 0, stream, w.offs, w.state, w.level_first,
-                               w.level_count);
+                               w.state, w.state + 1, next, w.wide, w.parent);
+            hipLaunchKernelGGL(finish_level_kernel, dim3(1), dim3(64), 0, stream, w.offs, w.state);
             if ((e = hipGetLastError()) != hipSuccess) return e;
             pq_bvh_node *t = cur;
             cur = next;
@@ -489,32 +481,9 @@ hipError_t build_ploc_bvh(const float *aabbs, uint32_t n, pq_bvh_node_wide *out,
         bound = host_state[0];
     }
     if (host_state[3] != 0u || host_state[1] != n - 1) return hipErrorUnknown;
-    const uint32_t nlev = host_state[2];
-    struct Range {
-        uint32_t first, count;
-    };
-    std::vector<Range> levels(nlev);
-    {
-        std::vector<uint32_t> firsts(nlev), counts(nlev);
-        if (nlev) {
-            if ((e = hipMemcpyAsync(firsts.data(), w.level_first, nlev * 4, hipMemcpyDeviceToHost, stream)) !=
-                    hipSuccess ||
-                (e = hipMemcpyAsync(counts.data(), w.level_count, nlev * 4, hipMemcpyDeviceToHost, stream)) !=
-                    hipSuccess ||
-                (e = hipStreamSynchronize(stream)) != hipSuccess)
-                return e;
-        }
-        for (uint32_t l = 0; l < nlev; ++l) levels[l] = Range{firsts[l], counts[l]};
-    }
-
-    // Depth-first relayout, from the root down: newer levels first.
-    hipLaunchKernelGGL(root_kernel, dim3(1), dim3(64), 0, stream, w.pos, w.level);
     if ((e = hipMemsetAsync(w.max_level, 0, sizeof(uint32_t), stream)) != hipSuccess) return e;
-    hipLaunchKernelGGL(positions_kernel, dim3(1), dim3(kBlock), 0, stream, w.wide, 0u, 1u, w.pos, w.level,
+    hipLaunchKernelGGL(positions_kernel, dim3(blocks(n - 1)), dim3(kBlock), 0, stream, w.wide, n - 1, w.parent, w.pos,
                        w.max_level);
-    for (size_t l = levels.size(); l-- > 0;)
-        hipLaunchKernelGGL(positions_kernel, dim3(blocks(levels[l].count)), dim3(kBlock), 0, stream, w.wide,
-                           levels[l].first, levels[l].count, w.pos, w.level, w.max_level);
     hipLaunchKernelGGL(scatter_kernel, dim3(blocks(n - 1)), dim3(kBlock), 0, stream, w.wide, n - 1, w.pos, out);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(depth, w.max_level, sizeof(uint32_t), hipMemcpyDeviceToHost, stream)) != hipSuccess)
