@@ -348,6 +348,27 @@ int pq_contact_get_tuning(pq_contact_tuning *tuning);
 /* Device workspace bytes pq_contact_pairs needs for `num_pairs` pairs. */
 size_t pq_contact_workspace_bytes(uint64_t num_pairs);
 
+/* ---- ConvexPolyhedron faces (for contact manifolds) --------------------------- */
+
+/* One face of a ConvexPolyhedron as parry stores it (convex_polyhedron.rs:59-63,
+ * 172-185): its unit normal, and the range [first, first + count) of
+ * face_vertices / face_edges (vertices_adj_to_face, edges_adj_to_face) that lists
+ * its vertices (indices into the polyhedron's own points) and edges in order. */
+typedef struct pq_polyhedron_face {
+    float normal[3];
+    uint32_t first;
+    uint32_t count;
+} pq_polyhedron_face;
+
+/* Gives the set's ConvexPolyhedra their faces: shape_faces holds 2 uint32 per
+ * shape (first face in `faces`, face count; count 0 = no faces, and for
+ * non-polyhedra it must be 0).  Only contact manifolds read the faces (a
+ * ConvexPolyhedron without faces has no feature map here).  Synchronous; replaces
+ * faces given before. */
+int pq_shapes_set_polyhedron_faces(pq_shapes *shapes, const pq_polyhedron_face *faces, uint32_t num_faces,
+                                   const uint32_t *face_vertices, const uint32_t *face_edges, uint32_t num_entries,
+                                   const uint32_t *shape_faces);
+
 /* ---- Contact manifolds between polygonal feature maps ------------------------ */
 
 /* TrackedContact (src/query/contact_manifolds/contact_manifold.rs:16-60): the
@@ -382,8 +403,8 @@ typedef struct pq_contact_manifold {
  * (no points, zero cached normal: the first contact of a pair, or every contact
  * whose manifold update fails) for every (i, j) of `pairs`.  Feature maps
  * (shape.rs as_polygonal_feature_map): Cuboid, Cone and Triangle are their own; a
- * Capsule is its segment with the radius as border; Ball and ConvexPolyhedron
- * have none here.  parry's dispatcher sends Cuboid-Cuboid, Capsule-Capsule and
+ * Capsule is its segment with the radius as border; a ConvexPolyhedron is its own
+ * once pq_shapes_set_polyhedron_faces gave it faces; a Ball has none.  parry's dispatcher sends Cuboid-Cuboid, Capsule-Capsule and
  * Cuboid-Triangle to other routines; this call runs the feature-map routine on
  * whatever pairs it is given.
  *   poses: device, one pq_isometry per shape; pairs: device, num_pairs x (i, j);

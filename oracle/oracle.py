@@ -43,6 +43,8 @@ _SIG = {
     "orc_trimesh_contact_shapes_f64": (None, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _f64, _vp, _vp, _vp, _vp, _int]),
     "orc_manifolds_pfm_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp, _int]),
     "orc_manifolds_pfm_f64": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f64, _vp, _vp, _vp, _vp, _vp, _int]),
+    "orc_manifolds_pfm_faces_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp, _int,
+                                           _vp, _vp, _vp, _u32, _vp, _vp, _vp]),
     "orc_bvh_create_ploc_f32": (_vp, [_vp, _u32]),
     "orc_ploc_keys_f32": (None, [_vp, _u32, _vp]),
     "orc_morton_encode_unorm": (_u64, [_f64, _f64, _f64]),
@@ -420,7 +422,8 @@ def morton_encode_unorm(x: float, y: float, z: float) -> int:
 MANIFOLD_POINTS = 25
 
 
-def manifolds_pfm(descs, vertices, poses, pairs, prediction: float, f64: bool = False, threads: int | None = None):
+def manifolds_pfm(descs, vertices, poses, pairs, prediction: float, f64: bool = False, threads: int | None = None,
+                  faces=None):
     """contact_manifold_pfm_pfm (contact_manifolds_pfm_pfm.rs:44-162) on a fresh manifold
     for every pair.  Returns dict(status int32 [m] (1 ClosestPoints, 0 NoIntersection,
     -1 no feature map), n1, n2 [m,3], count uint32 [m], p1, p2 [m,25,3], dist [m,25],
@@ -438,9 +441,25 @@ def manifolds_pfm(descs, vertices, poses, pairs, prediction: float, f64: bool = 
     counts = np.zeros(m, np.uint32)
     pts = np.zeros((m, MANIFOLD_POINTS, 7), dt)
     fids = np.zeros((m, MANIFOLD_POINTS, 2), np.uint32)
-    f = lib().orc_manifolds_pfm_f64 if f64 else lib().orc_manifolds_pfm_f32
-    f(d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data, pr.ctypes.data, m,
-      prediction, status.ctypes.data, normals.ctypes.data, counts.ctypes.data, pts.ctypes.data, fids.ctypes.data,
-      default_threads() if threads is None else threads)
+    th = default_threads() if threads is None else threads
+    if faces is not None:  # ConvexPolyhedron faces, as scenes.polyhedron_faces returns them (f32 only)
+        assert not f64
+        fc = faces["faces"]
+        nrm = np.ascontiguousarray(fc["normal"], np.float32)
+        ff = np.ascontiguousarray(fc["first"], np.uint32)
+        fn = np.ascontiguousarray(fc["count"], np.uint32)
+        fv = np.ascontiguousarray(faces["face_vertices"], np.uint32)
+        fe = np.ascontiguousarray(faces["face_edges"], np.uint32)
+        sf = np.ascontiguousarray(faces["shape_faces"], np.uint32)
+        lib().orc_manifolds_pfm_faces_f32(
+            d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data, pr.ctypes.data,
+            m, prediction, status.ctypes.data, normals.ctypes.data, counts.ctypes.data, pts.ctypes.data,
+            fids.ctypes.data, th, nrm.ctypes.data, ff.ctypes.data, fn.ctypes.data, len(fc), fv.ctypes.data,
+            fe.ctypes.data, sf.ctypes.data)
+    else:
+        f = lib().orc_manifolds_pfm_f64 if f64 else lib().orc_manifolds_pfm_f32
+        f(d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data, pr.ctypes.data,
+          m, prediction, status.ctypes.data, normals.ctypes.data, counts.ctypes.data, pts.ctypes.data,
+          fids.ctypes.data, th)
     return {"status": status, "n1": normals[:, :3], "n2": normals[:, 3:], "count": counts, "p1": pts[:, :, :3],
             "p2": pts[:, :, 3:6], "dist": pts[:, :, 6], "fid1": fids[:, :, 0], "fid2": fids[:, :, 1]}

@@ -591,10 +591,26 @@ constexpr int kManifoldPoints = 25;
 template <class T>
 void manifolds_pfm(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uint32_t nverts, const T *poses,
                    const uint32_t *pairs, uint64_t m, T prediction, int32_t *status, T *normals, uint32_t *counts,
-                   T *points, uint32_t *fids, int nthreads) {
+                   T *points, uint32_t *fids, int nthreads, const float *face_normals = nullptr,
+                   const uint32_t *face_first = nullptr, const uint32_t *face_count = nullptr, uint32_t nfaces = 0,
+                   const uint32_t *face_verts = nullptr, const uint32_t *face_edges = nullptr,
+                   const uint32_t *shape_faces = nullptr) {
     std::vector<V3<T>> pts(nverts);
     for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
     std::vector<Shape<T>> shapes = make_shapes(descs, nshapes, pts);
+    std::vector<PolyFace<T>> faces(nfaces);
+    for (uint32_t f = 0; f < nfaces; ++f)
+        faces[f] = PolyFace<T>{v3((T)face_normals[3 * f], (T)face_normals[3 * f + 1], (T)face_normals[3 * f + 2]),
+                               face_first[f], face_count[f]};
+    if (shape_faces) {
+        for (uint32_t i = 0; i < nshapes; ++i) {
+            if (shapes[i].type != CONVEX_POLYHEDRON || shape_faces[2 * i + 1] == 0) continue;
+            shapes[i].faces = faces.data() + shape_faces[2 * i];
+            shapes[i].nfaces = shape_faces[2 * i + 1];
+            shapes[i].fverts = face_verts;
+            shapes[i].fedges = face_edges;
+        }
+    }
     parallel_for(m, nthreads, 64, [&](uint64_t b, uint64_t e) {
         Manifold<T> mf;
         for (uint64_t k = b; k < e; ++k) {
@@ -639,6 +655,18 @@ void orc_manifolds_pfm_f32(const void *descs, uint32_t nshapes, const float *ver
                            float *normals, uint32_t *counts, float *points, uint32_t *fids, int nthreads) {
     manifolds_pfm<float>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, status,
                          normals, counts, points, fids, nthreads);
+}
+// The same with ConvexPolyhedron faces: nfaces faces (normal, first, count into
+// face_verts / face_edges) and 2 per shape in shape_faces (first face, count).
+void orc_manifolds_pfm_faces_f32(const void *descs, uint32_t nshapes, const float *verts, uint32_t nverts,
+                                 const float *poses, const uint32_t *pairs, uint64_t m, float prediction,
+                                 int32_t *status, float *normals, uint32_t *counts, float *points, uint32_t *fids,
+                                 int nthreads, const float *face_normals, const uint32_t *face_first,
+                                 const uint32_t *face_count, uint32_t nfaces, const uint32_t *face_verts,
+                                 const uint32_t *face_edges, const uint32_t *shape_faces) {
+    manifolds_pfm<float>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, status,
+                         normals, counts, points, fids, nthreads, face_normals, face_first, face_count, nfaces,
+                         face_verts, face_edges, shape_faces);
 }
 void orc_manifolds_pfm_f64(const void *descs, uint32_t nshapes, const double *verts, uint32_t nverts,
                            const double *poses, const uint32_t *pairs, uint64_t m, double prediction, int32_t *status,

@@ -41,11 +41,22 @@ enum ShapeType : uint32_t {
     BALL = 0, CUBOID = 1, CAPSULE = 2, CONE = 3, CONVEX_POLYHEDRON = 4, TRIANGLE = 5, ORIGIN = 6, SEGMENT = 7
 };
 
+// A ConvexPolyhedron face (convex_polyhedron.rs:59-63): normal and the range of
+// its vertices and edges in the face lists.
+template <class T> struct PolyFace {
+    V3<T> normal;
+    uint32_t first, count;
+};
+
 template <class T> struct Shape {
     uint32_t type;
     const V3<T> *pts;  // ConvexPolyhedron vertices
     uint32_t n;
     T p[4];  // Ball: r; Cuboid: half extents; Capsule: half_height, radius; Cone: half_height, radius
+    // ConvexPolyhedron faces (contact manifolds only): nfaces == 0 means none.
+    const PolyFace<T> *faces = nullptr;
+    uint32_t nfaces = 0;
+    const uint32_t *fverts = nullptr, *fedges = nullptr;
 };
 
 template <class T> inline T copy_sign(T from, T to) { return std::copysign(to, from); }

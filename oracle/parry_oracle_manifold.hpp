@@ -13,8 +13,8 @@
 // utils/wops.rs:92-110) follow the reference's operation order.  Same
 // conventions as parry_oracle.hpp: T = float is parry3d, T = double parry3d-f64.
 // A Capsule's feature map is its segment with the capsule radius as border
-// (shape.rs:855-857); a Ball has none; a ConvexPolyhedron's needs the face
-// topology, which the shape records here do not carry.
+// (shape.rs:855-857); a Ball has none; a ConvexPolyhedron's uses the faces given
+// with it (convex_polyhedron.rs:959-992).
 #pragma once
 
 #include <cmath>
@@ -205,6 +205,37 @@ template <class T> PolyFeature<T> cone_feature(T h, T r, V3<T> d) {
     return f;
 }
 
+// ConvexPolyhedron's feature map (convex_polyhedron.rs:959-992): the first face
+// with the largest normal . dir, its first four vertices and edges.
+template <class T> PolyFeature<T> polyhedron_feature(const Shape<T> &s, V3<T> d) {
+    PolyFeature<T> f{};
+    uint32_t best = 0;
+    T best_dot = dot(s.faces[0].normal, d);
+    for (uint32_t k = 1; k < s.nfaces; ++k) {
+        T nd = dot(s.faces[k].normal, d);
+        if (nd > best_dot) {
+            best = k;
+            best_dot = nd;
+        }
+    }
+    const PolyFace<T> &F = s.faces[best];
+    const int nv = F.count < 4u ? (int)F.count : 4;
+    for (int k = 0; k < 4; ++k) {
+        if (k < nv) {
+            const uint32_t vid = s.fverts[F.first + k];
+            f.v[k] = s.pts[vid];
+            f.vids[k] = pf_vertex(vid);
+            f.eids[k] = pf_edge(s.fedges[F.first + k]);
+        } else {
+            f.v[k] = v3(T(0), T(0), T(0));
+            f.vids[k] = f.eids[k] = PF_UNKNOWN;
+        }
+    }
+    f.fid = pf_face(best);
+    f.n = nv;
+    return f;
+}
+
 // The feature map and border radius of a shape (shape.rs as_polygonal_feature_map).
 // Returns false for shapes without one here.
 template <class T> bool pfm_of(const Shape<T> &s, Shape<T> &pfm, T &border) {
@@ -215,6 +246,8 @@ template <class T> bool pfm_of(const Shape<T> &s, Shape<T> &pfm, T &border) {
         case CONE:
         case TRIANGLE:
             return true;
+        case CONVEX_POLYHEDRON:
+            return s.nfaces > 0;
         case CAPSULE:
             pfm.type = SEGMENT;
             border = s.p[1];
@@ -226,6 +259,7 @@ template <class T> bool pfm_of(const Shape<T> &s, Shape<T> &pfm, T &border) {
 
 template <class T> PolyFeature<T> support_feature(const Shape<T> &pfm, V3<T> d) {
     switch (pfm.type) {
+        case CONVEX_POLYHEDRON: return polyhedron_feature(pfm, d);
         case CUBOID: return cuboid_feature(v3(pfm.p[0], pfm.p[1], pfm.p[2]), d);
         case CONE: return cone_feature(pfm.p[0], pfm.p[1], d);
         case TRIANGLE: return triangle_feature(pfm.pts);

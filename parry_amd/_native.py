@@ -65,6 +65,7 @@ SIGNATURES = {
     "pq_graph_destroy": (_int, [_vp]),
     "pq_shapes_create": (_int, [_vp, _u32, _vp, _u32, _c.POINTER(_vp)]),
     "pq_shapes_destroy": (_int, [_vp]),
+    "pq_shapes_set_polyhedron_faces": (_int, [_vp, _vp, _u32, _vp, _vp, _u32, _vp]),
     "pq_shapes_compute_aabbs": (_int, [_vp, _vp, _vp, _vp]),
     "pq_bvh_create": (_int, [_vp, _u32, _c.POINTER(_vp)]),
     "pq_bvh_destroy": (_int, [_vp]),

@@ -81,7 +81,63 @@ int pq_shapes_destroy(pq_shapes *s) {
     if (!s) return PQ_OK;
     if (s->d_rec) (void)hipFree(s->d_rec);
     if (s->d_verts) (void)hipFree(s->d_verts);
+    if (s->d_faces) (void)hipFree(s->d_faces);
     delete s;
+    return PQ_OK;
+}
+
+int pq_shapes_set_polyhedron_faces(pq_shapes *s, const pq_polyhedron_face *faces, uint32_t num_faces,
+                                   const uint32_t *face_vertices, const uint32_t *face_edges, uint32_t num_entries,
+                                   const uint32_t *shape_faces) {
+    if (!s || !shape_faces) return fail(PQ_ERR_INVALID, "pq_shapes_set_polyhedron_faces: null shapes or ranges");
+    if ((num_faces && !faces) || (num_entries && (!face_vertices || !face_edges)))
+        return fail(PQ_ERR_INVALID, "pq_shapes_set_polyhedron_faces: null face tables");
+    const uint32_t n = s->dev.n;
+    std::vector<ShapeRecord> rec = s->host_rec;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t first = shape_faces[2 * i], count = shape_faces[2 * i + 1];
+        if (rec[i].type != PQ_SHAPE_CONVEX_POLYHEDRON) {
+            if (count != 0)
+                return fail(PQ_ERR_INVALID, "pq_shapes_set_polyhedron_faces: shape " + std::to_string(i) +
+                                                " is not a ConvexPolyhedron");
+            continue;
+        }
+        if ((uint64_t)first + count > num_faces)
+            return fail(PQ_ERR_INVALID, "pq_shapes_set_polyhedron_faces: shape " + std::to_string(i) +
+                                            " has an out-of-range face range");
+        for (uint32_t f = first; f < first + count; ++f) {
+            const pq_polyhedron_face &F = faces[f];
+            if (F.count < 3 || (uint64_t)F.first + F.count > num_entries)
+                return fail(PQ_ERR_INVALID, "pq_shapes_set_polyhedron_faces: face " + std::to_string(f) +
+                                                " has fewer than 3 vertices or an out-of-range list");
+            for (uint32_t k = F.first; k < F.first + F.count; ++k)
+                if (face_vertices[k] >= rec[i].num_vertices)
+                    return fail(PQ_ERR_INVALID, "pq_shapes_set_polyhedron_faces: face " + std::to_string(f) +
+                                                    " names a vertex the polyhedron does not have");
+        }
+        std::memcpy(&rec[i].param[0], &first, 4);
+        std::memcpy(&rec[i].param[1], &count, 4);
+    }
+    const size_t fb = ((size_t)num_faces * sizeof(pq_polyhedron_face) + 255) / 256 * 256;
+    const size_t eb = ((size_t)num_entries * sizeof(uint32_t) + 255) / 256 * 256;
+    void *buf = nullptr;
+    hipError_t e = hipMalloc(&buf, fb + 2 * eb + 256);
+    if (e == hipSuccess && num_faces) e = hipMemcpy(buf, faces, num_faces * sizeof(pq_polyhedron_face), hipMemcpyHostToDevice);
+    if (e == hipSuccess && num_entries)
+        e = hipMemcpy((char *)buf + fb, face_vertices, num_entries * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && num_entries)
+        e = hipMemcpy((char *)buf + fb + eb, face_edges, num_entries * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n) e = hipMemcpy(s->d_rec, rec.data(), n * sizeof(ShapeRecord), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (buf) (void)hipFree(buf);
+        return hip_fail(e, "pq_shapes_set_polyhedron_faces");
+    }
+    if (s->d_faces) (void)hipFree(s->d_faces);
+    s->d_faces = buf;
+    s->host_rec = rec;
+    s->dev.faces = (const pq_polyhedron_face *)buf;
+    s->dev.face_verts = (const uint32_t *)((char *)buf + fb);
+    s->dev.face_edges = (const uint32_t *)((char *)buf + fb + eb);
     return PQ_OK;
 }
 

@@ -59,11 +59,12 @@ __device__ __forceinline__ bool is_local_mode(uint32_t mode) {
 }
 // A shape's polygonal feature map and border radius (shape.rs as_polygonal_feature_map):
 // Cuboid, Cone and Triangle are their own, a Capsule is its segment with the radius as
-// border.  False for shapes without one here (Ball; ConvexPolyhedron, whose map needs
-// face topology the shape records do not carry).
+// border, a ConvexPolyhedron is its own when it has faces (its face count, as uint32
+// bits in param[1], is not zero).  False for shapes without one here (Ball).
 __device__ __forceinline__ bool feature_map(ShapeRef &g, float &border) {
     border = 0.0f;
     if (g.type == PQ_SHAPE_CUBOID || g.type == PQ_SHAPE_CONE || g.type == PQ_SHAPE_TRIANGLE) return true;
+    if (g.type == PQ_SHAPE_CONVEX_POLYHEDRON) return __float_as_uint(g.p1) != 0u;
     if (g.type == PQ_SHAPE_CAPSULE) {
         g.type = kShapeSegment;
         border = g.p1;

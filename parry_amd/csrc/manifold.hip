@@ -9,7 +9,8 @@
 //      radii, results in local frames;
 //   2. one lane per pair: the support features of both maps along the contact
 //      normal (polygonal_feature_map.rs:30-154, cuboid.rs:267-352,
-//      triangle.rs:280-282), their clipping (PolygonalFeature::contacts,
+//      triangle.rs:280-282, convex_polyhedron.rs:959-992 from the faces given
+//      by pq_shapes_set_polyhedron_faces), their clipping (PolygonalFeature::contacts,
 //      polygonal_feature3d.rs:74-381: edge-edge or face-face in the plane of the
 //      normal's orthonormal basis, utils/wops.rs:96-109), GJK's own point last,
 //      and the border radii applied.
@@ -155,8 +156,44 @@ __device__ void cone_feature(float h, float r, f3 d, Feature &f) {
     }
 }
 
-__device__ void support_feature(const ShapeRef &g, f3 d, Feature &f) {
-    if (g.type == PQ_SHAPE_CUBOID) cuboid_feature(f3{g.p0, g.p1, g.p2}, d, f);
+// ConvexPolyhedron's feature map (convex_polyhedron.rs:959-992): the face whose
+// normal has the largest dot product with dir (the first on ties), its first four
+// vertices and edges.
+__device__ void polyhedron_feature(const ShapeRef &g, const pq_polyhedron_face *faces, const uint32_t *face_verts,
+                                   const uint32_t *face_edges, f3 d, Feature &f) {
+    const uint32_t first = __float_as_uint(g.p0), count = __float_as_uint(g.p1);
+    uint32_t best = 0;
+    float best_dot = dot3(f3{faces[first].normal[0], faces[first].normal[1], faces[first].normal[2]}, d);
+    for (uint32_t k = 1; k < count; ++k) {
+        const pq_polyhedron_face &F = faces[first + k];
+        const float nd = dot3(f3{F.normal[0], F.normal[1], F.normal[2]}, d);
+        if (nd > best_dot) {
+            best = k;
+            best_dot = nd;
+        }
+    }
+    const pq_polyhedron_face F = faces[first + best];
+    const int nv = F.count < 4u ? (int)F.count : 4;
+    for (int k = 0; k < 4; ++k) {
+        if (k < nv) {
+            const uint32_t vid = face_verts[F.first + k];
+            const float4 p = g.v[vid];
+            f.v[k] = f3{p.x, p.y, p.z};
+            f.vids[k] = kPfVertex | vid;
+            f.eids[k] = kPfEdge | face_edges[F.first + k];
+        } else {  // left as PolygonalFeature::default() has them
+            f.v[k] = f3{0.0f, 0.0f, 0.0f};
+            f.vids[k] = 0u;
+            f.eids[k] = 0u;
+        }
+    }
+    f.fid = kPfFace | best;
+    f.n = nv;
+}
+
+__device__ void support_feature(const ShapeRef &g, const DeviceShapes &s, f3 d, Feature &f) {
+    if (g.type == PQ_SHAPE_CONVEX_POLYHEDRON) polyhedron_feature(g, s.faces, s.face_verts, s.face_edges, d, f);
+    else if (g.type == PQ_SHAPE_CUBOID) cuboid_feature(f3{g.p0, g.p1, g.p2}, d, f);
     else if (g.type == PQ_SHAPE_CONE) cone_feature(g.p0, g.p1, d, f);
     else if (g.type == PQ_SHAPE_TRIANGLE) triangle_feature(g, f);
     else segment_feature(g.p0, f);
@@ -389,9 +426,7 @@ __device__ void contacts_face_face(const iso &pos12, const Feature &f1, f3 sep, 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void manifold_kernel(const ShapeRecord *__restrict__ rec,
-                                                          const float4 *__restrict__ verts,
-                                                          const float *__restrict__ poses,
+__global__ __launch_bounds__(kBlock) void manifold_kernel(const DeviceShapes s, const float *__restrict__ poses,
                                                           const uint32_t *__restrict__ pairs, uint64_t m,
                                                           const pq_contact *__restrict__ contacts,
                                                           pq_contact_manifold *__restrict__ out) {
@@ -404,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void manifold_kernel(const ShapeRecord *__r
     if (c.has_contact == 1u) {
         const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
         const iso pos12 = iso_inv_mul(load_iso(poses + 7 * (size_t)i), load_iso(poses + 7 * (size_t)j));
-        ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
+        ShapeRef g1 = load_shape(s.rec, s.verts, i), g2 = load_shape(s.rec, s.verts, j);
         float br1 = 0.0f, br2 = 0.0f;
         if (g1.type == PQ_SHAPE_CAPSULE) {
             g1.type = kShapeSegment;
@@ -417,8 +452,8 @@ __global__ __launch_bounds__(kBlock) void manifold_kernel(const ShapeRecord *__r
         n1 = f3{c.normal1[0], c.normal1[1], c.normal1[2]};
         n2 = f3{c.normal2[0], c.normal2[1], c.normal2[2]};
         Feature f1, f2;
-        support_feature(g1, n1, f1);
-        support_feature(g2, n2, f2);
+        support_feature(g1, s, n1, f1);
+        support_feature(g2, s, n2, f2);
         if (f1.n == 2 && f2.n == 2) contacts_edge_edge(pos12, f1, n1, f2, o);
         else contacts_face_face(pos12, f1, n1, f2, o);
         // GJK's own point (contact_manifolds_pfm_pfm.rs:108-120)
@@ -475,8 +510,8 @@ hipError_t launch_manifolds(const DeviceShapes &s, const float *poses, const uin
     hipError_t e = launch_contacts(s, poses, pairs, m, nullptr, prediction, nullptr, w.contacts, nullptr, w.contact_ws,
                                    stream, kContactFeatureMaps);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(manifold_kernel, dim3((uint32_t)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, s.rec,
-                       s.verts, poses, pairs, m, w.contacts, out);
+    hipLaunchKernelGGL(manifold_kernel, dim3((uint32_t)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, s, poses,
+                       pairs, m, w.contacts, out);
     return hipGetLastError();
 }
 

@@ -123,6 +123,12 @@ struct DeviceShapes {
     const float4 *verts;     // device, xyz + pad
     uint32_t n;
     uint32_t num_vertices;
+    // ConvexPolyhedron faces (null until pq_shapes_set_polyhedron_faces); a
+    // polyhedron's record holds its first face and face count in param[0] and
+    // param[1] as uint32 bit patterns.
+    const pq_polyhedron_face *faces = nullptr;
+    const uint32_t *face_verts = nullptr;
+    const uint32_t *face_edges = nullptr;
 };
 
 hipError_t launch_shape_aabbs(const DeviceShapes &s, const float *poses, float *aabbs, hipStream_t stream);
@@ -206,6 +212,7 @@ struct pq_shapes {
     std::vector<pq::ShapeRecord> host_rec;
     void *d_rec = nullptr;
     void *d_verts = nullptr;
+    void *d_faces = nullptr;  // faces, face vertices, face edges (one allocation)
 };
 
 struct pq_bvh {

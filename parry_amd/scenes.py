@@ -432,3 +432,53 @@ def feature_map_pairs(n: int, seed: int, spread: float = 0.6, types=(1, 2, 3, 5)
     poses[1::2, 4:] = poses[0::2, 4:] + rng.normal(0, spread, (n, 3))
     pairs = np.stack([np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)], 1).astype(np.uint32)
     return d, verts, poses.astype(np.float32), pairs
+
+
+def polyhedron_faces(descs, vertices, merge_tol: float = 1e-6):
+    """Face tables of every ConvexPolyhedron in ``descs`` (others get no faces), in
+    the layout ``ConvexPolyhedron`` keeps them (convex_polyhedron.rs:172-185, 395-540):
+    one face per plane of the hull (coplanar hull triangles merged), its outward
+    unit normal, its vertices counter-clockwise seen from outside, and the edge
+    from its k-th to its (k+1)-th vertex at position k of ``face_edges``.  Edge ids
+    are per polyhedron, numbered as faces first meet them.
+
+    Returns the dict ``ShapeSet.set_polyhedron_faces`` takes.
+    """
+    from scipy.spatial import ConvexHull
+
+    from .shape import POLYHEDRON_FACE_DTYPE, SHAPE_CONVEX_POLYHEDRON
+
+    d = np.asarray(descs)
+    v = np.asarray(vertices, np.float64).reshape(-1, 3)
+    faces, fverts, fedges = [], [], []
+    shape_faces = np.zeros((d.shape[0], 2), np.uint32)
+    for i in np.nonzero(d["type"] == SHAPE_CONVEX_POLYHEDRON)[0]:
+        first, nv = int(d["first_vertex"][i]), int(d["num_vertices"][i])
+        pts = v[first:first + nv]
+        h = ConvexHull(pts)
+        # coplanar triangles of one hull face: same plane up to merge_tol
+        key = np.round(h.equations / merge_tol).astype(np.int64)
+        _, first_of, group = np.unique(key, axis=0, return_index=True, return_inverse=True)
+        group = group.reshape(-1)
+        shape_faces[i] = (len(faces), len(first_of))
+        edge_ids: dict = {}
+        for g in np.argsort(first_of, kind="stable"):  # faces in the order Qhull lists their planes
+            tri = h.simplices[group == g]
+            ids = np.unique(tri)
+            nrm = h.equations[first_of[g], :3]
+            c = pts[ids].mean(0)
+            a = pts[ids[0]] - c
+            a /= np.linalg.norm(a)
+            b = np.cross(nrm, a)
+            r = pts[ids] - c
+            ids = ids[np.argsort(np.arctan2(r @ b, r @ a), kind="stable")]
+            faces.append((nrm, len(fverts), len(ids)))
+            for k in range(len(ids)):
+                e = (min(ids[k], ids[(k + 1) % len(ids)]), max(ids[k], ids[(k + 1) % len(ids)]))
+                fverts.append(int(ids[k]))
+                fedges.append(edge_ids.setdefault(e, len(edge_ids)))
+    f = np.zeros(len(faces), POLYHEDRON_FACE_DTYPE)
+    for k, (nrm, fi, cnt) in enumerate(faces):
+        f[k] = (nrm.astype(np.float32), fi, cnt)
+    return {"faces": f, "face_vertices": np.asarray(fverts, np.uint32), "face_edges": np.asarray(fedges, np.uint32),
+            "shape_faces": shape_faces}

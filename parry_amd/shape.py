@@ -397,6 +397,11 @@ SHAPE_DESC_DTYPE = np.dtype(
 assert SHAPE_DESC_DTYPE.itemsize == 32
 
 
+# pq_polyhedron_face: a ConvexPolyhedron face (convex_polyhedron.rs:59-63, 172-185).
+POLYHEDRON_FACE_DTYPE = np.dtype([("normal", "<f4", 3), ("first", "<u4"), ("count", "<u4")])
+assert POLYHEDRON_FACE_DTYPE.itemsize == 20
+
+
 class ShapeSet:
     """A device-resident set of shapes (pq_shapes): type tags, parameters and the
     ConvexPolyhedron vertex pool."""
@@ -423,6 +428,23 @@ class ShapeSet:
 
     def __len__(self) -> int:
         return self.descs.shape[0]
+
+    def set_polyhedron_faces(self, faces: dict) -> None:
+        """Gives the ConvexPolyhedra their faces (pq_shapes_set_polyhedron_faces), the
+        tables ``ConvexPolyhedron`` keeps (convex_polyhedron.rs:172-185): a dict with
+        ``faces`` (POLYHEDRON_FACE_DTYPE), ``face_vertices`` and ``face_edges`` (uint32)
+        and ``shape_faces`` (uint32 [n, 2]: first face, face count; 0 faces for other
+        shapes), as ``scenes.polyhedron_faces`` builds them."""
+        f = np.ascontiguousarray(faces["faces"], dtype=POLYHEDRON_FACE_DTYPE)
+        fv = np.ascontiguousarray(faces["face_vertices"], dtype=np.uint32)
+        fe = np.ascontiguousarray(faces["face_edges"], dtype=np.uint32)
+        sf = np.ascontiguousarray(faces["shape_faces"], dtype=np.uint32).reshape(-1, 2)
+        if fv.shape != fe.shape or sf.shape[0] != len(self):
+            raise ValueError("set_polyhedron_faces: face_vertices/face_edges differ in length, or shape_faces "
+                             f"has {sf.shape[0]} rows for {len(self)} shapes")
+        _native.check(_native.lib().pq_shapes_set_polyhedron_faces(
+            self._h, _ptr(f), f.shape[0], _ptr(fv), _ptr(fe), fv.shape[0], _ptr(sf)))
+        self.faces = {"faces": f, "face_vertices": fv, "face_edges": fe, "shape_faces": sf}
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -7,7 +7,9 @@ oracle; local_n1, local_n2 and every point's local_p1, local_p2 and dist within
 manifolds is printed.  Cases: random mixes of Cuboid, Capsule, Cone and
 Triangle in contact (face-face, edge-edge and mixed clipping, EPA for the
 penetrating ones), resting stacks, separated pairs (the cached direction), and
-pairs with a Ball or a ConvexPolyhedron (unsupported).
+pairs with a Ball or a ConvexPolyhedron without faces (unsupported), and
+ConvexPolyhedra given their faces (random hulls, boxes) against each other and
+the other feature maps.
 """
 import numpy as np
 import pytest
@@ -44,10 +46,12 @@ def _scene(n, seed, spread, types=(1, 2, 3, 5)):
     return d, verts, poses.astype(np.float32), pairs
 
 
-def _run(d, verts, poses, pairs, prediction):
+def _run(d, verts, poses, pairs, prediction, faces=None):
     import torch
 
     shapes = ShapeSet(d, verts)
+    if faces is not None:
+        shapes.set_polyhedron_faces(faces)
     out = contact_manifolds_pfm(shapes, torch.from_numpy(poses).cuda(),
                                 torch.from_numpy(pairs.view(np.int32)).cuda(), prediction)
     torch.cuda.synchronize()
@@ -125,3 +129,50 @@ def test_unsupported_shapes(orc):
     ref = orc.manifolds_pfm(d, verts, poses, pairs, 0.02)
     st = _compare("with balls", _run(d, verts, poses, pairs, 0.02), ref)
     assert (st == UNSUP).sum() > 1000
+
+
+def _polyhedra_scene(n, seed, spread):
+    """Pairs over Cuboid, Capsule, Cone, Triangle and ConvexPolyhedron (half random
+    hulls of 8..40 points, half boxes), polyhedra in at least one slot of most pairs."""
+    rng = np.random.default_rng(seed)
+    t = rng.choice([4, 4, 4, 1, 2, 3, 5], size=2 * n)
+    d = np.zeros(2 * n, SHAPE_DESC_DTYPE)
+    d["type"] = t
+    pts = []
+    nv = 0
+    for i, ty in enumerate(t):
+        if ty == 1:
+            d["params"][i][:3] = rng.uniform(0.3, 1.0, 3)
+        elif ty in (2, 3):
+            d["params"][i][:2] = rng.uniform(0.3, 1.0, 2)
+        else:
+            if ty == 5:
+                v = rng.normal(0, 0.7, (3, 3))
+            elif rng.uniform() < 0.5:
+                v = scenes.convex_hull_vertices(scenes.convex_point_cloud(rng, int(rng.integers(8, 41))))
+            else:
+                he = rng.uniform(0.3, 1.0, 3)
+                v = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]) * he
+            d["first_vertex"][i] = nv
+            d["num_vertices"][i] = len(v)
+            pts.append(v)
+            nv += len(v)
+    verts = np.concatenate(pts, 0).astype(np.float32)
+    poses = np.concatenate([scenes.random_rotations(2 * n, rng), np.zeros((2 * n, 3))], 1)
+    poses[0::2, 4:] = rng.uniform(-40, 40, (n, 3))
+    poses[1::2, 4:] = poses[0::2, 4:] + rng.normal(0, spread, (n, 3))
+    pairs = np.stack([np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)], 1).astype(np.uint32)
+    return d, verts, poses.astype(np.float32), pairs
+
+
+def test_polyhedra_with_faces_match_oracle(orc):
+    d, verts, poses, pairs = _polyhedra_scene(6000, 11, 0.8)
+    faces = scenes.polyhedron_faces(d, verts)
+    print(f"case: manifolds, 6000 pairs with ConvexPolyhedra ({len(faces['faces'])} faces)", flush=True)
+    ref = orc.manifolds_pfm(d, verts, poses, pairs, 0.02, faces=faces)
+    st = _compare("polyhedra", _run(d, verts, poses, pairs, 0.02, faces=faces), ref)
+    poly = (d["type"][pairs[:, 0]] == 4) | (d["type"][pairs[:, 1]] == 4)
+    assert (st[poly] == 1).sum() > 1500 and (st == UNSUP).sum() == 0
+    print("case: the same pairs, polyhedra without faces", flush=True)
+    st = _compare("no faces", _run(d, verts, poses, pairs, 0.02), orc.manifolds_pfm(d, verts, poses, pairs, 0.02))
+    assert (st[poly] == UNSUP).all()

@@ -15,7 +15,13 @@ on a fresh manifold.
     of the two feature maps;
   * separated pairs: status 0 with GJK's separating direction cached in n1;
     a Ball has no feature map (-1);
-  * f32 against f64.
+  * f32 against f64;
+  * a box-shaped ConvexPolyhedron (faces from the hull) against a Cuboid of the
+    same half extents in the same poses: the same statuses, normals and point
+    sets (the clipping walks the faces from other vertices, so points agree to
+    1e-4 but for crossings of nearly parallel edges), each vertex id naming the
+    polyhedron vertex at the point and each face id a face whose plane holds it;
+    a polyhedron without faces has no feature map.
 """
 import numpy as np
 
@@ -205,3 +211,61 @@ def test_f32_agrees_with_f64():
     assert same.mean() > 0.6
     dd = np.abs(a["dist"][same] - b["dist"][same])
     assert np.quantile(dd, 0.99) < 1e-3
+
+
+def _box_polyhedra(he):
+    signs = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float64)
+    verts = (he[:, None, :] * signs[None]).reshape(-1, 3).astype(np.float32)
+    d = np.zeros(len(he), SHAPE_DESC_DTYPE)
+    d["type"] = 4
+    d["first_vertex"] = 8 * np.arange(len(he))
+    d["num_vertices"] = 8
+    return d, verts
+
+
+def test_box_polyhedron_matches_cuboid():
+    from parry_amd import scenes
+
+    rng = np.random.default_rng(5)
+    n = 2000
+    dp, verts = _box_polyhedra(rng.uniform(0.3, 1.0, (2 * n, 3)))
+    dc = np.zeros(2 * n, SHAPE_DESC_DTYPE)
+    dc["type"] = 1
+    dc["params"][:, :3] = verts.reshape(-1, 8, 3)[:, 7]
+    faces = scenes.polyhedron_faces(dp, verts)
+    assert (faces["shape_faces"][:, 1] == 6).all() and (faces["faces"]["count"] == 4).all()
+    poses = np.concatenate([scenes.random_rotations(2 * n, rng), np.zeros((2 * n, 3))], 1)
+    poses[1::2, 4:] = rng.normal(0, 0.8, (n, 3))
+    poses = poses.astype(np.float32)
+    pairs = np.stack([np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)], 1).astype(np.uint32)
+    a = orc.manifolds_pfm(dp, verts, poses, pairs, 0.02, faces=faces)
+    b = orc.manifolds_pfm(dc, np.zeros((0, 3), np.float32), poses, pairs, 0.02)
+    assert np.array_equal(a["status"], b["status"]) and (a["status"] == 1).sum() > 1000
+    ok = a["status"] == 1
+    assert np.array_equal(a["count"][ok], b["count"][ok])
+    np.testing.assert_allclose(a["n1"][ok], b["n1"][ok], atol=1e-5)
+    close, checked_ids = 0, 0
+    fn = faces["faces"]["normal"].astype(np.float64)
+    for k in np.nonzero(ok)[0]:
+        c = int(a["count"][k])
+        pa = np.concatenate([a["p1"][k, :c], a["p2"][k, :c], a["dist"][k, :c, None]], 1)
+        pb = np.concatenate([b["p1"][k, :c], b["p2"][k, :c], b["dist"][k, :c, None]], 1)
+        e = np.abs(pa[:, None, :] - pb[None, :, :]).max(2).min(1).max()
+        assert e < 1e-2, f"pair {k}: points differ by {e}"
+        close += e <= 1e-4
+        for side, (fids, pts) in enumerate(((a["fid1"][k, :c], a["p1"][k, :c]), (a["fid2"][k, :c], a["p2"][k, :c]))):
+            s_idx = int(pairs[k, side])
+            for fid, p in zip(fids, pts):
+                kind, idx = int(fid) >> 30, int(fid) & ((1 << 30) - 1)
+                if kind == 1:  # vertex: the point is that vertex
+                    np.testing.assert_allclose(p, verts[8 * s_idx + idx], atol=1e-5)
+                    checked_ids += 1
+                elif kind == 3:  # face: the point lies in that face's plane
+                    f = faces["shape_faces"][s_idx, 0] + idx
+                    v0 = verts[8 * s_idx + faces["face_vertices"][faces["faces"]["first"][f]]]
+                    assert abs(np.dot(fn[f], p.astype(np.float64) - v0)) < 1e-5
+                    checked_ids += 1
+    assert close >= 0.995 * ok.sum() and checked_ids > 2000
+    # no faces: no feature map
+    r = orc.manifolds_pfm(dp[:2], verts[:16], poses[:2], np.array([[0, 1]], np.uint32), 0.02)
+    assert r["status"][0] == -1
