@@ -335,12 +335,16 @@ typedef struct pq_contact {
  *       staging, 2 = 24 vertices / 64 faces, 3 = 32 vertices / 160 faces,
  *       4 = 48 vertices / 224 faces, all three without staging (less LDS per pair
  *       means more pairs resident; a pair whose polytope outgrows the buffers
- *       moves on to the one-wave-per-pair tiers).
- * The default is 16 lanes per pair with polytope 1. */
+ *       moves on to the one-wave-per-pair tiers);
+ *   epa_schedule: with fewer than 64 lanes per pair, 0 = a group of lanes takes
+ *       its next pair as soon as it is done with one, 1 = a wave takes its next
+ *       pairs when all its groups are done.
+ * The default is 16 lanes per pair with polytope 1 and schedule 0. */
 typedef struct pq_contact_tuning {
     uint32_t epa_lanes_per_pair;
     uint32_t epa_polytope;
-    uint32_t reserved[2]; /* must be 0 */
+    uint32_t epa_schedule;
+    uint32_t reserved; /* must be 0 */
 } pq_contact_tuning;
 int pq_contact_set_tuning(const pq_contact_tuning *tuning);
 int pq_contact_get_tuning(pq_contact_tuning *tuning);

@@ -124,7 +124,7 @@ class RayTuning(ctypes.Structure):
 class ContactTuning(ctypes.Structure):
     """pq_contact_tuning (include/parry_mi355x.h)."""
 
-    _fields_ = [("epa_lanes_per_pair", _u32), ("epa_polytope", _u32), ("reserved", _u32 * 2)]
+    _fields_ = [("epa_lanes_per_pair", _u32), ("epa_polytope", _u32), ("epa_schedule", _u32), ("reserved", _u32)]
 
 
 class ShapeCastOptions(ctypes.Structure):
@@ -134,14 +134,14 @@ class ShapeCastOptions(ctypes.Structure):
                 ("compute_impact_geometry_on_penetration", _u32)]
 
 
-def set_contact_tuning(epa_lanes_per_pair: int, epa_polytope: int = 1) -> None:
-    check(lib().pq_contact_set_tuning(ctypes.byref(ContactTuning(epa_lanes_per_pair, epa_polytope))))
+def set_contact_tuning(epa_lanes_per_pair: int, epa_polytope: int = 1, epa_schedule: int = 0) -> None:
+    check(lib().pq_contact_set_tuning(ctypes.byref(ContactTuning(epa_lanes_per_pair, epa_polytope, epa_schedule))))
 
 
 def get_contact_tuning() -> tuple:
     t = ContactTuning()
     check(lib().pq_contact_get_tuning(ctypes.byref(t)))
-    return int(t.epa_lanes_per_pair), int(t.epa_polytope)
+    return int(t.epa_lanes_per_pair), int(t.epa_polytope), int(t.epa_schedule)
 
 
 class MlpTrainConfig(_c.Structure):

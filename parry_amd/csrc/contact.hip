@@ -559,7 +559,7 @@ template <class W> struct Heap {
     }
 };
 
-enum : uint32_t { EPA_OK = 0, EPA_NONE = 1, EPA_OVERFLOW = 2 };
+enum : uint32_t { EPA_OK = 0, EPA_NONE = 1, EPA_OVERFLOW = 2, EPA_RUNNING = 3 };
 
 // Barycentric coordinates of Face::new's projection (epa3.rs:96-118,
 // TrianglePointLocation::barycentric_coordinates); returns proj_inside.
@@ -777,7 +777,9 @@ __device__ void epa_import(W *w, float *orig, const EpaCont *r, EpaScalars &x, u
     }
 }
 
-// EPA::closest_points (epa3.rs:428-651) for one pair, executed by a group of G lanes.
+// EPA::closest_points (epa3.rs:428-651) for one pair, executed by a group of G
+// lanes one iteration at a time: epa_begin, then epa_step until it returns a
+// result (epa_run below runs the whole loop).
 //
 // Every iteration is checkpointable: the heap's top is only peeked, the new
 // vertex goes to the first free slot without being counted, the best face and
@@ -790,22 +792,30 @@ __device__ void epa_import(W *w, float *orig, const EpaCont *r, EpaScalars &x, u
 // undone and, if `sink` has room, the polytope is written out for the next
 // tier to continue (`cont` gets the record's index).  `resume` starts from such
 // a record instead of the GJK simplex.
+template <class W> struct EpaRegs {
+    Heap<W> heap;  // the polytope's face heap (LDS) and its length
+    uint32_t nv, nf, best_id, niter;
+    float max_dist, old_dist;
+};
+
+// The set-up before EPA's first iteration: the polytope from the GJK simplex, or
+// from a continuation record.  Returns EPA_RUNNING, or the result if EPA ends
+// here.
 template <int G, class W>
-__device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef &g1, const ShapeRef &g2,
-                            const EpaItem *it, const EpaCont *resume, const ContSink &sink, f3 &p1, f3 &p2, f3 &n,
-                            uint32_t &epa_iters, uint32_t &cont) {
-    const float eps_tol = kEps * 100.0f;
-    Staged st{stage_shape<G>(w, 0, g1), stage_shape<G>(w, 1, g2)};
+__device__ uint32_t epa_begin(W *w, float *orig, const iso &pos12, const ShapeRef &g1, const ShapeRef &g2,
+                              const EpaItem *it, const EpaCont *resume, Staged &st, EpaRegs<W> &r, f3 &p1, f3 &p2,
+                              f3 &n, uint32_t &epa_iters, uint32_t &cont) {
+    st = Staged{stage_shape<G>(w, 0, g1), stage_shape<G>(w, 1, g2)};
     // The group's lanes wrote the staged vertices; make them visible to the
     // whole wave before any lane reads another lane's entries.
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    uint32_t nv = 0, nf = 0;
-    Heap<W> heap{w, 0};
+    r = EpaRegs<W>{Heap<W>{w, 0}, 0, 0, 0, 0, kFltMax, 0.0f};
+    Heap<W> &heap = r.heap;
+    uint32_t &nv = r.nv, &nf = r.nf, &best_id = r.best_id, &niter = r.niter;
+    float &max_dist = r.max_dist, &old_dist = r.old_dist;
     epa_iters = 0;
     cont = kNoCont;
-    uint32_t best_id = 0, niter = 0;
-    float max_dist = kFltMax, old_dist = 0.0f;
     if (resume) {
         EpaScalars x;
         epa_import<G>(w, orig, resume, x, heap.len);
@@ -877,6 +887,26 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
         if (heap.len == 0) return EPA_NONE;
         best_id = w->heap_id[0];
     }
+    return EPA_RUNNING;
+}
+
+// One EPA iteration (the body of the loop of epa3.rs:499-647).  Returns
+// EPA_RUNNING while EPA goes on, else the result.
+template <int G, class W>
+__device__ uint32_t epa_step(W *w, float *orig, const iso &pos12, const ShapeRef &g1, const ShapeRef &g2,
+                             const Staged &st, const ContSink &sink, EpaRegs<W> &r, f3 &p1, f3 &p2, f3 &n,
+                             uint32_t &epa_iters, uint32_t &cont) {
+    const float eps_tol = kEps * 100.0f;
+    Heap<W> &heap = r.heap;
+    uint32_t &nv = r.nv, &nf = r.nf, &best_id = r.best_id, &niter = r.niter;
+    float &max_dist = r.max_dist, &old_dist = r.old_dist;
+    // parry's loop ends when the heap is empty or after 100 iterations
+    auto done = [&]() -> uint32_t {
+        epa_iters = niter + 1;
+        face_points(w, orig, best_id, p1, p2);
+        n = fnormal(w, best_id);
+        return EPA_OK;
+    };
     // Hands the polytope as it was at the start of this iteration to the next tier.
     auto overflow = [&](uint32_t ndel, uint32_t fid) -> uint32_t {
         for (uint32_t k = 0; k < ndel; ++k) w->face[w->dl[k]].deleted = 0;
@@ -886,14 +916,15 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
         cont = epa_export<G>(w, orig, sink, EpaScalars{nv, nf, best_id, niter, max_dist, old_dist}, heap.len);
         return EPA_OVERFLOW;
     };
-    while (heap.len > 0) {
+    if (heap.len == 0) return done();
+    {
         const uint32_t fid = w->heap_id[0];
         const float fnd = w->heap_nd[0];
         if (w->face[fid].deleted) {
             uint32_t pid;
             float pnd;
             heap.pop(pid, pnd);
-            continue;
+            return EPA_RUNNING;
         }
         const typename W::Face face = w->face[fid];
         const f3 fn = f3{face.n[0], face.n[1], face.n[2]};
@@ -1089,12 +1120,87 @@ __device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef 
         w->face[first_new].adj[2] = (typename W::Idx)(nf - 1);
         w->face[nf - 1].adj[1] = (typename W::Idx)first_new;
         ++niter;
-        if (niter > 100) break;
+        if (niter > 100) return done();
     }
-    epa_iters = niter + 1;
-    face_points(w, orig, best_id, p1, p2);
-    n = fnormal(w, best_id);
-    return EPA_OK;
+    return EPA_RUNNING;
+}
+
+// EPA::closest_points (epa3.rs:428-651) for one pair by a group of G lanes.
+template <int G, class W>
+__device__ uint32_t epa_run(W *w, float *orig, const iso &pos12, const ShapeRef &g1, const ShapeRef &g2,
+                            const EpaItem *it, const EpaCont *resume, const ContSink &sink, f3 &p1, f3 &p2, f3 &n,
+                            uint32_t &epa_iters, uint32_t &cont) {
+    Staged st;
+    EpaRegs<W> r;
+    uint32_t s = epa_begin<G>(w, orig, pos12, g1, g2, it, resume, st, r, p1, p2, n, epa_iters, cont);
+    while (s == EPA_RUNNING) s = epa_step<G>(w, orig, pos12, g1, g2, st, sink, r, p1, p2, n, epa_iters, cont);
+    return s;
+}
+
+
+// The EPA problem of a queued pair (contact_support_map_support_map.rs:60-76):
+// its two shapes (their feature maps in kPairFeatureMaps mode) and relative pose,
+// or, for a Ball, the other shape against the ball centre
+// (point_support_map.rs:39-49).
+struct EpaProblem {
+    iso pos;
+    ShapeRef a, b;
+};
+
+__device__ EpaProblem epa_problem(const ShapeRecord *rec, const float4 *verts, const float *poses,
+                                  const uint32_t *pairs, const EpaItem *it) {
+    const uint32_t k = it->pair;
+    const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+    const iso pos1 = load_iso(poses + 7 * (size_t)i), pos2 = load_iso(poses + 7 * (size_t)j);
+    const iso pos12 = iso_inv_mul(pos1, pos2);
+    ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
+    const uint32_t mode = it->mode;
+    if (mode == kPairFeatureMaps) {
+        float br;
+        (void)feature_map(g1, br);
+        (void)feature_map(g2, br);
+    }
+    if (is_ball_mode(mode)) {
+        if (mode == kBallFirst) return EpaProblem{point_proj_iso(iso_inverse(pos12).t), g2, origin_shape()};
+        return EpaProblem{point_proj_iso(pos12.t), g1, origin_shape()};
+    }
+    return EpaProblem{pos12, g1, g2};
+}
+
+// Writes the result of a queued pair's EPA (one lane per pair): the contact, or
+// the pair's entry in the next tier's list if its polytope outgrew this tier's.
+__device__ void epa_finish(const ShapeRecord *rec, const float4 *verts, const float *poses, const uint32_t *pairs,
+                           float prediction, const EpaItem *it, uint32_t st, f3 p1, f3 p2, f3 n, uint32_t ei,
+                           uint32_t cont, DevContact *out, uint32_t *iters, uint32_t *overflow_list,
+                           uint32_t *overflow_cont, uint32_t *overflow_count, uint32_t slot) {
+    const uint32_t k = it->pair;
+    if (st == EPA_OVERFLOW) {
+        if (overflow_list) {
+            const uint32_t pos = atomicAdd(overflow_count, 1u);
+            overflow_list[pos] = slot;
+            if (overflow_cont) overflow_cont[pos] = cont;
+        } else {
+            out[k] = no_contact(kUnsupportedPair);  // cannot happen with the large buffers
+        }
+        return;
+    }
+    const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+    const iso pos1 = load_iso(poses + 7 * (size_t)i), pos2 = load_iso(poses + 7 * (size_t)j);
+    const iso pos12 = iso_inv_mul(pos1, pos2);
+    const uint32_t mode = it->mode;
+    if (is_ball_mode(mode)) {
+        // the Ball-vs-convex contact from the projection of the ball centre
+        const iso pb = mode == kBallFirst ? iso_inverse(pos12) : pos12;
+        const float radius = rec[mode == kBallFirst ? i : j].param[0];
+        const PointProj proj{true, st == EPA_OK ? p1 : pb.t};
+        finish_ball_contact(pos1, pos2, pb, radius, prediction, proj, false, f3{0.0f, 0.0f, 0.0f}, mode, out + k);
+    } else if (st == EPA_OK) {
+        finish_contact(pos1, pos2, pos12, p1, p2, n, is_local_mode(mode), out + k);
+    } else {
+        out[k] = no_contact(0);  // EPA failed: NoIntersection (contact_support_map_support_map.rs:75-76)
+        if (mode == kPairFeatureMaps) out[k].n1[0] = 1.0f;  // ... of Vector::x_axis()
+    }
+    if (iters) iters[2 * k + 1] = ei;
 }
 
 template <int G, class W>
@@ -1102,66 +1208,13 @@ __device__ void epa_item(W *w, float *orig, const ShapeRecord *rec, const float4
                          const uint32_t *pairs, float prediction, const EpaItem *it, const EpaCont *resume,
                          const ContSink &sink, DevContact *out, uint32_t *iters, uint32_t *overflow_list,
                          uint32_t *overflow_cont, uint32_t *overflow_count, uint32_t slot) {
-    const uint32_t k = it->pair;
-    const uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
-    const iso pos1 = load_iso(poses + 7 * (size_t)i), pos2 = load_iso(poses + 7 * (size_t)j);
-    const iso pos12 = iso_inv_mul(pos1, pos2);
-    ShapeRef g1 = load_shape(rec, verts, i), g2 = load_shape(rec, verts, j);
+    const EpaProblem pr = epa_problem(rec, verts, poses, pairs, it);
     f3 p1, p2, n;
     uint32_t ei = 0, cont = kNoCont;
-    const uint32_t mode = it->mode;
-    if (mode == kPairFeatureMaps) {
-        float br;
-        (void)feature_map(g1, br);
-        (void)feature_map(g2, br);
-    }
-    // A pair that outgrows this tier's polytope goes to the next tier's list,
-    // with its continuation record if it has one.
-    auto hand_on = [&]() {
-        if (overflow_list) {
-            const uint32_t pos = atomicAdd(overflow_count, 1u);
-            overflow_list[pos] = slot;
-            if (overflow_cont) overflow_cont[pos] = cont;
-            return;
-        }
-        out[k] = no_contact(kUnsupportedPair);  // cannot happen with the large buffers
-    };
-    if (is_ball_mode(mode)) {
-        // EPA::project_origin of the ball centre on the ConvexPolyhedron
-        // (point_support_map.rs:39-49), then the Ball-vs-convex contact.
-        iso pb = pos12;
-        ShapeRef cv = g1;
-        float radius = g2.p0;
-        if (mode == kBallFirst) {
-            pb = iso_inverse(pos12);
-            cv = g2;
-            radius = g1.p0;
-        }
-        const uint32_t st =
-            epa_run<G>(w, orig, point_proj_iso(pb.t), cv, origin_shape(), it, resume, sink, p1, p2, n, ei, cont);
-        if (__lane_id() % G != 0) return;
-        if (st == EPA_OVERFLOW) {
-            hand_on();
-            return;
-        }
-        const PointProj proj{true, st == EPA_OK ? p1 : pb.t};
-        finish_ball_contact(pos1, pos2, pb, radius, prediction, proj, false, f3{0.0f, 0.0f, 0.0f}, mode, out + k);
-        if (iters) iters[2 * k + 1] = ei;
-        return;
-    }
-    const uint32_t st = epa_run<G>(w, orig, pos12, g1, g2, it, resume, sink, p1, p2, n, ei, cont);
+    const uint32_t st = epa_run<G>(w, orig, pr.pos, pr.a, pr.b, it, resume, sink, p1, p2, n, ei, cont);
     if (__lane_id() % G != 0) return;
-    if (st == EPA_OVERFLOW) {
-        hand_on();
-        return;
-    }
-    if (st == EPA_OK) {
-        finish_contact(pos1, pos2, pos12, p1, p2, n, is_local_mode(mode), out + k);
-    } else {
-        out[k] = no_contact(0);  // EPA failed: NoIntersection (contact_support_map_support_map.rs:75-76)
-        if (mode == kPairFeatureMaps) out[k].n1[0] = 1.0f;  // ... of Vector::x_axis()
-    }
-    if (iters) iters[2 * k + 1] = ei;
+    epa_finish(rec, verts, poses, pairs, prediction, it, st, p1, p2, n, ei, cont, out, iters, overflow_list,
+               overflow_cont, overflow_count, slot);
 }
 
 // EPA tiers.  Each reads a list of queued pairs (the queue itself for the first
@@ -1209,6 +1262,69 @@ __global__ __launch_bounds__(64, (epa_min_waves<G, W>())) void epa_tier_kernel(c
     }
 }
 
+// The first tier with a refill per group: a group that is done with its pair
+// takes the next list entry at once, while the other groups of its wave go on
+// with theirs, so a wave no longer waits for the longest of its pairs.  Each
+// trip through the loop runs, per group, either the set-up of a new pair or one
+// EPA iteration (the two are executed one after the other by the wave when its
+// groups differ).  The loop's exit test is wave-uniform, at a point where the
+// whole wave is converged; everything inside is group-uniform, and the
+// cross-lane operations of the EPA code stay within a group.
+template <int G, class W>
+__global__ __launch_bounds__(64, (epa_min_waves<G, W>())) void epa_refill_kernel(const ShapeRecord *__restrict__ rec,
+                                                      const float4 *__restrict__ verts, const float *__restrict__ poses,
+                                                      const uint32_t *__restrict__ pairs, float prediction,
+                                                      const EpaItem *__restrict__ queue, const uint32_t *in_list,
+                                                      const uint32_t *in_count, uint32_t *next,
+                                                      const uint32_t *in_cont, const EpaCont *cont_recs,
+                                                      ContSink sink, DevContact *__restrict__ out,
+                                                      uint32_t *__restrict__ iters, uint32_t *out_list,
+                                                      uint32_t *out_cont, uint32_t *out_count, float *orig_scratch) {
+    static_assert(G < 64, "a wave-per-pair tier has nothing to refill");
+    constexpr int kGroups = 64 / G;
+    __shared__ W w[kGroups];
+    const uint32_t group = threadIdx.x / G;
+    const uint32_t sub = __lane_id() % G;
+    const uint32_t gid = blockIdx.x * kGroups + group;
+    W *const wg = &w[group];
+    float *orig = orig_scratch + (size_t)gid * W::kV * kOrigFloats;
+    const uint32_t count = __hip_atomic_load(in_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool busy = false, drained = false;
+    uint32_t slot = 0, status = EPA_OK, ei = 0, cont = kNoCont;
+    EpaProblem pr{};
+    Staged st{nullptr, nullptr};
+    EpaRegs<W> r{Heap<W>{wg, 0}, 0, 0, 0, 0, kFltMax, 0.0f};
+    f3 p1{0.0f, 0.0f, 0.0f}, p2{0.0f, 0.0f, 0.0f}, n{0.0f, 0.0f, 0.0f};
+    for (;;) {
+        if (!busy && !drained) {
+            uint32_t q = 0;
+            if (sub == 0) q = atomicAdd(next, 1u);
+            q = __shfl(q, 0, G);
+            if (q >= count) {
+                drained = true;
+            } else {
+                slot = in_list ? in_list[q] : q;
+                const uint32_t ci = in_cont ? in_cont[q] : kNoCont;
+                pr = epa_problem(rec, verts, poses, pairs, queue + slot);
+                status = epa_begin<G>(wg, orig, pr.pos, pr.a, pr.b, queue + slot,
+                                      ci != kNoCont ? cont_recs + ci : nullptr, st, r, p1, p2, n, ei, cont);
+                busy = true;
+            }
+        }
+        if (__ballot(!drained) == 0ull) break;
+        if (busy) {
+            if (status == EPA_RUNNING)
+                status = epa_step<G>(wg, orig, pr.pos, pr.a, pr.b, st, sink, r, p1, p2, n, ei, cont);
+            if (status != EPA_RUNNING) {
+                if (sub == 0)
+                    epa_finish(rec, verts, poses, pairs, prediction, queue + slot, status, p1, p2, n, ei, cont, out,
+                               iters, out_list, out_cont, out_count, slot);
+                busy = false;
+            }
+        }
+    }
+}
+
 // Last tier: one wave per pair, polytope in global memory.
 __global__ __launch_bounds__(64) void epa_global_kernel(const ShapeRecord *__restrict__ rec,
                                                         const float4 *__restrict__ verts,
@@ -1241,8 +1357,13 @@ hipError_t launch_epa_tier(const DeviceShapes &s, const float *poses, const uint
     int dev = 0, cus = 0, per_cu = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, epa_tier_kernel<G, W>, 64, 0)) != hipSuccess)
-        return e;
+    bool refill = false;
+    if constexpr (G < 64) refill = g_epa_schedule == 0;
+    const void *kernel = (const void *)epa_tier_kernel<G, W>;
+    if constexpr (G < 64) {
+        if (refill) kernel = (const void *)epa_refill_kernel<G, W>;
+    }
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0)) != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     constexpr uint64_t kGroups = 64 / G;
     uint64_t waves = (uint64_t)cus * (uint64_t)per_cu;
@@ -1250,6 +1371,14 @@ hipError_t launch_epa_tier(const DeviceShapes &s, const float *poses, const uint
     if (waves > need) waves = need;
     if (waves * kGroups > max_groups) waves = max_groups / kGroups;
     if (waves < 1) waves = 1;
+    if constexpr (G < 64) {
+        if (refill) {
+            hipLaunchKernelGGL((epa_refill_kernel<G, W>), dim3((uint32_t)waves), dim3(64), 0, stream, s.rec, s.verts,
+                               poses, pairs, prediction, queue, in_list, in_count, next, in_cont, cont_recs, sink, out,
+                               iters, out_list, out_cont, out_count, orig);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((epa_tier_kernel<G, W>), dim3((uint32_t)waves), dim3(64), 0, stream, s.rec, s.verts, poses,
                        pairs, prediction, queue, in_list, in_count, next, in_cont, cont_recs, sink, out, iters,
                        out_list, out_cont, out_count, orig);
@@ -1271,6 +1400,7 @@ constexpr uint64_t kEpaBigWaves = 256ull * 2;
 
 int g_epa_lanes_per_pair = 16;
 int g_epa_polytope = 1;
+int g_epa_schedule = 0;
 
 namespace {
 struct ContactWorkspace {

@@ -202,6 +202,7 @@ hipError_t launch_casts(const DeviceShapes &s, const float *poses, const float *
 // Lanes that run one pair's EPA together (64, 16 or 8), set by pq_contact_set_tuning.
 extern int g_epa_lanes_per_pair;
 extern int g_epa_polytope;
+extern int g_epa_schedule;  // first EPA tier: 0 = a refill per group, 1 = per wave
 // Per-(device, stream) workspace owned by the library, grown on demand.
 hipError_t stream_workspace(hipStream_t stream, size_t bytes, void **ptr);
 

@@ -207,7 +207,7 @@ def test_ball_convex_pairs(orc):
 
 
 def test_every_epa_tuning_matches_oracle(orc):
-    """Each first-tier EPA mapping (lanes per pair x polytope buffers) gives the
+    """Each first-tier EPA mapping (lanes per pair x polytope buffers x schedule) gives the
     oracle's contacts and byte-identical records, on mixed broadphase pairs and
     on the deep-penetration set (whose long runs reach the wave-per-pair tiers)."""
     from parry_amd import _native
@@ -223,16 +223,19 @@ def test_every_epa_tuning_matches_oracle(orc):
     saved = _native.get_contact_tuning()
     first = [None, None]
     try:
-        for g in (64, 32, 16, 8, 4):
-            for poly in range(5):
-                _native.set_contact_tuning(g, poly)
-                for k, (label, shapes, _, _, p, pr, pred) in enumerate(cases):
-                    print(f"case: EPA tuning G={g} polytope={poly}, {label} ({pr.shape[0]} pairs)", flush=True)
-                    gpu, has, _ = _run_gpu(shapes, p, pr, pred)
-                    _compare(f"{label} G={g} polytope={poly}", gpu, has, refs[k])
-                    rec = np.concatenate([gpu.view(np.uint32), has[:, None]], 1)
-                    if first[k] is None:
-                        first[k] = rec
-                    assert np.array_equal(rec, first[k]), f"{label}: records differ between EPA tunings"
+        settings = [(64, poly, 0) for poly in range(5)]
+        for g in (32, 16, 8, 4):
+            settings += [(g, poly, 0) for poly in range(5)] + [(g, 0, 1), (g, 1, 1)]
+        for g, poly, sched in settings:
+            _native.set_contact_tuning(g, poly, sched)
+            for k, (label, shapes, _, _, p, pr, pred) in enumerate(cases):
+                print(f"case: EPA tuning G={g} polytope={poly} schedule={sched}, {label} ({pr.shape[0]} pairs)",
+                      flush=True)
+                gpu, has, _ = _run_gpu(shapes, p, pr, pred)
+                _compare(f"{label} G={g} polytope={poly} schedule={sched}", gpu, has, refs[k])
+                rec = np.concatenate([gpu.view(np.uint32), has[:, None]], 1)
+                if first[k] is None:
+                    first[k] = rec
+                assert np.array_equal(rec, first[k]), f"{label}: records differ between EPA tunings"
     finally:
         _native.set_contact_tuning(*saved)
