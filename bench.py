@@ -558,6 +558,31 @@ def manifold_leg(args, log):
                      "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes_per_launch": alg},
     }
+    # The update from the previous frame's manifolds (pq_contact_manifolds_pfm_update):
+    # the poses alternate between the scene's and a copy moved by N(0, 1e-4^2) per
+    # axis, so most manifolds in contact are kept and the rest are recomputed with
+    # GJK seeded by their cached normal.
+    from parry_amd import MANIFOLD_UPDATED, contact_manifolds_pfm_update
+
+    dp = np.zeros((2 * n, 7), np.float32)
+    dp[:, 4:] = np.random.default_rng(7).normal(0, 1e-4, (2 * n, 3))
+    poses_b = poses + torch.from_numpy(dp).cuda()
+    mf = out.clone()
+    for _ in range(args.warmup):
+        contact_manifolds_pfm_update(shapes, poses_b, pairs, 0.02, mf, workspace=ws, stream=stream)
+        contact_manifolds_pfm_update(shapes, poses, pairs, 0.02, mf, workspace=ws, stream=stream)
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for t in range(args.steps):
+        contact_manifolds_pfm_update(shapes, poses_b if t % 2 == 0 else poses, pairs, 0.02, mf, workspace=ws,
+                                     stream=stream)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    upd = mf.cpu().numpy().view(CONTACT_MANIFOLD_DTYPE).reshape(-1)
+    rec["update"] = {"gpu_ms_per_step": round(ev[0].elapsed_time(ev[1]) / args.steps, 4),
+                     "kept": int((upd["status"] == MANIFOLD_UPDATED).sum()),
+                     "recomputed_in_contact": int((upd["status"] == 1).sum()),
+                     "workload": "the same pairs, poses alternating with a copy moved by N(0, 1e-4^2) per axis"}
     if not args.no_cpu:
         from oracle import oracle
 

@@ -420,6 +420,27 @@ int pq_contact_manifolds_pfm_pairs(const pq_shapes *shapes, const pq_isometry *p
                                    void *stream);
 size_t pq_contact_manifolds_workspace_bytes(uint64_t num_pairs);
 
+/* Status of a manifold that ContactManifold::try_update_contacts_eps kept: its
+ * points (and their feature ids) are last frame's, with local_p1 and dist
+ * updated to the new poses, so a caller's per-point data stays with them. */
+#define PQ_MANIFOLD_UPDATED 2
+/* contact_manifold_pfm_pfm (contact_manifolds_pfm_pfm.rs:44-162) on the manifolds
+ * of the previous frame, in place: `manifolds` holds one pq_contact_manifold per
+ * pair as the last call left it (all zero for a new pair: a fresh manifold).  A
+ * manifold that try_update_contacts_eps(pos12, COS_1_DEGREES, 1e-6) keeps
+ * (contact_manifold.rs:662-699) is updated and gets PQ_MANIFOLD_UPDATED; every
+ * other pair is recomputed as pq_contact_manifolds_pfm_pairs does, with GJK seeded
+ * by the cached normal local_n1 (parry's default direction if it is zero).
+ * parry's match_contacts then moves a caller's per-point data from the old
+ * points to new ones with the same (fid1, fid2); that data is the caller's, so
+ * the caller does the matching, from its copy of the old points, for the pairs
+ * whose status is not PQ_MANIFOLD_UPDATED.  Workspace:
+ * pq_contact_manifolds_workspace_bytes(num_pairs) bytes, or NULL.  Asynchronous on
+ * `stream`. */
+int pq_contact_manifolds_pfm_update(const pq_shapes *shapes, const pq_isometry *poses, const uint32_t *pairs,
+                                    uint64_t num_pairs, float prediction, pq_contact_manifold *manifolds,
+                                    void *workspace, void *stream);
+
 /* ---- Linear shape casts ---------------------------------------------------- */
 
 /* ShapeCastOptions (src/query/shape_cast/shape_cast.rs:105-146).  parry's

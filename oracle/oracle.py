@@ -43,6 +43,8 @@ _SIG = {
     "orc_trimesh_contact_shapes_f64": (None, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _f64, _vp, _vp, _vp, _vp, _int]),
     "orc_manifolds_pfm_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp, _int]),
     "orc_manifolds_pfm_f64": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f64, _vp, _vp, _vp, _vp, _vp, _int]),
+    "orc_manifolds_pfm_update_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp,
+                                            _int]),
     "orc_manifolds_pfm_faces_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp, _int,
                                            _vp, _vp, _vp, _u32, _vp, _vp, _vp]),
     "orc_bvh_create_ploc_f32": (_vp, [_vp, _u32]),
@@ -423,11 +425,13 @@ MANIFOLD_POINTS = 25
 
 
 def manifolds_pfm(descs, vertices, poses, pairs, prediction: float, f64: bool = False, threads: int | None = None,
-                  faces=None):
+                  faces=None, previous=None):
     """contact_manifold_pfm_pfm (contact_manifolds_pfm_pfm.rs:44-162) on a fresh manifold
-    for every pair.  Returns dict(status int32 [m] (1 ClosestPoints, 0 NoIntersection,
-    -1 no feature map), n1, n2 [m,3], count uint32 [m], p1, p2 [m,25,3], dist [m,25],
-    fid1, fid2 uint32 [m,25]) in the shapes' local frames."""
+    for every pair, or, given ``previous`` (the dict an earlier call returned, f32), on
+    those manifolds.  Returns dict(status int32 [m] (2 kept by try_update_contacts_eps,
+    1 ClosestPoints, 0 NoIntersection, -1 no feature map), n1, n2 [m,3], count uint32
+    [m], p1, p2 [m,25,3], dist [m,25], fid1, fid2 uint32 [m,25]) in the shapes' local
+    frames."""
     from parry_amd.shape import SHAPE_DESC_DTYPE
 
     dt = np.float64 if f64 else np.float32
@@ -442,7 +446,17 @@ def manifolds_pfm(descs, vertices, poses, pairs, prediction: float, f64: bool = 
     pts = np.zeros((m, MANIFOLD_POINTS, 7), dt)
     fids = np.zeros((m, MANIFOLD_POINTS, 2), np.uint32)
     th = default_threads() if threads is None else threads
-    if faces is not None:  # ConvexPolyhedron faces, as scenes.polyhedron_faces returns them (f32 only)
+    if previous is not None:
+        assert not f64 and faces is None
+        normals[:] = np.concatenate([previous["n1"], previous["n2"]], 1)
+        counts[:] = previous["count"]
+        pts[:] = np.concatenate([previous["p1"], previous["p2"], previous["dist"][:, :, None]], 2)
+        fids[:] = np.stack([previous["fid1"], previous["fid2"]], 2)
+        lib().orc_manifolds_pfm_update_f32(
+            d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data, pr.ctypes.data,
+            m, prediction, status.ctypes.data, normals.ctypes.data, counts.ctypes.data, pts.ctypes.data,
+            fids.ctypes.data, th)
+    elif faces is not None:  # ConvexPolyhedron faces, as scenes.polyhedron_faces returns them (f32 only)
         assert not f64
         fc = faces["faces"]
         nrm = np.ascontiguousarray(fc["normal"], np.float32)

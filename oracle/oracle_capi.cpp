@@ -594,7 +594,7 @@ void manifolds_pfm(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uin
                    T *points, uint32_t *fids, int nthreads, const float *face_normals = nullptr,
                    const uint32_t *face_first = nullptr, const uint32_t *face_count = nullptr, uint32_t nfaces = 0,
                    const uint32_t *face_verts = nullptr, const uint32_t *face_edges = nullptr,
-                   const uint32_t *shape_faces = nullptr) {
+                   const uint32_t *shape_faces = nullptr, bool update = false) {
     std::vector<V3<T>> pts(nverts);
     for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
     std::vector<Shape<T>> shapes = make_shapes(descs, nshapes, pts);
@@ -618,6 +618,19 @@ void manifolds_pfm(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uin
             const T *a = poses + 7 * (size_t)i, *c = poses + 7 * (size_t)j;
             Iso<T> p1{Quat<T>{a[0], a[1], a[2], a[3]}, v3(a[4], a[5], a[6])};
             Iso<T> p2{Quat<T>{c[0], c[1], c[2], c[3]}, v3(c[4], c[5], c[6])};
+            mf.points.clear();
+            mf.n1 = mf.n2 = v3(T(0), T(0), T(0));
+            if (update) {  // the previous frame's manifold, as the output arrays hold it
+                const T *pn = normals + 6 * k;
+                mf.n1 = v3(pn[0], pn[1], pn[2]);
+                mf.n2 = v3(pn[3], pn[4], pn[5]);
+                const T *pq = points + (size_t)k * kManifoldPoints * 7;
+                const uint32_t *pf = fids + (size_t)k * kManifoldPoints * 2;
+                for (uint32_t t = 0; t < counts[k] && t < (uint32_t)kManifoldPoints; ++t)
+                    mf.points.push_back(TrackedContact<T>{v3(pq[7 * t], pq[7 * t + 1], pq[7 * t + 2]),
+                                                          v3(pq[7 * t + 3], pq[7 * t + 4], pq[7 * t + 5]), pf[2 * t],
+                                                          pf[2 * t + 1], pq[7 * t + 6]});
+            }
             int r = manifold_pfm_pfm(inv_mul(p1, p2), shapes[i], shapes[j], prediction, mf);
             status[k] = r;
             T *nrm = normals + 6 * k;
@@ -667,6 +680,16 @@ void orc_manifolds_pfm_faces_f32(const void *descs, uint32_t nshapes, const floa
     manifolds_pfm<float>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, status,
                          normals, counts, points, fids, nthreads, face_normals, face_first, face_count, nfaces,
                          face_verts, face_edges, shape_faces);
+}
+// The same on the previous frame's manifolds, read from and written back to the
+// output arrays (status 2: kept by try_update_contacts_eps).
+void orc_manifolds_pfm_update_f32(const void *descs, uint32_t nshapes, const float *verts, uint32_t nverts,
+                                  const float *poses, const uint32_t *pairs, uint64_t m, float prediction,
+                                  int32_t *status, float *normals, uint32_t *counts, float *points, uint32_t *fids,
+                                  int nthreads) {
+    manifolds_pfm<float>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, status,
+                         normals, counts, points, fids, nthreads, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
+                         nullptr, true);
 }
 void orc_manifolds_pfm_f64(const void *descs, uint32_t nshapes, const double *verts, uint32_t nverts,
                            const double *poses, const uint32_t *pairs, uint64_t m, double prediction, int32_t *status,

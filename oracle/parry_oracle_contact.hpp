@@ -899,7 +899,8 @@ struct ContactStats {
 // init_dir: nullptr for parry's default seed.
 template <class T>
 GjkResult<T> contact_sm_sm_with_params(const Iso<T> &pos12, const Shape<T> &g1, const Shape<T> &g2, T prediction,
-                                       Simplex<T> &simplex, const V3<T> *init_dir, ContactStats *st) {
+                                       Simplex<T> &simplex, const V3<T> *init_dir, ContactStats *st,
+                                       bool retry_unseeded = true) {
     const T e = Eps<T>::eps;
     V3<T> dflt;
     {
@@ -916,7 +917,7 @@ GjkResult<T> contact_sm_sm_with_params(const Iso<T> &pos12, const Shape<T> &g1, 
     simplex.reset(cso_from_shapes(pos12, g1, g2, dir));
     uint32_t gi = 0;
     GjkResult<T> c = gjk_closest_points(pos12, g1, g2, prediction, simplex, &gi);
-    if (seeded && gi == 100) {
+    if (seeded && retry_unseeded && gi == 100) {
         uint32_t gi2 = 0;
         simplex = Simplex<T>();
         simplex.reset(cso_from_shapes(pos12, g1, g2, dflt));

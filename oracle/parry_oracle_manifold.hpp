@@ -2,10 +2,10 @@
 //
 // CPU restatement of parry3d's contact manifold between two polygonal feature
 // maps, contact_manifold_pfm_pfm (src/query/contact_manifolds/
-// contact_manifolds_pfm_pfm.rs:44-162), for a fresh manifold (no points, zero
-// cached normal: try_update_contacts_eps fails at once, contact_manifold.rs:662-670,
-// GJK takes parry's default seed, and match_contacts has nothing to match) and no
-// normal constraints.  The feature maps (polygonal_feature_map.rs:30-154,
+// contact_manifolds_pfm_pfm.rs:44-162), on a fresh manifold or on the previous
+// frame's (try_update_contacts_eps, contact_manifold.rs:662-699, then GJK seeded
+// by the cached normal), with no normal constraints; match_contacts moves user
+// data only and is not modelled.  The feature maps (polygonal_feature_map.rs:30-154,
 // cuboid.rs:267-352, triangle.rs:280-282) and the clipping
 // (polygonal_feature3d.rs:74-439: contacts_edge_edge, contacts_face_face,
 // closest_points_line2d; clip_segment_segment.rs:76-140;
@@ -523,22 +523,56 @@ void contacts_face_face(const Iso<T> &pos12, const PolyFeature<T> &f1, V3<T> sep
     }
 }
 
-// contact_manifold_pfm_pfm (contact_manifolds_pfm_pfm.rs:44-162) on a fresh
-// manifold.  Returns 1 for ClosestPoints (points filled), 0 for NoIntersection
-// (n1 = GJK's separating direction, the cached normal), -1 if a shape has no
-// feature map here.
+// ContactManifold::try_update_contacts_eps (contact_manifold.rs:662-699).
+template <class T> bool try_update_contacts_eps(Manifold<T> &m, const Iso<T> &pos12, T angle_dot_threshold,
+                                                T dist_sq_threshold) {
+    if (m.points.empty()) return false;
+    const V3<T> local_n2 = rotate(pos12.q, m.n2);
+    if (-dot(m.n1, local_n2) < angle_dot_threshold) return false;
+    for (auto &pt : m.points) {
+        const V3<T> local_p2 = transform_point(pos12, pt.p2);
+        const V3<T> dpt = local_p2 - pt.p1;
+        const T dist = dot(dpt, m.n1);
+        if (dist * pt.dist < T(0)) return false;
+        const V3<T> new_p1 = local_p2 - m.n1 * dist;
+        const V3<T> d = new_p1 - pt.p1;
+        if (dot(d, d) > dist_sq_threshold) return false;
+        pt.dist = dist;
+        pt.p1 = new_p1;
+    }
+    return true;
+}
+
+template <class T> constexpr T cos_1_degrees() { return T(0.99984769515); }  // utils/consts.rs:9
+
+// contact_manifold_pfm_pfm (contact_manifolds_pfm_pfm.rs:44-162) on the manifold
+// `m` of the previous frame (no points and a zero normal for a fresh one), with
+// no normal constraints.  Returns 2 if try_update_contacts_eps kept it, 1 for
+// ClosestPoints (points filled), 0 for NoIntersection (n1 = GJK's separating
+// direction, the cached normal), -1 if a shape has no feature map here.  The
+// per-point data match_contacts transfers is not modelled.
 template <class T>
 int manifold_pfm_pfm(const Iso<T> &pos12, const Shape<T> &s1, const Shape<T> &s2, T prediction, Manifold<T> &m) {
-    m.points.clear();
-    m.n1 = m.n2 = v3(T(0), T(0), T(0));
     Shape<T> pfm1, pfm2;
     T br1, br2;
-    if (!pfm_of(s1, pfm1, br1) || !pfm_of(s2, pfm2, br2)) return -1;
+    if (!pfm_of(s1, pfm1, br1) || !pfm_of(s2, pfm2, br2)) {
+        m.points.clear();
+        m.n1 = m.n2 = v3(T(0), T(0), T(0));
+        return -1;
+    }
+    if (try_update_contacts_eps(m, pos12, cos_1_degrees<T>(), T(1.0e-6))) return 2;
+    // Unit::try_new(local_n1, DEFAULT_EPSILON)
+    const T e = Eps<T>::eps;
+    const T sq = dot(m.n1, m.n1);
+    V3<T> init{};
+    const bool seeded = sq > e * e;
+    if (seeded) init = m.n1 / std::sqrt(sq);
+    m.points.clear();
+    m.n1 = m.n2 = v3(T(0), T(0), T(0));
     const T total_prediction = prediction + br1 + br2;
     Simplex<T> simplex;
-    GjkResult<T> r =
-        contact_sm_sm_with_params(pos12, pfm1, pfm2, total_prediction, simplex, static_cast<const V3<T> *>(nullptr),
-                                  nullptr);
+    GjkResult<T> r = contact_sm_sm_with_params(pos12, pfm1, pfm2, total_prediction, simplex,
+                                               seeded ? &init : static_cast<const V3<T> *>(nullptr), nullptr, false);
     if (r.kind == GJK_CLOSEST_POINTS) {
         const V3<T> p1 = r.p1, p21 = r.p2, dir = r.n;
         V3<T> n1 = dir;

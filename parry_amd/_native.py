@@ -79,6 +79,7 @@ SIGNATURES = {
     "pq_contact_workspace_bytes": (_c.c_size_t, [_u64]),
     "pq_contact_manifolds_workspace_bytes": (_c.c_size_t, [_u64]),
     "pq_contact_manifolds_pfm_pairs": (_int, [_vp, _vp, _vp, _u64, _f32, _vp, _vp, _vp]),
+    "pq_contact_manifolds_pfm_update": (_int, [_vp, _vp, _vp, _u64, _f32, _vp, _vp, _vp]),
     "pq_cast_shapes_workspace_bytes": (_c.c_size_t, [_u64]),
     "pq_cast_shapes_pairs": (_int, [_vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     "pq_contact_set_tuning": (_int, [_vp]),

@@ -35,4 +35,16 @@ int pq_contact_manifolds_pfm_pairs(const pq_shapes *s, const pq_isometry *poses,
     return PQ_OK;
 }
 
+int pq_contact_manifolds_pfm_update(const pq_shapes *s, const pq_isometry *poses, const uint32_t *pairs, uint64_t m,
+                                    float prediction, pq_contact_manifold *manifolds, void *workspace, void *stream) {
+    if (!s) return fail(PQ_ERR_INVALID, "pq_contact_manifolds_pfm_update: null shapes");
+    if (m == 0) return PQ_OK;
+    if (!poses || !pairs || !manifolds) return fail(PQ_ERR_INVALID, "pq_contact_manifolds_pfm_update: null buffer");
+    void *ws = workspace;
+    if (!ws) PQ_HIP_CHECK(stream_workspace((hipStream_t)stream, manifold_workspace_bytes(m), &ws));
+    PQ_HIP_CHECK(launch_manifolds_update(s->dev, (const float *)poses, pairs, m, prediction, manifolds, ws,
+                                         (hipStream_t)stream));
+    return PQ_OK;
+}
+
 }  // extern "C"

@@ -272,8 +272,9 @@ __global__ __launch_bounds__(kBlock) void contact_gjk_kernel(
                 sx_reset(s, cso_from_shapes(seed_pos, ga, gb, dir));
                 r = gjk_closest_points(gjk_pos, ga, gb, max_dist, s);
                 gi += r.iters;
-                // a seeded run that reaches the cap falls back to the unseeded GJK
-                if (!(seeded && attempt == 0 && r.iters == kGjkMaxIters)) break;
+                // a seeded run that reaches the cap falls back to the unseeded GJK,
+                // except for a manifold's cached normal, which parry takes as is
+                if (!(seeded && !pfm && attempt == 0 && r.iters == kGjkMaxIters)) break;
                 dir = dflt;
             }
             if (is_ball_mode(ball_mode)) {

@@ -22,6 +22,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr float kCosFracPi8 = 0.92387953251f;  // utils/consts.rs:11
+constexpr float kCos1Degrees = 0.99984769515f;  // utils/consts.rs:9
 constexpr uint32_t kPfVertex = 1u << 30, kPfEdge = 2u << 30, kPfFace = 3u << 30;  // feature_id.rs:466-471
 
 struct Feature {
@@ -426,13 +427,18 @@ __device__ void contacts_face_face(const iso &pos12, const Feature &f1, f3 sep, 
     }
 }
 
+// The manifold of pair k of `pairs` (its GJK/EPA contact in contacts[k]) goes to
+// out[list[k]], or out[k] without a list; m_dev, if given, is the device-side
+// number of pairs.
 __global__ __launch_bounds__(kBlock) void manifold_kernel(const DeviceShapes s, const float *__restrict__ poses,
                                                           const uint32_t *__restrict__ pairs, uint64_t m,
+                                                          const unsigned long long *__restrict__ m_dev,
+                                                          const uint32_t *__restrict__ list,
                                                           const pq_contact *__restrict__ contacts,
                                                           pq_contact_manifold *__restrict__ out) {
     const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (k >= m) return;
-    pq_contact_manifold *const mf = out + k;
+    if (k >= (m_dev ? (uint64_t)*m_dev : m)) return;
+    pq_contact_manifold *const mf = out + (list ? list[k] : k);
     const pq_contact c = contacts[k];
     Out o{mf, 0u};
     f3 n1{0.0f, 0.0f, 0.0f}, n2{0.0f, 0.0f, 0.0f};
@@ -477,9 +483,97 @@ __global__ __launch_bounds__(kBlock) void manifold_kernel(const DeviceShapes s, 
     mf->num_points = o.n;
 }
 
+// ContactManifold::try_update_contacts_eps (contact_manifold.rs:662-699) with the
+// thresholds contact_manifold_pfm_pfm gives it (COS_1_DEGREES and 1e-6,
+// contact_manifolds_pfm_pfm.rs:60-65), one lane per pair.  A manifold it keeps is
+// updated in place and marked PQ_MANIFOLD_UPDATED.  Every other pair goes on the
+// list the contact and manifold kernels run on, with its GJK seed: the cached
+// normal as Unit::try_new(local_n1, DEFAULT_EPSILON) makes it (zero, for parry's
+// default direction, when that is None).  parry writes each point as it checks
+// it and drops them all when one fails; checking them all before writing any
+// leaves the same manifold.
+__global__ __launch_bounds__(kBlock) void manifold_update_kernel(const float *__restrict__ poses,
+                                                                 const uint32_t *__restrict__ pairs, uint64_t m,
+                                                                 pq_contact_manifold *__restrict__ mf,
+                                                                 uint32_t *__restrict__ list,
+                                                                 uint32_t *__restrict__ sub_pairs,
+                                                                 float *__restrict__ seeds,
+                                                                 unsigned long long *__restrict__ pending) {
+    const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool pend = false;
+    uint32_t i = 0, j = 0;
+    f3 seed{0.0f, 0.0f, 0.0f};
+    if (k < m) {
+        i = pairs[2 * k];
+        j = pairs[2 * k + 1];
+        pq_contact_manifold *const M = mf + k;
+        const uint32_t np = M->num_points;
+        const f3 n1{M->local_n1[0], M->local_n1[1], M->local_n1[2]};
+        bool kept = np != 0u && np <= (uint32_t)PQ_MANIFOLD_MAX_POINTS;
+        if (kept) {
+            const iso pos12 = iso_inv_mul(load_iso(poses + 7 * (size_t)i), load_iso(poses + 7 * (size_t)j));
+            const f3 ln2 = qrot(pos12.q, f3{M->local_n2[0], M->local_n2[1], M->local_n2[2]});
+            kept = !(-dot3(n1, ln2) < kCos1Degrees);
+            for (uint32_t q = 0; kept && q < np; ++q) {
+                const pq_tracked_contact &t = M->points[q];
+                const f3 p1{t.local_p1[0], t.local_p1[1], t.local_p1[2]};
+                const f3 lp2 = iso_point(pos12, f3{t.local_p2[0], t.local_p2[1], t.local_p2[2]});
+                const float dist = dot3(sub3(lp2, p1), n1);
+                if (dist * t.dist < 0.0f) kept = false;  // between penetrating and not
+                else if (norm_sq3(sub3(sub3(lp2, mul3(n1, dist)), p1)) > 1.0e-6f) kept = false;
+            }
+            if (kept) {
+                for (uint32_t q = 0; q < np; ++q) {
+                    pq_tracked_contact &t = M->points[q];
+                    const f3 p1{t.local_p1[0], t.local_p1[1], t.local_p1[2]};
+                    const f3 lp2 = iso_point(pos12, f3{t.local_p2[0], t.local_p2[1], t.local_p2[2]});
+                    const float dist = dot3(sub3(lp2, p1), n1);
+                    const f3 np1 = sub3(lp2, mul3(n1, dist));
+                    t.dist = dist;
+                    t.local_p1[0] = np1.x;
+                    t.local_p1[1] = np1.y;
+                    t.local_p1[2] = np1.z;
+                }
+                M->status = PQ_MANIFOLD_UPDATED;
+            }
+        }
+        if (!kept) {
+            pend = true;
+            const float sq = norm_sq3(n1);
+            if (sq > kEps * kEps) seed = div3(n1, sqrtf(sq));
+        }
+    }
+    // Wave-aggregated append of the pairs to recompute.
+    const uint64_t mask = __ballot(pend);
+    if (mask) {
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(pending, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader);
+        if (pend) {
+            const uint64_t p = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            list[p] = (uint32_t)k;
+            sub_pairs[2 * p] = i;
+            sub_pairs[2 * p + 1] = j;
+            seeds[3 * p] = seed.x;
+            seeds[3 * p + 1] = seed.y;
+            seeds[3 * p + 2] = seed.z;
+        }
+    }
+}
+
+__global__ void zero_pending_kernel(unsigned long long *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
+}
+
 struct ManifoldWorkspace {
     pq_contact *contacts;
     void *contact_ws;
+    uint32_t *list, *sub_pairs;
+    float *seeds;
+    unsigned long long *pending;
     size_t bytes;
 };
 
@@ -495,6 +589,10 @@ ManifoldWorkspace carve(void *base, uint64_t m) {
     ManifoldWorkspace w{};
     w.contacts = (pq_contact *)take(m * sizeof(pq_contact));
     w.contact_ws = take(contact_workspace_bytes(m));
+    w.list = (uint32_t *)take(m * sizeof(uint32_t));
+    w.sub_pairs = (uint32_t *)take(2 * m * sizeof(uint32_t));
+    w.seeds = (float *)take(3 * m * sizeof(float));
+    w.pending = (unsigned long long *)take(sizeof(unsigned long long));
     w.bytes = off;
     return w;
 }
@@ -511,7 +609,26 @@ hipError_t launch_manifolds(const DeviceShapes &s, const float *poses, const uin
                                    stream, kContactFeatureMaps);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(manifold_kernel, dim3((uint32_t)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, s, poses,
-                       pairs, m, w.contacts, out);
+                       pairs, m, nullptr, nullptr, w.contacts, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_manifolds_update(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
+                                   float prediction, pq_contact_manifold *manifolds, void *workspace,
+                                   hipStream_t stream) {
+    if (m == 0) return hipSuccess;
+    const ManifoldWorkspace w = carve(workspace, m);
+    const dim3 grid((uint32_t)((m + kBlock - 1) / kBlock)), block(kBlock);
+    hipLaunchKernelGGL(zero_pending_kernel, dim3(1), dim3(64), 0, stream, w.pending);
+    hipLaunchKernelGGL(manifold_update_kernel, grid, block, 0, stream, poses, pairs, m, manifolds, w.list, w.sub_pairs,
+                       w.seeds, w.pending);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = launch_contacts(s, poses, w.sub_pairs, m, (const uint64_t *)w.pending, prediction, w.seeds, w.contacts,
+                        nullptr, w.contact_ws, stream, kContactFeatureMaps);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(manifold_kernel, grid, block, 0, stream, s, poses, w.sub_pairs, m, w.pending, w.list,
+                       w.contacts, manifolds);
     return hipGetLastError();
 }
 

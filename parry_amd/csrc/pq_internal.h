@@ -192,6 +192,9 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
                            uint32_t *iters, void *workspace, hipStream_t stream, uint32_t flags = 0u);
 // Contact manifolds between polygonal feature maps (manifold.hip).
 size_t manifold_workspace_bytes(uint64_t m);
+hipError_t launch_manifolds_update(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
+                                   float prediction, pq_contact_manifold *manifolds, void *workspace,
+                                   hipStream_t stream);
 hipError_t launch_manifolds(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
                             float prediction, pq_contact_manifold *out, void *workspace, hipStream_t stream);
 // Linear shape casts (cast.hip).

@@ -275,3 +275,34 @@ def contact_manifolds_pfm(shapes, poses, pairs, prediction: float, out=None, wor
         shapes.handle, poses.data_ptr(), pairs.data_ptr(), m, float(prediction), out.data_ptr(),
         None if workspace is None else workspace.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
     return out
+
+
+MANIFOLD_UPDATED = 2  # PQ_MANIFOLD_UPDATED: try_update_contacts_eps kept the manifold
+
+
+def contact_manifolds_pfm_update(shapes, poses, pairs, prediction: float, manifolds, workspace=None, stream=None):
+    """``contact_manifold_pfm_pfm`` (contact_manifolds_pfm_pfm.rs:44-162) on last frame's
+    manifolds, in place: ``manifolds`` is the (m, 932) uint8 CUDA tensor of
+    ``pq_contact_manifold`` records a previous call left (zeros for new pairs).  Kept
+    manifolds (try_update_contacts_eps, contact_manifold.rs:662-699) come back with
+    status ``MANIFOLD_UPDATED``; the others are recomputed with GJK seeded by the
+    cached normal.  Returns ``manifolds``."""
+    import ctypes
+
+    import torch
+
+    from . import _native
+
+    m = pairs.shape[0]
+    for t in (poses, pairs, manifolds):
+        if not (t.is_cuda and t.is_contiguous()):
+            raise ValueError("contact_manifolds_pfm_update: tensors must be contiguous CUDA tensors")
+    if tuple(poses.shape) != (len(shapes), 7) or poses.dtype != torch.float32:
+        raise ValueError(f"contact_manifolds_pfm_update: poses must be float32 ({len(shapes)}, 7)")
+    if tuple(manifolds.shape) != (m, CONTACT_MANIFOLD_DTYPE.itemsize) or manifolds.dtype != torch.uint8:
+        raise ValueError(f"contact_manifolds_pfm_update: manifolds must be uint8 ({m}, {CONTACT_MANIFOLD_DTYPE.itemsize})")
+    s = stream if stream is not None else torch.cuda.current_stream(poses.device)
+    _native.check(_native.lib().pq_contact_manifolds_pfm_update(
+        shapes.handle, poses.data_ptr(), pairs.data_ptr(), m, float(prediction), manifolds.data_ptr(),
+        None if workspace is None else workspace.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
+    return manifolds

@@ -7,9 +7,11 @@ oracle; local_n1, local_n2 and every point's local_p1, local_p2 and dist within
 manifolds is printed.  Cases: random mixes of Cuboid, Capsule, Cone and
 Triangle in contact (face-face, edge-edge and mixed clipping, EPA for the
 penetrating ones), resting stacks, separated pairs (the cached direction), and
-pairs with a Ball or a ConvexPolyhedron without faces (unsupported), and
+pairs with a Ball or a ConvexPolyhedron without faces (unsupported),
 ConvexPolyhedra given their faces (random hulls, boxes) against each other and
-the other feature maps.
+the other feature maps, and four frames of the update from the previous frame's
+manifolds (pq_contact_manifolds_pfm_update) with shapes at rest, barely moving
+and moving: each frame, the oracle starts from the GPU's previous manifolds.
 """
 import numpy as np
 import pytest
@@ -68,7 +70,7 @@ def _compare(label, got, ref):
         assert err.max(initial=0) <= TOL, f"{label}: {name} error {err.max()}"
     exact = 0
     worst = 0.0
-    for k in np.nonzero(st == 1)[0]:
+    for k in np.nonzero((st == 1) | (st == 2))[0]:
         c = int(ref["count"][k])
         p = got["points"][k, :c]
         assert np.array_equal(p["fid1"], ref["fid1"][k, :c]) and np.array_equal(p["fid2"], ref["fid2"][k, :c]), (
@@ -78,8 +80,8 @@ def _compare(label, got, ref):
         err = np.abs(vals - rv) / np.maximum(1.0, np.abs(rv))
         worst = max(worst, float(err.max(initial=0)))
         exact += int(np.array_equal(vals, rv))
-    npts = int(ref["count"][st == 1].sum())
-    print(f"{label}: {int((st == 1).sum())} manifolds ({npts} points), {exact} bit-identical, "
+    npts = int(ref["count"][(st == 1) | (st == 2)].sum())
+    print(f"{label}: {int((st == 1).sum())} manifolds + {int((st == 2).sum())} kept ({npts} points), {exact} bit-identical, "
           f"{int((st == 0).sum())} separated, {int((st == UNSUP).sum())} unsupported, max rel err {worst:.3e}",
           flush=True)
     assert worst <= TOL
@@ -176,3 +178,55 @@ def test_polyhedra_with_faces_match_oracle(orc):
     print("case: the same pairs, polyhedra without faces", flush=True)
     st = _compare("no faces", _run(d, verts, poses, pairs, 0.02), orc.manifolds_pfm(d, verts, poses, pairs, 0.02))
     assert (st[poly] == UNSUP).all()
+
+
+def _as_oracle(rec):
+    """GPU manifold records -> the oracle's manifold dict (its `previous` input)."""
+    st = rec["status"].astype(np.int64)
+    st[st == UNSUP] = -1
+    pts = rec["points"]
+    return {"status": st.astype(np.int32), "n1": rec["local_n1"].copy(), "n2": rec["local_n2"].copy(),
+            "count": rec["num_points"].copy(), "p1": pts["local_p1"].copy(), "p2": pts["local_p2"].copy(),
+            "dist": pts["dist"].copy(), "fid1": pts["fid1"].copy(), "fid2": pts["fid2"].copy()}
+
+
+def test_update_from_previous_frames_matches_oracle(orc):
+    import torch
+
+    from parry_amd import contact_manifolds_pfm_update
+
+    n = 30000
+    d, verts, poses, pairs = _scene(n, 17, 0.6, types=(0, 1, 2, 3, 5))
+    shapes = ShapeSet(d, verts)
+    tp = torch.from_numpy(pairs.view(np.int32)).cuda()
+    mf = torch.zeros((n, CONTACT_MANIFOLD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    rng = np.random.default_rng(5)
+    prev = None
+    kept_total = 0
+    for frame in range(4):
+        if frame:
+            # a third of the shapes rest, a third move by ~1e-4, a third by ~0.02 and turn a little
+            kind = rng.integers(0, 3, 2 * n)
+            step = np.where(kind == 1, 1e-4, np.where(kind == 2, 0.02, 0.0))[:, None]
+            poses[:, 4:] += (rng.normal(0, 1, (2 * n, 3)) * step).astype(np.float32)
+            turn = kind == 2
+            dq = np.concatenate([rng.normal(0, 0.01, (int(turn.sum()), 3)), np.ones((int(turn.sum()), 1))], 1)
+            q = poses[turn, :4].astype(np.float64)
+            # q <- dq * q (x, y, z, w), renormalized
+            x1, y1, z1, w1 = dq.T
+            x2, y2, z2, w2 = q.T
+            nq = np.stack([w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2, w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+                           w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2, w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2], 1)
+            poses[turn, :4] = (nq / np.linalg.norm(nq, axis=1, keepdims=True)).astype(np.float32)
+        print(f"case: manifold update, frame {frame}, {n} pairs", flush=True)
+        ref = orc.manifolds_pfm(d, verts, poses, pairs, 0.02, previous=prev if prev is not None else
+                                _as_oracle(np.zeros(n, CONTACT_MANIFOLD_DTYPE)))
+        contact_manifolds_pfm_update(shapes, torch.from_numpy(poses).cuda(), tp, 0.02, mf)
+        torch.cuda.synchronize()
+        got = mf.cpu().numpy().view(CONTACT_MANIFOLD_DTYPE).reshape(-1)
+        st = _compare(f"frame {frame}", got, ref)
+        kept_total += int((st == 2).sum())
+        if frame == 0:
+            assert (st == 2).sum() == 0
+        prev = _as_oracle(got)
+    assert kept_total > 5000

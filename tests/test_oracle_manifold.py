@@ -21,7 +21,13 @@ on a fresh manifold.
     sets (the clipping walks the faces from other vertices, so points agree to
     1e-4 but for crossings of nearly parallel edges), each vertex id naming the
     polyhedron vertex at the point and each face id a face whose plane holds it;
-    a polyhedron without faces has no feature map.
+    a polyhedron without faces has no feature map;
+  * the update from a previous frame (try_update_contacts_eps,
+    contact_manifold.rs:662-699): a zero manifold gives the fresh result bit for
+    bit; a 1e-4 rise keeps the points (status 2) with dist + 1e-4 and local_p1 the
+    projection of the moved local_p2; a lateral move of 0.01, a 2 degree tilt and a
+    point crossing from penetration to separation are recomputed (status 1); a
+    separated pair's cached direction seeds GJK to the same separation.
 """
 import numpy as np
 
@@ -269,3 +275,70 @@ def test_box_polyhedron_matches_cuboid():
     # no faces: no feature map
     r = orc.manifolds_pfm(dp[:2], verts[:16], poses[:2], np.array([[0, 1]], np.uint32), 0.02)
     assert r["status"][0] == -1
+
+
+def _frame(d, poses, prev, verts=None):
+    v = np.zeros((0, 3), np.float32) if verts is None else verts
+    return orc.manifolds_pfm(d, v, np.asarray(poses, np.float32), np.array([[0, 1]], np.uint32), 0.02, previous=prev)
+
+
+def test_manifold_update_keeps_or_recomputes():
+    # try_update_contacts_eps (contact_manifold.rs:662-699) on the resting cuboid of
+    # test_cuboid_resting_on_cuboid, then seeded recomputation when it refuses
+    d = _descs([1, 1], [[1, 1, 1], [0.5, 0.5, 0.5]])
+    base = [0, 0, 0, 1, 0.2, 0.1, 1.4]
+    r0 = _frame(d, [[0, 0, 0, 1, 0, 0, 0], base], None)
+    assert r0["status"][0] == 1 and r0["count"][0] == 5
+    # a fresh manifold through the update path is the fresh result
+    zero = {k: np.zeros_like(v) for k, v in r0.items()}
+    rz = _frame(d, [[0, 0, 0, 1, 0, 0, 0], base], zero)
+    for k in ("status", "count", "n1", "n2", "p1", "p2", "dist", "fid1", "fid2"):
+        assert np.array_equal(rz[k], r0[k]), k
+    # shape 2 rises by 1e-4: every point moves by less than 1e-3, the normal is
+    # unchanged: kept, dist + 1e-4, local_p1 the projection of the moved local_p2
+    up = list(base)
+    up[6] += 1e-4
+    r1 = _frame(d, [[0, 0, 0, 1, 0, 0, 0], up], r0)
+    assert r1["status"][0] == 2 and r1["count"][0] == 5
+    for k in ("n1", "n2", "p2", "fid1", "fid2"):
+        assert np.array_equal(r1[k], r0[k]), k
+    c = 5
+    dz = np.float32(up[6]) - np.float32(base[6])
+    np.testing.assert_allclose(r1["dist"][0, :c], r0["dist"][0, :c] + dz, atol=1e-6)
+    lp2 = r0["p2"][0, :c].astype(np.float64) + np.array([0.2, 0.1, float(np.float32(up[6]))])
+    np.testing.assert_allclose(r1["p1"][0, :c], lp2 - np.outer(r1["dist"][0, :c], [0, 0, 1]), atol=1e-6)
+    # a lateral move of 0.01 moves each point by 1e-4 squared > 1e-6: recomputed (status 1),
+    # as a fresh manifold would be; GJK's own point (the last) is any point of the
+    # overlap and depends on the seed, the four clipped corners do not
+    side = list(base)
+    side[4] += 0.01
+    r2 = _frame(d, [[0, 0, 0, 1, 0, 0, 0], side], r0)
+    fresh = _frame(d, [[0, 0, 0, 1, 0, 0, 0], side], None)
+    assert r2["status"][0] == 1 and r2["count"][0] == fresh["count"][0] == 5
+    np.testing.assert_allclose(r2["p1"][0, :4], fresh["p1"][0, :4], atol=1e-5)
+    np.testing.assert_allclose(r2["dist"][0, :c], fresh["dist"][0, :c], atol=1e-5)
+    # a 2 degree turn of shape 2 about z keeps the normal (rotation about it) but
+    # one about x tilts it beyond cos(1 degree): recomputed
+    a = np.radians(2.0) / 2
+    tilt = [np.sin(a), 0, 0, np.cos(a), 0.2, 0.1, 1.4]
+    r3 = _frame(d, [[0, 0, 0, 1, 0, 0, 0], tilt], r0)
+    assert r3["status"][0] == 1
+    # penetration to separation by 2e-5 on a point at -1e-5: recomputed even
+    # though the points barely move
+    near = list(base)
+    near[6] = 1.5 - 1e-5
+    rn = _frame(d, [[0, 0, 0, 1, 0, 0, 0], near], None)
+    assert rn["status"][0] == 1 and (rn["dist"][0, : rn["count"][0]] < 0).all()
+    away = list(near)
+    away[6] += 2e-5
+    ra = _frame(d, [[0, 0, 0, 1, 0, 0, 0], away], rn)
+    assert ra["status"][0] == 1
+    rb = _frame(d, [[0, 0, 0, 1, 0, 0, 0], near], rn)  # no motion: kept
+    assert rb["status"][0] == 2
+    # separated: the cached direction seeds GJK, which finds the same separation
+    far = [0, 0, 0, 1, 0.2, 0.1, 1.6]
+    rs = _frame(d, [[0, 0, 0, 1, 0, 0, 0], far], None)
+    assert rs["status"][0] == 0
+    rs2 = _frame(d, [[0, 0, 0, 1, 0, 0, 0], far], rs)
+    assert rs2["status"][0] == 0
+    np.testing.assert_allclose(rs2["n1"][0], [0, 0, 1], atol=1e-5)  # GJK's separating direction
