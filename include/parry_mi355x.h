@@ -336,9 +336,10 @@ typedef struct pq_contact {
  *       4 = 48 vertices / 224 faces, all three without staging (less LDS per pair
  *       means more pairs resident; a pair whose polytope outgrows the buffers
  *       moves on to the one-wave-per-pair tiers);
- *   epa_schedule: with fewer than 64 lanes per pair, 0 = a group of lanes takes
- *       its next pair as soon as it is done with one, 1 = a wave takes its next
- *       pairs when all its groups are done.
+ *   epa_schedule: with fewer than 64 lanes per pair, 1 = a wave takes its next
+ *       pairs when all its groups are done, 2 = a group of lanes takes its next
+ *       pair as soon as it is done with one, 0 = automatic: 1 for the
+ *       TriMesh-shape contact (short EPA runs), 2 everywhere else.
  * The default is 16 lanes per pair with polytope 1 and schedule 0. */
 typedef struct pq_contact_tuning {
     uint32_t epa_lanes_per_pair;

@@ -334,10 +334,10 @@ size_t pq_contact_workspace_bytes(uint64_t num_pairs) { return contact_workspace
 int pq_contact_set_tuning(const pq_contact_tuning *t) {
     if (!t) return fail(PQ_ERR_INVALID, "pq_contact_set_tuning: null tuning");
     const uint32_t g = t->epa_lanes_per_pair;
-    if ((g != 64 && g != 32 && g != 16 && g != 8 && g != 4) || t->epa_polytope > 4 || t->epa_schedule > 1 ||
+    if ((g != 64 && g != 32 && g != 16 && g != 8 && g != 4) || t->epa_polytope > 4 || t->epa_schedule > 2 ||
         t->reserved)
         return fail(PQ_ERR_INVALID, "pq_contact_set_tuning: epa_lanes_per_pair must be 64, 32, 16, 8 or 4, "
-                                    "epa_polytope 0 to 4 and epa_schedule 0 or 1");
+                                    "epa_polytope 0 to 4 and epa_schedule 0 to 2");
     g_epa_lanes_per_pair = (int)g;
     g_epa_polytope = (int)t->epa_polytope;
     g_epa_schedule = (int)t->epa_schedule;

@@ -246,7 +246,7 @@ hipError_t launch_mesh_contacts(const DeviceMesh &m, const DeviceShapes &combine
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (capacity) {
         e = launch_contacts(combined, w.poses, w.pairs, capacity, (const uint64_t *)w.total, prediction, nullptr, w.cand, nullptr,
-                            w.contact_ws, stream);
+                            w.contact_ws, stream, kContactShortEpa);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(mesh_contact_select_kernel, grid, block, 0, stream, w.offsets, n, w.pairs, w.cand, m.tris,

@@ -1353,13 +1353,12 @@ hipError_t launch_epa_tier(const DeviceShapes &s, const float *poses, const uint
                            uint64_t max_items, const EpaItem *queue, const uint32_t *in_list, const uint32_t *in_count,
                            uint32_t *next, const uint32_t *in_cont, const EpaCont *cont_recs, ContSink sink,
                            DevContact *out, uint32_t *iters, uint32_t *out_list, uint32_t *out_cont,
-                           uint32_t *out_count, float *orig, uint64_t max_groups, hipStream_t stream) {
+                           uint32_t *out_count, float *orig, uint64_t max_groups, hipStream_t stream,
+                           bool refill = false) {
     hipError_t e;
     int dev = 0, cus = 0, per_cu = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    bool refill = false;
-    if constexpr (G < 64) refill = g_epa_schedule == 0;
     const void *kernel = (const void *)epa_tier_kernel<G, W>;
     if constexpr (G < 64) {
         if (refill) kernel = (const void *)epa_refill_kernel<G, W>;
@@ -1460,20 +1459,24 @@ namespace {
 // overflowing pairs on with continuation records.
 template <int G, class W>
 hipError_t launch_first(const DeviceShapes &s, const float *poses, const uint32_t *pairs, float prediction, uint64_t m,
-                        const ContactWorkspace &w, uint32_t *c, DevContact *o, uint32_t *iters, hipStream_t stream) {
+                        const ContactWorkspace &w, uint32_t *c, DevContact *o, uint32_t *iters, hipStream_t stream,
+                        bool refill) {
     return launch_epa_tier<G, W>(s, poses, pairs, prediction, m, w.queue, nullptr, c, c + 4, nullptr, nullptr,
                                  ContSink{w.conts, c + 8, w.cont_cap}, o, iters, w.lists[0], w.cont_list, c + 1,
-                                 w.orig_small, kEpaMaxGroups, stream);
+                                 w.orig_small, kEpaMaxGroups, stream, refill);
 }
 template <int G>
 hipError_t launch_small(const DeviceShapes &s, const float *poses, const uint32_t *pairs, float prediction, uint64_t m,
-                        const ContactWorkspace &w, uint32_t *c, DevContact *o, uint32_t *iters, hipStream_t stream) {
+                        const ContactWorkspace &w, uint32_t *c, DevContact *o, uint32_t *iters, hipStream_t stream,
+                        uint32_t flags) {
+    // the first tier's schedule: a refill per group, except for short EPA runs
+    const bool refill = G < 64 && (g_epa_schedule == 2 || (g_epa_schedule == 0 && !(flags & kContactShortEpa)));
     switch (g_epa_polytope) {
-        case 1: return launch_first<G, EpaSmallUnstaged>(s, poses, pairs, prediction, m, w, c, o, iters, stream);
-        case 2: return launch_first<G, EpaTiny>(s, poses, pairs, prediction, m, w, c, o, iters, stream);
-        case 3: return launch_first<G, EpaSmallWide>(s, poses, pairs, prediction, m, w, c, o, iters, stream);
-        case 4: return launch_first<G, EpaSmallLarge>(s, poses, pairs, prediction, m, w, c, o, iters, stream);
-        default: return launch_first<G, EpaSmall>(s, poses, pairs, prediction, m, w, c, o, iters, stream);
+        case 1: return launch_first<G, EpaSmallUnstaged>(s, poses, pairs, prediction, m, w, c, o, iters, stream, refill);
+        case 2: return launch_first<G, EpaTiny>(s, poses, pairs, prediction, m, w, c, o, iters, stream, refill);
+        case 3: return launch_first<G, EpaSmallWide>(s, poses, pairs, prediction, m, w, c, o, iters, stream, refill);
+        case 4: return launch_first<G, EpaSmallLarge>(s, poses, pairs, prediction, m, w, c, o, iters, stream, refill);
+        default: return launch_first<G, EpaSmall>(s, poses, pairs, prediction, m, w, c, o, iters, stream, refill);
     }
 }
 }  // namespace
@@ -1492,11 +1495,11 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
     if (e != hipSuccess) return e;
     uint32_t *const c = w.counters;
     switch (g_epa_lanes_per_pair) {
-        case 4: e = launch_small<4>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
-        case 8: e = launch_small<8>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
-        case 16: e = launch_small<16>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
-        case 32: e = launch_small<32>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
-        default: e = launch_small<64>(s, poses, pairs, prediction, m, w, c, o, iters, stream); break;
+        case 4: e = launch_small<4>(s, poses, pairs, prediction, m, w, c, o, iters, stream, flags); break;
+        case 8: e = launch_small<8>(s, poses, pairs, prediction, m, w, c, o, iters, stream, flags); break;
+        case 16: e = launch_small<16>(s, poses, pairs, prediction, m, w, c, o, iters, stream, flags); break;
+        case 32: e = launch_small<32>(s, poses, pairs, prediction, m, w, c, o, iters, stream, flags); break;
+        default: e = launch_small<64>(s, poses, pairs, prediction, m, w, c, o, iters, stream, flags); break;
     }
     if (e != hipSuccess) return e;
     const ContSink none{nullptr, nullptr, 0};

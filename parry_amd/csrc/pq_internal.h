@@ -187,6 +187,11 @@ constexpr uint32_t kContactLocalSupportMaps = 1u;
 // separating direction in normal1 of a pair without contact; a pair without
 // feature maps gets PQ_CONTACT_UNSUPPORTED.
 constexpr uint32_t kContactFeatureMaps = 2u;
+// kContactShortEpa: the pairs that reach EPA need few iterations each (a Triangle
+// against a shape), so the first EPA tier's automatic schedule takes them per
+// wave rather than refilling each group (the set-up of a pair would then run
+// beside the other groups' iterations about once per iteration).
+constexpr uint32_t kContactShortEpa = 4u;
 hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint32_t *pairs, uint64_t m,
                            const uint64_t *m_dev, float prediction, const float *seeds, pq_contact *out,
                            uint32_t *iters, void *workspace, hipStream_t stream, uint32_t flags = 0u);
@@ -205,7 +210,7 @@ hipError_t launch_casts(const DeviceShapes &s, const float *poses, const float *
 // Lanes that run one pair's EPA together (64, 16 or 8), set by pq_contact_set_tuning.
 extern int g_epa_lanes_per_pair;
 extern int g_epa_polytope;
-extern int g_epa_schedule;  // first EPA tier: 0 = a refill per group, 1 = per wave
+extern int g_epa_schedule;  // first EPA tier: 0 = automatic, 1 = per wave, 2 = a refill per group
 // Per-(device, stream) workspace owned by the library, grown on demand.
 hipError_t stream_workspace(hipStream_t stream, size_t bytes, void **ptr);
 

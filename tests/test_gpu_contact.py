@@ -225,7 +225,7 @@ def test_every_epa_tuning_matches_oracle(orc):
     try:
         settings = [(64, poly, 0) for poly in range(5)]
         for g in (32, 16, 8, 4):
-            settings += [(g, poly, 0) for poly in range(5)] + [(g, 0, 1), (g, 1, 1)]
+            settings += [(g, poly, 0) for poly in range(5)] + [(g, 0, 1), (g, 1, 1), (g, 1, 2)]
         for g, poly, sched in settings:
             _native.set_contact_tuning(g, poly, sched)
             for k, (label, shapes, _, _, p, pr, pred) in enumerate(cases):
