@@ -327,11 +327,31 @@ def pair_set(pairs) -> set:
 # ---------------------------------------------------------------------------
 
 _SIG.update({
-    "orc_contact_pairs_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _int]),
-    "orc_contact_pairs_f64": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f64, _vp, _vp, _vp, _vp, _int]),
+    "orc_contact_pairs_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp, _int]),
+    "orc_contact_pairs_f64": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f64, _vp, _vp, _vp, _vp, _vp, _int]),
 })
 
 SHAPE_TRIANGLE_TEST_ONLY = 5
+_SIG["orc_local_support_f32"] = (None, [_vp, _vp, _u32, _vp, _u64, _vp])
+_SIG["orc_local_support_f64"] = (None, [_vp, _vp, _u32, _vp, _u64, _vp])
+
+
+def local_support_points(desc, vertices, dirs, f64: bool = False) -> np.ndarray:
+    """SupportMap::local_support_point of one shape (a SHAPE_DESC_DTYPE record) along
+    each row of dirs (n, 3)."""
+    from parry_amd.shape import SHAPE_DESC_DTYPE
+
+    dt = np.float64 if f64 else np.float32
+    d = np.ascontiguousarray(np.atleast_1d(desc)[:1], dtype=SHAPE_DESC_DTYPE)
+    v = np.ascontiguousarray(vertices if vertices is not None else np.zeros((0, 3)), dtype=dt).reshape(-1, 3)
+    dr = np.ascontiguousarray(dirs, dtype=dt).reshape(-1, 3)
+    out = np.zeros_like(dr)
+    f = lib().orc_local_support_f64 if f64 else lib().orc_local_support_f32
+    f(d.ctypes.data, v.ctypes.data if v.size else None, v.shape[0], dr.ctypes.data, dr.shape[0], out.ctypes.data)
+    return out
+
+# EpaExit (parry_oracle_contact.hpp): how EPA::closest_points (epa3.rs:428-651) ended.
+EPA_EXIT_NAMES = ("not_run", "converged", "stuck", "numerical", "iter_cap", "heap_empty", "failed", "point")
 _SIG["orc_shape_aabbs_f32"] = (None, [_vp, _u32, _vp, _u32, _vp, _vp])
 
 
@@ -350,10 +370,13 @@ def shape_aabbs(descs, vertices, poses) -> np.ndarray:
 
 def contact_pairs(descs, vertices, poses, pairs, prediction: float, init_dirs=None, f64: bool = False,
                   threads: int | None = None):
-    """query::contact on every pair; returns dict(status int32 [m], out [m,13], iters uint32 [m,2]).
+    """query::contact on every pair; returns dict(status int32 [m], out [m,13], iters uint32 [m,2],
+    epa_exit uint32 [m]).
 
     out = point1, point2, normal1, normal2 (world frame), dist.  status: 1
-    contact, 0 None, -1 pair type not handled by the restatement.
+    contact, 0 None, -1 pair type not handled by the restatement.  iters: GJK
+    iterations, EPA iterations + 1 (0 without EPA).  epa_exit: how EPA ended
+    (EPA_EXIT_NAMES), 0 without EPA.
     """
     from parry_amd.shape import SHAPE_DESC_DTYPE
 
@@ -366,6 +389,7 @@ def contact_pairs(descs, vertices, poses, pairs, prediction: float, init_dirs=No
     out = np.zeros((m, 13), dtype=dt)
     status = np.zeros(m, np.int32)
     iters = np.zeros((m, 2), np.uint32)
+    epa_exit = np.zeros(m, np.uint32)
     seeds = None
     if init_dirs is not None:
         seeds = np.ascontiguousarray(init_dirs, dtype=dt).reshape(-1, 3)
@@ -373,8 +397,8 @@ def contact_pairs(descs, vertices, poses, pairs, prediction: float, init_dirs=No
     f = lib().orc_contact_pairs_f64 if f64 else lib().orc_contact_pairs_f32
     f(d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data, pr.ctypes.data, m,
       prediction, None if seeds is None else seeds.ctypes.data, out.ctypes.data, status.ctypes.data,
-      iters.ctypes.data, default_threads() if threads is None else threads)
-    return {"status": status, "out": out, "iters": iters}
+      iters.ctypes.data, epa_exit.ctypes.data, default_threads() if threads is None else threads)
+    return {"status": status, "out": out, "iters": iters, "epa_exit": epa_exit}
 
 
 CAST_NONE, CAST_OUT_OF_ITERATIONS, CAST_CONVERGED, CAST_FAILED, CAST_PENETRATING = range(5)

@@ -406,7 +406,7 @@ void shape_aabbs(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uint3
 template <class T>
 void contact_pairs(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uint32_t nverts, const T *poses,
                    const uint32_t *pairs, uint64_t m, T prediction, const T *init_dirs, T *out, int32_t *status,
-                   uint32_t *iters, int nthreads) {
+                   uint32_t *iters, uint32_t *epa_exit, int nthreads) {
     std::vector<V3<T>> pts(nverts);
     for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
     std::vector<Shape<T>> shapes = make_shapes(descs, nshapes, pts);
@@ -440,13 +440,35 @@ void contact_pairs(const ShapeDesc *descs, uint32_t nshapes, const T *verts, uin
                 iters[2 * k] = st.gjk_iters;
                 iters[2 * k + 1] = st.used_epa ? st.epa_iters + 1 : 0;
             }
+            if (epa_exit) epa_exit[k] = st.used_epa ? st.epa_exit : EPA_NOT_RUN;
         }
     });
+}
+
+template <class T>
+void local_supports(const ShapeDesc *desc, const T *verts, uint32_t nverts, const T *dirs, uint64_t n, T *out) {
+    std::vector<V3<T>> pts(nverts);
+    for (uint32_t k = 0; k < nverts; ++k) pts[k] = v3(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]);
+    std::vector<Shape<T>> shapes = make_shapes(desc, 1, pts);
+    for (uint64_t k = 0; k < n; ++k) {
+        V3<T> p = local_support_point(shapes[0], v3(dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]));
+        out[3 * k] = p.x; out[3 * k + 1] = p.y; out[3 * k + 2] = p.z;
+    }
 }
 
 }  // namespace
 
 extern "C" {
+
+// SupportMap::local_support_point of one shape along each of n directions (n x 3).
+void orc_local_support_f32(const void *desc, const float *verts, uint32_t nverts, const float *dirs, uint64_t n,
+                           float *out) {
+    local_supports<float>((const ShapeDesc *)desc, verts, nverts, dirs, n, out);
+}
+void orc_local_support_f64(const void *desc, const double *verts, uint32_t nverts, const double *dirs, uint64_t n,
+                           double *out) {
+    local_supports<double>((const ShapeDesc *)desc, verts, nverts, dirs, n, out);
+}
 
 // Shape::compute_aabb(pose) for every shape (n x 6).
 void orc_shape_aabbs_f32(const void *descs, uint32_t n, const float *verts, uint32_t nverts, const float *poses,
@@ -457,19 +479,22 @@ void orc_shape_aabbs_f32(const void *descs, uint32_t n, const float *verts, uint
 // query::contact for every (i, j) in `pairs` (m pairs).  out: 13 per pair
 // (point1, point2, normal1, normal2, dist, world frame); status: 1 contact,
 // 0 None, -1 shape pair not handled; iters (nullable): GJK iterations and
-// EPA iterations + 1 (0 when EPA did not run).  init_dirs (nullable): GJK seed
+// EPA iterations + 1 (0 when EPA did not run); epa_exit (nullable): how EPA
+// ended (EpaExit, 0 when it did not run).  init_dirs (nullable): GJK seed
 // directions in shape 1's local frame, replacing parry's default seed.
 void orc_contact_pairs_f32(const void *descs, uint32_t nshapes, const float *verts, uint32_t nverts,
                            const float *poses, const uint32_t *pairs, uint64_t m, float prediction,
-                           const float *init_dirs, float *out, int32_t *status, uint32_t *iters, int nthreads) {
+                           const float *init_dirs, float *out, int32_t *status, uint32_t *iters,
+                           uint32_t *epa_exit, int nthreads) {
     contact_pairs<float>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, init_dirs, out,
-                         status, iters, nthreads);
+                         status, iters, epa_exit, nthreads);
 }
 void orc_contact_pairs_f64(const void *descs, uint32_t nshapes, const double *verts, uint32_t nverts,
                            const double *poses, const uint32_t *pairs, uint64_t m, double prediction,
-                           const double *init_dirs, double *out, int32_t *status, uint32_t *iters, int nthreads) {
+                           const double *init_dirs, double *out, int32_t *status, uint32_t *iters,
+                           uint32_t *epa_exit, int nthreads) {
     contact_pairs<double>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, init_dirs,
-                          out, status, iters, nthreads);
+                          out, status, iters, epa_exit, nthreads);
 }
 
 }  // extern "C"

@@ -614,7 +614,24 @@ template <class T> struct EpaFace {
     bool deleted;
 };
 
+// How EPA::closest_points ended (recorded for tests and error reports only).
+// EPA_STUCK is the second clause of epa3.rs:568-572.  It needs
+// candidate_max_dist < max_dist right after max_dist = min(max_dist,
+// candidate_max_dist) (epa3.rs:560-563), so it cannot hold: it is counted
+// separately to show that no pair ever takes it.
+enum EpaExit : uint32_t {
+    EPA_NOT_RUN = 0,
+    EPA_CONVERGED = 1,   // max_dist - curr_dist < 100 EPS (epa3.rs:568)
+    EPA_STUCK = 2,       // |curr_dist - old_dist| < EPS && candidate < max_dist (epa3.rs:569-572)
+    EPA_NUMERICAL = 3,   // a new face closer than the popped one (epa3.rs:614-621): returns the popped face
+    EPA_ITER_CAP = 4,    // more than 100 iterations (epa3.rs:640-645): returns the best face
+    EPA_HEAP_EMPTY = 5,  // every face popped (epa3.rs:547, 648-650): returns the best face
+    EPA_FAILED = 6,      // None (no face, silhouette empty, all silhouette faces deleted, FaceId::new)
+    EPA_POINT = 7,       // simplex of dimension 0 (epa3.rs:449-453)
+};
+
 template <class T> struct Epa {
+    uint32_t exit = EPA_NOT_RUN;
     std::vector<CSO<T>> vertices;
     std::vector<EpaFace<T>> faces;
     std::vector<std::pair<size_t, size_t>> silhouette;  // (face_id, opp_pt_id)
@@ -706,6 +723,7 @@ template <class T> struct Epa {
         faces.clear();
         heap.d.clear();
         silhouette.clear();
+        exit = EPA_FAILED;
         if (iters) *iters = 0;
         for (size_t i = 0; i < simplex.dim + 1; ++i) vertices.push_back(simplex.vertices[i]);
         auto fid_new = [&](size_t id, T neg_dist, EpaFaceId<T> &out) -> bool {
@@ -716,6 +734,7 @@ template <class T> struct Epa {
         if (simplex.dim == 0) {
             p1 = p2 = v3(T(0), T(0), T(0));
             n = v3(T(0), T(1), T(0));
+            exit = EPA_POINT;
             return true;
         } else if (simplex.dim == 3) {
             V3<T> dp1 = vertices[1].point - vertices[0].point;
@@ -780,6 +799,7 @@ template <class T> struct Epa {
                 const EpaFace<T> &bf = faces[best_face_id.id];
                 closest_points_of(bf, p1, p2);
                 n = bf.normal;
+                exit = max_dist - curr_dist < eps_tol ? EPA_CONVERGED : EPA_STUCK;
                 return true;
             }
             old_dist = curr_dist;
@@ -809,6 +829,7 @@ template <class T> struct Epa {
                         if (dist < curr_dist) {
                             closest_points_of(face, p1, p2);
                             n = face.normal;
+                            exit = EPA_NUMERICAL;
                             return true;
                         }
                         EpaFaceId<T> f;
@@ -823,8 +844,12 @@ template <class T> struct Epa {
             silhouette.clear();
             niter += 1;
             if (iters) *iters = niter;
-            if (niter > 100) break;
+            if (niter > 100) {
+                exit = EPA_ITER_CAP;
+                break;
+            }
         }
+        if (exit != EPA_ITER_CAP) exit = EPA_HEAP_EMPTY;
         const EpaFace<T> &bf = faces[best_face_id.id];
         closest_points_of(bf, p1, p2);
         n = bf.normal;
@@ -893,6 +918,7 @@ struct ContactStats {
     uint32_t gjk_iters = 0;
     uint32_t epa_iters = 0;
     bool used_epa = false;
+    uint32_t epa_exit = EPA_NOT_RUN;  // EpaExit
 };
 
 // contact_support_map_support_map_with_params (contact_support_map_support_map.rs:40-77).
@@ -931,7 +957,10 @@ GjkResult<T> contact_sm_sm_with_params(const Iso<T> &pos12, const Shape<T> &g1, 
     uint32_t ei = 0;
     if (st) st->used_epa = true;
     bool ok = epa.closest_points(pos12, g1, g2, simplex, p1, p2, n, &ei);
-    if (st) st->epa_iters = ei;
+    if (st) {
+        st->epa_iters = ei;
+        st->epa_exit = epa.exit;
+    }
     if (ok) return GjkResult<T>{GJK_CLOSEST_POINTS, p1, p2, n};
     return GjkResult<T>{GJK_NO_INTERSECTION, {}, {}, v3(T(1), T(0), T(0))};
 }
@@ -1187,7 +1216,10 @@ template <class T> bool support_map_point(const Shape<T> &s, V3<T> pt, PointProj
     uint32_t ei = 0;
     if (st) st->used_epa = true;
     bool ok = epa.closest_points(pos12, s, origin, simplex, p1, p2, n, &ei);
-    if (st) st->epa_iters = ei;
+    if (st) {
+        st->epa_iters = ei;
+        st->epa_exit = epa.exit;
+    }
     out = PointProj<T>{true, ok ? p1 : pt};
     return true;
 }
