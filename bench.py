@@ -402,7 +402,7 @@ def cast_leg(args, log):
     stream = torch.cuda.current_stream()
 
     def step():
-        cast_shapes_pairs(shapes, poses, vels, pairs, out=out, workspace=ws, stream=stream)
+        cast_shapes_pairs(shapes, poses, vels, pairs, out=out, workspace=ws, stream=stream, check_indices=False)
 
     for _ in range(args.warmup):
         step()
@@ -525,7 +525,7 @@ def manifold_leg(args, log):
     stream = torch.cuda.current_stream()
 
     def step():
-        contact_manifolds_pfm(shapes, poses, pairs, 0.02, out=out, workspace=ws, stream=stream)
+        contact_manifolds_pfm(shapes, poses, pairs, 0.02, out=out, workspace=ws, stream=stream, check_indices=False)
 
     for _ in range(args.warmup):
         step()
@@ -569,13 +569,15 @@ def manifold_leg(args, log):
     poses_b = poses + torch.from_numpy(dp).cuda()
     mf = out.clone()
     for _ in range(args.warmup):
-        contact_manifolds_pfm_update(shapes, poses_b, pairs, 0.02, mf, workspace=ws, stream=stream)
-        contact_manifolds_pfm_update(shapes, poses, pairs, 0.02, mf, workspace=ws, stream=stream)
+        contact_manifolds_pfm_update(shapes, poses_b, pairs, 0.02, mf, workspace=ws, stream=stream,
+                                     check_indices=False)
+        contact_manifolds_pfm_update(shapes, poses, pairs, 0.02, mf, workspace=ws, stream=stream,
+                                     check_indices=False)
     torch.cuda.synchronize()
     ev[0].record(stream)
     for t in range(args.steps):
         contact_manifolds_pfm_update(shapes, poses_b if t % 2 == 0 else poses, pairs, 0.02, mf, workspace=ws,
-                                     stream=stream)
+                                     stream=stream, check_indices=False)
     ev[1].record(stream)
     torch.cuda.synchronize()
     upd = mf.cpu().numpy().view(CONTACT_MANIFOLD_DTYPE).reshape(-1)
@@ -637,7 +639,7 @@ def learned_leg(args, log):
             ev[0].record(stream)
             s = seed_fn()
             ev[1].record(stream)
-            contact_pairs(shapes, poses, pairs, 0.01, init_dirs=s, out=out, workspace=ws)
+            contact_pairs(shapes, poses, pairs, 0.01, init_dirs=s, out=out, workspace=ws, check_indices=False)
             ev[2].record(stream)
             torch.cuda.synchronize()
             t_seed += ev[0].elapsed_time(ev[1])

@@ -247,7 +247,8 @@ int pq_graph_destroy(pq_graph *graph);
 #define PQ_SHAPE_CUBOID 1u            /* params[0..3] = half extents (src/shape/cuboid.rs:69-79) */
 #define PQ_SHAPE_CAPSULE 2u           /* params[0] = half height along y, params[1] = radius (capsule.rs:76-82) */
 #define PQ_SHAPE_CONE 3u              /* params[0] = half height along y, params[1] = radius (cone.rs:80-92) */
-#define PQ_SHAPE_CONVEX_POLYHEDRON 4u /* vertex range [first_vertex, +num_vertices) (convex_polyhedron.rs:172-185) */
+#define PQ_SHAPE_CONVEX_POLYHEDRON 4u /* vertex range [first_vertex, +num_vertices) (convex_polyhedron.rs:172-185);
+                                         params must be 0 (pq_shapes_create rejects others) */
 #define PQ_SHAPE_TRIANGLE 5u          /* vertices a, b, c = [first_vertex, +3) (src/shape/triangle.rs) */
 
 typedef struct pq_shape_desc {

@@ -47,6 +47,8 @@ _SIG = {
                                             _int]),
     "orc_manifolds_pfm_faces_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp, _int,
                                            _vp, _vp, _vp, _u32, _vp, _vp, _vp]),
+    "orc_manifolds_pfm_faces_update_f32": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp, _int,
+                                           _vp, _vp, _vp, _u32, _vp, _vp, _vp]),
     "orc_bvh_create_ploc_f32": (_vp, [_vp, _u32]),
     "orc_ploc_keys_f32": (None, [_vp, _u32, _vp]),
     "orc_morton_encode_unorm": (_u64, [_f64, _f64, _f64]),
@@ -471,11 +473,12 @@ def manifolds_pfm(descs, vertices, poses, pairs, prediction: float, f64: bool = 
     fids = np.zeros((m, MANIFOLD_POINTS, 2), np.uint32)
     th = default_threads() if threads is None else threads
     if previous is not None:
-        assert not f64 and faces is None
+        assert not f64
         normals[:] = np.concatenate([previous["n1"], previous["n2"]], 1)
         counts[:] = previous["count"]
         pts[:] = np.concatenate([previous["p1"], previous["p2"], previous["dist"][:, :, None]], 2)
         fids[:] = np.stack([previous["fid1"], previous["fid2"]], 2)
+    if previous is not None and faces is None:
         lib().orc_manifolds_pfm_update_f32(
             d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data, pr.ctypes.data,
             m, prediction, status.ctypes.data, normals.ctypes.data, counts.ctypes.data, pts.ctypes.data,
@@ -489,7 +492,8 @@ def manifolds_pfm(descs, vertices, poses, pairs, prediction: float, f64: bool = 
         fv = np.ascontiguousarray(faces["face_vertices"], np.uint32)
         fe = np.ascontiguousarray(faces["face_edges"], np.uint32)
         sf = np.ascontiguousarray(faces["shape_faces"], np.uint32)
-        lib().orc_manifolds_pfm_faces_f32(
+        fn_faces = lib().orc_manifolds_pfm_faces_update_f32 if previous is not None else lib().orc_manifolds_pfm_faces_f32
+        fn_faces(
             d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0], p.ctypes.data, pr.ctypes.data,
             m, prediction, status.ctypes.data, normals.ctypes.data, counts.ctypes.data, pts.ctypes.data,
             fids.ctypes.data, th, nrm.ctypes.data, ff.ctypes.data, fn.ctypes.data, len(fc), fv.ctypes.data,

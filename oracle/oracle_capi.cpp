@@ -716,6 +716,18 @@ void orc_manifolds_pfm_update_f32(const void *descs, uint32_t nshapes, const flo
                          normals, counts, points, fids, nthreads, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
                          nullptr, true);
 }
+// The update with ConvexPolyhedron faces (the arguments of both variants above).
+void orc_manifolds_pfm_faces_update_f32(const void *descs, uint32_t nshapes, const float *verts, uint32_t nverts,
+                                        const float *poses, const uint32_t *pairs, uint64_t m, float prediction,
+                                        int32_t *status, float *normals, uint32_t *counts, float *points,
+                                        uint32_t *fids, int nthreads, const float *face_normals,
+                                        const uint32_t *face_first, const uint32_t *face_count, uint32_t nfaces,
+                                        const uint32_t *face_verts, const uint32_t *face_edges,
+                                        const uint32_t *shape_faces) {
+    manifolds_pfm<float>((const ShapeDesc *)descs, nshapes, verts, nverts, poses, pairs, m, prediction, status,
+                         normals, counts, points, fids, nthreads, face_normals, face_first, face_count, nfaces,
+                         face_verts, face_edges, shape_faces, true);
+}
 void orc_manifolds_pfm_f64(const void *descs, uint32_t nshapes, const double *verts, uint32_t nverts,
                            const double *poses, const uint32_t *pairs, uint64_t m, double prediction, int32_t *status,
                            double *normals, uint32_t *counts, double *points, uint32_t *fids, int nthreads) {

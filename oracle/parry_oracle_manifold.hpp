@@ -567,8 +567,7 @@ int manifold_pfm_pfm(const Iso<T> &pos12, const Shape<T> &s1, const Shape<T> &s2
     V3<T> init{};
     const bool seeded = sq > e * e;
     if (seeded) init = m.n1 / std::sqrt(sq);
-    m.points.clear();
-    m.n1 = m.n2 = v3(T(0), T(0), T(0));
+    m.points.clear();  // ContactManifold::clear (contact_manifold.rs:793-795) keeps both normals
     const T total_prediction = prediction + br1 + br2;
     Simplex<T> simplex;
     GjkResult<T> r = contact_sm_sm_with_params(pos12, pfm1, pfm2, total_prediction, simplex,
@@ -592,6 +591,8 @@ int manifold_pfm_pfm(const Iso<T> &pos12, const Shape<T> &s1, const Shape<T> &s2
         m.n2 = n2;
         return 1;
     }
+    // NoIntersection caches its direction in local_n1, anything else resets it;
+    // local_n2 keeps the previous frame's value (contact_manifolds_pfm_pfm.rs:150-157)
     m.n1 = r.kind == GJK_NO_INTERSECTION ? r.n : v3(T(0), T(0), T(0));
     return 0;
 }

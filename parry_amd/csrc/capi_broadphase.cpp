@@ -43,6 +43,11 @@ int pq_shapes_create(const pq_shape_desc *shapes, uint32_t n, const float *verti
                 if (d.num_vertices == 0 || (uint64_t)d.first_vertex + d.num_vertices > num_vertices)
                     return fail(PQ_ERR_INVALID, "pq_shapes_create: shape " + std::to_string(i) +
                                                     " has an empty or out-of-range vertex range");
+                // params[0..1] of a polyhedron's record hold its face range, written only by
+                // pq_shapes_set_polyhedron_faces together with the face tables.
+                if (d.params[0] != 0.0f || d.params[1] != 0.0f || d.params[2] != 0.0f || d.params[3] != 0.0f)
+                    return fail(PQ_ERR_INVALID, "pq_shapes_create: ConvexPolyhedron " + std::to_string(i) +
+                                                    " must have zero params");
                 break;
             case PQ_SHAPE_TRIANGLE:
                 if (d.num_vertices != 3 || (uint64_t)d.first_vertex + 3 > num_vertices)

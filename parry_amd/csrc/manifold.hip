@@ -429,13 +429,15 @@ __device__ void contacts_face_face(const iso &pos12, const Feature &f1, f3 sep, 
 
 // The manifold of pair k of `pairs` (its GJK/EPA contact in contacts[k]) goes to
 // out[list[k]], or out[k] without a list; m_dev, if given, is the device-side
-// number of pairs.
+// number of pairs.  keep_n2: out holds last frame's manifolds, whose local_n2 a
+// pair without ClosestPoints keeps (ContactManifold::clear drops only the
+// points, contact_manifolds_pfm_pfm.rs:150-157); a fresh manifold has it zero.
 __global__ __launch_bounds__(kBlock) void manifold_kernel(const DeviceShapes s, const float *__restrict__ poses,
                                                           const uint32_t *__restrict__ pairs, uint64_t m,
                                                           const unsigned long long *__restrict__ m_dev,
                                                           const uint32_t *__restrict__ list,
                                                           const pq_contact *__restrict__ contacts,
-                                                          pq_contact_manifold *__restrict__ out) {
+                                                          pq_contact_manifold *__restrict__ out, int keep_n2) {
     const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= (m_dev ? (uint64_t)*m_dev : m)) return;
     pq_contact_manifold *const mf = out + (list ? list[k] : k);
@@ -479,7 +481,9 @@ __global__ __launch_bounds__(kBlock) void manifold_kernel(const DeviceShapes s, 
         mf->status = c.has_contact;
     }
     mf->local_n1[0] = n1.x; mf->local_n1[1] = n1.y; mf->local_n1[2] = n1.z;
-    mf->local_n2[0] = n2.x; mf->local_n2[1] = n2.y; mf->local_n2[2] = n2.z;
+    if (c.has_contact == 1u || !keep_n2) {
+        mf->local_n2[0] = n2.x; mf->local_n2[1] = n2.y; mf->local_n2[2] = n2.z;
+    }
     mf->num_points = o.n;
 }
 
@@ -609,7 +613,7 @@ hipError_t launch_manifolds(const DeviceShapes &s, const float *poses, const uin
                                    stream, kContactFeatureMaps);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(manifold_kernel, dim3((uint32_t)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, s, poses,
-                       pairs, m, nullptr, nullptr, w.contacts, out);
+                       pairs, m, nullptr, nullptr, w.contacts, out, 0);
     return hipGetLastError();
 }
 
@@ -628,7 +632,7 @@ hipError_t launch_manifolds_update(const DeviceShapes &s, const float *poses, co
                         nullptr, w.contact_ws, stream, kContactFeatureMaps);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(manifold_kernel, grid, block, 0, stream, s, poses, w.sub_pairs, m, w.pending, w.list,
-                       w.contacts, manifolds);
+                       w.contacts, manifolds, 1);
     return hipGetLastError();
 }
 

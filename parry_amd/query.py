@@ -71,8 +71,33 @@ class Contact:
     dist: float
 
 
+def _check_pairs(fn: str, pairs, num_shapes: int, check_indices: bool) -> None:
+    """(m, 2) int32/uint32 pairs; with check_indices, every index below num_shapes
+    (one device reduction and a synchronisation)."""
+    import torch
+
+    if pairs.dtype not in (torch.int32, getattr(torch, "uint32", torch.int32)) or pairs.dim() != 2 \
+            or pairs.shape[1] != 2:
+        raise ValueError(f"{fn}: pairs must be an (m, 2) int32/uint32 tensor, got {tuple(pairs.shape)} {pairs.dtype}")
+    if check_indices and pairs.numel():
+        v = pairs.view(torch.int32)
+        lo, hi = int(v.min()), int(v.max())
+        if lo < 0 or hi >= num_shapes:
+            raise ValueError(f"{fn}: pair indices must lie in [0, {num_shapes}), got [{lo}, {hi}]")
+
+
+def _check_out(fn: str, t, shape: tuple, dtype) -> None:
+    if not (t.is_cuda and t.is_contiguous()) or tuple(t.shape) != shape or t.dtype != dtype:
+        raise ValueError(f"{fn}: out must be a contiguous CUDA {dtype} tensor of shape {shape}")
+
+
+def _check_workspace(fn: str, ws, need: int) -> None:
+    if ws is not None and (not ws.is_cuda or not ws.is_contiguous() or ws.numel() * ws.element_size() < need):
+        raise ValueError(f"{fn}: workspace must be a contiguous CUDA tensor of at least {need} bytes")
+
+
 def contact_pairs(shapes, poses, pairs, prediction: float, init_dirs=None, with_iters: bool = False, out=None,
-                  workspace=None, stream=None):
+                  workspace=None, stream=None, check_indices: bool = True):
     """Batched ``query::contact`` (contact_shape_shape.rs:123-138) on CUDA tensors.
 
     ``shapes``: a ``ShapeSet``; ``poses``: (n, 7) f32 (quaternion i j k w,
@@ -96,6 +121,11 @@ def contact_pairs(shapes, poses, pairs, prediction: float, init_dirs=None, with_
     for t in (poses, pairs) + ((init_dirs,) if init_dirs is not None else ()):
         if not (t.is_cuda and t.is_contiguous()):
             raise ValueError("contact_pairs: tensors must be contiguous CUDA tensors")
+    _check_pairs("contact_pairs", pairs, len(shapes), check_indices)
+    _check_out("contact_pairs", out, (m, 14), torch.float32)
+    _check_workspace("contact_pairs", workspace, contact_workspace_bytes(m))
+    if init_dirs is not None and (tuple(init_dirs.shape) != (m, 3) or init_dirs.dtype != torch.float32):
+        raise ValueError("contact_pairs: init_dirs must be float32 (m, 3)")
     _native.check(
         _native.lib().pq_contact_pairs(
             shapes.handle, poses.data_ptr(), pairs.data_ptr(), m, float(prediction),
@@ -198,7 +228,7 @@ def cast_shapes_workspace_bytes(num_pairs: int) -> int:
 
 def cast_shapes_pairs(shapes, poses, vels, pairs, max_time_of_impact: float = F32_MAX, target_distance: float = 0.0,
                       stop_at_penetration: bool = True, compute_impact_geometry_on_penetration: bool = True,
-                      with_iters: bool = False, out=None, workspace=None, stream=None):
+                      with_iters: bool = False, out=None, workspace=None, stream=None, check_indices: bool = True):
     """Batched ``query::cast_shapes`` (shape_cast.rs:274-286) on CUDA tensors, with
     ``ShapeCastOptions``' fields as keyword arguments (parry's defaults).
 
@@ -223,6 +253,8 @@ def cast_shapes_pairs(shapes, poses, vels, pairs, max_time_of_impact: float = F3
         raise ValueError("cast_shapes_pairs: poses and vels must be float32")
     if tuple(poses.shape) != (len(shapes), 7) or tuple(vels.shape) != (len(shapes), 3):
         raise ValueError(f"cast_shapes_pairs: poses must be ({len(shapes)}, 7) and vels ({len(shapes)}, 3)")
+    _check_pairs("cast_shapes_pairs", pairs, len(shapes), check_indices)
+    _check_workspace("cast_shapes_pairs", workspace, cast_shapes_workspace_bytes(m))
     if out is None:
         out = torch.empty((m, 14), dtype=torch.float32, device=dev)
     iters = torch.empty(m, dtype=torch.int32, device=dev) if with_iters else None
@@ -251,7 +283,8 @@ def contact_manifolds_workspace_bytes(num_pairs: int) -> int:
     return int(_native.lib().pq_contact_manifolds_workspace_bytes(int(num_pairs)))
 
 
-def contact_manifolds_pfm(shapes, poses, pairs, prediction: float, out=None, workspace=None, stream=None):
+def contact_manifolds_pfm(shapes, poses, pairs, prediction: float, out=None, workspace=None, stream=None,
+                          check_indices: bool = True):
     """Batched ``contact_manifold_pfm_pfm`` (contact_manifolds_pfm_pfm.rs:44-162) on fresh
     manifolds, CUDA tensors in and out.  Returns a uint8 CUDA tensor of (m, 932)
     ``pq_contact_manifold`` records; view them with ``CONTACT_MANIFOLD_DTYPE`` after a
@@ -268,8 +301,11 @@ def contact_manifolds_pfm(shapes, poses, pairs, prediction: float, out=None, wor
             raise ValueError("contact_manifolds_pfm: tensors must be contiguous CUDA tensors")
     if tuple(poses.shape) != (len(shapes), 7) or poses.dtype != torch.float32:
         raise ValueError(f"contact_manifolds_pfm: poses must be float32 ({len(shapes)}, 7)")
+    _check_pairs("contact_manifolds_pfm", pairs, len(shapes), check_indices)
+    _check_workspace("contact_manifolds_pfm", workspace, contact_manifolds_workspace_bytes(m))
     if out is None:
         out = torch.empty((m, CONTACT_MANIFOLD_DTYPE.itemsize), dtype=torch.uint8, device=poses.device)
+    _check_out("contact_manifolds_pfm", out, (m, CONTACT_MANIFOLD_DTYPE.itemsize), torch.uint8)
     s = stream if stream is not None else torch.cuda.current_stream(poses.device)
     _native.check(_native.lib().pq_contact_manifolds_pfm_pairs(
         shapes.handle, poses.data_ptr(), pairs.data_ptr(), m, float(prediction), out.data_ptr(),
@@ -280,7 +316,8 @@ def contact_manifolds_pfm(shapes, poses, pairs, prediction: float, out=None, wor
 MANIFOLD_UPDATED = 2  # PQ_MANIFOLD_UPDATED: try_update_contacts_eps kept the manifold
 
 
-def contact_manifolds_pfm_update(shapes, poses, pairs, prediction: float, manifolds, workspace=None, stream=None):
+def contact_manifolds_pfm_update(shapes, poses, pairs, prediction: float, manifolds, workspace=None, stream=None,
+                                 check_indices: bool = True):
     """``contact_manifold_pfm_pfm`` (contact_manifolds_pfm_pfm.rs:44-162) on last frame's
     manifolds, in place: ``manifolds`` is the (m, 932) uint8 CUDA tensor of
     ``pq_contact_manifold`` records a previous call left (zeros for new pairs).  Kept
@@ -301,6 +338,8 @@ def contact_manifolds_pfm_update(shapes, poses, pairs, prediction: float, manifo
         raise ValueError(f"contact_manifolds_pfm_update: poses must be float32 ({len(shapes)}, 7)")
     if tuple(manifolds.shape) != (m, CONTACT_MANIFOLD_DTYPE.itemsize) or manifolds.dtype != torch.uint8:
         raise ValueError(f"contact_manifolds_pfm_update: manifolds must be uint8 ({m}, {CONTACT_MANIFOLD_DTYPE.itemsize})")
+    _check_pairs("contact_manifolds_pfm_update", pairs, len(shapes), check_indices)
+    _check_workspace("contact_manifolds_pfm_update", workspace, contact_manifolds_workspace_bytes(m))
     s = stream if stream is not None else torch.cuda.current_stream(poses.device)
     _native.check(_native.lib().pq_contact_manifolds_pfm_update(
         shapes.handle, poses.data_ptr(), pairs.data_ptr(), m, float(prediction), manifolds.data_ptr(),

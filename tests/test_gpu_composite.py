@@ -81,6 +81,30 @@ def test_icosphere_shapes_match_oracle(orc):
             assert c.sum() > 2000 and (~c).sum() > 2000
 
 
+def test_non_default_epa_tunings(orc):
+    """The mesh contact's EPA pairs (a triangle against a shape) under first-tier
+    settings other than the default: wave batches with 8 lanes per pair and the
+    64-face polytope, and the per-group refill with 32 lanes per pair."""
+    from parry_amd import _native
+
+    v, f = scenes.noisy_icosphere(4, seed=2, radius=2.0)
+    mesh, om = TriMesh(v, f), orc.OracleTriMesh(v, f)
+    descs, verts, poses = scenes.shapes_around_mesh(8000, seed=14)
+    mc = MeshContact(mesh, ShapeSet(descs, verts))
+    ref = om.contact_shapes(IDENT, descs, verts, poses, 0.02)
+    saved = _native.get_contact_tuning()
+    try:
+        for g, poly, sched in ((8, 2, 1), (32, 0, 2)):
+            _native.set_contact_tuning(g, poly, sched)
+            print(f"case: mesh contact, icosphere s=4, 8000 shapes, EPA G={g} polytope={poly} schedule={sched}",
+                  flush=True)
+            c = _compare(f"G={g} polytope={poly} schedule={sched}", _run(mc, IDENT, poses, 0.02, ref["candidates"]),
+                         ref)
+            assert c.sum() > 500
+    finally:
+        _native.set_contact_tuning(*saved)
+
+
 def test_truncation(orc):
     v, f = scenes.noisy_icosphere(4, seed=2, radius=2.0)
     mesh, om = TriMesh(v, f), orc.OracleTriMesh(v, f)

@@ -225,7 +225,8 @@ def test_every_epa_tuning_matches_oracle(orc):
     try:
         settings = [(64, poly, 0) for poly in range(5)]
         for g in (32, 16, 8, 4):
-            settings += [(g, poly, 0) for poly in range(5)] + [(g, 0, 1), (g, 1, 1), (g, 1, 2)]
+            settings += [(g, poly, 0) for poly in range(5)] + [(g, poly, 1) for poly in range(5)] + [(g, 1, 2)]
+        settings += [(16, poly, 2) for poly in (0, 2, 3, 4)]
         for g, poly, sched in settings:
             _native.set_contact_tuning(g, poly, sched)
             for k, (label, shapes, _, _, p, pr, pred) in enumerate(cases):

@@ -190,6 +190,23 @@ def _as_oracle(rec):
             "dist": pts["dist"].copy(), "fid1": pts["fid1"].copy(), "fid2": pts["fid2"].copy()}
 
 
+def _move(poses, rng):
+    """A third of the shapes rest, a third move by ~1e-4, a third by ~0.02 and turn a little."""
+    m = poses.shape[0]
+    kind = rng.integers(0, 3, m)
+    step = np.where(kind == 1, 1e-4, np.where(kind == 2, 0.02, 0.0))[:, None]
+    poses[:, 4:] += (rng.normal(0, 1, (m, 3)) * step).astype(np.float32)
+    turn = kind == 2
+    dq = np.concatenate([rng.normal(0, 0.01, (int(turn.sum()), 3)), np.ones((int(turn.sum()), 1))], 1)
+    q = poses[turn, :4].astype(np.float64)
+    # q <- dq * q (x, y, z, w), renormalized
+    x1, y1, z1, w1 = dq.T
+    x2, y2, z2, w2 = q.T
+    nq = np.stack([w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2, w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+                   w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2, w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2], 1)
+    poses[turn, :4] = (nq / np.linalg.norm(nq, axis=1, keepdims=True)).astype(np.float32)
+
+
 def test_update_from_previous_frames_matches_oracle(orc):
     import torch
 
@@ -205,19 +222,7 @@ def test_update_from_previous_frames_matches_oracle(orc):
     kept_total = 0
     for frame in range(4):
         if frame:
-            # a third of the shapes rest, a third move by ~1e-4, a third by ~0.02 and turn a little
-            kind = rng.integers(0, 3, 2 * n)
-            step = np.where(kind == 1, 1e-4, np.where(kind == 2, 0.02, 0.0))[:, None]
-            poses[:, 4:] += (rng.normal(0, 1, (2 * n, 3)) * step).astype(np.float32)
-            turn = kind == 2
-            dq = np.concatenate([rng.normal(0, 0.01, (int(turn.sum()), 3)), np.ones((int(turn.sum()), 1))], 1)
-            q = poses[turn, :4].astype(np.float64)
-            # q <- dq * q (x, y, z, w), renormalized
-            x1, y1, z1, w1 = dq.T
-            x2, y2, z2, w2 = q.T
-            nq = np.stack([w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2, w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
-                           w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2, w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2], 1)
-            poses[turn, :4] = (nq / np.linalg.norm(nq, axis=1, keepdims=True)).astype(np.float32)
+            _move(poses, rng)
         print(f"case: manifold update, frame {frame}, {n} pairs", flush=True)
         ref = orc.manifolds_pfm(d, verts, poses, pairs, 0.02, previous=prev if prev is not None else
                                 _as_oracle(np.zeros(n, CONTACT_MANIFOLD_DTYPE)))
@@ -230,3 +235,35 @@ def test_update_from_previous_frames_matches_oracle(orc):
             assert (st == 2).sum() == 0
         prev = _as_oracle(got)
     assert kept_total > 5000
+
+
+def test_update_polyhedra_with_faces_matches_oracle(orc):
+    """The update path with ConvexPolyhedra that have face tables (kept manifolds
+    carry polyhedron feature ids across frames; separated pairs keep last
+    frame's local_n2)."""
+    import torch
+
+    from parry_amd import contact_manifolds_pfm_update
+
+    n = 6000
+    d, verts, poses, pairs = _polyhedra_scene(n, 12, 0.8)
+    faces = scenes.polyhedron_faces(d, verts)
+    shapes = ShapeSet(d, verts)
+    shapes.set_polyhedron_faces(faces)
+    tp = torch.from_numpy(pairs.view(np.int32)).cuda()
+    mf = torch.zeros((n, CONTACT_MANIFOLD_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    rng = np.random.default_rng(8)
+    prev = _as_oracle(np.zeros(n, CONTACT_MANIFOLD_DTYPE))
+    kept_total = 0
+    for frame in range(4):
+        if frame:
+            _move(poses, rng)
+        print(f"case: manifold update with polyhedron faces, frame {frame}, {n} pairs", flush=True)
+        ref = orc.manifolds_pfm(d, verts, poses, pairs, 0.02, faces=faces, previous=prev)
+        contact_manifolds_pfm_update(shapes, torch.from_numpy(poses).cuda(), tp, 0.02, mf)
+        torch.cuda.synchronize()
+        got = mf.cpu().numpy().view(CONTACT_MANIFOLD_DTYPE).reshape(-1)
+        st = _compare(f"polyhedra frame {frame}", got, ref)
+        kept_total += int((st == 2).sum())
+        prev = _as_oracle(got)
+    assert kept_total > 1000

@@ -130,3 +130,19 @@ def test_error_reporting(native):
     assert "out of range" in _native.last_error()
     rc = native.pq_trimesh_create(v.ctypes.data, 3, i.ctypes.data, 0, None, 0, ctypes.byref(h))
     assert rc == _native.PQ_ERR_INVALID and "empty index" in _native.last_error()
+
+
+def test_polyhedron_params_rejected(native):
+    # A ConvexPolyhedron record keeps its face range in params[0..1]; only
+    # pq_shapes_create's zero params plus pq_shapes_set_polyhedron_faces may set
+    # it, so nonzero params are refused before anything reaches the device.
+    from parry_amd import SHAPE_DESC_DTYPE
+
+    d = np.zeros(1, SHAPE_DESC_DTYPE)
+    d["type"], d["first_vertex"], d["num_vertices"] = 4, 0, 4
+    d["params"] = (0, 3, 0, 0)
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    h = ctypes.c_void_p()
+    rc = native.pq_shapes_create(d.ctypes.data, 1, v.ctypes.data, 4, ctypes.byref(h))
+    assert rc == _native.PQ_ERR_INVALID and "zero params" in _native.last_error()
+    assert not h.value
