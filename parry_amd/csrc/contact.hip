@@ -1268,9 +1268,12 @@ __global__ __launch_bounds__(64, (epa_min_waves<G, W>())) void epa_tier_kernel(c
 // with theirs, so a wave no longer waits for the longest of its pairs.  Each
 // trip through the loop runs, per group, either the set-up of a new pair or one
 // EPA iteration (the two are executed one after the other by the wave when its
-// groups differ).  The loop's exit test is wave-uniform, at a point where the
-// whole wave is converged; everything inside is group-uniform, and the
-// cross-lane operations of the EPA code stay within a group.
+// groups differ).  The fetch and the loop's exit test are at the top of the
+// loop, where the whole wave is converged: a ballot of the idle groups' first
+// lanes, one atomicAdd by lane 0 for all of them, the base read with
+// readfirstlane, and each idle group's entry at its rank among them (no
+// cross-lane exchange under divergence).  Everything after is group-uniform,
+// and the cross-lane operations of the EPA code stay within a group.
 template <int G, class W>
 __global__ __launch_bounds__(64, (epa_min_waves<G, W>())) void epa_refill_kernel(const ShapeRecord *__restrict__ rec,
                                                       const float4 *__restrict__ verts, const float *__restrict__ poses,
@@ -1297,13 +1300,16 @@ __global__ __launch_bounds__(64, (epa_min_waves<G, W>())) void epa_refill_kernel
     EpaRegs<W> r{Heap<W>{wg, 0}, 0, 0, 0, 0, kFltMax, 0.0f};
     f3 p1{0.0f, 0.0f, 0.0f}, p2{0.0f, 0.0f, 0.0f}, n{0.0f, 0.0f, 0.0f};
     for (;;) {
-        if (!busy && !drained) {
-            uint32_t q = 0;
-            if (sub == 0) q = atomicAdd(next, 1u);
-            q = __shfl(q, 0, G);
-            if (q >= count) {
+        const bool idle = !busy && !drained;  // group-uniform
+        const uint64_t want = __ballot(idle && sub == 0);
+        if (want != 0ull) {  // wave-uniform
+            uint32_t base = 0;
+            if (__lane_id() == 0) base = atomicAdd(next, (uint32_t)__popcll(want));
+            base = __builtin_amdgcn_readfirstlane(base);
+            const uint32_t q = base + (uint32_t)__popcll(want & ((1ull << (group * G)) - 1ull));
+            if (idle && q >= count) {
                 drained = true;
-            } else {
+            } else if (idle) {
                 slot = in_list ? in_list[q] : q;
                 const uint32_t ci = in_cont ? in_cont[q] : kNoCont;
                 pr = epa_problem(rec, verts, poses, pairs, queue + slot);
