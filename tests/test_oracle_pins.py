@@ -420,3 +420,41 @@ def test_cone_cuboid_and_polyhedron_distance(orc):
         assert abs(r32["out"][k, 12] - dref) <= 1e-2 * max(1.0, dref)
         checked += 1
     assert checked >= 60
+
+
+# ---- hull vertex order (ConvexPolyhedron::from_convex_hull) --------------------------
+
+
+def test_remove_unused_points_order_known_answers():
+    """remove_unused_points (cleanup.rs:6-34) traced by hand: an unused point is
+    replaced by the last point (Vec::swap_remove) and its slot examined again."""
+    from parry_amd.scenes import remove_unused_points_order as order
+
+    # [p0..p5], used {2, 3, 5}: swap_remove(0) -> [p5 p1 p2 p3 p4]; p5 used, i=1;
+    # swap_remove(1) -> [p5 p4 p2 p3]; p4 unused: swap_remove(1) -> [p5 p3 p2]
+    assert order(6, [0, 0, 1, 1, 0, 1]) == [5, 3, 2]
+    assert order(5, [1, 0, 1, 0, 1]) == [0, 4, 2]
+    assert order(4, [1, 1, 1, 1]) == [0, 1, 2, 3]
+    assert order(4, [0, 1, 1, 1]) == [3, 1, 2]
+    assert order(3, [1, 1, 0]) == [0, 1]  # the last point just goes
+    assert order(3, [0, 0, 0]) == []
+
+
+def test_hull_vertices_keep_triangles_pointing_at_the_same_points():
+    """With the remap remove_unused_points builds, every hull triangle still names
+    the same coordinates; every hull vertex is kept exactly once."""
+    from scipy.spatial import ConvexHull
+
+    from parry_amd.scenes import convex_hull_vertices, convex_point_cloud
+
+    rng = np.random.default_rng(61)
+    for _ in range(50):
+        pts = convex_point_cloud(rng, int(rng.integers(8, 65)))
+        h = ConvexHull(pts)
+        v = convex_hull_vertices(pts)
+        assert v.shape[0] == len(h.vertices)
+        pos = {tuple(p): k for k, p in enumerate(v)}
+        assert len(pos) == v.shape[0]
+        for tri in h.simplices:
+            for i in tri:
+                assert np.array_equal(v[pos[tuple(pts[i])]], pts[i])
