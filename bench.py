@@ -53,6 +53,97 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 vector peak
+
+
+def f64_error(val32, val64, ok=None) -> dict:
+    """Error of f32 results (rows of values) against the f64 oracle on the same
+    inputs: max and 99.9th-percentile absolute error, 99th-percentile relative
+    error (|d| / max(1, |f64|), worst component per row) and the share of rows
+    within 1e-5 relative."""
+    import numpy as np
+
+    a = np.asarray(val32, np.float64)
+    b = np.asarray(val64, np.float64)
+    if ok is not None:
+        a, b = a[ok], b[ok]
+    if a.shape[0] == 0:
+        return {"rows": 0}
+    d = np.abs(a - b).reshape(a.shape[0], -1)
+    rel = (d / np.maximum(1.0, np.abs(b).reshape(a.shape[0], -1))).max(1)
+    return {"rows": int(a.shape[0]), "abs_err_max": float(d.max()), "abs_err_p999": float(np.quantile(d.max(1), 0.999)),
+            "rel_err_p99": float(np.quantile(rel, 0.99)), "within_1e-5_rel": round(float((rel <= 1e-5).mean()), 6)}
+
+
+def contact_f64_error(gpu_out, descs, vertices, poses, pairs, prediction, init_dirs=None, threads=None) -> dict:
+    """query::contact of the GPU (f32) against the f64 oracle on the same pairs
+    and poses (given seeds, if any): status agreement, then dist, normal1 and the
+    two points.  The f32 oracle's EPA exit (bit-identical to the GPU's path)
+    splits the dist error: a converged EPA returns the best face's plane, which
+    the two precisions can reach by different paths."""
+    import numpy as np
+
+    from oracle import oracle
+
+    r64 = oracle.contact_pairs(descs, np.asarray(vertices, np.float64), np.asarray(poses, np.float64), pairs,
+                               prediction, init_dirs=None if init_dirs is None else np.asarray(init_dirs, np.float64),
+                               f64=True, threads=threads)
+    r32 = oracle.contact_pairs(descs, vertices, poses, pairs, prediction, init_dirs=init_dirs, threads=threads)
+    has = gpu_out.view(np.uint32)[:, 13].astype(np.int64)
+    both = (has == 1) & (r64["status"] == 1)
+    rec = {"pairs": int(pairs.shape[0]), "status_agreement": round(float((has == r64["status"]).mean()), 7),
+           "dist": f64_error(gpu_out[:, 12], r64["out"][:, 12], both),
+           "normal1": f64_error(gpu_out[:, 6:9], r64["out"][:, 6:9], both),
+           "points": f64_error(gpu_out[:, 0:6], r64["out"][:, 0:6], both), "dist_by_epa_exit": {}}
+    for k, name in enumerate(oracle.EPA_EXIT_NAMES):
+        sel = both & (r32["epa_exit"] == k)
+        if sel.any():
+            rec["dist_by_epa_exit"][name] = f64_error(gpu_out[:, 12], r64["out"][:, 12], sel)
+    return rec
+
+
+def ray_f64_error(v, f, rays, hits, threads=None) -> dict:
+    """Ray casts of the GPU (f32, (n, 5) pq_ray_hit rows) against the f64 oracle
+    on the same mesh and rays: hit/feature agreement, then toi and normal error
+    where both hit the same triangle."""
+    import numpy as np
+
+    from oracle import oracle
+
+    ref = oracle.OracleTriMesh(np.asarray(v, np.float64), f, f64=True).cast_rays(
+        np.asarray(rays, np.float64), MAX_TOI, threads=threads)
+    feat = hits[:, 4].view(np.uint32)
+    same = feat == ref["feature"]
+    hit = same & (feat != np.uint32(0xFFFFFFFF))
+    return {"rays": int(rays.shape[0]), "feature_agreement": round(float(same.mean()), 7),
+            "toi": f64_error(hits[:, 0], ref["toi"], hit), "normal": f64_error(hits[:, 1:4], ref["normal"], hit)}
+
+
+def contact_roofline(nbytes: int, flops: int, ms: float, kernel: str) -> dict:
+    """HBM fraction on algorithmic bytes and FP32 fraction on the oracle's
+    counted flops (add/sub, mul, div, sqrt) of the same pairs."""
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    tf = flops / (ms * 1e-3) / 1e12
+    return {"bound": "latency (GJK/EPA branchy scalar code)", "kernel": kernel, "achieved": round(gbs, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
+            "algorithmic_bytes_per_launch": int(nbytes), "kernel_ms": round(ms, 4),
+            "fp32": {"achieved_tflops": round(tf, 4), "peak_tflops": FP32_PEAK_TFLOPS,
+                     "frac": round(tf / FP32_PEAK_TFLOPS, 6), "algorithmic_flops_per_launch": int(flops),
+                     "source": "add/sub, mul, div and sqrt counted by the oracle's query::contact on the same "
+                               "pairs and seeds"}}
+
+
+def contact_bytes(descs, pairs) -> int:
+    """Algorithmic bytes of query::contact per call: 12 B per polyhedron or
+    triangle vertex of both shapes, two 28-B poses and the 56-B result per pair."""
+    import numpy as np
+
+    t = np.asarray(descs["type"])
+    nv = np.asarray(descs["num_vertices"], np.int64) * ((t == 4) | (t == 5))
+    pr = np.asarray(pairs).astype(np.int64)
+    return int((12 * (nv[pr[:, 0]] + nv[pr[:, 1]]) + 2 * 28 + 56).sum())
+
+
 def cpu_baseline(v, f, rays, gpu_feat):
     """Times the oracle on the host cores; returns (record, parity summary)."""
     import numpy as np
@@ -197,6 +288,16 @@ def batch_leg(args, log):
                                "sample": f"first min(16384, n_i) rays of each of the 64 meshes ({got} rays), "
                                          f"{cores} threads"}
         rec["parity"]["feature_mismatches_vs_oracle_sample"] = mism
+        # f64 error on the same sample (the first 4096 rays of every mesh)
+        hits = out.cpu().numpy()
+        errs = [ray_f64_error(v, f, r[:4096], hits[offs[i]:offs[i] + min(4096, r.shape[0])], threads=cores)
+                for i, ((v, f), r) in enumerate(zip(hosts, rays))]
+        nr = sum(e["rays"] for e in errs)
+        rec["f64_error"] = {
+            "rays": nr, "feature_agreement": round(sum(e["feature_agreement"] * e["rays"] for e in errs) / nr, 7),
+            "toi_abs_err_max": max(e["toi"].get("abs_err_max", 0.0) for e in errs),
+            "normal_abs_err_max": max(e["normal"].get("abs_err_max", 0.0) for e in errs),
+            "sample": "the first min(4096, n_i) rays of each of the 64 meshes"}
     return rec
 
 
@@ -290,6 +391,18 @@ def queries_leg(args, log):
                                  "sample": f"all {n} rays, median of 3 runs on {cores} threads"}
         rec_c["parity"] = {"feature_mismatches_vs_oracle": int(np.count_nonzero(
             hits.view(np.uint32)[:, 4] != ref["feature"]))}
+        o64 = oracle.OracleTriMesh(np.asarray(v, np.float64), f, f64=True)
+        r64 = o64.project_points(pts.astype(np.float64), False, threads=cores)
+        same = proj.view(np.uint32)[:, 4] == r64["triangle"]
+        rec_p["f64_error"] = {"points": n, "triangle_agreement": round(float(same.mean()), 7),
+                              "is_inside_agreement": round(float((proj.view(np.uint32)[:, 3] == r64["is_inside"]).mean()), 7),
+                              "projection": f64_error(proj[:, :3], r64["point"], same)}
+        r64 = o64.cast_rays_with_culling(rays.astype(np.float64), MAX_TOI, 1, threads=cores)
+        feat = hits.view(np.uint32)[:, 4]
+        same = (feat == r64["feature"]) & (feat != np.uint32(0xFFFFFFFF))
+        rec_c["f64_error"] = {"rays": n, "feature_agreement": round(float((feat == r64["feature"]).mean()), 7),
+                              "toi": f64_error(hits[:, 0], r64["toi"], same),
+                              "normal": f64_error(hits[:, 1:4], r64["normal"], same)}
     return {"project_points": rec_p, "culling": rec_c}
 
 
@@ -373,6 +486,14 @@ def mesh_contact_leg(args, log):
             "bit_identical_contacts": int(np.all(o[c, :13] == ref["out"][c], axis=1).sum()),
             "candidates_oracle": int(ref["candidates"]),
         }
+        r64 = oracle.OracleTriMesh(np.asarray(v, np.float64), f, f64=True).contact_shapes(
+            np.array([0, 0, 0, 1, 0, 0, 0], np.float64), descs, np.asarray(verts, np.float64),
+            np.asarray(poses_np, np.float64), 0.01, threads=cores)
+        both = (has == 1) & (r64["status"] == 1)
+        rec["f64_error"] = {"shapes": n, "status_agreement": round(float((has.astype(np.int64) == r64["status"]).mean()), 7),
+                            "triangle_agreement": round(float((got_tri[both] == r64["triangle"][both]).mean()), 7),
+                            "dist": f64_error(o[:, 12], r64["out"][:, 12], both),
+                            "normal1": f64_error(o[:, 6:9], r64["out"][:, 6:9], both)}
     return rec
 
 
@@ -448,6 +569,12 @@ def cast_leg(args, log):
         rec["parity"] = {"status_mismatches": int((st != ref["status"]).sum()),
                          "bit_identical_hits": int(np.all(o[h, :13] == ref["out"][h], axis=1).sum()),
                          "hits_oracle": int(h.sum())}
+        r64 = oracle.cast_shapes(descs, np.asarray(verts, np.float64), np.asarray(poses_np, np.float64),
+                                 np.asarray(vels_np, np.float64), pairs_np, f64=True, threads=cores)
+        both = (st != 0) & (r64["status"] != 0)
+        rec["f64_error"] = {"pairs": n, "status_agreement": round(float((st == r64["status"]).mean()), 7),
+                            "toi": f64_error(o[:, 0], r64["out"][:, 0], both),
+                            "normal1": f64_error(o[:, 7:10], r64["out"][:, 7:10], both)}
     return rec
 
 
@@ -598,28 +725,45 @@ def manifold_leg(args, log):
         same = (got["status"] == np.where(ref["status"] < 0, 0xFFFFFFFF, ref["status"]).astype(np.uint32)) & \
                (got["num_points"] == ref["count"])
         rec["parity"] = {"status_or_count_mismatches": int((~same).sum()), "manifolds_oracle": int(ok.sum())}
+        r64 = oracle.manifolds_pfm(d, np.asarray(verts, np.float64), np.asarray(poses_np, np.float64), pairs_np, 0.02,
+                                   f64=True, threads=cores)
+        st64 = np.where(r64["status"] < 0, 0xFFFFFFFF, r64["status"]).astype(np.uint32)
+        agree = (got["status"] == st64) & (got["num_points"] == r64["count"])
+        both = agree & (got["status"] == 1)
+        # GJK's own point (the last of a manifold) and the normal, where both precisions agree on the point count
+        last = np.maximum(got["num_points"].astype(np.int64) - 1, 0)
+        rows = np.arange(n)
+        rec["f64_error"] = {"pairs": n, "status_and_count_agreement": round(float(agree.mean()), 7),
+                            "local_n1": f64_error(got["local_n1"], r64["n1"], both),
+                            "gjk_point_dist": f64_error(got["points"]["dist"][rows, last], r64["dist"][rows, last],
+                                                        both)}
     return rec
 
 
 def learned_leg(args, log):
     """Learned GJK warm start vs the plain GJK on 1,048,576 mixed convex pairs
-    (ConvexPolyhedron 8-64 points, Cuboid, Ball, Capsule, Cone; the overlapping
-    AABB pairs of a 262,144-shape scene, seed 3), prediction 0.01.  One step =
-    query::contact on every pair; the learned step adds the fused seed kernel.
-    A "perfect seed" run (the plain GJK's own final direction as the seed)
-    bounds what any seed can gain."""
+    (ConvexPolyhedron 8-64 points, Cuboid, Ball, Capsule, Cone): the first 2^20
+    overlapping-AABB pairs of a 320,000-shape scene (seed 3), prediction 0.01.
+    One step = query::contact on every pair; the learned step adds the fused
+    seed kernel.  A "perfect seed" run (the plain GJK's own final direction as
+    the seed) bounds what any seed can gain, and the contact path's phase times
+    bound what any GJK speed-up can gain (Amdahl: the call with GJK taking no
+    time at all)."""
     import numpy as np
     import torch
 
-    from parry_amd import Bvh, ShapeSet, contact_pairs, contact_workspace_bytes, scenes
+    from parry_amd import Bvh, ShapeSet, _native, contact_pairs, contact_workspace_bytes, scenes
     from parry_amd.learned import WEIGHTS_FILE, WarmStartModel, gjk_directions, support_map_pairs
 
     if not os.path.exists(WEIGHTS_FILE):
         return {"skipped": f"no trained parameters at {os.path.relpath(WEIGHTS_FILE, ROOT)}"}
-    sc = scenes.mixed_scene(262144, seed=3)
+    sc = scenes.mixed_scene(320000, seed=3)
     shapes = ShapeSet(sc["descs"], sc["vertices"])
     poses = torch.from_numpy(sc["poses"]).cuda()
     pairs = Bvh(shapes.compute_aabbs(poses).cpu().numpy()).self_pairs()
+    total_pairs = int(pairs.shape[0])
+    if total_pairs < 1 << 20:
+        raise RuntimeError(f"the scene yields {total_pairs} pairs, fewer than 2^20")
     pairs = pairs[: 1 << 20].contiguous()
     m = pairs.shape[0]
     net = WarmStartModel.pretrained()
@@ -646,6 +790,15 @@ def learned_leg(args, log):
             t_contact += ev[1].elapsed_time(ev[2])
         return t_seed / steps, t_contact / steps
 
+    def phases(s):
+        """The contact call's phase times (one untimed call with events between phases)."""
+        _native.set_contact_phase_timing(True)
+        try:
+            contact_pairs(shapes, poses, pairs, 0.01, init_dirs=s, out=out, workspace=ws, check_indices=False)
+            return {k: round(v, 4) for k, v in _native.contact_phase_ms().items()}
+        finally:
+            _native.set_contact_phase_timing(False)
+
     steps = max(3, args.steps // 2)
     plain_ms = run(lambda: None, steps)
     learned_ms = run(lambda: net.seeds(poses, pairs, desc, out=seeds), steps)
@@ -655,38 +808,61 @@ def learned_leg(args, log):
     keep = torch.from_numpy(support_map_pairs(sc["descs"], pairs.cpu().numpy().view(np.uint32))).cuda()
     perfect[~keep] = 0.0
     perfect_ms = run(lambda: perfect, steps)
+    ph_plain = phases(None)
+    ph_learned = phases(net.seeds(poses, pairs, desc, out=seeds))
 
     def stats(s):
         c, it = contact_pairs(shapes, poses, pairs, 0.01, init_dirs=s, with_iters=True, workspace=ws)
-        it = it.cpu().numpy()
-        return c.cpu().numpy(), it
+        return c.cpu().numpy(), it.cpu().numpy()
 
     cp, ip = stats(None)
-    cl, il = stats(net.seeds(poses, pairs, desc))
+    learned_seeds = net.seeds(poses, pairs, desc)
+    cl, il = stats(learned_seeds)
     _, iq = stats(perfect)
     hp, hl = cp.view(np.uint32)[:, 13], cl.view(np.uint32)[:, 13]
     both = (hp == 1) & (hl == 1)
     derr = np.abs(cp[both, 12] - cl[both, 12]) / np.maximum(1.0, np.abs(cp[both, 12]))
     km = keep.cpu().numpy()
+    pr = pairs.cpu().numpy().view(np.uint32)
 
     def rate(ms):
         return round(m / ((ms[0] + ms[1]) * 1e-3), 1)
 
-    return {
+    # GJK iterations per shape-type pair (unordered), seeded pairs only
+    names = {0: "Ball", 1: "Cuboid", 2: "Capsule", 3: "Cone", 4: "ConvexPolyhedron"}
+    t = sc["descs"]["type"]
+    ta, tb = np.minimum(t[pr[:, 0]], t[pr[:, 1]]), np.maximum(t[pr[:, 0]], t[pr[:, 1]])
+    by_type = {}
+    for a in range(5):
+        for b in range(a, 5):
+            sel = km & (ta == a) & (tb == b)
+            if sel.any():
+                by_type[f"{names[a]}-{names[b]}"] = {
+                    "pairs": int(sel.sum()), "plain": round(float(ip[sel, 0].mean()), 4),
+                    "learned": round(float(il[sel, 0].mean()), 4),
+                    "perfect_seed": round(float(iq[sel, 0].mean()), 4)}
+    contact_ms = plain_ms[1]
+    gjk_ms = ph_plain["gjk"]
+    rec = {
         "metric": "contact pairs/s, learned warm start vs plain GJK",
-        "workload": f"{m} overlapping-AABB pairs of 262144 mixed convex shapes (5 types uniform), seed 3, "
-                    "prediction 0.01",
+        "workload": f"the first {m} of {total_pairs} overlapping-AABB pairs of 320000 mixed convex shapes "
+                    "(5 types uniform), seed 3, prediction 0.01",
         "pairs": m,
         "seeded_pairs": int(km.sum()),
-        "plain": {"pairs_per_s": rate(plain_ms), "contact_ms": round(plain_ms[1], 4)},
+        "plain": {"pairs_per_s": rate(plain_ms), "contact_ms": round(plain_ms[1], 4), "phases_ms": ph_plain},
         "learned": {"pairs_per_s": rate(learned_ms), "seed_kernel_ms": round(learned_ms[0], 4),
-                    "contact_ms": round(learned_ms[1], 4)},
+                    "contact_ms": round(learned_ms[1], 4), "phases_ms": ph_learned},
         "perfect_seed_bound": {"pairs_per_s": round(m / (perfect_ms[1] * 1e-3), 1),
                                "contact_ms": round(perfect_ms[1], 4)},
-        "speedup_learned_vs_plain": round(rate(plain_ms) and rate(learned_ms) / rate(plain_ms), 4),
+        "speedup_learned_vs_plain": round(rate(learned_ms) / rate(plain_ms), 4),
+        "amdahl_bound_any_gjk_speedup": {
+            "gjk_share_of_contact": round(gjk_ms / sum(ph_plain.values()), 4),
+            "max_speedup": round(sum(ph_plain.values()) / (sum(ph_plain.values()) - gjk_ms), 4),
+            "note": "contact time with the GJK/dispatch kernel taking no time at all; a seed only shortens GJK"},
         "mean_gjk_iterations_seeded_pairs": {"plain": round(float(ip[km, 0].mean()), 4),
                                              "learned": round(float(il[km, 0].mean()), 4),
                                              "perfect_seed": round(float(iq[km, 0].mean()), 4)},
+        "gjk_iterations_saved_by_type_pair": by_type,
         "mean_epa_iterations_seeded_pairs": {"plain": round(float(ip[km, 1].mean()), 4),
                                              "learned": round(float(il[km, 1].mean()), 4)},
         "agreement_with_plain": {"has_contact_mismatches": int((hp != hl).sum()),
@@ -698,6 +874,29 @@ def learned_leg(args, log):
                                                   / (learned_ms[0] * 1e-3) / 1e12, 2),
                          "peak_tflops": 2500.0},
     }
+    if not args.no_cpu:
+        from oracle import oracle
+
+        nbytes = contact_bytes(sc["descs"], pr)
+        log("learned leg: oracle flop count, CPU baseline and f64 error ...")
+        fl = oracle.contact_flops(sc["descs"], sc["vertices"], sc["poses"], pr, 0.01)
+        rec["roofline"] = contact_roofline(nbytes, fl["flops"], contact_ms, "contact (plain GJK + EPA kernels)")
+        rec["roofline"]["fp32"]["flops_per_pair"] = round(fl["flops"] / m, 1)
+        cores = oracle.default_threads()
+        oracle.contact_pairs(sc["descs"], sc["vertices"], sc["poses"], pr[: 1 << 14], 0.01, threads=cores)
+        t0 = time.perf_counter()
+        ref = oracle.contact_pairs(sc["descs"], sc["vertices"], sc["poses"], pr, 0.01, threads=cores)
+        t_cpu = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(m / t_cpu, 1), "unit": "pairs/s", "cores": cores, "kind": "port",
+                               "sample": f"query::contact (parry's default seed) on all {m} pairs, {cores} threads "
+                                         "(C++ restatement of parry3d f32)"}
+        rec["parity"] = {"plain_has_contact_mismatches":
+                         int((ref["status"].astype(np.int64) != hp.astype(np.int64)).sum())}
+        rec["f64_error"] = {"plain": contact_f64_error(cp, sc["descs"], sc["vertices"], sc["poses"], pr, 0.01,
+                                                       threads=cores),
+                            "learned": contact_f64_error(cl, sc["descs"], sc["vertices"], sc["poses"], pr, 0.01,
+                                                         init_dirs=learned_seeds.cpu().numpy(), threads=cores)}
+    return rec
 
 
 def dynamic_leg(args, log):
@@ -705,7 +904,9 @@ def dynamic_leg(args, log):
     dt = 1/60), 100 frames, each one hipGraph replay: AABBs, Bvh refit, pairs,
     seeds, query::contact (prediction 0.01), warm-start cache, pose update.
     Run with parry's default seeds, with the cached previous normals, and with
-    the learned seeds plus the cache; the f64 error is sampled on one frame."""
+    the learned seeds plus the cache.  The f64 error is given on frames 0, 49
+    and 99, the roofline on one frame's algorithmic bytes and oracle-counted
+    flops, and the CPU baseline is the oracle's frame 0 on the host cores."""
     import numpy as np
     import torch
 
@@ -755,21 +956,52 @@ def dynamic_leg(args, log):
     if not args.no_cpu:
         from oracle import oracle
 
-        # error against the f64 oracle on frame 0 (the same pairs and poses)
-        ds = DynamicScene(sc["descs"], sc["vertices"], sc["poses"], seed=104, seeding="plain",
-                          pair_capacity=4 * 262144 + (1 << 20))
-        ds.step(use_graph=False)
-        torch.cuda.synchronize()
-        m = ds.num_pairs()
-        pairs = ds.pairs[:m].cpu().numpy().view(np.uint32)
-        has = ds.contacts[:m].cpu().numpy()
-        r64 = oracle.contact_pairs(sc["descs"], sc["vertices"].astype(np.float64), sc["poses"].astype(np.float64),
-                                   pairs, 0.01, f64=True)
-        h32 = has.view(np.uint32)[:, 13]
-        both = (h32 == 1) & (r64["status"] == 1)
-        d = np.abs(has[both, 12] - r64["out"][both, 12])
-        rec["f64_error_frame0"] = {"has_contact_agreement": float((h32.astype(np.int64) == r64["status"]).mean()),
-                                   "dist_abs_err_p999": float(np.quantile(d, 0.999)), "dist_abs_err_max": float(d.max())}
+        cores = oracle.default_threads()
+        descs, verts = sc["descs"], sc["vertices"]
+        # frames 0, 49 and 99 of a plain run, issued eagerly: the poses each frame
+        # queries, its pairs and its contacts
+        ds = DynamicScene(descs, verts, sc["poses"], seed=104, seeding="plain", pair_capacity=4 * 262144 + (1 << 20))
+        snaps = {}
+        for f in range(frames):
+            if f in (0, 49, 99):
+                torch.cuda.synchronize()
+                p_f = ds.poses.cpu().numpy().copy()
+            ds.step(use_graph=False)
+            if f in (0, 49, 99):
+                m_f = ds.num_pairs()
+                snaps[f] = (p_f, ds.pairs[:m_f].cpu().numpy().view(np.uint32).copy(), ds.contacts[:m_f].cpu().numpy())
+        del ds
+        log("dynamic leg: f64 error on frames 0, 49, 99 ...")
+        rec["f64_error"] = {f"frame_{f}": contact_f64_error(c, descs, verts, p_f, pr, 0.01, threads=cores)
+                            for f, (p_f, pr, c) in snaps.items()}
+        # one frame's algorithmic traffic and flops, and the oracle's frame on the host cores
+        p0, pr0, c0 = snaps[0]
+        t0 = time.perf_counter()
+        aabbs = oracle.shape_aabbs(descs, verts, p0)
+        obvh = oracle.OracleBvh(aabbs)
+        ref_pairs, touched = obvh.self_pairs()
+        ref = oracle.contact_pairs(descs, verts, p0, ref_pairs, 0.01, threads=cores)
+        t_cpu = time.perf_counter() - t0
+        n = len(descs)
+        t = descs["type"]
+        nv = descs["num_vertices"].astype(np.int64) * (t == 4)
+        frame_bytes = {"aabbs": int((28 + 32 + 12 * nv + 24).sum()), "refit": 24 * n + 128 * (n - 1),
+                       "pairs": 64 * touched + 8 * int(pr0.shape[0]), "contact": contact_bytes(descs, pr0)}
+        fl = oracle.contact_flops(descs, verts, p0, pr0, 0.01)
+        plain_ms = rec["modes"]["plain"]["ms_per_frame"]
+        rec["roofline"] = contact_roofline(sum(frame_bytes.values()), fl["flops"], plain_ms,
+                                           "whole frame (AABBs, refit, pairs, contact), plain seeds")
+        rec["roofline"]["bytes_per_frame"] = frame_bytes
+        rec["roofline"]["fp32"]["flops_per_pair"] = round(fl["flops"] / max(1, pr0.shape[0]), 1)
+        rec["cpu_baseline"] = {"value": round(ref_pairs.shape[0] / t_cpu, 1), "unit": "pairs/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"frame 0 on the host: shape AABBs and a new Bvh (1 thread, as parry), "
+                                         f"its {ref_pairs.shape[0]} pairs and query::contact on {cores} threads "
+                                         "(C++ restatement of parry3d f32)"}
+        rec["parity_frame0"] = {
+            "pair_sets_equal": bool(oracle.pair_set(ref_pairs) == oracle.pair_set(pr0)),
+            "has_contact_mismatches": int((oracle.contact_pairs(descs, verts, p0, pr0, 0.01, threads=cores)["status"]
+                                           .astype(np.int64) != c0.view(np.uint32)[:, 13].astype(np.int64)).sum())}
     return rec
 
 
@@ -827,7 +1059,7 @@ def contact_leg(args, log):
     has = pipe.contacts[:m].cpu().numpy().view(np.uint32)[:, 13]
     offs = sc["offsets"].astype(np.int64)
     nverts = offs[1:] - offs[:-1]
-    contact_bytes = int((12 * (nverts[pairs[:, 0]] + nverts[pairs[:, 1]]) + 2 * 28 + 56).sum())
+    cbytes = int((12 * (nverts[pairs[:, 0]] + nverts[pairs[:, 1]]) + 2 * 28 + 56).sum())
     contact_ms = stages["contact"]
     rec = {
         "metric": "contact pairs/s",
@@ -844,11 +1076,11 @@ def contact_leg(args, log):
         "roofline": {
             "bound": "latency (GJK/EPA branchy scalar code)",
             "kernel": "contact (GJK + EPA kernels)",
-            "achieved": round(contact_bytes / (contact_ms * 1e-3) / 1e9, 2),
+            "achieved": round(cbytes / (contact_ms * 1e-3) / 1e9, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(contact_bytes / (contact_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
-            "algorithmic_bytes_per_launch": contact_bytes,
+            "frac": round(cbytes / (contact_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "algorithmic_bytes_per_launch": cbytes,
         },
     }
     if not args.no_cpu:
@@ -866,6 +1098,11 @@ def contact_leg(args, log):
             "sample": f"query::contact on all {m} pairs of one frame, {cores} threads (C++ restatement of parry3d f32)",
         }
         rec["parity"] = {"has_contact_mismatches": int((ref["status"].astype(np.int64) != has.astype(np.int64)).sum())}
+        fl = oracle.contact_flops(shapes.descs, sc["vertices"], sc["poses"], pairs, 0.01)
+        rec["roofline"] = contact_roofline(cbytes, fl["flops"], contact_ms, "contact (GJK + EPA kernels)")
+        rec["roofline"]["fp32"]["flops_per_pair"] = round(fl["flops"] / m, 1)
+        rec["f64_error"] = contact_f64_error(pipe.contacts[:m].cpu().numpy(), shapes.descs, sc["vertices"],
+                                             sc["poses"], pairs, 0.01, threads=cores)
     return rec
 
 
@@ -1020,6 +1257,9 @@ def main() -> int:
         cb, parity = cpu_baseline(v, f, rays, gpu_feat)
         rec["cpu_baseline"] = cb
         rec["parity"] = parity
+        from oracle import oracle
+
+        rec["f64_error"] = ray_f64_error(v, f, rays, out.cpu().numpy(), threads=oracle.default_threads())
     else:
         rec["cpu_baseline"] = None
     if not args.skip_batch:

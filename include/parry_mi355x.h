@@ -349,6 +349,14 @@ typedef struct pq_contact_tuning {
     uint32_t reserved; /* must be 0 */
 } pq_contact_tuning;
 int pq_contact_set_tuning(const pq_contact_tuning *tuning);
+/* Phase timing of the contact path, for measurement (no counterpart in parry).
+ * While enabled (process-wide), every contact call records HIP events on its
+ * own stream between its phases; pq_contact_phase_ms waits for the last call's
+ * events and writes ms[0..4]: the GJK/dispatch kernel, the first EPA tier, the
+ * 768-face tier, the 2048-face tier and the global-memory tier.  Calls that
+ * overlap on several streams share the events; leave it off under graph capture. */
+int pq_contact_set_phase_timing(int enable);
+int pq_contact_phase_ms(float *ms, uint32_t n);
 int pq_contact_get_tuning(pq_contact_tuning *tuning);
 
 /* Device workspace bytes pq_contact_pairs needs for `num_pairs` pairs. */

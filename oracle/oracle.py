@@ -36,6 +36,7 @@ _SIG = {
     "orc_aabb_cast_ray_f32": (_int, [_vp, _vp, _vp, _f32, _int, _vp]),
     "orc_triangle_cast_ray_f32": (_int, [_vp, _vp, _f32, _vp, _vp]),
     "orc_cast_rays_culling_f32": (None, [_vp, _vp, _u64, _f32, _int, _vp, _vp, _vp, _vp, _int, _int]),
+    "orc_cast_rays_culling_f64": (None, [_vp, _vp, _u64, _f64, _int, _vp, _vp, _vp, _vp, _int]),
     "orc_project_points_f32": (None, [_vp, _vp, _u64, _int, _vp, _vp, _vp, _vp, _int, _int]),
     "orc_project_points_f64": (None, [_vp, _vp, _u64, _int, _vp, _vp, _vp, _vp, _int, _int]),
     "orc_triangle_project_point_f32": (_int, [_vp, _vp, _int, _vp]),
@@ -154,19 +155,24 @@ class OracleTriMesh:
 
     def cast_rays_with_culling(self, rays: np.ndarray, max_toi: float, culling: int, threads: int | None = None,
                                brute: bool = False):
-        """TriMesh::cast_local_ray_with_culling (ray_trimesh.rs:152-178), f32: culling 1 =
+        """TriMesh::cast_local_ray_with_culling (ray_trimesh.rs:152-178): culling 1 =
         IgnoreBackfaces, 2 = IgnoreFrontfaces.  Returns dict(feature, toi, normal, W, K): W and K are
         the Bvh wide records fetched and the triangles tested, summed over the rays."""
-        assert not self.f64 and culling in (1, 2)
-        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        assert culling in (1, 2) and not (brute and self.f64)
+        dt = np.float64 if self.f64 else np.float32
+        r = np.ascontiguousarray(rays, dtype=dt).reshape(-1, 6)
         n = r.shape[0]
         fid = np.zeros(n, np.uint32)
-        toi = np.zeros(n, np.float32)
-        nrm = np.zeros((n, 3), np.float32)
+        toi = np.zeros(n, dt)
+        nrm = np.zeros((n, 3), dt)
         cnt = np.zeros(3, np.uint64)
         th = default_threads() if threads is None else threads
-        lib().orc_cast_rays_culling_f32(self.h, r.ctypes.data, n, max_toi, int(culling), fid.ctypes.data,
-                                        toi.ctypes.data, nrm.ctypes.data, cnt.ctypes.data, th, int(brute))
+        if self.f64:
+            lib().orc_cast_rays_culling_f64(self.h, r.ctypes.data, n, max_toi, int(culling), fid.ctypes.data,
+                                            toi.ctypes.data, nrm.ctypes.data, cnt.ctypes.data, th)
+        else:
+            lib().orc_cast_rays_culling_f32(self.h, r.ctypes.data, n, max_toi, int(culling), fid.ctypes.data,
+                                            toi.ctypes.data, nrm.ctypes.data, cnt.ctypes.data, th, int(brute))
         return {"feature": fid, "toi": toi, "normal": nrm, "W": int(cnt[0]), "K": int(cnt[1])}
 
     def project_points(self, points: np.ndarray, solid: bool, threads: int | None = None, brute: bool = False):
@@ -401,6 +407,35 @@ def contact_pairs(descs, vertices, poses, pairs, prediction: float, init_dirs=No
       prediction, None if seeds is None else seeds.ctypes.data, out.ctypes.data, status.ctypes.data,
       iters.ctypes.data, epa_exit.ctypes.data, default_threads() if threads is None else threads)
     return {"status": status, "out": out, "iters": iters, "epa_exit": epa_exit}
+
+
+_SIG["orc_contact_flops_f32"] = (None, [_vp, _u32, _vp, _u32, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp])
+FLOP_COUNT_NAMES = ("add", "mul", "div", "sqrt", "cmp", "other")
+
+
+def contact_flops(descs, vertices, poses, pairs, prediction: float, init_dirs=None):
+    """The f32 oracle's query::contact on every pair (one thread), counting its
+    arithmetic: returns dict(counts {add, mul, div, sqrt, cmp, other}, flops (total
+    add + mul + div + sqrt), per_pair uint32 [m], dist float32 [m], status int32 [m])."""
+    from parry_amd.shape import SHAPE_DESC_DTYPE
+
+    d = np.ascontiguousarray(descs, dtype=SHAPE_DESC_DTYPE)
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    p = np.ascontiguousarray(poses, dtype=np.float32).reshape(-1, 7)
+    pr = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    m = pr.shape[0]
+    seeds = None if init_dirs is None else np.ascontiguousarray(init_dirs, dtype=np.float32).reshape(-1, 3)
+    counts = np.zeros(6, np.uint64)
+    per_pair = np.zeros(m, np.uint32)
+    dist = np.zeros(m, np.float32)
+    status = np.zeros(m, np.int32)
+    lib().orc_contact_flops_f32(d.ctypes.data, d.shape[0], v.ctypes.data if v.size else None, v.shape[0],
+                                p.ctypes.data, pr.ctypes.data, m, prediction,
+                                None if seeds is None else seeds.ctypes.data, counts.ctypes.data,
+                                per_pair.ctypes.data, dist.ctypes.data, status.ctypes.data)
+    c = {k: int(x) for k, x in zip(FLOP_COUNT_NAMES, counts)}
+    return {"counts": c, "flops": c["add"] + c["mul"] + c["div"] + c["sqrt"], "per_pair": per_pair, "dist": dist,
+            "status": status}
 
 
 CAST_NONE, CAST_OUT_OF_ITERATIONS, CAST_CONVERGED, CAST_FAILED, CAST_PENETRATING = range(5)

@@ -183,6 +183,11 @@ void orc_cast_rays_culling_f32(void *m, const float *rays, uint64_t n, float max
     cast_rays<float>((TriMesh<float> *)m, rays, n, max_toi, 0, fid, toi, normal, counters, nullptr, nullptr, nthreads,
                      brute, culling);
 }
+void orc_cast_rays_culling_f64(void *m, const double *rays, uint64_t n, double max_toi, int culling, uint32_t *fid,
+                               double *toi, double *normal, uint64_t *counters, int nthreads) {
+    cast_rays<double>((TriMesh<double> *)m, rays, n, max_toi, 0, fid, toi, normal, counters, nullptr, nullptr,
+                      nthreads, 0, culling);
+}
 void orc_project_points_f32(void *m, const float *pts, uint64_t n, int solid, float *proj, uint32_t *inside,
                             uint32_t *tri, uint64_t *counters, int nthreads, int brute) {
     project_points<float>((TriMesh<float> *)m, pts, n, solid, proj, inside, tri, nthreads, brute, counters);

@@ -84,6 +84,8 @@ SIGNATURES = {
     "pq_cast_shapes_pairs": (_int, [_vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     "pq_contact_set_tuning": (_int, [_vp]),
     "pq_contact_get_tuning": (_int, [_vp]),
+    "pq_contact_set_phase_timing": (_int, [_int]),
+    "pq_contact_phase_ms": (_int, [_vp, _u32]),
     "pq_contact_pairs": (_int, [_vp, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp]),
     "pq_contact_pairs_device_count": (_int, [_vp, _vp, _vp, _u64, _vp, _f32, _vp, _vp, _vp, _vp, _vp]),
     "pq_mlp_create": (_int, [_vp, _u64, _c.POINTER(_vp)]),
@@ -137,6 +139,21 @@ class ShapeCastOptions(ctypes.Structure):
 
 def set_contact_tuning(epa_lanes_per_pair: int, epa_polytope: int = 1, epa_schedule: int = 0) -> None:
     check(lib().pq_contact_set_tuning(ctypes.byref(ContactTuning(epa_lanes_per_pair, epa_polytope, epa_schedule))))
+
+
+CONTACT_PHASES = ("gjk", "epa_first_tier", "epa_768_faces", "epa_2048_faces", "epa_global")
+
+
+def set_contact_phase_timing(enable: bool) -> None:
+    """Records HIP events between the contact path's phases on each call's stream."""
+    check(lib().pq_contact_set_phase_timing(int(bool(enable))))
+
+
+def contact_phase_ms() -> dict:
+    """Milliseconds of each phase of the last contact call (waits for it)."""
+    ms = (ctypes.c_float * len(CONTACT_PHASES))()
+    check(lib().pq_contact_phase_ms(ms, len(CONTACT_PHASES)))
+    return {k: float(v) for k, v in zip(CONTACT_PHASES, ms)}
 
 
 def get_contact_tuning() -> tuple:

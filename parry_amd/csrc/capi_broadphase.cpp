@@ -349,6 +349,25 @@ int pq_contact_set_tuning(const pq_contact_tuning *t) {
     return PQ_OK;
 }
 
+int pq_contact_set_phase_timing(int enable) {
+    if (enable) {
+        for (hipEvent_t &ev : g_contact_phase_events)
+            if (!ev) PQ_HIP_CHECK(hipEventCreate(&ev));
+    }
+    g_contact_phase_timing = enable != 0;
+    return PQ_OK;
+}
+
+int pq_contact_phase_ms(float *ms, uint32_t n) {
+    if (!ms || n < (uint32_t)kContactPhases) return fail(PQ_ERR_INVALID, "pq_contact_phase_ms: need 5 floats");
+    if (!g_contact_phase_events[kContactPhases])
+        return fail(PQ_ERR_INVALID, "pq_contact_phase_ms: phase timing was never enabled");
+    PQ_HIP_CHECK(hipEventSynchronize(g_contact_phase_events[kContactPhases]));
+    for (int k = 0; k < kContactPhases; ++k)
+        PQ_HIP_CHECK(hipEventElapsedTime(&ms[k], g_contact_phase_events[k], g_contact_phase_events[k + 1]));
+    return PQ_OK;
+}
+
 int pq_contact_get_tuning(pq_contact_tuning *t) {
     if (!t) return fail(PQ_ERR_INVALID, "pq_contact_get_tuning: null tuning");
     *t = pq_contact_tuning{(uint32_t)g_epa_lanes_per_pair, (uint32_t)g_epa_polytope, (uint32_t)g_epa_schedule, 0};

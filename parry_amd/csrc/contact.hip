@@ -1407,6 +1407,8 @@ constexpr uint64_t kEpaBigWaves = 256ull * 2;
 int g_epa_lanes_per_pair = 16;
 int g_epa_polytope = 1;
 int g_epa_schedule = 0;
+bool g_contact_phase_timing = false;
+hipEvent_t g_contact_phase_events[kContactPhases + 1] = {};
 
 namespace {
 struct ContactWorkspace {
@@ -1493,11 +1495,16 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
     if (m == 0) return hipSuccess;
     const ContactWorkspace w = carve_workspace(workspace, m);
     DevContact *o = (DevContact *)out;
+    const bool timed = g_contact_phase_timing;
+    auto mark = [&](int k) { return timed ? hipEventRecord(g_contact_phase_events[k], stream) : hipSuccess; };
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, w.counters);
+    hipError_t e = mark(0);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(contact_gjk_kernel, dim3((uint32_t)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
                        s.rec, s.verts, poses, pairs, m, (const unsigned long long *)m_dev, prediction, seeds, o, iters,
                        w.queue, w.counters, flags);
-    hipError_t e = hipGetLastError();
+    e = hipGetLastError();
+    if (e == hipSuccess) e = mark(1);
     if (e != hipSuccess) return e;
     uint32_t *const c = w.counters;
     switch (g_epa_lanes_per_pair) {
@@ -1507,18 +1514,23 @@ hipError_t launch_contacts(const DeviceShapes &s, const float *poses, const uint
         case 32: e = launch_small<32>(s, poses, pairs, prediction, m, w, c, o, iters, stream, flags); break;
         default: e = launch_small<64>(s, poses, pairs, prediction, m, w, c, o, iters, stream, flags); break;
     }
+    if (e == hipSuccess) e = mark(2);
     if (e != hipSuccess) return e;
     const ContSink none{nullptr, nullptr, 0};
     e = launch_epa_tier<64, EpaMedium>(s, poses, pairs, prediction, m, w.queue, w.lists[0], c + 1, c + 5, w.cont_list,
                                        w.conts, none, o, iters, w.lists[1], nullptr, c + 2, w.orig_medium,
                                        kEpaMediumWaves, stream);
+    if (e == hipSuccess) e = mark(3);
     if (e != hipSuccess) return e;
     e = launch_epa_tier<64, EpaBig>(s, poses, pairs, prediction, m, w.queue, w.lists[1], c + 2, c + 6, nullptr, nullptr,
                                     none, o, iters, w.lists[2], nullptr, c + 3, w.orig_big, kEpaBigWaves, stream);
+    if (e == hipSuccess) e = mark(4);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(epa_global_kernel, dim3(kEpaGlobalWaves), dim3(64), 0, stream, s.rec, s.verts, poses, pairs,
                        prediction, w.queue, o, iters, w.lists[2], c + 3, c + 7, w.large, w.orig_large);
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e == hipSuccess) e = mark(5);
+    return e;
 }
 
 }  // namespace pq

@@ -211,6 +211,11 @@ hipError_t launch_casts(const DeviceShapes &s, const float *poses, const float *
 extern int g_epa_lanes_per_pair;
 extern int g_epa_polytope;
 extern int g_epa_schedule;  // first EPA tier: 0 = automatic, 1 = per wave, 2 = a refill per group
+// Phase timing of launch_contacts (pq_contact_set_phase_timing): while on, each
+// call records kContactPhases + 1 events on its stream around its phases.
+constexpr int kContactPhases = 5;  // GJK/dispatch, first EPA tier, 768-face, 2048-face, global-memory tier
+extern bool g_contact_phase_timing;
+extern hipEvent_t g_contact_phase_events[kContactPhases + 1];
 // Per-(device, stream) workspace owned by the library, grown on demand.
 hipError_t stream_workspace(hipStream_t stream, size_t bytes, void **ptr);
 
