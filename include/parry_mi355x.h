@@ -564,6 +564,19 @@ int pq_contact_pairs_device_count(const pq_shapes *shapes, const pq_isometry *po
                                   const float *init_dirs, pq_contact *out, uint32_t *iters, void *workspace,
                                   void *stream);
 
+/* pq_contact_pairs without any device allocation.  No counterpart in parry: it
+ * is for callers that cannot allocate (code under hipGraph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The workspace is a fixed 128 MiB
+ * device array inside the library, one per GPU, held until the process ends;
+ * num_pairs above PQ_CONTACT_NOALLOC_MAX_PAIRS is rejected with PQ_ERR_INVALID.
+ * Results are those of pq_contact_pairs.  Every call on one GPU uses that GPU's
+ * array: calls that overlap on one GPU (several streams or host threads) race.
+ * Asynchronous on `stream`. */
+#define PQ_CONTACT_NOALLOC_MAX_PAIRS 262144u
+int pq_contact_pairs_noalloc(const pq_shapes *shapes, const pq_isometry *poses, const uint32_t *pairs,
+                             uint64_t num_pairs, float prediction, const float *init_dirs, pq_contact *out,
+                             uint32_t *iters, void *stream);
+
 /* ---- Learned GJK warm start (no counterpart in parry) ------------------------
  * A 44-128-128-128-3 MLP (ReLU, bf16 weights on MFMA with f32 accumulation)
  * predicts the GJK seed direction that pq_contact_pairs takes as `init_dirs`

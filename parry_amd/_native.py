@@ -88,6 +88,7 @@ SIGNATURES = {
     "pq_contact_phase_ms": (_int, [_vp, _u32]),
     "pq_contact_pairs": (_int, [_vp, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp, _vp]),
     "pq_contact_pairs_device_count": (_int, [_vp, _vp, _vp, _u64, _vp, _f32, _vp, _vp, _vp, _vp, _vp]),
+    "pq_contact_pairs_noalloc": (_int, [_vp, _vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp]),
     "pq_mlp_create": (_int, [_vp, _u64, _c.POINTER(_vp)]),
     "pq_mlp_destroy": (_int, [_vp]),
     "pq_mlp_get_params": (_int, [_vp, _vp]),
