@@ -107,6 +107,33 @@ size_t pq_bvh_ploc_workspace_bytes(uint32_t n);
 int pq_bvh_from_leaves_ploc(const float *aabbs, uint32_t n, pq_bvh_node_wide *out, uint32_t capacity,
                             uint32_t *num_nodes, uint32_t *depth, void *workspace, void *stream);
 
+/* ---- Serialized Bvh and TriMesh (host only, no GPU needed) --------------------
+ * parry derives serde for Bvh {nodes, parents, leaf_node_indices}
+ * (bvh_tree.rs:1748-1762; BvhNodeWide, BvhNode, BvhNodeData, BvhNodeIndex at
+ * bvh_tree.rs:148-156, 252-266, 447-467, 1243-1250; VecMap {n, v},
+ * utils/vec_map.rs:73-88) and TriMesh {bvh, vertices, indices, pseudo_normals,
+ * topology, connected_components, flags} (trimesh.rs:510-528).  These calls
+ * read and write those fields as bincode 1.x's default options lay them out
+ * (little-endian, fixed-width integers, usize as u64, Vec = u64 length then the
+ * elements, Option = u8 tag then the value, structs and fixed arrays with
+ * nothing in between), so a node is its 64-byte record and a BvhNodeIndex the
+ * u64 (wide << 1) | is_right.  The encoders derive parents and
+ * leaf_node_indices from depth-first nodes as Bvh::from_leaves and refit leave
+ * them (parents[0] = 0; v[leaf] = Some(slot) for leaves 0..n-1); the decoders
+ * check them against the nodes.  TriMesh: the mesh TriMesh::new makes
+ * (pseudo-normals, topology and connected components None, flags empty;
+ * trimesh.rs:607-612, 718-746); the decoder returns PQ_ERR_UNSUPPORTED for a
+ * mesh with any of them set.  *size / counts are set also on PQ_ERR_CAPACITY
+ * (out == NULL queries the size). */
+int pq_bvh_encode(const pq_bvh_node_wide *nodes, uint32_t num_nodes, uint8_t *out, size_t capacity, size_t *size);
+int pq_bvh_decode(const uint8_t *bytes, size_t size, pq_bvh_node_wide *nodes, uint32_t capacity, uint32_t *num_nodes);
+int pq_trimesh_encode(const float *vertices, uint32_t num_vertices, const uint32_t *indices, uint32_t num_triangles,
+                      const pq_bvh_node_wide *nodes, uint32_t num_nodes, uint8_t *out, size_t capacity,
+                      size_t *size);
+int pq_trimesh_decode(const uint8_t *bytes, size_t size, float *vertices, uint32_t vertex_capacity,
+                      uint32_t *num_vertices, uint32_t *indices, uint32_t triangle_capacity, uint32_t *num_triangles,
+                      pq_bvh_node_wide *nodes, uint32_t node_capacity, uint32_t *num_nodes);
+
 /* ---- TriMesh ray casts --------------------------------------------------- */
 
 typedef struct pq_trimesh pq_trimesh;
@@ -130,24 +157,46 @@ int pq_trimesh_info(const pq_trimesh *mesh, uint32_t *num_nodes, uint32_t *depth
 /* Copies the mesh's Bvh (depth-first wide nodes, leaf `children` = triangle index) to host memory. */
 int pq_trimesh_export_bvh(const pq_trimesh *mesh, pq_bvh_node_wide *out_host, uint32_t capacity);
 
-/* Lane mapping of the ray kernel, chosen per mesh (no counterpart in parry;
- * results are identical for every setting, only speed changes):
+/* Kernel mapping of the ray casts, chosen per mesh or per batch (no
+ * counterpart in parry; results, and the per-ray node and triangle counts, are
+ * identical for every setting, only speed changes):
  *   PQ_RAY_MAPPING_STATIC      one launch lane per ray;
  *   PQ_RAY_MAPPING_PERSISTENT  resident waves pull `chunk` ray indices at a time
- *                              from a per-stream queue in device memory and
- *                              refill finished lanes every iteration.
- * The persistent queue of a stream is allocated on the first persistent call
- * on that stream: make that first call outside hipGraph capture. */
+ *                              from a work queue in device memory and refill
+ *                              finished lanes every iteration;
+ *   PQ_RAY_MAPPING_WAVE        one wave per ray: the wave fetches 64 consecutive
+ *                              wide nodes at once (lane k: node base + k, with
+ *                              the slab tests of both children and the triangle
+ *                              tests of its leaves against max_toi), then walks
+ *                              find_best's order over them, refetching when the
+ *                              walk leaves the window; waves pull `chunk` rays at
+ *                              a time from the queue.  For few rays on a large
+ *                              mesh.  Trees deeper than 65 wide levels run the
+ *                              persistent mapping instead.
+ * block_size (static and persistent): lanes per workgroup, 64, 128 or 256.
+ * stack_regs (static and persistent): find_best's pending-node stack keeps its
+ * top 0, 1, 2 or 4 entries in registers and the rest in LDS.
+ * The queue (persistent and wave) of a stream is allocated on the first such
+ * call on that stream: make that first call outside hipGraph capture. */
 #define PQ_RAY_MAPPING_STATIC 0u
 #define PQ_RAY_MAPPING_PERSISTENT 1u
+#define PQ_RAY_MAPPING_WAVE 2u
 typedef struct pq_ray_tuning {
     uint32_t mapping;       /* PQ_RAY_MAPPING_* */
-    uint32_t chunk;         /* persistent: ray indices claimed per queue access (0 = 128) */
-    uint32_t blocks_per_cu; /* persistent: resident 256-lane blocks per CU (0 = occupancy limit) */
+    uint32_t chunk;         /* persistent and wave: rays claimed per queue access (0 = 128; wave: 0 = 4) */
+    uint32_t blocks_per_cu; /* persistent and wave: resident workgroups per CU (0 = occupancy limit) */
+    uint32_t block_size;    /* static and persistent: 64, 128 or 256 lanes (0 = 256) */
+    uint32_t stack_regs;    /* static and persistent: 0, 1, 2 or 4 stack entries in registers */
     uint32_t reserved;      /* must be 0 */
 } pq_ray_tuning;
 int pq_trimesh_set_tuning(pq_trimesh *mesh, const pq_ray_tuning *tuning);
 int pq_trimesh_get_tuning(const pq_trimesh *mesh, pq_ray_tuning *tuning);
+/* Times the kernel mappings (as pq_trimesh_batch_autotune) on these rays
+ * (device, num_rays > 0; hits receives their results), keeps the fastest in the
+ * mesh's tuning and returns it with its time per launch.  Synchronises
+ * `stream`.  The best mapping depends on the number of rays. */
+int pq_trimesh_autotune(pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays, float max_toi, pq_ray_hit *hits,
+                        void *stream, pq_ray_tuning *chosen, float *chosen_ms);
 
 /* Batched RayCast::cast_local_ray_and_get_normal on a TriMesh
  * (src/query/ray/ray.rs:393-403 -> ray_trimesh.rs:16-35 -> Bvh::find_best,
@@ -228,8 +277,9 @@ int pq_trimesh_batch_cast_local_rays(const pq_trimesh_batch *batch, const pq_ray
                                      pq_ray_hit *hits, void *stream);
 int pq_trimesh_batch_cast_local_rays_counted(const pq_trimesh_batch *batch, const pq_ray *rays, float max_toi,
                                              int solid, pq_ray_hit *hits, uint32_t *counts, void *stream);
-/* Times the lane mappings (static; persistent with several chunk sizes and
- * residencies) on these rays, keeps the fastest in the batch and returns it
+/* Times the kernel mappings on these rays (static and persistent lanes over
+ * block sizes and register stack depths, persistent chunk sizes and
+ * residencies, wave per ray), keeps the fastest in the batch and returns it
  * with its time per launch.  Synchronises `stream`. */
 int pq_trimesh_batch_autotune(pq_trimesh_batch *batch, const pq_ray *rays, float max_toi, pq_ray_hit *hits,
                               void *stream, pq_ray_tuning *chosen, float *chosen_ms);
@@ -286,6 +336,12 @@ typedef struct pq_bvh pq_bvh;
 /* Bvh::from_leaves(BvhBuildStrategy::Binned, aabbs) (bvh_tree.rs:1835-1837) built
  * on the host from `aabbs` (host, n x {mins[3], maxs[3]}) and uploaded. */
 int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out);
+/* A broadphase handle over a tree the caller already has (host, depth-first
+ * wide records over leaves 0..num_leaves-1, e.g. decoded with pq_bvh_decode or
+ * exported from a Rust Bvh): the Bvh a deserialized parry Bvh is
+ * (bvh_tree.rs:1748-1762).  Fails with PQ_ERR_INVALID if the records are not
+ * such a tree. */
+int pq_bvh_create_from_nodes(const pq_bvh_node_wide *nodes, uint32_t num_nodes, uint32_t num_leaves, pq_bvh **out);
 /* Bvh::from_leaves(BvhBuildStrategy::Ploc, aabbs) (bvh_tree.rs:1891-1955 with
  * rebuild_range_ploc, bvh_ploc_build.rs:10-93, and refit's depth-first layout)
  * built on the GPU, from `aabbs` in device memory, into a broadphase handle. */

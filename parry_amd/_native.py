@@ -41,6 +41,7 @@ SIGNATURES = {
     "pq_trimesh_info": (_int, [_vp, _c.POINTER(_u32), _c.POINTER(_u32), _c.POINTER(_u32)]),
     "pq_trimesh_set_tuning": (_int, [_vp, _vp]),
     "pq_trimesh_get_tuning": (_int, [_vp, _vp]),
+    "pq_trimesh_autotune": (_int, [_vp, _vp, _u64, _f32, _vp, _vp, _vp, _vp]),
     "pq_trimesh_export_bvh": (_int, [_vp, _vp, _u32]),
     "pq_trimesh_cast_local_rays": (_int, [_vp, _vp, _u64, _f32, _int, _vp, _vp]),
     "pq_trimesh_cast_local_rays_counted": (_int, [_vp, _vp, _u64, _f32, _int, _vp, _vp, _vp]),
@@ -68,6 +69,12 @@ SIGNATURES = {
     "pq_shapes_set_polyhedron_faces": (_int, [_vp, _vp, _u32, _vp, _vp, _u32, _vp]),
     "pq_shapes_compute_aabbs": (_int, [_vp, _vp, _vp, _vp]),
     "pq_bvh_create": (_int, [_vp, _u32, _c.POINTER(_vp)]),
+    "pq_bvh_create_from_nodes": (_int, [_vp, _u32, _u32, _c.POINTER(_vp)]),
+    "pq_bvh_encode": (_int, [_vp, _u32, _vp, _c.c_size_t, _c.POINTER(_c.c_size_t)]),
+    "pq_bvh_decode": (_int, [_vp, _c.c_size_t, _vp, _u32, _c.POINTER(_u32)]),
+    "pq_trimesh_encode": (_int, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _c.c_size_t, _c.POINTER(_c.c_size_t)]),
+    "pq_trimesh_decode": (_int, [_vp, _c.c_size_t, _vp, _u32, _c.POINTER(_u32), _vp, _u32, _c.POINTER(_u32), _vp, _u32,
+                                 _c.POINTER(_u32)]),
     "pq_bvh_destroy": (_int, [_vp]),
     "pq_bvh_create_ploc": (_int, [_vp, _u32, _c.POINTER(_vp), _vp]),
     "pq_bvh_ploc_workspace_bytes": (_c.c_size_t, [_u32]),
@@ -117,12 +124,29 @@ _lib = None
 
 PQ_RAY_MAPPING_STATIC = 0
 PQ_RAY_MAPPING_PERSISTENT = 1
+PQ_RAY_MAPPING_WAVE = 2
 
 
 class RayTuning(ctypes.Structure):
     """pq_ray_tuning (include/parry_mi355x.h)."""
 
-    _fields_ = [("mapping", _u32), ("chunk", _u32), ("blocks_per_cu", _u32), ("reserved", _u32)]
+    _fields_ = [("mapping", _u32), ("chunk", _u32), ("blocks_per_cu", _u32), ("block_size", _u32),
+                ("stack_regs", _u32), ("reserved", _u32)]
+
+
+RAY_MAPPINGS = {"static": 0, "persistent": 1, "wave": 2}
+
+
+def ray_tuning(mapping: str = "persistent", chunk: int = 0, blocks_per_cu: int = 0, block_size: int = 256,
+               stack_regs: int = 0) -> RayTuning:
+    """A pq_ray_tuning from names: mapping "static", "persistent" or "wave"."""
+    return RayTuning(RAY_MAPPINGS[mapping], chunk, blocks_per_cu, block_size, stack_regs, 0)
+
+
+def ray_tuning_dict(t: RayTuning) -> dict:
+    names = {v: k for k, v in RAY_MAPPINGS.items()}
+    return {"mapping": names[int(t.mapping)], "chunk": int(t.chunk), "blocks_per_cu": int(t.blocks_per_cu),
+            "block_size": int(t.block_size) or 256, "stack_regs": int(t.stack_regs)}
 
 
 class ContactTuning(ctypes.Structure):

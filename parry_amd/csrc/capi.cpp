@@ -25,6 +25,16 @@ int hip_fail(hipError_t e, const char *what) {
     return fail(e == hipErrorOutOfMemory ? PQ_ERR_OOM : PQ_ERR_HIP, m);
 }
 
+int check_ray_tuning(const pq_ray_tuning *t, const char *who) {
+    if (!t) return fail(PQ_ERR_INVALID, std::string(who) + ": null tuning");
+    const uint32_t b = t->block_size, r = t->stack_regs;
+    if (t->mapping > PQ_RAY_MAPPING_WAVE || t->reserved != 0 || t->chunk > (1u << 20) ||
+        (b != 0 && b != 64 && b != 128 && b != 256) || (r != 0 && r != 1 && r != 2 && r != 4))
+        return fail(PQ_ERR_INVALID, std::string(who) + ": invalid tuning (mapping 0-2, chunk <= 2^20, block_size "
+                                                       "0/64/128/256, stack_regs 0/1/2/4, reserved 0)");
+    return PQ_OK;
+}
+
 hipError_t stream_queue(hipStream_t stream, unsigned long long **queue) {
     static std::mutex mu;
     static std::map<std::pair<int, hipStream_t>, unsigned long long *> queues;
@@ -225,7 +235,7 @@ int pq_trimesh_create(const float *vertices, uint32_t num_vertices, const uint32
     m->dev.num_nodes = (uint32_t)dev_nodes.size();
     m->dev.num_triangles = num_triangles;
     m->dev.depth = depth;
-    m->dev.tuning = pq_ray_tuning{PQ_RAY_MAPPING_STATIC, 128, 0, 0};
+    m->dev.tuning = pq_ray_tuning{PQ_RAY_MAPPING_STATIC, 128, 0, 256, 0, 0};
     *out = m;
     return PQ_OK;
 }
@@ -247,11 +257,19 @@ int pq_trimesh_info(const pq_trimesh *m, uint32_t *num_nodes, uint32_t *depth, u
 }
 
 int pq_trimesh_set_tuning(pq_trimesh *m, const pq_ray_tuning *t) {
-    if (!m || !t) return fail(PQ_ERR_INVALID, "pq_trimesh_set_tuning: null argument");
-    if (t->mapping > PQ_RAY_MAPPING_PERSISTENT || t->reserved != 0 || t->chunk > (1u << 20))
-        return fail(PQ_ERR_INVALID, "pq_trimesh_set_tuning: invalid tuning");
+    if (!m) return fail(PQ_ERR_INVALID, "pq_trimesh_set_tuning: null mesh");
+    const int rc = check_ray_tuning(t, "pq_trimesh_set_tuning");
+    if (rc != PQ_OK) return rc;
     m->dev.tuning = *t;
     return PQ_OK;
+}
+
+int pq_trimesh_autotune(pq_trimesh *m, const pq_ray *rays, uint64_t num_rays, float max_toi, pq_ray_hit *hits,
+                        void *stream, pq_ray_tuning *chosen, float *chosen_ms) {
+    if (!m || !rays || !hits || num_rays == 0) return fail(PQ_ERR_INVALID, "pq_trimesh_autotune: null argument");
+    const hipStream_t s = (hipStream_t)stream;
+    return autotune_ray_mapping([&]() { return launch_cast_rays(m->dev, rays, num_rays, max_toi, hits, nullptr, s); },
+                                m->dev.tuning, s, chosen, chosen_ms, "pq_trimesh_autotune");
 }
 
 int pq_trimesh_get_tuning(const pq_trimesh *m, pq_ray_tuning *t) {

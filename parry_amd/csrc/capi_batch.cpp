@@ -1,6 +1,7 @@
 // capi_batch.cpp -- batched multi-mesh ray casts, their tuner and hipGraph capture.
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -8,19 +9,78 @@
 
 using namespace pq;
 
+namespace pq {
+
+// The candidates the ray tuners time: the lane mappings over block sizes and
+// register stack depths, the persistent mapping's chunk sizes and a capped
+// residency, and the wave mapping with one and four rays per claim.
+std::vector<pq_ray_tuning> ray_tuning_candidates() {
+    std::vector<pq_ray_tuning> c;
+    for (uint32_t block : {64u, 128u, 256u})
+        for (uint32_t regs : {0u, 1u, 2u, 4u}) c.push_back({PQ_RAY_MAPPING_STATIC, 128, 0, block, regs, 0});
+    for (uint32_t block : {64u, 256u})
+        for (uint32_t regs : {0u, 2u})
+            for (uint32_t chunk : {64u, 128u, 256u}) c.push_back({PQ_RAY_MAPPING_PERSISTENT, chunk, 0, block, regs, 0});
+    c.push_back({PQ_RAY_MAPPING_PERSISTENT, 128, 4, 256, 0, 0});
+    for (uint32_t chunk : {1u, 4u}) c.push_back({PQ_RAY_MAPPING_WAVE, chunk, 0, 64, 0, 0});
+    return c;
+}
+
+int autotune_ray_mapping(const std::function<hipError_t()> &launch, pq_ray_tuning &slot, hipStream_t s,
+                         pq_ray_tuning *chosen, float *chosen_ms, const char *who) {
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventCreate(&e2);
+    const pq_ray_tuning saved = slot;
+    float best_ms = 1e30f;
+    pq_ray_tuning best = saved;
+    for (const pq_ray_tuning &c : ray_tuning_candidates()) {
+        if (e != hipSuccess) break;
+        slot = c;
+        // One untimed-for-the-result launch (also warms the caches); a mapping
+        // more than 3x slower than the best so far is not timed further.
+        if ((e = hipEventRecord(e0, s)) != hipSuccess || (e = launch()) != hipSuccess ||
+            (e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)
+            break;
+        float warm = 0.0f;
+        if ((e = hipEventElapsedTime(&warm, e0, e1)) != hipSuccess) break;
+        if (warm > 3.0f * best_ms) continue;
+        if ((e = hipEventRecord(e1, s)) != hipSuccess) break;
+        for (int k = 0; k < 3 && e == hipSuccess; ++k) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(e2, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e2);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e1, e2);
+        if (e != hipSuccess) break;
+        ms /= 3.0f;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e2) (void)hipEventDestroy(e2);
+    if (e != hipSuccess) {
+        slot = saved;
+        return hip_fail(e, who);
+    }
+    slot = best;
+    if (chosen) *chosen = best;
+    if (chosen_ms) *chosen_ms = best_ms;
+    return PQ_OK;
+}
+
+}  // namespace pq
+
+
 namespace {
 
 int batch_alloc(pq_trimesh_batch *b, void **p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes ? bytes : 16);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
     b->allocations.push_back(*p);
-    return PQ_OK;
-}
-
-int check_tuning(const pq_ray_tuning *t) {
-    if (!t) return fail(PQ_ERR_INVALID, "null tuning");
-    if (t->mapping > PQ_RAY_MAPPING_PERSISTENT || t->reserved != 0 || t->chunk > (1u << 20))
-        return fail(PQ_ERR_INVALID, "invalid ray tuning");
     return PQ_OK;
 }
 
@@ -44,7 +104,7 @@ int pq_trimesh_batch_create(const pq_trimesh *const *meshes, uint32_t num_meshes
         const DeviceMesh &m = meshes[i]->dev;
         // A root whose right child has no leaf holds the mesh's single triangle.
         const bool single = (meshes[i]->host_nodes[0].right.data & 0x3fffffffu) == 0u;
-        bm[i] = BatchMesh{m.nodes, m.tris, m.num_triangles, single ? 1u : 0u};
+        bm[i] = BatchMesh{m.nodes, m.tris, m.num_triangles, single ? 1u : 0u, m.num_nodes, 0u};
         max_depth = std::max(max_depth, m.depth);
     }
     auto *b = new pq_trimesh_batch();
@@ -52,7 +112,7 @@ int pq_trimesh_batch_create(const pq_trimesh *const *meshes, uint32_t num_meshes
     d.num_meshes = num_meshes;
     d.max_depth = max_depth;
     d.num_rays = ray_offsets[num_meshes];
-    d.tuning = pq_ray_tuning{PQ_RAY_MAPPING_STATIC, 128, 0, 0};
+    d.tuning = pq_ray_tuning{PQ_RAY_MAPPING_STATIC, 128, 0, 256, 0, 0};
     int rc = batch_alloc(b, (void **)&d.meshes, num_meshes * sizeof(BatchMesh));
     if (rc == PQ_OK) rc = batch_alloc(b, (void **)&d.offsets, (num_meshes + 1) * sizeof(unsigned long long));
     if (rc == PQ_OK) rc = batch_alloc(b, (void **)&d.queue, 2 * sizeof(unsigned long long));
@@ -80,7 +140,7 @@ int pq_trimesh_batch_destroy(pq_trimesh_batch *b) {
 
 int pq_trimesh_batch_set_tuning(pq_trimesh_batch *b, const pq_ray_tuning *t) {
     if (!b) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_set_tuning: null batch");
-    int rc = check_tuning(t);
+    int rc = check_ray_tuning(t, "pq_trimesh_batch_set_tuning");
     if (rc != PQ_OK) return rc;
     b->dev.tuning = *t;
     return PQ_OK;
@@ -115,37 +175,9 @@ int pq_trimesh_batch_autotune(pq_trimesh_batch *b, const pq_ray *rays, float max
                               pq_ray_tuning *chosen, float *chosen_ms) {
     if (!b || !rays || !hits) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_autotune: null argument");
     const hipStream_t s = (hipStream_t)stream;
-    const pq_ray_tuning candidates[] = {
-        {PQ_RAY_MAPPING_STATIC, 128, 0, 0},     {PQ_RAY_MAPPING_PERSISTENT, 64, 0, 0},
-        {PQ_RAY_MAPPING_PERSISTENT, 128, 0, 0}, {PQ_RAY_MAPPING_PERSISTENT, 256, 0, 0},
-        {PQ_RAY_MAPPING_PERSISTENT, 512, 0, 0}, {PQ_RAY_MAPPING_PERSISTENT, 128, 4, 0},
-    };
-    hipEvent_t e0, e1;
-    PQ_HIP_CHECK(hipEventCreate(&e0));
-    PQ_HIP_CHECK(hipEventCreate(&e1));
-    float best_ms = 1e30f;
-    pq_ray_tuning best = candidates[0];
-    for (const pq_ray_tuning &c : candidates) {
-        b->dev.tuning = c;
-        PQ_HIP_CHECK(launch_cast_rays_batch(b->dev, rays, max_toi, hits, nullptr, s));  // warm-up
-        PQ_HIP_CHECK(hipEventRecord(e0, s));
-        for (int k = 0; k < 3; ++k) PQ_HIP_CHECK(launch_cast_rays_batch(b->dev, rays, max_toi, hits, nullptr, s));
-        PQ_HIP_CHECK(hipEventRecord(e1, s));
-        PQ_HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        PQ_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        ms /= 3.0f;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = c;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    b->dev.tuning = best;
-    if (chosen) *chosen = best;
-    if (chosen_ms) *chosen_ms = best_ms;
-    return PQ_OK;
+    return autotune_ray_mapping(
+        [&]() { return launch_cast_rays_batch(b->dev, rays, max_toi, hits, nullptr, s); }, b->dev.tuning, s, chosen,
+        chosen_ms, "pq_trimesh_batch_autotune");
 }
 
 int pq_trimesh_batch_capture(const pq_trimesh_batch *b, const pq_ray *rays, float max_toi, int solid,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,8 @@ namespace pq {
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
+// Validates a pq_ray_tuning (PQ_OK, or PQ_ERR_INVALID with the message set).
+int check_ray_tuning(const pq_ray_tuning *t, const char *who);
 
 #define PQ_HIP_CHECK(expr)                                   \
     do {                                                     \
@@ -71,6 +74,8 @@ struct BatchMesh {
     const TriRecord *tris;
     uint32_t ntri;
     uint32_t single_leaf;
+    uint32_t num_nodes;
+    uint32_t pad;
 };
 
 struct DeviceBatch {
@@ -92,6 +97,12 @@ hipError_t stream_queue(hipStream_t stream, unsigned long long **queue);
 
 // LDS stack entries the ray kernels need for a tree of `depth` wide levels.
 uint32_t ray_stack_entries(uint32_t depth);
+// Whether the wave-per-ray mapping can run a tree of `depth` wide levels.
+bool wave_mapping_fits(uint32_t depth);
+// Times every ray mapping candidate with `launch` (which reads `slot`), leaves
+// the fastest in `slot` and reports it; synchronises `s`.
+int autotune_ray_mapping(const std::function<hipError_t()> &launch, pq_ray_tuning &slot, hipStream_t s,
+                         pq_ray_tuning *chosen, float *chosen_ms, const char *who);
 
 // Launches the ray-cast kernel (ray_trimesh.hip).
 // `counts` (nullable, device, 2 per ray): wide-node records fetched, triangles tested.

@@ -11,29 +11,33 @@
 //   * a leaf replaces the best only if its toi is strictly smaller, and the
 //     right child is scored against the best already updated by the left leaf.
 // The node record is parry's 64-byte BvhNodeWide, fetched as four 16-byte
-// loads; the short traversal stack (at most depth-1 entries) lives in LDS,
-// strided by the block size so each lane's slot is bank-conflict free.
+// loads.  The pending-node stack (at most depth-1 entries) keeps its top
+// entries in registers and the rest in LDS, strided by the block size so each
+// lane's slots are bank-conflict free.
 //
-// Two lane mappings (chosen per mesh by the tuner, pq_trimesh_set_tuning):
+// Three kernel mappings (pq_ray_tuning, chosen per mesh or per batch by the
+// tuner; every one gives the same hits and the same node and triangle counts):
 //   * static: one launch lane per ray; a wave lasts as long as its slowest ray;
 //   * persistent: a grid sized to the resident waves; each wave pulls chunks
 //     of ray indices from a queue in device memory, and every iteration the
 //     lanes whose ray has finished (found with a wave ballot) take the next
 //     rays of the chunk, ranked by a masked popcount, so the wave stays full
 //     until the queue drains.  The queue counter resets itself: the last
-//     wave to finish zeroes it for the next launch on the stream.
+//     wave to finish zeroes it for the next launch on the stream;
+//   * wave: one wave per ray, 64 wide nodes fetched at once (see below).
+// The lane mappings are instantiated for 64-, 128- and 256-lane workgroups
+// and 0, 1, 2 or 4 stack entries in registers.
 //
 // Floating point: compiled with -ffp-contract=off; 1/x and sqrt are the
 // correctly rounded IEEE operations, min/max are IEEE maxNum/minNum (the
 // semantics of Rust's f32::min/max), so every value is the one parry computes.
 #include <cfloat>
+#include <type_traits>
 
 #include "pq_internal.h"
 
 namespace pq {
 namespace {
-
-constexpr int kRayBlock = 256;
 
 struct LaneRay {
     float ox, oy, oz;
@@ -136,12 +140,51 @@ __device__ __forceinline__ bool ray_triangle(const float4 q0, const float4 q1, c
 
 __device__ __forceinline__ bool node_is_leaf(uint32_t data) { return (data & 0x3fffffffu) == 1u; }
 
+// The pending-node stack of find_best (bvh_traverse.rs:361-416), one per lane:
+// the top REG entries in registers, the rest in this lane's LDS column (stride
+// BLOCK words, so the lanes of a wave hit distinct banks).  The register slots
+// shift by value with constant indices, so they never go to scratch.  A pop
+// refills the deepest register slot from LDS, a load nothing waits on until
+// REG more pops.  It never holds more than depth-1 entries (each pending entry
+// sits one level deeper than the one below it); the host sizes the LDS part to
+// that less REG.
+template <int BLOCK, int REG>
+struct LaneStack {
+    uint32_t *col;
+    uint32_t reg[REG > 0 ? REG : 1];
+    uint32_t sp;
+    __device__ __forceinline__ void reset() { sp = 0; }
+    __device__ __forceinline__ bool empty() const { return sp == 0; }
+    __device__ __forceinline__ void push(uint32_t x) {
+        if constexpr (REG == 0) {
+            col[sp * BLOCK] = x;
+        } else {
+            if (sp >= (uint32_t)REG) col[(sp - REG) * BLOCK] = reg[REG - 1];
+#pragma unroll
+            for (int k = REG - 1; k > 0; --k) reg[k] = reg[k - 1];
+            reg[0] = x;
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ uint32_t pop() {
+        --sp;
+        if constexpr (REG == 0) {
+            return col[sp * BLOCK];
+        } else {
+            const uint32_t x = reg[0];
+#pragma unroll
+            for (int k = 0; k < REG - 1; ++k) reg[k] = reg[k + 1];
+            if (sp >= (uint32_t)REG) reg[REG - 1] = col[(sp - REG) * BLOCK];
+            return x;
+        }
+    }
+};
+
 struct Traversal {
     float best_cost;
     TriHit best;
     uint32_t best_leaf;
     uint32_t curr;
-    uint32_t sp;
     bool have;
 };
 
@@ -168,7 +211,6 @@ __device__ __forceinline__ void start_traversal(Traversal &t, float max_toi) {
     t.best = TriHit{};
     t.best_leaf = 0;
     t.curr = 0;
-    t.sp = 0;
     t.have = false;
 }
 
@@ -188,13 +230,10 @@ __device__ __forceinline__ void visit_leaf(const float4 *tri4, uint32_t leaf, co
 }
 
 // One iteration of find_best's loop (bvh_traverse.rs:361-416) on wide node
-// t.curr.  Returns true when the traversal has finished.  `stk` is this lane's
-// LDS stack (stride kRayBlock); it never holds more than depth-1 entries, the
-// most a root-to-leaf path can leave pending (each pending entry sits one
-// level deeper than the one below it), and the host sizes it to that.
-template <bool COUNT, int CULL = 0>
+// t.curr.  Returns true when the traversal has finished.
+template <bool COUNT, int CULL, class Stack>
 __device__ __forceinline__ bool find_best_step(const float4 *node4, const float4 *tri4, const LaneRay &r,
-                                               Traversal &t, uint32_t *stk, uint32_t &n_wide, uint32_t &n_tri) {
+                                               Traversal &t, Stack &stk, uint32_t &n_wide, uint32_t &n_tri) {
     if (COUNT) ++n_wide;
     const float4 q0 = node4[4 * t.curr + 0];  // left mins, left children
     const float4 q1 = node4[4 * t.curr + 1];  // left maxs, left data
@@ -229,37 +268,39 @@ __device__ __forceinline__ bool find_best_step(const float4 *node4, const float4
         if (node_is_leaf(rd)) {
             visit_leaf<COUNT, CULL>(tri4, rc, r, t, n_tri);
         } else if (found_next) {
-            stk[t.sp * kRayBlock] = rc;
-            ++t.sp;
+            stk.push(rc);
         } else {
             t.curr = rc;
             found_next = true;
         }
     }
     if (!found_next) {
-        if (t.sp == 0) return true;
-        --t.sp;
-        t.curr = stk[t.sp * kRayBlock];
+        if (stk.empty()) return true;
+        t.curr = stk.pop();
     }
     return false;
 }
 
-__device__ __forceinline__ void write_hit(pq_ray_hit *hits, uint64_t i, const Traversal &t, const float4 *tri4,
-                                          uint32_t ntri) {
+__device__ __forceinline__ pq_ray_hit make_hit(bool have, const TriHit &h, uint32_t tri_id, uint32_t ntri) {
     pq_ray_hit out;
-    if (t.have) {
-        const uint32_t tri_id = __float_as_uint(tri4[3 * t.best_leaf + 2].y);
-        out.time_of_impact = t.best.toi;
-        out.normal[0] = t.best.nx;
-        out.normal[1] = t.best.ny;
-        out.normal[2] = t.best.nz;
-        out.feature = t.best.back ? tri_id + ntri : tri_id;  // ray_trimesh.rs:25-33
+    if (have) {
+        out.time_of_impact = h.toi;
+        out.normal[0] = h.nx;
+        out.normal[1] = h.ny;
+        out.normal[2] = h.nz;
+        out.feature = h.back ? tri_id + ntri : tri_id;  // ray_trimesh.rs:25-33
     } else {
         out.time_of_impact = 0.0f;
         out.normal[0] = out.normal[1] = out.normal[2] = 0.0f;
         out.feature = PQ_NO_HIT;
     }
-    hits[i] = out;
+    return out;
+}
+
+__device__ __forceinline__ void write_hit(pq_ray_hit *hits, uint64_t i, const Traversal &t, const float4 *tri4,
+                                          uint32_t ntri) {
+    const uint32_t tri_id = t.have ? __float_as_uint(tri4[3 * t.best_leaf + 2].y) : 0u;
+    hits[i] = make_hit(t.have, t.best, tri_id, ntri);
 }
 
 // Static mapping; also serves trees whose root holds a single leaf
@@ -267,16 +308,18 @@ __device__ __forceinline__ void write_hit(pq_ray_hit *hits, uint64_t i, const Tr
 // number of wide-node records fetched and of triangles tested (the traffic
 // model of the roofline figure).  CULL selects the culling variant
 // (TriMesh::cast_local_ray_with_culling, ray_trimesh.rs:152-178).
-template <bool COUNT, int CULL = 0>
-__global__ __launch_bounds__(kRayBlock) void cast_rays_kernel(const pq_bvh_node_wide *__restrict__ nodes,
-                                                              const TriRecord *__restrict__ tris, uint32_t ntri,
-                                                              const pq_ray *__restrict__ rays, uint64_t n,
-                                                              float max_toi, pq_ray_hit *__restrict__ hits,
-                                                              uint32_t *__restrict__ counts) {
+template <bool COUNT, int CULL, int BLOCK, int REG>
+__global__ __launch_bounds__(BLOCK) void cast_rays_kernel(const pq_bvh_node_wide *__restrict__ nodes,
+                                                          const TriRecord *__restrict__ tris, uint32_t ntri,
+                                                          const pq_ray *__restrict__ rays, uint64_t n, float max_toi,
+                                                          pq_ray_hit *__restrict__ hits,
+                                                          uint32_t *__restrict__ counts) {
     extern __shared__ uint32_t lds_stack[];
-    const uint64_t i = (uint64_t)blockIdx.x * kRayBlock + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    uint32_t *stk = lds_stack + threadIdx.x;
+    LaneStack<BLOCK, REG> stk;
+    stk.col = lds_stack + threadIdx.x;
+    stk.reset();
     const float4 *node4 = reinterpret_cast<const float4 *>(nodes);
     const float4 *tri4 = reinterpret_cast<const float4 *>(tris);
     const LaneRay r = load_ray(rays, i);
@@ -305,19 +348,32 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Persistent mapping over trees with at least two leaves.  queue[0] is the
-// next unclaimed ray index, queue[1] counts finished waves; both are zero at
-// launch and are zero again when the last wave exits.
-template <bool COUNT>
-__global__ __launch_bounds__(kRayBlock) void cast_rays_persistent_kernel(
+// The last wave of a persistent grid to finish zeroes the queue for the next
+// launch on the stream.  queue[0] is the next unclaimed ray index, queue[1]
+// counts finished waves.
+__device__ __forceinline__ void queue_wave_done(unsigned long long *queue, uint32_t waves) {
+    if (__lane_id() == 0) {
+        __threadfence();
+        const unsigned long long done = atomicAdd(&queue[1], 1ull);
+        if (done == waves - 1) {
+            queue[0] = 0;
+            queue[1] = 0;
+        }
+    }
+}
+
+// Persistent lane mapping over trees with at least two leaves.
+template <bool COUNT, int BLOCK, int REG>
+__global__ __launch_bounds__(BLOCK) void cast_rays_persistent_kernel(
     const pq_bvh_node_wide *__restrict__ nodes, const TriRecord *__restrict__ tris, uint32_t ntri,
     const pq_ray *__restrict__ rays, uint64_t n, float max_toi, pq_ray_hit *__restrict__ hits,
     uint32_t *__restrict__ counts, unsigned long long *__restrict__ queue, uint32_t chunk) {
     extern __shared__ uint32_t lds_stack[];
-    uint32_t *stk = lds_stack + threadIdx.x;
+    LaneStack<BLOCK, REG> stk;
+    stk.col = lds_stack + threadIdx.x;
+    stk.reset();
     const float4 *node4 = reinterpret_cast<const float4 *>(nodes);
     const float4 *tri4 = reinterpret_cast<const float4 *>(tris);
-    const uint32_t lane = __lane_id();
 
     uint64_t q_next = 0, q_end = 0;  // this wave's claimed range [q_next, q_end)
     bool drained = false;            // the shared queue is empty (wave-uniform)
@@ -333,7 +389,7 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_persistent_kernel(
         if (idle != 0 && !drained) {
             if (q_next >= q_end) {
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(&queue[0], (unsigned long long)chunk);
+                if (__lane_id() == 0) base = atomicAdd(&queue[0], (unsigned long long)chunk);
                 base = __shfl(base, 0);
                 q_next = base;
                 q_end = base + chunk < n ? base + chunk : n;
@@ -346,6 +402,7 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_persistent_kernel(
                     ray_id = q_next + rank;
                     r = load_ray(rays, ray_id);
                     start_traversal(t, max_toi);
+                    stk.reset();
                     n_wide = 0;
                     n_tri = 0;
                     active = true;
@@ -358,7 +415,7 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_persistent_kernel(
             if (drained) break;
             continue;
         }
-        if (active && find_best_step<COUNT>(node4, tri4, r, t, stk, n_wide, n_tri)) {
+        if (active && find_best_step<COUNT, 0>(node4, tri4, r, t, stk, n_wide, n_tri)) {
             if (COUNT) {
                 counts[2 * ray_id] = n_wide;
                 counts[2 * ray_id + 1] = n_tri;
@@ -367,15 +424,199 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_persistent_kernel(
             active = false;
         }
     }
-    if (lane == 0) {
-        __threadfence();
-        const uint32_t waves = gridDim.x * (kRayBlock / 64);
-        const unsigned long long done = atomicAdd(&queue[1], 1ull);
-        if (done == waves - 1) {
-            queue[0] = 0;
-            queue[1] = 0;
+    queue_wave_done(queue, gridDim.x * (BLOCK / 64));
+}
+
+// ---------------------------------------------------------------------------
+// Wave per ray.  find_best's decisions only compare a child's slab cost and a
+// leaf's time of impact with the current best, so both can be computed ahead
+// against max_toi instead of the best of the moment:
+//   * slab test: with tmax = best the cost is tmin if tmin <= min(best, exit)
+//     and f32::MAX otherwise; with tmax = max_toi (>= best) it is tmin in more
+//     cases, but only where tmin >= best, i.e. for children that are never
+//     entered either way, and ordering the two children by the max_toi costs
+//     puts an enterable child first exactly when the best-bounded costs do;
+//   * triangle: `toi <= best && toi < best` is `toi <= max_toi && toi < best`.
+// So a wave fetches a window of 64 consecutive wide nodes (in depth-first
+// order, the first 64 nodes of the subtree under the window's first node),
+// lane k computing both children of node base + k, including the triangle of
+// each leaf child it could enter, and then runs find_best's loop with uniform
+// (scalar) state, reading node k's results from lane k.  A node outside the
+// window starts a new window there.  The pending-node stack is one VGPR: entry
+// k in lane k (depth-1 <= 64 entries; the host keeps deeper trees on the lane
+// mappings).  The winner's normal is recomputed from its triangle at the end.
+// ---------------------------------------------------------------------------
+struct WaveWindow {
+    float ls, rs, lt, rt;   // child slab costs and leaf times of impact (+inf: no hit)
+    uint32_t lc, rc, ld, rd;
+};
+
+__device__ __forceinline__ WaveWindow fetch_window(const float4 *node4, uint32_t num_nodes, const float4 *tri4,
+                                                   uint32_t base, const LaneRay &r, float max_toi) {
+    WaveWindow w{FLT_MAX, FLT_MAX, INFINITY, INFINITY, 0u, 0u, 0u, 0u};
+    const uint32_t k = base + __lane_id();
+    if (k < num_nodes) {
+        const float4 q0 = node4[4 * k + 0], q1 = node4[4 * k + 1];
+        const float4 q2 = node4[4 * k + 2], q3 = node4[4 * k + 3];
+        w.ls = slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi);
+        w.rs = slab_cost(q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, r, max_toi);
+        w.lc = __float_as_uint(q0.w);
+        w.ld = __float_as_uint(q1.w);
+        w.rc = __float_as_uint(q2.w);
+        w.rd = __float_as_uint(q3.w);
+        TriHit h;
+        if (node_is_leaf(w.ld) && w.ls < max_toi && w.ls != FLT_MAX &&
+            ray_triangle<0>(tri4[3 * w.lc], tri4[3 * w.lc + 1], tri4[3 * w.lc + 2], r, max_toi, h))
+            w.lt = h.toi;
+        if (node_is_leaf(w.rd) && w.rs < max_toi && w.rs != FLT_MAX &&
+            ray_triangle<0>(tri4[3 * w.rc], tri4[3 * w.rc + 1], tri4[3 * w.rc + 2], r, max_toi, h))
+            w.rt = h.toi;
+    }
+    return w;
+}
+
+__device__ __forceinline__ float lane_f(float v, uint32_t l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)l));
+}
+__device__ __forceinline__ uint32_t lane_u(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+
+// find_best for one ray by the whole wave; every lane returns the hit.
+template <bool COUNT>
+__device__ __forceinline__ pq_ray_hit wave_cast_one(const float4 *node4, uint32_t num_nodes, const float4 *tri4,
+                                                    uint32_t ntri, bool single_leaf, const LaneRay &r,
+                                                    float max_toi, uint32_t &n_wide, uint32_t &n_tri) {
+    float best = max_toi;
+    uint32_t best_leaf = 0;
+    bool have = false;
+    n_wide = n_tri = 0;
+    if (single_leaf) {
+        const float4 q0 = node4[0], q1 = node4[1];
+        n_wide = 1;
+        n_tri = 0;
+        TriHit h;
+        if (slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi) < max_toi) {
+            n_tri = 1;
+            const uint32_t leaf = __float_as_uint(q0.w);
+            if (ray_triangle<0>(tri4[3 * leaf], tri4[3 * leaf + 1], tri4[3 * leaf + 2], r, max_toi, h) &&
+                h.toi < best) {
+                best_leaf = leaf;
+                have = true;
+            }
+        }
+    } else {
+        uint32_t stk = 0, sp = 0, curr = 0, base = 0;
+        WaveWindow w = fetch_window(node4, num_nodes, tri4, 0u, r, max_toi);
+        for (;;) {
+            if (curr - base >= 64u) {
+                base = curr;
+                w = fetch_window(node4, num_nodes, tri4, base, r, max_toi);
+            }
+            const uint32_t j = curr - base;
+            if (COUNT) ++n_wide;
+            float ls = lane_f(w.ls, j), rs = lane_f(w.rs, j);
+            float lt = lane_f(w.lt, j), rt = lane_f(w.rt, j);
+            uint32_t lc = lane_u(w.lc, j), rc = lane_u(w.rc, j);
+            uint32_t ld = lane_u(w.ld, j), rd = lane_u(w.rd, j);
+            if (ls > rs) {
+                float tf = ls;
+                ls = rs;
+                rs = tf;
+                tf = lt;
+                lt = rt;
+                rt = tf;
+                uint32_t tu = lc;
+                lc = rc;
+                rc = tu;
+                tu = ld;
+                ld = rd;
+                rd = tu;
+            }
+            bool found_next = false;
+            if (ls < best && ls != FLT_MAX) {
+                if (node_is_leaf(ld)) {
+                    if (COUNT) ++n_tri;
+                    if (lt < best) {
+                        best = lt;
+                        best_leaf = lc;
+                        have = true;
+                    }
+                } else {
+                    curr = lc;
+                    found_next = true;
+                }
+            }
+            if (rs < best && rs != FLT_MAX) {
+                if (node_is_leaf(rd)) {
+                    if (COUNT) ++n_tri;
+                    if (rt < best) {
+                        best = rt;
+                        best_leaf = rc;
+                        have = true;
+                    }
+                } else if (found_next) {
+                    stk = __lane_id() == sp ? rc : stk;
+                    ++sp;
+                } else {
+                    curr = rc;
+                    found_next = true;
+                }
+            }
+            if (!found_next) {
+                if (sp == 0) break;
+                --sp;
+                curr = lane_u(stk, sp);
+            }
         }
     }
+    TriHit h{};
+    uint32_t tri_id = 0;
+    if (have) {
+        (void)ray_triangle<0>(tri4[3 * best_leaf], tri4[3 * best_leaf + 1], tri4[3 * best_leaf + 2], r, max_toi, h);
+        tri_id = __float_as_uint(tri4[3 * best_leaf + 2].y);
+    }
+    return make_hit(have, h, tri_id, ntri);
+}
+
+// Claims the next `chunk` rays of the queue for the whole wave: [*begin, *end).
+__device__ __forceinline__ bool wave_claim(unsigned long long *queue, uint32_t chunk, uint64_t n, uint64_t &begin,
+                                           uint64_t &end) {
+    unsigned long long b = 0;
+    if (__lane_id() == 0) b = atomicAdd(&queue[0], (unsigned long long)chunk);
+    b = __builtin_amdgcn_readfirstlane((uint32_t)b) | ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(b >> 32)) << 32);
+    begin = b;
+    end = b + chunk < n ? b + chunk : n;
+    return b < n;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(64) void cast_rays_wave_kernel(const pq_bvh_node_wide *__restrict__ nodes,
+                                                            uint32_t num_nodes, const TriRecord *__restrict__ tris,
+                                                            uint32_t ntri, const pq_ray *__restrict__ rays, uint64_t n,
+                                                            float max_toi, pq_ray_hit *__restrict__ hits,
+                                                            uint32_t *__restrict__ counts,
+                                                            unsigned long long *__restrict__ queue, uint32_t chunk) {
+    const float4 *node4 = reinterpret_cast<const float4 *>(nodes);
+    const float4 *tri4 = reinterpret_cast<const float4 *>(tris);
+    const bool single_leaf = (__float_as_uint(node4[3].w) & 0x3fffffffu) == 0u;
+    uint64_t begin, end;
+    while (wave_claim(queue, chunk, n, begin, end)) {
+        for (uint64_t i = begin; i < end; ++i) {
+            const LaneRay r = load_ray(rays, i);
+            uint32_t n_wide, n_tri;
+            const pq_ray_hit h = wave_cast_one<COUNT>(node4, num_nodes, tri4, ntri, single_leaf, r, max_toi, n_wide,
+                                                      n_tri);
+            if (__lane_id() == 0) {
+                hits[i] = h;
+                if (COUNT) {
+                    counts[2 * i] = n_wide;
+                    counts[2 * i + 1] = n_tri;
+                }
+            }
+        }
+    }
+    queue_wave_done(queue, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -393,12 +634,13 @@ __device__ __forceinline__ uint32_t mesh_of(const unsigned long long *offsets, u
     return lo;
 }
 
-template <bool COUNT>
-__device__ __forceinline__ void cast_one(const BatchMesh &bm, const LaneRay &r, float max_toi, uint32_t *stk,
+template <bool COUNT, class Stack>
+__device__ __forceinline__ void cast_one(const BatchMesh &bm, const LaneRay &r, float max_toi, Stack &stk,
                                          Traversal &t, uint32_t &n_wide, uint32_t &n_tri) {
     const float4 *node4 = reinterpret_cast<const float4 *>(bm.nodes);
     const float4 *tri4 = reinterpret_cast<const float4 *>(bm.tris);
     start_traversal(t, max_toi);
+    stk.reset();
     n_wide = n_tri = 0;
     if (bm.single_leaf) {
         const float4 q0 = node4[0], q1 = node4[1];
@@ -407,25 +649,27 @@ __device__ __forceinline__ void cast_one(const BatchMesh &bm, const LaneRay &r, 
             visit_leaf<true>(tri4, __float_as_uint(q0.w), r, t, n_tri);
         return;
     }
-    while (!find_best_step<COUNT>(node4, tri4, r, t, stk, n_wide, n_tri)) {
+    while (!find_best_step<COUNT, 0>(node4, tri4, r, t, stk, n_wide, n_tri)) {
     }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(kRayBlock) void cast_rays_batch_kernel(const BatchMesh *__restrict__ meshes,
-                                                                    const unsigned long long *__restrict__ offsets,
-                                                                    uint32_t nmesh, const pq_ray *__restrict__ rays,
-                                                                    uint64_t n, float max_toi,
-                                                                    pq_ray_hit *__restrict__ hits,
-                                                                    uint32_t *__restrict__ counts) {
+template <bool COUNT, int BLOCK, int REG>
+__global__ __launch_bounds__(BLOCK) void cast_rays_batch_kernel(const BatchMesh *__restrict__ meshes,
+                                                                const unsigned long long *__restrict__ offsets,
+                                                                uint32_t nmesh, const pq_ray *__restrict__ rays,
+                                                                uint64_t n, float max_toi,
+                                                                pq_ray_hit *__restrict__ hits,
+                                                                uint32_t *__restrict__ counts) {
     extern __shared__ uint32_t lds_stack[];
-    const uint64_t i = (uint64_t)blockIdx.x * kRayBlock + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const BatchMesh bm = meshes[mesh_of(offsets, nmesh, i)];
     const LaneRay r = load_ray(rays, i);
+    LaneStack<BLOCK, REG> stk;
+    stk.col = lds_stack + threadIdx.x;
     Traversal t;
     uint32_t n_wide, n_tri;
-    cast_one<COUNT>(bm, r, max_toi, lds_stack + threadIdx.x, t, n_wide, n_tri);
+    cast_one<COUNT>(bm, r, max_toi, stk, t, n_wide, n_tri);
     if (COUNT) {
         counts[2 * i] = n_wide;
         counts[2 * i + 1] = n_tri;
@@ -435,14 +679,15 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_batch_kernel(const BatchM
 
 // Persistent variant: one device queue over the concatenated rays; finished
 // lanes take the next rays every iteration, whatever mesh they belong to.
-template <bool COUNT>
-__global__ __launch_bounds__(kRayBlock) void cast_rays_batch_persistent_kernel(
+template <bool COUNT, int BLOCK, int REG>
+__global__ __launch_bounds__(BLOCK) void cast_rays_batch_persistent_kernel(
     const BatchMesh *__restrict__ meshes, const unsigned long long *__restrict__ offsets, uint32_t nmesh,
     const pq_ray *__restrict__ rays, uint64_t n, float max_toi, pq_ray_hit *__restrict__ hits,
     uint32_t *__restrict__ counts, unsigned long long *__restrict__ queue, uint32_t chunk) {
     extern __shared__ uint32_t lds_stack[];
-    uint32_t *stk = lds_stack + threadIdx.x;
-    const uint32_t lane = __lane_id();
+    LaneStack<BLOCK, REG> stk;
+    stk.col = lds_stack + threadIdx.x;
+    stk.reset();
     uint64_t q_next = 0, q_end = 0;
     bool drained = false, active = false;
     uint64_t ray_id = 0;
@@ -456,7 +701,7 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_batch_persistent_kernel(
         if (idle != 0 && !drained) {
             if (q_next >= q_end) {
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(&queue[0], (unsigned long long)chunk);
+                if (__lane_id() == 0) base = atomicAdd(&queue[0], (unsigned long long)chunk);
                 base = __shfl(base, 0);
                 q_next = base;
                 q_end = base + chunk < n ? base + chunk : n;
@@ -470,6 +715,7 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_batch_persistent_kernel(
                     bm = meshes[mesh_of(offsets, nmesh, ray_id)];
                     r = load_ray(rays, ray_id);
                     start_traversal(t, max_toi);
+                    stk.reset();
                     n_wide = n_tri = 0;
                     active = true;
                     if (bm.single_leaf) {  // one-triangle mesh: a single test
@@ -491,8 +737,9 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_batch_persistent_kernel(
         }
         if (active) {
             const bool done = bm.single_leaf ||
-                              find_best_step<COUNT>(reinterpret_cast<const float4 *>(bm.nodes),
-                                                    reinterpret_cast<const float4 *>(bm.tris), r, t, stk, n_wide, n_tri);
+                              find_best_step<COUNT, 0>(reinterpret_cast<const float4 *>(bm.nodes),
+                                                       reinterpret_cast<const float4 *>(bm.tris), r, t, stk, n_wide,
+                                                       n_tri);
             if (done) {
                 if (COUNT) {
                     counts[2 * ray_id] = n_wide;
@@ -503,126 +750,206 @@ __global__ __launch_bounds__(kRayBlock) void cast_rays_batch_persistent_kernel(
             }
         }
     }
-    if (lane == 0) {
-        __threadfence();
-        const uint32_t waves = gridDim.x * (kRayBlock / 64);
-        const unsigned long long done = atomicAdd(&queue[1], 1ull);
-        if (done == waves - 1) {
-            queue[0] = 0;
-            queue[1] = 0;
+    queue_wave_done(queue, gridDim.x * (BLOCK / 64));
+}
+
+// Wave per ray over the batch: each ray's mesh by the same binary search.
+template <bool COUNT>
+__global__ __launch_bounds__(64) void cast_rays_batch_wave_kernel(const BatchMesh *__restrict__ meshes,
+                                                                  const unsigned long long *__restrict__ offsets,
+                                                                  uint32_t nmesh, const pq_ray *__restrict__ rays,
+                                                                  uint64_t n, float max_toi,
+                                                                  pq_ray_hit *__restrict__ hits,
+                                                                  uint32_t *__restrict__ counts,
+                                                                  unsigned long long *__restrict__ queue,
+                                                                  uint32_t chunk) {
+    uint64_t begin, end;
+    while (wave_claim(queue, chunk, n, begin, end)) {
+        for (uint64_t i = begin; i < end; ++i) {
+            const BatchMesh bm = meshes[mesh_of(offsets, nmesh, i)];
+            const LaneRay r = load_ray(rays, i);
+            uint32_t n_wide, n_tri;
+            const pq_ray_hit h = wave_cast_one<COUNT>(reinterpret_cast<const float4 *>(bm.nodes), bm.num_nodes,
+                                                      reinterpret_cast<const float4 *>(bm.tris), bm.ntri,
+                                                      bm.single_leaf != 0u, r, max_toi, n_wide, n_tri);
+            if (__lane_id() == 0) {
+                hits[i] = h;
+                if (COUNT) {
+                    counts[2 * i] = n_wide;
+                    counts[2 * i + 1] = n_tri;
+                }
+            }
         }
     }
+    queue_wave_done(queue, gridDim.x);
+}
+
+// Calls f(std::integral_constant<int, BLOCK>, std::integral_constant<int, REG>)
+// for a tuning's block size (64, 128, 256) and register stack depth (0, 1, 2, 4).
+template <class F>
+hipError_t with_shape(const pq_ray_tuning &tu, F &&f) {
+    using std::integral_constant;
+    const uint32_t block = tu.block_size ? tu.block_size : 256u;
+    switch (block * 8u + tu.stack_regs) {
+#define PQ_SHAPE_CASE(B, R) \
+    case B * 8u + R: return f(integral_constant<int, B>{}, integral_constant<int, R>{});
+        PQ_SHAPE_CASE(64, 0) PQ_SHAPE_CASE(64, 1) PQ_SHAPE_CASE(64, 2) PQ_SHAPE_CASE(64, 4)
+        PQ_SHAPE_CASE(128, 0) PQ_SHAPE_CASE(128, 1) PQ_SHAPE_CASE(128, 2) PQ_SHAPE_CASE(128, 4)
+        PQ_SHAPE_CASE(256, 0) PQ_SHAPE_CASE(256, 1) PQ_SHAPE_CASE(256, 2) PQ_SHAPE_CASE(256, 4)
+#undef PQ_SHAPE_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// LDS bytes of a workgroup's stacks: depth-1 entries per lane, less the REG in registers.
+size_t stack_lds_bytes(uint32_t depth, int block, int regs) {
+    const uint32_t entries = ray_stack_entries(depth);
+    const uint32_t in_lds = entries > (uint32_t)regs ? entries - (uint32_t)regs : 1u;
+    return (size_t)in_lds * (size_t)block * sizeof(uint32_t);
+}
+
+// Workgroups for a persistent grid: the occupancy limit (or the tuning's cap)
+// per CU, and no more waves than chunks of work.
+hipError_t persistent_blocks(const void *kernel, int block, size_t lds, const pq_ray_tuning &tu, uint64_t work_units,
+                             uint64_t *blocks) {
+    hipError_t e;
+    int dev = 0, cus = 0, per_cu = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds)) != hipSuccess) return e;
+    if (tu.blocks_per_cu > 0 && (int)tu.blocks_per_cu < per_cu) per_cu = (int)tu.blocks_per_cu;
+    if (per_cu < 1) per_cu = 1;
+    const uint64_t waves_per_block = (uint64_t)block / 64u;
+    uint64_t b = (uint64_t)cus * (uint64_t)per_cu;
+    if (b * waves_per_block > work_units) b = (work_units + waves_per_block - 1) / waves_per_block;
+    *blocks = b < 1 ? 1 : b;
+    return hipSuccess;
 }
 
 }  // namespace
 
 uint32_t ray_stack_entries(uint32_t depth) { return depth > 1 ? depth - 1 : 1; }
 
+// The wave mapping's stack is one VGPR (64 entries).
+bool wave_mapping_fits(uint32_t depth) { return ray_stack_entries(depth) <= 64u; }
+
 hipError_t launch_cast_rays_culling(const DeviceMesh &m, const pq_ray *rays, uint64_t n, float max_toi,
                                     uint32_t culling, pq_ray_hit *hits, uint32_t *counts, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const size_t lds = (size_t)ray_stack_entries(m.depth) * kRayBlock * sizeof(uint32_t);
-    const dim3 grid((uint32_t)((n + kRayBlock - 1) / kRayBlock)), block(kRayBlock);
+    constexpr int B = 256;
+    const size_t lds = stack_lds_bytes(m.depth, B, 0);
+    const dim3 grid((uint32_t)((n + B - 1) / B)), block(B);
     if (culling == 1 && counts)
-        hipLaunchKernelGGL((cast_rays_kernel<true, 1>), grid, block, lds, stream, m.nodes, m.tris, m.num_triangles,
-                           rays, n, max_toi, hits, counts);
+        hipLaunchKernelGGL((cast_rays_kernel<true, 1, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
+                           m.num_triangles, rays, n, max_toi, hits, counts);
     else if (culling == 1)
-        hipLaunchKernelGGL((cast_rays_kernel<false, 1>), grid, block, lds, stream, m.nodes, m.tris, m.num_triangles,
-                           rays, n, max_toi, hits, counts);
+        hipLaunchKernelGGL((cast_rays_kernel<false, 1, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
+                           m.num_triangles, rays, n, max_toi, hits, counts);
     else if (counts)
-        hipLaunchKernelGGL((cast_rays_kernel<true, 2>), grid, block, lds, stream, m.nodes, m.tris, m.num_triangles,
-                           rays, n, max_toi, hits, counts);
+        hipLaunchKernelGGL((cast_rays_kernel<true, 2, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
+                           m.num_triangles, rays, n, max_toi, hits, counts);
     else
-        hipLaunchKernelGGL((cast_rays_kernel<false, 2>), grid, block, lds, stream, m.nodes, m.tris, m.num_triangles,
-                           rays, n, max_toi, hits, counts);
+        hipLaunchKernelGGL((cast_rays_kernel<false, 2, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
+                           m.num_triangles, rays, n, max_toi, hits, counts);
     return hipGetLastError();
 }
+
+namespace {
+template <bool COUNT>
+hipError_t launch_single(const DeviceMesh &m, const pq_ray *rays, uint64_t n, float max_toi, pq_ray_hit *hits,
+                         uint32_t *counts, hipStream_t stream) {
+    const pq_ray_tuning &tu = m.tuning;
+    const bool single_leaf = m.num_nodes == 1 && m.num_triangles == 1;
+    uint32_t mapping = tu.mapping;
+    if (mapping == PQ_RAY_MAPPING_WAVE && !wave_mapping_fits(m.depth)) mapping = PQ_RAY_MAPPING_PERSISTENT;
+    if (mapping == PQ_RAY_MAPPING_PERSISTENT && single_leaf) mapping = PQ_RAY_MAPPING_STATIC;
+    if (mapping == PQ_RAY_MAPPING_WAVE) {
+        unsigned long long *queue = nullptr;
+        hipError_t e = stream_queue(stream, &queue);
+        if (e != hipSuccess) return e;
+        const uint32_t chunk = tu.chunk ? tu.chunk : 4u;
+        uint64_t blocks = 0;
+        if ((e = persistent_blocks((const void *)cast_rays_wave_kernel<COUNT>, 64, 0, tu, (n + chunk - 1) / chunk,
+                                   &blocks)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(cast_rays_wave_kernel<COUNT>, dim3((uint32_t)blocks), dim3(64), 0, stream, m.nodes,
+                           m.num_nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, queue, chunk);
+        return hipGetLastError();
+    }
+    return with_shape(tu, [&](auto bc, auto rc) -> hipError_t {
+        constexpr int B = decltype(bc)::value, R = decltype(rc)::value;
+        const size_t lds = stack_lds_bytes(m.depth, B, R);
+        if (mapping == PQ_RAY_MAPPING_PERSISTENT) {
+            unsigned long long *queue = nullptr;
+            hipError_t e = stream_queue(stream, &queue);
+            if (e != hipSuccess) return e;
+            const uint32_t chunk = tu.chunk ? tu.chunk : 128u;
+            uint64_t blocks = 0;
+            if ((e = persistent_blocks((const void *)cast_rays_persistent_kernel<COUNT, B, R>, B, lds, tu,
+                                       (n + chunk - 1) / chunk, &blocks)) != hipSuccess)
+                return e;
+            hipLaunchKernelGGL((cast_rays_persistent_kernel<COUNT, B, R>), dim3((uint32_t)blocks), dim3(B), lds,
+                               stream, m.nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, queue, chunk);
+        } else {
+            hipLaunchKernelGGL((cast_rays_kernel<COUNT, 0, B, R>), dim3((uint32_t)((n + B - 1) / B)), dim3(B), lds,
+                               stream, m.nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts);
+        }
+        return hipGetLastError();
+    });
+}
+
+template <bool COUNT>
+hipError_t launch_batch(const DeviceBatch &b, const pq_ray *rays, float max_toi, pq_ray_hit *hits, uint32_t *counts,
+                        hipStream_t stream) {
+    const uint64_t n = b.num_rays;
+    const pq_ray_tuning &tu = b.tuning;
+    uint32_t mapping = tu.mapping;
+    if (mapping == PQ_RAY_MAPPING_WAVE && !wave_mapping_fits(b.max_depth)) mapping = PQ_RAY_MAPPING_PERSISTENT;
+    if (mapping == PQ_RAY_MAPPING_WAVE) {
+        const uint32_t chunk = tu.chunk ? tu.chunk : 4u;
+        uint64_t blocks = 0;
+        hipError_t e;
+        if ((e = persistent_blocks((const void *)cast_rays_batch_wave_kernel<COUNT>, 64, 0, tu,
+                                   (n + chunk - 1) / chunk, &blocks)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(cast_rays_batch_wave_kernel<COUNT>, dim3((uint32_t)blocks), dim3(64), 0, stream, b.meshes,
+                           b.offsets, b.num_meshes, rays, n, max_toi, hits, counts, b.queue, chunk);
+        return hipGetLastError();
+    }
+    return with_shape(tu, [&](auto bc, auto rc) -> hipError_t {
+        constexpr int B = decltype(bc)::value, R = decltype(rc)::value;
+        const size_t lds = stack_lds_bytes(b.max_depth, B, R);
+        if (mapping == PQ_RAY_MAPPING_PERSISTENT) {
+            const uint32_t chunk = tu.chunk ? tu.chunk : 128u;
+            uint64_t blocks = 0;
+            hipError_t e;
+            if ((e = persistent_blocks((const void *)cast_rays_batch_persistent_kernel<COUNT, B, R>, B, lds, tu,
+                                       (n + chunk - 1) / chunk, &blocks)) != hipSuccess)
+                return e;
+            hipLaunchKernelGGL((cast_rays_batch_persistent_kernel<COUNT, B, R>), dim3((uint32_t)blocks), dim3(B), lds,
+                               stream, b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts, b.queue,
+                               chunk);
+        } else {
+            hipLaunchKernelGGL((cast_rays_batch_kernel<COUNT, B, R>), dim3((uint32_t)((n + B - 1) / B)), dim3(B), lds,
+                               stream, b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts);
+        }
+        return hipGetLastError();
+    });
+}
+}  // namespace
 
 hipError_t launch_cast_rays(const DeviceMesh &m, const pq_ray *rays, uint64_t n, float max_toi, pq_ray_hit *hits,
                             uint32_t *counts, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const size_t lds = (size_t)ray_stack_entries(m.depth) * kRayBlock * sizeof(uint32_t);
-    const bool single_leaf = m.num_nodes == 1 && m.num_triangles == 1;
-    if (m.tuning.mapping == PQ_RAY_MAPPING_PERSISTENT && !single_leaf) {
-        unsigned long long *queue = nullptr;
-        hipError_t e = stream_queue(stream, &queue);
-        if (e != hipSuccess) return e;
-        int dev = 0, cus = 0, per_cu = 0;
-        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-        if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-        if (counts)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cast_rays_persistent_kernel<true>, kRayBlock, lds);
-        else
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cast_rays_persistent_kernel<false>, kRayBlock, lds);
-        if (e != hipSuccess) return e;
-        if (m.tuning.blocks_per_cu > 0 && (int)m.tuning.blocks_per_cu < per_cu) per_cu = (int)m.tuning.blocks_per_cu;
-        if (per_cu < 1) per_cu = 1;
-        const uint32_t chunk = m.tuning.chunk ? m.tuning.chunk : 128;
-        uint64_t want = (n + chunk - 1) / chunk;  // no more waves than chunks
-        uint64_t blocks = (uint64_t)cus * (uint64_t)per_cu;
-        const uint64_t waves_per_block = kRayBlock / 64;
-        if (blocks * waves_per_block > want) blocks = (want + waves_per_block - 1) / waves_per_block;
-        if (blocks < 1) blocks = 1;
-        if (counts)
-            hipLaunchKernelGGL(cast_rays_persistent_kernel<true>, dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream,
-                               m.nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, queue, chunk);
-        else
-            hipLaunchKernelGGL(cast_rays_persistent_kernel<false>, dim3((uint32_t)blocks), dim3(kRayBlock), lds,
-                               stream, m.nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, queue, chunk);
-        return hipGetLastError();
-    }
-    const uint64_t blocks = (n + kRayBlock - 1) / kRayBlock;
-    if (counts)
-        hipLaunchKernelGGL(cast_rays_kernel<true>, dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream, m.nodes,
-                           m.tris, m.num_triangles, rays, n, max_toi, hits, counts);
-    else
-        hipLaunchKernelGGL(cast_rays_kernel<false>, dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream, m.nodes,
-                           m.tris, m.num_triangles, rays, n, max_toi, hits, counts);
-    return hipGetLastError();
+    return counts ? launch_single<true>(m, rays, n, max_toi, hits, counts, stream)
+                  : launch_single<false>(m, rays, n, max_toi, hits, counts, stream);
 }
 
 hipError_t launch_cast_rays_batch(const DeviceBatch &b, const pq_ray *rays, float max_toi, pq_ray_hit *hits,
                                   uint32_t *counts, hipStream_t stream) {
-    const uint64_t n = b.num_rays;
-    if (n == 0) return hipSuccess;
-    const size_t lds = (size_t)ray_stack_entries(b.max_depth) * kRayBlock * sizeof(uint32_t);
-    if (b.tuning.mapping == PQ_RAY_MAPPING_PERSISTENT) {
-        hipError_t e;
-        int dev = 0, cus = 0, per_cu = 0;
-        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-        if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-        if (counts)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cast_rays_batch_persistent_kernel<true>,
-                                                             kRayBlock, lds);
-        else
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cast_rays_batch_persistent_kernel<false>,
-                                                             kRayBlock, lds);
-        if (e != hipSuccess) return e;
-        if (b.tuning.blocks_per_cu > 0 && (int)b.tuning.blocks_per_cu < per_cu) per_cu = (int)b.tuning.blocks_per_cu;
-        if (per_cu < 1) per_cu = 1;
-        const uint32_t chunk = b.tuning.chunk ? b.tuning.chunk : 128;
-        const uint64_t want = (n + chunk - 1) / chunk;
-        uint64_t blocks = (uint64_t)cus * (uint64_t)per_cu;
-        if (blocks * (kRayBlock / 64) > want) blocks = (want + (kRayBlock / 64) - 1) / (kRayBlock / 64);
-        if (blocks < 1) blocks = 1;
-        if (counts)
-            hipLaunchKernelGGL(cast_rays_batch_persistent_kernel<true>, dim3((uint32_t)blocks), dim3(kRayBlock), lds,
-                               stream, b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts, b.queue,
-                               chunk);
-        else
-            hipLaunchKernelGGL(cast_rays_batch_persistent_kernel<false>, dim3((uint32_t)blocks), dim3(kRayBlock), lds,
-                               stream, b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts, b.queue,
-                               chunk);
-        return hipGetLastError();
-    }
-    const uint64_t blocks = (n + kRayBlock - 1) / kRayBlock;
-    if (counts)
-        hipLaunchKernelGGL(cast_rays_batch_kernel<true>, dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream,
-                           b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts);
-    else
-        hipLaunchKernelGGL(cast_rays_batch_kernel<false>, dim3((uint32_t)blocks), dim3(kRayBlock), lds, stream,
-                           b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts);
-    return hipGetLastError();
+    if (b.num_rays == 0) return hipSuccess;
+    return counts ? launch_batch<true>(b, rays, max_toi, hits, counts, stream)
+                  : launch_batch<false>(b, rays, max_toi, hits, counts, stream);
 }
 
 }  // namespace pq

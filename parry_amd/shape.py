@@ -94,18 +94,53 @@ class TriMesh:
         _native.check(_native.lib().pq_trimesh_export_bvh(self._h, _ptr(out), out.shape[0]))
         return out
 
+    def to_bytes(self) -> bytes:
+        """parry's serialized ``TriMesh`` (serde as bincode 1.x writes it, trimesh.rs:510-528)."""
+        from .wire import trimesh_to_bytes
+
+        return trimesh_to_bytes(self._vertices, self._indices, self.bvh_nodes())
+
+    @classmethod
+    def from_bytes(cls, data: bytes) -> "TriMesh":
+        """The mesh of a serialized parry ``TriMesh``, keeping its Bvh as serialized."""
+        from .wire import trimesh_from_bytes
+
+        v, i, nodes = trimesh_from_bytes(data)
+        return cls(v, i, bvh_nodes=nodes)
+
     # -- tuning ----------------------------------------------------------
-    def set_tuning(self, mapping: str = "persistent", chunk: int = 128, blocks_per_cu: int = 0) -> None:
-        """Selects the ray kernel's lane mapping ("static" or "persistent"); results never change."""
-        m = {"static": _native.PQ_RAY_MAPPING_STATIC, "persistent": _native.PQ_RAY_MAPPING_PERSISTENT}[mapping]
-        t = _native.RayTuning(m, chunk, blocks_per_cu, 0)
+    def set_tuning(self, mapping: str = "persistent", chunk: int = 0, blocks_per_cu: int = 0, block_size: int = 256,
+                   stack_regs: int = 0) -> None:
+        """Selects the ray kernel's mapping ("static", "persistent" or "wave"), workgroup size and the
+        stack entries kept in registers; results never change."""
+        t = _native.ray_tuning(mapping, chunk, blocks_per_cu, block_size, stack_regs)
         _native.check(_native.lib().pq_trimesh_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
         t = _native.RayTuning()
         _native.check(_native.lib().pq_trimesh_get_tuning(self._h, ctypes.byref(t)))
-        return {"mapping": "persistent" if t.mapping == _native.PQ_RAY_MAPPING_PERSISTENT else "static",
-                "chunk": t.chunk, "blocks_per_cu": t.blocks_per_cu}
+        return _native.ray_tuning_dict(t)
+
+    def autotune(self, rays, max_time_of_impact: float, out=None, stream=None):
+        """Times the kernel mappings on these rays and keeps the fastest; returns (tuning, ms per launch)."""
+        import torch
+
+        n = rays.shape[0]
+        if not (rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous() and rays.shape[1:] == (6,)):
+            raise ValueError("rays must be a contiguous CUDA float32 tensor of shape (n, 6)")
+        if n == 0:
+            raise ValueError("autotune needs at least one ray")
+        if out is None:
+            out = torch.empty((n, 5), dtype=torch.float32, device=rays.device)
+        elif not (out.is_cuda and out.dtype == torch.float32 and out.is_contiguous() and out.numel() >= 5 * n):
+            raise ValueError("out must be a contiguous CUDA float32 tensor with room for (n, 5)")
+        s = stream if stream is not None else torch.cuda.current_stream(rays.device)
+        t = _native.RayTuning()
+        ms = ctypes.c_float()
+        _native.check(_native.lib().pq_trimesh_autotune(
+            self._h, rays.data_ptr(), n, float(max_time_of_impact), out.data_ptr(), ctypes.c_void_p(s.cuda_stream),
+            ctypes.byref(t), ctypes.byref(ms)))
+        return self.tuning(), float(ms.value)
 
     # -- RayCast ---------------------------------------------------------
     def cast_local_rays_and_get_normal(self, rays, max_time_of_impact: float, solid: bool, out=None, stream=None):
@@ -289,16 +324,15 @@ class TriMeshBatch:
     def num_rays(self) -> int:
         return int(self.offsets[-1])
 
-    def set_tuning(self, mapping: str = "persistent", chunk: int = 128, blocks_per_cu: int = 0) -> None:
-        m = {"static": _native.PQ_RAY_MAPPING_STATIC, "persistent": _native.PQ_RAY_MAPPING_PERSISTENT}[mapping]
-        t = _native.RayTuning(m, chunk, blocks_per_cu, 0)
+    def set_tuning(self, mapping: str = "persistent", chunk: int = 0, blocks_per_cu: int = 0, block_size: int = 256,
+                   stack_regs: int = 0) -> None:
+        t = _native.ray_tuning(mapping, chunk, blocks_per_cu, block_size, stack_regs)
         _native.check(_native.lib().pq_trimesh_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
         t = _native.RayTuning()
         _native.check(_native.lib().pq_trimesh_batch_get_tuning(self._h, ctypes.byref(t)))
-        return {"mapping": "persistent" if t.mapping == _native.PQ_RAY_MAPPING_PERSISTENT else "static",
-                "chunk": t.chunk, "blocks_per_cu": t.blocks_per_cu}
+        return _native.ray_tuning_dict(t)
 
     def _check_rays(self, rays, out):
         import torch
@@ -338,7 +372,7 @@ class TriMeshBatch:
         return out, counts
 
     def autotune(self, rays, max_time_of_impact: float, out=None, stream=None):
-        """Times the lane mappings on these rays and keeps the fastest; returns (tuning, ms per launch)."""
+        """Times the kernel mappings on these rays and keeps the fastest; returns (tuning, ms per launch)."""
         out = self._check_rays(rays, out)
         t = _native.RayTuning()
         ms = ctypes.c_float()

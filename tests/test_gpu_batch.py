@@ -45,7 +45,16 @@ SMALL = [  # (level, rays, mesh seed, ray seed); -1 = single triangle, 0 rays = 
 ]
 
 
-def _check_batch(orc, meshes, rays, mappings=("static", "persistent"), chunks=(64, 128, 512)):
+# Kernel mappings of the batched launch (pq_ray_tuning): results must not depend on them.
+BATCH_MAPPINGS = (
+    [{"mapping": "static", "block_size": b, "stack_regs": r} for b, r in ((256, 0), (64, 1), (128, 2), (256, 4))]
+    + [{"mapping": "persistent", "chunk": c} for c in (64, 128, 512)]
+    + [{"mapping": "persistent", "chunk": 128, "block_size": 64, "stack_regs": 2},
+       {"mapping": "wave", "chunk": 1}, {"mapping": "wave", "chunk": 4}]
+)
+
+
+def _check_batch(orc, meshes, rays, mappings=("static", "persistent", "wave")):
     torch = _torch()
     batch = TriMeshBatch([m for _, _, m in meshes], [r.shape[0] for r in rays])
     all_rays = np.concatenate(rays) if rays else np.zeros((0, 6), np.float32)
@@ -58,12 +67,12 @@ def _check_batch(orc, meshes, rays, mappings=("static", "persistent"), chunks=(6
                       np.zeros(0, RAY_HIT_DTYPE))
         ref.append(orc.OracleTriMesh(v, f).cast_rays(r, MAX_TOI) if r.shape[0] else None)
     single = np.concatenate(single)
-    configs = [("static", 128)] + [("persistent", c) for c in chunks if "persistent" in mappings]
-    for mapping, chunk in configs:
-        print(f"case: batch of {len(meshes)} meshes, {all_rays.shape[0]} rays, {mapping}/{chunk}", flush=True)
-        batch.set_tuning(mapping, chunk)
+    for tu in [t for t in BATCH_MAPPINGS if t["mapping"] in mappings]:
+        label = "/".join(f"{k}={v}" for k, v in tu.items())
+        print(f"case: batch of {len(meshes)} meshes, {all_rays.shape[0]} rays, {label}", flush=True)
+        batch.set_tuning(**tu)
         hits = _as_records(batch.cast_local_rays_and_get_normal(d_rays, MAX_TOI, True))
-        assert hits.tobytes() == single.tobytes(), f"{mapping}/{chunk}: batch differs from per-mesh casts"
+        assert hits.tobytes() == single.tobytes(), f"{label}: batch differs from per-mesh casts"
         for i, rr in enumerate(ref):
             if rr is None:
                 continue
@@ -86,8 +95,9 @@ def test_counted_batch_matches_single_mesh_counts():
     meshes, rays = _build(SMALL)
     batch = TriMeshBatch([m for _, _, m in meshes], [r.shape[0] for r in rays])
     d_rays = torch.from_numpy(np.concatenate(rays)).cuda()
-    for mapping in ("static", "persistent"):
-        batch.set_tuning(mapping)
+    for tu in BATCH_MAPPINGS:
+        batch.set_tuning(**tu)
+        mapping = "/".join(f"{k}={v}" for k, v in tu.items())
         hits, counts = batch.cast_local_rays_counted(d_rays, MAX_TOI)
         counts = counts.cpu().numpy()
         for i, ((_, _, mesh), r) in enumerate(zip(meshes, rays)):
@@ -102,11 +112,11 @@ def test_counted_batch_matches_single_mesh_counts():
 def test_autotune_and_graph_replays(orc):
     torch = _torch()
     meshes, rays = _build(SMALL)
-    batch, d_rays, single = _check_batch(orc, meshes, rays, mappings=("static",), chunks=())
+    batch, d_rays, single = _check_batch(orc, meshes, rays, mappings=("static",))
     out = torch.empty((batch.num_rays, 5), dtype=torch.float32, device="cuda")
     tuning, ms = batch.autotune(d_rays, MAX_TOI, out=out)
     print(f"autotune chose {tuning} at {ms:.4f} ms", flush=True)
-    assert tuning["mapping"] in ("static", "persistent") and ms > 0
+    assert tuning["mapping"] in ("static", "persistent", "wave") and ms > 0
     assert _as_records(out).tobytes() == single.tobytes()
     out.zero_()
     graph = batch.capture(d_rays, MAX_TOI, True, out)
@@ -141,4 +151,4 @@ def test_uneven_64_mesh_workload(orc):
     """The benchmark workload: 64 meshes at levels 3-7 with 2^10-2^20 rays each."""
     spec = scenes.uneven_mesh_batch(64, seed=2)
     meshes, rays = _build(spec)
-    _check_batch(orc, meshes, rays, chunks=(128,))
+    _check_batch(orc, meshes, rays)

@@ -19,11 +19,32 @@ TOL = 1e-5  # relative tolerance for toi / normal (f32)
 NO_HIT = 0xFFFFFFFF
 
 
-MAPPINGS = ["static", "persistent"]
+# Kernel mappings (pq_ray_tuning): the results must not depend on them.
+MAPPINGS = [
+    {"mapping": "static"},
+    {"mapping": "persistent"},
+    {"mapping": "wave", "chunk": 4},
+    {"mapping": "static", "block_size": 64, "stack_regs": 1},
+    {"mapping": "static", "block_size": 128, "stack_regs": 2},
+    {"mapping": "persistent", "block_size": 64, "stack_regs": 4, "chunk": 64},
+]
+# Every setting the tuner can pick, for the traversal-count test.
+ALL_MAPPINGS = (
+    [{"mapping": "static", "block_size": b, "stack_regs": r} for b in (64, 128, 256) for r in (0, 1, 2, 4)]
+    + [{"mapping": "persistent", "block_size": b, "stack_regs": r, "chunk": c}
+       for b in (64, 256) for r in (0, 2) for c in (64, 256)]
+    + [{"mapping": "persistent", "blocks_per_cu": 4}, {"mapping": "wave", "chunk": 1}, {"mapping": "wave", "chunk": 4}]
+)
 
 
-def _cast_gpu_host(mesh, rays, max_toi, mapping="persistent"):
-    mesh.set_tuning(mapping)
+def _label(m):
+    return "/".join(f"{k}={v}" for k, v in m.items())
+
+
+def _cast_gpu_host(mesh, rays, max_toi, mapping=None):
+    if isinstance(mapping, str):
+        mapping = {"mapping": mapping}
+    mesh.set_tuning(**(mapping or {"mapping": "persistent"}))
     return mesh.cast_local_rays_and_get_normal(rays, max_toi, True)
 
 
@@ -50,10 +71,10 @@ def test_c0_full_size_parity_and_f64_error(orc):
     assert mesh.bvh_nodes().tobytes() == ref_mesh.bvh_nodes().tobytes()
     ref = ref_mesh.cast_rays(rays, 10.0)
     for mapping in MAPPINGS:
-        print(f"case: C0 icosphere s=6 seed=1, rays seed=2 n=1048576, mapping={mapping}", flush=True)
+        print(f"case: C0 icosphere s=6 seed=1, rays seed=2 n=1048576, {_label(mapping)}", flush=True)
         hits = _cast_gpu_host(mesh, rays, 10.0, mapping)
-        ntoi, nnrm = _assert_parity(hits, ref, f"C0/{mapping}")
-        print(f"C0/{mapping}: toi bit-differences {ntoi}, normal bit-differences {nnrm}")
+        ntoi, nnrm = _assert_parity(hits, ref, f"C0/{_label(mapping)}")
+        print(f"C0/{_label(mapping)}: toi bit-differences {ntoi}, normal bit-differences {nnrm}")
         assert ntoi == 0 and nnrm == 0
     # error against parry3d-f64 (stated, not asserted beyond sanity)
     r64 = orc.OracleTriMesh(v.astype(np.float64), f, f64=True).cast_rays(rays.astype(np.float64), 10.0)
@@ -68,7 +89,7 @@ def test_c0_full_size_parity_and_f64_error(orc):
     assert same.mean() > 0.9999 and np.quantile(rel, 0.9999) < 1e-5 and rel.max() < 1e-3
 
 
-@pytest.mark.parametrize("mapping", MAPPINGS)
+@pytest.mark.parametrize("mapping", MAPPINGS, ids=_label)
 @pytest.mark.parametrize("level", [0, 1, 3, 5, 7])
 def test_icosphere_levels(orc, level, mapping):
     v, f = scenes.noisy_icosphere(level, seed=100 + level)
@@ -125,8 +146,9 @@ def test_edge_cases(orc):
         assert (hits["feature"] == NO_HIT).all()  # max_toi = 0 never hits (strict <)
     # batch sizes around the persistent chunk and wave sizes
     for n_small in (1, 63, 64, 65, 127, 129, 1000):
-        hits = _cast_gpu_host(mesh, rays[:n_small], 10.0, "persistent")
-        _assert_parity(hits, om.cast_rays(rays[:n_small], 10.0), f"n={n_small}")
+        for mapping in ("persistent", "wave"):
+            hits = _cast_gpu_host(mesh, rays[:n_small], 10.0, mapping)
+            _assert_parity(hits, om.cast_rays(rays[:n_small], 10.0), f"n={n_small} {mapping}")
 
 
 def test_small_trees_and_duplicates(orc):
@@ -157,9 +179,9 @@ def test_traversal_counts_match_oracle(orc):
     rays = scenes.rays_into_unit_ball(100000, seed=22)
     ref = orc.OracleTriMesh(v, f).cast_rays(rays, 10.0, per_ray_counts=True)
     mesh = TriMesh(v, f)
-    for mapping in MAPPINGS:
-        print(f"case: traversal counts s=5 {mapping}", flush=True)
-        mesh.set_tuning(mapping)
+    for mapping in ALL_MAPPINGS:
+        print(f"case: traversal counts s=5 {_label(mapping)}", flush=True)
+        mesh.set_tuning(**mapping)
         hits, counts = mesh.cast_local_rays_counted(torch.from_numpy(rays).cuda(), 10.0)
         torch.cuda.synchronize()
         c = counts.cpu().numpy()
@@ -186,3 +208,27 @@ def test_empty_batch_and_device_path(orc):
     assert np.array_equal(toi.cpu().numpy(), ref["toi"])
     hit = mesh.cast_local_ray_and_get_normal(__import__("parry_amd").Ray((0.0, 0.0, 3.0), (0.0, 0.0, -1.0)), 10.0, True)
     assert hit is not None and hit.feature.kind == "Face" and 1.9 < hit.time_of_impact < 2.1
+
+
+def test_single_mesh_autotune(orc):
+    """pq_trimesh_autotune times every mapping on the given rays, keeps the
+    fastest and leaves the hits of the rays in `out`; a few rays on a large mesh
+    and many rays on a small one."""
+    import torch
+
+    for level, n, seed in ((7, 2048, 41), (3, 200000, 42)):
+        v, f = scenes.noisy_icosphere(level, seed=seed)
+        rays = scenes.rays_into_unit_ball(n, seed=seed + 1)
+        mesh = TriMesh(v, f)
+        d = torch.from_numpy(rays).cuda()
+        out = torch.empty((n, 5), dtype=torch.float32, device="cuda")
+        tuning, ms = mesh.autotune(d, 10.0, out=out)
+        torch.cuda.synchronize()
+        print(f"case: autotune s={level} n={n}: chose {_label(tuning)} at {ms:.4f} ms", flush=True)
+        assert tuning["mapping"] in ("static", "persistent", "wave") and ms > 0
+        assert mesh.tuning() == tuning
+        ref = orc.OracleTriMesh(v, f).cast_rays(rays, 10.0)
+        o = out.cpu().numpy()
+        assert np.array_equal(o.view(np.uint32)[:, 4], ref["feature"])
+        hits = _cast_gpu_host(mesh, rays, 10.0, tuning)
+        _assert_parity(hits, ref, f"autotuned s={level}")
