@@ -252,6 +252,28 @@ int pq_bvh_create(const float *aabbs, uint32_t n, pq_bvh **out) {
     return bvh_handle_from_nodes(nodes, n, depth, out);
 }
 
+int pq_bvh_create_from_nodes(const pq_bvh_node_wide *nodes, uint32_t num_nodes, uint32_t num_leaves, pq_bvh **out) {
+    if (!out) return fail(PQ_ERR_INVALID, "pq_bvh_create_from_nodes: null output");
+    *out = nullptr;
+    if (num_nodes && !nodes) return fail(PQ_ERR_INVALID, "pq_bvh_create_from_nodes: null nodes");
+    if (num_leaves == 0 && num_nodes == 0) return bvh_handle_from_nodes({}, 0, 0, out);
+    bool ok = false;
+    const uint32_t depth = bvh_depth(nodes, num_nodes, num_leaves, &ok);
+    // every leaf 0..num_leaves-1 exactly once
+    std::vector<uint8_t> seen(num_leaves, 0);
+    uint64_t leaves = 0;
+    for (uint32_t w = 0; ok && w < num_nodes; ++w)
+        for (const pq_bvh_node *c : {&nodes[w].left, &nodes[w].right})
+            if (is_leaf(*c)) {
+                ok = c->children < num_leaves && !seen[c->children];
+                if (ok) seen[c->children] = 1;
+                ++leaves;
+            }
+    if (!ok || num_leaves == 0 || leaves != num_leaves)
+        return fail(PQ_ERR_INVALID, "pq_bvh_create_from_nodes: not a well-formed depth-first tree over the leaves");
+    return bvh_handle_from_nodes(std::vector<pq_bvh_node_wide>(nodes, nodes + num_nodes), num_leaves, depth, out);
+}
+
 size_t pq_bvh_ploc_workspace_bytes(uint32_t n) { return ploc_workspace_bytes(n); }
 
 int pq_bvh_from_leaves_ploc(const float *aabbs, uint32_t n, pq_bvh_node_wide *out, uint32_t capacity,

@@ -56,6 +56,35 @@ class Bvh:
         self._count = None
         return self
 
+    @classmethod
+    def from_nodes(cls, nodes, num_leaves: int) -> "Bvh":
+        """A broadphase handle over existing depth-first wide records (pq_bvh_create_from_nodes)."""
+        n = np.ascontiguousarray(nodes, dtype=BVH_NODE_WIDE_DTYPE).reshape(-1)
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().pq_bvh_create_from_nodes(n.ctypes.data if n.size else None, n.shape[0],
+                                                             int(num_leaves), ctypes.byref(h)))
+        self._h = h
+        nl, nn, d = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _native.check(_native.lib().pq_bvh_info(h, ctypes.byref(nl), ctypes.byref(nn), ctypes.byref(d)))
+        self.num_leaves, self.num_nodes, self.depth = nl.value, nn.value, d.value
+        self._count = None
+        return self
+
+    def to_bytes(self) -> bytes:
+        """parry's serialized ``Bvh`` (serde as bincode 1.x writes it, bvh_tree.rs:1748-1762) of the current tree."""
+        from .wire import bvh_to_bytes
+
+        return bvh_to_bytes(self.nodes())
+
+    @classmethod
+    def from_bytes(cls, data: bytes) -> "Bvh":
+        from .wire import bvh_from_bytes
+
+        nodes = bvh_from_bytes(data)
+        leaves = sum(int(((nodes[s + "_data"] & 0x3FFFFFFF) == 1).sum()) for s in ("left", "right"))
+        return cls.from_nodes(nodes, leaves)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
