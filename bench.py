@@ -39,12 +39,13 @@ MAX_TOI = 10.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
 
 
-def measured_traffic(kernel: str):
+def measured_traffic(kernel: str, field: str = "hbm_bytes_per_launch"):
     """HBM bytes per launch of `kernel` from the committed counter summary
-    (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's readings), or None."""
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's readings), or
+    another field of its entry; None if absent."""
     try:
         with open(TRAFFIC_FILE) as fh:
-            return json.load(fh)[kernel]["hbm_bytes_per_launch"]
+            return json.load(fh)[kernel][field]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -238,7 +239,12 @@ def batch_leg(args, log):
         batch.set_tuning(mapping, chunk)
         modes[f"{mapping}/{chunk}"] = timed(
             lambda: batch.cast_local_rays_and_get_normal(d_rays, MAX_TOI, True, out=out, stream=stream), args.steps)
-    tuning, tune_ms = batch.autotune(d_rays, MAX_TOI, out=out)
+    if args.batch_tuning:
+        m, c, bpc, blk, regs = args.batch_tuning.split(":")
+        batch.set_tuning(m, int(c), int(bpc), int(blk), int(regs))
+        tuning, tune_ms = batch.tuning(), float("nan")
+    else:
+        tuning, tune_ms = batch.autotune(d_rays, MAX_TOI, out=out)
     graph = batch.capture(d_rays, MAX_TOI, True, out)
     wall_ms, gpu_ms = timed(lambda: graph.launch(stream), args.steps)
     identical = bool(torch.equal(out.view(torch.int32), ref.view(torch.int32)))
@@ -265,6 +271,8 @@ def batch_leg(args, log):
             "unit": "GB/s",
             "frac": round(alg_bytes / (gpu_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_launch": alg_bytes,
+            "traffic": measured_traffic("cast_rays_batch_persistent_kernel"),
+            "traffic_tuning": measured_traffic("cast_rays_batch_persistent_kernel", "tuning"),
         },
         "parity": {"identical_to_separate_launches": identical},
     }
@@ -299,6 +307,56 @@ def batch_leg(args, log):
             "normal_abs_err_max": max(e["normal"].get("abs_err_max", 0.0) for e in errs),
             "sample": "the first min(4096, n_i) rays of each of the 64 meshes"}
     return rec
+
+
+def mapping_sweep(args, log):
+    """Every ray-kernel mapping the tuner considers (lane mappings over workgroup
+    sizes and register stack depths, persistent chunk sizes, wave per ray), timed
+    on the C0 mesh and rays and on three small-batch workloads, plus the
+    autotuner's pick for each; every setting's hits are checked against the
+    default mapping's, bit for bit."""
+    import numpy as np
+    import torch
+
+    from parry_amd import TriMesh, scenes
+    from parry_amd import _native
+
+    cands = ([{"mapping": "static", "block_size": b, "stack_regs": r} for b in (64, 128, 256) for r in (0, 1, 2, 4)]
+             + [{"mapping": "persistent", "block_size": b, "stack_regs": r, "chunk": c}
+                for b in (64, 256) for r in (0, 2) for c in (64, 128, 256)]
+             + [{"mapping": "wave", "chunk": c} for c in (1, 4)])
+    out_rec = []
+    for level, n, label in ((6, 1 << 20, "C0 mesh, 1048576 rays"), (7, 65536, "icosphere s=7, 65536 rays"),
+                            (7, 4096, "icosphere s=7, 4096 rays"), (5, 1024, "icosphere s=5, 1024 rays")):
+        v, f = scenes.noisy_icosphere(level, seed=1)
+        rays = scenes.rays_into_unit_ball(n, seed=2)
+        mesh = TriMesh(v, f)
+        d = torch.from_numpy(rays).cuda()
+        out = torch.empty((n, 5), dtype=torch.float32, device="cuda")
+        mesh.set_tuning("static")
+        mesh.cast_local_rays_and_get_normal(d, MAX_TOI, True, out=out)
+        torch.cuda.synchronize()
+        ref = out.clone()
+        rows = []
+        for c in cands:
+            mesh.set_tuning(**c)
+            for _ in range(2):
+                mesh.cast_local_rays_and_get_normal(d, MAX_TOI, True, out=out)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            steps = 5 if c["mapping"] == "wave" else 10
+            e0.record()
+            for _ in range(steps):
+                mesh.cast_local_rays_and_get_normal(d, MAX_TOI, True, out=out)
+            e1.record()
+            torch.cuda.synchronize()
+            same = bool(torch.equal(out.view(torch.int32), ref.view(torch.int32)))
+            rows.append({**c, "ms": round(e0.elapsed_time(e1) / steps, 4), "identical": same})
+        tu, ms = mesh.autotune(d, MAX_TOI, out=out)
+        log(f"mapping sweep {label}: autotune chose {tu} at {ms:.4f} ms")
+        out_rec.append({"workload": label, "triangles": int(f.shape[0]), "rays": n, "settings": rows,
+                        "autotuned": tu, "autotuned_ms": round(ms, 4)})
+    return out_rec
 
 
 def queries_leg(args, log):
@@ -1132,6 +1190,10 @@ def main() -> int:
     ap.add_argument("--dynamic-only", action="store_true", help="profiling aid: run only the dynamic leg")
     ap.add_argument("--learned-only", action="store_true", help="profiling aid: run only the learned leg")
     ap.add_argument("--batch-only", action="store_true", help="profiling aid: run only the batched multi-mesh leg")
+    ap.add_argument("--batch-tuning", default="", metavar="MAPPING:CHUNK:BLOCKS_PER_CU:BLOCK:REGS",
+                    help="profiling aid: use this ray tuning for the batched leg instead of the autotuner's")
+    ap.add_argument("--mapping-sweep", action="store_true",
+                    help="profiling aid: time every ray-kernel mapping on four workloads and print them")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -1167,6 +1229,9 @@ def main() -> int:
         return 0
     if args.batch_only:
         print(json.dumps({"batch": batch_leg(args, log)}), flush=True)
+        return 0
+    if args.mapping_sweep:
+        print(json.dumps({"mapping_sweep": mapping_sweep(args, log)}), flush=True)
         return 0
     if args.queries_only:
         print(json.dumps({"queries": queries_leg(args, log)}), flush=True)
@@ -1215,7 +1280,12 @@ def main() -> int:
 
     _, _, feat = split_hits(out)
     gpu_feat = feat.cpu().numpy().view(np.uint32)
+    host_hits = out.cpu().numpy()
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    # The tuner's pick over every mapping (untimed for the line; the line keeps
+    # the default static mapping, 256-lane workgroups, stack in LDS).
+    tuned, tuned_ms = mesh.autotune(rays_d, MAX_TOI)
+    mesh.set_tuning("static")
     rec = {
         "metric": METRIC,
         "value": round(n * args.steps / wall / 1e6, 3),
@@ -1236,6 +1306,8 @@ def main() -> int:
             "mesh_seed": 1,
             "ray_seed": 2,
             "parallelism": "single GPU",
+            "mapping": "static, 256-lane workgroups, traversal stack in LDS",
+            "autotuner": {"chosen": tuned, "ms_per_launch": round(tuned_ms, 5)},
         },
         "roofline": {
             "bound": "hbm",
@@ -1259,7 +1331,7 @@ def main() -> int:
         rec["parity"] = parity
         from oracle import oracle
 
-        rec["f64_error"] = ray_f64_error(v, f, rays, out.cpu().numpy(), threads=oracle.default_threads())
+        rec["f64_error"] = ray_f64_error(v, f, rays, host_hits, threads=oracle.default_threads())
     else:
         rec["cpu_baseline"] = None
     if not args.skip_batch:
