@@ -396,12 +396,22 @@ template <int G> __device__ __forceinline__ f3 group_support(const ShapeRef &s, 
     const uint32_t sub = __lane_id() % G;
     float best = -INFINITY;
     uint32_t bi = 0xffffffffu;
-    for (uint32_t k = sub; k < s.n; k += G) {
-        const float4 p = s.v[k];
-        const float dd = dot3(f3{p.x, p.y, p.z}, d);
-        if (dd == dd && (bi == 0xffffffffu || dd > best)) {
-            best = dd;
-            bi = k;
+    // Each lane scans vertices sub, sub + G, ... in order, four loads in flight.
+    for (uint32_t k = sub; k < s.n; k += 4 * G) {
+        float4 p[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t kk = k + (uint32_t)j * G;
+            p[j] = kk < s.n ? s.v[kk] : p0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t kk = k + (uint32_t)j * G;
+            const float dd = dot3(f3{p[j].x, p[j].y, p[j].z}, d);
+            if (kk < s.n && dd == dd && (bi == 0xffffffffu || dd > best)) {
+                best = dd;
+                bi = kk;
+            }
         }
     }
     for (int off = G / 2; off >= 1; off >>= 1) {

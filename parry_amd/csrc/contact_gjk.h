@@ -45,20 +45,34 @@ __device__ __forceinline__ ShapeRef load_shape(const ShapeRecord *rec, const flo
     return ShapeRef{r.type, r.num_vertices, verts + r.first_vertex, r.param[0], r.param[1], r.param[2]};
 }
 
+// One step of the polyhedron support scan: vertex p replaces the best only if
+// strictly better (the first strict maximum wins, as in parry).
+__device__ __forceinline__ void support_step(const float4 p, f3 d, f3 &best, float &best_dot) {
+    const f3 c = f3{p.x, p.y, p.z};
+    const float dd = dot3(c, d);
+    const bool gt = dd > best_dot;
+    best_dot = gt ? dd : best_dot;
+    best.x = gt ? c.x : best.x;
+    best.y = gt ? c.y : best.y;
+    best.z = gt ? c.z : best.z;
+}
+
 __device__ __forceinline__ f3 local_support(const ShapeRef &s, f3 d) {
     if (s.type == PQ_SHAPE_CONVEX_POLYHEDRON) {
         const float4 p0 = s.v[0];
         f3 best = f3{p0.x, p0.y, p0.z};
         float best_dot = dot3(best, d);
-        for (uint32_t k = 1; k < s.n; ++k) {
-            const float4 p = s.v[k];
-            const f3 c = f3{p.x, p.y, p.z};
-            const float dd = dot3(c, d);
-            if (dd > best_dot) {
-                best_dot = dd;
-                best = c;
-            }
+        // Four vertex loads in flight per step, compared in index order: the
+        // scan's result is the sequential one, its latency a quarter of it.
+        uint32_t k = 1;
+        for (; k + 4 <= s.n; k += 4) {
+            const float4 a = s.v[k], b = s.v[k + 1], c = s.v[k + 2], e = s.v[k + 3];
+            support_step(a, d, best, best_dot);
+            support_step(b, d, best, best_dot);
+            support_step(c, d, best, best_dot);
+            support_step(e, d, best, best_dot);
         }
+        for (; k < s.n; ++k) support_step(s.v[k], d, best, best_dot);
         return best;
     }
     if (s.type == PQ_SHAPE_BALL) {
