@@ -396,6 +396,7 @@ template <int G> __device__ __forceinline__ f3 group_support(const ShapeRef &s, 
     const uint32_t sub = __lane_id() % G;
     float best = -INFINITY;
     uint32_t bi = 0xffffffffu;
+    f3 bp = f3{p0.x, p0.y, p0.z};  // the coordinates of vertex bi travel with it
     // Each lane scans vertices sub, sub + G, ... in order, four loads in flight.
     for (uint32_t k = sub; k < s.n; k += 4 * G) {
         float4 p[4];
@@ -411,19 +412,23 @@ template <int G> __device__ __forceinline__ f3 group_support(const ShapeRef &s, 
             if (kk < s.n && dd == dd && (bi == 0xffffffffu || dd > best)) {
                 best = dd;
                 bi = kk;
+                bp = f3{p[j].x, p[j].y, p[j].z};
             }
         }
     }
+    // Lane sub = 0 always holds a candidate (vertex 0's product is not NaN), so
+    // the group's reduction ends on a vertex some lane loaded: no reload.
     for (int off = G / 2; off >= 1; off >>= 1) {
         const float ob = __shfl_xor(best, off, G);
         const uint32_t oi = __shfl_xor(bi, off, G);
+        const float ox = __shfl_xor(bp.x, off, G), oy = __shfl_xor(bp.y, off, G), oz = __shfl_xor(bp.z, off, G);
         if (ob > best || (ob == best && oi < bi)) {
             best = ob;
             bi = oi;
+            bp = f3{ox, oy, oz};
         }
     }
-    const float4 p = s.v[bi];
-    return f3{p.x, p.y, p.z};
+    return bp;
 }
 
 // The same reduction over vertices staged in LDS (sv = nullptr: not staged).
@@ -927,16 +932,17 @@ __device__ uint32_t epa_step(W *w, float *orig, const iso &pos12, const ShapeRef
         cont = epa_export<G>(w, orig, sink, EpaScalars{nv, nf, best_id, niter, max_dist, old_dist}, heap.len);
         return EPA_OVERFLOW;
     };
+    // Faces deleted since they were pushed are popped and skipped here, as
+    // parry's loop skips them (`continue`), rather than one per call.
+    while (heap.len > 0 && w->face[w->heap_id[0]].deleted) {
+        uint32_t pid;
+        float pnd;
+        heap.pop(pid, pnd);
+    }
     if (heap.len == 0) return done();
     {
         const uint32_t fid = w->heap_id[0];
         const float fnd = w->heap_nd[0];
-        if (w->face[fid].deleted) {
-            uint32_t pid;
-            float pnd;
-            heap.pop(pid, pnd);
-            return EPA_RUNNING;
-        }
         const typename W::Face face = w->face[fid];
         const f3 fn = f3{face.n[0], face.n[1], face.n[2]};
         const Cso c = group_cso<G>(pos12, g1, g2, st, fn);
