@@ -383,6 +383,50 @@ using EpaLarge = EpaState<112, 8192, 8192, 2048, 8192, uint16_t, 0>;
 static_assert(sizeof(EpaFaceT<uint8_t>) == 20, "compact LDS face record");
 constexpr uint32_t kOrigFloats = 6;  // orig1, orig2 per CSO vertex
 
+// The argmax reduction over a group of G consecutive lanes: largest value, then
+// lowest index (a total order, so any pairing of the lanes gives the same
+// winner).  Groups of up to 16 lanes exchange through DPP row permutations --
+// mirror within 16 lanes, mirror within 8, then the two quad swaps -- which
+// reach every lane of the group in log2(G) steps without the LDS crossbar;
+// wider groups use shuffles.  A group's lanes are always active together.
+template <int CTRL> __device__ __forceinline__ uint32_t dpp_get(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL> __device__ __forceinline__ void argmax_dpp(float &best, uint32_t &bi, f3 &bp) {
+    const float ob = __uint_as_float(dpp_get<CTRL>(__float_as_uint(best)));
+    const uint32_t oi = dpp_get<CTRL>(bi);
+    const float ox = __uint_as_float(dpp_get<CTRL>(__float_as_uint(bp.x)));
+    const float oy = __uint_as_float(dpp_get<CTRL>(__float_as_uint(bp.y)));
+    const float oz = __uint_as_float(dpp_get<CTRL>(__float_as_uint(bp.z)));
+    if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+        bp = f3{ox, oy, oz};
+    }
+}
+constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppQuadXor2 = 0x4E, kDppQuadXor1 = 0xB1;
+template <int G> __device__ __forceinline__ void group_argmax(float &best, uint32_t &bi, f3 &bp) {
+    if constexpr (G <= 16) {
+        if constexpr (G == 16) argmax_dpp<kDppRowMirror>(best, bi, bp);
+        if constexpr (G >= 8) argmax_dpp<kDppRowHalfMirror>(best, bi, bp);
+        if constexpr (G >= 4) {
+            argmax_dpp<kDppQuadXor2>(best, bi, bp);
+            argmax_dpp<kDppQuadXor1>(best, bi, bp);
+        }
+    } else {
+        for (int off = G / 2; off >= 1; off >>= 1) {
+            const float ob = __shfl_xor(best, off, G);
+            const uint32_t oi = __shfl_xor(bi, off, G);
+            const float ox = __shfl_xor(bp.x, off, G), oy = __shfl_xor(bp.y, off, G), oz = __shfl_xor(bp.z, off, G);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+                bp = f3{ox, oy, oz};
+            }
+        }
+    }
+}
+
 // ConvexPolyhedron support point computed by a group of G consecutive lanes:
 // parry takes the first strict maximum of p.d over the vertex list; if vertex
 // 0's product is NaN nothing ever beats it, otherwise the result is the lowest
@@ -418,16 +462,7 @@ template <int G> __device__ __forceinline__ f3 group_support(const ShapeRef &s, 
     }
     // Lane sub = 0 always holds a candidate (vertex 0's product is not NaN), so
     // the group's reduction ends on a vertex some lane loaded: no reload.
-    for (int off = G / 2; off >= 1; off >>= 1) {
-        const float ob = __shfl_xor(best, off, G);
-        const uint32_t oi = __shfl_xor(bi, off, G);
-        const float ox = __shfl_xor(bp.x, off, G), oy = __shfl_xor(bp.y, off, G), oz = __shfl_xor(bp.z, off, G);
-        if (ob > best || (ob == best && oi < bi)) {
-            best = ob;
-            bi = oi;
-            bp = f3{ox, oy, oz};
-        }
-    }
+    group_argmax<G>(best, bi, bp);
     return bp;
 }
 
@@ -440,22 +475,18 @@ template <int G> __device__ __forceinline__ f3 group_support_staged(const ShapeR
     const uint32_t sub = __lane_id() % G;
     float best = -INFINITY;
     uint32_t bi = 0xffffffffu;
+    f3 bp = p0;
     for (uint32_t k = sub; k < s.n; k += G) {
-        const float dd = dot3(f3{sv[k][0], sv[k][1], sv[k][2]}, d);
+        const f3 q = f3{sv[k][0], sv[k][1], sv[k][2]};
+        const float dd = dot3(q, d);
         if (dd == dd && (bi == 0xffffffffu || dd > best)) {
             best = dd;
             bi = k;
+            bp = q;
         }
     }
-    for (int off = G / 2; off >= 1; off >>= 1) {
-        const float ob = __shfl_xor(best, off, G);
-        const uint32_t oi = __shfl_xor(bi, off, G);
-        if (ob > best || (ob == best && oi < bi)) {
-            best = ob;
-            bi = oi;
-        }
-    }
-    return f3{sv[bi][0], sv[bi][1], sv[bi][2]};
+    group_argmax<G>(best, bi, bp);
+    return bp;
 }
 
 struct Staged {
