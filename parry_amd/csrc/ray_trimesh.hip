@@ -28,6 +28,11 @@
 // The lane mappings are instantiated for 64-, 128- and 256-lane workgroups
 // and 0, 1, 2 or 4 stack entries in registers.
 //
+// Two exact shortcuts in the static mappings: a wave whose rays have no zero
+// direction component runs a slab test without the degenerate-axis terms
+// (identities for such rays), and the traversal keeps only the best time of
+// impact, leaf and side, computing the winner's normal once at the end.
+//
 // Floating point: compiled with -ffp-contract=off; 1/x and sqrt are the
 // correctly rounded IEEE operations, min/max are IEEE maxNum/minNum (the
 // semantics of Rust's f32::min/max), so every value is the one parry computes.
@@ -49,6 +54,12 @@ struct LaneRay {
 // unwrap_or(Real::MAX).  The reference exits as soon as tmin > tmax; because
 // tmin only grows and tmax only shrinks, testing once after the three axes
 // gives the same answer, so the three axes are evaluated branch-free.
+//
+// REGULAR: the caller guarantees dx, dy and dz are all nonzero (NaN counts as
+// nonzero, as in the test below).  Then every axis takes the slab branch, the
+// degenerate-axis terms are identities and are left out; the operations that
+// remain are the same, so the cost is the same bit for bit.
+template <bool REGULAR = false>
 __device__ __forceinline__ float slab_cost(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
                                            const LaneRay &r, float max_toi) {
     float tmin = 0.0f, tmax = max_toi;
@@ -57,15 +68,17 @@ __device__ __forceinline__ float slab_cost(float mnx, float mny, float mnz, floa
     unsigned miss = 0u;
 #define PQ_SLAB_AXIS(MN, MX, O, D, INV)                               \
     {                                                                 \
-        const unsigned z = (unsigned)(D == 0.0f);                      \
-        miss |= z & ((unsigned)(O < MN) | (unsigned)(O > MX));         \
         const float n = (MN - O) * INV;                                \
         const float f = (MX - O) * INV;                                \
         const bool s = n > f;                                          \
         float lo = s ? f : n;                                          \
         float hi = s ? n : f;                                          \
-        lo = z ? -INFINITY : lo;                                       \
-        hi = z ? INFINITY : hi;                                        \
+        if constexpr (!REGULAR) {                                      \
+            const unsigned z = (unsigned)(D == 0.0f);                  \
+            miss |= z & ((unsigned)(O < MN) | (unsigned)(O > MX));     \
+            lo = z ? -INFINITY : lo;                                   \
+            hi = z ? INFINITY : hi;                                    \
+        }                                                              \
         tmin = fmaxf(tmin, lo);                                        \
         tmax = fminf(tmax, hi);                                        \
     }
@@ -84,13 +97,16 @@ struct TriHit {
 };
 
 // local_ray_intersection_with_triangle (ray_triangle.rs:70-152) followed by
-// the `toi <= max_toi` acceptance of Triangle::cast_local_ray_and_get_normal.
+// the `toi <= max_toi` acceptance of Triangle::cast_local_ray_and_get_normal,
+// without the normal: only the time of impact and the side decide find_best's
+// path, so the traversal defers the normal to the winning triangle
+// (triangle_normal below, the same operations on the same values).
 // CULL 1 / 2 first drops the triangles RayCullingMode::IgnoreBackfaces /
 // IgnoreFrontfaces skips (ray_trimesh.rs:58-65): its test is the sign of the
 // scaled normal (ab x ac) dotted with the ray direction, which is `d` below.
 template <int CULL = 0>
-__device__ __forceinline__ bool ray_triangle(const float4 q0, const float4 q1, const float4 q2, const LaneRay &r,
-                                             float max_toi, TriHit &h) {
+__device__ __forceinline__ bool ray_triangle_toi(const float4 q0, const float4 q1, const float4 q2, const LaneRay &r,
+                                                 float max_toi, float &toi_out, uint32_t &back_out) {
     const float ax = q0.x, ay = q0.y, az = q0.z;
     const float bx = q0.w, by = q1.x, bz = q1.y;
     const float cx = q1.z, cy = q1.w, cz = q2.x;
@@ -122,19 +138,44 @@ __device__ __forceinline__ bool ray_triangle(const float4 q0, const float4 q1, c
     if (w < 0.0f || v + w > d) return false;
     const float invd = 1.0f / d;
     const float toi = neg ? (-t * invd) : (t * invd);
+    if (!(toi <= max_toi)) return false;
+    toi_out = toi;
+    back_out = back;
+    return true;
+}
+
+// The unit normal ray_triangle.rs:70-152 reports with a hit of this ray on
+// this triangle: (ab x ac) normalised, negated when the origin lies below the
+// triangle's plane.  Recomputes n and t exactly as ray_triangle_toi does.
+__device__ __forceinline__ void triangle_normal(const float4 q0, const float4 q1, const float4 q2, const LaneRay &r,
+                                                float &ux, float &uy, float &uz) {
+    const float ax = q0.x, ay = q0.y, az = q0.z;
+    const float bx = q0.w, by = q1.x, bz = q1.y;
+    const float cx = q1.z, cy = q1.w, cz = q2.x;
+    const float abx = bx - ax, aby = by - ay, abz = bz - az;
+    const float acx = cx - ax, acy = cy - ay, acz = cz - az;
+    const float nx = aby * acz - abz * acy;
+    const float ny = abz * acx - abx * acz;
+    const float nz = abx * acy - aby * acx;
+    const float apx = r.ox - ax, apy = r.oy - ay, apz = r.oz - az;
+    const float t = (apx * nx + apy * ny) + apz * nz;
     const float len = sqrtf((nx * nx + ny * ny) + nz * nz);
-    float ux = nx / len, uy = ny / len, uz = nz / len;
-    if (neg) {
+    ux = nx / len;
+    uy = ny / len;
+    uz = nz / len;
+    if (t < 0.0f) {
         ux = -ux;
         uy = -uy;
         uz = -uz;
     }
-    if (!(toi <= max_toi)) return false;
-    h.toi = toi;
-    h.nx = ux;
-    h.ny = uy;
-    h.nz = uz;
-    h.back = back;
+}
+
+// Time of impact, side and normal together (a single test, no traversal).
+template <int CULL = 0>
+__device__ __forceinline__ bool ray_triangle(const float4 q0, const float4 q1, const float4 q2, const LaneRay &r,
+                                             float max_toi, TriHit &h) {
+    if (!ray_triangle_toi<CULL>(q0, q1, q2, r, max_toi, h.toi, h.back)) return false;
+    triangle_normal(q0, q1, q2, r, h.nx, h.ny, h.nz);
     return true;
 }
 
@@ -180,12 +221,16 @@ struct LaneStack {
     }
 };
 
+// A lane's find_best state: the best time of impact so far (max_toi before
+// any hit), the leaf holding it and its side; the normal is computed once,
+// from the winning triangle, when the hit is written.
 struct Traversal {
     float best_cost;
-    TriHit best;
     uint32_t best_leaf;
+    uint32_t best_back;
     uint32_t curr;
     bool have;
+    float nx, ny, nz;  // the best hit's normal, kept only when it is not deferred
 };
 
 __device__ __forceinline__ LaneRay load_ray(const pq_ray *rays, uint64_t i) {
@@ -206,32 +251,41 @@ __device__ __forceinline__ LaneRay load_ray(const pq_ray *rays, uint64_t i) {
     return r;
 }
 
-__device__ __forceinline__ void start_traversal(Traversal &t, float max_toi) {
-    t.best_cost = max_toi;
-    t.best = TriHit{};
-    t.best_leaf = 0;
-    t.curr = 0;
-    t.have = false;
+// No zero direction component: slab_cost<true> applies to this ray.
+__device__ __forceinline__ bool ray_is_regular(const LaneRay &r) {
+    return r.dx != 0.0f && r.dy != 0.0f && r.dz != 0.0f;
 }
 
-// Tests one leaf (a triangle record) against the current best.
-template <bool COUNT, int CULL = 0>
+__device__ __forceinline__ void start_traversal(Traversal &t, float max_toi) {
+    t.best_cost = max_toi;
+    t.best_leaf = 0;
+    t.best_back = 0;
+    t.curr = 0;
+    t.have = false;
+    t.nx = t.ny = t.nz = 0.0f;
+}
+
+// Tests one leaf (a triangle record) against the current best.  DEFER leaves
+// the normal to write_hit; otherwise it is computed with each new best.
+template <bool COUNT, int CULL = 0, bool DEFER = true>
 __device__ __forceinline__ void visit_leaf(const float4 *tri4, uint32_t leaf, const LaneRay &r, Traversal &t,
                                            uint32_t &n_tri) {
-    TriHit h;
+    float toi;
+    uint32_t back;
     if (COUNT) ++n_tri;
-    if (ray_triangle<CULL>(tri4[3 * leaf], tri4[3 * leaf + 1], tri4[3 * leaf + 2], r, t.best_cost, h) &&
-        h.toi < t.best_cost) {
-        t.best = h;
-        t.best_cost = h.toi;
+    const float4 q0 = tri4[3 * leaf], q1 = tri4[3 * leaf + 1], q2 = tri4[3 * leaf + 2];
+    if (ray_triangle_toi<CULL>(q0, q1, q2, r, t.best_cost, toi, back) && toi < t.best_cost) {
+        t.best_cost = toi;
+        t.best_back = back;
         t.best_leaf = leaf;
         t.have = true;
+        if constexpr (!DEFER) triangle_normal(q0, q1, q2, r, t.nx, t.ny, t.nz);
     }
 }
 
 // One iteration of find_best's loop (bvh_traverse.rs:361-416) on wide node
 // t.curr.  Returns true when the traversal has finished.
-template <bool COUNT, int CULL, class Stack>
+template <bool COUNT, int CULL, bool REGULAR, bool DEFER = true, class Stack>
 __device__ __forceinline__ bool find_best_step(const float4 *node4, const float4 *tri4, const LaneRay &r,
                                                Traversal &t, Stack &stk, uint32_t &n_wide, uint32_t &n_tri) {
     if (COUNT) ++n_wide;
@@ -239,8 +293,8 @@ __device__ __forceinline__ bool find_best_step(const float4 *node4, const float4
     const float4 q1 = node4[4 * t.curr + 1];  // left maxs, left data
     const float4 q2 = node4[4 * t.curr + 2];  // right mins, right children
     const float4 q3 = node4[4 * t.curr + 3];  // right maxs, right data
-    float ls = slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, t.best_cost);
-    float rs = slab_cost(q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, r, t.best_cost);
+    float ls = slab_cost<REGULAR>(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, t.best_cost);
+    float rs = slab_cost<REGULAR>(q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, r, t.best_cost);
     uint32_t lc = __float_as_uint(q0.w), ld = __float_as_uint(q1.w);
     uint32_t rc = __float_as_uint(q2.w), rd = __float_as_uint(q3.w);
     const bool swap = ls > rs;
@@ -258,7 +312,7 @@ __device__ __forceinline__ bool find_best_step(const float4 *node4, const float4
     bool found_next = false;
     if (ls < t.best_cost && ls != FLT_MAX) {
         if (node_is_leaf(ld)) {
-            visit_leaf<COUNT, CULL>(tri4, lc, r, t, n_tri);
+            visit_leaf<COUNT, CULL, DEFER>(tri4, lc, r, t, n_tri);
         } else {
             t.curr = lc;
             found_next = true;
@@ -266,7 +320,7 @@ __device__ __forceinline__ bool find_best_step(const float4 *node4, const float4
     }
     if (rs < t.best_cost && rs != FLT_MAX) {
         if (node_is_leaf(rd)) {
-            visit_leaf<COUNT, CULL>(tri4, rc, r, t, n_tri);
+            visit_leaf<COUNT, CULL, DEFER>(tri4, rc, r, t, n_tri);
         } else if (found_next) {
             stk.push(rc);
         } else {
@@ -297,10 +351,25 @@ __device__ __forceinline__ pq_ray_hit make_hit(bool have, const TriHit &h, uint3
     return out;
 }
 
-__device__ __forceinline__ void write_hit(pq_ray_hit *hits, uint64_t i, const Traversal &t, const float4 *tri4,
-                                          uint32_t ntri) {
-    const uint32_t tri_id = t.have ? __float_as_uint(tri4[3 * t.best_leaf + 2].y) : 0u;
-    hits[i] = make_hit(t.have, t.best, tri_id, ntri);
+template <bool DEFER = true>
+__device__ __forceinline__ void write_hit(pq_ray_hit *hits, uint64_t i, const Traversal &t, const LaneRay &r,
+                                          const float4 *tri4, uint32_t ntri) {
+    TriHit h{};
+    uint32_t tri_id = 0;
+    if (t.have) {
+        const float4 q2 = tri4[3 * t.best_leaf + 2];
+        if constexpr (DEFER) {
+            triangle_normal(tri4[3 * t.best_leaf], tri4[3 * t.best_leaf + 1], q2, r, h.nx, h.ny, h.nz);
+        } else {
+            h.nx = t.nx;
+            h.ny = t.ny;
+            h.nz = t.nz;
+        }
+        h.toi = t.best_cost;
+        h.back = t.best_back;
+        tri_id = __float_as_uint(q2.y);
+    }
+    hits[i] = make_hit(t.have, h, tri_id, ntri);
 }
 
 // Static mapping; also serves trees whose root holds a single leaf
@@ -333,15 +402,18 @@ __global__ __launch_bounds__(BLOCK) void cast_rays_kernel(const pq_bvh_node_wide
         n_wide = 1;
         if (slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi) < max_toi)
             visit_leaf<true, CULL>(tri4, __float_as_uint(q0.w), r, t, n_tri);
+    } else if (__all(ray_is_regular(r))) {  // the usual case: no zero direction component in the wave
+        while (!find_best_step<COUNT, CULL, true>(node4, tri4, r, t, stk, n_wide, n_tri)) {
+        }
     } else {
-        while (!find_best_step<COUNT, CULL>(node4, tri4, r, t, stk, n_wide, n_tri)) {
+        while (!find_best_step<COUNT, CULL, false>(node4, tri4, r, t, stk, n_wide, n_tri)) {
         }
     }
     if (COUNT) {
         counts[2 * i] = n_wide;
         counts[2 * i + 1] = n_tri;
     }
-    write_hit(hits, i, t, tri4, ntri);
+    write_hit(hits, i, t, r, tri4, ntri);
 }
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
@@ -362,7 +434,11 @@ __device__ __forceinline__ void queue_wave_done(unsigned long long *queue, uint3
     }
 }
 
-// Persistent lane mapping over trees with at least two leaves.
+// Persistent lane mapping over trees with at least two leaves.  It runs the
+// general slab test and computes each new best's normal at once: a lane that
+// finishes writes its hit inside the loop, in a block of its own, so a
+// deferred normal would cost that block every iteration, and a second,
+// regular copy of the step raises register use; both were slower here.
 template <bool COUNT, int BLOCK, int REG>
 __global__ __launch_bounds__(BLOCK) void cast_rays_persistent_kernel(
     const pq_bvh_node_wide *__restrict__ nodes, const TriRecord *__restrict__ tris, uint32_t ntri,
@@ -415,12 +491,12 @@ __global__ __launch_bounds__(BLOCK) void cast_rays_persistent_kernel(
             if (drained) break;
             continue;
         }
-        if (active && find_best_step<COUNT, 0>(node4, tri4, r, t, stk, n_wide, n_tri)) {
+        if (active && find_best_step<COUNT, 0, false, false>(node4, tri4, r, t, stk, n_wide, n_tri)) {
             if (COUNT) {
                 counts[2 * ray_id] = n_wide;
                 counts[2 * ray_id + 1] = n_tri;
             }
-            write_hit(hits, ray_id, t, tri4, ntri);
+            write_hit<false>(hits, ray_id, t, r, tri4, ntri);
             active = false;
         }
     }
@@ -451,6 +527,7 @@ struct WaveWindow {
     uint32_t lc, rc, ld, rd;
 };
 
+template <bool REGULAR>
 __device__ __forceinline__ WaveWindow fetch_window(const float4 *node4, uint32_t num_nodes, const float4 *tri4,
                                                    uint32_t base, const LaneRay &r, float max_toi) {
     WaveWindow w{FLT_MAX, FLT_MAX, INFINITY, INFINITY, 0u, 0u, 0u, 0u};
@@ -458,19 +535,20 @@ __device__ __forceinline__ WaveWindow fetch_window(const float4 *node4, uint32_t
     if (k < num_nodes) {
         const float4 q0 = node4[4 * k + 0], q1 = node4[4 * k + 1];
         const float4 q2 = node4[4 * k + 2], q3 = node4[4 * k + 3];
-        w.ls = slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi);
-        w.rs = slab_cost(q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, r, max_toi);
+        w.ls = slab_cost<REGULAR>(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi);
+        w.rs = slab_cost<REGULAR>(q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, r, max_toi);
         w.lc = __float_as_uint(q0.w);
         w.ld = __float_as_uint(q1.w);
         w.rc = __float_as_uint(q2.w);
         w.rd = __float_as_uint(q3.w);
-        TriHit h;
+        float toi;
+        uint32_t back;
         if (node_is_leaf(w.ld) && w.ls < max_toi && w.ls != FLT_MAX &&
-            ray_triangle<0>(tri4[3 * w.lc], tri4[3 * w.lc + 1], tri4[3 * w.lc + 2], r, max_toi, h))
-            w.lt = h.toi;
+            ray_triangle_toi<0>(tri4[3 * w.lc], tri4[3 * w.lc + 1], tri4[3 * w.lc + 2], r, max_toi, toi, back))
+            w.lt = toi;
         if (node_is_leaf(w.rd) && w.rs < max_toi && w.rs != FLT_MAX &&
-            ray_triangle<0>(tri4[3 * w.rc], tri4[3 * w.rc + 1], tri4[3 * w.rc + 2], r, max_toi, h))
-            w.rt = h.toi;
+            ray_triangle_toi<0>(tri4[3 * w.rc], tri4[3 * w.rc + 1], tri4[3 * w.rc + 2], r, max_toi, toi, back))
+            w.rt = toi;
     }
     return w;
 }
@@ -495,23 +573,29 @@ __device__ __forceinline__ pq_ray_hit wave_cast_one(const float4 *node4, uint32_
         const float4 q0 = node4[0], q1 = node4[1];
         n_wide = 1;
         n_tri = 0;
-        TriHit h;
+        float toi;
+        uint32_t back;
         if (slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi) < max_toi) {
             n_tri = 1;
             const uint32_t leaf = __float_as_uint(q0.w);
-            if (ray_triangle<0>(tri4[3 * leaf], tri4[3 * leaf + 1], tri4[3 * leaf + 2], r, max_toi, h) &&
-                h.toi < best) {
+            if (ray_triangle_toi<0>(tri4[3 * leaf], tri4[3 * leaf + 1], tri4[3 * leaf + 2], r, max_toi, toi, back) &&
+                toi < best) {
                 best_leaf = leaf;
                 have = true;
             }
         }
     } else {
         uint32_t stk = 0, sp = 0, curr = 0, base = 0;
-        WaveWindow w = fetch_window(node4, num_nodes, tri4, 0u, r, max_toi);
+        const bool regular = ray_is_regular(r);  // the same ray in every lane: a uniform branch
+        auto fetch = [&](uint32_t at) {
+            return regular ? fetch_window<true>(node4, num_nodes, tri4, at, r, max_toi)
+                           : fetch_window<false>(node4, num_nodes, tri4, at, r, max_toi);
+        };
+        WaveWindow w = fetch(0u);
         for (;;) {
             if (curr - base >= 64u) {
                 base = curr;
-                w = fetch_window(node4, num_nodes, tri4, base, r, max_toi);
+                w = fetch(base);
             }
             const uint32_t j = curr - base;
             if (COUNT) ++n_wide;
@@ -649,7 +733,12 @@ __device__ __forceinline__ void cast_one(const BatchMesh &bm, const LaneRay &r, 
             visit_leaf<true>(tri4, __float_as_uint(q0.w), r, t, n_tri);
         return;
     }
-    while (!find_best_step<COUNT, 0>(node4, tri4, r, t, stk, n_wide, n_tri)) {
+    if (__all(ray_is_regular(r))) {
+        while (!find_best_step<COUNT, 0, true>(node4, tri4, r, t, stk, n_wide, n_tri)) {
+        }
+    } else {
+        while (!find_best_step<COUNT, 0, false>(node4, tri4, r, t, stk, n_wide, n_tri)) {
+        }
     }
 }
 
@@ -674,7 +763,7 @@ __global__ __launch_bounds__(BLOCK) void cast_rays_batch_kernel(const BatchMesh 
         counts[2 * i] = n_wide;
         counts[2 * i + 1] = n_tri;
     }
-    write_hit(hits, i, t, reinterpret_cast<const float4 *>(bm.tris), bm.ntri);
+    write_hit(hits, i, t, r, reinterpret_cast<const float4 *>(bm.tris), bm.ntri);
 }
 
 // Persistent variant: one device queue over the concatenated rays; finished
@@ -723,7 +812,7 @@ __global__ __launch_bounds__(BLOCK) void cast_rays_batch_persistent_kernel(
                         const float4 q0 = node4[0], q1 = node4[1];
                         n_wide = 1;
                         if (slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi) < max_toi)
-                            visit_leaf<true>(reinterpret_cast<const float4 *>(bm.tris), __float_as_uint(q0.w), r, t,
+                            visit_leaf<true, 0, false>(reinterpret_cast<const float4 *>(bm.tris), __float_as_uint(q0.w), r, t,
                                              n_tri);
                     }
                 }
@@ -737,15 +826,15 @@ __global__ __launch_bounds__(BLOCK) void cast_rays_batch_persistent_kernel(
         }
         if (active) {
             const bool done = bm.single_leaf ||
-                              find_best_step<COUNT, 0>(reinterpret_cast<const float4 *>(bm.nodes),
-                                                       reinterpret_cast<const float4 *>(bm.tris), r, t, stk, n_wide,
-                                                       n_tri);
+                              find_best_step<COUNT, 0, false, false>(reinterpret_cast<const float4 *>(bm.nodes),
+                                                                     reinterpret_cast<const float4 *>(bm.tris), r, t,
+                                                                     stk, n_wide, n_tri);
             if (done) {
                 if (COUNT) {
                     counts[2 * ray_id] = n_wide;
                     counts[2 * ray_id + 1] = n_tri;
                 }
-                write_hit(hits, ray_id, t, reinterpret_cast<const float4 *>(bm.tris), bm.ntri);
+                write_hit<false>(hits, ray_id, t, r, reinterpret_cast<const float4 *>(bm.tris), bm.ntri);
                 active = false;
             }
         }

@@ -90,6 +90,20 @@ def test_small_uneven_batch(orc):
     _check_batch(orc, meshes, rays)
 
 
+def test_batch_with_axis_aligned_rays(orc):
+    # Every 7th ray along a coordinate axis and a few with a zero direction,
+    # scattered so that most waves mix them with ordinary rays (the kernels
+    # take the degenerate-axis slab test for such waves).
+    meshes, rays = _build(SMALL)
+    for k, r in enumerate(rays):
+        idx = np.arange(k % 7, r.shape[0], 7)
+        r[idx, 3:] = 0.0
+        r[idx, 3 + idx % 3] = np.where(idx % 2 == 0, 1.0, -1.0)
+        r[idx, :3] *= 0.5
+        r[idx[::50], 3:] = 0.0
+    _check_batch(orc, meshes, rays)
+
+
 def test_counted_batch_matches_single_mesh_counts():
     torch = _torch()
     meshes, rays = _build(SMALL)

@@ -144,6 +144,10 @@ def test_edge_cases(orc):
             hits = _cast_gpu_host(mesh, rays, max_toi, mapping)
             _assert_parity(hits, om.cast_rays(rays, max_toi), f"edge max_toi={max_toi} {mapping}")
         assert (hits["feature"] == NO_HIT).all()  # max_toi = 0 never hits (strict <)
+        # the same rays shuffled: waves that mix axis-aligned and ordinary rays
+        perm = rng.permutation(n)
+        hits = _cast_gpu_host(mesh, rays[perm], 10.0, mapping)
+        _assert_parity(hits, om.cast_rays(rays[perm], 10.0), f"edge shuffled {mapping}")
     # batch sizes around the persistent chunk and wave sizes
     for n_small in (1, 63, 64, 65, 127, 129, 1000):
         for mapping in ("persistent", "wave"):
