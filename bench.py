@@ -322,6 +322,7 @@ def mapping_sweep(args, log):
     from parry_amd import _native
 
     cands = ([{"mapping": "static", "block_size": b, "stack_regs": r} for b in (64, 128, 256) for r in (0, 1, 2, 4)]
+             + [{"mapping": "binned", "block_size": b, "stack_regs": r} for b in (64, 128, 256) for r in (0, 2)]
              + [{"mapping": "persistent", "block_size": b, "stack_regs": r, "chunk": c}
                 for b in (64, 256) for r in (0, 2) for c in (64, 128, 256)]
              + [{"mapping": "wave", "chunk": c} for c in (1, 4)])

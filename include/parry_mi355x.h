@@ -172,15 +172,25 @@ int pq_trimesh_export_bvh(const pq_trimesh *mesh, pq_bvh_node_wide *out_host, ui
  *                              walk leaves the window; waves pull `chunk` rays at
  *                              a time from the queue.  For few rays on a large
  *                              mesh.  Trees deeper than 65 wide levels run the
- *                              persistent mapping instead.
- * block_size (static and persistent): lanes per workgroup, 64, 128 or 256.
- * stack_regs (static and persistent): find_best's pending-node stack keeps its
- * top 0, 1, 2 or 4 entries in registers and the rest in LDS.
- * The queue (persistent and wave) of a stream is allocated on the first such
- * call on that stream: make that first call outside hipGraph capture. */
+ *                              persistent mapping instead;
+ *   PQ_RAY_MAPPING_BINNED      the static mapping over the rays in the order of
+ *                              the cell (16^3 grid over the Bvh's root box) where
+ *                              each enters that box: a counting sort by cell
+ *                              runs first, so the rays of a wave enter the mesh
+ *                              near each other and take similar paths; hits
+ *                              still land at the rays' own indices.  For large
+ *                              batches of incoherent rays; at most 2^32 - 1 rays
+ *                              per call.
+ * block_size (static, persistent, binned): lanes per workgroup, 64, 128 or 256.
+ * stack_regs (static, persistent, binned): find_best's pending-node stack keeps
+ * its top 0, 1, 2 or 4 entries in registers and the rest in LDS.
+ * The queue (persistent and wave) and the sort's buffers (binned, grown to the
+ * largest call) of a stream are allocated on the first such call on that
+ * stream: make that first call outside hipGraph capture. */
 #define PQ_RAY_MAPPING_STATIC 0u
 #define PQ_RAY_MAPPING_PERSISTENT 1u
 #define PQ_RAY_MAPPING_WAVE 2u
+#define PQ_RAY_MAPPING_BINNED 3u
 typedef struct pq_ray_tuning {
     uint32_t mapping;       /* PQ_RAY_MAPPING_* */
     uint32_t chunk;         /* persistent and wave: rays claimed per queue access (0 = 128; wave: 0 = 4) */

@@ -125,6 +125,7 @@ _lib = None
 PQ_RAY_MAPPING_STATIC = 0
 PQ_RAY_MAPPING_PERSISTENT = 1
 PQ_RAY_MAPPING_WAVE = 2
+PQ_RAY_MAPPING_BINNED = 3
 
 
 class RayTuning(ctypes.Structure):
@@ -134,7 +135,7 @@ class RayTuning(ctypes.Structure):
                 ("stack_regs", _u32), ("reserved", _u32)]
 
 
-RAY_MAPPINGS = {"static": 0, "persistent": 1, "wave": 2}
+RAY_MAPPINGS = {"static": 0, "persistent": 1, "wave": 2, "binned": 3}
 
 
 def ray_tuning(mapping: str = "persistent", chunk: int = 0, blocks_per_cu: int = 0, block_size: int = 256,

@@ -28,9 +28,9 @@ int hip_fail(hipError_t e, const char *what) {
 int check_ray_tuning(const pq_ray_tuning *t, const char *who) {
     if (!t) return fail(PQ_ERR_INVALID, std::string(who) + ": null tuning");
     const uint32_t b = t->block_size, r = t->stack_regs;
-    if (t->mapping > PQ_RAY_MAPPING_WAVE || t->reserved != 0 || t->chunk > (1u << 20) ||
+    if (t->mapping > PQ_RAY_MAPPING_BINNED || t->reserved != 0 || t->chunk > (1u << 20) ||
         (b != 0 && b != 64 && b != 128 && b != 256) || (r != 0 && r != 1 && r != 2 && r != 4))
-        return fail(PQ_ERR_INVALID, std::string(who) + ": invalid tuning (mapping 0-2, chunk <= 2^20, block_size "
+        return fail(PQ_ERR_INVALID, std::string(who) + ": invalid tuning (mapping 0-3, chunk <= 2^20, block_size "
                                                        "0/64/128/256, stack_regs 0/1/2/4, reserved 0)");
     return PQ_OK;
 }
@@ -56,6 +56,33 @@ hipError_t stream_queue(hipStream_t stream, unsigned long long **queue) {
     }
     queues[key] = (unsigned long long *)p;
     *queue = (unsigned long long *)p;
+    return hipSuccess;
+}
+
+hipError_t ray_bin_workspace(hipStream_t stream, uint64_t num_rays, uint32_t blocks, RayBinSpace *ws) {
+    static std::mutex mu;
+    static std::map<std::pair<int, hipStream_t>, std::pair<uint32_t *, size_t>> spaces;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const size_t hist_words = (size_t)kRayBins * blocks + kRayBins;
+    const size_t words = hist_words + 2 * (size_t)num_rays;
+    std::lock_guard<std::mutex> lock(mu);
+    auto &slot = spaces[std::make_pair(dev, stream)];
+    if (slot.second < words) {
+        if (slot.first) {
+            if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+            (void)hipFree(slot.first);
+            slot = {nullptr, 0};
+        }
+        void *p = nullptr;
+        if ((e = hipMalloc(&p, words * sizeof(uint32_t))) != hipSuccess) return e;
+        slot = {(uint32_t *)p, words};
+    }
+    ws->hist = slot.first;
+    ws->totals = slot.first + (size_t)kRayBins * blocks;
+    ws->keyrank = slot.first + hist_words;
+    ws->order = ws->keyrank + num_rays;
     return hipSuccess;
 }
 
@@ -269,7 +296,7 @@ int pq_trimesh_autotune(pq_trimesh *m, const pq_ray *rays, uint64_t num_rays, fl
     if (!m || !rays || !hits || num_rays == 0) return fail(PQ_ERR_INVALID, "pq_trimesh_autotune: null argument");
     const hipStream_t s = (hipStream_t)stream;
     return autotune_ray_mapping([&]() { return launch_cast_rays(m->dev, rays, num_rays, max_toi, hits, nullptr, s); },
-                                m->dev.tuning, s, chosen, chosen_ms, "pq_trimesh_autotune");
+                                m->dev.tuning, s, chosen, chosen_ms, "pq_trimesh_autotune", true);
 }
 
 int pq_trimesh_get_tuning(const pq_trimesh *m, pq_ray_tuning *t) {

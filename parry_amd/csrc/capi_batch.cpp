@@ -12,12 +12,16 @@ using namespace pq;
 namespace pq {
 
 // The candidates the ray tuners time: the lane mappings over block sizes and
-// register stack depths, the persistent mapping's chunk sizes and a capped
-// residency, and the wave mapping with one and four rays per claim.
-std::vector<pq_ray_tuning> ray_tuning_candidates() {
+// register stack depths (for a single mesh the binned mapping too), the
+// persistent mapping's chunk sizes and a capped residency, and the wave
+// mapping with one and four rays per claim.
+std::vector<pq_ray_tuning> ray_tuning_candidates(bool single_mesh) {
     std::vector<pq_ray_tuning> c;
     for (uint32_t block : {64u, 128u, 256u})
         for (uint32_t regs : {0u, 1u, 2u, 4u}) c.push_back({PQ_RAY_MAPPING_STATIC, 128, 0, block, regs, 0});
+    if (single_mesh)
+        for (uint32_t block : {64u, 128u, 256u})
+            for (uint32_t regs : {0u, 2u}) c.push_back({PQ_RAY_MAPPING_BINNED, 128, 0, block, regs, 0});
     for (uint32_t block : {64u, 256u})
         for (uint32_t regs : {0u, 2u})
             for (uint32_t chunk : {64u, 128u, 256u}) c.push_back({PQ_RAY_MAPPING_PERSISTENT, chunk, 0, block, regs, 0});
@@ -27,7 +31,7 @@ std::vector<pq_ray_tuning> ray_tuning_candidates() {
 }
 
 int autotune_ray_mapping(const std::function<hipError_t()> &launch, pq_ray_tuning &slot, hipStream_t s,
-                         pq_ray_tuning *chosen, float *chosen_ms, const char *who) {
+                         pq_ray_tuning *chosen, float *chosen_ms, const char *who, bool single_mesh) {
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -35,7 +39,7 @@ int autotune_ray_mapping(const std::function<hipError_t()> &launch, pq_ray_tunin
     const pq_ray_tuning saved = slot;
     float best_ms = 1e30f;
     pq_ray_tuning best = saved;
-    for (const pq_ray_tuning &c : ray_tuning_candidates()) {
+    for (const pq_ray_tuning &c : ray_tuning_candidates(single_mesh)) {
         if (e != hipSuccess) break;
         slot = c;
         // One untimed-for-the-result launch (also warms the caches); a mapping
@@ -142,6 +146,8 @@ int pq_trimesh_batch_set_tuning(pq_trimesh_batch *b, const pq_ray_tuning *t) {
     if (!b) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_set_tuning: null batch");
     int rc = check_ray_tuning(t, "pq_trimesh_batch_set_tuning");
     if (rc != PQ_OK) return rc;
+    if (t->mapping == PQ_RAY_MAPPING_BINNED)
+        return fail(PQ_ERR_INVALID, "pq_trimesh_batch_set_tuning: the binned mapping is for single meshes");
     b->dev.tuning = *t;
     return PQ_OK;
 }
@@ -177,7 +183,7 @@ int pq_trimesh_batch_autotune(pq_trimesh_batch *b, const pq_ray *rays, float max
     const hipStream_t s = (hipStream_t)stream;
     return autotune_ray_mapping(
         [&]() { return launch_cast_rays_batch(b->dev, rays, max_toi, hits, nullptr, s); }, b->dev.tuning, s, chosen,
-        chosen_ms, "pq_trimesh_batch_autotune");
+        chosen_ms, "pq_trimesh_batch_autotune", false);
 }
 
 int pq_trimesh_batch_capture(const pq_trimesh_batch *b, const pq_ray *rays, float max_toi, int solid,

@@ -95,14 +95,33 @@ hipError_t launch_cast_rays_batch(const DeviceBatch &b, const pq_ray *rays, floa
 // (zeroed at allocation; every persistent kernel leaves it zeroed).
 hipError_t stream_queue(hipStream_t stream, unsigned long long **queue);
 
+// Buffers of the binned ray mapping's counting sort, per (device, stream):
+// `hist` (kRayBins x blocks: each sort block's count per cell, scanned per
+// cell into offsets), `totals` (kRayBins: rays per cell), `keyrank` (per ray:
+// cell | rank within its block and cell << kRayBinKeyBits) and `order` (per
+// sorted position: the ray index).  Grown on demand (the stream is
+// synchronised before a smaller buffer is freed).
+constexpr uint32_t kRayBinBits = 4;  // cells per axis: 16
+constexpr uint32_t kRayBinKeyBits = 3 * kRayBinBits;
+constexpr uint32_t kRayBins = 1u << kRayBinKeyBits;
+constexpr uint32_t kRayBinMaxBlocks = 65536;  // sort blocks; a block ranks < 2^20 rays
+struct RayBinSpace {
+    uint32_t *hist;
+    uint32_t *totals;
+    uint32_t *keyrank;
+    uint32_t *order;
+};
+hipError_t ray_bin_workspace(hipStream_t stream, uint64_t num_rays, uint32_t blocks, RayBinSpace *ws);
+
 // LDS stack entries the ray kernels need for a tree of `depth` wide levels.
 uint32_t ray_stack_entries(uint32_t depth);
 // Whether the wave-per-ray mapping can run a tree of `depth` wide levels.
 bool wave_mapping_fits(uint32_t depth);
 // Times every ray mapping candidate with `launch` (which reads `slot`), leaves
 // the fastest in `slot` and reports it; synchronises `s`.
+// `single_mesh` adds the binned mapping to the candidates.
 int autotune_ray_mapping(const std::function<hipError_t()> &launch, pq_ray_tuning &slot, hipStream_t s,
-                         pq_ray_tuning *chosen, float *chosen_ms, const char *who);
+                         pq_ray_tuning *chosen, float *chosen_ms, const char *who, bool single_mesh);
 
 // Launches the ray-cast kernel (ray_trimesh.hip).
 // `counts` (nullable, device, 2 per ray): wide-node records fetched, triangles tested.

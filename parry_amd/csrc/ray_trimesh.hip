@@ -233,23 +233,32 @@ struct Traversal {
     float nx, ny, nz;  // the best hit's normal, kept only when it is not deferred
 };
 
-__device__ __forceinline__ LaneRay load_ray(const pq_ray *rays, uint64_t i) {
+// A ray record as loaded (origin, direction), before the inverses.
+struct RawRay {
+    float2 r01, r23, r45;
+};
+
+__device__ __forceinline__ RawRay load_raw_ray(const pq_ray *rays, uint64_t i) {
     const float *rp = reinterpret_cast<const float *>(rays + i);
-    const float2 r01 = *reinterpret_cast<const float2 *>(rp);
-    const float2 r23 = *reinterpret_cast<const float2 *>(rp + 2);
-    const float2 r45 = *reinterpret_cast<const float2 *>(rp + 4);
+    return RawRay{*reinterpret_cast<const float2 *>(rp), *reinterpret_cast<const float2 *>(rp + 2),
+                  *reinterpret_cast<const float2 *>(rp + 4)};
+}
+
+__device__ __forceinline__ LaneRay lane_ray(const RawRay &w) {
     LaneRay r;
-    r.ox = r01.x;
-    r.oy = r01.y;
-    r.oz = r23.x;
-    r.dx = r23.y;
-    r.dy = r45.x;
-    r.dz = r45.y;
+    r.ox = w.r01.x;
+    r.oy = w.r01.y;
+    r.oz = w.r23.x;
+    r.dx = w.r23.y;
+    r.dy = w.r45.x;
+    r.dz = w.r45.y;
     r.ix = 1.0f / r.dx;
     r.iy = 1.0f / r.dy;
     r.iz = 1.0f / r.dz;
     return r;
 }
+
+__device__ __forceinline__ LaneRay load_ray(const pq_ray *rays, uint64_t i) { return lane_ray(load_raw_ray(rays, i)); }
 
 // No zero direction component: slab_cost<true> applies to this ray.
 __device__ __forceinline__ bool ray_is_regular(const LaneRay &r) {
@@ -382,10 +391,12 @@ __global__ __launch_bounds__(BLOCK) void cast_rays_kernel(const pq_bvh_node_wide
                                                           const TriRecord *__restrict__ tris, uint32_t ntri,
                                                           const pq_ray *__restrict__ rays, uint64_t n, float max_toi,
                                                           pq_ray_hit *__restrict__ hits,
-                                                          uint32_t *__restrict__ counts) {
+                                                          uint32_t *__restrict__ counts,
+                                                          const uint32_t *__restrict__ order) {
     extern __shared__ uint32_t lds_stack[];
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t i = order ? (uint64_t)order[j] : j;  // the binned mapping's j-th ray in sorted order
     LaneStack<BLOCK, REG> stk;
     stk.col = lds_stack + threadIdx.x;
     stk.reset();
@@ -873,6 +884,160 @@ __global__ __launch_bounds__(64) void cast_rays_batch_wave_kernel(const BatchMes
     queue_wave_done(queue, gridDim.x);
 }
 
+// ---------------------------------------------------------------------------
+// The binned mapping's counting sort.  A ray's cell is where it enters the
+// sphere inscribed in the Bvh's root box (where it passes closest to the
+// sphere's centre if it misses it; its origin if it starts inside), on a 16^3
+// grid over the box, numbered along a Morton curve so nearby cells get nearby
+// numbers.  The cell only orders the work: every ray still runs its own
+// find_best, so the hits are those of the static mapping.  The rays are split
+// into `blocks` contiguous ranges of `per_block` rays, one sort workgroup each,
+// and no global atomics are used:
+//   bin_count_kernel   per range: each ray's cell and its rank among the
+//                      range's rays of that cell (LDS atomics), and the
+//                      range's count per cell into hist[cell * blocks + range];
+//   bin_scan_kernel    per cell (one wave each): exclusive prefix sum over the
+//                      ranges in place, and the cell's total;
+//   bin_cells_kernel   exclusive prefix sum of the totals: each cell's first
+//                      sorted position;
+//   bin_scatter_kernel order[first[cell] + hist[cell * blocks + range] + rank]
+//                      = ray index.
+// ---------------------------------------------------------------------------
+constexpr int kBinBlock = 256;          // scatter workgroup
+constexpr int kCountBlock = 1024;       // count workgroup
+constexpr int kCountPer = 4;            // rays per count lane per round (loads in flight together)
+constexpr uint32_t kBinRange = kCountBlock * kCountPer;  // rays per sort range (before the block cap)
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // bit k -> bit 3k (4 bits)
+    v &= 0xfu;
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4) | ((v & 8u) << 6);
+}
+
+__device__ __forceinline__ uint32_t ray_cell(const float mn[3], const float mx[3], const LaneRay &r) {
+    float c[3], rad = FLT_MAX;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c[a] = 0.5f * (mn[a] + mx[a]);
+        rad = fminf(rad, 0.5f * (mx[a] - mn[a]));
+    }
+    const float oc[3] = {r.ox - c[0], r.oy - c[1], r.oz - c[2]};
+    const float dd = r.dx * r.dx + r.dy * r.dy + r.dz * r.dz;
+    const float b = oc[0] * r.dx + oc[1] * r.dy + oc[2] * r.dz;
+    const float cc = oc[0] * oc[0] + oc[1] * oc[1] + oc[2] * oc[2] - rad * rad;
+    float t = dd > 0.0f ? (-b - sqrtf(fmaxf(b * b - dd * cc, 0.0f))) / dd : 0.0f;
+    t = fmaxf(t, 0.0f);  // NaN -> 0
+    const float p[3] = {r.ox + t * r.dx, r.oy + t * r.dy, r.oz + t * r.dz};
+    uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float ext = mx[a] - mn[a];
+        const float u = ext > 0.0f ? (p[a] - mn[a]) / ext * (float)(1u << kRayBinBits) : 0.0f;
+        q[a] = (uint32_t)fminf(fmaxf(u, 0.0f), (float)((1u << kRayBinBits) - 1u));
+    }
+    return (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+}
+
+__global__ __launch_bounds__(kCountBlock) void bin_count_kernel(const pq_bvh_node_wide *__restrict__ nodes,
+                                                                const pq_ray *__restrict__ rays, uint64_t n,
+                                                                uint64_t per_block, uint32_t *__restrict__ hist,
+                                                                uint32_t *__restrict__ keyrank) {
+    __shared__ uint32_t cnt[kRayBins];
+    for (uint32_t c = threadIdx.x; c < kRayBins; c += kCountBlock) cnt[c] = 0u;
+    __syncthreads();
+    const float4 *node4 = reinterpret_cast<const float4 *>(nodes);
+    const float4 q0 = node4[0], q1 = node4[1], q2 = node4[2], q3 = node4[3];
+    const bool two = (__float_as_uint(q3.w) & 0x3fffffffu) != 0u;  // a root with one leaf has no right child
+    const float mn[3] = {two ? fminf(q0.x, q2.x) : q0.x, two ? fminf(q0.y, q2.y) : q0.y,
+                         two ? fminf(q0.z, q2.z) : q0.z};
+    const float mx[3] = {two ? fmaxf(q1.x, q3.x) : q1.x, two ? fmaxf(q1.y, q3.y) : q1.y,
+                         two ? fmaxf(q1.z, q3.z) : q1.z};
+    const uint64_t begin = (uint64_t)blockIdx.x * per_block;
+    const uint64_t end = begin + per_block < n ? begin + per_block : n;
+    for (uint64_t base = begin; base < end; base += kBinRange) {
+        RawRay raw[kCountPer];
+#pragma unroll
+        for (int k = 0; k < kCountPer; ++k) {
+            const uint64_t j = base + threadIdx.x + (uint64_t)k * kCountBlock;
+            if (j < end) raw[k] = load_raw_ray(rays, j);
+        }
+#pragma unroll
+        for (int k = 0; k < kCountPer; ++k) {
+            const uint64_t j = base + threadIdx.x + (uint64_t)k * kCountBlock;
+            if (j < end) {
+                const uint32_t cell = ray_cell(mn, mx, lane_ray(raw[k]));
+                const uint32_t rank = atomicAdd(&cnt[cell], 1u);
+                keyrank[j] = cell | (rank << kRayBinKeyBits);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < kRayBins; c += kCountBlock) hist[(uint64_t)c * gridDim.x + blockIdx.x] = cnt[c];
+}
+
+// One wave per cell: exclusive prefix sum over the ranges of that cell's
+// counts (hist[cell * blocks + range]), in place, 64 ranges at a time; the
+// cell's total to totals[cell].
+__global__ __launch_bounds__(64) void bin_scan_kernel(uint32_t *__restrict__ hist, uint32_t blocks,
+                                                      uint32_t *__restrict__ totals) {
+    uint32_t *h = hist + (uint64_t)blockIdx.x * blocks;
+    const uint32_t lane = threadIdx.x;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < blocks; base += 64u) {
+        const uint32_t k = base + lane;
+        const uint32_t v = k < blocks ? h[k] : 0u;
+        uint32_t incl = v;
+#pragma unroll
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d);
+            if (lane >= d) incl += y;
+        }
+        if (k < blocks) h[k] = carry + incl - v;
+        carry += __shfl(incl, 63);
+    }
+    if (lane == 0) totals[blockIdx.x] = carry;
+}
+
+// One workgroup: totals -> each cell's first sorted position, in place.
+__global__ __launch_bounds__(kBinBlock) void bin_cells_kernel(uint32_t *__restrict__ totals) {
+    static_assert(kRayBins % kBinBlock == 0, "cells split evenly over the workgroup");
+    constexpr uint32_t kPer = kRayBins / kBinBlock;
+    __shared__ uint32_t wave_sum[kBinBlock / 64];
+    const uint32_t t = threadIdx.x, lane = t % 64u, wave = t / 64u;
+    uint32_t c[kPer], s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        c[k] = totals[kPer * t + k];
+        s += c[k];
+    }
+    uint32_t incl = s;
+#pragma unroll
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63u) wave_sum[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - s;
+    for (uint32_t w = 0; w < wave; ++w) run += wave_sum[w];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        totals[kPer * t + k] = run;
+        run += c[k];
+    }
+}
+
+__global__ __launch_bounds__(kBinBlock) void bin_scatter_kernel(const uint32_t *__restrict__ keyrank,
+                                                                const uint32_t *__restrict__ hist,
+                                                                const uint32_t *__restrict__ first, uint64_t n,
+                                                                uint64_t per_block, uint32_t blocks,
+                                                                uint32_t *__restrict__ order) {
+    const uint64_t j = (uint64_t)blockIdx.x * kBinBlock + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t kr = keyrank[j];
+    const uint32_t cell = kr & (kRayBins - 1u), rank = kr >> kRayBinKeyBits;
+    order[first[cell] + hist[(uint64_t)cell * blocks + j / per_block] + rank] = (uint32_t)j;
+}
+
 // Calls f(std::integral_constant<int, BLOCK>, std::integral_constant<int, REG>)
 // for a tuning's block size (64, 128, 256) and register stack depth (0, 1, 2, 4).
 template <class F>
@@ -930,16 +1095,16 @@ hipError_t launch_cast_rays_culling(const DeviceMesh &m, const pq_ray *rays, uin
     const dim3 grid((uint32_t)((n + B - 1) / B)), block(B);
     if (culling == 1 && counts)
         hipLaunchKernelGGL((cast_rays_kernel<true, 1, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts);
+                           m.num_triangles, rays, n, max_toi, hits, counts, nullptr);
     else if (culling == 1)
         hipLaunchKernelGGL((cast_rays_kernel<false, 1, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts);
+                           m.num_triangles, rays, n, max_toi, hits, counts, nullptr);
     else if (counts)
         hipLaunchKernelGGL((cast_rays_kernel<true, 2, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts);
+                           m.num_triangles, rays, n, max_toi, hits, counts, nullptr);
     else
         hipLaunchKernelGGL((cast_rays_kernel<false, 2, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts);
+                           m.num_triangles, rays, n, max_toi, hits, counts, nullptr);
     return hipGetLastError();
 }
 
@@ -952,6 +1117,7 @@ hipError_t launch_single(const DeviceMesh &m, const pq_ray *rays, uint64_t n, fl
     uint32_t mapping = tu.mapping;
     if (mapping == PQ_RAY_MAPPING_WAVE && !wave_mapping_fits(m.depth)) mapping = PQ_RAY_MAPPING_PERSISTENT;
     if (mapping == PQ_RAY_MAPPING_PERSISTENT && single_leaf) mapping = PQ_RAY_MAPPING_STATIC;
+    if (mapping == PQ_RAY_MAPPING_BINNED && (single_leaf || n > 0xffffffffull)) mapping = PQ_RAY_MAPPING_STATIC;
     if (mapping == PQ_RAY_MAPPING_WAVE) {
         unsigned long long *queue = nullptr;
         hipError_t e = stream_queue(stream, &queue);
@@ -980,8 +1146,26 @@ hipError_t launch_single(const DeviceMesh &m, const pq_ray *rays, uint64_t n, fl
             hipLaunchKernelGGL((cast_rays_persistent_kernel<COUNT, B, R>), dim3((uint32_t)blocks), dim3(B), lds,
                                stream, m.nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, queue, chunk);
         } else {
+            const uint32_t *order = nullptr;
+            if (mapping == PQ_RAY_MAPPING_BINNED) {
+                uint64_t blocks = (n + kBinRange - 1) / kBinRange;
+                if (blocks > kRayBinMaxBlocks) blocks = kRayBinMaxBlocks;
+                const uint64_t per_block = (n + blocks - 1) / blocks;
+                RayBinSpace ws{};
+                hipError_t e = ray_bin_workspace(stream, n, (uint32_t)blocks, &ws);
+                if (e != hipSuccess) return e;
+                hipLaunchKernelGGL(bin_count_kernel, dim3((uint32_t)blocks), dim3(kCountBlock), 0, stream, m.nodes,
+                                   rays, n, per_block, ws.hist, ws.keyrank);
+                hipLaunchKernelGGL(bin_scan_kernel, dim3(kRayBins), dim3(64), 0, stream, ws.hist, (uint32_t)blocks,
+                                   ws.totals);
+                hipLaunchKernelGGL(bin_cells_kernel, dim3(1), dim3(kBinBlock), 0, stream, ws.totals);
+                hipLaunchKernelGGL(bin_scatter_kernel, dim3((uint32_t)((n + kBinBlock - 1) / kBinBlock)),
+                                   dim3(kBinBlock), 0, stream, ws.keyrank, ws.hist, ws.totals, n, per_block,
+                                   (uint32_t)blocks, ws.order);
+                order = ws.order;
+            }
             hipLaunchKernelGGL((cast_rays_kernel<COUNT, 0, B, R>), dim3((uint32_t)((n + B - 1) / B)), dim3(B), lds,
-                               stream, m.nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts);
+                               stream, m.nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, order);
         }
         return hipGetLastError();
     });

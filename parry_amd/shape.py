@@ -111,7 +111,8 @@ class TriMesh:
     # -- tuning ----------------------------------------------------------
     def set_tuning(self, mapping: str = "persistent", chunk: int = 0, blocks_per_cu: int = 0, block_size: int = 256,
                    stack_regs: int = 0) -> None:
-        """Selects the ray kernel's mapping ("static", "persistent" or "wave"), workgroup size and the
+        """Selects the ray kernel's mapping ("static", "persistent", "wave" or "binned": static lanes
+        over the rays sorted by the cell where they enter the Bvh's root box), workgroup size and the
         stack entries kept in registers; results never change."""
         t = _native.ray_tuning(mapping, chunk, blocks_per_cu, block_size, stack_regs)
         _native.check(_native.lib().pq_trimesh_set_tuning(self._h, ctypes.byref(t)))

@@ -10,7 +10,8 @@ import os
 import numpy as np
 import pytest
 
-from parry_amd import RAY_HIT_DTYPE, TriMesh, scenes
+from parry_amd import RAY_HIT_DTYPE, TriMesh, TriMeshBatch, scenes
+from parry_amd._native import NativeError
 
 pytestmark = pytest.mark.gpu
 
@@ -27,12 +28,15 @@ MAPPINGS = [
     {"mapping": "static", "block_size": 64, "stack_regs": 1},
     {"mapping": "static", "block_size": 128, "stack_regs": 2},
     {"mapping": "persistent", "block_size": 64, "stack_regs": 4, "chunk": 64},
+    {"mapping": "binned"},
+    {"mapping": "binned", "block_size": 64, "stack_regs": 2},
 ]
 # Every setting the tuner can pick, for the traversal-count test.
 ALL_MAPPINGS = (
     [{"mapping": "static", "block_size": b, "stack_regs": r} for b in (64, 128, 256) for r in (0, 1, 2, 4)]
     + [{"mapping": "persistent", "block_size": b, "stack_regs": r, "chunk": c}
        for b in (64, 256) for r in (0, 2) for c in (64, 256)]
+    + [{"mapping": "binned", "block_size": b, "stack_regs": r} for b in (64, 128, 256) for r in (0, 2)]
     + [{"mapping": "persistent", "blocks_per_cu": 4}, {"mapping": "wave", "chunk": 1}, {"mapping": "wave", "chunk": 4}]
 )
 
@@ -150,7 +154,7 @@ def test_edge_cases(orc):
         _assert_parity(hits, om.cast_rays(rays[perm], 10.0), f"edge shuffled {mapping}")
     # batch sizes around the persistent chunk and wave sizes
     for n_small in (1, 63, 64, 65, 127, 129, 1000):
-        for mapping in ("persistent", "wave"):
+        for mapping in ("persistent", "wave", "binned"):
             hits = _cast_gpu_host(mesh, rays[:n_small], 10.0, mapping)
             _assert_parity(hits, om.cast_rays(rays[:n_small], 10.0), f"n={n_small} {mapping}")
 
@@ -172,6 +176,30 @@ def test_small_trees_and_duplicates(orc):
     rays = scenes.rays_into_unit_ball(50000, seed=13)
     hits = _cast_gpu_host(TriMesh(v, f), rays, 10.0)
     _assert_parity(hits, orc.OracleTriMesh(v, f).cast_rays(rays, 10.0), "duplicates")
+
+
+def test_binned_mapping_buffers_and_validation(orc):
+    """The binned mapping's sort buffers grow with the call and its counters
+    return to zero: a large, a small and again a large call on one stream all
+    equal the static mapping; the batch refuses the mapping."""
+    import torch
+
+    v, f = scenes.noisy_icosphere(4, seed=31)
+    mesh = TriMesh(v, f)
+    om = orc.OracleTriMesh(v, f)
+    for n in (50000, 777, 120000, 50000):
+        rays = scenes.rays_into_unit_ball(n, seed=n)
+        print(f"case: binned n={n}", flush=True)
+        hits = _cast_gpu_host(mesh, rays, 10.0, {"mapping": "binned", "block_size": 128})
+        _assert_parity(hits, om.cast_rays(rays, 10.0), f"binned n={n}")
+        static = _cast_gpu_host(mesh, rays, 10.0, "static")
+        assert hits.tobytes() == static.tobytes()
+    batch = TriMeshBatch([mesh], [10])
+    with pytest.raises(NativeError):
+        batch.set_tuning("binned")
+    with pytest.raises(NativeError):
+        mesh.set_tuning("binned", block_size=96)
+    torch.cuda.synchronize()
 
 
 def test_traversal_counts_match_oracle(orc):
