@@ -1193,6 +1193,8 @@ def main() -> int:
     ap.add_argument("--batch-only", action="store_true", help="profiling aid: run only the batched multi-mesh leg")
     ap.add_argument("--batch-tuning", default="", metavar="MAPPING:CHUNK:BLOCKS_PER_CU:BLOCK:REGS",
                     help="profiling aid: use this ray tuning for the batched leg instead of the autotuner's")
+    ap.add_argument("--ray-tuning", default="", metavar="MAPPING:CHUNK:BLOCKS_PER_CU:BLOCK:REGS",
+                    help="profiling aid: use this ray tuning for the headline leg instead of the autotuner's")
     ap.add_argument("--mapping-sweep", action="store_true",
                     help="profiling aid: time every ray-kernel mapping on four workloads and print them")
     args = ap.parse_args()
@@ -1264,6 +1266,15 @@ def main() -> int:
     sum_k = int(counts[:, 1].sum(dtype=torch.int64).item())
     alg_bytes = n * (24 + 20) + 64 * sum_w + 48 * sum_k
 
+    # The autotuner picks the mapping (it keeps the fastest in the mesh), untimed.
+    if args.ray_tuning:
+        m_, c_, bpc_, blk_, regs_ = args.ray_tuning.split(":")
+        mesh.set_tuning(m_, int(c_), int(bpc_), int(blk_), int(regs_))
+        tuned, tuned_ms = mesh.tuning(), float("nan")
+    else:
+        tuned, tuned_ms = mesh.autotune(rays_d, MAX_TOI, stream=stream)
+    log(f"ray tuning: {tuned} (autotuner's time {tuned_ms:.4f} ms)")
+
     for _ in range(args.warmup):
         mesh.cast_local_rays_and_get_normal(rays_d, MAX_TOI, True, out=out, stream=stream)
     torch.cuda.synchronize()
@@ -1283,10 +1294,14 @@ def main() -> int:
     gpu_feat = feat.cpu().numpy().view(np.uint32)
     host_hits = out.cpu().numpy()
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
-    # The tuner's pick over every mapping (untimed for the line; the line keeps
-    # the default static mapping, 256-lane workgroups, stack in LDS).
-    tuned, tuned_ms = mesh.autotune(rays_d, MAX_TOI)
-    mesh.set_tuning("static")
+    binned = tuned["mapping"] == "binned"
+    kernels = (["bin_count_kernel", "bin_scan_kernel", "bin_cells_kernel", "bin_scatter_kernel"] if binned else []) + [
+        f"cast_rays_kernel<false, 0, {tuned['block_size'] or 256}, {tuned['stack_regs']}>"]
+    traffic_entry = "cast_rays_step_binned" if binned else "cast_rays_kernel"
+    # the committed counters were taken at one tuning: used for the same mapping
+    traffic_tuning = measured_traffic(traffic_entry, "tuning")
+    traffic = (measured_traffic(traffic_entry)
+               if traffic_tuning is None or traffic_tuning.get("mapping") == tuned["mapping"] else None)
     rec = {
         "metric": METRIC,
         "value": round(n * args.steps / wall / 1e6, 3),
@@ -1307,8 +1322,9 @@ def main() -> int:
             "mesh_seed": 1,
             "ray_seed": 2,
             "parallelism": "single GPU",
-            "mapping": "static, 256-lane workgroups, traversal stack in LDS",
-            "autotuner": {"chosen": tuned, "ms_per_launch": round(tuned_ms, 5)},
+            "mapping": tuned,
+            "autotuner": {"chosen": tuned, "ms_per_launch": None if tuned_ms != tuned_ms else round(tuned_ms, 5),
+                          "forced": bool(args.ray_tuning)},
         },
         "roofline": {
             "bound": "hbm",
@@ -1316,9 +1332,12 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("cast_rays_kernel"),
-            "kernel": "cast_rays_kernel",
+            "traffic": traffic,
+            "traffic_entry": traffic_entry,
+            "traffic_tuning": traffic_tuning,
+            "kernel": " + ".join(kernels),
             "kernel_avg_ms": round(kern_ms, 5),
+            "kernel_avg_covers": "every kernel of one cast call (HIP events around the timed calls)",
             "algorithmic_bytes_per_launch": alg_bytes,
             "mean_wide_nodes_per_ray": round(sum_w / n, 4),
             "mean_triangles_per_ray": round(sum_k / n, 4),
