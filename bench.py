@@ -364,8 +364,8 @@ def queries_leg(args, log):
     """Two more TriMesh queries on the C0 mesh (noisy icosphere s=6, 81,920 triangles):
     point projection (TriMesh::project_local_point, solid = false) of 1,048,576 points
     uniform in [-1.5, 1.5]^3 (seed 6), and the C0 rays cast with back faces culled
-    (TriMesh::cast_local_ray_with_culling); one step = one launch over every point
-    (ray).  Algorithmic bytes per query: 12 (point) + 20 (record), or 24 (ray) + 20
+    (TriMesh::cast_local_ray_with_culling); one step = one call over every point
+    (ray): for the culled casts with the binned mapping, the sort and one cast launch.  Algorithmic bytes per query: 12 (point) + 20 (record), or 24 (ray) + 20
     (hit), plus 64 per wide node fetched and 48 per triangle tested, the counts
     coming from the kernels' instrumented launch."""
     import numpy as np
@@ -415,6 +415,11 @@ def queries_leg(args, log):
                          "points uniform in [-1.5,1.5]^3, seed 6",
              "roofline": roofline("project_points_kernel", 32, p_gpu)}
     mesh.cast_local_rays_with_culling(d_rays, MAX_TOI, "ignore_backfaces", out=out, counts=counts)
+    mesh.set_tuning("static")
+    s_wall, s_gpu = timed(lambda: mesh.cast_local_rays_with_culling(d_rays, MAX_TOI, "ignore_backfaces", out=out,
+                                                                    stream=stream))
+    static_hits = out.cpu().numpy().copy()
+    mesh.set_tuning("binned")  # the C0 rays' mapping: the culled casts sort them too
     c_wall, c_gpu = timed(lambda: mesh.cast_local_rays_with_culling(d_rays, MAX_TOI, "ignore_backfaces", out=out,
                                                                     stream=stream))
     hits = out.cpu().numpy().copy()
@@ -422,7 +427,10 @@ def queries_leg(args, log):
              "ms_per_step": round(c_wall, 4), "gpu_ms_per_step": round(c_gpu, 4),
              "workload": "TriMesh::cast_local_ray_with_culling(IgnoreBackfaces): the C0 mesh and its 1048576 rays, "
                          "max_toi=10",
-             "roofline": roofline("cast_rays_kernel<false, 1>", 44, c_gpu)}
+             "mapping": "binned (the sort, then cast_rays_kernel<false, 1, 256, 0> over the sorted order)",
+             "static_mapping": {"ms_per_step": round(s_wall, 4), "gpu_ms_per_step": round(s_gpu, 4),
+                                "identical": bool(np.array_equal(static_hits.view(np.uint32), hits.view(np.uint32)))},
+             "roofline": roofline("bin_*_kernel + cast_rays_kernel<false, 1, 256, 0>", 44, c_gpu)}
     if not args.no_cpu:
         from oracle import oracle
 

@@ -1038,6 +1038,25 @@ __global__ __launch_bounds__(kBinBlock) void bin_scatter_kernel(const uint32_t *
     order[first[cell] + hist[(uint64_t)cell * blocks + j / per_block] + rank] = (uint32_t)j;
 }
 
+// The binned mapping's sort: leaves in *order the ray index at each sorted position.
+hipError_t launch_ray_bins(const DeviceMesh &m, const pq_ray *rays, uint64_t n, hipStream_t stream,
+                           const uint32_t **order) {
+    uint64_t blocks = (n + kBinRange - 1) / kBinRange;
+    if (blocks > kRayBinMaxBlocks) blocks = kRayBinMaxBlocks;
+    const uint64_t per_block = (n + blocks - 1) / blocks;
+    RayBinSpace ws{};
+    hipError_t e = ray_bin_workspace(stream, n, (uint32_t)blocks, &ws);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bin_count_kernel, dim3((uint32_t)blocks), dim3(kCountBlock), 0, stream, m.nodes, rays, n,
+                       per_block, ws.hist, ws.keyrank);
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(kRayBins), dim3(64), 0, stream, ws.hist, (uint32_t)blocks, ws.totals);
+    hipLaunchKernelGGL(bin_cells_kernel, dim3(1), dim3(kBinBlock), 0, stream, ws.totals);
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3((uint32_t)((n + kBinBlock - 1) / kBinBlock)), dim3(kBinBlock), 0,
+                       stream, ws.keyrank, ws.hist, ws.totals, n, per_block, (uint32_t)blocks, ws.order);
+    *order = ws.order;
+    return hipGetLastError();
+}
+
 // Calls f(std::integral_constant<int, BLOCK>, std::integral_constant<int, REG>)
 // for a tuning's block size (64, 128, 256) and register stack depth (0, 1, 2, 4).
 template <class F>
@@ -1093,18 +1112,25 @@ hipError_t launch_cast_rays_culling(const DeviceMesh &m, const pq_ray *rays, uin
     constexpr int B = 256;
     const size_t lds = stack_lds_bytes(m.depth, B, 0);
     const dim3 grid((uint32_t)((n + B - 1) / B)), block(B);
+    // the mesh's binned mapping sorts these rays too; any other mapping runs static lanes
+    const bool single_leaf = m.num_nodes == 1 && m.num_triangles == 1;
+    const uint32_t *order = nullptr;
+    if (m.tuning.mapping == PQ_RAY_MAPPING_BINNED && !single_leaf && n <= 0xffffffffull) {
+        const hipError_t e = launch_ray_bins(m, rays, n, stream, &order);
+        if (e != hipSuccess) return e;
+    }
     if (culling == 1 && counts)
         hipLaunchKernelGGL((cast_rays_kernel<true, 1, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts, nullptr);
+                           m.num_triangles, rays, n, max_toi, hits, counts, order);
     else if (culling == 1)
         hipLaunchKernelGGL((cast_rays_kernel<false, 1, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts, nullptr);
+                           m.num_triangles, rays, n, max_toi, hits, counts, order);
     else if (counts)
         hipLaunchKernelGGL((cast_rays_kernel<true, 2, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts, nullptr);
+                           m.num_triangles, rays, n, max_toi, hits, counts, order);
     else
         hipLaunchKernelGGL((cast_rays_kernel<false, 2, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts, nullptr);
+                           m.num_triangles, rays, n, max_toi, hits, counts, order);
     return hipGetLastError();
 }
 
@@ -1148,21 +1174,8 @@ hipError_t launch_single(const DeviceMesh &m, const pq_ray *rays, uint64_t n, fl
         } else {
             const uint32_t *order = nullptr;
             if (mapping == PQ_RAY_MAPPING_BINNED) {
-                uint64_t blocks = (n + kBinRange - 1) / kBinRange;
-                if (blocks > kRayBinMaxBlocks) blocks = kRayBinMaxBlocks;
-                const uint64_t per_block = (n + blocks - 1) / blocks;
-                RayBinSpace ws{};
-                hipError_t e = ray_bin_workspace(stream, n, (uint32_t)blocks, &ws);
+                const hipError_t e = launch_ray_bins(m, rays, n, stream, &order);
                 if (e != hipSuccess) return e;
-                hipLaunchKernelGGL(bin_count_kernel, dim3((uint32_t)blocks), dim3(kCountBlock), 0, stream, m.nodes,
-                                   rays, n, per_block, ws.hist, ws.keyrank);
-                hipLaunchKernelGGL(bin_scan_kernel, dim3(kRayBins), dim3(64), 0, stream, ws.hist, (uint32_t)blocks,
-                                   ws.totals);
-                hipLaunchKernelGGL(bin_cells_kernel, dim3(1), dim3(kBinBlock), 0, stream, ws.totals);
-                hipLaunchKernelGGL(bin_scatter_kernel, dim3((uint32_t)((n + kBinBlock - 1) / kBinBlock)),
-                                   dim3(kBinBlock), 0, stream, ws.keyrank, ws.hist, ws.totals, n, per_block,
-                                   (uint32_t)blocks, ws.order);
-                order = ws.order;
             }
             hipLaunchKernelGGL((cast_rays_kernel<COUNT, 0, B, R>), dim3((uint32_t)((n + B - 1) / B)), dim3(B), lds,
                                stream, m.nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, order);

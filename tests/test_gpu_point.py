@@ -87,13 +87,15 @@ def test_culling_matches_oracle(orc):
     inside = scenes.rays_into_unit_ball(60000, seed=9, origin_radius=0.5)  # start inside: back faces first
     rays = np.concatenate([outside, inside]).astype(np.float32)
     for code, mode in ((1, "ignore_backfaces"), (2, "ignore_frontfaces")):
-        print(f"case: culling {mode}, {rays.shape[0]} rays", flush=True)
         ref = om.cast_rays_with_culling(rays, 10.0, code)
-        toi, nrm, feat = _cast_culled(mesh, rays, mode)
-        assert np.array_equal(feat, ref["feature"]), f"{mode}: {np.count_nonzero(feat != ref['feature'])} features differ"
-        hit = feat != 0xFFFFFFFF
-        assert np.array_equal(toi[hit], ref["toi"][hit]) and np.array_equal(nrm[hit], ref["normal"][hit])
-        print(f"  {int(hit.sum())} hits")
+        for mapping in ("static", "binned"):  # the mesh's binned mapping sorts culled casts too
+            print(f"case: culling {mode}, {rays.shape[0]} rays, {mapping}", flush=True)
+            mesh.set_tuning(mapping)
+            toi, nrm, feat = _cast_culled(mesh, rays, mode)
+            assert np.array_equal(feat, ref["feature"]), f"{mode}: {np.count_nonzero(feat != ref['feature'])} features differ"
+            hit = feat != 0xFFFFFFFF
+            assert np.array_equal(toi[hit], ref["toi"][hit]) and np.array_equal(nrm[hit], ref["normal"][hit])
+            print(f"  {int(hit.sum())} hits")
 
 
 def test_culling_reference_kat():
@@ -124,10 +126,12 @@ def test_traversal_counts_match_oracle(orc):
     ref = om.project_points(pts, False)
     assert (int(c[0]), int(c[1])) == (ref["W"], ref["K"])
     rays = scenes.rays_into_unit_ball(65536, seed=12)
-    counts = torch.zeros((rays.shape[0], 2), dtype=torch.int32, device="cuda")
-    print("case: culling counts, 65536 rays", flush=True)
-    mesh.cast_local_rays_with_culling(torch.from_numpy(rays).cuda(), 10.0, "ignore_frontfaces", counts=counts)
-    torch.cuda.synchronize()
-    c = counts.cpu().numpy().astype(np.int64).sum(axis=0)
     ref = om.cast_rays_with_culling(rays, 10.0, 2)
-    assert (int(c[0]), int(c[1])) == (ref["W"], ref["K"])
+    for mapping in ("static", "binned"):
+        counts = torch.zeros((rays.shape[0], 2), dtype=torch.int32, device="cuda")
+        print(f"case: culling counts, 65536 rays, {mapping}", flush=True)
+        mesh.set_tuning(mapping)
+        mesh.cast_local_rays_with_culling(torch.from_numpy(rays).cuda(), 10.0, "ignore_frontfaces", counts=counts)
+        torch.cuda.synchronize()
+        c = counts.cpu().numpy().astype(np.int64).sum(axis=0)
+        assert (int(c[0]), int(c[1])) == (ref["W"], ref["K"])
