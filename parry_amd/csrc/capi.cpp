@@ -1,5 +1,6 @@
 // capi.cpp -- extern "C" entry points declared in include/parry_mi355x.h.
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -262,6 +263,12 @@ int pq_trimesh_create(const float *vertices, uint32_t num_vertices, const uint32
     m->dev.num_nodes = (uint32_t)dev_nodes.size();
     m->dev.num_triangles = num_triangles;
     m->dev.depth = depth;
+    bool finite = true;
+    for (size_t k = 0; k < 3 * (size_t)num_vertices && finite; ++k) finite = std::isfinite(vertices[k]);
+    for (const pq_bvh_node_wide &w : dev_nodes)
+        for (const pq_bvh_node *c : {&w.left, &w.right})
+            for (int d = 0; d < 3; ++d) finite = finite && std::isfinite(c->mins[d]) && std::isfinite(c->maxs[d]);
+    m->dev.finite = finite ? 1u : 0u;
     m->dev.tuning = pq_ray_tuning{PQ_RAY_MAPPING_STATIC, 128, 0, 256, 0, 0};
     *out = m;
     return PQ_OK;

@@ -108,7 +108,7 @@ int pq_trimesh_batch_create(const pq_trimesh *const *meshes, uint32_t num_meshes
         const DeviceMesh &m = meshes[i]->dev;
         // A root whose right child has no leaf holds the mesh's single triangle.
         const bool single = (meshes[i]->host_nodes[0].right.data & 0x3fffffffu) == 0u;
-        bm[i] = BatchMesh{m.nodes, m.tris, m.num_triangles, single ? 1u : 0u, m.num_nodes, 0u};
+        bm[i] = BatchMesh{m.nodes, m.tris, m.num_triangles, single ? 1u : 0u, m.num_nodes, m.finite};
         max_depth = std::max(max_depth, m.depth);
     }
     auto *b = new pq_trimesh_batch();

@@ -65,6 +65,7 @@ struct DeviceMesh {
     uint32_t num_nodes;
     uint32_t num_triangles;
     uint32_t depth;
+    uint32_t finite;  // every vertex and every node bound is finite
     pq_ray_tuning tuning;
 };
 
@@ -75,7 +76,7 @@ struct BatchMesh {
     uint32_t ntri;
     uint32_t single_leaf;
     uint32_t num_nodes;
-    uint32_t pad;
+    uint32_t finite;  // DeviceMesh::finite
 };
 
 struct DeviceBatch {

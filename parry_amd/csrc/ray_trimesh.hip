@@ -55,10 +55,12 @@ struct LaneRay {
 // tmin only grows and tmax only shrinks, testing once after the three axes
 // gives the same answer, so the three axes are evaluated branch-free.
 //
-// REGULAR: the caller guarantees dx, dy and dz are all nonzero (NaN counts as
-// nonzero, as in the test below).  Then every axis takes the slab branch, the
-// degenerate-axis terms are identities and are left out; the operations that
-// remain are the same, so the cost is the same bit for bit.
+// REGULAR: the caller guarantees dx, dy and dz are nonzero, the origin and
+// 1/dir finite, and the node bounds finite.  Then every axis takes the slab
+// branch and the degenerate-axis terms are identities, left out; and no
+// product (finite bound - finite origin) * finite inverse is NaN, so ordering
+// the two slab ends by select equals taking min and max (they can differ only
+// in the sign of a zero end, which no comparison of the cost distinguishes).
 template <bool REGULAR = false>
 __device__ __forceinline__ float slab_cost(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
                                            const LaneRay &r, float max_toi) {
@@ -70,10 +72,14 @@ __device__ __forceinline__ float slab_cost(float mnx, float mny, float mnz, floa
     {                                                                 \
         const float n = (MN - O) * INV;                                \
         const float f = (MX - O) * INV;                                \
-        const bool s = n > f;                                          \
-        float lo = s ? f : n;                                          \
-        float hi = s ? n : f;                                          \
-        if constexpr (!REGULAR) {                                      \
+        float lo, hi;                                                  \
+        if constexpr (REGULAR) {                                       \
+            lo = fminf(n, f);                                          \
+            hi = fmaxf(n, f);                                          \
+        } else {                                                       \
+            const bool s = n > f;                                      \
+            lo = s ? f : n;                                            \
+            hi = s ? n : f;                                            \
             const unsigned z = (unsigned)(D == 0.0f);                  \
             miss |= z & ((unsigned)(O < MN) | (unsigned)(O > MX));     \
             lo = z ? -INFINITY : lo;                                   \
@@ -260,9 +266,11 @@ __device__ __forceinline__ LaneRay lane_ray(const RawRay &w) {
 
 __device__ __forceinline__ LaneRay load_ray(const pq_ray *rays, uint64_t i) { return lane_ray(load_raw_ray(rays, i)); }
 
-// No zero direction component: slab_cost<true> applies to this ray.
+// No zero direction component, a finite origin and finite inverses:
+// slab_cost<true> applies to this ray on a mesh with finite bounds.
 __device__ __forceinline__ bool ray_is_regular(const LaneRay &r) {
-    return r.dx != 0.0f && r.dy != 0.0f && r.dz != 0.0f;
+    return r.dx != 0.0f && r.dy != 0.0f && r.dz != 0.0f && isfinite(r.ox) && isfinite(r.oy) && isfinite(r.oz) &&
+           isfinite(r.ix) && isfinite(r.iy) && isfinite(r.iz);
 }
 
 __device__ __forceinline__ void start_traversal(Traversal &t, float max_toi) {
@@ -392,7 +400,7 @@ __global__ __launch_bounds__(BLOCK) void cast_rays_kernel(const pq_bvh_node_wide
                                                           const pq_ray *__restrict__ rays, uint64_t n, float max_toi,
                                                           pq_ray_hit *__restrict__ hits,
                                                           uint32_t *__restrict__ counts,
-                                                          const uint32_t *__restrict__ order) {
+                                                          const uint32_t *__restrict__ order, uint32_t finite) {
     extern __shared__ uint32_t lds_stack[];
     const uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= n) return;
@@ -413,7 +421,7 @@ __global__ __launch_bounds__(BLOCK) void cast_rays_kernel(const pq_bvh_node_wide
         n_wide = 1;
         if (slab_cost(q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, r, max_toi) < max_toi)
             visit_leaf<true, CULL>(tri4, __float_as_uint(q0.w), r, t, n_tri);
-    } else if (__all(ray_is_regular(r))) {  // the usual case: no zero direction component in the wave
+    } else if (finite && __all(ray_is_regular(r))) {  // the usual case: the whole wave takes the short test
         while (!find_best_step<COUNT, CULL, true>(node4, tri4, r, t, stk, n_wide, n_tri)) {
         }
     } else {
@@ -574,6 +582,7 @@ __device__ __forceinline__ uint32_t lane_u(uint32_t v, uint32_t l) {
 // find_best for one ray by the whole wave; every lane returns the hit.
 template <bool COUNT>
 __device__ __forceinline__ pq_ray_hit wave_cast_one(const float4 *node4, uint32_t num_nodes, const float4 *tri4,
+                                                    uint32_t finite,
                                                     uint32_t ntri, bool single_leaf, const LaneRay &r,
                                                     float max_toi, uint32_t &n_wide, uint32_t &n_tri) {
     float best = max_toi;
@@ -597,7 +606,7 @@ __device__ __forceinline__ pq_ray_hit wave_cast_one(const float4 *node4, uint32_
         }
     } else {
         uint32_t stk = 0, sp = 0, curr = 0, base = 0;
-        const bool regular = ray_is_regular(r);  // the same ray in every lane: a uniform branch
+        const bool regular = finite && ray_is_regular(r);  // the same ray in every lane: a uniform branch
         auto fetch = [&](uint32_t at) {
             return regular ? fetch_window<true>(node4, num_nodes, tri4, at, r, max_toi)
                            : fetch_window<false>(node4, num_nodes, tri4, at, r, max_toi);
@@ -691,7 +700,8 @@ __global__ __launch_bounds__(64) void cast_rays_wave_kernel(const pq_bvh_node_wi
                                                             uint32_t ntri, const pq_ray *__restrict__ rays, uint64_t n,
                                                             float max_toi, pq_ray_hit *__restrict__ hits,
                                                             uint32_t *__restrict__ counts,
-                                                            unsigned long long *__restrict__ queue, uint32_t chunk) {
+                                                            unsigned long long *__restrict__ queue, uint32_t chunk,
+                                                            uint32_t finite) {
     const float4 *node4 = reinterpret_cast<const float4 *>(nodes);
     const float4 *tri4 = reinterpret_cast<const float4 *>(tris);
     const bool single_leaf = (__float_as_uint(node4[3].w) & 0x3fffffffu) == 0u;
@@ -700,7 +710,8 @@ __global__ __launch_bounds__(64) void cast_rays_wave_kernel(const pq_bvh_node_wi
         for (uint64_t i = begin; i < end; ++i) {
             const LaneRay r = load_ray(rays, i);
             uint32_t n_wide, n_tri;
-            const pq_ray_hit h = wave_cast_one<COUNT>(node4, num_nodes, tri4, ntri, single_leaf, r, max_toi, n_wide,
+            const pq_ray_hit h = wave_cast_one<COUNT>(node4, num_nodes, tri4, finite, ntri, single_leaf, r, max_toi,
+                                                      n_wide,
                                                       n_tri);
             if (__lane_id() == 0) {
                 hits[i] = h;
@@ -744,7 +755,7 @@ __device__ __forceinline__ void cast_one(const BatchMesh &bm, const LaneRay &r, 
             visit_leaf<true>(tri4, __float_as_uint(q0.w), r, t, n_tri);
         return;
     }
-    if (__all(ray_is_regular(r))) {
+    if (bm.finite && __all(ray_is_regular(r))) {
         while (!find_best_step<COUNT, 0, true>(node4, tri4, r, t, stk, n_wide, n_tri)) {
         }
     } else {
@@ -870,7 +881,7 @@ __global__ __launch_bounds__(64) void cast_rays_batch_wave_kernel(const BatchMes
             const LaneRay r = load_ray(rays, i);
             uint32_t n_wide, n_tri;
             const pq_ray_hit h = wave_cast_one<COUNT>(reinterpret_cast<const float4 *>(bm.nodes), bm.num_nodes,
-                                                      reinterpret_cast<const float4 *>(bm.tris), bm.ntri,
+                                                      reinterpret_cast<const float4 *>(bm.tris), bm.finite, bm.ntri,
                                                       bm.single_leaf != 0u, r, max_toi, n_wide, n_tri);
             if (__lane_id() == 0) {
                 hits[i] = h;
@@ -1121,16 +1132,16 @@ hipError_t launch_cast_rays_culling(const DeviceMesh &m, const pq_ray *rays, uin
     }
     if (culling == 1 && counts)
         hipLaunchKernelGGL((cast_rays_kernel<true, 1, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts, order);
+                           m.num_triangles, rays, n, max_toi, hits, counts, order, m.finite);
     else if (culling == 1)
         hipLaunchKernelGGL((cast_rays_kernel<false, 1, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts, order);
+                           m.num_triangles, rays, n, max_toi, hits, counts, order, m.finite);
     else if (counts)
         hipLaunchKernelGGL((cast_rays_kernel<true, 2, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts, order);
+                           m.num_triangles, rays, n, max_toi, hits, counts, order, m.finite);
     else
         hipLaunchKernelGGL((cast_rays_kernel<false, 2, B, 0>), grid, block, lds, stream, m.nodes, m.tris,
-                           m.num_triangles, rays, n, max_toi, hits, counts, order);
+                           m.num_triangles, rays, n, max_toi, hits, counts, order, m.finite);
     return hipGetLastError();
 }
 
@@ -1154,7 +1165,8 @@ hipError_t launch_single(const DeviceMesh &m, const pq_ray *rays, uint64_t n, fl
                                    &blocks)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(cast_rays_wave_kernel<COUNT>, dim3((uint32_t)blocks), dim3(64), 0, stream, m.nodes,
-                           m.num_nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, queue, chunk);
+                           m.num_nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, queue, chunk,
+                           m.finite);
         return hipGetLastError();
     }
     return with_shape(tu, [&](auto bc, auto rc) -> hipError_t {
@@ -1178,7 +1190,8 @@ hipError_t launch_single(const DeviceMesh &m, const pq_ray *rays, uint64_t n, fl
                 if (e != hipSuccess) return e;
             }
             hipLaunchKernelGGL((cast_rays_kernel<COUNT, 0, B, R>), dim3((uint32_t)((n + B - 1) / B)), dim3(B), lds,
-                               stream, m.nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, order);
+                               stream, m.nodes, m.tris, m.num_triangles, rays, n, max_toi, hits, counts, order,
+                               m.finite);
         }
         return hipGetLastError();
     });

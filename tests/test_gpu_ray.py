@@ -140,6 +140,12 @@ def test_edge_cases(orc):
     rays[4000:5000, 3:] = -rays[4000:5000, :3]
     rays[5000:6000, 3:] *= 3.7
     rays[6000:6010, 3:] = 0.0
+    # non-finite origins and a direction with a tiny component (its inverse overflows):
+    # such rays take the full slab test
+    rays[6010:6014, 0] = np.inf
+    rays[6014:6016, 1] = -np.inf
+    rays[6016:6018, 2] = np.nan
+    rays[6018:6020, 3] = 1e-39
     mesh = TriMesh(v, f)
     om = orc.OracleTriMesh(v, f)
     for mapping in MAPPINGS:
