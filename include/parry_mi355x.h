@@ -195,16 +195,19 @@ typedef struct pq_ray_tuning {
     uint32_t mapping;       /* PQ_RAY_MAPPING_* */
     uint32_t chunk;         /* persistent and wave: rays claimed per queue access (0 = 128; wave: 0 = 4) */
     uint32_t blocks_per_cu; /* persistent and wave: resident workgroups per CU (0 = occupancy limit) */
-    uint32_t block_size;    /* static and persistent: 64, 128 or 256 lanes (0 = 256) */
-    uint32_t stack_regs;    /* static and persistent: 0, 1, 2 or 4 stack entries in registers */
+    uint32_t block_size;    /* static, persistent, binned: 64, 128 or 256 lanes (0 = 256) */
+    uint32_t stack_regs;    /* static, persistent, binned: 0, 1, 2 or 4 stack entries in registers */
     uint32_t reserved;      /* must be 0 */
 } pq_ray_tuning;
 int pq_trimesh_set_tuning(pq_trimesh *mesh, const pq_ray_tuning *tuning);
 int pq_trimesh_get_tuning(const pq_trimesh *mesh, pq_ray_tuning *tuning);
-/* Times the kernel mappings (as pq_trimesh_batch_autotune) on these rays
+/* Times the kernel mappings (as pq_trimesh_batch_autotune, and the binned
+ * mapping over block sizes and 0 or 2 register stack entries) on these rays
  * (device, num_rays > 0; hits receives their results), keeps the fastest in the
  * mesh's tuning and returns it with its time per launch.  Synchronises
- * `stream`.  The best mapping depends on the number of rays. */
+ * `stream`.  The best mapping depends on the number of rays.  The culled casts
+ * (pq_trimesh_cast_local_rays_with_culling) follow the binned mapping when it
+ * is selected and run static lanes otherwise. */
 int pq_trimesh_autotune(pq_trimesh *mesh, const pq_ray *rays, uint64_t num_rays, float max_toi, pq_ray_hit *hits,
                         void *stream, pq_ray_tuning *chosen, float *chosen_ms);
 
