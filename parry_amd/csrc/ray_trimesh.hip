@@ -15,9 +15,12 @@
 // entries in registers and the rest in LDS, strided by the block size so each
 // lane's slots are bank-conflict free.
 //
-// Three kernel mappings (pq_ray_tuning, chosen per mesh or per batch by the
+// Four kernel mappings (pq_ray_tuning, chosen per mesh or per batch by the
 // tuner; every one gives the same hits and the same node and triangle counts):
 //   * static: one launch lane per ray; a wave lasts as long as its slowest ray;
+//   * binned (single meshes): static lanes over the rays counting-sorted by the
+//     cell where they enter the mesh's bounding region (see the sort below),
+//     so the rays of a wave walk similar paths and share cached nodes;
 //   * persistent: a grid sized to the resident waves; each wave pulls chunks
 //     of ray indices from a queue in device memory, and every iteration the
 //     lanes whose ray has finished (found with a wave ballot) take the next
@@ -28,10 +31,11 @@
 // The lane mappings are instantiated for 64-, 128- and 256-lane workgroups
 // and 0, 1, 2 or 4 stack entries in registers.
 //
-// Two exact shortcuts in the static mappings: a wave whose rays have no zero
-// direction component runs a slab test without the degenerate-axis terms
-// (identities for such rays), and the traversal keeps only the best time of
-// impact, leaf and side, computing the winner's normal once at the end.
+// Two exact shortcuts in the static and binned mappings: a wave whose rays
+// have no zero direction component (and finite origins and inverses, on a mesh
+// with finite bounds) runs a slab test without the degenerate-axis terms, and
+// the traversal keeps only the best time of impact, leaf and side, computing
+// the winner's normal once at the end.
 //
 // Floating point: compiled with -ffp-contract=off; 1/x and sqrt are the
 // correctly rounded IEEE operations, min/max are IEEE maxNum/minNum (the
