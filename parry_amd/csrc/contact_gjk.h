@@ -62,9 +62,16 @@ __device__ __forceinline__ f3 local_support(const ShapeRef &s, f3 d) {
         const float4 p0 = s.v[0];
         f3 best = f3{p0.x, p0.y, p0.z};
         float best_dot = dot3(best, d);
-        // Four vertex loads in flight per step, compared in index order: the
-        // scan's result is the sequential one, its latency a quarter of it.
+        // Eight vertex loads in flight per step, compared in index order: the
+        // scan's result is the sequential one, its latency an eighth of it.
         uint32_t k = 1;
+        for (; k + 8 <= s.n; k += 8) {
+            float4 q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) q[u] = s.v[k + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) support_step(q[u], d, best, best_dot);
+        }
         for (; k + 4 <= s.n; k += 4) {
             const float4 a = s.v[k], b = s.v[k + 1], c = s.v[k + 2], e = s.v[k + 3];
             support_step(a, d, best, best_dot);
