@@ -910,12 +910,13 @@ __global__ __launch_bounds__(64) void cast_rays_batch_wave_kernel(const BatchMes
 // and no global atomics are used:
 //   bin_count_kernel   per range: each ray's cell and its rank among the
 //                      range's rays of that cell (LDS atomics), and the
-//                      range's count per cell into hist[cell * blocks + range];
-//   bin_scan_kernel    per cell (one wave each): exclusive prefix sum over the
-//                      ranges in place, and the cell's total;
+//                      range's count per cell into hist[range * kRayBins + cell];
+//   bin_scan_kernel    per 64 cells (one workgroup, a cell per lane, the ranges
+//                      split over its waves): exclusive prefix sum over the
+//                      ranges in place, and each cell's total;
 //   bin_cells_kernel   exclusive prefix sum of the totals: each cell's first
 //                      sorted position;
-//   bin_scatter_kernel order[first[cell] + hist[cell * blocks + range] + rank]
+//   bin_scatter_kernel order[first[cell] + hist[range * kRayBins + cell] + rank]
 //                      = ray index.
 // ---------------------------------------------------------------------------
 constexpr int kBinBlock = 256;          // scatter workgroup
@@ -986,30 +987,48 @@ __global__ __launch_bounds__(kCountBlock) void bin_count_kernel(const pq_bvh_nod
         }
     }
     __syncthreads();
-    for (uint32_t c = threadIdx.x; c < kRayBins; c += kCountBlock) hist[(uint64_t)c * gridDim.x + blockIdx.x] = cnt[c];
+    for (uint32_t c = threadIdx.x; c < kRayBins; c += kCountBlock) hist[(uint64_t)blockIdx.x * kRayBins + c] = cnt[c];
 }
 
-// One wave per cell: exclusive prefix sum over the ranges of that cell's
-// counts (hist[cell * blocks + range]), in place, 64 ranges at a time; the
-// cell's total to totals[cell].
-__global__ __launch_bounds__(64) void bin_scan_kernel(uint32_t *__restrict__ hist, uint32_t blocks,
-                                                      uint32_t *__restrict__ totals) {
-    uint32_t *h = hist + (uint64_t)blockIdx.x * blocks;
-    const uint32_t lane = threadIdx.x;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < blocks; base += 64u) {
-        const uint32_t k = base + lane;
-        const uint32_t v = k < blocks ? h[k] : 0u;
-        uint32_t incl = v;
+// 64 cells per workgroup, one per lane; its kScanWaves waves split the ranges
+// into consecutive chunks.  Each wave sums its chunk of a cell's counts (one
+// coalesced 256-byte row of 64 cells per range), the chunk totals are
+// combined in LDS, and each wave then writes its chunk's exclusive prefixes
+// in place.  The last wave's running total is the cell's total.
+constexpr uint32_t kScanWaves = 16;
+__global__ __launch_bounds__(64 * kScanWaves) void bin_scan_kernel(uint32_t *__restrict__ hist, uint32_t blocks,
+                                                                  uint32_t *__restrict__ totals) {
+    __shared__ uint32_t part[kScanWaves][64];
+    const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
+    const uint32_t cell = blockIdx.x * 64u + lane;
+    const uint32_t chunk = (blocks + kScanWaves - 1u) / kScanWaves;
+    const uint32_t r0 = wave * chunk < blocks ? wave * chunk : blocks;
+    const uint32_t r1 = r0 + chunk < blocks ? r0 + chunk : blocks;
+    uint32_t sum = 0;
+    for (uint32_t base = r0; base < r1; base += 16u) {
+        uint32_t v[16];
 #pragma unroll
-        for (uint32_t d = 1; d < 64u; d <<= 1) {
-            const uint32_t y = __shfl_up(incl, d);
-            if (lane >= d) incl += y;
-        }
-        if (k < blocks) h[k] = carry + incl - v;
-        carry += __shfl(incl, 63);
+        for (uint32_t k = 0; k < 16u; ++k) v[k] = base + k < r1 ? hist[(uint64_t)(base + k) * kRayBins + cell] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 16u; ++k) sum += v[k];
     }
-    if (lane == 0) totals[blockIdx.x] = carry;
+    part[wave][lane] = sum;
+    __syncthreads();
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < wave; ++w) carry += part[w][lane];
+    for (uint32_t base = r0; base < r1; base += 16u) {
+        uint32_t v[16];
+#pragma unroll
+        for (uint32_t k = 0; k < 16u; ++k) v[k] = base + k < r1 ? hist[(uint64_t)(base + k) * kRayBins + cell] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 16u; ++k) {
+            if (base + k < r1) {
+                hist[(uint64_t)(base + k) * kRayBins + cell] = carry;
+                carry += v[k];
+            }
+        }
+    }
+    if (wave == kScanWaves - 1u) totals[cell] = carry;
 }
 
 // One workgroup: totals -> each cell's first sorted position, in place.
@@ -1050,7 +1069,7 @@ __global__ __launch_bounds__(kBinBlock) void bin_scatter_kernel(const uint32_t *
     if (j >= n) return;
     const uint32_t kr = keyrank[j];
     const uint32_t cell = kr & (kRayBins - 1u), rank = kr >> kRayBinKeyBits;
-    order[first[cell] + hist[(uint64_t)cell * blocks + j / per_block] + rank] = (uint32_t)j;
+    order[first[cell] + hist[(j / per_block) * kRayBins + cell] + rank] = (uint32_t)j;
 }
 
 // The binned mapping's sort: leaves in *order the ray index at each sorted position.
@@ -1064,7 +1083,8 @@ hipError_t launch_ray_bins(const DeviceMesh &m, const pq_ray *rays, uint64_t n, 
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bin_count_kernel, dim3((uint32_t)blocks), dim3(kCountBlock), 0, stream, m.nodes, rays, n,
                        per_block, ws.hist, ws.keyrank);
-    hipLaunchKernelGGL(bin_scan_kernel, dim3(kRayBins), dim3(64), 0, stream, ws.hist, (uint32_t)blocks, ws.totals);
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(kRayBins / 64), dim3(64 * kScanWaves), 0, stream, ws.hist, (uint32_t)blocks,
+                       ws.totals);
     hipLaunchKernelGGL(bin_cells_kernel, dim3(1), dim3(kBinBlock), 0, stream, ws.totals);
     hipLaunchKernelGGL(bin_scatter_kernel, dim3((uint32_t)((n + kBinBlock - 1) / kBinBlock)), dim3(kBinBlock), 0,
                        stream, ws.keyrank, ws.hist, ws.totals, n, per_block, (uint32_t)blocks, ws.order);
