@@ -246,6 +246,10 @@ def batch_leg(args, log):
     else:
         tuning, tune_ms = batch.autotune(d_rays, MAX_TOI, out=out)
     graph = batch.capture(d_rays, MAX_TOI, True, out)
+    # counter summary of the kernel the tuned mapping launches (null if the summary is of another mapping)
+    batch_traffic_entry = "cast_rays_batch_kernel" if tuning["mapping"] == "static" else "cast_rays_batch_persistent_kernel"
+    batch_traffic = measured_traffic(batch_traffic_entry)
+    batch_traffic_tuning = measured_traffic(batch_traffic_entry, "tuning")
     wall_ms, gpu_ms = timed(lambda: graph.launch(stream), args.steps)
     identical = bool(torch.equal(out.view(torch.int32), ref.view(torch.int32)))
     rec = {
@@ -271,8 +275,10 @@ def batch_leg(args, log):
             "unit": "GB/s",
             "frac": round(alg_bytes / (gpu_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_launch": alg_bytes,
-            "traffic": measured_traffic("cast_rays_batch_persistent_kernel"),
-            "traffic_tuning": measured_traffic("cast_rays_batch_persistent_kernel", "tuning"),
+            "traffic": batch_traffic if batch_traffic_tuning and batch_traffic_tuning.get("mapping") == tuning["mapping"]
+            else None,
+            "traffic_entry": batch_traffic_entry,
+            "traffic_tuning": batch_traffic_tuning,
         },
         "parity": {"identical_to_separate_launches": identical},
     }
