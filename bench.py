@@ -235,7 +235,7 @@ def batch_leg(args, log):
     sep_wall, sep_gpu = timed(per_mesh, max(1, args.steps // 2))
     ref = out.clone()
     modes = {}
-    for mapping, chunk in (("static", 128), ("persistent", 128)):
+    for mapping, chunk in (("static", 128), ("persistent", 128), ("binned", 128)):
         batch.set_tuning(mapping, chunk)
         modes[f"{mapping}/{chunk}"] = timed(
             lambda: batch.cast_local_rays_and_get_normal(d_rays, MAX_TOI, True, out=out, stream=stream), args.steps)
@@ -247,7 +247,8 @@ def batch_leg(args, log):
         tuning, tune_ms = batch.autotune(d_rays, MAX_TOI, out=out)
     graph = batch.capture(d_rays, MAX_TOI, True, out)
     # counter summary of the kernel the tuned mapping launches (null if the summary is of another mapping)
-    batch_traffic_entry = "cast_rays_batch_kernel" if tuning["mapping"] == "static" else "cast_rays_batch_persistent_kernel"
+    batch_traffic_entry = {"static": "cast_rays_batch_kernel", "binned": "cast_rays_batch_step_binned"}.get(
+        tuning["mapping"], "cast_rays_batch_persistent_kernel")
     batch_traffic = measured_traffic(batch_traffic_entry)
     batch_traffic_tuning = measured_traffic(batch_traffic_entry, "tuning")
     wall_ms, gpu_ms = timed(lambda: graph.launch(stream), args.steps)
@@ -269,7 +270,8 @@ def batch_leg(args, log):
                               "value": round(n / (sep_wall * 1e-3) / 1e6, 3)},
         "roofline": {
             "bound": "hbm",
-            "kernel": "cast_rays_batch_*_kernel",
+            "kernel": ("bin_count_batch_kernel + bin_scan_kernel + bin_cells_kernel + bin_scatter_batch_kernel"
+                       " + cast_rays_batch_kernel" if tuning["mapping"] == "binned" else "cast_rays_batch_*_kernel"),
             "achieved": round(alg_bytes / (gpu_ms * 1e-3) / 1e9, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

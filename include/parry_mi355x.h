@@ -180,7 +180,12 @@ int pq_trimesh_export_bvh(const pq_trimesh *mesh, pq_bvh_node_wide *out_host, ui
  *                              near each other and take similar paths; hits
  *                              still land at the rays' own indices.  For large
  *                              batches of incoherent rays; at most 2^32 - 1 rays
- *                              per call.
+ *                              per call.  In a batch each mesh's rays are sorted
+ *                              on their own (cells of that mesh's root box) into
+ *                              sort buffers the batch owns: 8 bytes per ray and
+ *                              16 KiB per 4096 rays of a mesh, allocated when the
+ *                              mapping is first set or tuned (at most 65535
+ *                              meshes and 65536 sort ranges).
  * block_size (static, persistent, binned): lanes per workgroup, 64, 128 or 256.
  * stack_regs (static, persistent, binned): find_best's pending-node stack keeps
  * its top 0, 1, 2 or 4 entries in registers and the rest in LDS.
@@ -284,16 +289,18 @@ int pq_trimesh_batch_set_tuning(pq_trimesh_batch *batch, const pq_ray_tuning *tu
 int pq_trimesh_batch_get_tuning(const pq_trimesh_batch *batch, pq_ray_tuning *tuning);
 /* Every ray of every mesh: per-ray results exactly as pq_trimesh_cast_local_rays
  * on the ray's own mesh.  Device pointers; asynchronous on `stream`.  The
- * persistent mapping uses a work queue owned by the batch: launches on one
- * batch must be ordered (one stream or a graph). */
+ * persistent mapping uses a work queue owned by the batch, the binned mapping
+ * the batch's sort buffers: launches on one batch must be ordered (one stream
+ * or a graph). */
 int pq_trimesh_batch_cast_local_rays(const pq_trimesh_batch *batch, const pq_ray *rays, float max_toi, int solid,
                                      pq_ray_hit *hits, void *stream);
 int pq_trimesh_batch_cast_local_rays_counted(const pq_trimesh_batch *batch, const pq_ray *rays, float max_toi,
                                              int solid, pq_ray_hit *hits, uint32_t *counts, void *stream);
 /* Times the kernel mappings on these rays (static and persistent lanes over
  * block sizes and register stack depths, persistent chunk sizes and
- * residencies, wave per ray), keeps the fastest in the batch and returns it
- * with its time per launch.  Synchronises `stream`. */
+ * residencies, wave per ray, and the binned mapping when the batch fits its
+ * sort, whose buffers this then allocates), keeps the fastest in the batch and
+ * returns it with its time per launch.  Synchronises `stream`. */
 int pq_trimesh_batch_autotune(pq_trimesh_batch *batch, const pq_ray *rays, float max_toi, pq_ray_hit *hits,
                               void *stream, pq_ray_tuning *chosen, float *chosen_ms);
 /* Captures one batched cast (with the batch's current tuning) into a hipGraph;

@@ -12,14 +12,14 @@ using namespace pq;
 namespace pq {
 
 // The candidates the ray tuners time: the lane mappings over block sizes and
-// register stack depths (for a single mesh the binned mapping too), the
+// register stack depths (the binned mapping where its sort buffers exist), the
 // persistent mapping's chunk sizes and a capped residency, and the wave
 // mapping with one and four rays per claim.
-std::vector<pq_ray_tuning> ray_tuning_candidates(bool single_mesh) {
+std::vector<pq_ray_tuning> ray_tuning_candidates(bool binned) {
     std::vector<pq_ray_tuning> c;
     for (uint32_t block : {64u, 128u, 256u})
         for (uint32_t regs : {0u, 1u, 2u, 4u}) c.push_back({PQ_RAY_MAPPING_STATIC, 128, 0, block, regs, 0});
-    if (single_mesh)
+    if (binned)
         for (uint32_t block : {64u, 128u, 256u})
             for (uint32_t regs : {0u, 2u}) c.push_back({PQ_RAY_MAPPING_BINNED, 128, 0, block, regs, 0});
     for (uint32_t block : {64u, 256u})
@@ -31,7 +31,7 @@ std::vector<pq_ray_tuning> ray_tuning_candidates(bool single_mesh) {
 }
 
 int autotune_ray_mapping(const std::function<hipError_t()> &launch, pq_ray_tuning &slot, hipStream_t s,
-                         pq_ray_tuning *chosen, float *chosen_ms, const char *who, bool single_mesh) {
+                         pq_ray_tuning *chosen, float *chosen_ms, const char *who, bool binned) {
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -39,7 +39,7 @@ int autotune_ray_mapping(const std::function<hipError_t()> &launch, pq_ray_tunin
     const pq_ray_tuning saved = slot;
     float best_ms = 1e30f;
     pq_ray_tuning best = saved;
-    for (const pq_ray_tuning &c : ray_tuning_candidates(single_mesh)) {
+    for (const pq_ray_tuning &c : ray_tuning_candidates(binned)) {
         if (e != hipSuccess) break;
         slot = c;
         // One untimed-for-the-result launch (also warms the caches); a mapping
@@ -88,6 +88,63 @@ int batch_alloc(pq_trimesh_batch *b, void **p, size_t bytes) {
     return PQ_OK;
 }
 
+constexpr uint32_t kMaxBatchSortRanges = 1u << 16;  // 16 KiB of counts per range: at most 1 GiB
+
+// The sort ranges of the batch's binned mapping: ceil(rays / kRayBinRange) per
+// mesh, in mesh order.  False if the batch is too large for the sort (sorted
+// positions are 32-bit, and the counts are capped at 1 GiB).
+bool batch_sort_ranges(const pq_trimesh_batch *b, std::vector<uint32_t> *range_mesh,
+                       std::vector<uint32_t> *range_first) {
+    const uint32_t nm = b->dev.num_meshes;
+    if (b->dev.num_rays >= (1ull << 32) || nm > 65535u) return false;  // the scan's grid has one row per mesh
+    range_first->assign(nm + 1, 0u);
+    range_mesh->clear();
+    uint64_t total = 0;
+    for (uint32_t m = 0; m < nm; ++m) {
+        (*range_first)[m] = (uint32_t)total;
+        const uint64_t q = (b->offsets[m + 1] - b->offsets[m] + kRayBinRange - 1) / kRayBinRange;
+        total += q;
+        if (total > kMaxBatchSortRanges) return false;
+        range_mesh->insert(range_mesh->end(), (size_t)q, m);
+    }
+    (*range_first)[nm] = (uint32_t)total;
+    return true;
+}
+
+// Allocates the batch's sort buffers once (outside any graph capture: the
+// callers are set_tuning and the autotuner).
+int ensure_batch_bins(pq_trimesh_batch *b) {
+    BatchBins &bb = b->dev.bins;
+    if (bb.order) return PQ_OK;
+    std::vector<uint32_t> rm, rf;
+    if (!batch_sort_ranges(b, &rm, &rf))
+        return fail(PQ_ERR_INVALID,
+                    "the binned mapping: batch too large for the ray sort (2^32 rays, 65535 meshes, 65536 sort ranges)");
+    const uint32_t nr = (uint32_t)rm.size();
+    const uint64_t n = b->dev.num_rays;
+    void *range_mesh = nullptr, *range_first = nullptr, *hist = nullptr, *first = nullptr, *keyrank = nullptr,
+         *order = nullptr;
+    int rc = batch_alloc(b, &range_mesh, rm.size() * sizeof(uint32_t));
+    if (rc == PQ_OK) rc = batch_alloc(b, &range_first, rf.size() * sizeof(uint32_t));
+    if (rc == PQ_OK) rc = batch_alloc(b, &hist, (size_t)nr * kRayBins * sizeof(uint32_t));
+    if (rc == PQ_OK) rc = batch_alloc(b, &first, (size_t)b->dev.num_meshes * kRayBins * sizeof(uint32_t));
+    if (rc == PQ_OK) rc = batch_alloc(b, &keyrank, n * sizeof(uint32_t));
+    if (rc == PQ_OK) rc = batch_alloc(b, &order, n * sizeof(uint32_t));
+    if (rc != PQ_OK) return rc;
+    hipError_t e = rm.empty() ? hipSuccess : hipMemcpy(range_mesh, rm.data(), rm.size() * sizeof(uint32_t),
+                                                        hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(range_first, rf.data(), rf.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "the binned mapping: upload of the sort ranges");
+    bb.num_ranges = nr;
+    bb.range_mesh = (const uint32_t *)range_mesh;
+    bb.range_first = (const uint32_t *)range_first;
+    bb.hist = (uint32_t *)hist;
+    bb.first = (uint32_t *)first;
+    bb.keyrank = (uint32_t *)keyrank;
+    bb.order = (uint32_t *)order;
+    return PQ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -112,6 +169,7 @@ int pq_trimesh_batch_create(const pq_trimesh *const *meshes, uint32_t num_meshes
         max_depth = std::max(max_depth, m.depth);
     }
     auto *b = new pq_trimesh_batch();
+    b->offsets.assign(ray_offsets, ray_offsets + num_meshes + 1);
     DeviceBatch &d = b->dev;
     d.num_meshes = num_meshes;
     d.max_depth = max_depth;
@@ -146,8 +204,7 @@ int pq_trimesh_batch_set_tuning(pq_trimesh_batch *b, const pq_ray_tuning *t) {
     if (!b) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_set_tuning: null batch");
     int rc = check_ray_tuning(t, "pq_trimesh_batch_set_tuning");
     if (rc != PQ_OK) return rc;
-    if (t->mapping == PQ_RAY_MAPPING_BINNED)
-        return fail(PQ_ERR_INVALID, "pq_trimesh_batch_set_tuning: the binned mapping is for single meshes");
+    if (t->mapping == PQ_RAY_MAPPING_BINNED && (rc = ensure_batch_bins(b)) != PQ_OK) return rc;
     b->dev.tuning = *t;
     return PQ_OK;
 }
@@ -181,9 +238,16 @@ int pq_trimesh_batch_autotune(pq_trimesh_batch *b, const pq_ray *rays, float max
                               pq_ray_tuning *chosen, float *chosen_ms) {
     if (!b || !rays || !hits) return fail(PQ_ERR_INVALID, "pq_trimesh_batch_autotune: null argument");
     const hipStream_t s = (hipStream_t)stream;
+    // The binned mapping is a candidate when the batch fits the sort.
+    std::vector<uint32_t> rm, rf;
+    const bool binned = b->dev.num_rays > 0 && batch_sort_ranges(b, &rm, &rf);
+    if (binned) {
+        const int rc = ensure_batch_bins(b);
+        if (rc != PQ_OK) return rc;
+    }
     return autotune_ray_mapping(
         [&]() { return launch_cast_rays_batch(b->dev, rays, max_toi, hits, nullptr, s); }, b->dev.tuning, s, chosen,
-        chosen_ms, "pq_trimesh_batch_autotune", false);
+        chosen_ms, "pq_trimesh_batch_autotune", binned);
 }
 
 int pq_trimesh_batch_capture(const pq_trimesh_batch *b, const pq_ray *rays, float max_toi, int solid,

@@ -79,6 +79,22 @@ struct BatchMesh {
     uint32_t finite;  // DeviceMesh::finite
 };
 
+// The binned mapping of a batch sorts each mesh's rays on its own: sort ranges
+// of kRayBinRange rays never straddle two meshes, the counts are scanned per
+// mesh and the cells' first positions start at the mesh's first ray, so every
+// sorted position stays inside its mesh's range of rays.  The buffers belong
+// to the batch (allocated when the mapping is first set or tuned, never during
+// a graph capture): casts of one batch with this mapping must not overlap.
+struct BatchBins {
+    uint32_t num_ranges;
+    const uint32_t *range_mesh;   // device, per range: its mesh
+    const uint32_t *range_first;  // device, num_meshes + 1: each mesh's first range
+    uint32_t *hist;               // device, num_ranges x kRayBins: counts, scanned per mesh and cell
+    uint32_t *first;              // device, num_meshes x kRayBins: rays per cell, then first sorted positions
+    uint32_t *keyrank;            // device, per ray: cell | rank within its range and cell << kRayBinKeyBits
+    uint32_t *order;              // device, per sorted position: the ray index (null until allocated)
+};
+
 struct DeviceBatch {
     const BatchMesh *meshes;               // device
     const unsigned long long *offsets;     // device, num_meshes + 1 ray offsets
@@ -87,6 +103,7 @@ struct DeviceBatch {
     uint32_t max_depth;
     uint64_t num_rays;
     pq_ray_tuning tuning;
+    BatchBins bins;
 };
 
 hipError_t launch_cast_rays_batch(const DeviceBatch &b, const pq_ray *rays, float max_toi, pq_ray_hit *hits,
@@ -106,6 +123,7 @@ constexpr uint32_t kRayBinBits = 4;  // cells per axis: 16
 constexpr uint32_t kRayBinKeyBits = 3 * kRayBinBits;
 constexpr uint32_t kRayBins = 1u << kRayBinKeyBits;
 constexpr uint32_t kRayBinMaxBlocks = 65536;  // sort blocks; a block ranks < 2^20 rays
+constexpr uint32_t kRayBinRange = 4096;       // rays per sort range (before the block cap)
 struct RayBinSpace {
     uint32_t *hist;
     uint32_t *totals;
@@ -120,9 +138,9 @@ uint32_t ray_stack_entries(uint32_t depth);
 bool wave_mapping_fits(uint32_t depth);
 // Times every ray mapping candidate with `launch` (which reads `slot`), leaves
 // the fastest in `slot` and reports it; synchronises `s`.
-// `single_mesh` adds the binned mapping to the candidates.
+// `binned` adds the binned mapping to the candidates.
 int autotune_ray_mapping(const std::function<hipError_t()> &launch, pq_ray_tuning &slot, hipStream_t s,
-                         pq_ray_tuning *chosen, float *chosen_ms, const char *who, bool single_mesh);
+                         pq_ray_tuning *chosen, float *chosen_ms, const char *who, bool binned);
 
 // Launches the ray-cast kernel (ray_trimesh.hip).
 // `counts` (nullable, device, 2 per ray): wide-node records fetched, triangles tested.
@@ -274,6 +292,7 @@ struct pq_graph {
 struct pq_trimesh_batch {
     pq::DeviceBatch dev;
     std::vector<void *> allocations;
+    std::vector<uint64_t> offsets;  // host copy of the ray offsets
 };
 
 struct pq_mesh_contact {

@@ -784,11 +784,15 @@ __global__ __launch_bounds__(BLOCK) void cast_rays_batch_kernel(const BatchMesh 
                                                                 uint32_t nmesh, const pq_ray *__restrict__ rays,
                                                                 uint64_t n, float max_toi,
                                                                 pq_ray_hit *__restrict__ hits,
-                                                                uint32_t *__restrict__ counts) {
+                                                                uint32_t *__restrict__ counts,
+                                                                const uint32_t *__restrict__ order) {
     extern __shared__ uint32_t lds_stack[];
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t mesh = mesh_of(offsets, nmesh, i);
+    const uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= n) return;
+    // The binned mapping's j-th ray in sorted order; the sort keeps every
+    // position inside its mesh's range, so j and i belong to the same mesh.
+    const uint64_t i = order ? (uint64_t)order[j] : j;
+    const uint32_t mesh = mesh_of(offsets, nmesh, j);
     const uint32_t first = __builtin_amdgcn_readfirstlane(mesh);
     const LaneRay r = load_ray(rays, i);
     LaneStack<BLOCK, REG> stk;
@@ -963,7 +967,8 @@ __global__ __launch_bounds__(64) void cast_rays_batch_wave_kernel(const BatchMes
 constexpr int kBinBlock = 256;          // scatter workgroup
 constexpr int kCountBlock = 1024;       // count workgroup
 constexpr int kCountPer = 4;            // rays per count lane per round (loads in flight together)
-constexpr uint32_t kBinRange = kCountBlock * kCountPer;  // rays per sort range (before the block cap)
+constexpr uint32_t kBinRange = kRayBinRange;  // rays per sort range (before the block cap)
+static_assert(kBinRange == kCountBlock * kCountPer, "one round of loads covers a range");
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {  // bit k -> bit 3k (4 bits)
     v &= 0xfu;
@@ -994,22 +999,16 @@ __device__ __forceinline__ uint32_t ray_cell(const float mn[3], const float mx[3
     return (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
 }
 
-__global__ __launch_bounds__(kCountBlock) void bin_count_kernel(const pq_bvh_node_wide *__restrict__ nodes,
-                                                                const pq_ray *__restrict__ rays, uint64_t n,
-                                                                uint64_t per_block, uint32_t *__restrict__ hist,
-                                                                uint32_t *__restrict__ keyrank) {
-    __shared__ uint32_t cnt[kRayBins];
-    for (uint32_t c = threadIdx.x; c < kRayBins; c += kCountBlock) cnt[c] = 0u;
-    __syncthreads();
-    const float4 *node4 = reinterpret_cast<const float4 *>(nodes);
+// Ranks rays [begin, end) in the cells of the sphere inscribed in the root box
+// of node4's tree: keyrank[j] = cell | rank << kRayBinKeyBits, counts in cnt.
+__device__ __forceinline__ void count_cells(const float4 *node4, const pq_ray *__restrict__ rays, uint64_t begin,
+                                            uint64_t end, uint32_t *cnt, uint32_t *__restrict__ keyrank) {
     const float4 q0 = node4[0], q1 = node4[1], q2 = node4[2], q3 = node4[3];
     const bool two = (__float_as_uint(q3.w) & 0x3fffffffu) != 0u;  // a root with one leaf has no right child
     const float mn[3] = {two ? fminf(q0.x, q2.x) : q0.x, two ? fminf(q0.y, q2.y) : q0.y,
                          two ? fminf(q0.z, q2.z) : q0.z};
     const float mx[3] = {two ? fmaxf(q1.x, q3.x) : q1.x, two ? fmaxf(q1.y, q3.y) : q1.y,
                          two ? fmaxf(q1.z, q3.z) : q1.z};
-    const uint64_t begin = (uint64_t)blockIdx.x * per_block;
-    const uint64_t end = begin + per_block < n ? begin + per_block : n;
     for (uint64_t base = begin; base < end; base += kBinRange) {
         RawRay raw[kCountPer];
 #pragma unroll
@@ -1027,6 +1026,35 @@ __global__ __launch_bounds__(kCountBlock) void bin_count_kernel(const pq_bvh_nod
             }
         }
     }
+}
+
+__global__ __launch_bounds__(kCountBlock) void bin_count_kernel(const pq_bvh_node_wide *__restrict__ nodes,
+                                                                const pq_ray *__restrict__ rays, uint64_t n,
+                                                                uint64_t per_block, uint32_t *__restrict__ hist,
+                                                                uint32_t *__restrict__ keyrank) {
+    __shared__ uint32_t cnt[kRayBins];
+    for (uint32_t c = threadIdx.x; c < kRayBins; c += kCountBlock) cnt[c] = 0u;
+    __syncthreads();
+    const uint64_t begin = (uint64_t)blockIdx.x * per_block;
+    const uint64_t end = begin + per_block < n ? begin + per_block : n;
+    count_cells(reinterpret_cast<const float4 *>(nodes), rays, begin, end, cnt, keyrank);
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < kRayBins; c += kCountBlock) hist[(uint64_t)blockIdx.x * kRayBins + c] = cnt[c];
+}
+
+// A batch: one workgroup per sort range; range r holds rays of mesh
+// range_mesh[r] only, and the cells are those of that mesh's root box.
+__global__ __launch_bounds__(kCountBlock) void bin_count_batch_kernel(
+    const BatchMesh *__restrict__ meshes, const unsigned long long *__restrict__ offsets,
+    const uint32_t *__restrict__ range_mesh, const uint32_t *__restrict__ range_first,
+    const pq_ray *__restrict__ rays, uint32_t *__restrict__ hist, uint32_t *__restrict__ keyrank) {
+    __shared__ uint32_t cnt[kRayBins];
+    for (uint32_t c = threadIdx.x; c < kRayBins; c += kCountBlock) cnt[c] = 0u;
+    __syncthreads();
+    const uint32_t m = range_mesh[blockIdx.x];
+    const uint64_t begin = offsets[m] + (uint64_t)(blockIdx.x - range_first[m]) * kBinRange;
+    const uint64_t end = begin + kBinRange < offsets[m + 1] ? begin + kBinRange : offsets[m + 1];
+    count_cells(global_float4(meshes[m].nodes), rays, begin, end, cnt, keyrank);
     __syncthreads();
     for (uint32_t c = threadIdx.x; c < kRayBins; c += kCountBlock) hist[(uint64_t)blockIdx.x * kRayBins + c] = cnt[c];
 }
@@ -1035,16 +1063,25 @@ __global__ __launch_bounds__(kCountBlock) void bin_count_kernel(const pq_bvh_nod
 // into consecutive chunks.  Each wave sums its chunk of a cell's counts (one
 // coalesced 256-byte row of 64 cells per range), the chunk totals are
 // combined in LDS, and each wave then writes its chunk's exclusive prefixes
-// in place.  The last wave's running total is the cell's total.
+// in place.  The last wave's running total is the cell's total.  With
+// range_first (a batch), blockIdx.y is the mesh: its ranges are
+// [range_first[m], range_first[m + 1]) and its totals start at m * kRayBins.
 constexpr uint32_t kScanWaves = 16;
 __global__ __launch_bounds__(64 * kScanWaves) void bin_scan_kernel(uint32_t *__restrict__ hist, uint32_t blocks,
+                                                                  const uint32_t *__restrict__ range_first,
                                                                   uint32_t *__restrict__ totals) {
     __shared__ uint32_t part[kScanWaves][64];
     const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
     const uint32_t cell = blockIdx.x * 64u + lane;
-    const uint32_t chunk = (blocks + kScanWaves - 1u) / kScanWaves;
-    const uint32_t r0 = wave * chunk < blocks ? wave * chunk : blocks;
-    const uint32_t r1 = r0 + chunk < blocks ? r0 + chunk : blocks;
+    uint32_t lo = 0, span = blocks;
+    if (range_first) {
+        lo = range_first[blockIdx.y];
+        span = range_first[blockIdx.y + 1] - lo;
+        totals += (uint64_t)blockIdx.y * kRayBins;
+    }
+    const uint32_t chunk = (span + kScanWaves - 1u) / kScanWaves;
+    const uint32_t r0 = lo + (wave * chunk < span ? wave * chunk : span);
+    const uint32_t r1 = lo + (wave * chunk + chunk < span ? wave * chunk + chunk : span);
     uint32_t sum = 0;
     for (uint32_t base = r0; base < r1; base += 16u) {
         uint32_t v[16];
@@ -1072,12 +1109,15 @@ __global__ __launch_bounds__(64 * kScanWaves) void bin_scan_kernel(uint32_t *__r
     if (wave == kScanWaves - 1u) totals[cell] = carry;
 }
 
-// One workgroup: totals -> each cell's first sorted position, in place.
-__global__ __launch_bounds__(kBinBlock) void bin_cells_kernel(uint32_t *__restrict__ totals) {
+// One workgroup: totals -> each cell's first sorted position, in place.  With
+// offsets (a batch), workgroup m does mesh m's cells, starting at its first ray.
+__global__ __launch_bounds__(kBinBlock) void bin_cells_kernel(uint32_t *__restrict__ totals,
+                                                              const unsigned long long *__restrict__ offsets) {
     static_assert(kRayBins % kBinBlock == 0, "cells split evenly over the workgroup");
     constexpr uint32_t kPer = kRayBins / kBinBlock;
     __shared__ uint32_t wave_sum[kBinBlock / 64];
     const uint32_t t = threadIdx.x, lane = t % 64u, wave = t / 64u;
+    totals += (uint64_t)blockIdx.x * kRayBins;
     uint32_t c[kPer], s = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k) {
@@ -1092,7 +1132,7 @@ __global__ __launch_bounds__(kBinBlock) void bin_cells_kernel(uint32_t *__restri
     }
     if (lane == 63u) wave_sum[wave] = incl;
     __syncthreads();
-    uint32_t run = incl - s;
+    uint32_t run = (offsets ? (uint32_t)offsets[blockIdx.x] : 0u) + incl - s;
     for (uint32_t w = 0; w < wave; ++w) run += wave_sum[w];
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k) {
@@ -1113,6 +1153,19 @@ __global__ __launch_bounds__(kBinBlock) void bin_scatter_kernel(const uint32_t *
     order[first[cell] + hist[(j / per_block) * kRayBins + cell] + rank] = (uint32_t)j;
 }
 
+__global__ __launch_bounds__(kBinBlock) void bin_scatter_batch_kernel(
+    const uint32_t *__restrict__ keyrank, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ first,
+    const unsigned long long *__restrict__ offsets, uint32_t nmesh, const uint32_t *__restrict__ range_first,
+    uint64_t n, uint32_t *__restrict__ order) {
+    const uint64_t j = (uint64_t)blockIdx.x * kBinBlock + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t kr = keyrank[j];
+    const uint32_t cell = kr & (kRayBins - 1u), rank = kr >> kRayBinKeyBits;
+    const uint32_t m = mesh_of(offsets, nmesh, j);
+    const uint64_t range = range_first[m] + (j - offsets[m]) / kBinRange;
+    order[first[(uint64_t)m * kRayBins + cell] + hist[range * kRayBins + cell] + rank] = (uint32_t)j;
+}
+
 // The binned mapping's sort: leaves in *order the ray index at each sorted position.
 hipError_t launch_ray_bins(const DeviceMesh &m, const pq_ray *rays, uint64_t n, hipStream_t stream,
                            const uint32_t **order) {
@@ -1125,11 +1178,26 @@ hipError_t launch_ray_bins(const DeviceMesh &m, const pq_ray *rays, uint64_t n, 
     hipLaunchKernelGGL(bin_count_kernel, dim3((uint32_t)blocks), dim3(kCountBlock), 0, stream, m.nodes, rays, n,
                        per_block, ws.hist, ws.keyrank);
     hipLaunchKernelGGL(bin_scan_kernel, dim3(kRayBins / 64), dim3(64 * kScanWaves), 0, stream, ws.hist, (uint32_t)blocks,
-                       ws.totals);
-    hipLaunchKernelGGL(bin_cells_kernel, dim3(1), dim3(kBinBlock), 0, stream, ws.totals);
+                       nullptr, ws.totals);
+    hipLaunchKernelGGL(bin_cells_kernel, dim3(1), dim3(kBinBlock), 0, stream, ws.totals, nullptr);
     hipLaunchKernelGGL(bin_scatter_kernel, dim3((uint32_t)((n + kBinBlock - 1) / kBinBlock)), dim3(kBinBlock), 0,
                        stream, ws.keyrank, ws.hist, ws.totals, n, per_block, (uint32_t)blocks, ws.order);
     *order = ws.order;
+    return hipGetLastError();
+}
+
+// A batch's sort, per mesh, into the batch's own buffers (b.bins).
+hipError_t launch_batch_bins(const DeviceBatch &b, const pq_ray *rays, hipStream_t stream) {
+    const BatchBins &bb = b.bins;
+    if (!bb.order) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bin_count_batch_kernel, dim3(bb.num_ranges), dim3(kCountBlock), 0, stream, b.meshes, b.offsets,
+                       bb.range_mesh, bb.range_first, rays, bb.hist, bb.keyrank);
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(kRayBins / 64, b.num_meshes), dim3(64 * kScanWaves), 0, stream, bb.hist,
+                       0u, bb.range_first, bb.first);
+    hipLaunchKernelGGL(bin_cells_kernel, dim3(b.num_meshes), dim3(kBinBlock), 0, stream, bb.first, b.offsets);
+    hipLaunchKernelGGL(bin_scatter_batch_kernel, dim3((uint32_t)((b.num_rays + kBinBlock - 1) / kBinBlock)),
+                       dim3(kBinBlock), 0, stream, bb.keyrank, bb.hist, bb.first, b.offsets, b.num_meshes,
+                       bb.range_first, b.num_rays, bb.order);
     return hipGetLastError();
 }
 
@@ -1280,6 +1348,12 @@ hipError_t launch_batch(const DeviceBatch &b, const pq_ray *rays, float max_toi,
                            b.offsets, b.num_meshes, rays, n, max_toi, hits, counts, b.queue, chunk);
         return hipGetLastError();
     }
+    const uint32_t *order = nullptr;
+    if (mapping == PQ_RAY_MAPPING_BINNED) {
+        hipError_t e = launch_batch_bins(b, rays, stream);
+        if (e != hipSuccess) return e;
+        order = b.bins.order;
+    }
     return with_shape(tu, [&](auto bc, auto rc) -> hipError_t {
         constexpr int B = decltype(bc)::value, R = decltype(rc)::value;
         const size_t lds = stack_lds_bytes(b.max_depth, B, R);
@@ -1295,7 +1369,7 @@ hipError_t launch_batch(const DeviceBatch &b, const pq_ray *rays, float max_toi,
                                chunk);
         } else {
             hipLaunchKernelGGL((cast_rays_batch_kernel<COUNT, B, R>), dim3((uint32_t)((n + B - 1) / B)), dim3(B), lds,
-                               stream, b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts);
+                               stream, b.meshes, b.offsets, b.num_meshes, rays, n, max_toi, hits, counts, order);
         }
         return hipGetLastError();
     });

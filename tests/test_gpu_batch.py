@@ -51,10 +51,11 @@ BATCH_MAPPINGS = (
     + [{"mapping": "persistent", "chunk": c} for c in (64, 128, 512)]
     + [{"mapping": "persistent", "chunk": 128, "block_size": 64, "stack_regs": 2},
        {"mapping": "wave", "chunk": 1}, {"mapping": "wave", "chunk": 4}]
+    + [{"mapping": "binned", "block_size": 64}, {"mapping": "binned", "block_size": 256, "stack_regs": 2}]
 )
 
 
-def _check_batch(orc, meshes, rays, mappings=("static", "persistent", "wave")):
+def _check_batch(orc, meshes, rays, mappings=("static", "persistent", "wave", "binned")):
     torch = _torch()
     batch = TriMeshBatch([m for _, _, m in meshes], [r.shape[0] for r in rays])
     all_rays = np.concatenate(rays) if rays else np.zeros((0, 6), np.float32)
@@ -130,7 +131,7 @@ def test_autotune_and_graph_replays(orc):
     out = torch.empty((batch.num_rays, 5), dtype=torch.float32, device="cuda")
     tuning, ms = batch.autotune(d_rays, MAX_TOI, out=out)
     print(f"autotune chose {tuning} at {ms:.4f} ms", flush=True)
-    assert tuning["mapping"] in ("static", "persistent", "wave") and ms > 0
+    assert tuning["mapping"] in ("static", "persistent", "wave", "binned") and ms > 0
     assert _as_records(out).tobytes() == single.tobytes()
     out.zero_()
     graph = batch.capture(d_rays, MAX_TOI, True, out)
@@ -147,6 +148,32 @@ def test_autotune_and_graph_replays(orc):
     graph.launch()
     torch.cuda.synchronize()
     # compare bit patterns: a miss stores feature 0xFFFFFFFF, a NaN when read as f32
+    assert torch.equal(out.view(torch.int32), eager.view(torch.int32))
+
+
+def test_binned_batch_graph_replays(orc):
+    """The binned mapping of a batch (a sort per mesh into the batch's own
+    buffers, then the static kernel over the sorted order) under graph capture:
+    every replay equals the per-mesh casts, and a replay after the rays change
+    in place sees the new rays."""
+    torch = _torch()
+    meshes, rays = _build(SMALL)
+    batch, d_rays, single = _check_batch(orc, meshes, rays, mappings=("binned",))
+    out = torch.empty((batch.num_rays, 5), dtype=torch.float32, device="cuda")
+    batch.set_tuning("binned", block_size=128)
+    graph = batch.capture(d_rays, MAX_TOI, True, out)
+    for rep in range(4):
+        out.fill_(float(rep + 1))
+        torch.cuda.synchronize()
+        graph.launch()
+        torch.cuda.synchronize()
+        assert _as_records(out).tobytes() == single.tobytes(), f"binned graph replay {rep} differs"
+    perm = torch.randperm(d_rays.shape[0], device="cuda")
+    d_rays.copy_(d_rays[perm].contiguous())
+    batch.set_tuning("static")
+    eager = batch.cast_local_rays_and_get_normal(d_rays, MAX_TOI, True)
+    graph.launch()
+    torch.cuda.synchronize()
     assert torch.equal(out.view(torch.int32), eager.view(torch.int32))
 
 

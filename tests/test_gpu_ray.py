@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from parry_amd import RAY_HIT_DTYPE, TriMesh, TriMeshBatch, scenes
+from parry_amd import RAY_HIT_DTYPE, TriMesh, scenes
 from parry_amd._native import NativeError
 
 pytestmark = pytest.mark.gpu
@@ -187,7 +187,7 @@ def test_small_trees_and_duplicates(orc):
 def test_binned_mapping_buffers_and_validation(orc):
     """The binned mapping's sort buffers grow with the call and its counters
     return to zero: a large, a small and again a large call on one stream all
-    equal the static mapping; the batch refuses the mapping."""
+    equal the static mapping; a block size outside 64/128/256 is refused."""
     import torch
 
     v, f = scenes.noisy_icosphere(4, seed=31)
@@ -200,9 +200,6 @@ def test_binned_mapping_buffers_and_validation(orc):
         _assert_parity(hits, om.cast_rays(rays, 10.0), f"binned n={n}")
         static = _cast_gpu_host(mesh, rays, 10.0, "static")
         assert hits.tobytes() == static.tobytes()
-    batch = TriMeshBatch([mesh], [10])
-    with pytest.raises(NativeError):
-        batch.set_tuning("binned")
     with pytest.raises(NativeError):
         mesh.set_tuning("binned", block_size=96)
     torch.cuda.synchronize()
