@@ -158,11 +158,14 @@ __global__ __launch_bounds__(kBlock) void self_pairs_kernel(const pq_bvh_node_wi
                 next = true;
             }
         }
-        if (!next) {
-            if (sp == 0) break;
-            --sp;
-            curr = stk[sp * kBlock];
-        }
+        // The pop as a select over a read every continuing lane makes (slot 0
+        // exists: the host sizes the stack to at least one entry).  Written as
+        // a branch, the compiler split the loop in two with the pop outside the
+        // inner loop, so a wave's lanes that had to pop waited for the others.
+        if (!next && sp == 0) break;
+        const uint32_t top = stk[(sp != 0u ? sp - 1u : 0u) * kBlock];
+        curr = next ? curr : top;
+        sp -= next ? 0u : 1u;
     }
     if (!WRITE) counts[i] = found;
 }

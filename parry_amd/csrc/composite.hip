@@ -95,11 +95,15 @@ __global__ __launch_bounds__(kBlock) void mesh_candidates_kernel(
         if ((__float_as_uint(r1.w) & 0x3fffffffu) > 0u && node_meets(r0, r1, lo, hi)) stk[(sp++) * kBlock] = 1u;
     }
     for (;;) {
-        if (!have) {
-            if (sp == 0) break;
-            next = stk[(--sp) * kBlock];
-            have = true;
-        }
+        // The pop as a select over a read every continuing lane makes (slot 0
+        // exists: the host sizes the stack to depth + 1 entries), and the leaf
+        // as an if/else: with a branch and a `continue` the compiler split the
+        // loop into nested loops, and a wave's lanes that had to pop waited for
+        // the others.
+        if (!have && sp == 0) break;
+        const uint32_t top = stk[(sp != 0u ? sp - 1u : 0u) * kBlock];
+        next = have ? next : top;
+        sp -= have ? 0u : 1u;
         have = false;
         const float4 mn = node4[2 * next], mx = node4[2 * next + 1];
         const uint32_t children = __float_as_uint(mn.w), data = __float_as_uint(mx.w);
@@ -113,20 +117,20 @@ __global__ __launch_bounds__(kBlock) void mesh_candidates_kernel(
             } else {
                 ++found;
             }
-            continue;
-        }
-        const float4 cl0 = node4[4 * children], cl1 = node4[4 * children + 1];
-        const float4 cr0 = node4[4 * children + 2], cr1 = node4[4 * children + 3];
-        if (node_meets(cl0, cl1, lo, hi)) {
-            next = 2u * children;
-            have = true;
-        }
-        if (node_meets(cr0, cr1, lo, hi)) {
-            if (!have) {
-                next = 2u * children + 1u;
+        } else {
+            const float4 cl0 = node4[4 * children], cl1 = node4[4 * children + 1];
+            const float4 cr0 = node4[4 * children + 2], cr1 = node4[4 * children + 3];
+            if (node_meets(cl0, cl1, lo, hi)) {
+                next = 2u * children;
                 have = true;
-            } else {
-                stk[(sp++) * kBlock] = 2u * children + 1u;
+            }
+            if (node_meets(cr0, cr1, lo, hi)) {
+                if (!have) {
+                    next = 2u * children + 1u;
+                    have = true;
+                } else {
+                    stk[(sp++) * kBlock] = 2u * children + 1u;
+                }
             }
         }
     }
