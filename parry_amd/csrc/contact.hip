@@ -1042,47 +1042,52 @@ __device__ uint32_t epa_step(W *w, float *orig, const iso &pos12, const ShapeRef
             sk = 1;
             depth = 2;
         }
+        // One latch: both ways on (into a deleted face's first far neighbour, or
+        // back to the stack's top) end in the same read of the next face.  With
+        // the stack path as a separate branch the compiler split the walk into
+        // nested loops, and a wave's groups that had to pop waited for the others.
         for (;;) {
             const uint32_t id = cur & 0xffffu, opp = cur >> 16;
-            bool from_stack = true;
-            if (!cf.deleted) {
-                if (!((cvis >> opp) & 1u)) {
-                    if (ns >= (uint32_t)W::kS) return overflow(ndel, fid);
-                    w->sil[ns++] = cur;
-                } else {
-                    if (ndel >= (uint32_t)W::kS) return overflow(ndel, fid);
-                    w->dl[ndel++] = (typename W::Idx)id;
-                    w->face[id].deleted = 1;
-                    // a1 = (opp + 2) % 3 and a2 = opp, selected by value so the
-                    // record stays in registers
-                    const uint32_t adj1 = pick3(opp, cf.adj[2], cf.adj[0], cf.adj[1]);
-                    const uint32_t adj2 = pick3(opp, cf.adj[0], cf.adj[1], cf.adj[2]);
-                    const uint32_t pt1 = pick3(opp, cf.pts[2], cf.pts[0], cf.pts[1]);
-                    const uint32_t pt2 = pick3(opp, cf.pts[0], cf.pts[1], cf.pts[2]);
-                    const typename W::Face f1 = w->face[adj1];
-                    const uint32_t v1 = w->vis[adj1];
-                    const uint32_t o2 = next_ccw(w, adj2, pt2);
-                    const uint32_t o1 = ccw_index(f1, pt1);
-                    if (depth > 0) {
-                        if (sk >= (uint32_t)W::kK) return overflow(ndel, fid);
-                        w->stk[sk++] = top;
-                    }
-                    top = adj2 | (o2 << 16);
-                    ++depth;
-                    cur = adj1 | (o1 << 16);
-                    cf = f1;
-                    if (adj1 == id) cf.deleted = 1;  // a face adjacent to itself (degenerate ring)
-                    cvis = v1;
-                    from_stack = false;
+            const bool del = !cf.deleted && ((cvis >> opp) & 1u);
+            if (!cf.deleted && !del) {
+                if (ns >= (uint32_t)W::kS) return overflow(ndel, fid);
+                w->sil[ns++] = cur;
+            }
+            if (!del && depth == 0) break;
+            uint32_t nid, pt1 = 0, popped = 0;
+            if (del) {
+                if (ndel >= (uint32_t)W::kS) return overflow(ndel, fid);
+                w->dl[ndel++] = (typename W::Idx)id;
+                w->face[id].deleted = 1;
+                // a1 = (opp + 2) % 3 and a2 = opp, selected by value so the
+                // record stays in registers
+                const uint32_t adj1 = pick3(opp, cf.adj[2], cf.adj[0], cf.adj[1]);
+                const uint32_t adj2 = pick3(opp, cf.adj[0], cf.adj[1], cf.adj[2]);
+                pt1 = pick3(opp, cf.pts[2], cf.pts[0], cf.pts[1]);
+                const uint32_t pt2 = pick3(opp, cf.pts[0], cf.pts[1], cf.pts[2]);
+                const uint32_t o2 = next_ccw(w, adj2, pt2);
+                if (depth > 0) {
+                    if (sk >= (uint32_t)W::kK) return overflow(ndel, fid);
+                    w->stk[sk++] = top;
+                }
+                top = adj2 | (o2 << 16);
+                ++depth;
+                nid = adj1;
+            } else {
+                popped = top;
+                nid = top & 0xffffu;
+                --depth;
+                const uint32_t below = w->stk[sk != 0u ? sk - 1u : 0u];  // slot 0 always exists
+                if (depth > 0) {
+                    top = below;
+                    --sk;
                 }
             }
-            if (from_stack) {
-                if (depth == 0) break;
-                cur = top;
-                if (--depth > 0) top = w->stk[--sk];
-                cf = w->face[cur & 0xffffu];
-                cvis = w->vis[cur & 0xffffu];
-            }
+            const typename W::Face f = w->face[nid];
+            cvis = w->vis[nid];
+            cur = del ? (nid | (ccw_index(f, pt1) << 16)) : popped;
+            cf = f;
+            if (del && nid == id) cf.deleted = 1;  // a face adjacent to itself (degenerate ring)
         }
         const uint32_t first_new = nf;
         if (ns == 0) return EPA_NONE;
