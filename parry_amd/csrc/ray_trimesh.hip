@@ -18,9 +18,10 @@
 // Four kernel mappings (pq_ray_tuning, chosen per mesh or per batch by the
 // tuner; every one gives the same hits and the same node and triangle counts):
 //   * static: one launch lane per ray; a wave lasts as long as its slowest ray;
-//   * binned (single meshes): static lanes over the rays counting-sorted by the
-//     cell where they enter the mesh's bounding region (see the sort below),
-//     so the rays of a wave walk similar paths and share cached nodes;
+//   * binned: static lanes over the rays counting-sorted by the cell where they
+//     enter the mesh's bounding region (see the sort below), so the rays of a
+//     wave walk similar paths and share cached nodes; in a batch each mesh's
+//     rays are sorted on their own and stay in their mesh's range;
 //   * persistent: a grid sized to the resident waves; each wave pulls chunks
 //     of ray indices from a queue in device memory, and every iteration the
 //     lanes whose ray has finished (found with a wave ballot) take the next
@@ -963,6 +964,11 @@ __global__ __launch_bounds__(64) void cast_rays_batch_wave_kernel(const BatchMes
 //                      sorted position;
 //   bin_scatter_kernel order[first[cell] + hist[range * kRayBins + cell] + rank]
 //                      = ray index.
+// A batch sorts per mesh: ranges of kRayBinRange rays of one mesh each
+// (bin_count_batch_kernel, cells of that mesh's root box), the scan over each
+// mesh's ranges (a grid row per mesh), each mesh's cell offsets starting at its
+// first ray (one bin_cells_kernel workgroup per mesh), and the scatter finding
+// a ray's mesh and range from its index (bin_scatter_batch_kernel).
 // ---------------------------------------------------------------------------
 constexpr int kBinBlock = 256;          // scatter workgroup
 constexpr int kCountBlock = 1024;       // count workgroup
