@@ -582,13 +582,18 @@ __device__ __forceinline__ GjkOut gjk_closest_points(const iso &pos12, const Sha
         old_dir = neg3(div3(proj, sqrtf(sq)));
     }
     float max_bound = kFltMax;
+    // One exit: each outcome records its kind, normal and which simplex gives
+    // the points, then leaves the loop; the points are taken once below.  With
+    // a return per outcome the compiler nested the loop once per exit path.
+    enum : uint32_t { kNoPoints = 0, kCurrent = 1, kPrevious = 2 };
+    uint32_t points = kNoPoints;
     for (uint32_t niter = 0;;) {
         const float old_max_bound = max_bound;
         const f3 mp = neg3(proj);
         const float sq = norm_sq3(mp);
         if (!(sq > kEpsTol * kEpsTol)) {
             out.kind = GJK_INTERSECTION;
-            return out;
+            break;
         }
         const float nrm = sqrtf(sq);
         const f3 dir = div3(mp, nrm);
@@ -596,32 +601,26 @@ __device__ __forceinline__ GjkOut gjk_closest_points(const iso &pos12, const Sha
         if (max_bound >= old_max_bound) {
             out.kind = GJK_CLOSEST;
             out.n = old_dir;
-            sx_points(s, true, out.p1, out.p2);
-            return out;
+            points = kPrevious;
+            break;
         }
         const Cso c = cso_from_shapes(pos12, g1, g2, dir);
         const float min_bound = -dot3(dir, c.p);
         if (!isfinite(min_bound)) {  // parry asserts here (gjk.rs:407)
             out.kind = GJK_NO_INTERSECTION;
             out.n = f3{1.0f, 0.0f, 0.0f};
-            return out;
+            break;
         }
         if (min_bound > max_dist) {
             out.kind = GJK_NO_INTERSECTION;
             out.n = dir;
-            return out;
+            break;
         }
-        if (max_bound - min_bound <= eps_rel * max_bound) {
+        if (max_bound - min_bound <= eps_rel * max_bound || !sx_add(s, c)) {
             out.kind = GJK_CLOSEST;
             out.n = dir;
-            sx_points(s, false, out.p1, out.p2);
-            return out;
-        }
-        if (!sx_add(s, c)) {
-            out.kind = GJK_CLOSEST;
-            out.n = dir;
-            sx_points(s, false, out.p1, out.p2);
-            return out;
+            points = kCurrent;
+            break;
         }
         old_dir = dir;
         proj = sx_project(s);
@@ -629,20 +628,22 @@ __device__ __forceinline__ GjkOut gjk_closest_points(const iso &pos12, const Sha
             if (min_bound >= kEpsTol) {
                 out.kind = GJK_CLOSEST;
                 out.n = old_dir;
-                sx_points(s, true, out.p1, out.p2);
+                points = kPrevious;
             } else {
                 out.kind = GJK_INTERSECTION;
             }
-            return out;
+            break;
         }
         ++niter;
         out.iters = niter;
         if (niter == kGjkMaxIters) {
             out.kind = GJK_NO_INTERSECTION;
             out.n = f3{1.0f, 0.0f, 0.0f};
-            return out;
+            break;
         }
     }
+    if (points != kNoPoints) sx_points(s, points == kPrevious, out.p1, out.p2);
+    return out;
 }
 
 
